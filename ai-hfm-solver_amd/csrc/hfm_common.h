@@ -1,0 +1,79 @@
+// Shared device helpers for libhfm (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "hfm.h"
+
+namespace hfm {
+
+// ---------------------------------------------------------------- errors
+void set_error(const char* fmt, ...);
+int launch_status(const char* what);  // HFM_OK or HFM_EDEVICE after a launch
+
+inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+// ---------------------------------------------------------------- storage types
+struct bf16 {
+  uint16_t v;
+};
+
+__device__ __forceinline__ float to_f(float v) { return v; }
+__device__ __forceinline__ float to_f(bf16 v) { return __uint_as_float(uint32_t(v.v) << 16); }
+
+template <typename T>
+__device__ __forceinline__ T from_f(float f);
+template <>
+__device__ __forceinline__ float from_f<float>(float f) {
+  return f;
+}
+// Round to nearest even; finite inputs (NaN payloads are not preserved).
+template <>
+__device__ __forceinline__ bf16 from_f<bf16>(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return bf16{uint16_t(u >> 16)};
+}
+// Value after a round trip through storage type T.
+template <typename T>
+__device__ __forceinline__ float round_to(float f) {
+  return to_f(from_f<T>(f));
+}
+
+// ---------------------------------------------------------------- division
+// q = n / d for n < 2^32 with mag = floor(2^64 / d) + 1 (mag = 0 means d = 1).
+__device__ __forceinline__ uint32_t fdiv(uint32_t n, uint64_t mag) {
+  return mag ? uint32_t(__umul64hi(uint64_t(n), mag)) : n;
+}
+
+// ---------------------------------------------------------------- reductions
+constexpr int kBlock = 256;
+constexpr int kWaves = kBlock / 64;
+
+// Deterministic block sum: fixed shuffle tree per wave, waves summed in order.
+__device__ __forceinline__ double block_sum(double v) {
+  __shared__ double part[kWaves];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) part[wid] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0) {
+    for (int w = 0; w < kWaves; ++w) s += part[w];
+  }
+  __syncthreads();
+  return s;  // valid in thread 0
+}
+
+// ---------------------------------------------------------------- work table
+// Block g of a launch belongs to task blk_task[g]; tasks start at a multiple
+// of 8 blocks and own a multiple of 8 blocks, so (g - blk0) % 8 is the XCD the
+// block is dispatched to.  Each XCD gets a contiguous range of chunks.
+__device__ __forceinline__ int xcd_chunk(int local, int nblk) {
+  const int per = nblk >> 3;
+  return (local & 7) * per + (local >> 3);
+}
+
+}  // namespace hfm

@@ -1,0 +1,479 @@
+// Reference-mode kernels for gfx950: the reference's fixed-weight Keras
+// layers (3x3x3 SAME convolutions, the six-direction variable-coefficient
+// operator, 2^d mean restriction, nearest upsampling) and their fused
+// smoother/residual forms, plus the boundary-plane fills.  FP32 throughout.
+//
+// One thread per output cell, 64 x 4 thread blocks over (x, y) with one z
+// plane per grid row, so a wavefront reads 64 consecutive floats of a row.
+#include <stdarg.h>
+#include <stdio.h>
+
+#include "hfm_common.h"
+
+namespace hfm {
+
+// ------------------------------------------------------------------ errors
+namespace {
+thread_local char g_err[512] = "";
+}
+
+void set_error(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+
+int launch_status(const char* what) {
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    set_error("%s: launch failed: %s", what, hipGetErrorString(e));
+    return HFM_EDEVICE;
+  }
+  return HFM_OK;
+}
+
+// ------------------------------------------------------------------ helpers
+struct Weights {
+  float w[27];
+};
+
+constexpr int kBX = 64, kBY = 4;
+
+struct Grid3 {
+  int D, H, W;
+  __device__ __forceinline__ int64_t idx(int k, int j, int i) const {
+    return (int64_t(k) * H + j) * W + i;
+  }
+  __device__ __forceinline__ bool in(int k, int j, int i) const {
+    return k >= 0 && k < D && j >= 0 && j < H && i >= 0 && i < W;
+  }
+};
+
+// sum_t K[t] x(k+a-1, j+b-1, i+c-1) over the non-zero taps, row-major t = 9a+3b+c
+template <typename XF>
+__device__ __forceinline__ float conv_at(const Weights& K, uint32_t mask, const Grid3& g, int k,
+                                         int j, int i, XF xat) {
+  float acc = 0.0f;
+#pragma unroll
+  for (int t = 0; t < 27; ++t) {
+    if (!((mask >> t) & 1u)) continue;
+    const int kk = k + t / 9 - 1, jj = j + (t / 3) % 3 - 1, ii = i + t % 3 - 1;
+    if (g.in(kk, jj, ii)) acc += K.w[t] * xat(kk, jj, ii);
+  }
+  return acc;
+}
+
+// six-direction operator sum_dir m_dir (x - x_nb), order x-, x+, y-, y+, z-, z+
+struct Var6 {
+  const float* m[6];
+  const float* tot;
+};
+
+template <typename XF>
+__device__ __forceinline__ float var6_at(const Var6& V, const Grid3& g, int k, int j, int i,
+                                         XF xat) {
+  const int64_t n = g.idx(k, j, i);
+  const float xc = xat(k, j, i);
+  auto nb = [&](int kk, int jj, int ii) { return g.in(kk, jj, ii) ? xat(kk, jj, ii) : 0.0f; };
+  float acc = 0.0f;
+  acc += (xc - nb(k, j, i - 1)) * V.m[0][n];
+  acc += (xc - nb(k, j, i + 1)) * V.m[1][n];
+  acc += (xc - nb(k, j - 1, i)) * V.m[2][n];
+  acc += (xc - nb(k, j + 1, i)) * V.m[3][n];
+  acc += (xc - nb(k - 1, j, i)) * V.m[4][n];
+  acc += (xc - nb(k + 1, j, i)) * V.m[5][n];
+  return acc;
+}
+
+__device__ __forceinline__ bool cell_of_thread(const Grid3& g, int& k, int& j, int& i) {
+  i = blockIdx.x * kBX + threadIdx.x;
+  j = blockIdx.y * kBY + threadIdx.y;
+  k = blockIdx.z;
+  return i < g.W && j < g.H;
+}
+
+// (x - A/d) + b/d, the reference's evaluation order
+__device__ __forceinline__ float jacobi_form(float x, float ax, float b, float d) {
+  return (x - ax / d) + b / d;
+}
+
+// ------------------------------------------------------------------ kernels
+__global__ __launch_bounds__(kBX* kBY) void k_conv27(const float* __restrict__ x,
+                                                     float* __restrict__ y,
+                                                     const float* __restrict__ b, Weights K,
+                                                     uint32_t mask, float d, Grid3 g, int form) {
+  int k, j, i;
+  if (!cell_of_thread(g, k, j, i)) return;
+  const int64_t n = g.idx(k, j, i);
+  const float ax = conv_at(K, mask, g, k, j, i, [&](int kk, int jj, int ii) { return x[g.idx(kk, jj, ii)]; });
+  float out = ax;
+  if (form == HFM_FORM_RESID) out = ax - b[n];
+  else if (form == HFM_FORM_JACOBI) out = jacobi_form(x[n], ax, b[n], d);
+  y[n] = out;
+}
+
+__global__ __launch_bounds__(kBX* kBY) void k_var6(const float* __restrict__ x,
+                                                   float* __restrict__ y,
+                                                   const float* __restrict__ b, Var6 V, Grid3 g,
+                                                   int form) {
+  int k, j, i;
+  if (!cell_of_thread(g, k, j, i)) return;
+  const int64_t n = g.idx(k, j, i);
+  const float ax = var6_at(V, g, k, j, i, [&](int kk, int jj, int ii) { return x[g.idx(kk, jj, ii)]; });
+  float out = ax;
+  if (form == HFM_FORM_RESID) out = ax - b[n];
+  else if (form == HFM_FORM_JACOBI) out = jacobi_form(x[n], ax, b[n], V.tot[n]);
+  y[n] = out;
+}
+
+__global__ __launch_bounds__(kBX* kBY) void k_restrict_mean(const float* __restrict__ x,
+                                                            float* __restrict__ y, Grid3 gf,
+                                                            Grid3 gc, int fz, int fy, int fx,
+                                                            float wgt) {
+  int K, J, I;
+  if (!cell_of_thread(gc, K, J, I)) return;
+  float acc = 0.0f;
+  for (int a = 0; a < fz; ++a)
+    for (int b = 0; b < fy; ++b)
+      for (int c = 0; c < fx; ++c) acc += wgt * x[gf.idx(K * fz + a, J * fy + b, I * fx + c)];
+  y[gc.idx(K, J, I)] = acc;
+}
+
+__global__ __launch_bounds__(kBX* kBY) void k_prolong_nearest(const float* __restrict__ xc,
+                                                              float* __restrict__ y, Grid3 gf,
+                                                              Grid3 gc, int fz, int fy, int fx) {
+  int k, j, i;
+  if (!cell_of_thread(gf, k, j, i)) return;
+  y[gf.idx(k, j, i)] = xc[gc.idx(k / fz, j / fy, i / fx)];
+}
+
+template <bool VAR6>
+__global__ __launch_bounds__(kBX* kBY) void k_prolong_jacobi(const float* __restrict__ wc,
+                                                             const float* __restrict__ r,
+                                                             float* __restrict__ y, Weights K,
+                                                             uint32_t mask, float d, Var6 V,
+                                                             Grid3 gf, Grid3 gc, int fz, int fy,
+                                                             int fx) {
+  int k, j, i;
+  if (!cell_of_thread(gf, k, j, i)) return;
+  const int64_t n = gf.idx(k, j, i);
+  auto wat = [&](int kk, int jj, int ii) { return wc[gc.idx(kk / fz, jj / fy, ii / fx)]; };
+  const float wp = wat(k, j, i);
+  if (VAR6) {
+    y[n] = jacobi_form(wp, var6_at(V, gf, k, j, i, wat), r[n], V.tot[n]);
+  } else {
+    y[n] = jacobi_form(wp, conv_at(K, mask, gf, k, j, i, wat), r[n], d);
+  }
+}
+
+template <bool VAR6>
+__global__ __launch_bounds__(kBX* kBY) void k_correct_jacobi(const float* __restrict__ x,
+                                                             const float* __restrict__ w,
+                                                             const float* __restrict__ b,
+                                                             float* __restrict__ y, Weights K,
+                                                             uint32_t mask, float d, Var6 V,
+                                                             Grid3 g) {
+  int k, j, i;
+  if (!cell_of_thread(g, k, j, i)) return;
+  const int64_t n = g.idx(k, j, i);
+  auto vat = [&](int kk, int jj, int ii) {
+    const int64_t m = g.idx(kk, jj, ii);
+    return x[m] - w[m];
+  };
+  const float v = vat(k, j, i);
+  if (VAR6) {
+    y[n] = jacobi_form(v, var6_at(V, g, k, j, i, vat), b[n], V.tot[n]);
+  } else {
+    y[n] = jacobi_form(v, conv_at(K, mask, g, k, j, i, vat), b[n], d);
+  }
+}
+
+// Boundary fills: the value of the last reference assignment covering a cell.
+__global__ __launch_bounds__(kBX* kBY) void k_bc(const float* __restrict__ in,
+                                                 float* __restrict__ out, Grid3 g, int kind) {
+  int k, j, i;
+  if (!cell_of_thread(g, k, j, i)) return;
+  const int D = g.D, H = g.H, W = g.W;
+  float v = in[g.idx(k, j, i)];
+  switch (kind) {
+    case HFM_BC_TOP_COPY:
+      if (k == 0) v = in[g.idx(1, j, i)];
+      break;
+    case HFM_BC_PRESSURE:
+      if (i == W - 1) v = 0.0f;
+      if (i == 0) v = in[g.idx(k, j, 1)];
+      if (j == 0) v = in[g.idx(k, 1, i)];
+      if (j == H - 1) v = in[g.idx(k, H - 2, i)];
+      if (k == 0) v = in[g.idx(1, j, i)];
+      if (k == D - 1) v = in[g.idx(D - 2, j, i)];
+      break;
+    case HFM_BC_SOURCE:
+      if (i == 0) v = in[g.idx(k, j, 1)];
+      break;
+    case HFM_BC_INDICATOR:
+      if (i == 0 || i == W - 1 || j == 0 || j == H - 1 || k == 0 || k == D - 1) v = 0.0f;
+      break;
+    case HFM_BC_ZERO_OUTLET:
+      if (i == W - 1) v = 0.0f;
+      break;
+  }
+  out[g.idx(k, j, i)] = v;
+}
+
+__global__ __launch_bounds__(kBX* kBY) void k_bc_velocity(const float* __restrict__ u,
+                                                          const float* __restrict__ v,
+                                                          const float* __restrict__ w,
+                                                          float* __restrict__ uo,
+                                                          float* __restrict__ vo,
+                                                          float* __restrict__ wo, Grid3 g,
+                                                          float ub) {
+  int k, j, i;
+  if (!cell_of_thread(g, k, j, i)) return;
+  const int D = g.D, H = g.H, W = g.W;
+  const int64_t n = g.idx(k, j, i);
+  float a = u[n], b = v[n], c = w[n];
+  if (i == 0 || i == W - 1) {
+    a = ub;
+    b = 0.0f;
+    c = 0.0f;
+  }
+  if (j == 0) {
+    a = u[g.idx(k, 1, i)];
+    b = 0.0f;
+    c = 0.0f;
+  }
+  if (j == H - 1) {
+    a = u[g.idx(k, H - 2, i)];
+    b = 0.0f;
+    c = 0.0f;
+  }
+  if (k == 0) {
+    a = 0.0f;
+    b = 0.0f;
+    c = 0.0f;
+  }
+  if (k == D - 1) {
+    a = u[g.idx(D - 2, j, i)];
+    b = v[g.idx(D - 2, j, i)];
+    c = 0.0f;
+  }
+  uo[n] = a;
+  vo[n] = b;
+  wo[n] = c;
+}
+
+__global__ __launch_bounds__(256) void k_bluff(const float* __restrict__ in, float* out,
+                                               const float* __restrict__ sigma, float dt,
+                                               int64_t n) {
+  const int64_t q = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (q < n) out[q] = in[q] / (1.0f + dt * sigma[q]);
+}
+
+}  // namespace hfm
+
+// ====================================================================== C ABI
+using namespace hfm;
+
+namespace {
+
+bool dims_ok(int64_t D, int64_t H, int64_t W) {
+  return D > 0 && H > 0 && W > 0 && D <= 65535 && D * H * W <= INT32_MAX;
+}
+
+dim3 grid_for(const Grid3& g) {
+  return dim3(unsigned((g.W + kBX - 1) / kBX), unsigned((g.H + kBY - 1) / kBY), unsigned(g.D));
+}
+
+bool make_weights(const float* K, Weights& w, uint32_t& mask) {
+  if (!K) return false;
+  mask = 0;
+  for (int t = 0; t < 27; ++t) {
+    w.w[t] = K[t];
+    if (K[t] != 0.0f) mask |= 1u << t;
+  }
+  return true;
+}
+
+bool make_var6(const float* const m[6], const float* tot, Var6& V, bool need_tot) {
+  if (!m) return false;
+  for (int q = 0; q < 6; ++q) {
+    if (!m[q]) return false;
+    V.m[q] = m[q];
+  }
+  V.tot = tot;
+  return !(need_tot && !tot);
+}
+
+}  // namespace
+
+extern "C" const char* hfm_last_error(void) { return g_err; }
+
+extern "C" int hfm_version(void) { return 1; }
+
+extern "C" int hfm_struct_sizes(int64_t* task_size, int64_t* scalars_size) {
+  if (!task_size || !scalars_size) {
+    set_error("hfm_struct_sizes: null output");
+    return HFM_EINVAL;
+  }
+  *task_size = int64_t(sizeof(hfm_mg_task));
+  *scalars_size = int64_t(sizeof(hfm_mg_scalars));
+  return HFM_OK;
+}
+
+extern "C" int hfm_conv27_f32(const float* x, float* y, const float* b, const float K[27],
+                              float d, int64_t D, int64_t H, int64_t W, int form, void* stream) {
+  Weights w;
+  uint32_t mask;
+  if (!x || !y || !dims_ok(D, H, W) || !make_weights(K, w, mask) || x == y ||
+      form < HFM_FORM_APPLY || form > HFM_FORM_JACOBI || (form != HFM_FORM_APPLY && !b)) {
+    set_error("hfm_conv27_f32: bad arguments");
+    return HFM_EINVAL;
+  }
+  const Grid3 g{int(D), int(H), int(W)};
+  hipLaunchKernelGGL(k_conv27, grid_for(g), dim3(kBX, kBY), 0, as_stream(stream), x, y, b, w,
+                     mask, d, g, form);
+  return launch_status("hfm_conv27_f32");
+}
+
+extern "C" int hfm_var6_f32(const float* x, float* y, const float* b, const float* const m[6],
+                            const float* tot, int64_t D, int64_t H, int64_t W, int form,
+                            void* stream) {
+  Var6 V;
+  if (!x || !y || x == y || !dims_ok(D, H, W) ||
+      !make_var6(m, tot, V, form == HFM_FORM_JACOBI) || form < HFM_FORM_APPLY ||
+      form > HFM_FORM_JACOBI || (form != HFM_FORM_APPLY && !b)) {
+    set_error("hfm_var6_f32: bad arguments");
+    return HFM_EINVAL;
+  }
+  const Grid3 g{int(D), int(H), int(W)};
+  hipLaunchKernelGGL(k_var6, grid_for(g), dim3(kBX, kBY), 0, as_stream(stream), x, y, b, V, g,
+                     form);
+  return launch_status("hfm_var6_f32");
+}
+
+static bool factors_ok(int fz, int fy, int fx) {
+  return (fz == 1 || fz == 2) && (fy == 1 || fy == 2) && (fx == 1 || fx == 2);
+}
+
+extern "C" int hfm_restrict_mean_f32(const float* x, float* y, int64_t D, int64_t H, int64_t W,
+                                     int fz, int fy, int fx, void* stream) {
+  if (!x || !y || !dims_ok(D, H, W) || !factors_ok(fz, fy, fx) || D % fz || H % fy || W % fx) {
+    set_error("hfm_restrict_mean_f32: bad arguments (D,H,W)=(%lld,%lld,%lld) f=(%d,%d,%d)",
+              (long long)D, (long long)H, (long long)W, fz, fy, fx);
+    return HFM_EINVAL;
+  }
+  const Grid3 gf{int(D), int(H), int(W)};
+  const Grid3 gc{int(D / fz), int(H / fy), int(W / fx)};
+  hipLaunchKernelGGL(k_restrict_mean, grid_for(gc), dim3(kBX, kBY), 0, as_stream(stream), x, y,
+                     gf, gc, fz, fy, fx, 1.0f / float(fz * fy * fx));
+  return launch_status("hfm_restrict_mean_f32");
+}
+
+extern "C" int hfm_prolong_nearest_f32(const float* xc, float* y, int64_t Dc, int64_t Hc,
+                                       int64_t Wc, int fz, int fy, int fx, void* stream) {
+  if (!xc || !y || !factors_ok(fz, fy, fx) || !dims_ok(Dc * fz, Hc * fy, Wc * fx)) {
+    set_error("hfm_prolong_nearest_f32: bad arguments");
+    return HFM_EINVAL;
+  }
+  const Grid3 gc{int(Dc), int(Hc), int(Wc)};
+  const Grid3 gf{int(Dc * fz), int(Hc * fy), int(Wc * fx)};
+  hipLaunchKernelGGL(k_prolong_nearest, grid_for(gf), dim3(kBX, kBY), 0, as_stream(stream), xc,
+                     y, gf, gc, fz, fy, fx);
+  return launch_status("hfm_prolong_nearest_f32");
+}
+
+extern "C" int hfm_prolong_jacobi_f32(const float* wc, const float* r, float* y,
+                                      const float K[27], float d, const float* const m[6],
+                                      const float* tot, int64_t D, int64_t H, int64_t W, int fz,
+                                      int fy, int fx, void* stream) {
+  Weights w{};
+  uint32_t mask = 0;
+  Var6 V{};
+  const bool var6 = m != nullptr;
+  if (!wc || !r || !y || !dims_ok(D, H, W) || !factors_ok(fz, fy, fx) || D % fz || H % fy ||
+      W % fx || (var6 ? !make_var6(m, tot, V, true) : !make_weights(K, w, mask))) {
+    set_error("hfm_prolong_jacobi_f32: bad arguments");
+    return HFM_EINVAL;
+  }
+  const Grid3 gf{int(D), int(H), int(W)};
+  const Grid3 gc{int(D / fz), int(H / fy), int(W / fx)};
+  if (var6)
+    hipLaunchKernelGGL(k_prolong_jacobi<true>, grid_for(gf), dim3(kBX, kBY), 0,
+                       as_stream(stream), wc, r, y, w, mask, d, V, gf, gc, fz, fy, fx);
+  else
+    hipLaunchKernelGGL(k_prolong_jacobi<false>, grid_for(gf), dim3(kBX, kBY), 0,
+                       as_stream(stream), wc, r, y, w, mask, d, V, gf, gc, fz, fy, fx);
+  return launch_status("hfm_prolong_jacobi_f32");
+}
+
+extern "C" int hfm_correct_jacobi_f32(const float* x, const float* wv, const float* b, float* y,
+                                      const float K[27], float d, const float* const m[6],
+                                      const float* tot, int64_t D, int64_t H, int64_t W,
+                                      void* stream) {
+  Weights w{};
+  uint32_t mask = 0;
+  Var6 V{};
+  const bool var6 = m != nullptr;
+  if (!x || !wv || !b || !y || y == x || y == wv || !dims_ok(D, H, W) ||
+      (var6 ? !make_var6(m, tot, V, true) : !make_weights(K, w, mask))) {
+    set_error("hfm_correct_jacobi_f32: bad arguments");
+    return HFM_EINVAL;
+  }
+  const Grid3 g{int(D), int(H), int(W)};
+  if (var6)
+    hipLaunchKernelGGL(k_correct_jacobi<true>, grid_for(g), dim3(kBX, kBY), 0,
+                       as_stream(stream), x, wv, b, y, w, mask, d, V, g);
+  else
+    hipLaunchKernelGGL(k_correct_jacobi<false>, grid_for(g), dim3(kBX, kBY), 0,
+                       as_stream(stream), x, wv, b, y, w, mask, d, V, g);
+  return launch_status("hfm_correct_jacobi_f32");
+}
+
+extern "C" int hfm_bc_f32(const float* in, float* out, int64_t D, int64_t H, int64_t W, int kind,
+                          void* stream) {
+  const bool needs2 = kind == HFM_BC_TOP_COPY || kind == HFM_BC_PRESSURE || kind == HFM_BC_SOURCE;
+  if (!in || !out || in == out || !dims_ok(D, H, W) || kind < HFM_BC_TOP_COPY ||
+      kind > HFM_BC_ZERO_OUTLET ||
+      (needs2 && ((kind != HFM_BC_SOURCE && D < 2) || (kind == HFM_BC_PRESSURE && H < 2) ||
+                  (kind != HFM_BC_TOP_COPY && W < 2)))) {
+    set_error("hfm_bc_f32: bad arguments (kind %d)", kind);
+    return HFM_EINVAL;
+  }
+  const Grid3 g{int(D), int(H), int(W)};
+  hipLaunchKernelGGL(k_bc, grid_for(g), dim3(kBX, kBY), 0, as_stream(stream), in, out, g, kind);
+  return launch_status("hfm_bc_f32");
+}
+
+extern "C" int hfm_bc_velocity_f32(const float* u, const float* v, const float* w, float* uo,
+                                   float* vo, float* wo, int64_t D, int64_t H, int64_t W,
+                                   float ub, void* stream) {
+  if (!u || !v || !w || !uo || !vo || !wo || uo == u || vo == v || wo == w ||
+      !dims_ok(D, H, W) || D < 2 || H < 2 || W < 2) {
+    set_error("hfm_bc_velocity_f32: bad arguments");
+    return HFM_EINVAL;
+  }
+  const Grid3 g{int(D), int(H), int(W)};
+  hipLaunchKernelGGL(k_bc_velocity, grid_for(g), dim3(kBX, kBY), 0, as_stream(stream), u, v, w,
+                     uo, vo, wo, g, ub);
+  return launch_status("hfm_bc_velocity_f32");
+}
+
+extern "C" int hfm_bluff_body_f32(const float* const* in, float* const* out, int nfields,
+                                  const float* sigma, float dt, int64_t ncells, void* stream) {
+  if (!in || !out || !sigma || nfields < 0 || ncells < 0) {
+    set_error("hfm_bluff_body_f32: bad arguments");
+    return HFM_EINVAL;
+  }
+  for (int f = 0; f < nfields; ++f) {
+    if (!in[f] || !out[f]) {
+      set_error("hfm_bluff_body_f32: null field %d", f);
+      return HFM_EINVAL;
+    }
+    if (ncells == 0) continue;
+    hipLaunchKernelGGL(k_bluff, dim3(unsigned((ncells + 255) / 256)), dim3(256), 0,
+                       as_stream(stream), in[f], out[f], sigma, dt, ncells);
+    if (int e = launch_status("hfm_bluff_body_f32")) return e;
+  }
+  return HFM_OK;
+}

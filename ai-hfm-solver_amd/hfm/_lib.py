@@ -1,0 +1,145 @@
+"""ctypes binding of libhfm.so (the C ABI declared in include/hfm.h).
+
+torch is imported first so that the HIP runtime torch ships is the one the
+library binds to (both carry the soname libamdhip64.so.7); the library then
+launches on torch's streams.  There is no fallback: if the library is missing
+every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import threading
+
+import numpy as np
+import torch  # noqa: F401  (loads the HIP runtime before libhfm.so)
+
+from ._build import LIB_PATH
+
+HFM_OK = 0
+
+STORE = {"f32": 0, "bf16": 1}
+FORM = {"apply": 0, "resid": 1, "jacobi": 2}
+BC = {"top_copy": 0, "pressure": 1, "source": 2, "indicator": 3, "zero_outlet": 4}
+SWEEP = {"zero": 0, "plain": 1, "correct": 2}
+FIN = {"init": 0, "beta": 1, "alpha": 2, "check": 3}
+STATUS = {0: "running", 1: "converged", 2: "maxit", 3: "breakdown"}
+
+TASK_DTYPE = np.dtype([
+    ("D", "<i4"), ("H", "<i4"), ("W", "<i4"), ("inst", "<i4"),
+    ("blk0", "<i4"), ("nblk", "<i4"), ("slot0", "<i4"), ("flags", "<i4"),
+    ("magW", "<u8"), ("magH", "<u8"),
+    ("Dc", "<i4"), ("Hc", "<i4"), ("Wc", "<i4"), ("pad0", "<i4"),
+    ("magWc", "<u8"), ("magHc", "<u8"),
+    ("c", "<u8", (6,)), ("xin", "<u8"), ("xout", "<u8"), ("b", "<u8"), ("ec", "<u8"),
+    ("x64", "<u8"), ("r64", "<u8"), ("rlow", "<u8"), ("b32", "<u8"),
+])
+
+SCALARS_DTYPE = np.dtype([
+    ("bnorm2", "<f8"), ("rz", "<f8"), ("alpha", "<f8"), ("beta", "<f8"),
+    ("rr", "<f8"), ("rel", "<f8"), ("tol", "<f8"), ("pap", "<f8"),
+    ("iter", "<i4"), ("maxit", "<i4"), ("active", "<i4"), ("status", "<i4"),
+    ("slot0", "<i4"), ("nslot", "<i4"), ("pad0", "<i4"), ("pad1", "<i4"),
+])
+
+_vp = C.c_void_p
+_i64 = C.c_int64
+_int = C.c_int
+_f32 = C.c_float
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "hfm_last_error": (C.c_char_p, []),
+    "hfm_version": (_int, []),
+    "hfm_struct_sizes": (_int, [C.POINTER(_i64), C.POINTER(_i64)]),
+    "hfm_conv27_f32": (_int, [_vp, _vp, _vp, _vp, _f32, _i64, _i64, _i64, _int, _vp]),
+    "hfm_var6_f32": (_int, [_vp, _vp, _vp, _vp, _vp, _i64, _i64, _i64, _int, _vp]),
+    "hfm_restrict_mean_f32": (_int, [_vp, _vp, _i64, _i64, _i64, _int, _int, _int, _vp]),
+    "hfm_prolong_nearest_f32": (_int, [_vp, _vp, _i64, _i64, _i64, _int, _int, _int, _vp]),
+    "hfm_prolong_jacobi_f32": (_int, [_vp, _vp, _vp, _vp, _f32, _vp, _vp, _i64, _i64, _i64,
+                                      _int, _int, _int, _vp]),
+    "hfm_correct_jacobi_f32": (_int, [_vp, _vp, _vp, _vp, _vp, _f32, _vp, _vp, _i64, _i64, _i64,
+                                      _vp]),
+    "hfm_bc_f32": (_int, [_vp, _vp, _i64, _i64, _i64, _int, _vp]),
+    "hfm_bc_velocity_f32": (_int, [_vp, _vp, _vp, _vp, _vp, _vp, _i64, _i64, _i64, _f32, _vp]),
+    "hfm_bluff_body_f32": (_int, [_vp, _vp, _int, _vp, _f32, _i64, _vp]),
+    "hfm_mg_sweep": (_int, [_vp, _vp, _i64, _vp, _int, _int, _f32, _f32, _vp, _vp]),
+    "hfm_mg_resid_restrict": (_int, [_vp, _vp, _i64, _vp, _int, _vp]),
+    "hfm_mg_coarse_solve": (_int, [_vp, _i64, _vp, _int, _vp]),
+    "hfm_fcg_init": (_int, [_vp, _vp, _i64, _vp, _int, _vp, _vp]),
+    "hfm_fcg_dir": (_int, [_vp, _vp, _i64, _vp, _int, _vp, _vp]),
+    "hfm_fcg_update": (_int, [_vp, _vp, _i64, _vp, _int, _vp, _vp]),
+    "hfm_fcg_finalize": (_int, [_vp, _i64, _vp, _int, _vp]),
+    "hfm_mg_faces": (_int, [_vp, _vp, _vp, _vp, _i64, _i64, _i64, _f32, _vp]),
+    "hfm_mg_galerkin": (_int, [_vp, _vp, _i64, _i64, _i64, _vp]),
+    "hfm_mg_convert": (_int, [_vp, _vp, _i64, _int, _vp]),
+    "hfm_mg_coarse_inverse": (_int, [_vp, _i64, _vp]),
+}
+
+_lock = threading.Lock()
+_lib = None
+
+
+class HfmError(RuntimeError):
+    """An entry point of libhfm.so returned an error code."""
+
+
+def lib_path() -> str:
+    return os.environ.get("HFM_LIB", LIB_PATH)
+
+
+def load() -> C.CDLL:
+    """Load libhfm.so once (raises if it has not been built)."""
+    global _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        path = lib_path()
+        if not os.path.exists(path):
+            raise HfmError(f"libhfm.so not found at {path}: run __graft_entry__.build() "
+                           "(the HIP extension is required; there is no CPU fallback)")
+        lib = C.CDLL(path)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        ts, ss = _i64(), _i64()
+        if lib.hfm_struct_sizes(C.byref(ts), C.byref(ss)) != HFM_OK:
+            raise HfmError("hfm_struct_sizes failed")
+        if ts.value != TASK_DTYPE.itemsize or ss.value != SCALARS_DTYPE.itemsize:
+            raise HfmError(f"ABI mismatch: task {ts.value} vs {TASK_DTYPE.itemsize}, "
+                           f"scalars {ss.value} vs {SCALARS_DTYPE.itemsize}")
+        _lib = lib
+        return lib
+
+
+def call(name: str, *args) -> None:
+    """Call entry point ``name``; raise HfmError with hfm_last_error() on failure."""
+    lib = load()
+    rc = getattr(lib, name)(*args)
+    if rc != HFM_OK:
+        msg = lib.hfm_last_error().decode(errors="replace")
+        raise HfmError(f"{name} returned {rc}: {msg}")
+
+
+def stream_handle(device=None) -> int:
+    """hipStream_t of torch's current stream (as an int for ctypes)."""
+    return torch.cuda.current_stream(device).cuda_stream
+
+
+def ptr(t) -> int:
+    """Device address of a tensor (None -> NULL)."""
+    return 0 if t is None else t.data_ptr()
+
+
+def ptr_array(ptrs):
+    """A ctypes array of void* (host memory) for the pointer-array arguments."""
+    arr = (C.c_void_p * len(ptrs))()
+    for q, p in enumerate(ptrs):
+        arr[q] = p
+    return arr
+
+
+def magic(d: int) -> int:
+    """floor(2^64 / d) + 1 (0 for d == 1): exact division of n < 2^32 by d."""
+    return 0 if d == 1 else ((1 << 64) - 1) // d + 1

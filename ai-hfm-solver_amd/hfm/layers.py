@@ -1,0 +1,369 @@
+"""Reference-shaped operator callables on device tensors.
+
+The reference builds one Keras ``Sequential`` per grid level and operator
+(``CNN3D_A_<n>``, ``CNN3D_res_<n>``, ``CNN3D_prol_<n>``,
+ART/Anisotropic_3d_Multigrid_7point.ipynb:124-284; ``A_operator_<n>``,
+ART/BGS_multigrid_v1.ipynb#c40; the 27-point layers of UM/AI_HFM_v3.ipynb
+cells 13-17) and calls them on eager tensors of shape (1, D, H, W, 1)
+(or (1, H, W, 1) / (1, W, 1) in 2D / 1D).  Here each is a callable over a
+torch tensor on the GPU with the same layout; each call launches a
+libhfm.so kernel on torch's current stream and returns a new tensor
+(inputs are never mutated, as with the reference's functional layers).
+
+Shape errors raise ValueError (the reference's ``InputLayer`` shape check);
+a tensor that is not on a GPU raises ValueError; there is no CPU path.
+"""
+from __future__ import annotations
+
+from typing import Callable, Dict, List, Optional, Sequence
+
+import numpy as np
+import torch
+
+from . import _lib as L
+
+# ---------------------------------------------------------------------------
+# tensor plumbing
+
+
+def _spatial(x: torch.Tensor):
+    """(D, H, W) of a (1, ..., 1) tensor with 1-3 spatial axes (or a bare field)."""
+    s = tuple(x.shape)
+    if len(s) >= 3 and s[0] == 1 and s[-1] == 1:
+        s = s[1:-1]
+    if not 1 <= len(s) <= 3:
+        raise ValueError(f"expected a (1, ..., 1) field with 1-3 spatial axes, got {tuple(x.shape)}")
+    return (1,) * (3 - len(s)) + s, len(s)
+
+
+def _prep(x, name="x") -> torch.Tensor:
+    if not isinstance(x, torch.Tensor):
+        raise ValueError(f"{name} must be a torch tensor on the GPU")
+    if x.device.type != "cuda":
+        raise ValueError(f"{name} is on {x.device}; libhfm kernels need a GPU tensor")
+    if x.dtype != torch.float32:
+        x = x.to(torch.float32)  # the reference's layers cast float64 inputs to float32
+    return x.contiguous()
+
+
+def kernel27(K) -> np.ndarray:
+    """Reference weight array (1,3,3,3,1) / (3,3,3) / (1,3,3,1) / (3,3) / (1,3,1) / (3,)
+    as the 27 row-major taps of a 3x3x3 kernel (2D/1D kernels sit in the middle planes)."""
+    K = np.asarray(K, dtype=np.float64)
+    K = np.squeeze(K)
+    if K.shape == (3, 3, 3):
+        K3 = K
+    elif K.shape == (3, 3):
+        K3 = np.zeros((3, 3, 3)); K3[1] = K
+    elif K.shape == (3,):
+        K3 = np.zeros((3, 3, 3)); K3[1, 1] = K
+    else:
+        raise ValueError(f"unsupported stencil weight shape {np.shape(K)}")
+    return np.ascontiguousarray(K3.astype(np.float32).ravel())
+
+
+def _kptr(K27: np.ndarray):
+    return K27.ctypes.data_as(L.C.c_void_p)
+
+
+# ---------------------------------------------------------------------------
+# stencil layers
+
+
+class FixedConv:
+    """``ConvNd(1, kernel_size=3, strides=1, padding='SAME')`` with fixed weights, bias 0.
+
+    ``input_shape`` (spatial, optional) mirrors the reference's ``InputLayer``:
+    calls with another shape raise ValueError.
+    """
+
+    def __init__(self, weights, input_shape: Optional[Sequence[int]] = None):
+        self.K = kernel27(weights)
+        self.input_shape = tuple(input_shape) if input_shape is not None else None
+        self.center = float(self.K[13])
+
+    def _check(self, x):
+        dims, _ = _spatial(x)
+        if self.input_shape is not None:
+            want = (1,) * (3 - len(self.input_shape)) + self.input_shape
+            if dims != want:
+                raise ValueError(f"layer expects spatial shape {self.input_shape}, got {tuple(x.shape)}")
+        return dims
+
+    def __call__(self, x):
+        x = _prep(x)
+        D, H, W = self._check(x)
+        y = torch.empty_like(x)
+        L.call("hfm_conv27_f32", x.data_ptr(), y.data_ptr(), 0, _kptr(self.K), 1.0, D, H, W,
+               L.FORM["apply"], L.stream_handle(x.device))
+        return y
+
+    def residual(self, x, b):
+        """``A(x) - b`` in one pass (the reference's ``r = CNN3D_A(values) - b``)."""
+        x, b = _prep(x), _prep(b, "b")
+        D, H, W = self._check(x)
+        if b.shape != x.shape:
+            raise ValueError("b must have the shape of x")
+        y = torch.empty_like(x)
+        L.call("hfm_conv27_f32", x.data_ptr(), y.data_ptr(), b.data_ptr(), _kptr(self.K), 1.0,
+               D, H, W, L.FORM["resid"], L.stream_handle(x.device))
+        return y
+
+    def jacobi(self, w, r, d: Optional[float] = None):
+        """``w - A(w)/d + r/d`` in one pass; ``d`` defaults to the kernel centre."""
+        w, r = _prep(w, "w"), _prep(r, "r")
+        D, H, W = self._check(w)
+        if r.shape != w.shape:
+            raise ValueError("r must have the shape of w")
+        dd = self.center if d is None else float(np.float32(d))
+        y = torch.empty_like(w)
+        L.call("hfm_conv27_f32", w.data_ptr(), y.data_ptr(), r.data_ptr(), _kptr(self.K), dd,
+               D, H, W, L.FORM["jacobi"], L.stream_handle(w.device))
+        return y
+
+
+VAR6_DIRS = ("xm1", "xp1", "ym1", "yp1", "zm1", "zp1")
+
+
+class AOperatorVar6:
+    """``A_operator_n`` of ART/BGS_multigrid_v1.ipynb#c40: sum over the six
+    directions of ``A_MAT_dir * (x - x_nb)``; ``mats`` maps xm1..zp1 (and
+    optionally ``tot``) to (1, D, H, W, 1) coefficient tensors."""
+
+    def __init__(self, mats: Dict[str, torch.Tensor]):
+        self.m = [_prep(mats[k], k) for k in VAR6_DIRS]
+        tot = mats.get("tot")
+        if tot is None:
+            tot = sum(self.m)
+        self.tot = _prep(tot, "tot")
+        self.dims, _ = _spatial(self.m[0])
+        for t in (*self.m, self.tot):
+            if _spatial(t)[0] != self.dims:
+                raise ValueError("all coefficient fields must share one shape")
+        self._arr = L.ptr_array([t.data_ptr() for t in self.m])
+
+    def _run(self, x, b, form):
+        x = _prep(x)
+        dims, _ = _spatial(x)
+        if dims != self.dims:
+            raise ValueError(f"operator built for {self.dims}, got {dims}")
+        y = torch.empty_like(x)
+        L.call("hfm_var6_f32", x.data_ptr(), y.data_ptr(), L.ptr(b), self._arr,
+               self.tot.data_ptr(), *dims, form, L.stream_handle(x.device))
+        return y
+
+    def __call__(self, x):
+        return self._run(x, None, L.FORM["apply"])
+
+    def residual(self, x, b):
+        return self._run(x, _prep(b, "b"), L.FORM["resid"])
+
+    def jacobi(self, w, r):
+        """``w - A(w)/A_MAT_TOT + r/A_MAT_TOT`` (BGS ``#c42``)."""
+        return self._run(w, _prep(r, "r"), L.FORM["jacobi"])
+
+
+def _factors(nd):
+    return (1,) * (3 - nd) + (2,) * nd
+
+
+def restrict_mean(x):
+    """``CNN3D_res_n``: stride-2 mean of the 2^d children (…7point.ipynb:194-235)."""
+    x = _prep(x)
+    (D, H, W), nd = _spatial(x)
+    fz, fy, fx = _factors(nd)
+    out_shape = list(x.shape)
+    off = 1 if x.dim() > nd else 0
+    for a in range(nd):
+        out_shape[off + a] //= 2
+    y = torch.empty(out_shape, dtype=torch.float32, device=x.device)
+    L.call("hfm_restrict_mean_f32", x.data_ptr(), y.data_ptr(), D, H, W, fz, fy, fx,
+           L.stream_handle(x.device))
+    return y
+
+
+def prolong_nearest(x):
+    """``CNN3D_prol_n``: UpSampling by 2 on every spatial axis (…7point.ipynb:251-284)."""
+    x = _prep(x)
+    (D, H, W), nd = _spatial(x)
+    fz, fy, fx = _factors(nd)
+    out_shape = list(x.shape)
+    off = 1 if x.dim() > nd else 0
+    for a in range(nd):
+        out_shape[off + a] *= 2
+    y = torch.empty(out_shape, dtype=torch.float32, device=x.device)
+    L.call("hfm_prolong_nearest_f32", x.data_ptr(), y.data_ptr(), D, H, W, fz, fy, fx,
+           L.stream_handle(x.device))
+    return y
+
+
+# ---------------------------------------------------------------------------
+# the reference's sawtooth cycle
+
+
+class SawtoothMG:
+    """The reference's multigrid cycle (…7point.ipynb:402-454, BGS ``#c42``).
+
+    ``ops[l]`` is a FixedConv (constant weights, divisor = its centre or
+    ``divisors[l]``) or an AOperatorVar6 for level l (fine first).  One call
+    of ``cycle`` does: r = A x - b; restrict r to every level; from w = 0 do
+    ``coarsest_sweeps`` sweeps on the coarsest level; at each finer level
+    prolong and do one sweep against that level's r; ``x -= w``; one fine
+    sweep on A x = b; optional boundary fill ``bc``.  Prolongation+sweep and
+    correction+sweep are single fused kernels.
+    """
+
+    def __init__(self, ops: List, divisors: Optional[List[float]] = None, coarsest_sweeps: int = 1,
+                 bc: Optional[Callable] = None):
+        self.ops = ops
+        self.divisors = divisors
+        self.coarsest_sweeps = int(coarsest_sweeps)
+        self.bc = bc
+
+    def _d(self, lev):
+        op = self.ops[lev]
+        if isinstance(op, AOperatorVar6):
+            return None
+        if self.divisors is not None:
+            return float(np.float32(self.divisors[lev]))
+        return op.center
+
+    def _prolong_jacobi(self, lev, wc, r):
+        op = self.ops[lev]
+        r = _prep(r, "r")
+        (D, H, W), nd = _spatial(r)
+        fz, fy, fx = _factors(nd)
+        if wc.numel() * fz * fy * fx != D * H * W:
+            raise ValueError(f"coarse correction of {wc.numel()} cells does not match level {(D, H, W)}")
+        y = torch.empty_like(r)
+        s = L.stream_handle(r.device)
+        if isinstance(op, AOperatorVar6):
+            L.call("hfm_prolong_jacobi_f32", wc.data_ptr(), r.data_ptr(), y.data_ptr(), None, 0.0,
+                   op._arr, op.tot.data_ptr(), D, H, W, fz, fy, fx, s)
+        else:
+            L.call("hfm_prolong_jacobi_f32", wc.data_ptr(), r.data_ptr(), y.data_ptr(),
+                   _kptr(op.K), self._d(lev), None, None, D, H, W, fz, fy, fx, s)
+        return y
+
+    def _correct_jacobi(self, x, w, b):
+        op = self.ops[0]
+        (D, H, W), _ = _spatial(x)
+        if w.shape != x.shape or b.shape != x.shape:
+            raise ValueError("x, w and b must share a shape")
+        y = torch.empty_like(x)
+        s = L.stream_handle(x.device)
+        if isinstance(op, AOperatorVar6):
+            L.call("hfm_correct_jacobi_f32", x.data_ptr(), w.data_ptr(), b.data_ptr(), y.data_ptr(),
+                   None, 0.0, op._arr, op.tot.data_ptr(), D, H, W, s)
+        else:
+            L.call("hfm_correct_jacobi_f32", x.data_ptr(), w.data_ptr(), b.data_ptr(), y.data_ptr(),
+                   _kptr(op.K), self._d(0), None, None, D, H, W, s)
+        return y
+
+    def cycle(self, values, b, record: Optional[dict] = None):
+        values, b = _prep(values, "values"), _prep(b, "b")
+        r = self.ops[0].residual(values, b)
+        rs = [r]
+        for _ in range(1, len(self.ops)):
+            rs.append(restrict_mean(rs[-1]))
+        w = torch.zeros_like(rs[-1])
+        last = self.ops[-1]
+        for _ in range(self.coarsest_sweeps):
+            w = last.jacobi(w, rs[-1]) if isinstance(last, AOperatorVar6) else \
+                last.jacobi(w, rs[-1], self._d(len(self.ops) - 1))
+        if record is not None:
+            record[len(self.ops) - 1] = w
+        for lev in range(len(self.ops) - 2, -1, -1):
+            w = self._prolong_jacobi(lev, w, rs[lev])
+            if record is not None:
+                record[lev] = w
+        values = self._correct_jacobi(values, w, b)
+        if self.bc is not None:
+            values = self.bc(values)
+        return values
+
+
+def level_layers(K, fine_shape: Sequence[int], nlevels: int, prefix: str = "CNN3D") -> dict:
+    """The reference's per-level layer registry (UM/AI_HFM_v3.ipynb:587-623):
+    ``{prefix}_A_<tag>``, ``{prefix}_res_<tag>``, ``{prefix}_prol_<tag>`` with
+    tag = 2^(nlevels-1-l) for level l (fine first), as built with ``locals()``."""
+    out = {}
+    shape = tuple(fine_shape)
+    for lev in range(nlevels):
+        tag = 2 ** (nlevels - 1 - lev)
+        s = tuple(n // 2 ** lev for n in shape)
+        out[f"{prefix}_A_{tag}"] = FixedConv(K, s)
+        if lev < nlevels - 1:
+            out[f"{prefix}_res_{tag}"] = restrict_mean
+        if lev > 0:
+            out[f"{prefix}_prol_{tag}"] = prolong_nearest
+    return out
+
+
+# ---------------------------------------------------------------------------
+# boundary and obstacle functions (UM/AI_HFM_v3.ipynb cell 19)
+
+
+def _bc(field, kind):
+    f = _prep(field)
+    (D, H, W), _ = _spatial(f)
+    out = torch.empty_like(f)
+    L.call("hfm_bc_f32", f.data_ptr(), out.data_ptr(), D, H, W, L.BC[kind],
+           L.stream_handle(f.device))
+    return out
+
+
+def _check_nx(f, nx):
+    if nx is not None and _spatial(f)[0][2] != int(nx):
+        raise ValueError(f"nx={nx} does not match the field's x extent {_spatial(f)[0][2]}")
+
+
+def boundary_condition_top(values):
+    """Neumann copy of plane 1 into plane 0 of axis 1 (…_BC.ipynb:340-344)."""
+    return _bc(values, "top_copy")
+
+
+def boundary_condition_pressure(values_p, nx=None):
+    _check_nx(values_p, nx)
+    return _bc(values_p, "pressure")
+
+
+def boundary_condition_source(b, nx=None):
+    _check_nx(b, nx)
+    return _bc(b, "source")
+
+
+def boundary_condition_indicator(alpha, nx=None):
+    _check_nx(alpha, nx)
+    return _bc(alpha, "indicator")
+
+
+def zero_outlet(r):
+    """``r[..., nx-1] = 0`` (UM/AI_HFM_v3.ipynb:2574,2612)."""
+    return _bc(r, "zero_outlet")
+
+
+def boundary_condition_velocity(values_u, values_v, values_w, nx=None, ub: float = 1.0):
+    u, v, w = (_prep(a, n) for a, n in ((values_u, "u"), (values_v, "v"), (values_w, "w")))
+    _check_nx(u, nx)
+    (D, H, W), _ = _spatial(u)
+    if v.shape != u.shape or w.shape != u.shape:
+        raise ValueError("u, v, w must share a shape")
+    uo, vo, wo = torch.empty_like(u), torch.empty_like(v), torch.empty_like(w)
+    L.call("hfm_bc_velocity_f32", u.data_ptr(), v.data_ptr(), w.data_ptr(), uo.data_ptr(),
+           vo.data_ptr(), wo.data_ptr(), D, H, W, float(ub), L.stream_handle(u.device))
+    return uo, vo, wo
+
+
+def bluff_body(values_u, values_v, values_w, alpha, sigma, dt: float):
+    """``f / (1 + dt*sigma)`` for u, v, w and the scalar (immersed obstacle damping)."""
+    fields = [_prep(a) for a in (values_u, values_v, values_w, alpha)]
+    sigma = _prep(sigma, "sigma")
+    for f in fields:
+        if f.shape != sigma.shape:
+            raise ValueError("fields and sigma must share a shape")
+    outs = [torch.empty_like(f) for f in fields]
+    L.call("hfm_bluff_body_f32", L.ptr_array([f.data_ptr() for f in fields]),
+           L.ptr_array([o.data_ptr() for o in outs]), len(fields), sigma.data_ptr(),
+           float(dt), sigma.numel(), L.stream_handle(sigma.device))
+    return tuple(outs)

@@ -1,0 +1,60 @@
+"""Seeded synthetic inputs for the anisotropic solve workloads, generated on the device.
+
+* single solve: 64^3 (or any box), log-normal (mu=0, sigma) diagonal
+  conductivity per cell, a +1/-1 point-current dipole at two seeded interior cells;
+* batched uneven solves: edges drawn from {32, 48, 64, 96, 128}, per-instance
+  sigma in [0.5, 2], 1-4 dipoles.
+
+Conductivities come from a seeded ``torch.Generator`` on the GPU (the field is
+large); the instance parameters and dipole positions from
+``numpy.random.default_rng``.
+"""
+from __future__ import annotations
+
+from typing import List, Tuple
+
+import numpy as np
+import torch
+
+EDGES = (32, 48, 64, 96, 128)
+
+
+def batched_specs(n: int, seed: int, edges=EDGES) -> List[Tuple[Tuple[int, int, int], float, int, int]]:
+    """Per-instance ((D,H,W), sigma_ln, n_dipoles, seed)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        e = int(rng.choice(edges))
+        s = float(rng.uniform(0.5, 2.0))
+        k = int(rng.integers(1, 5))
+        out.append(((e, e, e), s, k, int(seed) * 100003 + i))
+    return out
+
+
+def dipoles(shape, n_dipoles: int, seed: int) -> List[Tuple[Tuple[int, int, int], float]]:
+    """Distinct interior cell pairs (+1, -1) per dipole."""
+    D, H, W = shape
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n_dipoles):
+        while True:
+            p = tuple(int(v) for v in (rng.integers(1, D - 1), rng.integers(1, H - 1), rng.integers(1, W - 1)))
+            q = tuple(int(v) for v in (rng.integers(1, D - 1), rng.integers(1, H - 1), rng.integers(1, W - 1)))
+            if p != q:
+                break
+        out += [(p, 1.0), (q, -1.0)]
+    return out
+
+
+def device_fields(shape, sigma_ln: float, n_dipoles: int, seed: int, device):
+    """(sx, sy, sz, b) FP32 tensors on ``device``."""
+    g = torch.Generator(device=device)
+    g.manual_seed(int(seed))
+    sig = []
+    for _ in range(3):
+        z = torch.randn(shape, generator=g, device=device, dtype=torch.float32)
+        sig.append(torch.exp(z * float(sigma_ln)))
+    b = torch.zeros(shape, device=device, dtype=torch.float32)
+    for (p, v) in dipoles(shape, n_dipoles, seed):
+        b[p] += v
+    return sig[0], sig[1], sig[2], b

@@ -1,0 +1,200 @@
+"""Benchmark: batched uneven anisotropic multigrid solves to 1e-8 relative residual.
+
+Workload (one step): 512 independent anisotropic resistivity forward solves
+div(sigma grad phi) = b, cubic edges drawn from {32, 48, 64, 96, 128},
+log-normal conductivity with per-instance sigma_ln in [0.5, 2], 1-4 dipole
+sources each; every solve is taken from its conductivities and right-hand side
+(resident in HBM) to ||b - A x|| / ||b|| <= 1e-8: face coefficients, Galerkin
+hierarchy, coarsest inverses, then BF16-storage V(2,2) cycles (FP32
+arithmetic) preconditioning FP64 conjugate gradients, all in libhfm.so
+work-table kernels, one launch per phase covering all 512 instances.
+
+Printed: one JSON line with solves/s, roofline of the dominant kernel
+(measured live with HIP events on the launch stream), the CPU baseline (the
+oracle's restatement of the same solve on the host, bounded sample) and the
+FP32 one-at-a-time GPU path on a sample of instances.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (ROOT, os.path.join(ROOT, "ai-hfm-solver_amd")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "MG solves/s at 1e-8 rel. residual vs FP64"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def load_batch(mg, specs, dev):
+    from hfm.workloads import device_fields
+    for i, (shape, sig, nd, seed) in enumerate(specs):
+        sx, sy, sz, b = device_fields(shape, sig, nd, seed, dev)
+        mg.stage_sigma(i, sx, sy, sz, b)
+
+
+def one_step(mg):
+    """faces from sigma -> hierarchy -> solve, for every instance."""
+    mg.build_faces()
+    mg.setup()
+    return mg.solve()
+
+
+def kernel_time_ms(fn, reps, torch):
+    """Average duration of the launches issued by fn() on torch's current stream."""
+    s = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    fn()
+    e0.record(s)
+    for _ in range(reps):
+        fn()
+    e1.record(s)
+    e1.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def cpu_baseline(specs, budget_s):
+    """The oracle's FCG solve (numpy, FP32 storage emulation) on host cores."""
+    import numpy as np
+    from oracle import hfm_oracle as O
+    done, t_used, cells = 0, 0.0, 0
+    sample = []
+    for (shape, sig, nd, seed) in sorted(specs, key=lambda s: math.prod(s[0])):
+        sx, sy, sz, b = O.make_anisotropic_fields(shape, seed, sig, nd)
+        op = O.harmonic_faces(sx, sy, sz)
+        t0 = time.perf_counter()
+        res = O.fcg_solve(op, b, tol=1e-8, storage="f32")
+        t_used += time.perf_counter() - t0
+        assert res.status == "converged"
+        done += 1
+        cells += math.prod(shape)
+        sample.append(f"{shape[0]}^3")
+        if t_used >= budget_s:
+            break
+    return {"value": done / t_used, "unit": "solves/s", "cores": 1, "kind": "port",
+            "sample": f"{done} solves from the same workload distribution ({', '.join(sample)}), "
+                      f"smallest edges first; numpy oracle, FP64 CG + FP32-storage V(2,2); {t_used:.1f} s"}
+
+
+def fp32_one_at_a_time(specs, dev, torch):
+    """The plain FP32 path, one instance at a time, on a sample (solves/s)."""
+    from hfm.mg import MGBatch
+    from hfm.workloads import device_fields
+    t_tot = 0.0
+    for (shape, sig, nd, seed) in specs:
+        sx, sy, sz, b = device_fields(shape, sig, nd, seed, dev)
+        mg = MGBatch([shape], storage="f32", tol=1e-8, device=dev)
+        mg.stage_sigma(0, sx, sy, sz, b)
+        one_step(mg)   # warm: graph capture
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        info = one_step(mg)
+        torch.cuda.synchronize()
+        t_tot += time.perf_counter() - t0
+        assert int(info["status"][0]) == 1
+    return len(specs) / t_tot
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--instances", type=int, default=512)
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--storage", default="bf16", choices=["bf16", "f32"])
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--fp32-sample", type=int, default=16)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: the solves are independent and this build runs "
+              "on a single MI355X; multi-GPU runs are not part of it", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from hfm.mg import MGBatch
+    from hfm.workloads import batched_specs
+
+    dev = torch.device("cuda:0")
+    specs = batched_specs(args.instances, args.seed)
+    shapes = [s[0] for s in specs]
+    cells = sum(math.prod(s) for s in shapes)
+    log(f"bench: {len(specs)} instances, {cells / 1e6:.1f} M fine cells, storage {args.storage}")
+    mg = MGBatch(shapes, storage=args.storage, tol=1e-8, maxit=300, device=dev, iters_per_graph=4)
+    load_batch(mg, specs, dev)
+    torch.cuda.synchronize()
+    log(f"bench: resident {mg.nbytes() / 2**30:.1f} GiB; warmup {args.warmup}")
+    for _ in range(args.warmup):
+        info = one_step(mg)
+    torch.cuda.synchronize()
+    times = []
+    for k in range(args.steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        info = one_step(mg)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+        log(f"bench: step {k} {times[-1] * 1e3:.1f} ms, iters mean {info['iter'].mean():.1f} "
+            f"max {info['iter'].max()}")
+    status = info["status"]
+    if not np.all(status == 1):
+        raise SystemExit(f"bench: {int(np.sum(status != 1))} instances did not converge")
+    total = sum(times)
+    value = args.steps * len(specs) / total
+
+    # dominant kernel: the fine-level FCG update (x += a p, r -= a A p), timed live
+    mg.restart()
+    fine_cells = mg.lvl_cells[0]
+    upd_ms = kernel_time_ms(lambda: mg.launch_update(0), 20, torch)
+    upd_bytes = fine_cells * mg.update_bytes_per_cell()
+    achieved = upd_bytes / (upd_ms * 1e-3) / 1e9
+    sweep_ms = kernel_time_ms(lambda: mg.launch_fine_sweep(), 20, torch)
+    sweep_bytes = fine_cells * mg.sweep_bytes_per_cell()
+    roofline = {"bound": "hbm", "kernel": "k_fcg_update", "achieved": round(achieved, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": None, "launch_ms": round(upd_ms, 4),
+                "alg_bytes_per_launch": int(upd_bytes),
+                "fine_sweep": {"kernel": "k_sweep PLAIN", "launch_ms": round(sweep_ms, 4),
+                               "achieved": round(sweep_bytes / (sweep_ms * 1e-3) / 1e9, 1)}}
+    extra = {}
+    if args.fp32_sample > 0:
+        sub = specs[:args.fp32_sample]
+        sps = fp32_one_at_a_time(sub, dev, torch)
+        extra["fp32_one_at_a_time_solves_per_s"] = round(sps, 2)
+        extra["speedup_vs_fp32_one_at_a_time"] = round(value / sps, 2)
+    cpu = None
+    if not args.no_cpu_baseline:
+        torch.set_num_threads(1)
+        cpu = cpu_baseline(specs, args.cpu_budget)
+    out = {
+        "metric": METRIC, "value": round(value, 2), "unit": "solves/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(total / args.steps * 1e3, 2), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64", "storage": args.storage,
+        "data": "synthetic",
+        "config": {"workload": f"batched uneven anisotropic solves: {len(specs)} instances, edges "
+                               "{32,48,64,96,128}^3, log-normal sigma_ln U[0.5,2], 1-4 dipoles, "
+                               f"{args.storage}-storage V(2,2) preconditioned FP64 CG to 1e-8",
+                   "instances": len(specs), "fine_cells": cells, "seed": args.seed},
+        "iters_mean": round(float(info["iter"].mean()), 2), "iters_max": int(info["iter"].max()),
+        "max_rel_residual": float(info["rel"].max()),
+        "latency_ms": {"p50": round(float(np.percentile(times, 50)) * 1e3, 2),
+                       "p99": round(float(np.percentile(times, 99)) * 1e3, 2)},
+        "roofline": roofline, "cpu_baseline": cpu, **extra,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,226 @@
+/*
+ * hfm.h -- C ABI of libhfm.so, the gfx950 stencil + multigrid hot path of
+ * the AI-HFM-Solver operator surface.
+ *
+ * Conventions
+ *   - Fields are contiguous row-major (D, H, W) arrays: the reference's
+ *     (1, D, H, W, 1) tensors without the unit batch/channel axes.  A 2D field
+ *     (1, H, W, 1) is passed as D = 1, a 1D field as D = H = 1.
+ *   - Every pointer is device memory; every launcher is asynchronous and
+ *     stream-ordered on `stream` (a hipStream_t passed as void*; NULL = the
+ *     null stream).
+ *   - Every entry point returns 0 on success or a negative HFM_E* code and
+ *     never throws.  hfm_last_error() returns a thread-local message.
+ *   - Nothing here allocates device memory; all buffers are caller-owned.
+ *
+ * Reference mode (the reference's Keras layers; FP32):
+ *   hfm_conv27_f32          <- Conv3D/Conv2D/Conv1D(k=3, s=1, 'SAME') layers:
+ *                              CNN3D_A_n (ART/Anisotropic_3d_Multigrid_7point.ipynb:124-178),
+ *                              the 27-point layers of UM/AI_HFM_v3.ipynb cells 13-15
+ *                              (CNN3D_central_2nd_dif/xadv/yadv/zadv, CNN3D_Su/Sv/Sw,
+ *                              CNN3D_pu/pv/pw), and the 2D/1D twins
+ *                              (ARTn/Parallel_multigrid_2D.ipynb#c9, _1D#c8); plus
+ *                              the fused residual (A x - b) and smoother
+ *                              (w - A(w)/d + r/d, …7point.ipynb:424-444) forms.
+ *   hfm_var6_f32            <- A_operator_n (ART/BGS_multigrid_v1.ipynb#c40) and its
+ *                              Jacobi sweep with divisor A_MAT_TOT (#c42).
+ *   hfm_restrict_mean_f32   <- CNN3D_res_n (…7point.ipynb:194-235).
+ *   hfm_prolong_nearest_f32 <- CNN3D_prol_n / UpSampling3D (…7point.ipynb:251-284).
+ *   hfm_prolong_jacobi_f32  <- "w_f = prol(w_c); w_f = w_f - A(w_f)/d + r_f/d"
+ *                              fused (…7point.ipynb:426-444).
+ *   hfm_correct_jacobi_f32  <- "values = values - w; values = values - A(values)/d + b/d"
+ *                              fused (…7point.ipynb:448-449).
+ *   hfm_bc_f32              <- boundary_condition_top (…_BC.ipynb:340-344) and the
+ *                              boundary_condition_velocity/pressure/source/indicator
+ *                              plane fills (UM/AI_HFM_v3.ipynb cell 19).
+ *   hfm_bluff_body_f32      <- bluff_body: f / (1 + dt*sigma) (UM/AI_HFM_v3.ipynb cell 19).
+ *
+ * Config mode (anisotropic solve div(sigma grad phi) = b; no reference
+ * counterpart beyond the operator form of A_operator_n): work-table driven
+ * kernels over any number of instances and levels per launch.
+ */
+#ifndef HFM_H
+#define HFM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HFM_OK 0
+#define HFM_EINVAL (-1)
+#define HFM_EDEVICE (-2)
+#define HFM_ENOMEM (-3)
+
+/* storage types of config-mode fields */
+#define HFM_STORE_F32 0
+#define HFM_STORE_BF16 1
+
+/* reference-mode stencil forms */
+#define HFM_FORM_APPLY 0  /* y = A x                         */
+#define HFM_FORM_RESID 1  /* y = A x - b                     */
+#define HFM_FORM_JACOBI 2 /* y = x - A(x)/d + b/d            */
+
+/* boundary fills for hfm_bc_f32 */
+#define HFM_BC_TOP_COPY 0      /* plane z=0 <- plane z=1                       */
+#define HFM_BC_PRESSURE 1      /* x=W-1 -> 0; other faces copy their neighbour */
+#define HFM_BC_SOURCE 2        /* x=0 plane <- x=1 plane                        */
+#define HFM_BC_INDICATOR 3     /* all six faces -> 0                            */
+#define HFM_BC_ZERO_OUTLET 4   /* x=W-1 plane -> 0 (MG residual / correction)   */
+
+const char* hfm_last_error(void);
+int hfm_version(void);
+/* sizeof(hfm_mg_task) and sizeof(hfm_mg_scalars) as compiled, for host-side
+ * table builders to check their layout against. */
+int hfm_struct_sizes(int64_t* task_size, int64_t* scalars_size);
+
+/* ---------------- reference mode ---------------- */
+
+/* y = form(K (*) x): K is the 27-entry row-major 3x3x3 kernel w[0,:,:,:,0];
+ * zero ghosts.  b: rhs for RESID/JACOBI (may be NULL for APPLY); d: the
+ * scalar divisor for JACOBI.  y must not alias x. */
+int hfm_conv27_f32(const float* x, float* y, const float* b, const float K[27], float d,
+                   int64_t D, int64_t H, int64_t W, int form, void* stream);
+
+/* y = form(A6 x) with A6 x = sum_dir m_dir (x - x_nb) over the six directions
+ * (x-, x+, y-, y+, z-, z+ order of `m`), zero ghosts; JACOBI divides by `tot`. */
+int hfm_var6_f32(const float* x, float* y, const float* b, const float* const m[6],
+                 const float* tot, int64_t D, int64_t H, int64_t W, int form, void* stream);
+
+/* y (D/fz, H/fy, W/fx) = mean of the fz*fy*fx children (factors 1 or 2). */
+int hfm_restrict_mean_f32(const float* x, float* y, int64_t D, int64_t H, int64_t W,
+                          int fz, int fy, int fx, void* stream);
+
+/* y (Dc*fz, Hc*fy, Wc*fx) = nearest-neighbour repeat of x (Dc, Hc, Wc). */
+int hfm_prolong_nearest_f32(const float* xc, float* y, int64_t Dc, int64_t Hc, int64_t Wc,
+                            int fz, int fy, int fx, void* stream);
+
+/* y = J(P wc) against r: P = nearest repeat by (fz,fy,fx) onto (D,H,W).
+ * Operator: constant kernel K with divisor d (m == NULL) or the var6 fields
+ * m/tot (K ignored). */
+int hfm_prolong_jacobi_f32(const float* wc, const float* r, float* y, const float K[27], float d,
+                           const float* const m[6], const float* tot, int64_t D, int64_t H,
+                           int64_t W, int fz, int fy, int fx, void* stream);
+
+/* y = J(x - w) against b (same operator choice as above). */
+int hfm_correct_jacobi_f32(const float* x, const float* w, const float* b, float* y,
+                           const float K[27], float d, const float* const m[6], const float* tot,
+                           int64_t D, int64_t H, int64_t W, void* stream);
+
+/* out = in with one boundary fill applied (kind = HFM_BC_*).  Copies read
+ * the input field, as the reference's tf.Variable copies do; out != in. */
+int hfm_bc_f32(const float* in, float* out, int64_t D, int64_t H, int64_t W, int kind,
+               void* stream);
+
+/* (uo, vo, wo) = velocity boundary fill of (u, v, w) (UM/AI_HFM_v3.ipynb
+ * cell 19, boundary_condition_velocity); outputs must not alias inputs. */
+int hfm_bc_velocity_f32(const float* u, const float* v, const float* w, float* uo, float* vo,
+                        float* wo, int64_t D, int64_t H, int64_t W, float ub, void* stream);
+
+/* out_i = in_i / (1 + dt*sigma) for n fields (in place allowed). */
+int hfm_bluff_body_f32(const float* const* in, float* const* out, int nfields,
+                       const float* sigma, float dt, int64_t ncells, void* stream);
+
+/* ---------------- config mode ---------------- */
+
+/* One grid level of one instance, as seen by one launch.  The meaning of the
+ * pointer slots depends on the kernel (see each entry point). */
+typedef struct hfm_mg_task {
+  int32_t D, H, W, inst;        /* extents of this level, instance index       */
+  int32_t blk0, nblk, slot0, flags; /* first block, block count, first dot slot */
+  uint64_t magW, magH;          /* floor(2^64/W)+1, floor(2^64/H)+1 (0 if 1)   */
+  int32_t Dc, Hc, Wc, pad0;     /* coarse extents (restriction / correction)   */
+  uint64_t magWc, magHc;
+  const void* c[6];             /* cx, cy, cz, bxh, byh, bzh                    */
+  const void* xin;
+  void* xout;
+  const void* b;
+  const void* ec;
+  double* x64;
+  double* r64;
+  void* rlow;
+  const float* b32;
+} hfm_mg_task;
+
+/* Per-instance solver state (device memory), FP64 flexible CG. */
+typedef struct hfm_mg_scalars {
+  double bnorm2, rz, alpha, beta;
+  double rr, rel, tol, pap;
+  int32_t iter, maxit, active, status; /* status: 0 running 1 converged 2 maxit 3 breakdown */
+  int32_t slot0, nslot, pad0, pad1;
+} hfm_mg_scalars;
+
+#define HFM_MG_RUNNING 0
+#define HFM_MG_CONVERGED 1
+#define HFM_MG_MAXIT 2
+#define HFM_MG_BREAKDOWN 3
+
+/* Sweep modes */
+#define HFM_SWEEP_ZERO 0    /* out = omega b / d                              */
+#define HFM_SWEEP_PLAIN 1   /* out = x + omega (b - A x)/d                    */
+#define HFM_SWEEP_CORRECT 2 /* x := xin + alpha P ec, then a PLAIN sweep      */
+
+/* Launch geometry shared by all work-table kernels: `blk_task[g]` is the
+ * task index of block g (uint16), nblocks the grid size.  Tasks are padded to
+ * a multiple of 8 blocks so that blocks of one task spread evenly over XCDs. */
+
+/* Weighted-Jacobi sweep on the face operator (storage `store`).  Slots: c =
+ * this level's coefficients, xin (unused for ZERO), b, xout, ec (coarse
+ * correction for CORRECT, extents Dc/Hc/Wc).  dots != NULL: partial
+ * sum(b*out) per block into dots[slot0 + local block]. */
+int hfm_mg_sweep(const hfm_mg_task* tasks, const uint16_t* blk_task, int64_t nblocks,
+                 const hfm_mg_scalars* sc, int store, int mode, float omega, float alpha,
+                 double* dots, void* stream);
+
+/* bc = R (b - A x): one coarse cell per thread over its <= 8 children,
+ * R = P^T/8.  Slots: c, xin = x, b, xout = bc (coarse, extents Dc/Hc/Wc).
+ * Blocks cover the coarse grid. */
+int hfm_mg_resid_restrict(const hfm_mg_task* tasks, const uint16_t* blk_task, int64_t nblocks,
+                          const hfm_mg_scalars* sc, int store, void* stream);
+
+/* Coarsest solve e = Ainv b (Ainv FP64 dense, N = D*H*W <= 64): one block
+ * per task.  Slots: b, xout = e, ec = Ainv. */
+int hfm_mg_coarse_solve(const hfm_mg_task* tasks, int64_t ntasks, const hfm_mg_scalars* sc,
+                        int store, void* stream);
+
+/* FCG kernels on the fine level (coefficients c in FP32).
+ * init:   x64 = 0, r64 = b32, rlow = T(b32), dots = b.b
+ * dir:    xout = p = T(b + beta xin) (b = z, xin = p_old), dots = p.(A p)
+ * update: x64 += alpha p, r64 -= alpha A p, rlow = T(r64), dots = r.r  (xin = p) */
+int hfm_fcg_init(const hfm_mg_task* tasks, const uint16_t* blk_task, int64_t nblocks,
+                 const hfm_mg_scalars* sc, int store, double* dots, void* stream);
+int hfm_fcg_dir(const hfm_mg_task* tasks, const uint16_t* blk_task, int64_t nblocks,
+                const hfm_mg_scalars* sc, int store, double* dots, void* stream);
+int hfm_fcg_update(const hfm_mg_task* tasks, const uint16_t* blk_task, int64_t nblocks,
+                   const hfm_mg_scalars* sc, int store, double* dots, void* stream);
+
+/* Per-instance fixed-order reduction of dots[slot0 .. slot0+nslot) and the
+ * scalar step:  kind 0 INIT (bnorm2), 1 BETA (rz, beta), 2 ALPHA (pap,
+ * alpha), 3 CHECK (rr, rel, iter, status).  One block per instance. */
+#define HFM_FIN_INIT 0
+#define HFM_FIN_BETA 1
+#define HFM_FIN_ALPHA 2
+#define HFM_FIN_CHECK 3
+int hfm_fcg_finalize(hfm_mg_scalars* sc, int64_t ninst, const double* dots, int kind,
+                     void* stream);
+
+/* Setup kernels.
+ * faces:    cx/cy/cz (D,H,W) and bxh (D,H), byh (D,W), bzh (H,W) from cell
+ *           conductivities sx, sy, sz (harmonic face means, boundary face =
+ *           cell value), all FP32, scaled by inv_h2.
+ * galerkin: coarse faces (ceil halves) = sum of crossing fine faces / 8.
+ * convert:  n FP32 values -> storage `store`.
+ * coarse_inverse: dense FP64 inverse of the face operator of each task's
+ *           level (N <= 64) into the task's xout (double[N*N]); c = FP32 faces. */
+int hfm_mg_faces(const float* sx, const float* sy, const float* sz, float* const out[6],
+                 int64_t D, int64_t H, int64_t W, float inv_h2, void* stream);
+int hfm_mg_galerkin(const float* const fine[6], float* const coarse[6], int64_t D, int64_t H,
+                    int64_t W, void* stream);
+int hfm_mg_convert(const float* src, void* dst, int64_t n, int store, void* stream);
+int hfm_mg_coarse_inverse(const hfm_mg_task* tasks, int64_t ntasks, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HFM_H */

@@ -1,0 +1,192 @@
+"""Parity of the reference-mode HIP kernels with the oracle (GPU)."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle import hfm_oracle as O
+
+pytestmark = pytest.mark.gpu
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_outputs.json")))
+TOL = 1e-6  # relative L2, FP32
+
+
+def rel(a, b):
+    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def T(a, dev):
+    import torch
+    return torch.as_tensor(np.ascontiguousarray(a, np.float32)).to(dev)
+
+
+def H(t):
+    return t.detach().cpu().numpy().astype(np.float64)
+
+
+KERNELS = {
+    "a7": O.a7_const_kernel(200.0, 100.0, 50.0),
+    "dif27": O.diffusion27_kernel(),
+    "poisson27": O.poisson27_kernel(),
+    "xadv": O.advection_kernels()[0],
+    "rand": np.random.default_rng(0).standard_normal((3, 3, 3)),
+}
+
+
+@pytest.mark.parametrize("name", sorted(KERNELS))
+@pytest.mark.parametrize("shape", [(16, 24, 40), (5, 7, 130), (1, 1, 9)])
+def test_conv27_forms(dev, name, shape):
+    from hfm import layers
+    K = KERNELS[name]
+    rng = np.random.default_rng(1)
+    x = rng.standard_normal(shape).astype(np.float32)
+    b = rng.standard_normal(shape).astype(np.float32)
+    lay = layers.FixedConv(K)
+    xt, bt = T(x[None, ..., None], dev), T(b[None, ..., None], dev)
+    ax = O.conv_same(x.astype(np.float64), K)
+    assert rel(H(lay(xt))[0, ..., 0], ax) < TOL
+    assert rel(H(lay.residual(xt, bt))[0, ..., 0], ax - b) < TOL
+    d = float(np.float32(K[1, 1, 1])) if K[1, 1, 1] != 0 else 1.0
+    ref = x - ax / d + b / d
+    assert rel(H(lay.jacobi(xt, bt, d))[0, ..., 0], ref) < TOL
+
+
+def test_conv_2d_and_1d_layouts(dev):
+    from hfm import layers
+    K2 = O.implicit_ad2d_kernel()
+    x = np.random.default_rng(2).standard_normal((33, 20)).astype(np.float32)
+    y = layers.FixedConv(K2)(T(x[None, ..., None], dev))
+    assert tuple(y.shape) == (1, 33, 20, 1)
+    assert rel(H(y)[0, ..., 0], O.conv_same(x.astype(np.float64), K2)) < TOL
+    K1 = O.implicit_ad1d_kernel()
+    x1 = np.random.default_rng(3).standard_normal(50).astype(np.float32)
+    y1 = layers.FixedConv(K1)(T(x1[None, :, None], dev))
+    assert rel(H(y1)[0, :, 0], O.conv_same(x1.astype(np.float64), K1)) < TOL
+
+
+def test_layer_shape_check_raises(dev):
+    from hfm import layers
+    lay = layers.FixedConv(KERNELS["a7"], input_shape=(8, 8, 8))
+    with pytest.raises(ValueError):
+        lay(T(np.zeros((1, 4, 4, 4, 1)), dev))
+
+
+def _mats(shape, seed):
+    rng = np.random.default_rng(seed)
+    return {k: rng.uniform(0.5, 3.0, shape).astype(np.float32) for k in O.VAR6_DIRS}
+
+
+@pytest.mark.parametrize("shape", [(10, 12, 70), (3, 5, 4)])
+def test_var6_forms(dev, shape):
+    from hfm import layers
+    mats = _mats(shape, 4)
+    tot = O.var6_tot({k: v.astype(np.float64) for k, v in mats.items()}).astype(np.float32)
+    op = layers.AOperatorVar6({**{k: T(v[None, ..., None], dev) for k, v in mats.items()},
+                               "tot": T(tot[None, ..., None], dev)})
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal(shape).astype(np.float32)
+    b = rng.standard_normal(shape).astype(np.float32)
+    m64 = {k: v.astype(np.float64) for k, v in mats.items()}
+    ax = O.a_operator_var6(x.astype(np.float64), m64)
+    xt, bt = T(x[None, ..., None], dev), T(b[None, ..., None], dev)
+    assert rel(H(op(xt))[0, ..., 0], ax) < TOL
+    assert rel(H(op.residual(xt, bt))[0, ..., 0], ax - b) < TOL
+    assert rel(H(op.jacobi(xt, bt))[0, ..., 0], x - ax / tot + b / tot) < TOL
+
+
+def test_restrict_prolong(dev):
+    from hfm import layers
+    x = np.random.default_rng(6).standard_normal((8, 12, 16)).astype(np.float32)
+    r = H(layers.restrict_mean(T(x[None, ..., None], dev)))[0, ..., 0]
+    assert rel(r, O.restrict_mean(x.astype(np.float64))) < 1e-7
+    p = H(layers.prolong_nearest(T(x[None, ..., None], dev)))[0, ..., 0]
+    np.testing.assert_array_equal(p, O.prolong_nearest(x).astype(np.float64))
+    x2 = np.random.default_rng(7).standard_normal((6, 10)).astype(np.float32)
+    r2 = H(layers.restrict_mean(T(x2[None, ..., None], dev)))[0, ..., 0]
+    assert r2.shape == (3, 5)
+    assert rel(r2, O.restrict_mean(x2.astype(np.float64))) < 1e-7
+
+
+def _golden_sawtooth(dev, nd):
+    from hfm import layers
+    n = 128
+    shape = (n,) * nd
+    Tf = np.zeros(shape, np.float32)
+    Tf[(slice(43, 83),) * nd] = 1.0
+    K = O.implicit_ad2d_kernel() if nd == 2 else O.implicit_ad1d_kernel()
+    nlev = int(np.log2(n))
+    ops = [layers.FixedConv(K) for _ in range(nlev)]
+    mg = layers.SawtoothMG(ops)
+    rec = {}
+    vt = T(Tf[None, ..., None], dev)
+    mg.cycle(vt, vt, record=rec)
+    return rec, nlev
+
+
+def test_sawtooth_2d_golden_on_gpu(dev):
+    rec, nlev = _golden_sawtooth(dev, 2)
+    np.testing.assert_array_equal(H(rec[nlev - 1]).ravel(), GOLD["mg2d_w2"])
+    np.testing.assert_allclose(H(rec[nlev - 2]).ravel(), GOLD["mg2d_w4"], rtol=0, atol=5e-9)
+
+
+def test_sawtooth_1d_golden_on_gpu(dev):
+    rec, nlev = _golden_sawtooth(dev, 1)
+    np.testing.assert_allclose(H(rec[nlev - 5]).ravel(), GOLD["mg1d_w32_jacobi"], rtol=5e-6, atol=1e-10)
+    np.testing.assert_allclose(H(rec[nlev - 2]).ravel(), GOLD["mg1d_w4"], rtol=5e-6, atol=1e-8)
+
+
+@pytest.mark.parametrize("var", [False, True])
+def test_sawtooth_3d_cycles_match_oracle(dev, var):
+    import torch
+    from hfm import layers
+    n, nlev = 32, 6
+    rng = np.random.default_rng(8)
+    x0 = np.zeros((n, n, n), np.float32)
+    b = np.zeros((n, n, n), np.float32)
+    b[8:20, 10:22, 12:24] = 1.0
+    if var:
+        mats = [_mats((n >> l,) * 3, 10 + l) for l in range(nlev)]
+        m64 = [{k: v.astype(np.float64) for k, v in m.items()} for m in mats]
+        oops = [lambda x, m=m: O.a_operator_var6(x, m) for m in m64]
+        odiag = [O.var6_tot(m) for m in m64]
+        gops = [layers.AOperatorVar6({k: T(v[None, ..., None], dev) for k, v in m.items()}) for m in mats]
+    else:
+        K = O.a7_const_kernel(200.0, 100.0, 50.0)
+        oops = [lambda x: O.conv_same(x, K)] * nlev
+        odiag = [np.float64(np.float32(K[1, 1, 1]))] * nlev
+        gops = [layers.FixedConv(K) for _ in range(nlev)]
+    mg = layers.SawtoothMG(gops, bc=layers.boundary_condition_top)
+    xo = x0.astype(np.float64)
+    xg = T(x0[None, ..., None], dev)
+    bt = T(b[None, ..., None], dev)
+    for _ in range(3):
+        xo = O.sawtooth_cycle(xo, b.astype(np.float64), oops, odiag, bc=O.boundary_condition_top)
+        xg = mg.cycle(xg, bt)
+    torch.cuda.synchronize()
+    assert rel(H(xg)[0, ..., 0], xo) < TOL
+
+
+def test_boundary_fills_bitwise(dev):
+    from hfm import layers
+    rng = np.random.default_rng(9)
+    u, v, w = (rng.standard_normal((6, 7, 9)).astype(np.float32) for _ in range(3))
+    ut, vt, wt = (T(a[None, ..., None], dev) for a in (u, v, w))
+    go = [H(a)[0, ..., 0] for a in layers.boundary_condition_velocity(ut, vt, wt, ub=1.25)]
+    oo = O.boundary_condition_velocity(u, v, w, ub=np.float32(1.25))
+    for g, o in zip(go, oo):
+        np.testing.assert_array_equal(g, o)
+    np.testing.assert_array_equal(H(layers.boundary_condition_pressure(ut))[0, ..., 0],
+                                  O.boundary_condition_pressure(u))
+    np.testing.assert_array_equal(H(layers.boundary_condition_source(ut))[0, ..., 0],
+                                  O.boundary_condition_source(u))
+    np.testing.assert_array_equal(H(layers.boundary_condition_indicator(ut))[0, ..., 0],
+                                  O.boundary_condition_indicator(u))
+    np.testing.assert_array_equal(H(layers.boundary_condition_top(ut))[0, ..., 0],
+                                  O.boundary_condition_top(u))
+    np.testing.assert_array_equal(H(layers.zero_outlet(ut))[0, ..., 0], O.zero_outlet(u))
+    sig = np.where(rng.random(u.shape) < 0.3, 1e5, 0.0).astype(np.float32)
+    outs = layers.bluff_body(ut, vt, wt, ut, T(sig[None, ..., None], dev), dt=0.4)
+    ref = O.bluff_body((u,), sig, np.float32(0.4))[0]
+    assert rel(H(outs[0])[0, ..., 0], ref) < 1e-7
