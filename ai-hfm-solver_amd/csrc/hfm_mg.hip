@@ -4,23 +4,86 @@
 //
 // Every kernel is driven by a work table (hfm_mg_task[] + per-block task
 // index), so one launch covers any mix of instances of different sizes.
-// Memory-bound: one thread per cell (per coarse cell for the restriction),
-// flattened row-major chunks of kChunk cells per block, all neighbour loads
-// from L1/L2, chunks of a task spread contiguously over the 8 XCDs.
+// Memory-bound: flattened row-major chunks per block, 16-byte row loads,
+// neighbour rows and planes from L1/L2, chunks of a task spread contiguously
+// over the 8 XCDs.
 #include <math.h>
 
 #include "hfm_common.h"
 
 namespace hfm {
 
-constexpr int kCPT = 4;                 // cells per thread
-constexpr int kChunk = kBlock * kCPT;   // cells per block
+// Cells per block: every cell kernel covers kCellChunk flattened cells per
+// block; the restriction covers kCoarseChunk coarse cells per block.
+// Vector path (row width W % 8 == 0, all offsets 8-element aligned by the
+// host): a thread owns a run of 8 consecutive x-cells, loads 16-byte rows of
+// every field and takes its x-neighbours from the adjacent lanes.  Scalar
+// path (other widths, small coarse levels): 8 cells per thread, stride 256.
+constexpr int kRun = 8;
+constexpr int kCellChunk = kBlock * kRun;
+constexpr int kCoarseChunk = kBlock * 4;
 
 template <typename C>
 __device__ __forceinline__ float ldc(const void* p, int64_t i) {
   return to_f(static_cast<const C*>(p)[i]);
 }
 
+// ------------------------------------------------------------------ 8-wide row access
+__device__ __forceinline__ void ld8(const float* p, float v[8]) {
+  const float4 a = reinterpret_cast<const float4*>(p)[0];
+  const float4 b = reinterpret_cast<const float4*>(p)[1];
+  v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+  v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+}
+__device__ __forceinline__ void ld8(const bf16* p, float v[8]) {
+  const uint4 u = *reinterpret_cast<const uint4*>(p);
+  const uint32_t w[4] = {u.x, u.y, u.z, u.w};
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    v[2 * q] = __uint_as_float(w[q] << 16);
+    v[2 * q + 1] = __uint_as_float(w[q] & 0xFFFF0000u);
+  }
+}
+__device__ __forceinline__ void ld4(const float* p, float v[4]) {
+  const float4 a = *reinterpret_cast<const float4*>(p);
+  v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+}
+__device__ __forceinline__ void ld4(const bf16* p, float v[4]) {
+  const uint2 u = *reinterpret_cast<const uint2*>(p);
+  v[0] = __uint_as_float(u.x << 16); v[1] = __uint_as_float(u.x & 0xFFFF0000u);
+  v[2] = __uint_as_float(u.y << 16); v[3] = __uint_as_float(u.y & 0xFFFF0000u);
+}
+__device__ __forceinline__ void st8(float* p, const float v[8]) {
+  reinterpret_cast<float4*>(p)[0] = make_float4(v[0], v[1], v[2], v[3]);
+  reinterpret_cast<float4*>(p)[1] = make_float4(v[4], v[5], v[6], v[7]);
+}
+__device__ __forceinline__ uint32_t pack2(float a, float b) {
+  return uint32_t(from_f<bf16>(a).v) | (uint32_t(from_f<bf16>(b).v) << 16);
+}
+__device__ __forceinline__ void st8(bf16* p, const float v[8]) {
+  *reinterpret_cast<uint4*>(p) =
+      make_uint4(pack2(v[0], v[1]), pack2(v[2], v[3]), pack2(v[4], v[5]), pack2(v[6], v[7]));
+}
+__device__ __forceinline__ void st4(float* p, const float v[4]) {
+  *reinterpret_cast<float4*>(p) = make_float4(v[0], v[1], v[2], v[3]);
+}
+__device__ __forceinline__ void st4(bf16* p, const float v[4]) {
+  *reinterpret_cast<uint2*>(p) = make_uint2(pack2(v[0], v[1]), pack2(v[2], v[3]));
+}
+__device__ __forceinline__ void ld8d(const double* p, double v[8]) {
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const double2 a = reinterpret_cast<const double2*>(p)[q];
+    v[2 * q] = a.x;
+    v[2 * q + 1] = a.y;
+  }
+}
+__device__ __forceinline__ void st8d(double* p, const double v[8]) {
+#pragma unroll
+  for (int q = 0; q < 4; ++q) reinterpret_cast<double2*>(p)[q] = make_double2(v[2 * q], v[2 * q + 1]);
+}
+
+// ------------------------------------------------------------------ scalar stencil row
 // Off-diagonal sum and diagonal of the face operator at cell (k, j, i), index n.
 // xat(m, kk, jj, ii) returns x at a neighbour.  Accumulation in type A.
 template <typename A, typename C, typename XF>
@@ -51,6 +114,164 @@ __device__ __forceinline__ void decompose(const hfm_mg_task& t, int n, int& i, i
   j = int(row) - k * t.H;
 }
 
+// ------------------------------------------------------------------ vector stencil run
+struct Run {
+  int n0, i0, j, k;
+};
+
+template <typename A>
+__device__ __forceinline__ A shfl_up1(A v) { return __shfl_up(v, 1, 64); }
+template <typename A>
+__device__ __forceinline__ A shfl_down1(A v) { return __shfl_down(v, 1, 64); }
+
+// Face coefficients of a run: west face cw[m], east ce[m], south cs, north cn,
+// down cd, up cu.  Every lane executes the shuffle; `valid` gates the loads.
+template <typename C>
+struct RunCoef {
+  float cw[8], ce[8], cs[8], cn[8], cd[8], cu[8];
+  __device__ __forceinline__ void load(const hfm_mg_task& t, const Run& r, bool valid) {
+    const C* cx = static_cast<const C*>(t.c[0]);
+    const C* cy = static_cast<const C*>(t.c[1]);
+    const C* cz = static_cast<const C*>(t.c[2]);
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int m = 0; m < 8; ++m) cw[m] = cs[m] = cn[m] = cd[m] = cu[m] = 0.0f;
+    if (valid) ld8(cx + r.n0, cw);
+    float eface = shfl_down1(cw[0]);
+    if (valid) {
+      if (r.i0 + kRun == t.W) eface = ldc<C>(t.c[3], r.k * t.H + r.j);
+      else if (lane == 63) eface = ldc<C>(cx, r.n0 + kRun);
+      ld8(cy + r.n0, cs);
+      if (r.j + 1 < t.H) ld8(cy + r.n0 + t.W, cn);
+      else ld8(static_cast<const C*>(t.c[4]) + r.k * t.W + r.i0, cn);
+      ld8(cz + r.n0, cd);
+      if (r.k + 1 < t.D) ld8(cz + r.n0 + t.H * t.W, cu);
+      else ld8(static_cast<const C*>(t.c[5]) + r.j * t.W + r.i0, cu);
+    }
+#pragma unroll
+    for (int m = 0; m < 7; ++m) ce[m] = cw[m + 1];
+    ce[7] = eface;
+  }
+  template <typename A>
+  __device__ __forceinline__ void diag(A d[8]) const {
+#pragma unroll
+    for (int m = 0; m < 8; ++m)
+      d[m] = (A)cw[m] + (A)ce[m] + (A)cs[m] + (A)cn[m] + (A)cd[m] + (A)cu[m];
+  }
+};
+
+// x sources for the run stencil: run(n, k, j, i0, v) loads 8 values at a
+// row-aligned position, one(n, k, j, i) a single value.
+template <typename T>
+struct SrcPlain {
+  const T* x;
+  template <typename A>
+  __device__ __forceinline__ void run(int n, int, int, int, A v[8]) const {
+    float f[8];
+    ld8(x + n, f);
+#pragma unroll
+    for (int m = 0; m < 8; ++m) v[m] = (A)f[m];
+  }
+  __device__ __forceinline__ float one(int n, int, int, int) const { return to_f(x[n]); }
+};
+
+// x := xin + alpha * P ec (nearest prolongation of the coarse correction)
+template <typename T>
+struct SrcCorrect {
+  const T* x;
+  const T* ec;
+  int Hc, Wc;
+  float alpha;
+  template <typename A>
+  __device__ __forceinline__ void run(int n, int k, int j, int i0, A v[8]) const {
+    float f[8], e[4];
+    ld8(x + n, f);
+    ld4(ec + ((k >> 1) * Hc + (j >> 1)) * Wc + (i0 >> 1), e);
+#pragma unroll
+    for (int m = 0; m < 8; ++m) v[m] = (A)(f[m] + alpha * e[m >> 1]);
+  }
+  __device__ __forceinline__ float one(int n, int k, int j, int i) const {
+    return to_f(x[n]) + alpha * to_f(ec[((k >> 1) * Hc + (j >> 1)) * Wc + (i >> 1)]);
+  }
+};
+
+// p := T(z + beta p_old) (p_old unread when beta == 0), in FP64 then rounded
+template <typename T>
+struct SrcDir {
+  const T* z;
+  const T* pold;
+  double beta;
+  __device__ __forceinline__ double make(float zv, float pv) const {
+    const double v = beta == 0.0 ? double(zv) : double(zv) + beta * double(pv);
+    return double(round_to<T>(float(v)));
+  }
+  template <typename A>
+  __device__ __forceinline__ void run(int n, int, int, int, A v[8]) const {
+    float zf[8], pf[8];
+    ld8(z + n, zf);
+    if (beta != 0.0) ld8(pold + n, pf);
+#pragma unroll
+    for (int m = 0; m < 8; ++m) v[m] = (A)make(zf[m], beta != 0.0 ? pf[m] : 0.0f);
+  }
+  __device__ __forceinline__ double one(int n, int, int, int) const {
+    return make(to_f(z[n]), beta != 0.0 ? to_f(pold[n]) : 0.0f);
+  }
+};
+
+// xc = x on the run, off = sum of face * neighbour (w, e, s, n, d, u order).
+template <typename A, typename C, typename SRC>
+__device__ __forceinline__ void run_offdiag(const hfm_mg_task& t, const Run& r, bool valid,
+                                            const RunCoef<C>& q, const SRC& src, A xc[8],
+                                            A off[8]) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int m = 0; m < 8; ++m) xc[m] = (A)0;
+  if (valid) src.run(r.n0, r.k, r.j, r.i0, xc);
+  A wv = shfl_up1(xc[7]);
+  A ev = shfl_down1(xc[0]);
+  if (valid) {
+    if (r.i0 == 0) wv = (A)0;
+    else if (lane == 0) wv = (A)src.one(r.n0 - 1, r.k, r.j, r.i0 - 1);
+    if (r.i0 + kRun == t.W) ev = (A)0;
+    else if (lane == 63) ev = (A)src.one(r.n0 + kRun, r.k, r.j, r.i0 + kRun);
+  }
+#pragma unroll
+  for (int m = 0; m < 8; ++m) off[m] = (A)q.cw[m] * (m ? xc[m - 1] : wv);
+#pragma unroll
+  for (int m = 0; m < 8; ++m) off[m] += (A)q.ce[m] * (m < 7 ? xc[m + 1] : ev);
+  A v[8];
+  const int W = t.W, HW = t.H * t.W;
+#pragma unroll
+  for (int m = 0; m < 8; ++m) v[m] = (A)0;
+  if (valid && r.j > 0) src.run(r.n0 - W, r.k, r.j - 1, r.i0, v);
+#pragma unroll
+  for (int m = 0; m < 8; ++m) off[m] += (A)q.cs[m] * v[m];
+#pragma unroll
+  for (int m = 0; m < 8; ++m) v[m] = (A)0;
+  if (valid && r.j + 1 < t.H) src.run(r.n0 + W, r.k, r.j + 1, r.i0, v);
+#pragma unroll
+  for (int m = 0; m < 8; ++m) off[m] += (A)q.cn[m] * v[m];
+#pragma unroll
+  for (int m = 0; m < 8; ++m) v[m] = (A)0;
+  if (valid && r.k > 0) src.run(r.n0 - HW, r.k - 1, r.j, r.i0, v);
+#pragma unroll
+  for (int m = 0; m < 8; ++m) off[m] += (A)q.cd[m] * v[m];
+#pragma unroll
+  for (int m = 0; m < 8; ++m) v[m] = (A)0;
+  if (valid && r.k + 1 < t.D) src.run(r.n0 + HW, r.k + 1, r.j, r.i0, v);
+#pragma unroll
+  for (int m = 0; m < 8; ++m) off[m] += (A)q.cu[m] * v[m];
+}
+
+__device__ __forceinline__ bool vec_task(const hfm_mg_task& t) { return (t.W & (kRun - 1)) == 0; }
+
+__device__ __forceinline__ Run run_at(const hfm_mg_task& t, int n0, bool valid) {
+  Run r;
+  r.n0 = n0;
+  decompose(t, valid ? n0 : 0, r.i0, r.j, r.k);
+  return r;
+}
+
 // ------------------------------------------------------------------ sweeps
 template <typename T, int MODE, bool DOT>
 __global__ __launch_bounds__(kBlock) void k_sweep(const hfm_mg_task* __restrict__ tasks,
@@ -62,14 +283,43 @@ __global__ __launch_bounds__(kBlock) void k_sweep(const hfm_mg_task* __restrict_
   if (!sc[t.inst].active) return;
   const int chunk = xcd_chunk(g - t.blk0, t.nblk);
   const int N = t.D * t.H * t.W;
-  const int base = chunk * kChunk;
+  const int base = chunk * kCellChunk;
   const T* xin = static_cast<const T*>(t.xin);
   const T* b = static_cast<const T*>(t.b);
   const T* ec = static_cast<const T*>(t.ec);
   T* out = static_cast<T*>(t.xout);
   double acc = 0.0;
-  if (base < N) {
-    for (int s = 0; s < kCPT; ++s) {
+  if (vec_task(t)) {
+    const int n0 = base + threadIdx.x * kRun;
+    const bool valid = n0 < N;
+    const Run r = run_at(t, n0, valid);
+    RunCoef<T> q;
+    q.load(t, r, valid);
+    float d[8], bv[8], o[8];
+    q.diag(d);
+    if (valid) ld8(b + n0, bv);
+    if (MODE == HFM_SWEEP_ZERO) {
+#pragma unroll
+      for (int m = 0; m < 8; ++m) o[m] = omega * bv[m] / d[m];
+    } else {
+      float xc[8], off[8];
+      if (MODE == HFM_SWEEP_PLAIN) {
+        run_offdiag<float>(t, r, valid, q, SrcPlain<T>{xin}, xc, off);
+      } else {
+        run_offdiag<float>(t, r, valid, q, SrcCorrect<T>{xin, ec, t.Hc, t.Wc, alpha}, xc, off);
+      }
+#pragma unroll
+      for (int m = 0; m < 8; ++m) o[m] = xc[m] + omega * (bv[m] - (d[m] * xc[m] - off[m])) / d[m];
+    }
+    if (valid) {
+      st8(out + n0, o);
+      if (DOT) {
+#pragma unroll
+        for (int m = 0; m < 8; ++m) acc += double(bv[m]) * double(round_to<T>(o[m]));
+      }
+    }
+  } else if (base < N) {
+    for (int s = 0; s < kRun; ++s) {
       const int n = base + s * kBlock + threadIdx.x;
       if (n >= N) break;
       int i, j, k;
@@ -77,7 +327,6 @@ __global__ __launch_bounds__(kBlock) void k_sweep(const hfm_mg_task* __restrict_
       const float bn = to_f(b[n]);
       float r;
       if (MODE == HFM_SWEEP_ZERO) {
-        // diagonal only: sum of the six faces
         float diag;
         face_row<float, T>(t, n, i, j, k, [](int, int, int, int) { return 0.0f; }, diag);
         r = omega * bn / diag;
@@ -87,7 +336,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep(const hfm_mg_task* __restrict_
         const float off = face_row<float, T>(t, n, i, j, k, xat, diag);
         const float xc = to_f(xin[n]);
         r = xc + omega * (bn - (diag * xc - off)) / diag;
-      } else {  // CORRECT: x := xin + alpha * P ec on the fly
+      } else {
         const int Hc = t.Hc, Wc = t.Wc;
         auto xat = [&](int m, int kk, int jj, int ii) {
           return to_f(xin[m]) + alpha * to_f(ec[((kk >> 1) * Hc + (jj >> 1)) * Wc + (ii >> 1)]);
@@ -109,7 +358,9 @@ __global__ __launch_bounds__(kBlock) void k_sweep(const hfm_mg_task* __restrict_
 }
 
 // ------------------------------------------------------------------ residual + restriction
-// One thread per coarse cell: bc = sum over children (b - A x) / 8.
+// bc = sum over the (up to 2x2x2) children of (b - A x), over 8.
+// Vector path: a thread owns 4 consecutive coarse cells = a run of 8 fine
+// cells in each of its 2 fine rows x 2 fine planes.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_resid_restrict(const hfm_mg_task* __restrict__ tasks,
                                                            const uint16_t* __restrict__ blk_task,
@@ -119,14 +370,57 @@ __global__ __launch_bounds__(kBlock) void k_resid_restrict(const hfm_mg_task* __
   if (!sc[t.inst].active) return;
   const int chunk = xcd_chunk(g - t.blk0, t.nblk);
   const int Nc = t.Dc * t.Hc * t.Wc;
-  const int base = chunk * kChunk;
-  if (base >= Nc) return;
+  const int base = chunk * kCoarseChunk;
   const T* x = static_cast<const T*>(t.xin);
   const T* b = static_cast<const T*>(t.b);
   T* bc = static_cast<T*>(t.xout);
-  auto xat = [&](int m, int, int, int) { return to_f(x[m]); };
   const int W = t.W, H = t.H, D = t.D;
-  for (int s = 0; s < kCPT; ++s) {
+  if (vec_task(t)) {
+    const int nc0 = base + threadIdx.x * 4;
+    const bool cvalid = nc0 < Nc;
+    int I0 = 0, J = 0, K = 0;
+    if (cvalid) {
+      const uint32_t rowc = fdiv(uint32_t(nc0), t.magWc);
+      I0 = nc0 - int(rowc) * t.Wc;
+      K = int(fdiv(rowc, t.magHc));
+      J = int(rowc) - K * t.Hc;
+    }
+    float s[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (int a = 0; a < 2; ++a) {
+      for (int bb = 0; bb < 2; ++bb) {
+        const int k = 2 * K + a, j = 2 * J + bb;
+        const bool valid = cvalid && k < D && j < H;
+        Run r;
+        r.i0 = 2 * I0;
+        r.j = j;
+        r.k = k;
+        r.n0 = (k * H + j) * W + r.i0;
+        RunCoef<T> q;
+        q.load(t, r, valid);
+        float d[8], xc[8], off[8], bv[8];
+        q.diag(d);
+        run_offdiag<float>(t, r, valid, q, SrcPlain<T>{x}, xc, off);
+        if (valid) {
+          ld8(b + r.n0, bv);
+#pragma unroll
+          for (int m = 0; m < 4; ++m) {
+            s[m] += bv[2 * m] - (d[2 * m] * xc[2 * m] - off[2 * m]);
+            s[m] += bv[2 * m + 1] - (d[2 * m + 1] * xc[2 * m + 1] - off[2 * m + 1]);
+          }
+        }
+      }
+    }
+    if (cvalid) {
+      float o[4];
+#pragma unroll
+      for (int m = 0; m < 4; ++m) o[m] = s[m] * 0.125f;
+      st4(bc + nc0, o);
+    }
+    return;
+  }
+  if (base >= Nc) return;
+  auto xat = [&](int m, int, int, int) { return to_f(x[m]); };
+  for (int s = 0; s < 4; ++s) {
     const int nc = base + s * kBlock + threadIdx.x;
     if (nc >= Nc) break;
     const uint32_t rowc = fdiv(uint32_t(nc), t.magWc);
@@ -187,19 +481,17 @@ __global__ __launch_bounds__(kBlock) void k_fcg_init(const hfm_mg_task* __restri
   if (!sc[t.inst].active) return;
   const int chunk = xcd_chunk(g - t.blk0, t.nblk);
   const int N = t.D * t.H * t.W;
-  const int base = chunk * kChunk;
+  const int base = chunk * kCellChunk;
   double acc = 0.0;
   T* rlow = static_cast<T*>(t.rlow);
-  if (base < N) {
-    for (int s = 0; s < kCPT; ++s) {
-      const int n = base + s * kBlock + threadIdx.x;
-      if (n >= N) break;
-      const double bv = double(t.b32[n]);
-      t.x64[n] = 0.0;
-      t.r64[n] = bv;
-      rlow[n] = from_f<T>(float(bv));
-      acc += bv * bv;
-    }
+  for (int s = 0; s < kRun; ++s) {
+    const int n = base + s * kBlock + threadIdx.x;
+    if (n >= N) break;
+    const double bv = double(t.b32[n]);
+    t.x64[n] = 0.0;
+    t.r64[n] = bv;
+    rlow[n] = from_f<T>(float(bv));
+    acc += bv * bv;
   }
   const double s = block_sum(acc);
   if (threadIdx.x == 0) dots[t.slot0 + chunk] = s;
@@ -217,18 +509,31 @@ __global__ __launch_bounds__(kBlock) void k_fcg_dir(const hfm_mg_task* __restric
   if (!S.active) return;
   const int chunk = xcd_chunk(g - t.blk0, t.nblk);
   const int N = t.D * t.H * t.W;
-  const int base = chunk * kChunk;
-  const double beta = S.beta;
-  const T* z = static_cast<const T*>(t.b);
-  const T* pold = static_cast<const T*>(t.xin);
+  const int base = chunk * kCellChunk;
+  const SrcDir<T> src{static_cast<const T*>(t.b), static_cast<const T*>(t.xin), S.beta};
   T* pnew = static_cast<T*>(t.xout);
-  auto pat = [&](int m, int, int, int) -> double {
-    const double v = beta == 0.0 ? double(to_f(z[m])) : double(to_f(z[m])) + beta * double(to_f(pold[m]));
-    return double(round_to<T>(float(v)));
-  };
   double acc = 0.0;
-  if (base < N) {
-    for (int s = 0; s < kCPT; ++s) {
+  if (vec_task(t)) {
+    const int n0 = base + threadIdx.x * kRun;
+    const bool valid = n0 < N;
+    const Run r = run_at(t, n0, valid);
+    RunCoef<float> q;
+    q.load(t, r, valid);
+    double d[8], p[8], off[8];
+    q.diag(d);
+    run_offdiag<double>(t, r, valid, q, src, p, off);
+    if (valid) {
+      float pf[8];
+#pragma unroll
+      for (int m = 0; m < 8; ++m) {
+        pf[m] = float(p[m]);
+        acc += p[m] * (d[m] * p[m] - off[m]);
+      }
+      st8(pnew + n0, pf);
+    }
+  } else if (base < N) {
+    auto pat = [&](int m, int, int, int) -> double { return src.one(m, 0, 0, 0); };
+    for (int s = 0; s < kRun; ++s) {
       const int n = base + s * kBlock + threadIdx.x;
       if (n >= N) break;
       int i, j, k;
@@ -256,14 +561,39 @@ __global__ __launch_bounds__(kBlock) void k_fcg_update(const hfm_mg_task* __rest
   if (!S.active) return;
   const int chunk = xcd_chunk(g - t.blk0, t.nblk);
   const int N = t.D * t.H * t.W;
-  const int base = chunk * kChunk;
+  const int base = chunk * kCellChunk;
   const double alpha = S.alpha;
-  const T* p = static_cast<const T*>(t.xin);
+  const SrcPlain<T> src{static_cast<const T*>(t.xin)};
   T* rlow = static_cast<T*>(t.rlow);
-  auto pat = [&](int m, int, int, int) -> double { return double(to_f(p[m])); };
   double acc = 0.0;
-  if (base < N) {
-    for (int s = 0; s < kCPT; ++s) {
+  if (vec_task(t)) {
+    const int n0 = base + threadIdx.x * kRun;
+    const bool valid = n0 < N;
+    const Run r = run_at(t, n0, valid);
+    RunCoef<float> q;
+    q.load(t, r, valid);
+    double d[8], p[8], off[8];
+    q.diag(d);
+    run_offdiag<double>(t, r, valid, q, src, p, off);
+    if (valid) {
+      double xv[8], rv[8];
+      float rl[8];
+      ld8d(t.x64 + n0, xv);
+      ld8d(t.r64 + n0, rv);
+#pragma unroll
+      for (int m = 0; m < 8; ++m) {
+        xv[m] += alpha * p[m];
+        rv[m] = rv[m] - alpha * (d[m] * p[m] - off[m]);
+        rl[m] = float(rv[m]);
+        acc += rv[m] * rv[m];
+      }
+      st8d(t.x64 + n0, xv);
+      st8d(t.r64 + n0, rv);
+      st8(rlow + n0, rl);
+    }
+  } else if (base < N) {
+    auto pat = [&](int m, int, int, int) -> double { return double(src.one(m, 0, 0, 0)); };
+    for (int s = 0; s < kRun; ++s) {
       const int n = base + s * kBlock + threadIdx.x;
       if (n >= N) break;
       int i, j, k;
@@ -273,10 +603,10 @@ __global__ __launch_bounds__(kBlock) void k_fcg_update(const hfm_mg_task* __rest
       const double pn = pat(n, k, j, i);
       const double q = diag * pn - off;
       t.x64[n] += alpha * pn;
-      const double r = t.r64[n] - alpha * q;
-      t.r64[n] = r;
-      rlow[n] = from_f<T>(float(r));
-      acc += r * r;
+      const double rr = t.r64[n] - alpha * q;
+      t.r64[n] = rr;
+      rlow[n] = from_f<T>(float(rr));
+      acc += rr * rr;
     }
   }
   const double s = block_sum(acc);

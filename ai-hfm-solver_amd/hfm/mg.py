@@ -28,7 +28,9 @@ import torch
 
 from . import _lib as L
 
-CHUNK = 1024          # cells per block (256 threads x 4)
+CHUNK = 2048          # cells per block of the cell kernels (256 threads x 8)
+RCHUNK = 1024         # coarse cells per block of the residual + restriction
+ALIGN = 8             # element alignment of every per-instance array offset
 ELEM = {"f32": 4, "bf16": 2}
 TORCH_STORE = {"f32": torch.float32, "bf16": torch.bfloat16}
 
@@ -43,6 +45,10 @@ def level_shapes(shape: Sequence[int], max_coarse_edge: int = 4) -> List[Tuple[i
 
 def _round8(n: int) -> int:
     return (n + 7) // 8 * 8
+
+
+def _align(n: int) -> int:
+    return (n + ALIGN - 1) // ALIGN * ALIGN
 
 
 def _plane_sizes(s):
@@ -127,14 +133,14 @@ class MGBatch:
             for i in members:
                 s = self.levels[i][lev]
                 self.cell_off[i][lev] = ncell
-                ncell += math.prod(s)
+                ncell += _align(math.prod(s))
             # face buffer: [cx | cy | cz | bxh | byh | bzh], each concatenated over members
             for i in members:
                 ps = _plane_sizes(self.levels[i][lev])
                 self.face_off[i][lev] = [None] * 6
                 for q in range(3):
                     self.face_off[i][lev][3 + q] = planes[q]
-                    planes[q] += ps[q]
+                    planes[q] += _align(ps[q])
             base = [0, ncell, 2 * ncell, 3 * ncell, 3 * ncell + planes[0],
                     3 * ncell + planes[0] + planes[1]]
             total = 3 * ncell + sum(planes)
@@ -189,11 +195,11 @@ class MGBatch:
         return buf.data_ptr() + self.cell_off[i][0] * buf.element_size()
 
     # ------------------------------------------------------------------ tables
-    def _layout(self, members, cells_of) -> _Layout:
+    def _layout(self, members, cells_of, chunk=CHUNK) -> _Layout:
         blk0, nblk, bt = [], [], []
         g = 0
         for idx, i in enumerate(members):
-            nb = _round8((cells_of(i) + CHUNK - 1) // CHUNK)
+            nb = _round8((cells_of(i) + chunk - 1) // chunk)
             blk0.append(g)
             nblk.append(nb)
             bt.append(np.full(nb, idx, dtype=np.uint16))
@@ -253,7 +259,7 @@ class MGBatch:
                 self.ops_down.append(("sweep", L.SWEEP["zero" if s == 0 else "plain"],
                                       self._upload(tasks), lay, False))
             # residual + restriction to lev+1
-            rlay = self._layout(members, lambda i: math.prod(lv[i][lev + 1]))
+            rlay = self._layout(members, lambda i: math.prod(lv[i][lev + 1]), RCHUNK)
             tasks = []
             for idx, i in enumerate(members):
                 t = self._task(i, lev, rlay, idx, coarse=lv[i][lev + 1])

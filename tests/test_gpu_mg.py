@@ -59,8 +59,10 @@ def test_coarse_inverse(dev):
 
 
 @pytest.mark.parametrize("storage", ["f32", "bf16"])
-def test_solve_matches_oracle(dev, storage):
-    spec = ((32, 32, 32), 7, 1.0, 1)
+@pytest.mark.parametrize("shape", [(32, 32, 32), (20, 12, 36)])
+def test_solve_matches_oracle(dev, storage, shape):
+    # W = 32: 8-wide row kernels on the fine levels; W = 36: per-cell kernels
+    spec = (shape, 7, 1.0, 1)
     mg, probs = _batch(dev, [spec], storage, tol=1e-8, maxit=100)
     info = mg.solve()
     op, b = probs[0]

@@ -57,7 +57,7 @@ def test_fast_division_magic_is_exact():
 def test_work_tables_cover_every_cell_once():
     torch = pytest.importorskip("torch")
     from hfm import _lib as L
-    from hfm.mg import CHUNK, MGBatch
+    from hfm.mg import ALIGN, CHUNK, RCHUNK, MGBatch
     shapes = [(32, 32, 32), (48, 48, 48), (13, 20, 7), (96, 40, 33)]
     mg = MGBatch(shapes, storage="bf16", device="cpu")
     # the fine layout: blocks start at multiples of 8, cover every cell, map to the right task
@@ -84,6 +84,12 @@ def test_work_tables_cover_every_cell_once():
         for t in arr:
             lev = next(l for l in range(mg.nlev) if mg.levels[t["inst"]][l] == (t["D"], t["H"], t["W"]))
             N = t["D"] * t["H"] * t["W"]
+            chunk = RCHUNK if kind == "restrict" else CHUNK
+            cells = t["Dc"] * t["Hc"] * t["Wc"] if kind == "restrict" else N
+            assert t["nblk"] * chunk >= cells and t["nblk"] % 8 == 0 and t["blk0"] % 8 == 0
+            for ptr in list(t["c"]) + [t["xin"], t["xout"], t["b"], t["ec"]]:
+                if ptr:
+                    assert ptr % (ALIGN * 2) == 0   # 8-element aligned for 16-byte bf16 rows
             for f in ("xin", "xout", "b"):
                 if t[f] == 0:
                     continue
