@@ -675,29 +675,94 @@ __global__ __launch_bounds__(kBlock) void k_fcg_finalize(hfm_mg_scalars* sc, con
 }
 
 // ------------------------------------------------------------------ setup kernels
-__global__ __launch_bounds__(kBlock) void k_faces(const float* __restrict__ sx,
-                                                  const float* __restrict__ sy,
-                                                  const float* __restrict__ sz, float* cx,
-                                                  float* cy, float* cz, float* bxh, float* byh,
-                                                  float* bzh, int D, int H, int W, float inv_h2) {
-  const int64_t n = int64_t(blockIdx.x) * kBlock + threadIdx.x;
-  const int64_t N = int64_t(D) * H * W;
-  if (n >= N) return;
+// Harmonic face coefficients of cell n (FP64 arithmetic, rounded to FP32).
+__device__ __forceinline__ void faces_cell(int64_t n, const float* __restrict__ sx,
+                                           const float* __restrict__ sy,
+                                           const float* __restrict__ sz, float* const* o, int D,
+                                           int H, int W, float inv_h2) {
   const int i = int(n % W);
   const int j = int((n / W) % H);
   const int k = int(n / (int64_t(W) * H));
   const double s2 = double(inv_h2);
   auto hm = [](double a, double b) { return 2.0 * a * b / (a + b); };
   const double ax = sx[n], ay = sy[n], az = sz[n];
-  cx[n] = float((i == 0 ? ax : hm(ax, double(sx[n - 1]))) * s2);
-  cy[n] = float((j == 0 ? ay : hm(ay, double(sy[n - W]))) * s2);
-  cz[n] = float((k == 0 ? az : hm(az, double(sz[n - int64_t(W) * H]))) * s2);
-  if (i == W - 1) bxh[int64_t(k) * H + j] = float(ax * s2);
-  if (j == H - 1) byh[int64_t(k) * W + i] = float(ay * s2);
-  if (k == D - 1) bzh[int64_t(j) * W + i] = float(az * s2);
+  o[0][n] = float((i == 0 ? ax : hm(ax, double(sx[n - 1]))) * s2);
+  o[1][n] = float((j == 0 ? ay : hm(ay, double(sy[n - W]))) * s2);
+  o[2][n] = float((k == 0 ? az : hm(az, double(sz[n - int64_t(W) * H]))) * s2);
+  if (i == W - 1) o[3][int64_t(k) * H + j] = float(ax * s2);
+  if (j == H - 1) o[4][int64_t(k) * W + i] = float(ay * s2);
+  if (k == D - 1) o[5][int64_t(j) * W + i] = float(az * s2);
 }
 
-// Coarse faces: (pair over the first listed axis, then the second) / 8, in FP64.
+__global__ __launch_bounds__(kBlock) void k_faces(const float* __restrict__ sx,
+                                                  const float* __restrict__ sy,
+                                                  const float* __restrict__ sz, float* cx,
+                                                  float* cy, float* cz, float* bxh, float* byh,
+                                                  float* bzh, int D, int H, int W, float inv_h2) {
+  const int64_t n = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (n >= int64_t(D) * H * W) return;
+  float* const o[6] = {cx, cy, cz, bxh, byh, bzh};
+  faces_cell(n, sx, sy, sz, o, D, H, W, inv_h2);
+}
+
+// Batched: task slots xin = sx, b = sy, ec = sz, o[0..5] = output faces.
+__global__ __launch_bounds__(kBlock) void k_faces_batched(const hfm_mg_task* __restrict__ tasks,
+                                                          const uint16_t* __restrict__ blk_task,
+                                                          float inv_h2) {
+  const int g = blockIdx.x;
+  const hfm_mg_task& t = tasks[blk_task[g]];
+  const int64_t n = int64_t(g - t.blk0) * kBlock + threadIdx.x;
+  if (n >= int64_t(t.D) * t.H * t.W) return;
+  float* const o[6] = {static_cast<float*>(t.o[0]), static_cast<float*>(t.o[1]),
+                       static_cast<float*>(t.o[2]), static_cast<float*>(t.o[3]),
+                       static_cast<float*>(t.o[4]), static_cast<float*>(t.o[5])};
+  faces_cell(n, static_cast<const float*>(t.xin), static_cast<const float*>(t.b),
+             static_cast<const float*>(t.ec), o, t.D, t.H, t.W, inv_h2);
+}
+
+// Coarse faces of coarse cell n: (pair over the first listed axis, then the
+// second) / 8, in FP64, rounded to FP32.
+__device__ __forceinline__ void galerkin_cell(int64_t n, const float* const* f, float* const* o,
+                                              int D, int H, int W) {
+  const int Dc = (D + 1) / 2, Hc = (H + 1) / 2, Wc = (W + 1) / 2;
+  const int I = int(n % Wc);
+  const int J = int((n / Wc) % Hc);
+  const int K = int(n / (int64_t(Wc) * Hc));
+  auto F = [&](const float* a, int k, int j, int i) -> double {
+    return (k < D && j < H && i < W) ? double(a[(int64_t(k) * H + j) * W + i]) : 0.0;
+  };
+  // x faces at fine i = 2I: sum k pairs, then j pairs
+  o[0][n] = float(((F(f[0], 2 * K, 2 * J, 2 * I) + F(f[0], 2 * K + 1, 2 * J, 2 * I)) +
+                   (F(f[0], 2 * K, 2 * J + 1, 2 * I) + F(f[0], 2 * K + 1, 2 * J + 1, 2 * I))) /
+                  8.0);
+  // y faces at fine j = 2J: sum k pairs, then i pairs
+  o[1][n] = float(((F(f[1], 2 * K, 2 * J, 2 * I) + F(f[1], 2 * K + 1, 2 * J, 2 * I)) +
+                   (F(f[1], 2 * K, 2 * J, 2 * I + 1) + F(f[1], 2 * K + 1, 2 * J, 2 * I + 1))) /
+                  8.0);
+  // z faces at fine k = 2K: sum j pairs, then i pairs
+  o[2][n] = float(((F(f[2], 2 * K, 2 * J, 2 * I) + F(f[2], 2 * K, 2 * J + 1, 2 * I)) +
+                   (F(f[2], 2 * K, 2 * J, 2 * I + 1) + F(f[2], 2 * K, 2 * J + 1, 2 * I + 1))) /
+                  8.0);
+  auto P = [](const float* a, int r, int c, int R, int Cn) -> double {
+    return (r < R && c < Cn) ? double(a[int64_t(r) * Cn + c]) : 0.0;
+  };
+  if (I == Wc - 1)  // bxh (D,H): k pairs, then j pairs
+    o[3][int64_t(K) * Hc + J] =
+        float(((P(f[3], 2 * K, 2 * J, D, H) + P(f[3], 2 * K + 1, 2 * J, D, H)) +
+               (P(f[3], 2 * K, 2 * J + 1, D, H) + P(f[3], 2 * K + 1, 2 * J + 1, D, H))) /
+              8.0);
+  if (J == Hc - 1)  // byh (D,W): k pairs, then i pairs
+    o[4][int64_t(K) * Wc + I] =
+        float(((P(f[4], 2 * K, 2 * I, D, W) + P(f[4], 2 * K + 1, 2 * I, D, W)) +
+               (P(f[4], 2 * K, 2 * I + 1, D, W) + P(f[4], 2 * K + 1, 2 * I + 1, D, W))) /
+              8.0);
+  if (K == Dc - 1)  // bzh (H,W): j pairs, then i pairs
+    o[5][int64_t(J) * Wc + I] =
+        float(((P(f[5], 2 * J, 2 * I, H, W) + P(f[5], 2 * J + 1, 2 * I, H, W)) +
+               (P(f[5], 2 * J, 2 * I + 1, H, W) + P(f[5], 2 * J + 1, 2 * I + 1, H, W))) /
+              8.0);
+}
+
 __global__ __launch_bounds__(kBlock) void k_galerkin(const float* __restrict__ fcx,
                                                      const float* __restrict__ fcy,
                                                      const float* __restrict__ fcz,
@@ -706,46 +771,28 @@ __global__ __launch_bounds__(kBlock) void k_galerkin(const float* __restrict__ f
                                                      const float* __restrict__ fbzh, float* cx,
                                                      float* cy, float* cz, float* bxh, float* byh,
                                                      float* bzh, int D, int H, int W) {
-  const int Dc = (D + 1) / 2, Hc = (H + 1) / 2, Wc = (W + 1) / 2;
   const int64_t n = int64_t(blockIdx.x) * kBlock + threadIdx.x;
-  const int64_t Nc = int64_t(Dc) * Hc * Wc;
-  if (n >= Nc) return;
-  const int I = int(n % Wc);
-  const int J = int((n / Wc) % Hc);
-  const int K = int(n / (int64_t(Wc) * Hc));
-  auto F = [&](const float* a, int k, int j, int i) -> double {
-    return (k < D && j < H && i < W) ? double(a[(int64_t(k) * H + j) * W + i]) : 0.0;
-  };
-  // x faces at fine i = 2I: sum k pairs, then j pairs
-  cx[n] = float(((F(fcx, 2 * K, 2 * J, 2 * I) + F(fcx, 2 * K + 1, 2 * J, 2 * I)) +
-                 (F(fcx, 2 * K, 2 * J + 1, 2 * I) + F(fcx, 2 * K + 1, 2 * J + 1, 2 * I))) /
-                8.0);
-  // y faces at fine j = 2J: sum k pairs, then i pairs
-  cy[n] = float(((F(fcy, 2 * K, 2 * J, 2 * I) + F(fcy, 2 * K + 1, 2 * J, 2 * I)) +
-                 (F(fcy, 2 * K, 2 * J, 2 * I + 1) + F(fcy, 2 * K + 1, 2 * J, 2 * I + 1))) /
-                8.0);
-  // z faces at fine k = 2K: sum j pairs, then i pairs
-  cz[n] = float(((F(fcz, 2 * K, 2 * J, 2 * I) + F(fcz, 2 * K, 2 * J + 1, 2 * I)) +
-                 (F(fcz, 2 * K, 2 * J, 2 * I + 1) + F(fcz, 2 * K, 2 * J + 1, 2 * I + 1))) /
-                8.0);
-  auto P = [](const float* a, int r, int c, int R, int Cn) -> double {
-    return (r < R && c < Cn) ? double(a[int64_t(r) * Cn + c]) : 0.0;
-  };
-  if (I == Wc - 1)  // bxh (D,H): k pairs, then j pairs
-    bxh[int64_t(K) * Hc + J] =
-        float(((P(fbxh, 2 * K, 2 * J, D, H) + P(fbxh, 2 * K + 1, 2 * J, D, H)) +
-               (P(fbxh, 2 * K, 2 * J + 1, D, H) + P(fbxh, 2 * K + 1, 2 * J + 1, D, H))) /
-              8.0);
-  if (J == Hc - 1)  // byh (D,W): k pairs, then i pairs
-    byh[int64_t(K) * Wc + I] =
-        float(((P(fbyh, 2 * K, 2 * I, D, W) + P(fbyh, 2 * K + 1, 2 * I, D, W)) +
-               (P(fbyh, 2 * K, 2 * I + 1, D, W) + P(fbyh, 2 * K + 1, 2 * I + 1, D, W))) /
-              8.0);
-  if (K == Dc - 1)  // bzh (H,W): j pairs, then i pairs
-    bzh[int64_t(J) * Wc + I] =
-        float(((P(fbzh, 2 * J, 2 * I, H, W) + P(fbzh, 2 * J + 1, 2 * I, H, W)) +
-               (P(fbzh, 2 * J, 2 * I + 1, H, W) + P(fbzh, 2 * J + 1, 2 * I + 1, H, W))) /
-              8.0);
+  if (n >= int64_t((D + 1) / 2) * ((H + 1) / 2) * ((W + 1) / 2)) return;
+  const float* const f[6] = {fcx, fcy, fcz, fbxh, fbyh, fbzh};
+  float* const o[6] = {cx, cy, cz, bxh, byh, bzh};
+  galerkin_cell(n, f, o, D, H, W);
+}
+
+// Batched: task extents D/H/W are the fine level's, c[0..5] the fine FP32
+// faces, o[0..5] the coarse FP32 faces; blocks cover the coarse cells.
+__global__ __launch_bounds__(kBlock) void k_galerkin_batched(const hfm_mg_task* __restrict__ tasks,
+                                                             const uint16_t* __restrict__ blk_task) {
+  const int g = blockIdx.x;
+  const hfm_mg_task& t = tasks[blk_task[g]];
+  const int64_t n = int64_t(g - t.blk0) * kBlock + threadIdx.x;
+  if (n >= int64_t((t.D + 1) / 2) * ((t.H + 1) / 2) * ((t.W + 1) / 2)) return;
+  const float* const f[6] = {static_cast<const float*>(t.c[0]), static_cast<const float*>(t.c[1]),
+                             static_cast<const float*>(t.c[2]), static_cast<const float*>(t.c[3]),
+                             static_cast<const float*>(t.c[4]), static_cast<const float*>(t.c[5])};
+  float* const o[6] = {static_cast<float*>(t.o[0]), static_cast<float*>(t.o[1]),
+                       static_cast<float*>(t.o[2]), static_cast<float*>(t.o[3]),
+                       static_cast<float*>(t.o[4]), static_cast<float*>(t.o[5])};
+  galerkin_cell(n, f, o, t.D, t.H, t.W);
 }
 
 template <typename T>
@@ -968,6 +1015,30 @@ extern "C" int hfm_mg_galerkin(const float* const fine[6], float* const coarse[6
                      coarse[0], coarse[1], coarse[2], coarse[3], coarse[4], coarse[5], int(D),
                      int(H), int(W));
   return launch_status("hfm_mg_galerkin");
+}
+
+extern "C" int hfm_mg_faces_batched(const hfm_mg_task* tasks, const uint16_t* blk_task,
+                                    int64_t nblocks, float inv_h2, void* stream) {
+  if (!tasks || (nblocks > 0 && !blk_task) || nblocks < 0 || nblocks > INT32_MAX) {
+    set_error("hfm_mg_faces_batched: bad arguments");
+    return HFM_EINVAL;
+  }
+  if (nblocks == 0) return HFM_OK;
+  hipLaunchKernelGGL(k_faces_batched, dim3(unsigned(nblocks)), dim3(kBlock), 0,
+                     as_stream(stream), tasks, blk_task, inv_h2);
+  return launch_status("hfm_mg_faces_batched");
+}
+
+extern "C" int hfm_mg_galerkin_batched(const hfm_mg_task* tasks, const uint16_t* blk_task,
+                                       int64_t nblocks, void* stream) {
+  if (!tasks || (nblocks > 0 && !blk_task) || nblocks < 0 || nblocks > INT32_MAX) {
+    set_error("hfm_mg_galerkin_batched: bad arguments");
+    return HFM_EINVAL;
+  }
+  if (nblocks == 0) return HFM_OK;
+  hipLaunchKernelGGL(k_galerkin_batched, dim3(unsigned(nblocks)), dim3(kBlock), 0,
+                     as_stream(stream), tasks, blk_task);
+  return launch_status("hfm_mg_galerkin_batched");
 }
 
 extern "C" int hfm_mg_convert(const float* src, void* dst, int64_t n, int store, void* stream) {

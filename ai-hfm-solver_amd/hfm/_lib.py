@@ -33,6 +33,7 @@ TASK_DTYPE = np.dtype([
     ("magWc", "<u8"), ("magHc", "<u8"),
     ("c", "<u8", (6,)), ("xin", "<u8"), ("xout", "<u8"), ("b", "<u8"), ("ec", "<u8"),
     ("x64", "<u8"), ("r64", "<u8"), ("rlow", "<u8"), ("b32", "<u8"),
+    ("o", "<u8", (6,)),
 ])
 
 SCALARS_DTYPE = np.dtype([
@@ -73,6 +74,8 @@ SIGNATURES = {
     "hfm_mg_faces": (_int, [_vp, _vp, _vp, _vp, _i64, _i64, _i64, _f32, _vp]),
     "hfm_mg_galerkin": (_int, [_vp, _vp, _i64, _i64, _i64, _vp]),
     "hfm_mg_convert": (_int, [_vp, _vp, _i64, _int, _vp]),
+    "hfm_mg_faces_batched": (_int, [_vp, _vp, _i64, _f32, _vp]),
+    "hfm_mg_galerkin_batched": (_int, [_vp, _vp, _i64, _vp]),
     "hfm_mg_coarse_inverse": (_int, [_vp, _i64, _vp]),
 }
 

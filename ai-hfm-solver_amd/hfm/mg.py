@@ -31,6 +31,7 @@ from . import _lib as L
 CHUNK = 2048          # cells per block of the cell kernels (256 threads x 8)
 RCHUNK = 1024         # coarse cells per block of the residual + restriction
 ALIGN = 8             # element alignment of every per-instance array offset
+SCHUNK = 256          # cells per block of the batched setup kernels
 ELEM = {"f32": 4, "bf16": 2}
 TORCH_STORE = {"f32": torch.float32, "bf16": torch.bfloat16}
 
@@ -158,6 +159,7 @@ class MGBatch:
             self.Y.append(torch.zeros(ncell, dtype=st, device=dev))
             self.B.append(torch.zeros(ncell, dtype=st, device=dev))
         N0 = self.lvl_cells[0]
+        self.sigma = torch.zeros(3 * N0, dtype=torch.float32, device=dev)  # staged sx | sy | sz
         self.x64 = torch.zeros(N0, dtype=torch.float64, device=dev)
         self.r64 = torch.zeros(N0, dtype=torch.float64, device=dev)
         self.b32 = torch.zeros(N0, dtype=torch.float32, device=dev)
@@ -175,9 +177,7 @@ class MGBatch:
     def nbytes(self) -> int:
         """Device bytes held by this batch."""
         ts = [*self.face32, *self.X, *self.Y, *self.B, self.x64, self.r64, self.b32, *self.P,
-              self.ainv, self.sc, self.dots]
-        if hasattr(self, "sigma"):
-            ts.append(self.sigma)
+              self.ainv, self.sc, self.dots, self.sigma]
         if self.storage != "f32":
             ts += self.faceT
         return sum(t.numel() * t.element_size() for t in ts)
@@ -341,6 +341,28 @@ class MGBatch:
         for i in range(n):
             self.sc_np[i]["slot0"] = flay.blk0[i]
             self.sc_np[i]["nslot"] = flay.nblk[i]
+        # setup kernels: faces from the staged conductivities, Galerkin per level
+        slay = self._layout(list(range(n)), lambda i: math.prod(lv[i][0]), SCHUNK)
+        N0, es = self.lvl_cells[0], 4
+        tasks = []
+        for idx, i in enumerate(range(n)):
+            t = self._task(i, 0, slay, idx)
+            base = self.sigma.data_ptr() + self.cell_off[i][0] * es
+            t["xin"], t["b"], t["ec"] = base, base + N0 * es, base + 2 * N0 * es
+            t["o"] = self._faces(i, 0, False)
+            tasks.append(t)
+        self.faces_tasks, self.faces_layout = self._upload(tasks), slay
+        self.galerkin_tables = []
+        for lev in range(nlev - 1):
+            members = [i for i in range(n) if len(lv[i]) > lev + 1]
+            glay = self._layout(members, lambda i: math.prod(lv[i][lev + 1]), SCHUNK)
+            tasks = []
+            for idx, i in enumerate(members):
+                t = self._task(i, lev, glay, idx)
+                t["c"] = self._faces(i, lev, False)
+                t["o"] = self._faces(i, lev + 1, False)
+                tasks.append(t)
+            self.galerkin_tables.append((self._upload(tasks), glay))
 
     # ------------------------------------------------------------------ problem setup
     def set_problem(self, i: int, sx, sy, sz, b, inv_h2: float = 1.0):
@@ -365,8 +387,6 @@ class MGBatch:
         built from them by ``build_faces``) and load its right-hand side."""
         shape = self.shapes[i]
         N = math.prod(shape)
-        if not hasattr(self, "sigma"):
-            self.sigma = torch.zeros(3 * self.lvl_cells[0], dtype=torch.float32, device=self.device)
         o = self.cell_off[i][0]
         for q, a in enumerate((sx, sy, sz)):
             a = torch.as_tensor(a).to(device=self.device, dtype=torch.float32).reshape(-1)
@@ -379,17 +399,11 @@ class MGBatch:
         self.b32[o:o + N].copy_(b)
 
     def build_faces(self, inv_h2: float = 1.0):
-        """Harmonic face coefficients of every staged instance (device kernels)."""
+        """Harmonic face coefficients of every staged instance (one batched launch)."""
         self._need_gpu()
-        s = L.stream_handle(self.device)
-        N0 = self.lvl_cells[0]
-        es = 4
-        for i in range(self.ninst):
-            o = self.cell_off[i][0]
-            base = self.sigma.data_ptr() + o * es
-            out = L.ptr_array(self._faces(i, 0, False))
-            L.call("hfm_mg_faces", base, base + N0 * es, base + 2 * N0 * es, out, *self.shapes[i],
-                   float(inv_h2), s)
+        lay = self.faces_layout
+        L.call("hfm_mg_faces_batched", self.faces_tasks.data_ptr(), lay.blk_task.data_ptr(),
+               lay.nblocks, float(inv_h2), L.stream_handle(self.device))
 
     def set_faces(self, i: int, faces, b):
         """Load instance ``i`` from FP32 face arrays (cx, cy, cz, bxh, byh, bzh) and rhs."""
@@ -416,11 +430,9 @@ class MGBatch:
         """Galerkin coarse operators, storage copies and coarsest inverses."""
         self._need_gpu()
         s = L.stream_handle(self.device)
-        for i in range(self.ninst):
-            for lev in range(len(self.levels[i]) - 1):
-                fine = L.ptr_array(self._faces(i, lev, False))
-                coarse = L.ptr_array(self._faces(i, lev + 1, False))
-                L.call("hfm_mg_galerkin", fine, coarse, *self.levels[i][lev], s)
+        for tasks, lay in self.galerkin_tables:
+            L.call("hfm_mg_galerkin_batched", tasks.data_ptr(), lay.blk_task.data_ptr(),
+                   lay.nblocks, s)
         if self.storage != "f32":
             for lev in range(self.nlev):
                 L.call("hfm_mg_convert", self.face32[lev].data_ptr(), self.faceT[lev].data_ptr(),

@@ -141,6 +141,7 @@ typedef struct hfm_mg_task {
   double* r64;
   void* rlow;
   const float* b32;
+  void* o[6];                   /* output faces of the batched setup kernels     */
 } hfm_mg_task;
 
 /* Per-instance solver state (device memory), FP64 flexible CG. */
@@ -218,6 +219,18 @@ int hfm_mg_faces(const float* sx, const float* sy, const float* sz, float* const
 int hfm_mg_galerkin(const float* const fine[6], float* const coarse[6], int64_t D, int64_t H,
                     int64_t W, void* stream);
 int hfm_mg_convert(const float* src, void* dst, int64_t n, int store, void* stream);
+
+/* Batched setup over a work table with 256 cells per block (blk_task as for
+ * the cell kernels, tasks need not be padded to 8 blocks):
+ * faces_batched:    per task xin = sx, b = sy, ec = sz (FP32 cell
+ *                   conductivities of extents D/H/W), o[0..5] = output faces;
+ *                   blocks cover the cells.
+ * galerkin_batched: per task D/H/W = fine extents, c[0..5] = fine FP32 faces,
+ *                   o[0..5] = coarse FP32 faces; blocks cover the coarse cells. */
+int hfm_mg_faces_batched(const hfm_mg_task* tasks, const uint16_t* blk_task, int64_t nblocks,
+                         float inv_h2, void* stream);
+int hfm_mg_galerkin_batched(const hfm_mg_task* tasks, const uint16_t* blk_task, int64_t nblocks,
+                            void* stream);
 int hfm_mg_coarse_inverse(const hfm_mg_task* tasks, int64_t ntasks, void* stream);
 
 #ifdef __cplusplus

@@ -30,7 +30,7 @@ def test_abi_struct_sizes_match_host_dtypes():
     lib = L.load()
     a, b = L.C.c_int64(), L.C.c_int64()
     assert lib.hfm_struct_sizes(L.C.byref(a), L.C.byref(b)) == 0
-    assert a.value == L.TASK_DTYPE.itemsize == 192
+    assert a.value == L.TASK_DTYPE.itemsize == 240
     assert b.value == L.SCALARS_DTYPE.itemsize == 96
 
 
