@@ -478,7 +478,8 @@ __global__ __launch_bounds__(kBlock) void k_fcg_init(const hfm_mg_task* __restri
                                                      double* dots) {
   const int g = blockIdx.x;
   const hfm_mg_task& t = tasks[blk_task[g]];
-  if (!sc[t.inst].active) return;
+  const hfm_mg_scalars& S = sc[t.inst];
+  if (!S.active || S.iter != -1) return;  // staged slots only
   const int chunk = xcd_chunk(g - t.blk0, t.nblk);
   const int N = t.D * t.H * t.W;
   const int base = chunk * kCellChunk;
@@ -618,6 +619,9 @@ __global__ __launch_bounds__(kBlock) void k_fcg_finalize(hfm_mg_scalars* sc, con
                                                          int kind) {
   hfm_mg_scalars& S = sc[blockIdx.x];
   if (!S.active) return;
+  // INIT applies to slots staged by the host (iter == -1), the other steps to
+  // initialised slots; a pool can then admit instances while others iterate.
+  if ((kind == HFM_FIN_INIT) != (S.iter == -1)) return;
   double acc = 0.0;
   for (int q = threadIdx.x; q < S.nslot; q += kBlock) acc += dots[S.slot0 + q];
   const double s = block_sum(acc);
@@ -800,6 +804,16 @@ __global__ __launch_bounds__(kBlock) void k_convert(const float* __restrict__ sr
                                                     int64_t n) {
   const int64_t q = int64_t(blockIdx.x) * kBlock + threadIdx.x;
   if (q < n) dst[q] = from_f<T>(src[q]);
+}
+
+// Batched: task xin = FP32 source, xout = destination (T), D = element count.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_convert_batched(const hfm_mg_task* __restrict__ tasks,
+                                                            const uint16_t* __restrict__ blk_task) {
+  const int g = blockIdx.x;
+  const hfm_mg_task& t = tasks[blk_task[g]];
+  const int64_t q = int64_t(g - t.blk0) * kBlock + threadIdx.x;
+  if (q < t.D) static_cast<T*>(t.xout)[q] = from_f<T>(static_cast<const float*>(t.xin)[q]);
 }
 
 // Dense FP64 inverse of the coarsest face operator by Gauss-Jordan elimination
@@ -1051,6 +1065,20 @@ extern "C" int hfm_mg_convert(const float* src, void* dst, int64_t n, int store,
     using T = decltype(tag);
     hipLaunchKernelGGL(k_convert<T>, dim3(unsigned((n + kBlock - 1) / kBlock)), dim3(kBlock), 0,
                        as_stream(stream), src, static_cast<T*>(dst), n);
+  });
+}
+
+extern "C" int hfm_mg_convert_batched(const hfm_mg_task* tasks, const uint16_t* blk_task,
+                                      int64_t nblocks, int store, void* stream) {
+  if (!tasks || (nblocks > 0 && !blk_task) || nblocks < 0 || nblocks > INT32_MAX) {
+    set_error("hfm_mg_convert_batched: bad arguments");
+    return HFM_EINVAL;
+  }
+  if (nblocks == 0) return HFM_OK;
+  return launch_store(store, "hfm_mg_convert_batched", [&](auto tag) {
+    using T = decltype(tag);
+    hipLaunchKernelGGL(k_convert_batched<T>, dim3(unsigned(nblocks)), dim3(kBlock), 0,
+                       as_stream(stream), tasks, blk_task);
   });
 }
 

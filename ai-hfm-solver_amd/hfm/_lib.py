@@ -76,6 +76,7 @@ SIGNATURES = {
     "hfm_mg_convert": (_int, [_vp, _vp, _i64, _int, _vp]),
     "hfm_mg_faces_batched": (_int, [_vp, _vp, _i64, _f32, _vp]),
     "hfm_mg_galerkin_batched": (_int, [_vp, _vp, _i64, _vp]),
+    "hfm_mg_convert_batched": (_int, [_vp, _vp, _i64, _int, _vp]),
     "hfm_mg_coarse_inverse": (_int, [_vp, _i64, _vp]),
 }
 

@@ -1,0 +1,188 @@
+"""Continuous-batching request loop for anisotropic forward solves.
+
+Requests (a grid, conductivities and a right-hand side) are queued on
+arrival.  Each epoch the loop
+  1. admits queued requests while their resident hierarchies fit the HBM
+     budget and a solver slot is free (first come, first served),
+  2. builds the new hierarchies (one batched setup launch sequence),
+  3. rebuilds the launch plan over every resident request when the set
+     changed and initialises the newly admitted ones,
+  4. runs ``iters_per_epoch`` FCG iterations for all residents in shared
+     launches (requests that arrive meanwhile join at the next epoch),
+  5. retires converged (or failed) requests and frees their memory.
+Every request's arithmetic is independent of the others (per-instance
+kernels and fixed-order reductions), so a result is bitwise the same alone
+and in any mix.
+
+The reference has no serving layer; this is the request loop around the
+multigrid solve of ART/BGS_multigrid_v1.ipynb#c42's operator.
+"""
+from __future__ import annotations
+
+import collections
+import dataclasses
+import time
+from typing import Deque, Dict, List, Optional
+
+import numpy as np
+import torch
+
+from . import _lib as L
+from .mg import Instance, Params, Plan, Pool, SetupPlan, check_shape
+
+
+@dataclasses.dataclass
+class Request:
+    rid: int
+    shape: tuple
+    sx: torch.Tensor
+    sy: torch.Tensor
+    sz: torch.Tensor
+    b: torch.Tensor
+    t_arrival: float = 0.0
+
+
+@dataclasses.dataclass
+class Result:
+    rid: int
+    shape: tuple
+    status: str
+    iters: int
+    rel_residual: float
+    x: Optional[torch.Tensor]
+    t_arrival: float
+    t_admitted: float
+    t_done: float
+
+    @property
+    def latency(self) -> float:
+        return self.t_done - self.t_arrival
+
+
+class SolveServer:
+    """HBM-budgeted continuous batching of multigrid solves on one GPU."""
+
+    def __init__(self, params: Optional[Params] = None, hbm_budget: float = 200e9,
+                 capacity: int = 4096, dot_slots: int = 1 << 22, iters_per_epoch: int = 4,
+                 device=None, clock=time.perf_counter):
+        self.params = params or Params()
+        self.pool = Pool(capacity, dot_slots, self.params, device)
+        if self.pool.plan_only:
+            raise RuntimeError("SolveServer needs a GPU")
+        self.device = self.pool.device
+        self.budget = float(hbm_budget)
+        self.used = 0.0
+        self.iters_per_epoch = int(iters_per_epoch)
+        self.clock = clock
+        self.queue: Deque[Request] = collections.deque()
+        self.resident: Dict[int, tuple] = {}   # rid -> (request, instance, bytes, t_admitted)
+        self.plan: Optional[Plan] = None
+        self.parity = 0
+        self.epochs = 0
+        self.peak_used = 0.0
+
+    # ------------------------------------------------------------------ intake
+    def submit(self, rid: int, sx, sy, sz, b, t_arrival: Optional[float] = None) -> None:
+        """Queue a solve of div(sigma grad phi) = b on the grid of ``b``'s shape."""
+        shape = check_shape(tuple(b.shape), self.params.max_coarse_edge)[0]
+        need = Instance.estimate_bytes(shape, self.params)
+        if need > self.budget:
+            raise ValueError(f"request {rid}: {need / 1e9:.1f} GB exceeds the HBM budget")
+        self.queue.append(Request(rid, shape, sx, sy, sz, b,
+                                  self.clock() if t_arrival is None else t_arrival))
+
+    def pending(self) -> int:
+        return len(self.queue) + len(self.resident)
+
+    # ------------------------------------------------------------------ epoch
+    def _admit(self) -> List[Instance]:
+        new = []
+        while self.queue and self.pool.free_slots() > 0:
+            req = self.queue[0]
+            need = Instance.estimate_bytes(req.shape, self.params)
+            if self.used + need > self.budget:
+                break
+            self.queue.popleft()
+            inst = Instance(self.pool, req.shape)
+            inst.stage_sigma(req.sx, req.sy, req.sz, req.b)
+            req.sx = req.sy = req.sz = req.b = None   # the instance holds its own copies
+            self.used += need
+            self.peak_used = max(self.peak_used, self.used)
+            self.resident[req.rid] = (req, inst, need, self.clock())
+            new.append(inst)
+        return new
+
+    def step(self) -> List[Result]:
+        """One epoch; returns the requests that finished in it."""
+        s = L.stream_handle(self.device)
+        new = self._admit()
+        if new:
+            sp = SetupPlan(self.pool, new)
+            sp.build_faces(s)
+            sp.build_hierarchy(s)
+            self.pool.stage(new)
+        if new or self.plan is None:
+            insts = [v[1] for v in self.resident.values()]
+            self.plan = Plan(self.pool, insts) if insts else None
+        if self.plan is None:
+            return []
+        if new:
+            self.plan.init(s)
+        for _ in range(self.iters_per_epoch):
+            self.plan.iteration(s, self.parity)
+            self.parity ^= 1
+        self.epochs += 1
+        sc = self.pool.scalars()           # synchronises with the epoch's launches
+        now = self.clock()
+        done = []
+        for rid, (req, inst, need, t_adm) in list(self.resident.items()):
+            rec = sc[inst.slot]
+            if int(rec["active"]) == 1:
+                continue
+            status = L.STATUS.get(int(rec["status"]), "unknown")
+            done.append(Result(rid, req.shape, status, int(rec["iter"]), float(rec["rel"]),
+                               inst.x64, req.t_arrival, t_adm, now))
+            inst.release()
+            self.used -= need
+            del self.resident[rid]
+        if done:
+            self.plan = None      # rebuilt over the remaining residents next epoch
+        return done
+
+    def drain(self) -> List[Result]:
+        out = []
+        while self.pending():
+            out += self.step()
+        return out
+
+
+def poisson_stream(n: int, rate: float, seed: int, lo: int = 32, hi: int = 256):
+    """(t_arrival, shape, seed) for n requests: exponential inter-arrival times
+    at ``rate``/s, cube edge log-uniform in [lo, hi] rounded to a multiple of 16."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, n))
+    edges = 16 * np.maximum(1, np.rint(np.exp(rng.uniform(np.log(lo), np.log(hi), n)) / 16))
+    return [(float(t[q]), (int(e),) * 3, int(seed) * 7919 + q) for q, e in enumerate(edges)]
+
+
+def run_stream(server: SolveServer, stream, make_fields, clock=time.perf_counter) -> List[Result]:
+    """Replay a timed request stream against ``server`` in wall-clock time."""
+    t0 = clock()
+    results, q = [], 0
+    while q < len(stream) or server.pending():
+        now = clock() - t0
+        while q < len(stream) and stream[q][0] <= now:
+            t, shape, seed = stream[q]
+            server.submit(q, *make_fields(shape, seed), t_arrival=t0 + t)
+            q += 1
+        if server.pending():
+            results += server.step()
+        elif q < len(stream):
+            time.sleep(max(0.0, stream[q][0] - (clock() - t0)))
+    return results
+
+
+def latency_stats(results: List[Result]) -> dict:
+    lat = np.array([r.latency for r in results]) * 1e3
+    return {"p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+            "mean_ms": float(lat.mean()), "n": len(results)}

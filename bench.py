@@ -113,6 +113,41 @@ def fp32_one_at_a_time(specs, dev, torch):
     return len(specs) / t_tot
 
 
+def serving(args, dev, torch):
+    """Poisson stream of anisotropic solves, edges 32..256 log-uniform (multiples of
+    16), sigma_ln = 1, one dipole: continuous batching (BF16 storage, 200 GB budget)
+    vs the plain FP32 path serving one request at a time."""
+    from hfm.mg import Params
+    from hfm.server import SolveServer, latency_stats, poisson_stream, run_stream
+    from hfm.workloads import device_fields
+    stream = poisson_stream(args.serve_requests, args.serve_rate, args.seed + 1)
+    mk = lambda shape, seed: device_fields(shape, 1.0, 1, seed, dev)  # noqa: E731
+    out = {}
+    for name, params, cap in (("batched_bf16", Params(storage="bf16", tol=1e-8), 4096),
+                              ("fp32_one_at_a_time", Params(storage="f32", tol=1e-8), 1)):
+        srv = SolveServer(params, hbm_budget=200e9, capacity=cap, device=dev)
+        warm = SolveServer(params, hbm_budget=200e9, capacity=cap, device=dev)
+        warm.submit(-1, *mk((64, 64, 64), 1))
+        warm.drain()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        res = run_stream(srv, stream, mk)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        bad = [r for r in res if r.status != "converged" or r.rel_residual > 1e-8]
+        if bad:
+            raise SystemExit(f"serving {name}: {len(bad)} requests failed")
+        st = latency_stats(res)
+        out[name] = {"solves_per_s": round(len(res) / wall, 2), "p50_ms": round(st["p50_ms"], 2),
+                     "p99_ms": round(st["p99_ms"], 2), "requests": len(res),
+                     "peak_resident_GB": round(srv.peak_used / 1e9, 2), "epochs": srv.epochs}
+        log(f"bench: serving {name}: {out[name]}")
+    out["arrival_rate_per_s"] = args.serve_rate
+    out["speedup_solves_per_s"] = round(out["batched_bf16"]["solves_per_s"] /
+                                        out["fp32_one_at_a_time"]["solves_per_s"], 2)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -124,6 +159,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--fp32-sample", type=int, default=16)
+    ap.add_argument("--serve-requests", type=int, default=300)
+    ap.add_argument("--serve-rate", type=float, default=300.0)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the solves are independent and this build runs "
@@ -183,6 +220,8 @@ def main():
         sps = fp32_one_at_a_time(sub, dev, torch)
         extra["fp32_one_at_a_time_solves_per_s"] = round(sps, 2)
         extra["speedup_vs_fp32_one_at_a_time"] = round(value / sps, 2)
+    if args.serve_requests > 0:
+        extra["serving"] = serving(args, dev, torch)
     cpu = None
     if not args.no_cpu_baseline:
         torch.set_num_threads(1)

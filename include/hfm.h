@@ -144,7 +144,9 @@ typedef struct hfm_mg_task {
   void* o[6];                   /* output faces of the batched setup kernels     */
 } hfm_mg_task;
 
-/* Per-instance solver state (device memory), FP64 flexible CG. */
+/* Per-instance solver state (device memory), FP64 flexible CG.  The host
+ * stages a slot with active = 1 and iter = -1; HFM_FIN_INIT then applies to
+ * staged slots only and the other finalize steps to initialised ones. */
 typedef struct hfm_mg_scalars {
   double bnorm2, rz, alpha, beta;
   double rr, rel, tol, pap;
@@ -186,7 +188,7 @@ int hfm_mg_coarse_solve(const hfm_mg_task* tasks, int64_t ntasks, const hfm_mg_s
                         int store, void* stream);
 
 /* FCG kernels on the fine level (coefficients c in FP32).
- * init:   x64 = 0, r64 = b32, rlow = T(b32), dots = b.b
+ * init:   x64 = 0, r64 = b32, rlow = T(b32), dots = b.b  (staged slots only)
  * dir:    xout = p = T(b + beta xin) (b = z, xin = p_old), dots = p.(A p)
  * update: x64 += alpha p, r64 -= alpha A p, rlow = T(r64), dots = r.r  (xin = p) */
 int hfm_fcg_init(const hfm_mg_task* tasks, const uint16_t* blk_task, int64_t nblocks,
@@ -231,6 +233,10 @@ int hfm_mg_faces_batched(const hfm_mg_task* tasks, const uint16_t* blk_task, int
                          float inv_h2, void* stream);
 int hfm_mg_galerkin_batched(const hfm_mg_task* tasks, const uint16_t* blk_task, int64_t nblocks,
                             void* stream);
+/* convert_batched:  per task xin = FP32 source, xout = destination of storage
+ *                   `store`, D = element count; blocks cover the elements. */
+int hfm_mg_convert_batched(const hfm_mg_task* tasks, const uint16_t* blk_task, int64_t nblocks,
+                           int store, void* stream);
 int hfm_mg_coarse_inverse(const hfm_mg_task* tasks, int64_t ntasks, void* stream);
 
 #ifdef __cplusplus

@@ -26,12 +26,12 @@ def _batch(dev, specs, storage, **kw):
 
 
 def _faces_of(mg, i, lev):
-    import torch
-    D, H, W = mg.levels[i][lev]
+    inst = mg.insts[i]
+    D, H, W = inst.levels[lev]
     sizes = [D * H * W] * 3 + [D * H, D * W, H * W]
     out = []
-    for o, n in zip(mg.face_off[i][lev], sizes):
-        out.append(mg.face32[lev][o:o + n].double().cpu().numpy())
+    for o, n in zip(inst.face_off[lev], sizes):
+        out.append(inst.face32[lev][o:o + n].double().cpu().numpy())
     return out
 
 
@@ -40,11 +40,11 @@ def test_faces_and_galerkin_bitwise(dev):
     mg, probs = _batch(dev, specs, "f32")
     for i, (op, _) in enumerate(probs):
         ex = op.astype("f32")
-        for lev in range(len(mg.levels[i])):
+        for lev in range(len(mg.insts[i].levels)):
             got = _faces_of(mg, i, lev)
             for g, e in zip(got, (ex.cx, ex.cy, ex.cz, ex.bxh, ex.byh, ex.bzh)):
                 np.testing.assert_array_equal(g, e.ravel())
-            if lev + 1 < len(mg.levels[i]):
+            if lev + 1 < len(mg.insts[i].levels):
                 ex = O.galerkin_coarsen(ex).astype("f32")
 
 
@@ -53,8 +53,8 @@ def test_coarse_inverse(dev):
     mg, probs = _batch(dev, specs, "f32")
     for i, (op, _) in enumerate(probs):
         h = O.build_hierarchy(op, "f32")
-        n = int(np.prod(mg.levels[i][-1]))
-        got = mg.ainv[mg.ainv_off[i]:mg.ainv_off[i] + n * n].cpu().numpy().reshape(n, n)
+        n = int(np.prod(mg.insts[i].levels[-1]))
+        got = mg.insts[i].ainv.cpu().numpy().reshape(n, n)
         np.testing.assert_allclose(got, h.coarse_inv, rtol=1e-10, atol=1e-12 * np.abs(h.coarse_inv).max())
 
 

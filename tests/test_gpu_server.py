@@ -1,0 +1,73 @@
+"""The continuous-batching request loop (GPU)."""
+import numpy as np
+import pytest
+
+from oracle import hfm_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _fields(dev, shape, seed, sig=1.0, nd=1):
+    import torch
+    sx, sy, sz, b = O.make_anisotropic_fields(shape, seed, sig, nd)
+    return [torch.from_numpy(a).to(dev) for a in (sx, sy, sz, b)]
+
+
+REQS = [((32, 32, 32), 1, 0.7, 1), ((48, 48, 48), 2, 1.8, 2), ((40, 24, 32), 3, 1.0, 3),
+        ((64, 64, 64), 4, 1.2, 1), ((16, 40, 24), 5, 0.5, 4)]
+
+
+def test_result_is_the_same_alone_and_in_the_mix(dev):
+    from hfm.mg import Params
+    from hfm.server import SolveServer
+    params = Params(storage="bf16", tol=1e-8)
+    # alone
+    alone = {}
+    for rid in (2, 4):
+        srv = SolveServer(params, hbm_budget=8e9, capacity=16, device=dev)
+        srv.submit(rid, *_fields(dev, *REQS[rid][:2], *REQS[rid][2:]))
+        (r,) = srv.drain()
+        assert r.status == "converged"
+        alone[rid] = (r.iters, r.x.clone())
+    # mixed: two requests start, the others arrive while they iterate
+    srv = SolveServer(params, hbm_budget=8e9, capacity=16, iters_per_epoch=3, device=dev)
+    for rid in (0, 1):
+        srv.submit(rid, *_fields(dev, *REQS[rid][:2], *REQS[rid][2:]))
+    results = srv.step()
+    for rid in (2, 3):
+        srv.submit(rid, *_fields(dev, *REQS[rid][:2], *REQS[rid][2:]))
+    results += srv.step()
+    srv.submit(4, *_fields(dev, *REQS[4][:2], *REQS[4][2:]))
+    results += srv.drain()
+    by = {r.rid: r for r in results}
+    assert sorted(by) == [0, 1, 2, 3, 4]
+    for rid in (2, 4):
+        assert by[rid].iters == alone[rid][0]
+        assert bool((by[rid].x == alone[rid][1]).all())
+    for rid, (shape, seed, sig, nd) in enumerate(REQS):
+        assert by[rid].status == "converged" and by[rid].rel_residual <= 1e-8
+        # every result solves its own problem (FP64 residual on the host)
+        sx, sy, sz, b = O.make_anisotropic_fields(shape, seed, sig, nd)
+        op = O.harmonic_faces(sx, sy, sz)
+        x = by[rid].x.cpu().numpy()
+        r = b.astype(np.float64) - O.face_apply(op.astype("f32"), x)
+        assert np.linalg.norm(r) / np.linalg.norm(b) <= 1e-8
+
+
+def test_admission_respects_the_hbm_budget(dev):
+    from hfm.mg import Instance, Params
+    from hfm.server import SolveServer
+    params = Params(storage="bf16", tol=1e-8)
+    one = Instance.estimate_bytes((48, 48, 48), params)
+    srv = SolveServer(params, hbm_budget=2.5 * one, capacity=16, device=dev)
+    for rid in range(5):
+        srv.submit(rid, *_fields(dev, (48, 48, 48), 10 + rid))
+    res = []
+    while srv.pending():
+        assert len(srv.resident) <= 2
+        res += srv.step()
+        assert srv.used <= srv.budget
+    assert sorted(r.rid for r in res) == list(range(5))
+    assert all(r.status == "converged" for r in res)
+    with pytest.raises(ValueError):
+        srv.submit(99, *_fields(dev, (96, 96, 96), 1))

@@ -54,54 +54,66 @@ def test_fast_division_magic_is_exact():
             assert q == int(n) // d, (n, d)
 
 
+def _owned_by(inst, ptr, nbytes):
+    bufs = [*inst.X, *inst.Y, *inst.B, *inst.P, *inst.face32, *inst.faceT, inst.x64, inst.r64,
+            inst.b32, inst.sigma, inst.ainv]
+    return any(b.data_ptr() <= ptr and ptr + nbytes <= b.data_ptr() + b.numel() * b.element_size()
+               for b in bufs)
+
+
 def test_work_tables_cover_every_cell_once():
     torch = pytest.importorskip("torch")
     from hfm import _lib as L
     from hfm.mg import ALIGN, CHUNK, RCHUNK, MGBatch
     shapes = [(32, 32, 32), (48, 48, 48), (13, 20, 7), (96, 40, 33)]
     mg = MGBatch(shapes, storage="bf16", device="cpu")
-    # the fine layout: blocks start at multiples of 8, cover every cell, map to the right task
-    fl = mg.flay
+    plan = mg.plan
+    by_slot = {inst.slot: inst for inst in mg.insts}
+    fl = plan.flay
     bt = fl.blk_task.numpy().view(np.uint16)
     assert bt.size == fl.nblocks
-    tasks = mg.init_tasks.numpy().view(L.TASK_DTYPE)
+    tasks = plan.init_tasks.numpy().view(L.TASK_DTYPE)
+    dot_ranges = []
     for idx, t in enumerate(tasks):
+        inst = by_slot[int(t["inst"])]
         assert t["blk0"] % 8 == 0 and t["nblk"] % 8 == 0
         assert np.all(bt[t["blk0"]:t["blk0"] + t["nblk"]] == idx)
-        N = t["D"] * t["H"] * t["W"]
+        N = int(t["D"]) * int(t["H"]) * int(t["W"])
         assert t["nblk"] * CHUNK >= N > (t["nblk"] - 8) * CHUNK
-        # XCD chunk map is a permutation of 0..nblk-1
         per = t["nblk"] // 8
-        ch = sorted((l % 8) * per + l // 8 for l in range(t["nblk"]))
-        assert ch == list(range(t["nblk"]))
-        # pointers land inside the buffers they belong to
-        for name, buf in (("x64", mg.x64), ("r64", mg.r64), ("b32", mg.b32)):
-            lo = buf.data_ptr()
-            assert lo <= t[name] and t[name] + N * buf.element_size() <= lo + buf.numel() * buf.element_size()
-    # every sweep / restriction task's vectors stay inside its level's buffers
-    for kind, mode, tt, lay, dot in mg.ops_down + [op for ups in mg.ops_up for op in ups]:
-        arr = tt.numpy().view(L.TASK_DTYPE)
-        for t in arr:
-            lev = next(l for l in range(mg.nlev) if mg.levels[t["inst"]][l] == (t["D"], t["H"], t["W"]))
-            N = t["D"] * t["H"] * t["W"]
+        assert sorted((l % 8) * per + l // 8 for l in range(t["nblk"])) == list(range(t["nblk"]))
+        for name, es in (("x64", 8), ("r64", 8), ("b32", 4)):
+            assert _owned_by(inst, int(t[name]), N * es)
+        assert t["slot0"] == inst.dots_off and t["nblk"] == inst.nblk
+        dot_ranges.append((inst.dots_off, inst.dots_off + inst.nblk))
+    # dot-partial ranges of different instances never overlap and fit the pool
+    dot_ranges.sort()
+    for (a0, a1), (b0, b1) in zip(dot_ranges, dot_ranges[1:]):
+        assert a1 <= b0
+    assert dot_ranges[-1][1] <= mg.pool.dots.numel()
+    ops = plan.ops_down + [op for ups in plan.ops_up for op in ups]
+    for kind, mode, tt, lay, dot, lev in ops:
+        for t in tt.numpy().view(L.TASK_DTYPE):
+            inst = by_slot[int(t["inst"])]
+            assert inst.levels[lev] == (t["D"], t["H"], t["W"])
+            N = int(t["D"]) * int(t["H"]) * int(t["W"])
             chunk = RCHUNK if kind == "restrict" else CHUNK
-            cells = t["Dc"] * t["Hc"] * t["Wc"] if kind == "restrict" else N
+            cells = int(t["Dc"]) * int(t["Hc"]) * int(t["Wc"]) if kind == "restrict" else N
             assert t["nblk"] * chunk >= cells and t["nblk"] % 8 == 0 and t["blk0"] % 8 == 0
             for ptr in list(t["c"]) + [t["xin"], t["xout"], t["b"], t["ec"]]:
                 if ptr:
                     assert ptr % (ALIGN * 2) == 0   # 8-element aligned for 16-byte bf16 rows
-            for f in ("xin", "xout", "b"):
-                if t[f] == 0:
-                    continue
-                target_lev = lev + 1 if (kind == "restrict" and f == "xout") else lev
-                n = N if target_lev == lev else t["Dc"] * t["Hc"] * t["Wc"]
-                bufs = [mg.X[target_lev], mg.Y[target_lev], mg.B[target_lev]]
-                assert any(b.data_ptr() <= t[f] and t[f] + n * b.element_size()
-                           <= b.data_ptr() + b.numel() * b.element_size() for b in bufs), (kind, f)
+            es = 2
+            for f in ("xin", "b"):
+                if t[f]:
+                    assert _owned_by(inst, int(t[f]), N * es)
+            n_out = cells if kind == "restrict" else N
+            assert _owned_by(inst, int(t["xout"]), n_out * es)
             if kind == "restrict":
-                assert (t["Dc"], t["Hc"], t["Wc"]) == tuple((v + 1) // 2 for v in (t["D"], t["H"], t["W"]))
-    # coarsest tasks fit the dense-inverse kernel
-    for t in mg.coarse_tasks.numpy().view(L.TASK_DTYPE):
+                assert (t["Dc"], t["Hc"], t["Wc"]) == inst.levels[lev + 1]
+            if t["ec"]:
+                assert _owned_by(inst, int(t["ec"]), int(t["Dc"]) * int(t["Hc"]) * int(t["Wc"]) * es)
+    for t in plan.coarse_tasks.numpy().view(L.TASK_DTYPE):
         assert t["D"] * t["H"] * t["W"] <= 64
 
 
