@@ -141,7 +141,7 @@ class SolveServer:
                 continue
             status = L.STATUS.get(int(rec["status"]), "unknown")
             done.append(Result(rid, req.shape, status, int(rec["iter"]), float(rec["rel"]),
-                               inst.x64, req.t_arrival, t_adm, now))
+                               inst.solution(), req.t_arrival, t_adm, now))
             inst.release()
             self.used -= need
             del self.resident[rid]
