@@ -148,6 +148,35 @@ def serving(args, dev, torch):
     return out
 
 
+def profile_kernel(args, dev, torch):
+    """Launch only one kernel (all instances active) and a copy of known size,
+    for rocprofv3 kernel-trace / PMC runs; prints one JSON line."""
+    import numpy as np
+    from hfm.mg import MGBatch
+    from hfm.workloads import batched_specs
+    specs = batched_specs(args.instances, args.seed)
+    cells = sum(math.prod(s[0]) for s in specs)
+    mg = MGBatch([s[0] for s in specs], storage=args.storage, tol=1e-8, device=dev)
+    load_batch(mg, specs, dev)
+    mg.build_faces()
+    mg.setup()
+    mg.restart()
+    fn = {"update": lambda: mg.launch_update(0), "sweep": mg.launch_fine_sweep}[args.profile_kernel]
+    per_cell = {"update": mg.update_bytes_per_cell(), "sweep": mg.sweep_bytes_per_cell()}
+    ms = kernel_time_ms(fn, args.reps, torch)
+    nbytes = 1 << 30
+    src = torch.empty(nbytes // 4, dtype=torch.float32, device=dev).fill_(1.0)
+    dst = torch.empty_like(src)
+    cp_ms = kernel_time_ms(lambda: dst.copy_(src), 5, torch)
+    alg = cells * per_cell[args.profile_kernel]
+    print(json.dumps({"kernel": args.profile_kernel, "instances": len(specs), "fine_cells": cells,
+                      "alg_bytes_per_launch": alg, "launch_ms": round(ms, 4),
+                      "achieved_GBs": round(alg / (ms * 1e-3) / 1e9, 1), "reps": args.reps + 1,
+                      "copy_bytes_each_way": nbytes, "copy_ms": round(cp_ms, 4),
+                      "copy_GBs_read_plus_write": round(2 * nbytes / (cp_ms * 1e-3) / 1e9, 1)}),
+          flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -161,6 +190,9 @@ def main():
     ap.add_argument("--fp32-sample", type=int, default=16)
     ap.add_argument("--serve-requests", type=int, default=300)
     ap.add_argument("--serve-rate", type=float, default=300.0)
+    ap.add_argument("--profile-kernel", choices=["update", "sweep"], default=None,
+                    help="launch only this kernel (after setup) for profiler runs")
+    ap.add_argument("--reps", type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the solves are independent and this build runs "
@@ -174,6 +206,8 @@ def main():
     from hfm.workloads import batched_specs
 
     dev = torch.device("cuda:0")
+    if args.profile_kernel:
+        return profile_kernel(args, dev, torch)
     specs = batched_specs(args.instances, args.seed)
     shapes = [s[0] for s in specs]
     cells = sum(math.prod(s) for s in shapes)
