@@ -83,6 +83,50 @@ __device__ __forceinline__ void st8d(double* p, const double v[8]) {
   for (int q = 0; q < 4; ++q) reinterpret_cast<double2*>(p)[q] = make_double2(v[2 * q], v[2 * q + 1]);
 }
 
+// Streaming (non-temporal) variants for arrays read or written exactly once
+// per launch: they are not kept in the caches that serve the stencil reuse.
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef double d2v __attribute__((ext_vector_type(2)));
+typedef unsigned int u4v __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ void ld8_nt(const float* p, float v[8]) {
+  const f4v a = __builtin_nontemporal_load(reinterpret_cast<const f4v*>(p));
+  const f4v b = __builtin_nontemporal_load(reinterpret_cast<const f4v*>(p) + 1);
+  v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+  v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+}
+__device__ __forceinline__ void ld8_nt(const bf16* p, float v[8]) {
+  const u4v u = __builtin_nontemporal_load(reinterpret_cast<const u4v*>(p));
+  const uint32_t w[4] = {u.x, u.y, u.z, u.w};
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    v[2 * q] = __uint_as_float(w[q] << 16);
+    v[2 * q + 1] = __uint_as_float(w[q] & 0xFFFF0000u);
+  }
+}
+__device__ __forceinline__ void st8_nt(float* p, const float v[8]) {
+  __builtin_nontemporal_store(f4v{v[0], v[1], v[2], v[3]}, reinterpret_cast<f4v*>(p));
+  __builtin_nontemporal_store(f4v{v[4], v[5], v[6], v[7]}, reinterpret_cast<f4v*>(p) + 1);
+}
+__device__ __forceinline__ void st8_nt(bf16* p, const float v[8]) {
+  __builtin_nontemporal_store(u4v{pack2(v[0], v[1]), pack2(v[2], v[3]), pack2(v[4], v[5]),
+                                  pack2(v[6], v[7])},
+                              reinterpret_cast<u4v*>(p));
+}
+__device__ __forceinline__ void ld8d_nt(const double* p, double v[8]) {
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const d2v a = __builtin_nontemporal_load(reinterpret_cast<const d2v*>(p) + q);
+    v[2 * q] = a.x;
+    v[2 * q + 1] = a.y;
+  }
+}
+__device__ __forceinline__ void st8d_nt(double* p, const double v[8]) {
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+    __builtin_nontemporal_store(d2v{v[2 * q], v[2 * q + 1]}, reinterpret_cast<d2v*>(p) + q);
+}
+
 // ------------------------------------------------------------------ scalar stencil row
 // Off-diagonal sum and diagonal of the face operator at cell (k, j, i), index n.
 // xat(m, kk, jj, ii) returns x at a neighbour.  Accumulation in type A.
@@ -297,7 +341,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep(const hfm_mg_task* __restrict_
     q.load(t, r, valid);
     float d[8], bv[8], o[8];
     q.diag(d);
-    if (valid) ld8(b + n0, bv);
+    if (valid) ld8_nt(b + n0, bv);
     if (MODE == HFM_SWEEP_ZERO) {
 #pragma unroll
       for (int m = 0; m < 8; ++m) o[m] = omega * bv[m] / d[m];
@@ -312,7 +356,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep(const hfm_mg_task* __restrict_
       for (int m = 0; m < 8; ++m) o[m] = xc[m] + omega * (bv[m] - (d[m] * xc[m] - off[m])) / d[m];
     }
     if (valid) {
-      st8(out + n0, o);
+      st8_nt(out + n0, o);
       if (DOT) {
 #pragma unroll
         for (int m = 0; m < 8; ++m) acc += double(bv[m]) * double(round_to<T>(o[m]));
@@ -530,7 +574,7 @@ __global__ __launch_bounds__(kBlock) void k_fcg_dir(const hfm_mg_task* __restric
         pf[m] = float(p[m]);
         acc += p[m] * (d[m] * p[m] - off[m]);
       }
-      st8(pnew + n0, pf);
+      st8_nt(pnew + n0, pf);
     }
   } else if (base < N) {
     auto pat = [&](int m, int, int, int) -> double { return src.one(m, 0, 0, 0); };
@@ -579,8 +623,8 @@ __global__ __launch_bounds__(kBlock) void k_fcg_update(const hfm_mg_task* __rest
     if (valid) {
       double xv[8], rv[8];
       float rl[8];
-      ld8d(t.x64 + n0, xv);
-      ld8d(t.r64 + n0, rv);
+      ld8d_nt(t.x64 + n0, xv);
+      ld8d_nt(t.r64 + n0, rv);
 #pragma unroll
       for (int m = 0; m < 8; ++m) {
         xv[m] += alpha * p[m];
@@ -588,9 +632,9 @@ __global__ __launch_bounds__(kBlock) void k_fcg_update(const hfm_mg_task* __rest
         rl[m] = float(rv[m]);
         acc += rv[m] * rv[m];
       }
-      st8d(t.x64 + n0, xv);
-      st8d(t.r64 + n0, rv);
-      st8(rlow + n0, rl);
+      st8d_nt(t.x64 + n0, xv);
+      st8d_nt(t.r64 + n0, rv);
+      st8_nt(rlow + n0, rl);
     }
   } else if (base < N) {
     auto pat = [&](int m, int, int, int) -> double { return double(src.one(m, 0, 0, 0)); };

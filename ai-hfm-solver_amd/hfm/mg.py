@@ -581,7 +581,7 @@ class MGBatch:
     def __init__(self, shapes: Sequence[Sequence[int]], storage: str = "bf16", tol: float = 1e-8,
                  maxit: int = 200, nu1: int = 2, nu2: int = 2, omega: float = 2.0 / 3.0,
                  alpha: float = 1.6, max_coarse_edge: int = 4, device=None,
-                 iters_per_graph: int = 4, use_graph: bool = True):
+                 iters_per_graph: int = 4, use_graph: bool = True, shrink: float = 0.5):
         self.params = Params(storage, tol, maxit, nu1, nu2, omega, alpha, max_coarse_edge)
         shapes = [check_shape(s, max_coarse_edge)[0] for s in shapes]
         if not shapes:
@@ -597,6 +597,8 @@ class MGBatch:
         self.use_graph = bool(use_graph)
         self.setup_plan = SetupPlan(self.pool, self.insts)
         self.plan = Plan(self.pool, self.insts)
+        self.shrink = float(shrink)
+        self._subplans = {}
         self.last_info = None
 
     def _need_gpu(self):
@@ -634,27 +636,44 @@ class MGBatch:
         self.pool.stage(self.insts)
         self.plan.init(L.stream_handle(self.device))
 
+    def _subplan(self, live):
+        key = tuple(i.slot for i in live)
+        plan = self._subplans.get(key)
+        if plan is None:
+            plan = Plan(self.pool, live)
+            self._subplans[key] = plan
+        return plan
+
     def solve(self):
-        """Run FCG to ``tol`` (or ``maxit``) for every instance; returns the scalar records."""
+        """Run FCG to ``tol`` (or ``maxit``) for every instance; returns the scalar records.
+
+        Iterations run ``iters_per_graph`` at a time (one HIP-graph replay);
+        between replays the active flags are read and, once at most
+        ``shrink`` of the plan's instances are still iterating, the launches
+        continue on a plan of the active ones only (no empty blocks).
+        """
         self._need_gpu()
         s = L.stream_handle(self.device)
         self.restart()
         act = self.pool.active_view()
         its = 0
         maxit = self.params.maxit
-        if self.use_graph:
-            g, m = self.plan.graph(self.iters_per_graph)
-            while its < maxit:
+        plan = self.plan
+        m = self.iters_per_graph + self.iters_per_graph % 2
+        while its < maxit:
+            if self.use_graph:
+                g, m = plan.graph(m)
                 g.replay()
-                its += m
-                if not bool(act.any()):
-                    break
-        else:
-            while its < maxit:
-                self.plan.iteration(s, its % 2)
-                its += 1
-                if its % self.iters_per_graph == 0 and not bool(act.any()):
-                    break
+            else:
+                for q in range(m):
+                    plan.iteration(s, q % 2)
+            its += m
+            flags = act.cpu().numpy()
+            live = [i for i in plan.insts if flags[i.slot]]
+            if not live:
+                break
+            if len(live) <= self.shrink * len(plan.insts):
+                plan = self._subplan(live)
         info = self.scalars()
         self.last_info = info
         return info
