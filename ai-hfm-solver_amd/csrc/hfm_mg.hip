@@ -623,8 +623,8 @@ __global__ __launch_bounds__(kBlock) void k_fcg_update(const hfm_mg_task* __rest
     if (valid) {
       double xv[8], rv[8];
       float rl[8];
-      ld8d_nt(t.x64 + n0, xv);
-      ld8d_nt(t.r64 + n0, rv);
+      ld8d(t.x64 + n0, xv);
+      ld8d(t.r64 + n0, rv);
 #pragma unroll
       for (int m = 0; m < 8; ++m) {
         xv[m] += alpha * p[m];
@@ -632,9 +632,9 @@ __global__ __launch_bounds__(kBlock) void k_fcg_update(const hfm_mg_task* __rest
         rl[m] = float(rv[m]);
         acc += rv[m] * rv[m];
       }
-      st8d_nt(t.x64 + n0, xv);
-      st8d_nt(t.r64 + n0, rv);
-      st8_nt(rlow + n0, rl);
+      st8d(t.x64 + n0, xv);
+      st8d(t.r64 + n0, rv);
+      st8(rlow + n0, rl);
     }
   } else if (base < N) {
     auto pat = [&](int m, int, int, int) -> double { return double(src.one(m, 0, 0, 0)); };
