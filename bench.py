@@ -148,6 +148,28 @@ def serving(args, dev, torch):
     return out
 
 
+def profile_iteration(mg, cells, torch):
+    """Time every launch of one FCG iteration separately (all instances active)."""
+    from hfm import _lib as L
+    plan, pool = mg.plan, mg.pool
+    s = L.stream_handle(mg.device)
+    names = {0: "zero", 1: "plain", 2: "correct"}
+    rows = []
+    for op in plan.ops_down + [o for ups in plan.ops_up for o in ups]:
+        kind, mode, lev = op[0], op[1], op[5]
+        label = f"L{lev} " + ("restrict" if kind == "restrict" else "sweep " + names[mode])
+        rows.append((label, kernel_time_ms(lambda op=op: plan._launch(op, s), 5, torch)))
+    rows.append(("coarse solve", kernel_time_ms(
+        lambda: L.call("hfm_mg_coarse_solve", plan.coarse_tasks.data_ptr(), len(plan.insts),
+                       pool.sc.data_ptr(), pool.store_id, s), 5, torch)))
+    rows.append(("fcg dir", kernel_time_ms(lambda: plan.launch_dir(0, s), 5, torch)))
+    rows.append(("fcg update", kernel_time_ms(lambda: plan.launch_update(0, s), 5, torch)))
+    rows.append(("finalize", kernel_time_ms(lambda: pool.finalize("check", s), 5, torch)))
+    total = sum(t for _, t in rows)
+    print(json.dumps({"fine_cells": cells, "iteration_ms": round(total, 3),
+                      "launches": {k: round(v, 4) for k, v in rows}}), flush=True)
+
+
 def profile_kernel(args, dev, torch):
     """Launch only one kernel (all instances active) and a copy of known size,
     for rocprofv3 kernel-trace / PMC runs; prints one JSON line."""
@@ -161,6 +183,8 @@ def profile_kernel(args, dev, torch):
     mg.build_faces()
     mg.setup()
     mg.restart()
+    if args.profile_kernel == "all":
+        return profile_iteration(mg, cells, torch)
     fn = {"update": lambda: mg.launch_update(0), "sweep": mg.launch_fine_sweep}[args.profile_kernel]
     per_cell = {"update": mg.update_bytes_per_cell(), "sweep": mg.sweep_bytes_per_cell()}
     ms = kernel_time_ms(fn, args.reps, torch)
@@ -190,7 +214,7 @@ def main():
     ap.add_argument("--fp32-sample", type=int, default=16)
     ap.add_argument("--serve-requests", type=int, default=300)
     ap.add_argument("--serve-rate", type=float, default=300.0)
-    ap.add_argument("--profile-kernel", choices=["update", "sweep"], default=None,
+    ap.add_argument("--profile-kernel", choices=["update", "sweep", "all"], default=None,
                     help="launch only this kernel (after setup) for profiler runs")
     ap.add_argument("--reps", type=int, default=20)
     args = ap.parse_args()
