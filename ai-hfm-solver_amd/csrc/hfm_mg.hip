@@ -239,28 +239,41 @@ struct SrcCorrect {
   }
 };
 
-// p := T(z + beta p_old) (p_old unread when beta == 0), in FP64 then rounded
+// p := T(z + beta p_old) (one fused multiply-add in FP32; p_old unread when beta == 0)
 template <typename T>
 struct SrcDir {
   const T* z;
   const T* pold;
-  double beta;
-  __device__ __forceinline__ double make(float zv, float pv) const {
-    const double v = beta == 0.0 ? double(zv) : double(zv) + beta * double(pv);
-    return double(round_to<T>(float(v)));
+  float beta;
+  __device__ __forceinline__ float make(float zv, float pv) const {
+    return round_to<T>(beta == 0.0f ? zv : fmaf(beta, pv, zv));
   }
   template <typename A>
   __device__ __forceinline__ void run(int n, int, int, int, A v[8]) const {
     float zf[8], pf[8];
     ld8(z + n, zf);
-    if (beta != 0.0) ld8(pold + n, pf);
+    if (beta != 0.0f) ld8(pold + n, pf);
 #pragma unroll
-    for (int m = 0; m < 8; ++m) v[m] = (A)make(zf[m], beta != 0.0 ? pf[m] : 0.0f);
+    for (int m = 0; m < 8; ++m) v[m] = (A)make(zf[m], beta != 0.0f ? pf[m] : 0.0f);
   }
-  __device__ __forceinline__ double one(int n, int, int, int) const {
-    return make(to_f(z[n]), beta != 0.0 ? to_f(pold[n]) : 0.0f);
+  __device__ __forceinline__ float one(int n, int, int, int) const {
+    return make(to_f(z[n]), beta != 0.0f ? to_f(pold[n]) : 0.0f);
   }
 };
+
+// Diagonal of the storage-rounded operator at cell (k, j, i): the six faces
+// rounded to T and summed in the order of RunCoef::diag / face_row.
+template <typename T>
+__device__ __forceinline__ float diag_rounded(const hfm_mg_task& t, int n, int i, int j, int k) {
+  const int W = t.W, H = t.H, D = t.D, HW = H * W;
+  const float cw = round_to<T>(ldc<float>(t.c[0], n));
+  const float ce = round_to<T>(i < W - 1 ? ldc<float>(t.c[0], n + 1) : ldc<float>(t.c[3], k * H + j));
+  const float cs = round_to<T>(ldc<float>(t.c[1], n));
+  const float cn = round_to<T>(j < H - 1 ? ldc<float>(t.c[1], n + W) : ldc<float>(t.c[4], k * W + i));
+  const float cd = round_to<T>(ldc<float>(t.c[2], n));
+  const float cu = round_to<T>(k < D - 1 ? ldc<float>(t.c[2], n + HW) : ldc<float>(t.c[5], j * W + i));
+  return cw + ce + cs + cn + cd + cu;
+}
 
 // xc = x on the run, off = sum of face * neighbour (w, e, s, n, d, u order).
 template <typename A, typename C, typename SRC>
@@ -515,11 +528,11 @@ __global__ __launch_bounds__(64) void k_coarse_solve(const hfm_mg_task* __restri
 
 // ------------------------------------------------------------------ FCG passes (fine level)
 // Coefficients c are the FP32 fine faces; products accumulated in FP64.
-template <typename T>
+template <typename T, bool SWEEP>
 __global__ __launch_bounds__(kBlock) void k_fcg_init(const hfm_mg_task* __restrict__ tasks,
                                                      const uint16_t* __restrict__ blk_task,
                                                      const hfm_mg_scalars* __restrict__ sc,
-                                                     double* dots) {
+                                                     float omega, double* dots) {
   const int g = blockIdx.x;
   const hfm_mg_task& t = tasks[blk_task[g]];
   const hfm_mg_scalars& S = sc[t.inst];
@@ -529,20 +542,29 @@ __global__ __launch_bounds__(kBlock) void k_fcg_init(const hfm_mg_task* __restri
   const int base = chunk * kCellChunk;
   double acc = 0.0;
   T* rlow = static_cast<T*>(t.rlow);
+  T* x0 = static_cast<T*>(t.xout);
   for (int s = 0; s < kRun; ++s) {
     const int n = base + s * kBlock + threadIdx.x;
     if (n >= N) break;
     const double bv = double(t.b32[n]);
     t.x64[n] = 0.0;
     t.r64[n] = bv;
-    rlow[n] = from_f<T>(float(bv));
+    const T rt = from_f<T>(float(bv));
+    rlow[n] = rt;
     acc += bv * bv;
+    if (SWEEP) {
+      int i, j, k;
+      decompose(t, n, i, j, k);
+      x0[n] = from_f<T>(omega * to_f(rt) / diag_rounded<T>(t, n, i, j, k));
+    }
   }
   const double s = block_sum(acc);
   if (threadIdx.x == 0) dots[t.slot0 + chunk] = s;
 }
 
-// p = T(z + beta p_old) (p_old not read when beta == 0); dots = p . (A p)
+// p = T(z + beta p_old) (p_old not read when beta == 0); dots = p . (A p) with
+// A p in FP32 (it only sets the step length; the residual update recomputes
+// A p in FP64).
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_fcg_dir(const hfm_mg_task* __restrict__ tasks,
                                                     const uint16_t* __restrict__ blk_task,
@@ -555,7 +577,7 @@ __global__ __launch_bounds__(kBlock) void k_fcg_dir(const hfm_mg_task* __restric
   const int chunk = xcd_chunk(g - t.blk0, t.nblk);
   const int N = t.D * t.H * t.W;
   const int base = chunk * kCellChunk;
-  const SrcDir<T> src{static_cast<const T*>(t.b), static_cast<const T*>(t.xin), S.beta};
+  const SrcDir<T> src{static_cast<const T*>(t.b), static_cast<const T*>(t.xin), float(S.beta)};
   T* pnew = static_cast<T*>(t.xout);
   double acc = 0.0;
   if (vec_task(t)) {
@@ -564,42 +586,41 @@ __global__ __launch_bounds__(kBlock) void k_fcg_dir(const hfm_mg_task* __restric
     const Run r = run_at(t, n0, valid);
     RunCoef<float> q;
     q.load(t, r, valid);
-    double d[8], p[8], off[8];
+    float d[8], p[8], off[8];
     q.diag(d);
-    run_offdiag<double>(t, r, valid, q, src, p, off);
+    run_offdiag<float>(t, r, valid, q, src, p, off);
     if (valid) {
-      float pf[8];
 #pragma unroll
-      for (int m = 0; m < 8; ++m) {
-        pf[m] = float(p[m]);
-        acc += p[m] * (d[m] * p[m] - off[m]);
-      }
-      st8_nt(pnew + n0, pf);
+      for (int m = 0; m < 8; ++m) acc += double(p[m] * (d[m] * p[m] - off[m]));
+      st8_nt(pnew + n0, p);
     }
   } else if (base < N) {
-    auto pat = [&](int m, int, int, int) -> double { return src.one(m, 0, 0, 0); };
+    auto pat = [&](int m, int, int, int) -> float { return src.one(m, 0, 0, 0); };
     for (int s = 0; s < kRun; ++s) {
       const int n = base + s * kBlock + threadIdx.x;
       if (n >= N) break;
       int i, j, k;
       decompose(t, n, i, j, k);
-      double diag;
-      const double off = face_row<double, float>(t, n, i, j, k, pat, diag);
-      const double p = pat(n, k, j, i);
-      pnew[n] = from_f<T>(float(p));
-      acc += p * (diag * p - off);
+      float diag;
+      const float off = face_row<float, float>(t, n, i, j, k, pat, diag);
+      const float p = pat(n, k, j, i);
+      pnew[n] = from_f<T>(p);
+      acc += double(p * (diag * p - off));
     }
   }
   const double s = block_sum(acc);
   if (threadIdx.x == 0) dots[t.slot0 + chunk] = s;
 }
 
-// x64 += alpha p; r64 -= alpha A p; rlow = T(r64); dots = r . r
-template <typename T>
+// x64 += alpha p; r64 -= alpha A p; rlow = T(r64); dots = r . r.
+// SWEEP: also the next preconditioner's first (zero initial guess) fine-level
+// sweep, xout = omega rlow / d with d the diagonal of the storage-rounded faces
+// (identical to a separate HFM_SWEEP_ZERO launch).
+template <typename T, bool SWEEP>
 __global__ __launch_bounds__(kBlock) void k_fcg_update(const hfm_mg_task* __restrict__ tasks,
                                                        const uint16_t* __restrict__ blk_task,
                                                        const hfm_mg_scalars* __restrict__ sc,
-                                                       double* dots) {
+                                                       float omega, double* dots) {
   const int g = blockIdx.x;
   const hfm_mg_task& t = tasks[blk_task[g]];
   const hfm_mg_scalars& S = sc[t.inst];
@@ -610,6 +631,7 @@ __global__ __launch_bounds__(kBlock) void k_fcg_update(const hfm_mg_task* __rest
   const double alpha = S.alpha;
   const SrcPlain<T> src{static_cast<const T*>(t.xin)};
   T* rlow = static_cast<T*>(t.rlow);
+  T* x0 = static_cast<T*>(t.xout);
   double acc = 0.0;
   if (vec_task(t)) {
     const int n0 = base + threadIdx.x * kRun;
@@ -635,6 +657,16 @@ __global__ __launch_bounds__(kBlock) void k_fcg_update(const hfm_mg_task* __rest
       st8d(t.x64 + n0, xv);
       st8d(t.r64 + n0, rv);
       st8(rlow + n0, rl);
+      if (SWEEP) {
+        float o[8];
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {
+          const float dT = round_to<T>(q.cw[m]) + round_to<T>(q.ce[m]) + round_to<T>(q.cs[m]) +
+                           round_to<T>(q.cn[m]) + round_to<T>(q.cd[m]) + round_to<T>(q.cu[m]);
+          o[m] = omega * round_to<T>(rl[m]) / dT;
+        }
+        st8_nt(x0 + n0, o);
+      }
     }
   } else if (base < N) {
     auto pat = [&](int m, int, int, int) -> double { return double(src.one(m, 0, 0, 0)); };
@@ -650,8 +682,10 @@ __global__ __launch_bounds__(kBlock) void k_fcg_update(const hfm_mg_task* __rest
       t.x64[n] += alpha * pn;
       const double rr = t.r64[n] - alpha * q;
       t.r64[n] = rr;
-      rlow[n] = from_f<T>(float(rr));
+      const T rt = from_f<T>(float(rr));
+      rlow[n] = rt;
       acc += rr * rr;
+      if (SWEEP) x0[n] = from_f<T>(omega * to_f(rt) / diag_rounded<T>(t, n, i, j, k));
     }
   }
   const double s = block_sum(acc);
@@ -1003,26 +1037,55 @@ extern "C" int hfm_mg_coarse_solve(const hfm_mg_task* tasks, int64_t ntasks,
   });
 }
 
-#define HFM_FCG_ENTRY(NAME, KERNEL)                                                          \
-  extern "C" int NAME(const hfm_mg_task* tasks, const uint16_t* blk_task, int64_t nblocks,     \
-                      const hfm_mg_scalars* sc, int store, double* dots, void* stream) {       \
-    if (int e = check_table(tasks, blk_task, nblocks, sc, #NAME)) return e;                    \
-    if (!dots) {                                                                               \
-      set_error(#NAME ": null dot slots");                                                     \
-      return HFM_EINVAL;                                                                       \
-    }                                                                                          \
-    if (nblocks == 0) return HFM_OK;                                                           \
-    return launch_store(store, #NAME, [&](auto tag) {                                          \
-      using T = decltype(tag);                                                                 \
-      hipLaunchKernelGGL(KERNEL<T>, dim3(unsigned(nblocks)), dim3(kBlock), 0,                  \
-                         as_stream(stream), tasks, blk_task, sc, dots);                        \
-    });                                                                                        \
+namespace {
+int check_fcg(const hfm_mg_task* tasks, const uint16_t* blk_task, int64_t nblocks,
+              const hfm_mg_scalars* sc, double* dots, const char* what) {
+  if (int e = check_table(tasks, blk_task, nblocks, sc, what)) return e;
+  if (!dots) {
+    set_error("%s: null dot slots", what);
+    return HFM_EINVAL;
+  }
+  return HFM_OK;
+}
+}  // namespace
+
+extern "C" int hfm_fcg_dir(const hfm_mg_task* tasks, const uint16_t* blk_task, int64_t nblocks,
+                           const hfm_mg_scalars* sc, int store, double* dots, void* stream) {
+  if (int e = check_fcg(tasks, blk_task, nblocks, sc, dots, "hfm_fcg_dir")) return e;
+  if (nblocks == 0) return HFM_OK;
+  return launch_store(store, "hfm_fcg_dir", [&](auto tag) {
+    using T = decltype(tag);
+    hipLaunchKernelGGL(k_fcg_dir<T>, dim3(unsigned(nblocks)), dim3(kBlock), 0, as_stream(stream),
+                       tasks, blk_task, sc, dots);
+  });
+}
+
+#define HFM_FCG_SWEEP_ENTRY(NAME, KERNEL, SWEEP)                                                \
+  extern "C" int NAME(const hfm_mg_task* tasks, const uint16_t* blk_task, int64_t nblocks,      \
+                      const hfm_mg_scalars* sc, int store, HFM_OMEGA_ARG double* dots,          \
+                      void* stream) {                                                            \
+    if (int e = check_fcg(tasks, blk_task, nblocks, sc, dots, #NAME)) return e;                 \
+    if (nblocks == 0) return HFM_OK;                                                            \
+    return launch_store(store, #NAME, [&](auto tag) {                                           \
+      using T = decltype(tag);                                                                  \
+      hipLaunchKernelGGL((KERNEL<T, SWEEP>), dim3(unsigned(nblocks)), dim3(kBlock), 0,          \
+                         as_stream(stream), tasks, blk_task, sc, HFM_OMEGA_VAL, dots);          \
+    });                                                                                         \
   }
 
-HFM_FCG_ENTRY(hfm_fcg_init, k_fcg_init)
-HFM_FCG_ENTRY(hfm_fcg_dir, k_fcg_dir)
-HFM_FCG_ENTRY(hfm_fcg_update, k_fcg_update)
-#undef HFM_FCG_ENTRY
+#define HFM_OMEGA_ARG
+#define HFM_OMEGA_VAL 1.0f
+HFM_FCG_SWEEP_ENTRY(hfm_fcg_init, k_fcg_init, false)
+HFM_FCG_SWEEP_ENTRY(hfm_fcg_update, k_fcg_update, false)
+#undef HFM_OMEGA_ARG
+#undef HFM_OMEGA_VAL
+#define HFM_OMEGA_ARG float omega,
+#define HFM_OMEGA_VAL omega
+HFM_FCG_SWEEP_ENTRY(hfm_fcg_init_sweep, k_fcg_init, true)
+HFM_FCG_SWEEP_ENTRY(hfm_fcg_update_sweep, k_fcg_update, true)
+#undef HFM_OMEGA_ARG
+#undef HFM_OMEGA_VAL
+#undef HFM_FCG_SWEEP_ENTRY
 
 extern "C" int hfm_fcg_finalize(hfm_mg_scalars* sc, int64_t ninst, const double* dots, int kind,
                                 void* stream) {

@@ -375,6 +375,7 @@ def instance_templates(inst: Instance, p: Params) -> dict:
     base["rlow"] = inst.B[0].data_ptr()
     base["b32"] = inst.b32.data_ptr()
     base["b"] = xy(post_k, 0)
+    base["xout"] = xy(0, 0)     # first fine-level pre-sweep, fused into init / update
     tpl[("init",)] = base
     for par in range(2):
         d = base.copy()
@@ -482,6 +483,8 @@ class Plan:
         for lev in range(nlev - 1):
             cells = lambda i, lev=lev: math.prod(i.levels[lev])  # noqa: E731
             for s in range(p.nu1):
+                if lev == 0 and s == 0:
+                    continue    # computed by the FCG update (and init) kernels
                 tab = op(("down", lev, s), CHUNK, cells)
                 if tab:
                     self.ops_down.append(("sweep", L.SWEEP["zero" if s == 0 else "plain"],
@@ -520,8 +523,9 @@ class Plan:
     def init(self, s):
         """x = 0, r = b for the plan's staged instances, then finalize INIT."""
         pool, fl = self.pool, self.flay
-        L.call("hfm_fcg_init", self.init_tasks.data_ptr(), fl.blk_task.data_ptr(), fl.nblocks,
-               pool.sc.data_ptr(), pool.store_id, pool.dots.data_ptr(), s)
+        L.call("hfm_fcg_init_sweep", self.init_tasks.data_ptr(), fl.blk_task.data_ptr(),
+               fl.nblocks, pool.sc.data_ptr(), pool.store_id, pool.params.omega,
+               pool.dots.data_ptr(), s)
         pool.finalize("init", s)
 
     def precondition(self, s):
@@ -541,8 +545,9 @@ class Plan:
 
     def launch_update(self, parity, s):
         pool, fl = self.pool, self.flay
-        L.call("hfm_fcg_update", self.upd_tasks[parity].data_ptr(), fl.blk_task.data_ptr(),
-               fl.nblocks, pool.sc.data_ptr(), pool.store_id, pool.dots.data_ptr(), s)
+        L.call("hfm_fcg_update_sweep", self.upd_tasks[parity].data_ptr(), fl.blk_task.data_ptr(),
+               fl.nblocks, pool.sc.data_ptr(), pool.store_id, pool.params.omega,
+               pool.dots.data_ptr(), s)
 
     def iteration(self, s, parity):
         """One FCG iteration for every active instance of the plan."""
@@ -698,9 +703,9 @@ class MGBatch:
         self.plan._launch(op, L.stream_handle(self.device))
 
     def update_bytes_per_cell(self) -> int:
-        """Algorithmic bytes of k_fcg_update per fine cell: p (T), 3 FP32 faces,
-        x64 and r64 read + written, rlow (T) written."""
-        return 2 * ELEM[self.storage] + 12 + 32
+        """Algorithmic bytes of the fused k_fcg_update per fine cell: p (T), 3 FP32
+        faces, x64 and r64 read + written, rlow (T) and the first sweep (T) written."""
+        return 3 * ELEM[self.storage] + 12 + 32
 
     def sweep_bytes_per_cell(self) -> int:
         """Algorithmic bytes of a PLAIN sweep per cell: x, b, 3 faces, out (all T)."""

@@ -190,13 +190,22 @@ int hfm_mg_coarse_solve(const hfm_mg_task* tasks, int64_t ntasks, const hfm_mg_s
 /* FCG kernels on the fine level (coefficients c in FP32).
  * init:   x64 = 0, r64 = b32, rlow = T(b32), dots = b.b  (staged slots only)
  * dir:    xout = p = T(b + beta xin) (b = z, xin = p_old), dots = p.(A p)
- * update: x64 += alpha p, r64 -= alpha A p, rlow = T(r64), dots = r.r  (xin = p) */
+ * update: x64 += alpha p, r64 -= alpha A p, rlow = T(r64), dots = r.r  (xin = p)
+ * *_sweep variants also write the next V-cycle's first fine-level sweep from
+ * a zero guess, xout = omega rlow / d (d = diagonal of the faces rounded to
+ * storage), so that sweep is not launched separately. */
 int hfm_fcg_init(const hfm_mg_task* tasks, const uint16_t* blk_task, int64_t nblocks,
                  const hfm_mg_scalars* sc, int store, double* dots, void* stream);
 int hfm_fcg_dir(const hfm_mg_task* tasks, const uint16_t* blk_task, int64_t nblocks,
                 const hfm_mg_scalars* sc, int store, double* dots, void* stream);
 int hfm_fcg_update(const hfm_mg_task* tasks, const uint16_t* blk_task, int64_t nblocks,
                    const hfm_mg_scalars* sc, int store, double* dots, void* stream);
+int hfm_fcg_init_sweep(const hfm_mg_task* tasks, const uint16_t* blk_task, int64_t nblocks,
+                       const hfm_mg_scalars* sc, int store, float omega, double* dots,
+                       void* stream);
+int hfm_fcg_update_sweep(const hfm_mg_task* tasks, const uint16_t* blk_task, int64_t nblocks,
+                         const hfm_mg_scalars* sc, int store, float omega, double* dots,
+                         void* stream);
 
 /* Per-instance fixed-order reduction of dots[slot0 .. slot0+nslot) and the
  * scalar step:  kind 0 INIT (bnorm2), 1 BETA (rz, beta), 2 ALPHA (pap,

@@ -676,7 +676,7 @@ def fcg_solve(fine: FaceOp, b, tol=1e-8, storage="f32", maxit=200, nu1=2, nu2=2,
     Mirrors the device algorithm step for step:
       r = b, rl = T(r);  per iteration: z = V(rl) from zero;
       rz = rl . z;  beta = rz / rz_prev (0 on the first iteration);
-      p = T(z + beta p);  q = A p (FP32 coefficients, FP64 arithmetic);
+      p = T(fma32(beta, p, z));  q = A p (FP32 coefficients, FP64 arithmetic);
       a = rz / (p . q);  x += a p;  r -= a q;  rl = T(r);
       stop when ||r|| / ||b|| <= tol.
     """
@@ -698,7 +698,9 @@ def fcg_solve(fine: FaceOp, b, tol=1e-8, storage="f32", maxit=200, nu1=2, nu2=2,
         rzn = float(np.sum(rl * z))
         beta = 0.0 if it == 0 else rzn / rz
         rz = rzn
-        p = round_storage(z if beta == 0.0 else z + beta * p, s)
+        # one FP32 fused multiply-add, then the storage rounding
+        p = round_storage(z if beta == 0.0 else
+                          (np.float64(np.float32(beta)) * p + z).astype(np.float32), s)
         q = face_apply(A32, p)
         pap = float(np.sum(p * q))
         if not (pap > 0.0) or not np.isfinite(pap):
