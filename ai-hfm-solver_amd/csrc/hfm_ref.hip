@@ -107,9 +107,12 @@ __global__ __launch_bounds__(kBX* kBY) void k_conv27(const float* __restrict__ x
   if (!cell_of_thread(g, k, j, i)) return;
   const int64_t n = g.idx(k, j, i);
   const float ax = conv_at(K, mask, g, k, j, i, [&](int kk, int jj, int ii) { return x[g.idx(kk, jj, ii)]; });
+  const int f = form & HFM_FORM_MASK;
   float out = ax;
-  if (form == HFM_FORM_RESID) out = ax - b[n];
-  else if (form == HFM_FORM_JACOBI) out = jacobi_form(x[n], ax, b[n], d);
+  if (f == HFM_FORM_RESID) out = ax - b[n];
+  else if (f == HFM_FORM_JACOBI) out = jacobi_form(x[n], ax, b[n], d);
+  else if (f == HFM_FORM_SUBTRACT) out = b[n] - ax;
+  if ((form & HFM_FORM_ZERO_OUTLET) && i == g.W - 1) out = 0.0f;
   y[n] = out;
 }
 
@@ -173,13 +176,14 @@ __global__ __launch_bounds__(kBX* kBY) void k_correct_jacobi(const float* __rest
                                                              const float* __restrict__ b,
                                                              float* __restrict__ y, Weights K,
                                                              uint32_t mask, float d, Var6 V,
-                                                             Grid3 g) {
+                                                             Grid3 g, int flags) {
   int k, j, i;
   if (!cell_of_thread(g, k, j, i)) return;
   const int64_t n = g.idx(k, j, i);
+  const bool zo = flags & HFM_CORRECT_ZERO_OUTLET;
   auto vat = [&](int kk, int jj, int ii) {
     const int64_t m = g.idx(kk, jj, ii);
-    return x[m] - w[m];
+    return (zo && ii == g.W - 1) ? 0.0f : x[m] - w[m];
   };
   const float v = vat(k, j, i);
   if (VAR6) {
@@ -187,6 +191,64 @@ __global__ __launch_bounds__(kBX* kBY) void k_correct_jacobi(const float* __rest
   } else {
     y[n] = jacobi_form(v, conv_at(K, mask, g, k, j, i, vat), b[n], d);
   }
+}
+
+// Explicit advection-diffusion of NF fields phi by the velocity (u, v, w):
+//   out_f = f * (((dif(phi) - u*ax(phi)) - v*ay(phi)) - w*az(phi)) + base_f
+// (UM/AI_HFM_v3.ipynb cell 25: a_u = dif(u) - u*xadv(u) - v*yadv(u) - w*zadv(u);
+// b_u = 0.5*a_u + u; values_u = values_u + c_u; the scalar alpha the same way).
+struct Kern4 {
+  Weights k[4];       // dif, ax, ay, az
+  uint32_t mask[4];
+};
+
+template <int NF>
+__global__ __launch_bounds__(kBX* kBY) void k_advect27_v(
+    const float* __restrict__ u, const float* __restrict__ v, const float* __restrict__ w,
+    const float* p0, const float* p1, const float* p2, const float* b0, const float* b1,
+    const float* b2, float* o0, float* o1, float* o2, Kern4 K, float f, Grid3 g) {
+  int k, j, i;
+  if (!cell_of_thread(g, k, j, i)) return;
+  const int64_t n = g.idx(k, j, i);
+  const float uc = u[n], vc = v[n], wc = w[n];
+  const float* phi[3] = {p0, p1, p2};
+  const float* base[3] = {b0, b1, b2};
+  float* out[3] = {o0, o1, o2};
+#pragma unroll
+  for (int q = 0; q < NF; ++q) {
+    const float* x = phi[q];
+    auto xat = [&](int kk, int jj, int ii) { return x[g.idx(kk, jj, ii)]; };
+    const float dif = conv_at(K.k[0], K.mask[0], g, k, j, i, xat);
+    const float ax = conv_at(K.k[1], K.mask[1], g, k, j, i, xat);
+    const float ay = conv_at(K.k[2], K.mask[2], g, k, j, i, xat);
+    const float az = conv_at(K.k[3], K.mask[3], g, k, j, i, xat);
+    const float a = ((dif - uc * ax) - vc * ay) - wc * az;
+    out[q][n] = f * a + base[q][n];
+  }
+}
+
+// y = scale * sum_q K_q (*) x_q (q < n <= 3); SOURCE_BC: the x = 0 plane takes
+// the value computed at x = 1 (boundary_condition_source, UM/AI_HFM_v3.ipynb cell 19).
+struct Kern3 {
+  Weights k[3];
+  uint32_t mask[3];
+};
+
+__global__ __launch_bounds__(kBX* kBY) void k_conv27_sum_v(const float* x0, const float* x1,
+                                                           const float* x2, float* __restrict__ y,
+                                                           Kern3 K, int nf, float scale, Grid3 g,
+                                                           int flags) {
+  int k, j, i;
+  if (!cell_of_thread(g, k, j, i)) return;
+  const int ic = ((flags & HFM_SUM_SOURCE_BC) && i == 0 && g.W > 1) ? 1 : i;
+  const float* xs[3] = {x0, x1, x2};
+  float acc = 0.0f;
+  for (int q = 0; q < nf; ++q) {
+    const float* xq = xs[q];
+    acc += conv_at(K.k[q], K.mask[q], g, k, j, ic,
+                   [&](int kk, int jj, int ii) { return xq[g.idx(kk, jj, ii)]; });
+  }
+  y[g.idx(k, j, i)] = scale * acc;
 }
 
 // Boundary fills: the value of the last reference assignment covering a cell.
@@ -325,8 +387,10 @@ extern "C" int hfm_conv27_f32(const float* x, float* y, const float* b, const fl
                               float d, int64_t D, int64_t H, int64_t W, int form, void* stream) {
   Weights w;
   uint32_t mask;
+  const int fm = form & HFM_FORM_MASK;
   if (!x || !y || !dims_ok(D, H, W) || !make_weights(K, w, mask) || x == y ||
-      form < HFM_FORM_APPLY || form > HFM_FORM_JACOBI || (form != HFM_FORM_APPLY && !b)) {
+      fm < HFM_FORM_APPLY || fm > HFM_FORM_SUBTRACT || (form & ~(HFM_FORM_MASK | HFM_FORM_ZERO_OUTLET)) ||
+      (fm != HFM_FORM_APPLY && !b)) {
     set_error("hfm_conv27_f32: bad arguments");
     return HFM_EINVAL;
   }
@@ -409,7 +473,7 @@ extern "C" int hfm_prolong_jacobi_f32(const float* wc, const float* r, float* y,
 
 extern "C" int hfm_correct_jacobi_f32(const float* x, const float* wv, const float* b, float* y,
                                       const float K[27], float d, const float* const m[6],
-                                      const float* tot, int64_t D, int64_t H, int64_t W,
+                                      const float* tot, int64_t D, int64_t H, int64_t W, int flags,
                                       void* stream) {
   Weights w{};
   uint32_t mask = 0;
@@ -423,11 +487,82 @@ extern "C" int hfm_correct_jacobi_f32(const float* x, const float* wv, const flo
   const Grid3 g{int(D), int(H), int(W)};
   if (var6)
     hipLaunchKernelGGL(k_correct_jacobi<true>, grid_for(g), dim3(kBX, kBY), 0,
-                       as_stream(stream), x, wv, b, y, w, mask, d, V, g);
+                       as_stream(stream), x, wv, b, y, w, mask, d, V, g, flags);
   else
     hipLaunchKernelGGL(k_correct_jacobi<false>, grid_for(g), dim3(kBX, kBY), 0,
-                       as_stream(stream), x, wv, b, y, w, mask, d, V, g);
+                       as_stream(stream), x, wv, b, y, w, mask, d, V, g, flags);
   return launch_status("hfm_correct_jacobi_f32");
+}
+
+extern "C" int hfm_advect27_f32(const float* const vel[3], const float* const* phi,
+                                const float* const* base, float* const* out, int nf,
+                                const float K[4][27], float f, int64_t D, int64_t H, int64_t W,
+                                void* stream) {
+  Kern4 kk;
+  if (!vel || !vel[0] || !vel[1] || !vel[2] || !phi || !base || !out || !K ||
+      (nf != 1 && nf != 3) || !dims_ok(D, H, W)) {
+    set_error("hfm_advect27_f32: bad arguments");
+    return HFM_EINVAL;
+  }
+  for (int q = 0; q < 4; ++q) make_weights(K[q], kk.k[q], kk.mask[q]);
+  struct P3 {
+    const float* a[3];
+  } pp{}, bb{};
+  struct O3 {
+    float* a[3];
+  } oo{};
+  for (int q = 0; q < nf; ++q) {
+    if (!phi[q] || !base[q] || !out[q]) {
+      set_error("hfm_advect27_f32: null field %d", q);
+      return HFM_EINVAL;
+    }
+    for (int r = 0; r < 3; ++r)
+      if (out[q] == vel[r] || out[q] == phi[r < nf ? r : 0]) {
+        set_error("hfm_advect27_f32: output %d aliases an input", q);
+        return HFM_EINVAL;
+      }
+    pp.a[q] = phi[q];
+    bb.a[q] = base[q];
+    oo.a[q] = out[q];
+  }
+  const Grid3 g{int(D), int(H), int(W)};
+  hipStream_t s = as_stream(stream);
+  if (nf == 3)
+    hipLaunchKernelGGL(k_advect27_v<3>, grid_for(g), dim3(kBX, kBY), 0, s, vel[0], vel[1],
+                       vel[2], pp.a[0], pp.a[1], pp.a[2], bb.a[0], bb.a[1], bb.a[2], oo.a[0],
+                       oo.a[1], oo.a[2], kk, f, g);
+  else
+    hipLaunchKernelGGL(k_advect27_v<1>, grid_for(g), dim3(kBX, kBY), 0, s, vel[0], vel[1],
+                       vel[2], pp.a[0], pp.a[1], pp.a[2], bb.a[0], bb.a[1], bb.a[2], oo.a[0],
+                       oo.a[1], oo.a[2], kk, f, g);
+  return launch_status("hfm_advect27_f32");
+}
+
+extern "C" int hfm_conv27_sum_f32(const float* const* x, float* y, int nf, const float K[3][27],
+                                  float scale, int64_t D, int64_t H, int64_t W, int flags,
+                                  void* stream) {
+  Kern3 kk;
+  if (!x || !y || !K || nf < 1 || nf > 3 || !dims_ok(D, H, W) || (flags & ~HFM_SUM_SOURCE_BC)) {
+    set_error("hfm_conv27_sum_f32: bad arguments");
+    return HFM_EINVAL;
+  }
+  const float* xs[3] = {nullptr, nullptr, nullptr};
+  for (int q = 0; q < nf; ++q) {
+    if (!x[q] || x[q] == y) {
+      set_error("hfm_conv27_sum_f32: bad field %d", q);
+      return HFM_EINVAL;
+    }
+    xs[q] = x[q];
+    make_weights(K[q], kk.k[q], kk.mask[q]);
+  }
+  for (int q = nf; q < 3; ++q) {
+    kk.mask[q] = 0;
+    xs[q] = xs[0];
+  }
+  const Grid3 g{int(D), int(H), int(W)};
+  hipLaunchKernelGGL(k_conv27_sum_v, grid_for(g), dim3(kBX, kBY), 0, as_stream(stream), xs[0],
+                     xs[1], xs[2], y, kk, nf, scale, g, flags);
+  return launch_status("hfm_conv27_sum_f32");
 }
 
 extern "C" int hfm_bc_f32(const float* in, float* out, int64_t D, int64_t H, int64_t W, int kind,

@@ -19,7 +19,10 @@ from ._build import LIB_PATH
 HFM_OK = 0
 
 STORE = {"f32": 0, "bf16": 1}
-FORM = {"apply": 0, "resid": 1, "jacobi": 2}
+FORM = {"apply": 0, "resid": 1, "jacobi": 2, "subtract": 3}
+FORM_ZERO_OUTLET = 16
+CORRECT_ZERO_OUTLET = 1
+SUM_SOURCE_BC = 1
 BC = {"top_copy": 0, "pressure": 1, "source": 2, "indicator": 3, "zero_outlet": 4}
 SWEEP = {"zero": 0, "plain": 1, "correct": 2}
 FIN = {"init": 0, "beta": 1, "alpha": 2, "check": 3}
@@ -60,7 +63,9 @@ SIGNATURES = {
     "hfm_prolong_jacobi_f32": (_int, [_vp, _vp, _vp, _vp, _f32, _vp, _vp, _i64, _i64, _i64,
                                       _int, _int, _int, _vp]),
     "hfm_correct_jacobi_f32": (_int, [_vp, _vp, _vp, _vp, _vp, _f32, _vp, _vp, _i64, _i64, _i64,
-                                      _vp]),
+                                      _int, _vp]),
+    "hfm_advect27_f32": (_int, [_vp, _vp, _vp, _vp, _int, _vp, _f32, _i64, _i64, _i64, _vp]),
+    "hfm_conv27_sum_f32": (_int, [_vp, _vp, _int, _vp, _f32, _i64, _i64, _i64, _int, _vp]),
     "hfm_bc_f32": (_int, [_vp, _vp, _i64, _i64, _i64, _int, _vp]),
     "hfm_bc_velocity_f32": (_int, [_vp, _vp, _vp, _vp, _vp, _vp, _i64, _i64, _i64, _f32, _vp]),
     "hfm_bluff_body_f32": (_int, [_vp, _vp, _int, _vp, _f32, _i64, _vp]),

@@ -98,15 +98,23 @@ class FixedConv:
                L.FORM["apply"], L.stream_handle(x.device))
         return y
 
-    def residual(self, x, b):
+    def residual(self, x, b, zero_outlet: bool = False):
         """``A(x) - b`` in one pass (the reference's ``r = CNN3D_A(values) - b``)."""
+        return self._form(x, b, "resid", zero_outlet)
+
+    def subtract_from(self, b, x):
+        """``b - A(x)`` in one pass (the reference's ``values_u - CNN3D_pu(values_p)``)."""
+        return self._form(x, b, "subtract", False)
+
+    def _form(self, x, b, form, zero_outlet):
         x, b = _prep(x), _prep(b, "b")
         D, H, W = self._check(x)
         if b.shape != x.shape:
             raise ValueError("b must have the shape of x")
         y = torch.empty_like(x)
+        f = L.FORM[form] | (L.FORM_ZERO_OUTLET if zero_outlet else 0)
         L.call("hfm_conv27_f32", x.data_ptr(), y.data_ptr(), b.data_ptr(), _kptr(self.K), 1.0,
-               D, H, W, L.FORM["resid"], L.stream_handle(x.device))
+               D, H, W, f, L.stream_handle(x.device))
         return y
 
     def jacobi(self, w, r, d: Optional[float] = None):
@@ -214,11 +222,12 @@ class SawtoothMG:
     """
 
     def __init__(self, ops: List, divisors: Optional[List[float]] = None, coarsest_sweeps: int = 1,
-                 bc: Optional[Callable] = None):
+                 bc: Optional[Callable] = None, zero_outlet: bool = False):
         self.ops = ops
         self.divisors = divisors
         self.coarsest_sweeps = int(coarsest_sweeps)
         self.bc = bc
+        self.zero_outlet = bool(zero_outlet)
 
     def _d(self, lev):
         op = self.ops[lev]
@@ -247,6 +256,7 @@ class SawtoothMG:
 
     def _correct_jacobi(self, x, w, b):
         op = self.ops[0]
+        flags = L.CORRECT_ZERO_OUTLET if self.zero_outlet else 0
         (D, H, W), _ = _spatial(x)
         if w.shape != x.shape or b.shape != x.shape:
             raise ValueError("x, w and b must share a shape")
@@ -254,15 +264,18 @@ class SawtoothMG:
         s = L.stream_handle(x.device)
         if isinstance(op, AOperatorVar6):
             L.call("hfm_correct_jacobi_f32", x.data_ptr(), w.data_ptr(), b.data_ptr(), y.data_ptr(),
-                   None, 0.0, op._arr, op.tot.data_ptr(), D, H, W, s)
+                   None, 0.0, op._arr, op.tot.data_ptr(), D, H, W, flags, s)
         else:
             L.call("hfm_correct_jacobi_f32", x.data_ptr(), w.data_ptr(), b.data_ptr(), y.data_ptr(),
-                   _kptr(op.K), self._d(0), None, None, D, H, W, s)
+                   _kptr(op.K), self._d(0), None, None, D, H, W, flags, s)
         return y
 
     def cycle(self, values, b, record: Optional[dict] = None):
+        """One cycle; with ``zero_outlet`` the fine residual and the corrected
+        field are zeroed on the x = W-1 plane (UM/AI_HFM_v3.ipynb:2574,2611-2612)."""
         values, b = _prep(values, "values"), _prep(b, "b")
-        r = self.ops[0].residual(values, b)
+        r = self.ops[0].residual(values, b, zero_outlet=self.zero_outlet) if isinstance(
+            self.ops[0], FixedConv) else self.ops[0].residual(values, b)
         rs = [r]
         for _ in range(1, len(self.ops)):
             rs.append(restrict_mean(rs[-1]))
@@ -367,3 +380,45 @@ def bluff_body(values_u, values_v, values_w, alpha, sigma, dt: float):
            L.ptr_array([o.data_ptr() for o in outs]), len(fields), sigma.data_ptr(),
            float(dt), sigma.numel(), L.stream_handle(sigma.device))
     return tuple(outs)
+
+
+# ---------------------------------------------------------------------------
+# fused time-step kernels (UM/AI_HFM_v3.ipynb cell 25)
+
+
+def advect(vel, phis, bases, f: float, kernels):
+    """``out_q = f * (dif(phi_q) - u ax(phi_q) - v ay(phi_q) - w az(phi_q)) + base_q``
+    for 1 or 3 fields in one pass; ``kernels`` = (dif, ax, ay, az) weight arrays."""
+    vel = [_prep(a, "vel") for a in vel]
+    phis = [_prep(a, "phi") for a in phis]
+    bases = [_prep(a, "base") for a in bases]
+    if len(vel) != 3 or len(phis) not in (1, 3) or len(bases) != len(phis):
+        raise ValueError("advect takes 3 velocity fields and 1 or 3 fields with bases")
+    shape = vel[0].shape
+    for t in (*vel, *phis, *bases):
+        if t.shape != shape:
+            raise ValueError("all fields must share one shape")
+    (D, H, W), _ = _spatial(vel[0])
+    outs = [torch.empty_like(a) for a in phis]
+    K = np.ascontiguousarray(np.stack([kernel27(k) for k in kernels]), dtype=np.float32)
+    L.call("hfm_advect27_f32", L.ptr_array([a.data_ptr() for a in vel]),
+           L.ptr_array([a.data_ptr() for a in phis]), L.ptr_array([a.data_ptr() for a in bases]),
+           L.ptr_array([a.data_ptr() for a in outs]), len(phis), K.ctypes.data_as(L.C.c_void_p),
+           float(f), D, H, W, L.stream_handle(vel[0].device))
+    return outs
+
+
+def conv_sum(xs, kernels, scale: float = 1.0, source_bc: bool = False):
+    """``scale * sum_q K_q (*) x_q`` (1-3 fields) in one pass; ``source_bc`` copies the
+    x = 1 plane into x = 0 (boundary_condition_source)."""
+    xs = [_prep(a) for a in xs]
+    (D, H, W), _ = _spatial(xs[0])
+    for t in xs:
+        if t.shape != xs[0].shape:
+            raise ValueError("all fields must share one shape")
+    y = torch.empty_like(xs[0])
+    K = np.ascontiguousarray(np.stack([kernel27(k) for k in kernels]), dtype=np.float32)
+    L.call("hfm_conv27_sum_f32", L.ptr_array([a.data_ptr() for a in xs]), y.data_ptr(), len(xs),
+           K.ctypes.data_as(L.C.c_void_p), float(scale), D, H, W,
+           L.SUM_SOURCE_BC if source_bc else 0, L.stream_handle(xs[0].device))
+    return y

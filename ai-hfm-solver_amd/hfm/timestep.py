@@ -1,0 +1,126 @@
+"""Time-step drivers on the libhfm.so kernels.
+
+UrbanFlow follows the reference's urban-flow loop (UM/AI_HFM_v3.ipynb cell 25;
+script twin ARTs/naiver_stokes_3D.py:571-727) step for step:
+
+  velocity / pressure BCs
+  -> momentum predictor  b = phi + 0.5 * (dif(phi) - u xadv(phi) - v yadv(phi) - w zadv(phi))
+  -> velocity BCs on b
+  -> momentum corrector  phi += dif(b_phi) - b_u xadv(b_phi) - b_v yadv(b_phi) - b_w zadv(b_phi)
+  -> scalar transport + indicator BCs
+  -> bluff body  f / (1 + dt sigma)
+  -> u -= pu(p), v -= pv(p), w -= pw(p); velocity BCs
+  -> b = -(Su(u) + Sv(v) + Sw(w)); source BC
+  -> multi_itr sawtooth cycles of the 27-point Poisson layer (outlet zeroed)
+  -> pressure BCs; gradient correction; velocity BCs; bluff body
+
+Each arrow is one fused kernel launch (the three momentum components share
+one 27-point advection pass; the divergence is one pass over u, v, w).
+"""
+from __future__ import annotations
+
+from typing import Optional
+
+import numpy as np
+import torch
+
+from . import layers as Ly
+
+_P27 = np.array([[[2, 3, 2], [3, 6, 3], [2, 3, 2]],
+                 [[3, 6, 3], [6, -88, 6], [3, 6, 3]],
+                 [[2, 3, 2], [3, 6, 3], [2, 3, 2]]], dtype=np.float64) / 26.0
+_PLANE = np.array([[0.014, 0.056, 0.014], [0.056, 0.22, 0.056], [0.014, 0.056, 0.014]])
+
+
+def _dx_kernel():
+    K = np.zeros((3, 3, 3)); K[:, :, 0] = -_PLANE; K[:, :, 2] = _PLANE
+    return K
+
+
+def _dy_kernel():
+    K = np.zeros((3, 3, 3)); K[:, 0, :] = _PLANE; K[:, 2, :] = -_PLANE
+    return K
+
+
+def _dz_kernel():
+    K = np.zeros((3, 3, 3)); K[0] = _PLANE; K[2] = -_PLANE
+    return K
+
+
+def urban_kernels(dt: float, dx: float, Re: float):
+    """The reference's fixed weights (UM/AI_HFM_v3.ipynb cells 7-10): w1 diffusion,
+    w2/w3/w4 advection and gradient (x dt/dx x 0.5), w5 Poisson, w6/w7/w8 divergence
+    (x 0.5/(dx dt)).  The y/z kernels differentiate toward decreasing index."""
+    s_adv = dt / dx * 0.5
+    s_div = 0.5 / (dx * dt)
+    return {
+        "dif": _P27 * dt * Re / dx ** 2,
+        "adv": (_dx_kernel() * s_adv, _dy_kernel() * s_adv, _dz_kernel() * s_adv),
+        "div": (_dx_kernel() * s_div, _dy_kernel() * s_div, _dz_kernel() * s_div),
+        "poisson": -_P27 / dx ** 2,
+    }
+
+
+class UrbanFlow:
+    """The reference urban-flow time step on (1, D, H, W, 1) or (D, H, W) GPU fields."""
+
+    def __init__(self, shape, sigma: Optional[torch.Tensor] = None, dt: float = 0.4,
+                 dx: float = 1.0, Re: float = 1 / 20, ub: float = 1.0, multi_itr: int = 5,
+                 lscalar: bool = True, device=None):
+        self.shape = tuple(int(v) for v in shape)
+        if len(self.shape) != 3 or min(self.shape) < 2:
+            raise ValueError("UrbanFlow needs a (D, H, W) box with every edge >= 2")
+        self.device = torch.device(device or "cuda")
+        self.dt, self.ub, self.multi_itr, self.lscalar = float(dt), float(ub), int(multi_itr), lscalar
+        k = urban_kernels(dt, dx, Re)
+        self.k_mom = (k["dif"], *k["adv"])
+        self.k_div = k["div"]
+        self.grad = [Ly.FixedConv(K) for K in k["adv"]]
+        shapes = [self.shape]
+        while min(shapes[-1]) > 1:
+            shapes.append(tuple(n // 2 for n in shapes[-1]))
+        for s_f, s_c in zip(shapes, shapes[1:]):
+            if any(a != 2 * b for a, b in zip(s_f, s_c)):
+                raise ValueError(f"grid {self.shape} does not halve evenly down to its coarsest level")
+        self.mg = Ly.SawtoothMG([Ly.FixedConv(k["poisson"], s) for s in shapes], zero_outlet=True)
+        if sigma is None:
+            sigma = torch.zeros(self.shape, device=self.device)
+        self.sigma = sigma.reshape(self.shape).to(device=self.device, dtype=torch.float32).contiguous()
+        self.last_w = None
+
+    def _bluff(self, u, v, w, alpha):
+        return Ly.bluff_body(u, v, w, alpha, self.sigma, dt=self.dt)
+
+    def _gradient(self, u, v, w, p):
+        return (self.grad[0].subtract_from(u, p), self.grad[1].subtract_from(v, p),
+                self.grad[2].subtract_from(w, p))
+
+    def step(self, u, v, w, p, alpha):
+        """One time step; returns (u, v, w, p, alpha) as new tensors."""
+        shape = tuple(u.shape)
+        u, v, w, p, alpha = (t.reshape(self.shape).float().contiguous() for t in (u, v, w, p, alpha))
+        u, v, w = Ly.boundary_condition_velocity(u, v, w, ub=self.ub)
+        p = Ly.boundary_condition_pressure(p)
+        bu, bv, bw = Ly.advect((u, v, w), (u, v, w), (u, v, w), 0.5, self.k_mom)
+        bu, bv, bw = Ly.boundary_condition_velocity(bu, bv, bw, ub=self.ub)
+        u, v, w = Ly.advect((bu, bv, bw), (bu, bv, bw), (u, v, w), 1.0, self.k_mom)
+        if self.lscalar:
+            (alpha,) = Ly.advect((u, v, w), (alpha,), (alpha,), 1.0, self.k_mom)
+            alpha = Ly.boundary_condition_indicator(alpha)
+        u, v, w, alpha = self._bluff(u, v, w, alpha)
+        u, v, w = self._gradient(u, v, w, p)
+        u, v, w = Ly.boundary_condition_velocity(u, v, w, ub=self.ub)
+        b = Ly.conv_sum((u, v, w), self.k_div, scale=-1.0, source_bc=True)
+        rec = {}
+        for _ in range(self.multi_itr):
+            p = self.mg.cycle(p, b, record=rec)
+        self.last_w = rec.get(0)
+        p = Ly.boundary_condition_pressure(p)
+        u, v, w = self._gradient(u, v, w, p)
+        u, v, w = Ly.boundary_condition_velocity(u, v, w, ub=self.ub)
+        u, v, w, alpha = self._bluff(u, v, w, alpha)
+        return tuple(t.reshape(shape) for t in (u, v, w, p, alpha))
+
+    def diverged(self, limit: float = 0.2) -> bool:
+        """The reference's guard: max |w_fine| of the last cycle above ``limit``."""
+        return self.last_w is not None and float(self.last_w.abs().max()) > limit

@@ -58,9 +58,16 @@ extern "C" {
 #define HFM_STORE_BF16 1
 
 /* reference-mode stencil forms */
-#define HFM_FORM_APPLY 0  /* y = A x                         */
-#define HFM_FORM_RESID 1  /* y = A x - b                     */
-#define HFM_FORM_JACOBI 2 /* y = x - A(x)/d + b/d            */
+#define HFM_FORM_APPLY 0    /* y = A x                         */
+#define HFM_FORM_RESID 1    /* y = A x - b                     */
+#define HFM_FORM_JACOBI 2   /* y = x - A(x)/d + b/d            */
+#define HFM_FORM_SUBTRACT 3 /* y = b - A x                     */
+#define HFM_FORM_MASK 15
+#define HFM_FORM_ZERO_OUTLET 16 /* flag: y = 0 on the x = W-1 plane */
+
+/* flags of hfm_correct_jacobi_f32 / hfm_conv27_sum_f32 */
+#define HFM_CORRECT_ZERO_OUTLET 1 /* (x - w) = 0 on the x = W-1 plane before the sweep */
+#define HFM_SUM_SOURCE_BC 1       /* x = 0 plane takes the value of the x = 1 plane   */
 
 /* boundary fills for hfm_bc_f32 */
 #define HFM_BC_TOP_COPY 0      /* plane z=0 <- plane z=1                       */
@@ -78,8 +85,10 @@ int hfm_struct_sizes(int64_t* task_size, int64_t* scalars_size);
 /* ---------------- reference mode ---------------- */
 
 /* y = form(K (*) x): K is the 27-entry row-major 3x3x3 kernel w[0,:,:,:,0];
- * zero ghosts.  b: rhs for RESID/JACOBI (may be NULL for APPLY); d: the
- * scalar divisor for JACOBI.  y must not alias x. */
+ * zero ghosts.  b: rhs for RESID/JACOBI/SUBTRACT (may be NULL for APPLY); d:
+ * the scalar divisor for JACOBI; form may carry HFM_FORM_ZERO_OUTLET
+ * (the MG residual's r[..., nx-1] = 0, UM/AI_HFM_v3.ipynb:2574).  y must not
+ * alias x. */
 int hfm_conv27_f32(const float* x, float* y, const float* b, const float K[27], float d,
                    int64_t D, int64_t H, int64_t W, int form, void* stream);
 
@@ -103,10 +112,28 @@ int hfm_prolong_jacobi_f32(const float* wc, const float* r, float* y, const floa
                            const float* const m[6], const float* tot, int64_t D, int64_t H,
                            int64_t W, int fz, int fy, int fx, void* stream);
 
-/* y = J(x - w) against b (same operator choice as above). */
+/* y = J(x - w) against b (same operator choice as above); flags
+ * HFM_CORRECT_ZERO_OUTLET zeroes (x - w) on the outlet plane first
+ * (UM/AI_HFM_v3.ipynb:2611-2613). */
 int hfm_correct_jacobi_f32(const float* x, const float* w, const float* b, float* y,
                            const float K[27], float d, const float* const m[6], const float* tot,
-                           int64_t D, int64_t H, int64_t W, void* stream);
+                           int64_t D, int64_t H, int64_t W, int flags, void* stream);
+
+/* Explicit advection-diffusion step of nf (1 or 3) fields by the velocity
+ * vel = (u, v, w):  out_q = f * (((dif(phi_q) - u ax(phi_q)) - v ay(phi_q)) -
+ * w az(phi_q)) + base_q, with K = {dif, ax, ay, az} 27-tap kernels
+ * (UM/AI_HFM_v3.ipynb cell 25: the momentum predictor / corrector with
+ * CNN3D_central_2nd_dif/xadv/yadv/zadv, and the scalar transport).  Host
+ * arrays of device pointers; outputs alias no input field. */
+int hfm_advect27_f32(const float* const vel[3], const float* const* phi, const float* const* base,
+                     float* const* out, int nf, const float K[4][27], float f, int64_t D,
+                     int64_t H, int64_t W, void* stream);
+
+/* y = scale * sum_q K_q (*) x_q over nf <= 3 fields: the divergence source
+ * b = -(CNN3D_Su(u) + CNN3D_Sv(v) + CNN3D_Sw(w)) (UM/AI_HFM_v3.ipynb:2568);
+ * flags HFM_SUM_SOURCE_BC applies boundary_condition_source. */
+int hfm_conv27_sum_f32(const float* const* x, float* y, int nf, const float K[3][27], float scale,
+                       int64_t D, int64_t H, int64_t W, int flags, void* stream);
 
 /* out = in with one boundary fill applied (kind = HFM_BC_*).  Copies read
  * the input field, as the reference's tf.Variable copies do; out != in. */

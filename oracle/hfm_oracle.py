@@ -202,7 +202,7 @@ def restrict_var6(mats: dict) -> dict:
 
 def sawtooth_cycle(values, b, ops: List[Callable], diags: list, restrict=restrict_mean,
                    prolong=prolong_nearest, coarsest_sweeps: int = 1, bc=None,
-                   record: Optional[dict] = None):
+                   record: Optional[dict] = None, zero_outlet: bool = False):
     """One reference "sawtooth" multigrid cycle.
 
     ``…7point.ipynb:402-454`` / ``BGS#c42`` / ``ARTn/Parallel_multigrid_2D.ipynb#c20``:
@@ -212,9 +212,13 @@ def sawtooth_cycle(values, b, ops: List[Callable], diags: list, restrict=restric
     do one sweep against that level's restricted residual, then
     ``x -= w``, one fine sweep on ``A x = b`` and the optional boundary fill.
     ``ops[l]`` / ``diags[l]`` are the level-l operator and divisor, fine first.
+    ``zero_outlet``: r[..., nx-1] = 0 after the residual and values[..., nx-1] = 0
+    after the correction (``UM/AI_HFM_v3.ipynb:2574,2611-2612``).
     """
     nlev = len(ops)
     r = ops[0](values) - b
+    if zero_outlet:
+        r = zero_outlet_plane(r)
     rs = [r]
     for _ in range(1, nlev):
         rs.append(restrict(rs[-1]))
@@ -232,10 +236,18 @@ def sawtooth_cycle(values, b, ops: List[Callable], diags: list, restrict=restric
         if record is not None:
             record[lev] = w
     values = values - w
+    if zero_outlet:
+        values = zero_outlet_plane(values)
     values = values - ops[0](values) / diags[0] + b / diags[0]
     if bc is not None:
         values = bc(values)
     return values
+
+
+def zero_outlet_plane(r):
+    out = np.array(r, copy=True)
+    out[..., -1] = 0
+    return out
 
 
 def boundary_condition_top(values: np.ndarray) -> np.ndarray:
@@ -389,6 +401,65 @@ def implicit_ad1d_kernel(dt=10.0, dx=1.0, Dx=0.1, ub=1.0):
     return np.array([-ub * dt / dx - Dx * dt / dx ** 2,
                      1 + 2 * Dx * dt / dx ** 2 + ub * dt / dx,
                      -Dx * dt / dx ** 2])
+
+
+def urban_levels(shape):
+    """Reference-mode pressure hierarchy: halve every axis until the shortest is 1."""
+    shapes = [tuple(shape)]
+    while min(shapes[-1]) > 1:
+        shapes.append(tuple(n // 2 for n in shapes[-1]))
+    return shapes
+
+
+def urban_step(u, v, w, p, alpha, sigma, dt=0.4, dx=1.0, Re=1 / 20, ub=1.0, multi_itr=5,
+               lscalar=True):
+    """One time step of ``UM/AI_HFM_v3.ipynb`` cell 25 (fields (D,H,W), float32 in/out).
+
+    BCs; explicit two-stage momentum with the 27-point diffusion and smoothed
+    central advection layers; scalar transport; bluff body; pressure-gradient
+    correction; divergence source; ``multi_itr`` sawtooth cycles of the
+    27-point Poisson layer with outlet zeroing; BCs, correction and bluff body
+    again.  Returns (u, v, w, p, alpha, w_fine) with w_fine the last cycle's
+    fine-level correction (the reference's divergence guard reads max|w_512|).
+    """
+    f32 = np.float32
+    dif = diffusion27_kernel(dt, Re, dx)
+    ax, ay, az = advection_kernels(dt, dx)
+    sx, sy, sz = divergence_kernels(dt, dx)
+    w5 = poisson27_kernel(dx)
+    C = lambda x, K: conv_same(x.astype(f32), K)  # noqa: E731
+
+    def adv(vel, phi):
+        uu, vv, ww = vel
+        return ((C(phi, dif) - uu * C(phi, ax)) - vv * C(phi, ay)) - ww * C(phi, az)
+
+    u, v, w = boundary_condition_velocity(u, v, w, ub=f32(ub))
+    p = boundary_condition_pressure(p)
+    a = [adv((u, v, w), phi) for phi in (u, v, w)]
+    bu, bv, bw = (f32(0.5) * ai + phi for ai, phi in zip(a, (u, v, w)))
+    bu, bv, bw = boundary_condition_velocity(bu, bv, bw, ub=f32(ub))
+    c = [adv((bu, bv, bw), phi) for phi in (bu, bv, bw)]
+    u, v, w = (ci + phi for ci, phi in zip(c, (u, v, w)))
+    if lscalar:
+        alpha = adv((u, v, w), alpha) + alpha
+        alpha = boundary_condition_indicator(alpha)
+    damp = f32(1.0) + f32(dt) * sigma
+    u, v, w, alpha = (x / damp for x in (u, v, w, alpha))
+    u, v, w = u - C(p, ax), v - C(p, ay), w - C(p, az)
+    u, v, w = boundary_condition_velocity(u, v, w, ub=f32(ub))
+    b = f32(-1.0) * ((C(u, sx) + C(v, sy)) + C(w, sz))
+    b = boundary_condition_source(b)
+    shapes = urban_levels(p.shape)
+    ops = [lambda x: C(x, w5)] * len(shapes)
+    d = f32(w5[1, 1, 1])
+    rec = {}
+    for _ in range(multi_itr):
+        p = sawtooth_cycle(p, b, ops, [d] * len(shapes), zero_outlet=True, record=rec)
+    p = boundary_condition_pressure(p)
+    u, v, w = u - C(p, ax), v - C(p, ay), w - C(p, az)
+    u, v, w = boundary_condition_velocity(u, v, w, ub=f32(ub))
+    u, v, w, alpha = (x / damp for x in (u, v, w, alpha))
+    return u, v, w, p, alpha, rec.get(0)
 
 
 # ---------------------------------------------------------------------------

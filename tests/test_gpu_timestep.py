@@ -1,0 +1,64 @@
+"""The urban-flow time step and its fused kernels against the oracle (GPU)."""
+import numpy as np
+import pytest
+
+from oracle import hfm_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def T(a, dev):
+    import torch
+    return torch.as_tensor(np.ascontiguousarray(a, np.float32)).to(dev)
+
+
+def H(t):
+    return t.detach().cpu().numpy().astype(np.float64)
+
+
+def test_fused_advection_matches_layers(dev):
+    from hfm import layers
+    from hfm.timestep import urban_kernels
+    k = urban_kernels(0.4, 1.0, 1 / 20)
+    K = (k["dif"], *k["adv"])
+    rng = np.random.default_rng(0)
+    sh = (6, 10, 24)
+    u, v, w, a = (rng.standard_normal(sh).astype(np.float32) for _ in range(4))
+    outs = layers.advect([T(x, dev) for x in (u, v, w)], [T(x, dev) for x in (u, v, w)],
+                         [T(x, dev) for x in (u, v, w)], 0.5, K)
+    C = lambda x, Kk: O.conv_same(x.astype(np.float64), Kk)  # noqa: E731
+    for g, phi in zip(outs, (u, v, w)):
+        ref = 0.5 * (((C(phi, K[0]) - u * C(phi, K[1])) - v * C(phi, K[2])) - w * C(phi, K[3])) + phi
+        assert rel(H(g), ref) < 1e-6
+    (ga,) = layers.advect([T(x, dev) for x in (u, v, w)], [T(a, dev)], [T(a, dev)], 1.0, K)
+    ref = (((C(a, K[0]) - u * C(a, K[1])) - v * C(a, K[2])) - w * C(a, K[3])) + a
+    assert rel(H(ga), ref) < 1e-6
+    b = layers.conv_sum([T(x, dev) for x in (u, v, w)], k["div"], scale=-1.0, source_bc=True)
+    refb = O.boundary_condition_source(-(C(u, k["div"][0]) + C(v, k["div"][1]) + C(w, k["div"][2])))
+    assert rel(H(b), refb) < 1e-6
+    g0 = layers.FixedConv(k["adv"][0]).subtract_from(T(u, dev), T(a, dev))
+    assert rel(H(g0), u - C(a, k["adv"][0])) < 1e-6
+    r = layers.FixedConv(k["poisson"]).residual(T(a, dev), T(u, dev), zero_outlet=True)
+    assert rel(H(r), O.zero_outlet_plane(C(a, k["poisson"]) - u)) < 1e-6
+
+
+def test_urban_steps_match_oracle(dev):
+    from hfm.timestep import UrbanFlow
+    sh = (8, 16, 16)
+    u = np.ones(sh, np.float32); v = np.zeros(sh, np.float32); w = np.zeros(sh, np.float32)
+    p = np.zeros(sh, np.float32); a = np.zeros(sh, np.float32); a[:4] = 1
+    sig = np.zeros(sh, np.float32); sig[1:5, 6:9, 6:9] = 1e5
+    flow = UrbanFlow(sh, sigma=T(sig, dev), multi_itr=3)
+    g = [T(x, dev) for x in (u, v, w, p, a)]
+    o = (u, v, w, p, a)
+    for _ in range(3):
+        g = flow.step(*g)
+        o = O.urban_step(*o, sig, multi_itr=3)[:5]
+    for gi, oi in zip(g, o):
+        assert rel(H(gi), oi) < 1e-5
+    assert not flow.diverged()
