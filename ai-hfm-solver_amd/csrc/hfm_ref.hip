@@ -112,6 +112,7 @@ __global__ __launch_bounds__(kBX* kBY) void k_conv27(const float* __restrict__ x
   if (f == HFM_FORM_RESID) out = ax - b[n];
   else if (f == HFM_FORM_JACOBI) out = jacobi_form(x[n], ax, b[n], d);
   else if (f == HFM_FORM_SUBTRACT) out = b[n] - ax;
+  else if (f == HFM_FORM_AXPY) out = b[n] + d * ax;
   if ((form & HFM_FORM_ZERO_OUTLET) && i == g.W - 1) out = 0.0f;
   y[n] = out;
 }
@@ -289,14 +290,15 @@ __global__ __launch_bounds__(kBX* kBY) void k_bc_velocity(const float* __restric
                                                           float* __restrict__ uo,
                                                           float* __restrict__ vo,
                                                           float* __restrict__ wo, Grid3 g,
-                                                          float ub) {
+                                                          float ub,
+                                                          const float* __restrict__ inflow) {
   int k, j, i;
   if (!cell_of_thread(g, k, j, i)) return;
   const int D = g.D, H = g.H, W = g.W;
   const int64_t n = g.idx(k, j, i);
   float a = u[n], b = v[n], c = w[n];
   if (i == 0 || i == W - 1) {
-    a = ub;
+    a = inflow ? inflow[k] : ub;
     b = 0.0f;
     c = 0.0f;
   }
@@ -389,7 +391,7 @@ extern "C" int hfm_conv27_f32(const float* x, float* y, const float* b, const fl
   uint32_t mask;
   const int fm = form & HFM_FORM_MASK;
   if (!x || !y || !dims_ok(D, H, W) || !make_weights(K, w, mask) || x == y ||
-      fm < HFM_FORM_APPLY || fm > HFM_FORM_SUBTRACT || (form & ~(HFM_FORM_MASK | HFM_FORM_ZERO_OUTLET)) ||
+      fm < HFM_FORM_APPLY || fm > HFM_FORM_AXPY || (form & ~(HFM_FORM_MASK | HFM_FORM_ZERO_OUTLET)) ||
       (fm != HFM_FORM_APPLY && !b)) {
     set_error("hfm_conv27_f32: bad arguments");
     return HFM_EINVAL;
@@ -582,7 +584,7 @@ extern "C" int hfm_bc_f32(const float* in, float* out, int64_t D, int64_t H, int
 
 extern "C" int hfm_bc_velocity_f32(const float* u, const float* v, const float* w, float* uo,
                                    float* vo, float* wo, int64_t D, int64_t H, int64_t W,
-                                   float ub, void* stream) {
+                                   float ub, const float* inflow, void* stream) {
   if (!u || !v || !w || !uo || !vo || !wo || uo == u || vo == v || wo == w ||
       !dims_ok(D, H, W) || D < 2 || H < 2 || W < 2) {
     set_error("hfm_bc_velocity_f32: bad arguments");
@@ -590,7 +592,7 @@ extern "C" int hfm_bc_velocity_f32(const float* u, const float* v, const float* 
   }
   const Grid3 g{int(D), int(H), int(W)};
   hipLaunchKernelGGL(k_bc_velocity, grid_for(g), dim3(kBX, kBY), 0, as_stream(stream), u, v, w,
-                     uo, vo, wo, g, ub);
+                     uo, vo, wo, g, ub, inflow);
   return launch_status("hfm_bc_velocity_f32");
 }
 

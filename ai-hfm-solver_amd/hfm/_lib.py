@@ -19,7 +19,7 @@ from ._build import LIB_PATH
 HFM_OK = 0
 
 STORE = {"f32": 0, "bf16": 1}
-FORM = {"apply": 0, "resid": 1, "jacobi": 2, "subtract": 3}
+FORM = {"apply": 0, "resid": 1, "jacobi": 2, "subtract": 3, "axpy": 4}
 FORM_ZERO_OUTLET = 16
 CORRECT_ZERO_OUTLET = 1
 SUM_SOURCE_BC = 1
@@ -67,7 +67,8 @@ SIGNATURES = {
     "hfm_advect27_f32": (_int, [_vp, _vp, _vp, _vp, _int, _vp, _f32, _i64, _i64, _i64, _vp]),
     "hfm_conv27_sum_f32": (_int, [_vp, _vp, _int, _vp, _f32, _i64, _i64, _i64, _int, _vp]),
     "hfm_bc_f32": (_int, [_vp, _vp, _i64, _i64, _i64, _int, _vp]),
-    "hfm_bc_velocity_f32": (_int, [_vp, _vp, _vp, _vp, _vp, _vp, _i64, _i64, _i64, _f32, _vp]),
+    "hfm_bc_velocity_f32": (_int, [_vp, _vp, _vp, _vp, _vp, _vp, _i64, _i64, _i64, _f32, _vp,
+                                   _vp]),
     "hfm_bluff_body_f32": (_int, [_vp, _vp, _int, _vp, _f32, _i64, _vp]),
     "hfm_mg_sweep": (_int, [_vp, _vp, _i64, _vp, _int, _int, _f32, _f32, _vp, _vp]),
     "hfm_mg_resid_restrict": (_int, [_vp, _vp, _i64, _vp, _int, _vp]),

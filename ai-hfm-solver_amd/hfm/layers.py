@@ -106,14 +106,18 @@ class FixedConv:
         """``b - A(x)`` in one pass (the reference's ``values_u - CNN3D_pu(values_p)``)."""
         return self._form(x, b, "subtract", False)
 
-    def _form(self, x, b, form, zero_outlet):
+    def axpy(self, b, x, scale: float):
+        """``b + scale * A(x)`` in one pass (explicit stage updates)."""
+        return self._form(x, b, "axpy", False, scale)
+
+    def _form(self, x, b, form, zero_outlet, d=1.0):
         x, b = _prep(x), _prep(b, "b")
         D, H, W = self._check(x)
         if b.shape != x.shape:
             raise ValueError("b must have the shape of x")
         y = torch.empty_like(x)
         f = L.FORM[form] | (L.FORM_ZERO_OUTLET if zero_outlet else 0)
-        L.call("hfm_conv27_f32", x.data_ptr(), y.data_ptr(), b.data_ptr(), _kptr(self.K), 1.0,
+        L.call("hfm_conv27_f32", x.data_ptr(), y.data_ptr(), b.data_ptr(), _kptr(self.K), float(d),
                D, H, W, f, L.stream_handle(x.device))
         return y
 
@@ -356,15 +360,23 @@ def zero_outlet(r):
     return _bc(r, "zero_outlet")
 
 
-def boundary_condition_velocity(values_u, values_v, values_w, nx=None, ub: float = 1.0):
+def boundary_condition_velocity(values_u, values_v, values_w, nx=None, ub: float = 1.0,
+                                inflow=None):
+    """Velocity boundary planes; ``inflow`` (D values, optional) replaces the constant
+    ``ub`` on the x faces by a height profile."""
     u, v, w = (_prep(a, n) for a, n in ((values_u, "u"), (values_v, "v"), (values_w, "w")))
     _check_nx(u, nx)
     (D, H, W), _ = _spatial(u)
     if v.shape != u.shape or w.shape != u.shape:
         raise ValueError("u, v, w must share a shape")
+    if inflow is not None:
+        inflow = _prep(inflow, "inflow")
+        if inflow.numel() != D:
+            raise ValueError("inflow profile needs one value per height")
     uo, vo, wo = torch.empty_like(u), torch.empty_like(v), torch.empty_like(w)
     L.call("hfm_bc_velocity_f32", u.data_ptr(), v.data_ptr(), w.data_ptr(), uo.data_ptr(),
-           vo.data_ptr(), wo.data_ptr(), D, H, W, float(ub), L.stream_handle(u.device))
+           vo.data_ptr(), wo.data_ptr(), D, H, W, float(ub), L.ptr(inflow),
+           L.stream_handle(u.device))
     return uo, vo, wo
 
 

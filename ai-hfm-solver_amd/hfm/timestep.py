@@ -124,3 +124,82 @@ class UrbanFlow:
     def diverged(self, limit: float = 0.2) -> bool:
         """The reference's guard: max |w_fine| of the last cycle above ``limit``."""
         return self.last_w is not None and float(self.last_w.abs().max()) > limit
+
+
+class AdvectionDiffusion2D:
+    """The reference's 2D explicit advection-diffusion step
+    (ARTn/Advection_diffusion.ipynb:291-296, ARTs/advection_diffusion_2D.py:131-141):
+    one fixed 3x3 kernel K (diffusion and fixed-velocity advection folded in)
+    and the two-stage update ``x <- x + K(x + K(x)/2)``, as two fused passes."""
+
+    UPWIND = np.array([[0.0, 0.2, 0.0], [0.3, -1.0, 0.2], [0.0, 0.3, 0.0]])
+    CENTRAL = np.array([[0.0, 0.15, 0.0], [0.25, -0.8, 0.15], [0.0, 0.25, 0.0]])
+
+    def __init__(self, kernel=None):
+        self.layer = Ly.FixedConv(self.UPWIND if kernel is None else kernel)
+
+    def step(self, x):
+        half = self.layer.axpy(x, x, 0.5)
+        return self.layer.axpy(x, half, 1.0)
+
+
+class UrbanProjection(UrbanFlow):
+    """The urban-flow step with a 7-point multigrid pressure projection solved to a
+    relative residual (the configuration's form of the step): log-law inflow on the
+    x faces, free-slip top and no-slip ground (the reference's velocity BCs), the
+    sigma-mask obstacle term, and instead of ``multi_itr`` sawtooth cycles of the
+    27-point layer, ``A p = b`` with A the 7-point -Laplacian (Neumann faces, a
+    Dirichlet outlet at x = W-1) solved by the FCG multigrid of ``hfm.mg`` in FP64
+    with ``storage`` (FP32 or BF16) V-cycles, from zero, to ``tol``."""
+
+    def __init__(self, shape, sigma=None, inflow=None, dt: float = 0.4, dx: float = 1.0,
+                 Re: float = 1 / 20, storage: str = "bf16", tol: float = 1e-8,
+                 lscalar: bool = True, device=None):
+        from .mg import MGBatch
+        super().__init__(shape, sigma=sigma, dt=dt, dx=dx, Re=Re, ub=1.0, multi_itr=0,
+                         lscalar=lscalar, device=device)
+        D, H, W = self.shape
+        self.inflow = None if inflow is None else torch.as_tensor(inflow, dtype=torch.float32).to(self.device)
+        inv = 1.0 / dx ** 2
+        one = torch.full(self.shape, inv, dtype=torch.float32)
+        cx, cy, cz = one.clone(), one.clone(), one.clone()
+        cx[:, :, 0] = 0.0       # Neumann at x = 0
+        cy[:, 0, :] = 0.0       # Neumann at y = 0
+        cz[0] = 0.0             # Neumann at the ground
+        faces = (cx, cy, cz, torch.full((D, H), inv),   # Dirichlet outlet at x = W-1
+                 torch.zeros((D, W)), torch.zeros((H, W)))
+        self.solver = MGBatch([self.shape], storage=storage, tol=tol, maxit=500, device=self.device)
+        self.solver.set_faces(0, faces, torch.zeros(self.shape))
+        self.solver.setup()
+        self.last_info = None
+
+    def _bcv(self, u, v, w):
+        return Ly.boundary_condition_velocity(u, v, w, ub=self.ub, inflow=self.inflow)
+
+    def pressure(self, b):
+        inst = self.solver.insts[0]
+        inst.b32.copy_(b.reshape(-1))
+        self.last_info = self.solver.solve()
+        return inst.x64.view(self.shape).float()
+
+    def step(self, u, v, w, p, alpha):
+        shape = tuple(u.shape)
+        u, v, w, p, alpha = (t.reshape(self.shape).float().contiguous() for t in (u, v, w, p, alpha))
+        u, v, w = self._bcv(u, v, w)
+        p = Ly.boundary_condition_pressure(p)
+        bu, bv, bw = Ly.advect((u, v, w), (u, v, w), (u, v, w), 0.5, self.k_mom)
+        bu, bv, bw = self._bcv(bu, bv, bw)
+        u, v, w = Ly.advect((bu, bv, bw), (bu, bv, bw), (u, v, w), 1.0, self.k_mom)
+        if self.lscalar:
+            (alpha,) = Ly.advect((u, v, w), (alpha,), (alpha,), 1.0, self.k_mom)
+            alpha = Ly.boundary_condition_indicator(alpha)
+        u, v, w, alpha = self._bluff(u, v, w, alpha)
+        u, v, w = self._gradient(u, v, w, p)
+        u, v, w = self._bcv(u, v, w)
+        b = Ly.conv_sum((u, v, w), self.k_div, scale=-1.0, source_bc=True)
+        p = self.pressure(b)
+        p = Ly.boundary_condition_pressure(p)
+        u, v, w = self._gradient(u, v, w, p)
+        u, v, w = self._bcv(u, v, w)
+        u, v, w, alpha = self._bluff(u, v, w, alpha)
+        return tuple(t.reshape(shape) for t in (u, v, w, p, alpha))

@@ -58,3 +58,30 @@ def device_fields(shape, sigma_ln: float, n_dipoles: int, seed: int, device):
     for (p, v) in dipoles(shape, n_dipoles, seed):
         b[p] += v
     return sig[0], sig[1], sig[2], b
+
+
+def urban_blocks(shape, seed: int, cover: float = 0.2, foot=(8, 24), height=(5, 30)):
+    """Obstacle mask of axis-aligned blocks standing on the ground plane (k = 0).
+
+    Blocks with footprints drawn from U{foot} x U{foot} cells and heights
+    U{height} are added at seeded positions until at least ``cover`` of the
+    ground plane is built on.  Returns a bool array of ``shape`` (D, H, W).
+    """
+    D, H, W = shape
+    rng = np.random.default_rng(seed)
+    mask = np.zeros(shape, dtype=bool)
+    ground = np.zeros((H, W), dtype=bool)
+    while ground.mean() < cover:
+        fy, fx = (int(rng.integers(foot[0], foot[1] + 1)) for _ in range(2))
+        hz = int(rng.integers(height[0], height[1] + 1))
+        y0 = int(rng.integers(1, max(2, H - fy - 1)))
+        x0 = int(rng.integers(1, max(2, W - fx - 1)))
+        mask[:min(hz, D - 1), y0:y0 + fy, x0:x0 + fx] = True
+        ground[y0:y0 + fy, x0:x0 + fx] = True
+    return mask
+
+
+def log_law_profile(D: int, ustar: float = 0.5, z0: float = 0.1, kappa: float = 0.41, dz=1.0):
+    """Inflow u(z) = (u*/kappa) ln((z + z0)/z0) at cell-centre heights z = (k + 1/2) dz."""
+    z = (np.arange(D) + 0.5) * dz
+    return (ustar / kappa * np.log((z + z0) / z0)).astype(np.float32)

@@ -21,6 +21,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 for _p in (ROOT, os.path.join(ROOT, "ai-hfm-solver_amd")):
     if _p not in sys.path:
@@ -148,6 +150,52 @@ def serving(args, dev, torch):
     return out
 
 
+def urban(args, dev, torch):
+    """Urban flow on 256 x 256 x 64 (D = 64 heights): seeded blocks covering 20 % of
+    the ground, log-law inflow, the reference's explicit 27-point momentum and
+    scalar transport, and a 7-point MG pressure projection to 1e-8 per step;
+    BF16-storage V-cycles vs FP32, with the final pressure of each compared with
+    an FP64-converged (1e-12) solve of the same system."""
+    from hfm.timestep import UrbanProjection
+    from hfm.workloads import log_law_profile, urban_blocks
+    sh = (64, 256, 256)
+    mask = urban_blocks(sh, args.seed)
+    sig = torch.from_numpy(np.where(mask, 1e5, 0.0).astype(np.float32)).to(dev)
+    prof = log_law_profile(sh[0])
+    out = {"grid": list(sh), "steps": args.urban_steps, "ground_cover": round(float(mask[0].mean()), 3)}
+    finals = {}
+    for storage in ("bf16", "f32"):
+        flow = UrbanProjection(sh, sigma=sig, inflow=prof, storage=storage, tol=1e-8, device=dev)
+        u0 = torch.from_numpy(np.broadcast_to(prof[:, None, None], sh).copy()).to(dev)
+        z = torch.zeros(sh, device=dev)
+        state = (u0, z, z.clone(), z.clone(), z.clone())
+        state = flow.step(*state)          # warm (graph capture)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        its = []
+        for _ in range(args.urban_steps):
+            state = flow.step(*state)
+            its.append(int(flow.last_info["iter"][0]))
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / args.urban_steps
+        out[storage] = {"ms_per_step": round(dt * 1e3, 3),
+                        "cell_steps_per_s": round(np.prod(sh) / dt, 1),
+                        "mg_iters_mean": round(float(np.mean(its)), 2),
+                        "rel_residual": float(flow.last_info["rel"][0])}
+        inst = flow.solver.insts[0]
+        finals[storage] = (inst.b32.clone(), inst.x64.clone())
+        if storage == "f32":
+            ref_solver = UrbanProjection(sh, sigma=sig, inflow=prof, storage="f32", tol=1e-12,
+                                         device=dev).solver
+    for storage, (b, x) in finals.items():
+        ref_solver.insts[0].b32.copy_(b)
+        ref_solver.solve()
+        xr = ref_solver.insts[0].x64
+        out[storage]["pressure_rel_err_vs_fp64"] = float((x - xr).norm() / xr.norm())
+    log(f"bench: urban {out}")
+    return out
+
+
 def profile_iteration(mg, cells, torch):
     """Time every launch of one FCG iteration separately (all instances active)."""
     from hfm import _lib as L
@@ -217,13 +265,13 @@ def main():
     ap.add_argument("--profile-kernel", choices=["update", "sweep", "all"], default=None,
                     help="launch only this kernel (after setup) for profiler runs")
     ap.add_argument("--reps", type=int, default=20)
+    ap.add_argument("--urban-steps", type=int, default=10)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the solves are independent and this build runs "
               "on a single MI355X; multi-GPU runs are not part of it", file=sys.stderr)
         sys.exit(2)
 
-    import numpy as np
     import torch
 
     from hfm.mg import MGBatch
@@ -280,6 +328,8 @@ def main():
         extra["speedup_vs_fp32_one_at_a_time"] = round(value / sps, 2)
     if args.serve_requests > 0:
         extra["serving"] = serving(args, dev, torch)
+    if args.urban_steps > 0:
+        extra["urban_flow"] = urban(args, dev, torch)
     cpu = None
     if not args.no_cpu_baseline:
         torch.set_num_threads(1)

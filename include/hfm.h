@@ -62,6 +62,7 @@ extern "C" {
 #define HFM_FORM_RESID 1    /* y = A x - b                     */
 #define HFM_FORM_JACOBI 2   /* y = x - A(x)/d + b/d            */
 #define HFM_FORM_SUBTRACT 3 /* y = b - A x                     */
+#define HFM_FORM_AXPY 4     /* y = b + d A x  (explicit stages) */
 #define HFM_FORM_MASK 15
 #define HFM_FORM_ZERO_OUTLET 16 /* flag: y = 0 on the x = W-1 plane */
 
@@ -141,9 +142,11 @@ int hfm_bc_f32(const float* in, float* out, int64_t D, int64_t H, int64_t W, int
                void* stream);
 
 /* (uo, vo, wo) = velocity boundary fill of (u, v, w) (UM/AI_HFM_v3.ipynb
- * cell 19, boundary_condition_velocity); outputs must not alias inputs. */
+ * cell 19, boundary_condition_velocity); outputs must not alias inputs.
+ * inflow (nullable, D values): u on the x faces per height instead of ub. */
 int hfm_bc_velocity_f32(const float* u, const float* v, const float* w, float* uo, float* vo,
-                        float* wo, int64_t D, int64_t H, int64_t W, float ub, void* stream);
+                        float* wo, int64_t D, int64_t H, int64_t W, float ub, const float* inflow,
+                        void* stream);
 
 /* out_i = in_i / (1 + dt*sigma) for n fields (in place allowed). */
 int hfm_bluff_body_f32(const float* const* in, float* const* out, int nfields,

@@ -62,3 +62,51 @@ def test_urban_steps_match_oracle(dev):
     for gi, oi in zip(g, o):
         assert rel(H(gi), oi) < 1e-5
     assert not flow.diverged()
+
+
+@pytest.mark.parametrize("which", ["UPWIND", "CENTRAL"])
+def test_ad2d_steps_match_oracle(dev, which):
+    from hfm.timestep import AdvectionDiffusion2D
+    K = getattr(O, "AD2D_" + which)
+    x = np.zeros((60, 70), np.float32)
+    x[20:40, 25:45] = 1.0
+    drv = AdvectionDiffusion2D(K)
+    g = T(x[None, ..., None], dev)
+    o = x.astype(np.float64)
+    for _ in range(20):
+        g = drv.step(g)
+        o = O.ad2d_step(o, K)
+    assert tuple(g.shape) == (1, 60, 70, 1)
+    assert rel(H(g)[0, ..., 0], o) < 1e-6
+    # mass: the kernels conserve the sum away from the zero-ghost boundary
+    assert abs(H(g).sum() - x.sum()) / x.sum() < 1e-3
+
+
+@pytest.mark.parametrize("storage", ["f32", "bf16"])
+def test_urban_projection_pressure_solve(dev, storage):
+    """The projection's pressure solve reaches 1e-8 against the FP64 sparse solution."""
+    import torch
+    from hfm.timestep import UrbanProjection
+    from hfm.workloads import log_law_profile, urban_blocks
+    sh = (8, 32, 32)
+    mask = urban_blocks(sh, 3, foot=(3, 6), height=(2, 5))
+    sig = np.where(mask, 1e5, 0.0).astype(np.float32)
+    flow = UrbanProjection(sh, sigma=T(sig, dev), inflow=log_law_profile(sh[0]), storage=storage)
+    z = torch.zeros(sh, device=dev)
+    u = torch.ones(sh, device=dev)
+    state = (u, z, z.clone(), z.clone(), z.clone())
+    for _ in range(2):
+        state = flow.step(*state)
+    info = flow.last_info
+    assert int(info["status"][0]) == 1 and float(info["rel"][0]) <= 1e-8
+    # FP64 check of the last pressure system on the host
+    inst = flow.solver.insts[0]
+    b = inst.b32.cpu().numpy().reshape(sh).astype(np.float64)
+    ones = np.ones(sh, np.float32)
+    op = O.harmonic_faces(ones, ones, ones)
+    op.cx[:, :, 0] = 0; op.cy[:, 0, :] = 0; op.cz[0] = 0; op.byh[:] = 0; op.bzh[:] = 0
+    x64 = O.solve_fp64_direct(op, b)
+    x = inst.x64.cpu().numpy().reshape(sh)
+    assert rel(x, x64) < 1e-6
+    # the inflow profile sits on the x faces
+    assert rel(H(state[0])[2, 5, 0], log_law_profile(sh[0])[2]) < 1e-6
