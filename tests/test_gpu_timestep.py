@@ -91,7 +91,8 @@ def test_urban_projection_pressure_solve(dev, storage):
     sh = (8, 32, 32)
     mask = urban_blocks(sh, 3, foot=(3, 6), height=(2, 5))
     sig = np.where(mask, 1e5, 0.0).astype(np.float32)
-    flow = UrbanProjection(sh, sigma=T(sig, dev), inflow=log_law_profile(sh[0]), storage=storage)
+    flow = UrbanProjection(sh, sigma=T(sig, dev), inflow=log_law_profile(sh[0]), dt=0.05,
+                           storage=storage)
     z = torch.zeros(sh, device=dev)
     u = torch.ones(sh, device=dev)
     state = (u, z, z.clone(), z.clone(), z.clone())
