@@ -330,7 +330,10 @@ __device__ __forceinline__ Run run_at(const hfm_mg_task& t, int n0, bool valid) 
 }
 
 // ------------------------------------------------------------------ sweeps
-template <typename T, int MODE, bool DOT>
+// T: storage of the level's vectors; O: storage of this sweep's output (the
+// fine level's last post-sweep may write the preconditioned residual z in a
+// wider type than the cycle's internals).
+template <typename T, int MODE, bool DOT, typename O>
 __global__ __launch_bounds__(kBlock) void k_sweep(const hfm_mg_task* __restrict__ tasks,
                                                   const uint16_t* __restrict__ blk_task,
                                                   const hfm_mg_scalars* __restrict__ sc,
@@ -344,7 +347,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep(const hfm_mg_task* __restrict_
   const T* xin = static_cast<const T*>(t.xin);
   const T* b = static_cast<const T*>(t.b);
   const T* ec = static_cast<const T*>(t.ec);
-  T* out = static_cast<T*>(t.xout);
+  O* out = static_cast<O*>(t.xout);
   double acc = 0.0;
   if (vec_task(t)) {
     const int n0 = base + threadIdx.x * kRun;
@@ -372,7 +375,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep(const hfm_mg_task* __restrict_
       st8_nt(out + n0, o);
       if (DOT) {
 #pragma unroll
-        for (int m = 0; m < 8; ++m) acc += double(bv[m]) * double(round_to<T>(o[m]));
+        for (int m = 0; m < 8; ++m) acc += double(bv[m]) * double(round_to<O>(o[m]));
       }
     }
   } else if (base < N) {
@@ -403,7 +406,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep(const hfm_mg_task* __restrict_
         const float xc = xat(n, k, j, i);
         r = xc + omega * (bn - (diag * xc - off)) / diag;
       }
-      const T rs = from_f<T>(r);
+      const O rs = from_f<O>(r);
       out[n] = rs;
       if (DOT) acc += double(bn) * double(to_f(rs));
     }
@@ -613,10 +616,11 @@ __global__ __launch_bounds__(kBlock) void k_fcg_dir(const hfm_mg_task* __restric
 }
 
 // x64 += alpha p; r64 -= alpha A p; rlow = T(r64); dots = r . r.
+// p is stored as P (the search-direction type), rlow and xout as T.
 // SWEEP: also the next preconditioner's first (zero initial guess) fine-level
 // sweep, xout = omega rlow / d with d the diagonal of the storage-rounded faces
 // (identical to a separate HFM_SWEEP_ZERO launch).
-template <typename T, bool SWEEP>
+template <typename T, bool SWEEP, typename P>
 __global__ __launch_bounds__(kBlock) void k_fcg_update(const hfm_mg_task* __restrict__ tasks,
                                                        const uint16_t* __restrict__ blk_task,
                                                        const hfm_mg_scalars* __restrict__ sc,
@@ -629,7 +633,7 @@ __global__ __launch_bounds__(kBlock) void k_fcg_update(const hfm_mg_task* __rest
   const int N = t.D * t.H * t.W;
   const int base = chunk * kCellChunk;
   const double alpha = S.alpha;
-  const SrcPlain<T> src{static_cast<const T*>(t.xin)};
+  const SrcPlain<P> src{static_cast<const P*>(t.xin)};
   T* rlow = static_cast<T*>(t.rlow);
   T* x0 = static_cast<T*>(t.xout);
   double acc = 0.0;
@@ -964,6 +968,21 @@ int launch_store(int store, const char* what, F&& f) {
   return launch_status(what);
 }
 
+// Two storage types (inner, outer): f(T{}, O{}).
+template <typename F>
+int launch_store2(int store, int outer, const char* what, F&& f) {
+  if (outer != HFM_STORE_F32 && outer != HFM_STORE_BF16) {
+    set_error("%s: unknown storage type %d", what, outer);
+    return HFM_EINVAL;
+  }
+  return launch_store(store, what, [&](auto tag) {
+    if (outer == HFM_STORE_F32)
+      f(tag, float{});
+    else
+      f(tag, bf16{});
+  });
+}
+
 int check_table(const void* tasks, const void* blk_task, int64_t nblocks, const void* sc,
                 const char* what) {
   if (!tasks || !sc || (nblocks > 0 && !blk_task)) {
@@ -980,8 +999,8 @@ int check_table(const void* tasks, const void* blk_task, int64_t nblocks, const 
 }  // namespace
 
 extern "C" int hfm_mg_sweep(const hfm_mg_task* tasks, const uint16_t* blk_task, int64_t nblocks,
-                            const hfm_mg_scalars* sc, int store, int mode, float omega,
-                            float alpha, double* dots, void* stream) {
+                            const hfm_mg_scalars* sc, int store, int out_store, int mode,
+                            float omega, float alpha, double* dots, void* stream) {
   if (int e = check_table(tasks, blk_task, nblocks, sc, "hfm_mg_sweep")) return e;
   if (mode < HFM_SWEEP_ZERO || mode > HFM_SWEEP_CORRECT) {
     set_error("hfm_mg_sweep: bad mode %d", mode);
@@ -989,16 +1008,17 @@ extern "C" int hfm_mg_sweep(const hfm_mg_task* tasks, const uint16_t* blk_task, 
   }
   if (nblocks == 0) return HFM_OK;
   hipStream_t s = as_stream(stream);
-  return launch_store(store, "hfm_mg_sweep", [&](auto tag) {
+  return launch_store2(store, out_store, "hfm_mg_sweep", [&](auto tag, auto otag) {
     using T = decltype(tag);
+    using O = decltype(otag);
     const dim3 grid{unsigned(nblocks)}, block{unsigned(kBlock)};
     const bool dot = dots != nullptr;
-#define HFM_SWEEP_CASE(M)                                                                 \
-  if (dot)                                                                                \
-    hipLaunchKernelGGL((k_sweep<T, M, true>), grid, block, 0, s, tasks, blk_task, sc, omega, \
-                       alpha, dots);                                                      \
-  else                                                                                    \
-    hipLaunchKernelGGL((k_sweep<T, M, false>), grid, block, 0, s, tasks, blk_task, sc,     \
+#define HFM_SWEEP_CASE(M)                                                                  \
+  if (dot)                                                                                 \
+    hipLaunchKernelGGL((k_sweep<T, M, true, O>), grid, block, 0, s, tasks, blk_task, sc,    \
+                       omega, alpha, dots);                                                \
+  else                                                                                     \
+    hipLaunchKernelGGL((k_sweep<T, M, false, O>), grid, block, 0, s, tasks, blk_task, sc,   \
                        omega, alpha, dots);
     if (mode == HFM_SWEEP_ZERO) {
       HFM_SWEEP_CASE(HFM_SWEEP_ZERO)
@@ -1060,7 +1080,7 @@ extern "C" int hfm_fcg_dir(const hfm_mg_task* tasks, const uint16_t* blk_task, i
   });
 }
 
-#define HFM_FCG_SWEEP_ENTRY(NAME, KERNEL, SWEEP)                                                \
+#define HFM_FCG_INIT_ENTRY(NAME, SWEEP)                                                         \
   extern "C" int NAME(const hfm_mg_task* tasks, const uint16_t* blk_task, int64_t nblocks,      \
                       const hfm_mg_scalars* sc, int store, HFM_OMEGA_ARG double* dots,          \
                       void* stream) {                                                            \
@@ -1068,24 +1088,38 @@ extern "C" int hfm_fcg_dir(const hfm_mg_task* tasks, const uint16_t* blk_task, i
     if (nblocks == 0) return HFM_OK;                                                            \
     return launch_store(store, #NAME, [&](auto tag) {                                           \
       using T = decltype(tag);                                                                  \
-      hipLaunchKernelGGL((KERNEL<T, SWEEP>), dim3(unsigned(nblocks)), dim3(kBlock), 0,          \
+      hipLaunchKernelGGL((k_fcg_init<T, SWEEP>), dim3(unsigned(nblocks)), dim3(kBlock), 0,      \
+                         as_stream(stream), tasks, blk_task, sc, HFM_OMEGA_VAL, dots);          \
+    });                                                                                         \
+  }
+#define HFM_FCG_UPDATE_ENTRY(NAME, SWEEP)                                                       \
+  extern "C" int NAME(const hfm_mg_task* tasks, const uint16_t* blk_task, int64_t nblocks,      \
+                      const hfm_mg_scalars* sc, int store, int pstore, HFM_OMEGA_ARG            \
+                      double* dots, void* stream) {                                             \
+    if (int e = check_fcg(tasks, blk_task, nblocks, sc, dots, #NAME)) return e;                 \
+    if (nblocks == 0) return HFM_OK;                                                            \
+    return launch_store2(store, pstore, #NAME, [&](auto tag, auto ptag) {                       \
+      using T = decltype(tag);                                                                  \
+      using P = decltype(ptag);                                                                 \
+      hipLaunchKernelGGL((k_fcg_update<T, SWEEP, P>), dim3(unsigned(nblocks)), dim3(kBlock), 0, \
                          as_stream(stream), tasks, blk_task, sc, HFM_OMEGA_VAL, dots);          \
     });                                                                                         \
   }
 
 #define HFM_OMEGA_ARG
 #define HFM_OMEGA_VAL 1.0f
-HFM_FCG_SWEEP_ENTRY(hfm_fcg_init, k_fcg_init, false)
-HFM_FCG_SWEEP_ENTRY(hfm_fcg_update, k_fcg_update, false)
+HFM_FCG_INIT_ENTRY(hfm_fcg_init, false)
+HFM_FCG_UPDATE_ENTRY(hfm_fcg_update, false)
 #undef HFM_OMEGA_ARG
 #undef HFM_OMEGA_VAL
 #define HFM_OMEGA_ARG float omega,
 #define HFM_OMEGA_VAL omega
-HFM_FCG_SWEEP_ENTRY(hfm_fcg_init_sweep, k_fcg_init, true)
-HFM_FCG_SWEEP_ENTRY(hfm_fcg_update_sweep, k_fcg_update, true)
+HFM_FCG_INIT_ENTRY(hfm_fcg_init_sweep, true)
+HFM_FCG_UPDATE_ENTRY(hfm_fcg_update_sweep, true)
 #undef HFM_OMEGA_ARG
 #undef HFM_OMEGA_VAL
-#undef HFM_FCG_SWEEP_ENTRY
+#undef HFM_FCG_INIT_ENTRY
+#undef HFM_FCG_UPDATE_ENTRY
 
 extern "C" int hfm_fcg_finalize(hfm_mg_scalars* sc, int64_t ninst, const double* dots, int kind,
                                 void* stream) {

@@ -27,7 +27,7 @@ from __future__ import annotations
 
 import dataclasses
 import math
-from typing import List, Sequence, Tuple
+from typing import List, Optional, Sequence, Tuple
 
 import numpy as np
 import torch
@@ -86,10 +86,15 @@ class Params:
     omega: float = 2.0 / 3.0
     alpha: float = 1.6
     max_coarse_edge: int = 4
+    outer: Optional[str] = None   # z and p storage (default: ``storage``)
 
     def __post_init__(self):
         if self.storage not in ELEM:
             raise ValueError(f"storage must be one of {sorted(ELEM)}")
+        if self.outer is None:
+            self.outer = self.storage
+        if self.outer not in ELEM:
+            raise ValueError(f"outer must be one of {sorted(ELEM)}")
         if self.nu1 < 1 or self.nu2 < 1:
             raise ValueError("nu1 and nu2 must be >= 1")
 
@@ -137,6 +142,7 @@ class Pool:
                 raise RuntimeError("the solver needs a GPU (the HIP library has no CPU path)")
             L.load()
         self.store_id = L.STORE[params.storage]
+        self.outer_id = L.STORE[params.outer]
         self.capacity = capacity
         self.sc = torch.zeros(capacity * L.SCALARS_DTYPE.itemsize, dtype=torch.uint8,
                               device=self.device)
@@ -210,7 +216,10 @@ class Instance:
         self.b32 = torch.zeros(N0, dtype=torch.float32, device=dev)
         self.x64 = torch.zeros(N0, dtype=torch.float64, device=dev)
         self.r64 = torch.zeros(N0, dtype=torch.float64, device=dev)
-        self.P = [torch.zeros(N0, dtype=st, device=dev), torch.zeros(N0, dtype=st, device=dev)]
+        so = TORCH_STORE[p.outer]
+        self.P = [torch.zeros(N0, dtype=so, device=dev), torch.zeros(N0, dtype=so, device=dev)]
+        # z = V(rlow) gets its own buffer when it is wider than the cycle's vectors
+        self.Z = torch.zeros(N0, dtype=so, device=dev) if p.outer != p.storage else None
         nc = math.prod(self.levels[-1])
         self.ainv = torch.zeros(nc * nc, dtype=torch.float64, device=dev)
         self.nblk = fine_blocks(self.shape)
@@ -224,12 +233,12 @@ class Instance:
             self.released = True
             for name in ("face32", "faceT", "X", "Y", "B", "P"):
                 setattr(self, name, [])
-            self.sigma = self.b32 = self.r64 = self.ainv = None
+            self.sigma = self.b32 = self.r64 = self.ainv = self.Z = None
             self._tpl = None      # its addresses are no longer valid
 
     def nbytes(self) -> int:
         ts = [*self.face32, *self.X, *self.Y, *self.B, *self.P, self.sigma, self.b32, self.x64,
-              self.r64, self.ainv]
+              self.r64, self.ainv] + ([self.Z] if self.Z is not None else [])
         if self.pool.params.storage != "f32":
             ts += self.faceT
         return sum(t.numel() * t.element_size() for t in ts)
@@ -246,7 +255,9 @@ class Instance:
             tot += faces * 4 + (faces * e if params.storage != "f32" else 0) + 3 * N * e
         N0 = math.prod(shape)
         nc = math.prod(lv[-1])
-        return tot + 3 * _align(N0) * 4 + N0 * (4 + 16 + 2 * e) + nc * nc * 8
+        eo = ELEM[params.outer]
+        z = N0 * eo if params.outer != params.storage else 0
+        return tot + 3 * _align(N0) * 4 + N0 * (4 + 16 + 2 * eo) + z + nc * nc * 8
 
     def faces(self, lev: int, low: bool) -> List[int]:
         """Addresses of the six face arrays of a level (storage copy if ``low``)."""
@@ -362,6 +373,8 @@ def instance_templates(inst: Instance, p: Params) -> dict:
             t["b"] = inst.B[lev].data_ptr()
             if s == 0:
                 t["ec"] = E(lev + 1)
+            if lev == 0 and s == p.nu2 - 1 and inst.Z is not None:
+                t["xout"] = inst.Z.data_ptr()
             tpl[("up", lev, s)] = t
     lc = len(inst.levels) - 1
     t = _record(inst, lc)
@@ -374,7 +387,7 @@ def instance_templates(inst: Instance, p: Params) -> dict:
     base["x64"], base["r64"] = inst.x64.data_ptr(), inst.r64.data_ptr()
     base["rlow"] = inst.B[0].data_ptr()
     base["b32"] = inst.b32.data_ptr()
-    base["b"] = xy(post_k, 0)
+    base["b"] = xy(post_k, 0) if inst.Z is None else inst.Z.data_ptr()   # z
     base["xout"] = xy(0, 0)     # first fine-level pre-sweep, fused into init / update
     tpl[("init",)] = base
     for par in range(2):
@@ -513,8 +526,10 @@ class Plan:
         kind, mode, tasks, lay, dot = op[:5]
         pool, p = self.pool, self.pool.params
         if kind == "sweep":
+            # the sweep carrying the r.z dot is the fine level's last: it writes z
+            out = pool.outer_id if dot else pool.store_id
             L.call("hfm_mg_sweep", tasks.data_ptr(), lay.blk_task.data_ptr(), lay.nblocks,
-                   pool.sc.data_ptr(), pool.store_id, mode, p.omega, p.alpha,
+                   pool.sc.data_ptr(), pool.store_id, out, mode, p.omega, p.alpha,
                    pool.dots.data_ptr() if dot else 0, s)
         else:
             L.call("hfm_mg_resid_restrict", tasks.data_ptr(), lay.blk_task.data_ptr(),
@@ -541,12 +556,12 @@ class Plan:
     def launch_dir(self, parity, s):
         pool, fl = self.pool, self.flay
         L.call("hfm_fcg_dir", self.dir_tasks[parity].data_ptr(), fl.blk_task.data_ptr(),
-               fl.nblocks, pool.sc.data_ptr(), pool.store_id, pool.dots.data_ptr(), s)
+               fl.nblocks, pool.sc.data_ptr(), pool.outer_id, pool.dots.data_ptr(), s)
 
     def launch_update(self, parity, s):
         pool, fl = self.pool, self.flay
         L.call("hfm_fcg_update_sweep", self.upd_tasks[parity].data_ptr(), fl.blk_task.data_ptr(),
-               fl.nblocks, pool.sc.data_ptr(), pool.store_id, pool.params.omega,
+               fl.nblocks, pool.sc.data_ptr(), pool.store_id, pool.outer_id, pool.params.omega,
                pool.dots.data_ptr(), s)
 
     def iteration(self, s, parity):
@@ -586,8 +601,9 @@ class MGBatch:
     def __init__(self, shapes: Sequence[Sequence[int]], storage: str = "bf16", tol: float = 1e-8,
                  maxit: int = 200, nu1: int = 2, nu2: int = 2, omega: float = 2.0 / 3.0,
                  alpha: float = 1.6, max_coarse_edge: int = 4, device=None,
-                 iters_per_graph: int = 4, use_graph: bool = True, shrink: float = 0.5):
-        self.params = Params(storage, tol, maxit, nu1, nu2, omega, alpha, max_coarse_edge)
+                 iters_per_graph: int = 4, use_graph: bool = True, shrink: float = 0.5,
+                 outer: Optional[str] = None):
+        self.params = Params(storage, tol, maxit, nu1, nu2, omega, alpha, max_coarse_edge, outer)
         shapes = [check_shape(s, max_coarse_edge)[0] for s in shapes]
         if not shapes:
             raise ValueError("empty batch")
@@ -703,9 +719,10 @@ class MGBatch:
         self.plan._launch(op, L.stream_handle(self.device))
 
     def update_bytes_per_cell(self) -> int:
-        """Algorithmic bytes of the fused k_fcg_update per fine cell: p (T), 3 FP32
-        faces, x64 and r64 read + written, rlow (T) and the first sweep (T) written."""
-        return 3 * ELEM[self.storage] + 12 + 32
+        """Algorithmic bytes of the fused k_fcg_update per fine cell: p (outer type),
+        3 FP32 faces, x64 and r64 read + written, rlow (T) and the first sweep (T)
+        written."""
+        return 2 * ELEM[self.storage] + ELEM[self.params.outer] + 12 + 32
 
     def sweep_bytes_per_cell(self) -> int:
         """Algorithmic bytes of a PLAIN sweep per cell: x, b, 3 faces, out (all T)."""

@@ -150,11 +150,12 @@ class UrbanProjection(UrbanFlow):
     sigma-mask obstacle term, and instead of ``multi_itr`` sawtooth cycles of the
     27-point layer, ``A p = b`` with A the 7-point -Laplacian (Neumann faces, a
     Dirichlet outlet at x = W-1) solved by the FCG multigrid of ``hfm.mg`` in FP64
-    with ``storage`` (FP32 or BF16) V-cycles, from zero, to ``tol``."""
+    with ``storage`` (FP32 or BF16) V-cycles, from zero, to ``tol``; ``outer``
+    is the storage of the preconditioned residual and search direction."""
 
     def __init__(self, shape, sigma=None, inflow=None, dt: float = 0.4, dx: float = 1.0,
                  Re: float = 1 / 20, storage: str = "bf16", tol: float = 1e-8,
-                 lscalar: bool = True, device=None):
+                 lscalar: bool = True, device=None, outer=None):
         from .mg import MGBatch
         super().__init__(shape, sigma=sigma, dt=dt, dx=dx, Re=Re, ub=1.0, multi_itr=0,
                          lscalar=lscalar, device=device)
@@ -168,7 +169,8 @@ class UrbanProjection(UrbanFlow):
         cz[0] = 0.0             # Neumann at the ground
         faces = (cx, cy, cz, torch.full((D, H), inv),   # Dirichlet outlet at x = W-1
                  torch.zeros((D, W)), torch.zeros((H, W)))
-        self.solver = MGBatch([self.shape], storage=storage, tol=tol, maxit=500, device=self.device)
+        self.solver = MGBatch([self.shape], storage=storage, tol=tol, maxit=500,
+                              device=self.device, outer=outer)
         self.solver.set_faces(0, faces, torch.zeros(self.shape))
         self.solver.setup()
         self.last_info = None

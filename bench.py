@@ -154,7 +154,8 @@ def urban(args, dev, torch):
     """Urban flow on 256 x 256 x 64 (D = 64 heights): seeded blocks covering 20 % of
     the ground, log-law inflow, the reference's explicit 27-point momentum and
     scalar transport, and a 7-point MG pressure projection to 1e-8 per step;
-    BF16-storage V-cycles vs FP32, with the final pressure of each compared with
+    BF16-storage V-cycles (z and p in BF16, or in FP32) vs FP32, with the final
+    pressure of each compared with
     an FP64-converged (1e-12) solve of the same system."""
     from hfm.timestep import UrbanProjection
     from hfm.workloads import log_law_profile, urban_blocks
@@ -166,9 +167,10 @@ def urban(args, dev, torch):
     out = {"grid": list(sh), "steps": args.urban_steps, "dt": dt,
            "ground_cover": round(float(mask[0].mean()), 3)}
     finals = {}
-    for storage in ("bf16", "f32"):
-        flow = UrbanProjection(sh, sigma=sig, inflow=prof, dt=dt, storage=storage, tol=1e-8,
-                               device=dev)
+    variants = {"bf16": ("bf16", None), "bf16_fp32_zp": ("bf16", "f32"), "f32": ("f32", None)}
+    for name, (storage, outer) in variants.items():
+        flow = UrbanProjection(sh, sigma=sig, inflow=prof, dt=dt, storage=storage, outer=outer,
+                               tol=1e-8, device=dev)
         u0 = torch.from_numpy(np.broadcast_to(prof[:, None, None], sh).copy()).to(dev)
         z = torch.zeros(sh, device=dev)
         state = (u0, z, z.clone(), z.clone(), z.clone())
@@ -180,23 +182,23 @@ def urban(args, dev, torch):
             state = flow.step(*state)
             its.append(int(flow.last_info["iter"][0]))
             if int(flow.last_info["status"][0]) != 1:
-                raise SystemExit(f"bench: urban {storage} pressure solve failed: {flow.last_info}")
+                raise SystemExit(f"bench: urban {name} pressure solve failed: {flow.last_info}")
         torch.cuda.synchronize()
         tstep = (time.perf_counter() - t0) / args.urban_steps
-        out[storage] = {"ms_per_step": round(tstep * 1e3, 3),
+        out[name] = {"ms_per_step": round(tstep * 1e3, 3),
                         "cell_steps_per_s": round(float(np.prod(sh)) / tstep, 1),
                         "mg_iters_mean": round(float(np.mean(its)), 2),
                         "rel_residual": float(flow.last_info["rel"][0])}
         inst = flow.solver.insts[0]
-        finals[storage] = (inst.b32.clone(), inst.x64.clone())
-        if storage == "f32":
+        finals[name] = (inst.b32.clone(), inst.x64.clone())
+        if name == "f32":
             ref_solver = UrbanProjection(sh, sigma=sig, inflow=prof, dt=dt, storage="f32",
                                          tol=1e-12, device=dev).solver
-    for storage, (b, x) in finals.items():
+    for name, (b, x) in finals.items():
         ref_solver.insts[0].b32.copy_(b)
         ref_solver.solve()
         xr = ref_solver.insts[0].x64
-        out[storage]["pressure_rel_err_vs_fp64"] = float((x - xr).norm() / xr.norm())
+        out[name]["pressure_rel_err_vs_fp64"] = float((x - xr).norm() / xr.norm())
     log(f"bench: urban {out}")
     return out
 
@@ -231,7 +233,8 @@ def profile_kernel(args, dev, torch):
     from hfm.workloads import batched_specs
     specs = batched_specs(args.instances, args.seed)
     cells = sum(math.prod(s[0]) for s in specs)
-    mg = MGBatch([s[0] for s in specs], storage=args.storage, tol=1e-8, device=dev)
+    mg = MGBatch([s[0] for s in specs], storage=args.storage, tol=1e-8, device=dev,
+                 outer=args.outer)
     load_batch(mg, specs, dev)
     mg.build_faces()
     mg.setup()
@@ -262,6 +265,8 @@ def main():
     ap.add_argument("--instances", type=int, default=512)
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--storage", default="bf16", choices=["bf16", "f32"])
+    ap.add_argument("--outer", default=None, choices=["bf16", "f32"],
+                    help="storage of z and the search direction (default: --storage)")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--fp32-sample", type=int, default=16)
@@ -289,7 +294,8 @@ def main():
     shapes = [s[0] for s in specs]
     cells = sum(math.prod(s) for s in shapes)
     log(f"bench: {len(specs)} instances, {cells / 1e6:.1f} M fine cells, storage {args.storage}")
-    mg = MGBatch(shapes, storage=args.storage, tol=1e-8, maxit=300, device=dev, iters_per_graph=4)
+    mg = MGBatch(shapes, storage=args.storage, tol=1e-8, maxit=300, device=dev, iters_per_graph=4,
+                 outer=args.outer)
     load_batch(mg, specs, dev)
     torch.cuda.synchronize()
     log(f"bench: resident {mg.nbytes() / 2**30:.1f} GiB; warmup {args.warmup}")
@@ -344,6 +350,7 @@ def main():
         "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(total / args.steps * 1e3, 2), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64", "storage": args.storage,
+        "outer_storage": mg.params.outer,
         "data": "synthetic",
         "config": {"workload": f"batched uneven anisotropic solves: {len(specs)} instances, edges "
                                "{32,48,64,96,128}^3, log-normal sigma_ln U[0.5,2], 1-4 dipoles, "

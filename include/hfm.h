@@ -198,13 +198,16 @@ typedef struct hfm_mg_scalars {
  * task index of block g (uint16), nblocks the grid size.  Tasks are padded to
  * a multiple of 8 blocks so that blocks of one task spread evenly over XCDs. */
 
-/* Weighted-Jacobi sweep on the face operator (storage `store`).  Slots: c =
+/* Weighted-Jacobi sweep on the face operator (storage `store` for c, xin, b,
+ * ec; `out_store` for xout: the fine level's last post-sweep may write the
+ * preconditioned residual z wider than the cycle's internals).  Slots: c =
  * this level's coefficients, xin (unused for ZERO), b, xout, ec (coarse
  * correction for CORRECT, extents Dc/Hc/Wc).  dots != NULL: partial
- * sum(b*out) per block into dots[slot0 + local block]. */
+ * sum(b*out) per block into dots[slot0 + local block], out rounded to
+ * out_store. */
 int hfm_mg_sweep(const hfm_mg_task* tasks, const uint16_t* blk_task, int64_t nblocks,
-                 const hfm_mg_scalars* sc, int store, int mode, float omega, float alpha,
-                 double* dots, void* stream);
+                 const hfm_mg_scalars* sc, int store, int out_store, int mode, float omega,
+                 float alpha, double* dots, void* stream);
 
 /* bc = R (b - A x): one coarse cell per thread over its <= 8 children,
  * R = P^T/8.  Slots: c, xin = x, b, xout = bc (coarse, extents Dc/Hc/Wc).
@@ -217,9 +220,11 @@ int hfm_mg_resid_restrict(const hfm_mg_task* tasks, const uint16_t* blk_task, in
 int hfm_mg_coarse_solve(const hfm_mg_task* tasks, int64_t ntasks, const hfm_mg_scalars* sc,
                         int store, void* stream);
 
-/* FCG kernels on the fine level (coefficients c in FP32).
+/* FCG kernels on the fine level (coefficients c in FP32).  T = `store` (the
+ * V-cycle's storage: rlow and the fused first sweep), P = `pstore` (z and the
+ * search direction p; hfm_fcg_dir's `store` argument is P).
  * init:   x64 = 0, r64 = b32, rlow = T(b32), dots = b.b  (staged slots only)
- * dir:    xout = p = T(b + beta xin) (b = z, xin = p_old), dots = p.(A p)
+ * dir:    xout = p = P(b + beta xin) (b = z, xin = p_old), dots = p.(A p)
  * update: x64 += alpha p, r64 -= alpha A p, rlow = T(r64), dots = r.r  (xin = p)
  * *_sweep variants also write the next V-cycle's first fine-level sweep from
  * a zero guess, xout = omega rlow / d (d = diagonal of the faces rounded to
@@ -229,13 +234,13 @@ int hfm_fcg_init(const hfm_mg_task* tasks, const uint16_t* blk_task, int64_t nbl
 int hfm_fcg_dir(const hfm_mg_task* tasks, const uint16_t* blk_task, int64_t nblocks,
                 const hfm_mg_scalars* sc, int store, double* dots, void* stream);
 int hfm_fcg_update(const hfm_mg_task* tasks, const uint16_t* blk_task, int64_t nblocks,
-                   const hfm_mg_scalars* sc, int store, double* dots, void* stream);
+                   const hfm_mg_scalars* sc, int store, int pstore, double* dots, void* stream);
 int hfm_fcg_init_sweep(const hfm_mg_task* tasks, const uint16_t* blk_task, int64_t nblocks,
                        const hfm_mg_scalars* sc, int store, float omega, double* dots,
                        void* stream);
 int hfm_fcg_update_sweep(const hfm_mg_task* tasks, const uint16_t* blk_task, int64_t nblocks,
-                         const hfm_mg_scalars* sc, int store, float omega, double* dots,
-                         void* stream);
+                         const hfm_mg_scalars* sc, int store, int pstore, float omega,
+                         double* dots, void* stream);
 
 /* Per-instance fixed-order reduction of dots[slot0 .. slot0+nslot) and the
  * scalar step:  kind 0 INIT (bnorm2), 1 BETA (rz, beta), 2 ALPHA (pap,

@@ -652,8 +652,12 @@ def jacobi_face(op: FaceOp, x, b, omega, storage):
     return round_storage(x + omega * (b - face_apply(op, x)) / d, storage)
 
 
-def vcycle(h: Hierarchy, b, nu1=2, nu2=2, omega=2.0 / 3.0, alpha=1.0, level=0, x0=None):
+def vcycle(h: Hierarchy, b, nu1=2, nu2=2, omega=2.0 / 3.0, alpha=1.0, level=0, x0=None,
+           out_storage: Optional[str] = None):
     """One V(nu1, nu2) cycle on level ``level`` (``x0=None``: zero initial guess).
+
+    ``out_storage`` rounds the cycle's result (its last post-sweep) to another
+    type than the hierarchy's storage (the device's wider z buffer).
 
     Kernel order per non-coarsest level (each output rounded to storage):
       pre-sweep 1 (from zero when x0 is None), pre-sweeps 2..nu1, residual +
@@ -673,12 +677,13 @@ def vcycle(h: Hierarchy, b, nu1=2, nu2=2, omega=2.0 / 3.0, alpha=1.0, level=0, x
     bc = round_storage(restrict_sum8(b - face_apply(op, x)), s)
     ec = vcycle(h, bc, nu1, nu2, omega, alpha, level + 1)
     xc = x + alpha * prolong_agg(ec, op.shape)   # not stored: fused into the sweep
+    so = out_storage or s
     if nu2 >= 1:
-        x = jacobi_face(op, xc, b, omega, s)
-        for _ in range(nu2 - 1):
-            x = jacobi_face(op, x, b, omega, s)
+        x = jacobi_face(op, xc, b, omega, s if nu2 > 1 else so)
+        for q in range(nu2 - 1):
+            x = jacobi_face(op, x, b, omega, s if q < nu2 - 2 else so)
     else:
-        x = round_storage(xc, s)
+        x = round_storage(xc, so)
     return x
 
 
@@ -741,18 +746,20 @@ class FCGResult:
 
 def fcg_solve(fine: FaceOp, b, tol=1e-8, storage="f32", maxit=200, nu1=2, nu2=2,
               omega=2.0 / 3.0, alpha=1.6, hierarchy: Optional[Hierarchy] = None,
-              max_coarse_edge: int = 4) -> FCGResult:
+              max_coarse_edge: int = 4, outer: Optional[str] = None) -> FCGResult:
     """FP64 conjugate gradients preconditioned by one storage-precision V-cycle.
 
-    Mirrors the device algorithm step for step:
-      r = b, rl = T(r);  per iteration: z = V(rl) from zero;
+    Mirrors the device algorithm step for step (T = storage, P = outer,
+    default T):
+      r = b, rl = T(r);  per iteration: z = P(V(rl)) from zero;
       rz = rl . z;  beta = rz / rz_prev (0 on the first iteration);
-      p = T(fma32(beta, p, z));  q = A p (FP32 coefficients, FP64 arithmetic);
+      p = P(fma32(beta, p, z));  q = A p (FP32 coefficients, FP64 arithmetic);
       a = rz / (p . q);  x += a p;  r -= a q;  rl = T(r);
       stop when ||r|| / ||b|| <= tol.
     """
     h = hierarchy or build_hierarchy(fine, storage, max_coarse_edge)
     s = h.storage
+    so = outer or s
     A32 = h.fine32
     b64 = np.asarray(b, dtype=np.float32).astype(np.float64)
     bn2 = float(np.sum(b64 * b64))
@@ -765,13 +772,13 @@ def fcg_solve(fine: FaceOp, b, tol=1e-8, storage="f32", maxit=200, nu1=2, nu2=2,
     p = None
     hist = [1.0]
     for it in range(maxit):
-        z = vcycle(h, rl, nu1, nu2, omega, alpha)
+        z = vcycle(h, rl, nu1, nu2, omega, alpha, out_storage=so)
         rzn = float(np.sum(rl * z))
         beta = 0.0 if it == 0 else rzn / rz
         rz = rzn
         # one FP32 fused multiply-add, then the storage rounding
         p = round_storage(z if beta == 0.0 else
-                          (np.float64(np.float32(beta)) * p + z).astype(np.float32), s)
+                          (np.float64(np.float32(beta)) * p + z).astype(np.float32), so)
         q = face_apply(A32, p)
         pap = float(np.sum(p * q))
         if not (pap > 0.0) or not np.isfinite(pap):

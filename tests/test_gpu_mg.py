@@ -58,15 +58,16 @@ def test_coarse_inverse(dev):
         np.testing.assert_allclose(got, h.coarse_inv, rtol=1e-10, atol=1e-12 * np.abs(h.coarse_inv).max())
 
 
-@pytest.mark.parametrize("storage", ["f32", "bf16"])
+@pytest.mark.parametrize("storage,outer", [("f32", None), ("bf16", None), ("bf16", "f32")])
 @pytest.mark.parametrize("shape", [(32, 32, 32), (20, 12, 36)])
-def test_solve_matches_oracle(dev, storage, shape):
-    # W = 32: 8-wide row kernels on the fine levels; W = 36: per-cell kernels
+def test_solve_matches_oracle(dev, storage, outer, shape):
+    # W = 32: 8-wide row kernels on the fine levels; W = 36: per-cell kernels.
+    # outer = "f32": BF16 V-cycle with z and p kept in FP32.
     spec = (shape, 7, 1.0, 1)
-    mg, probs = _batch(dev, [spec], storage, tol=1e-8, maxit=100)
+    mg, probs = _batch(dev, [spec], storage, tol=1e-8, maxit=100, outer=outer)
     info = mg.solve()
     op, b = probs[0]
-    ref = O.fcg_solve(op, b, tol=1e-8, storage=storage, maxit=100)
+    ref = O.fcg_solve(op, b, tol=1e-8, storage=storage, maxit=100, outer=outer)
     assert info["status"][0] == 1 and ref.status == "converged"
     assert int(info["iter"][0]) == ref.iters
     x = mg.solution(0).cpu().numpy()

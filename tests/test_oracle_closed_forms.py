@@ -145,6 +145,28 @@ def test_fcg_solve_matches_fp64_direct(storage):
     assert np.linalg.norm(r) / np.linalg.norm(b) == pytest.approx(res.rel_history[-1], rel=1e-3)
 
 
+def test_vcycle_wider_output_changes_only_the_last_rounding():
+    # z kept in FP32 over a BF16 cycle: rounding it to BF16 gives the BF16 cycle
+    # bit for bit (BF16 rounding goes through FP32 in both), and it is not BF16
+    op, b = O.make_anisotropic_problem((16, 12, 20), 5, 1.0, 2)
+    h = O.build_hierarchy(op, "bf16")
+    for nu2 in (1, 2, 3):
+        zb = O.vcycle(h, O.round_storage(b, "bf16"), nu2=nu2, alpha=1.6)
+        zf = O.vcycle(h, O.round_storage(b, "bf16"), nu2=nu2, alpha=1.6, out_storage="f32")
+        np.testing.assert_array_equal(O.round_storage(zf, "bf16"), zb)
+        assert np.any(zf != zb)
+        np.testing.assert_array_equal(O.round_storage(zf, "f32"), zf)
+
+
+def test_fcg_wider_direction_converges_to_fp64_direct():
+    op, b = O.make_anisotropic_problem((16, 16, 16), 11, 1.5, 2)
+    res = O.fcg_solve(op, b, tol=1e-8, storage="bf16", outer="f32")
+    ref = O.fcg_solve(op, b, tol=1e-8, storage="bf16")
+    assert res.status == "converged" and res.iters <= ref.iters
+    xd = O.solve_fp64_direct(op.astype("f32"), b.astype(np.float32))
+    assert np.linalg.norm(res.x - xd) / np.linalg.norm(xd) < 1e-6
+
+
 def test_vcycle_is_symmetric_positive_definite():
     # the FP64 V(2,2) cycle with exact coarse solve is an SPD preconditioner
     op, _ = _rand_faces((6, 6, 6), seed=9)
