@@ -9,7 +9,7 @@ PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(os.path.dirname(PKG_DIR), "csrc")
 INCLUDE = os.path.join(os.path.dirname(os.path.dirname(PKG_DIR)), "include")
 LIB_PATH = os.path.join(PKG_DIR, "libhfm.so")
-SOURCES = ("hfm_ref.hip", "hfm_mg.hip")
+SOURCES = ("hfm_ref.hip", "hfm_mg.hip", "hfm_flow2d.hip")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 # Packed-FP32 instructions are disabled for every device pass.

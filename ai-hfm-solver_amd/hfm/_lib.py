@@ -86,7 +86,26 @@ SIGNATURES = {
     "hfm_mg_galerkin_batched": (_int, [_vp, _vp, _i64, _vp]),
     "hfm_mg_convert_batched": (_int, [_vp, _vp, _i64, _int, _vp]),
     "hfm_mg_coarse_inverse": (_int, [_vp, _i64, _vp]),
+    "hfm_flow2d_steps": (_int, [_vp, _i64, _vp, _vp, _int, _int, _vp]),
+    "hfm_flow2d_struct_sizes": (_int, [C.POINTER(_i64), C.POINTER(_i64)]),
 }
+
+FLOW_TASK_DTYPE = np.dtype([
+    ("u", "<u8"), ("v", "<u8"), ("uh", "<u8"), ("vh", "<u8"), ("z", "<u8"), ("p", "<u8"),
+    ("phi", "<u8"), ("r", "<u8"), ("mgx", "<u8"), ("mgy", "<u8"), ("mgb", "<u8"),
+    ("fu", "<u8"), ("stats", "<u8"),
+])
+
+FLOW_PARAMS_DTYPE = np.dtype([
+    ("nu_h2", "<f4"), ("inv_h", "<f4"), ("h", "<f4"), ("dt", "<f4"), ("cfl_coef", "<f4"),
+    ("omega", "<f4"), ("alpha", "<f4"),
+    ("N", "<i4"), ("nu1", "<i4"), ("nu2", "<i4"), ("nlev", "<i4"), ("maxit", "<i4"),
+    ("max_sub", "<i4"), ("pad", "<i4"), ("tol", "<f8"),
+])
+
+FLOW_STATS = ("iters", "rel", "iters_sum", "status", "steps", "substeps", "rel_max", "nsub")
+FLOW_STATUS = {**STATUS, 4: "cfl"}
+FLOW_PROJECT_ONLY = 1
 
 _lock = threading.Lock()
 _lib = None
@@ -121,6 +140,11 @@ def load() -> C.CDLL:
         if ts.value != TASK_DTYPE.itemsize or ss.value != SCALARS_DTYPE.itemsize:
             raise HfmError(f"ABI mismatch: task {ts.value} vs {TASK_DTYPE.itemsize}, "
                            f"scalars {ss.value} vs {SCALARS_DTYPE.itemsize}")
+        if lib.hfm_flow2d_struct_sizes(C.byref(ts), C.byref(ss)) != HFM_OK:
+            raise HfmError("hfm_flow2d_struct_sizes failed")
+        if ts.value != FLOW_TASK_DTYPE.itemsize or ss.value != FLOW_PARAMS_DTYPE.itemsize:
+            raise HfmError(f"ABI mismatch: flow task {ts.value} vs {FLOW_TASK_DTYPE.itemsize}, "
+                           f"params {ss.value} vs {FLOW_PARAMS_DTYPE.itemsize}")
         _lib = lib
         return lib
 

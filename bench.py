@@ -203,6 +203,80 @@ def urban(args, dev, torch):
     return out
 
 
+def kolmogorov(args, dev, torch):
+    """2D Kolmogorov flow, 256^2 periodic, nu = 1/100, f_x = sin 4y, dt = 0.005:
+    one flow for ``--flow-steps`` steps (the configuration's run is 1000), an
+    ensemble of ``--flow-ensemble`` flows in one launch, and the oracle's step
+    on one host core over a bounded sample."""
+    from hfm.flow2d import KolmogorovFlow
+    from oracle import hfm_flow2d as F
+    N = 256
+    out = {"grid": [N, N], "nu": 0.01, "dt": 0.005, "tol": 1e-6, "levels": 6}
+    fl = KolmogorovFlow(N=N, nflows=1, device=dev)
+    u0, v0 = F.random_modes_ic(N, args.seed)
+    fl.set_state(0, torch.from_numpy(u0), torch.from_numpy(v0))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    done = 0
+    while done < args.flow_steps:            # progress every 250 steps
+        k = min(250, args.flow_steps - done)
+        fl.step(k)
+        torch.cuda.synchronize()
+        done += k
+        log(f"bench: kolmogorov step {done}")
+    t1 = time.perf_counter() - t0
+    st = fl.stats()[0]
+    if int(st["status"]) != 1 or int(st["steps"]) != args.flow_steps:
+        raise SystemExit(f"bench: kolmogorov flow stopped: {st}")
+    div = fl.divergence(0)
+    u, v, _ = fl.state(0)
+    out["single_flow"] = {
+        "steps": args.flow_steps, "wall_s": round(t1, 3),
+        "ms_per_step": round(t1 / args.flow_steps * 1e3, 4),
+        "steps_per_s": round(args.flow_steps / t1, 1),
+        "cell_steps_per_s": round(N * N * args.flow_steps / t1, 1),
+        "substeps": int(st["substeps"]), "cg_iters_per_substep": round(st["iters_sum"] / st["substeps"], 3),
+        "max_rel_residual": float(st["rel_max"]),
+        "max_abs_divergence": float(div.abs().max()),
+        "kinetic_energy": float(0.5 * (u.double() ** 2 + v.double() ** 2).mean())}
+    if args.flow_ensemble > 0:
+        ens = KolmogorovFlow(N=N, nflows=args.flow_ensemble, device=dev)
+        for q in range(args.flow_ensemble):
+            u, v = F.random_modes_ic(N, args.seed + q)
+            ens.set_state(q, torch.from_numpy(u), torch.from_numpy(v))
+        ens.step(1)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ens.step(args.flow_ensemble_steps)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter() - t0
+        es = ens.stats()
+        if not np.all(es["status"] == 1):
+            raise SystemExit("bench: kolmogorov ensemble flow stopped")
+        fs = args.flow_ensemble * args.flow_ensemble_steps / t2
+        out["ensemble"] = {"flows": args.flow_ensemble, "steps": args.flow_ensemble_steps,
+                           "flow_steps_per_s": round(fs, 1),
+                           "cell_steps_per_s": round(fs * N * N, 1),
+                           "resident_GiB": round(ens.nbytes() / 2 ** 30, 2)}
+        del ens
+    if not args.no_cpu_baseline:
+        mg = F.build_mg(N)
+        stc = F.FlowState(u0.copy(), v0.copy(), np.zeros((N, N)))
+        fu = F.forcing_u(N)
+        t0 = time.perf_counter()
+        n = 0
+        while n < 10 or (time.perf_counter() - t0 < args.cpu_budget / 3 and n < args.flow_steps):
+            F.step(stc, mg, 0.005, 0.01, fu)
+            n += 1
+        tc = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(n / tc, 2), "unit": "steps/s", "cores": 1,
+                               "kind": "port", "sample": f"oracle step (numpy), first {n} "
+                               "steps of the same flow"}
+        out["speedup_vs_cpu"] = round(out["single_flow"]["steps_per_s"] / (n / tc), 1)
+    log(f"bench: kolmogorov {out}")
+    return out
+
+
 def profile_iteration(mg, cells, torch):
     """Time every launch of one FCG iteration separately (all instances active)."""
     from hfm import _lib as L
@@ -276,6 +350,10 @@ def main():
                     help="launch only this kernel (after setup) for profiler runs")
     ap.add_argument("--reps", type=int, default=20)
     ap.add_argument("--urban-steps", type=int, default=10)
+    ap.add_argument("--flow-steps", type=int, default=1000,
+                    help="Kolmogorov 2D flow steps of the single-flow run (0: skip)")
+    ap.add_argument("--flow-ensemble", type=int, default=256)
+    ap.add_argument("--flow-ensemble-steps", type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the solves are independent and this build runs "
@@ -341,6 +419,8 @@ def main():
         extra["serving"] = serving(args, dev, torch)
     if args.urban_steps > 0:
         extra["urban_flow"] = urban(args, dev, torch)
+    if args.flow_steps > 0:
+        extra["kolmogorov_2d"] = kolmogorov(args, dev, torch)
     cpu = None
     if not args.no_cpu_baseline:
         torch.set_num_threads(1)

@@ -283,6 +283,87 @@ int hfm_mg_convert_batched(const hfm_mg_task* tasks, const uint16_t* blk_task, i
                            int store, void* stream);
 int hfm_mg_coarse_inverse(const hfm_mg_task* tasks, int64_t ntasks, void* stream);
 
+/* ---------------- 2D periodic flow (Kolmogorov configuration) ----------------
+ * The reference's 2D layers (explicit two-stage stencil update,
+ * ARTn/Advection_diffusion.ipynb:291-296; mean restriction / nearest
+ * prolongation / Jacobi multigrid, ARTn/Parallel_multigrid_2D.ipynb#c5-#c20)
+ * on an N x N periodic staggered grid: u on x-faces, v on y-faces, phi at
+ * cell centres, arrays row-major [j][i].  One workgroup (1024 threads) runs
+ * a whole step of one flow; a launch advances `nflows` independent flows.
+ *
+ * Per step of dt: n = max(1, ceil((max|u| + max|v|) * cfl_coef)) substeps of
+ * dt/n (cfl_coef = dt / (h cfl_max)), each
+ *   uh = u + dt/2 F(u);  u* = u + dt F(uh)      (F: 5-point diffusion,
+ *        velocity-signed first-order upwind advection, forcing fu[j] on u)
+ *   b  = h (u*_i - u*_{i+1} + v*_j - v*_{j+1}) - mean(b)
+ *   A phi = b, A = 4 phi - sum of the 4 neighbours, by FP64 CG (phi, r FP64,
+ *        warm start) preconditioned by one FP32 V(nu1, nu2) cycle: weighted
+ *        Jacobi, 2x2 mean restriction, nearest prolongation times alpha,
+ *        level l operator 2^-l A, coarsest level by the FP64 matrix `pinv`
+ *        (the pseudo-inverse of the coarsest level's operator, n_c^2 x n_c^2,
+ *        n_c = N >> (nlev-1), n_c^2 <= 1024);
+ *   u = u* - (phi - phi_w)/h,  v = v* - (phi - phi_s)/h.
+ * Stopping rule per substep: ||r||_2 <= tol ||b||_2.
+ *
+ * Buffers per flow (device): u, v, uh, vh, z, p: N*N floats; phi, r: N*N
+ * doubles; mgx, mgy, mgb: sum over levels of (N >> l)^2 floats; fu: N floats;
+ * stats: HFM_FLOW_STATS doubles. */
+#define HFM_FLOW_STATS 8
+#define HFM_FLOW_ST_ITERS 0      /* CG iterations of the last substep         */
+#define HFM_FLOW_ST_REL 1        /* its final relative residual               */
+#define HFM_FLOW_ST_ITERS_SUM 2  /* iterations summed over all substeps        */
+#define HFM_FLOW_ST_STATUS 3     /* HFM_MG_* of the last substep; HFM_FLOW_CFL */
+#define HFM_FLOW_ST_STEPS 4      /* steps completed                           */
+#define HFM_FLOW_ST_SUBSTEPS 5   /* substeps summed                           */
+#define HFM_FLOW_ST_REL_MAX 6    /* largest final relative residual            */
+#define HFM_FLOW_ST_NSUB 7       /* substeps of the last step                  */
+#define HFM_FLOW_CFL 4           /* status: more than max_sub substeps needed */
+#define HFM_FLOW_PROJECT_ONLY 1  /* flag: skip the momentum stages (project u, v) */
+
+typedef struct hfm_flow2d_task {
+  float* u;
+  float* v;
+  float* uh;
+  float* vh;
+  float* z;
+  float* p;
+  double* phi;
+  double* r;
+  float* mgx;
+  float* mgy;
+  float* mgb;
+  const float* fu;
+  double* stats;
+} hfm_flow2d_task; /* 104 bytes */
+
+typedef struct hfm_flow2d_params {
+  float nu_h2;    /* nu / h^2 */
+  float inv_h;    /* 1 / h */
+  float h;
+  float dt;
+  float cfl_coef; /* dt / (h cfl_max) */
+  float omega;
+  float alpha;
+  int32_t N;      /* grid edge, a power of two; the same for every flow */
+  int32_t nu1;
+  int32_t nu2;
+  int32_t nlev;   /* levels N, N/2, ..., N >> (nlev-1) >= 2 */
+  int32_t maxit;
+  int32_t max_sub;
+  int32_t pad;
+  double tol;
+} hfm_flow2d_params; /* 64 bytes */
+
+/* Advance every flow of `tasks` (device array) by `nsteps` steps (flags:
+ * HFM_FLOW_PROJECT_ONLY projects u, v once instead).  A flow whose status is
+ * not converged stops stepping.  No reference counterpart: the reference's
+ * time-step drivers are Python loops over Keras layer calls
+ * (UM/AI_HFM_v3.ipynb cell 25). */
+int hfm_flow2d_steps(const hfm_flow2d_task* tasks, int64_t nflows, const hfm_flow2d_params* prm,
+                     const double* pinv, int nsteps, int flags, void* stream);
+/* sizeof(hfm_flow2d_task) and sizeof(hfm_flow2d_params) as compiled. */
+int hfm_flow2d_struct_sizes(int64_t* task_size, int64_t* params_size);
+
 #ifdef __cplusplus
 }
 #endif

@@ -34,6 +34,31 @@ def test_abi_struct_sizes_match_host_dtypes():
     assert b.value == L.SCALARS_DTYPE.itemsize == 96
 
 
+def test_flow2d_struct_sizes_match_host_dtypes():
+    from hfm import _lib as L
+    lib = L.load()
+    a, b = L.C.c_int64(), L.C.c_int64()
+    assert lib.hfm_flow2d_struct_sizes(L.C.byref(a), L.C.byref(b)) == 0
+    assert a.value == L.FLOW_TASK_DTYPE.itemsize == 104
+    assert b.value == L.FLOW_PARAMS_DTYPE.itemsize == 64
+
+
+def test_flow2d_rejects_bad_parameters():
+    import numpy as np
+    from hfm import _lib as L
+    prm = np.zeros((), dtype=L.FLOW_PARAMS_DTYPE)
+    prm["h"], prm["N"], prm["nu1"], prm["nu2"], prm["maxit"], prm["max_sub"] = 0.1, 96, 2, 2, 10, 4
+    prm["nlev"] = 4
+    with pytest.raises(L.HfmError, match="not supported"):       # 96 is not a power of two
+        L.call("hfm_flow2d_steps", 1, 1, prm.ctypes.data, 1, 1, 0, None)
+    prm["N"], prm["nlev"] = 256, 2                                # coarsest 128^2 > 1024 cells
+    with pytest.raises(L.HfmError, match="not supported"):
+        L.call("hfm_flow2d_steps", 1, 1, prm.ctypes.data, 1, 1, 0, None)
+    prm["nlev"], prm["maxit"] = 6, 0
+    with pytest.raises(L.HfmError, match="bad parameters"):
+        L.call("hfm_flow2d_steps", 1, 1, prm.ctypes.data, 1, 1, 0, None)
+
+
 def test_error_path_reports_message():
     from hfm import _lib as L
     with pytest.raises(L.HfmError, match="hfm_restrict_mean_f32"):
