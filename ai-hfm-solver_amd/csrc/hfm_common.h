@@ -41,6 +41,31 @@ __device__ __forceinline__ float round_to(float f) {
   return to_f(from_f<T>(f));
 }
 
+// ---------------------------------------------------------------- global pointers
+// Pointers read from work tables are generic to the compiler, which then
+// emits FLAT accesses (they also count against the LDS counter that shuffles
+// and reductions wait on).  Kernels cast them once to the global address space.
+#define HFM_GLOBAL __attribute__((address_space(1)))
+template <typename T>
+__device__ __forceinline__ const HFM_GLOBAL T* gin(const void* p) {
+  return (const HFM_GLOBAL T*)p;
+}
+template <typename T>
+__device__ __forceinline__ HFM_GLOBAL T* gout(void* p) {
+  return (HFM_GLOBAL T*)p;
+}
+// Element access through global pointers (bf16 as its 16-bit pattern).
+__device__ __forceinline__ float ldg(const HFM_GLOBAL float* p) { return *p; }
+__device__ __forceinline__ float ldg(const HFM_GLOBAL bf16* p) {
+  return __uint_as_float(uint32_t(*(const HFM_GLOBAL uint16_t*)p) << 16);
+}
+__device__ __forceinline__ void put(HFM_GLOBAL float* p, float v) { *p = v; }
+__device__ __forceinline__ void put(HFM_GLOBAL float* p, bf16 v) { *p = to_f(v); }
+__device__ __forceinline__ void put(HFM_GLOBAL bf16* p, bf16 v) { *(HFM_GLOBAL uint16_t*)p = v.v; }
+__device__ __forceinline__ void put(HFM_GLOBAL bf16* p, float v) {
+  *(HFM_GLOBAL uint16_t*)p = from_f<bf16>(v).v;
+}
+
 // ---------------------------------------------------------------- division
 // q = n / d for n < 2^32 with mag = floor(2^64 / d) + 1 (mag = 0 means d = 1).
 __device__ __forceinline__ uint32_t fdiv(uint32_t n, uint64_t mag) {

@@ -19,6 +19,14 @@ namespace {
 constexpr int kFT = 1024;      // threads per flow
 constexpr int kFWv = kFT / 64;  // waves per flow
 
+// Field pointers are read from the task table, so the compiler cannot tell
+// they address global memory and would emit FLAT accesses (which also hold
+// the LDS counter that the shuffles and reductions wait on).  Every field is
+// accessed through these global-address-space types instead.
+#define HFM_GLOBAL __attribute__((address_space(1)))
+typedef HFM_GLOBAL float gf;
+typedef HFM_GLOBAL double gd;
+
 // Workgroup sums in a fixed order; every thread gets the result.
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -78,12 +86,38 @@ struct Nb {
   }
 };
 
+// Cells c = threadIdx.x + k kFT of an nn-cell pass, kU at a time: load(c)
+// for each cell of a group before put(c, value) for any of them, so a
+// thread has kU cells' loads in flight instead of one cell's.
+constexpr int kU = 4;
+
+template <typename V, typename Load, typename Put>
+__device__ __forceinline__ void cells(int nn, Load&& load, Put&& put) {
+  for (int c0 = threadIdx.x; c0 < nn; c0 += kU * kFT) {
+    V val[kU];
+#pragma unroll
+    for (int k = 0; k < kU; ++k) {
+      const int c = c0 + k * kFT;
+      if (c < nn) val[k] = load(c);
+    }
+#pragma unroll
+    for (int k = 0; k < kU; ++k) {
+      const int c = c0 + k * kFT;
+      if (c < nn) put(c, val[k]);
+    }
+  }
+}
+
+struct F2 {
+  float a, b;
+};
+
 // ------------------------------------------------------------------ momentum
 // F_u and F_v at cell (j, i): nu/h^2 lap5 - (velocity-signed upwind advection)/h
 // + forcing (u only).  vbar / ubar are the 4-point averages of the other
 // component at the u / v point.
-__device__ __forceinline__ void momentum_rhs(const float* __restrict__ u,
-                                             const float* __restrict__ v, int c, int sh,
+__device__ __forceinline__ void momentum_rhs(const gf* __restrict__ u,
+                                             const gf* __restrict__ v, int c, int sh,
                                              float nu_h2, float inv_h, float f, float& Fu,
                                              float& Fv) {
   const int m = (1 << sh) - 1;
@@ -116,9 +150,9 @@ struct Levels {
 };
 
 // out = x + omega (b - s A x) / (4 s) with x = xin (+ alpha e[parent] when ec).
-__device__ __forceinline__ float sweep_cell(const float* __restrict__ xin,
-                                            const float* __restrict__ ec, float alpha,
-                                            const float* __restrict__ b, int c, int sh, float s,
+__device__ __forceinline__ float sweep_cell(const gf* __restrict__ xin,
+                                            const gf* __restrict__ ec, float alpha,
+                                            const gf* __restrict__ b, int c, int sh, float s,
                                             float omega) {
   const Nb q(c, sh);
   float xc = xin[q.c], xe = xin[q.e], xw = xin[q.w], xn = xin[q.n], xs = xin[q.s];
@@ -135,8 +169,8 @@ __device__ __forceinline__ float sweep_cell(const float* __restrict__ xin,
   return xc + omega * (b[c] - Ax) / (4.0f * s);
 }
 
-__device__ __forceinline__ float resid_cell(const float* __restrict__ x,
-                                            const float* __restrict__ b, int c, int sh,
+__device__ __forceinline__ float resid_cell(const gf* __restrict__ x,
+                                            const gf* __restrict__ b, int c, int sh,
                                             float s) {
   const Nb q(c, sh);
   const float Ax = s * (4.0f * x[q.c] - ((x[q.e] + x[q.w]) + (x[q.n] + x[q.s])));
@@ -154,38 +188,43 @@ __device__ double vcycle(const hfm_flow2d_task& T, const hfm_flow2d_params& P,
   for (int l = 0; l < L.nlev - 1; ++l) {
     const int sh = L.sh - l, nn = 1 << (2 * sh), o = L.off(l);
     const float s = ldexpf(1.0f, -l), d = 4.0f * s;
-    float* buf[2] = {T.mgx + o, T.mgy + o};
-    const float* b = T.mgb + o;
+    gf* buf[2] = {(gf*)T.mgx + o, (gf*)T.mgy + o};
+    const gf* b = (const gf*)T.mgb + o;
     if (l > 0) {
-      for (int c = tid; c < nn; c += kFT) buf[0][c] = omega * b[c] / d;
+      gf* x0 = buf[0];
+      cells<float>(nn, [&](int c) { return omega * b[c] / d; }, [&](int c, float y) { x0[c] = y; });
       __syncthreads();
     }
     for (int k = 1; k < nu1; ++k) {
-      const float* xin = buf[(k - 1) & 1];
-      float* out = buf[k & 1];
-      for (int c = tid; c < nn; c += kFT) out[c] = sweep_cell(xin, nullptr, 0.0f, b, c, sh, s, omega);
+      const gf* xin = buf[(k - 1) & 1];
+      gf* out = buf[k & 1];
+      cells<float>(nn, [&](int c) { return sweep_cell(xin, nullptr, 0.0f, b, c, sh, s, omega); },
+                   [&](int c, float y) { out[c] = y; });
       __syncthreads();
     }
     // bc = mean over the 2x2 children of (b - A x)
-    const float* x = buf[(nu1 - 1) & 1];
+    const gf* x = buf[(nu1 - 1) & 1];
     const int shc = sh - 1, ncc = 1 << (2 * shc), mc = (1 << shc) - 1;
-    float* bc = T.mgb + L.off(l + 1);
-    for (int C = tid; C < ncc; C += kFT) {
-      const int J = C >> shc, I = C & mc;
-      const int f00 = ((2 * J) << sh) + 2 * I, f10 = f00 + (1 << sh);
-      const float r00 = resid_cell(x, b, f00, sh, s), r01 = resid_cell(x, b, f00 + 1, sh, s);
-      const float r10 = resid_cell(x, b, f10, sh, s), r11 = resid_cell(x, b, f10 + 1, sh, s);
-      bc[C] = 0.25f * ((r00 + r01) + (r10 + r11));
-    }
+    gf* bc = (gf*)T.mgb + L.off(l + 1);
+    cells<float>(
+        ncc,
+        [&](int C) {
+          const int J = C >> shc, I = C & mc;
+          const int f00 = ((2 * J) << sh) + 2 * I, f10 = f00 + (1 << sh);
+          const float r00 = resid_cell(x, b, f00, sh, s), r01 = resid_cell(x, b, f00 + 1, sh, s);
+          const float r10 = resid_cell(x, b, f10, sh, s), r11 = resid_cell(x, b, f10 + 1, sh, s);
+          return 0.25f * ((r00 + r01) + (r10 + r11));
+        },
+        [&](int C, float y) { bc[C] = y; });
     __syncthreads();
   }
   // coarsest: x = pinv b (FP64)
   {
     const int l = L.nlev - 1, nn = 1 << (2 * (L.sh - l)), o = L.off(l);
-    const float* b = T.mgb + o;
-    float* x = T.mgx + o;
+    const gf* b = (const gf*)T.mgb + o;
+    gf* x = (gf*)T.mgx + o;
     for (int r = tid; r < nn; r += kFT) {
-      const double* row = pinv + int64_t(r) * nn;
+      const gd* row = (const gd*)pinv + int64_t(r) * nn;
       double acc = 0.0;
       for (int c = 0; c < nn; ++c) acc += row[c] * double(b[c]);
       x[r] = float(acc);
@@ -197,19 +236,25 @@ __device__ double vcycle(const hfm_flow2d_task& T, const hfm_flow2d_params& P,
   for (int l = L.nlev - 2; l >= 0; --l) {
     const int sh = L.sh - l, nn = 1 << (2 * sh), o = L.off(l), oc = L.off(l + 1);
     const float s = ldexpf(1.0f, -l);
-    float* buf[2] = {T.mgx + o, T.mgy + o};
-    const float* b = T.mgb + o;
-    const float* ec = (l + 1 == L.nlev - 1) ? T.mgx + oc
-                                            : ((nu1 + nu2 - 1) & 1 ? T.mgy : T.mgx) + oc;
+    gf* buf[2] = {(gf*)T.mgx + o, (gf*)T.mgy + o};
+    const gf* b = (const gf*)T.mgb + o;
+    const gf* ec = (const gf*)((l + 1 == L.nlev - 1) ? T.mgx + oc
+                                                     : ((nu1 + nu2 - 1) & 1 ? T.mgy : T.mgx) + oc);
     for (int k = 0; k < nu2; ++k) {
       const int K = nu1 + k;
-      const float* xin = buf[(K - 1) & 1];
+      const gf* xin = buf[(K - 1) & 1];
       const bool last0 = (l == 0 && k == nu2 - 1);
-      float* out = last0 ? T.z : buf[K & 1];
-      for (int c = tid; c < nn; c += kFT) {
-        const float y = sweep_cell(xin, k == 0 ? ec : nullptr, alpha, b, c, sh, s, omega);
-        out[c] = y;
-        if (last0) rz += double(b[c]) * double(y);
+      gf* out = last0 ? (gf*)T.z : buf[K & 1];
+      const gf* e = k == 0 ? ec : nullptr;
+      if (last0) {
+        cells<F2>(nn, [&](int c) { return F2{sweep_cell(xin, e, alpha, b, c, sh, s, omega), b[c]}; },
+                  [&](int c, F2 y) {
+                    out[c] = y.a;
+                    rz += double(y.b) * double(y.a);
+                  });
+      } else {
+        cells<float>(nn, [&](int c) { return sweep_cell(xin, e, alpha, b, c, sh, s, omega); },
+                     [&](int c, float y) { out[c] = y; });
       }
       __syncthreads();
     }
@@ -219,7 +264,7 @@ __device__ double vcycle(const hfm_flow2d_task& T, const hfm_flow2d_params& P,
 
 // A x for an FP64 or FP32 field, evaluated in FP64.
 template <typename X>
-__device__ __forceinline__ double apply64(const X* __restrict__ x, const Nb& q) {
+__device__ __forceinline__ double apply64(const HFM_GLOBAL X* __restrict__ x, const Nb& q) {
   return 4.0 * double(x[q.c]) - ((double(x[q.e]) + double(x[q.w])) + (double(x[q.n]) + double(x[q.s])));
 }
 
@@ -230,48 +275,79 @@ __device__ int substep(const hfm_flow2d_task& T, const hfm_flow2d_params& P,
                        double* red) {
   const int tid = threadIdx.x;
   const int sh = L.sh, NN = 1 << (2 * sh), m = (1 << sh) - 1;
-  float* __restrict__ u = T.u;
-  float* __restrict__ v = T.v;
+  gf* __restrict__ u = (gf*)T.u;
+  gf* __restrict__ v = (gf*)T.v;
+  const gf* __restrict__ fu = (const gf*)T.fu;
+  gf* __restrict__ uh = (gf*)T.uh;
+  gf* __restrict__ vh = (gf*)T.vh;
+  gd* __restrict__ phi = (gd*)T.phi;
+  gd* __restrict__ r = (gd*)T.r;
+  gf* __restrict__ mgb = (gf*)T.mgb;
+  gf* __restrict__ mgx = (gf*)T.mgx;
+  gf* __restrict__ p = (gf*)T.p;
+  const gf* __restrict__ z = (const gf*)T.z;
   if (momentum) {
     const float hdt = 0.5f * dt;
-    for (int c = tid; c < NN; c += kFT) {
-      float Fu, Fv;
-      momentum_rhs(u, v, c, sh, P.nu_h2, P.inv_h, T.fu[c >> sh], Fu, Fv);
-      T.uh[c] = u[c] + hdt * Fu;
-      T.vh[c] = v[c] + hdt * Fv;
-    }
+    cells<F2>(
+        NN,
+        [&](int c) {
+          F2 F;
+          momentum_rhs(u, v, c, sh, P.nu_h2, P.inv_h, fu[c >> sh], F.a, F.b);
+          return F2{u[c] + hdt * F.a, v[c] + hdt * F.b};
+        },
+        [&](int c, F2 y) {
+          uh[c] = y.a;
+          vh[c] = y.b;
+        });
     __syncthreads();
-    for (int c = tid; c < NN; c += kFT) {  // u* in place: only u[c] and uh/vh are read
-      float Fu, Fv;
-      momentum_rhs(T.uh, T.vh, c, sh, P.nu_h2, P.inv_h, T.fu[c >> sh], Fu, Fv);
-      u[c] = u[c] + dt * Fu;
-      v[c] = v[c] + dt * Fv;
-    }
+    // u* in place: a cell reads only its own u, v (before any store of its group)
+    cells<F2>(
+        NN,
+        [&](int c) {
+          F2 F;
+          momentum_rhs(uh, vh, c, sh, P.nu_h2, P.inv_h, fu[c >> sh], F.a, F.b);
+          return F2{u[c] + dt * F.a, v[c] + dt * F.b};
+        },
+        [&](int c, F2 y) {
+          u[c] = y.a;
+          v[c] = y.b;
+        });
     __syncthreads();
   }
   // divergence source b (into r), its mean
   double sb = 0.0;
-  for (int c = tid; c < NN; c += kFT) {
-    const int j = c >> sh, i = c & m;
-    const float bf = P.h * ((u[c] - u[(j << sh) + ((i + 1) & m)]) + (v[c] - v[(((j + 1) & m) << sh) + i]));
-    T.r[c] = double(bf);
-    sb += double(bf);
-  }
+  cells<float>(
+      NN,
+      [&](int c) {
+        const int j = c >> sh, i = c & m;
+        return P.h * ((u[c] - u[(j << sh) + ((i + 1) & m)]) + (v[c] - v[(((j + 1) & m) << sh) + i]));
+      },
+      [&](int c, float bf) {
+        r[c] = double(bf);
+        sb += double(bf);
+      });
   const double mean = wg_sum(sb, red) / double(NN);
   // r = (b - mean) - A phi; rl = f32(r); first level-0 sweep
   double bb = 0.0, rr = 0.0;
   const float w4 = P.omega;
-  for (int c = tid; c < NN; c += kFT) {
-    const Nb q(c, sh);
-    const double bm = T.r[c] - mean;
-    const double rc = bm - apply64(T.phi, q);
-    T.r[c] = rc;
-    const float rl = float(rc);
-    T.mgb[c] = rl;
-    T.mgx[c] = w4 * rl / 4.0f;
-    bb += bm * bm;
-    rr += rc * rc;
-  }
+  struct D2 {
+    double a, b;
+  };
+  cells<D2>(
+      NN,
+      [&](int c) {
+        const Nb q(c, sh);
+        const double bm = r[c] - mean;
+        return D2{bm, bm - apply64(phi, q)};
+      },
+      [&](int c, D2 y) {
+        r[c] = y.b;
+        const float rl = float(y.b);
+        mgb[c] = rl;
+        mgx[c] = w4 * rl / 4.0f;
+        bb += y.a * y.a;
+        rr += y.b * y.b;
+      });
   wg_sum2(bb, rr, red);
   int status = HFM_MG_RUNNING;
   int it = 0;
@@ -287,14 +363,16 @@ __device__ int substep(const hfm_flow2d_task& T, const hfm_flow2d_params& P,
     const double beta = it == 0 ? 0.0 : rz / rz_old;
     rz_old = rz;
     const double bd = double(float(beta));
-    for (int c = tid; c < NN; c += kFT)
-      T.p[c] = it == 0 ? T.z[c] : float(bd * double(T.p[c]) + double(T.z[c]));
+    if (it == 0) {
+      cells<float>(NN, [&](int c) { return z[c]; }, [&](int c, float y) { p[c] = y; });
+    } else {
+      cells<float>(NN, [&](int c) { return float(bd * double(p[c]) + double(z[c])); },
+                   [&](int c, float y) { p[c] = y; });
+    }
     __syncthreads();
     double pq = 0.0;
-    for (int c = tid; c < NN; c += kFT) {
-      const Nb q(c, sh);
-      pq += double(T.p[c]) * apply64(T.p, q);
-    }
+    cells<double>(NN, [&](int c) { return double(p[c]) * apply64(p, Nb(c, sh)); },
+                  [&](int, double y) { pq += y; });
     pq = wg_sum(pq, red);
     if (!(pq > 0.0) || !isfinite(pq)) {
       status = HFM_MG_BREAKDOWN;
@@ -302,17 +380,21 @@ __device__ int substep(const hfm_flow2d_task& T, const hfm_flow2d_params& P,
     }
     const double a = rz / pq;
     rr = 0.0;
-    for (int c = tid; c < NN; c += kFT) {
-      const Nb q(c, sh);
-      const double qv = apply64(T.p, q);
-      T.phi[c] += a * double(T.p[c]);
-      const double rc = T.r[c] - a * qv;
-      T.r[c] = rc;
-      const float rl = float(rc);
-      T.mgb[c] = rl;
-      T.mgx[c] = w4 * rl / 4.0f;
-      rr += rc * rc;
-    }
+    cells<D2>(
+        NN,
+        [&](int c) {
+          const Nb q(c, sh);
+          const double pc = double(p[c]);
+          return D2{phi[c] + a * pc, r[c] - a * apply64(p, q)};
+        },
+        [&](int c, D2 y) {
+          phi[c] = y.a;
+          r[c] = y.b;
+          const float rl = float(y.b);
+          mgb[c] = rl;
+          mgx[c] = w4 * rl / 4.0f;
+          rr += y.b * y.b;
+        });
     rr = wg_sum(rr, red);
     ++it;
     rel = sqrt(rr / bb);
@@ -321,12 +403,17 @@ __device__ int substep(const hfm_flow2d_task& T, const hfm_flow2d_params& P,
   }
   // gradient correction (phi final: the loop ended on a barrier)
   const double ih = double(P.inv_h);
-  for (int c = tid; c < NN; c += kFT) {
-    const Nb q(c, sh);
-    const double ph = T.phi[c];
-    u[c] = float(double(u[c]) - (ph - T.phi[q.w]) * ih);
-    v[c] = float(double(v[c]) - (ph - T.phi[q.s]) * ih);
-  }
+  cells<F2>(
+      NN,
+      [&](int c) {
+        const Nb q(c, sh);
+        const double ph = phi[c];
+        return F2{float(double(u[c]) - (ph - phi[q.w]) * ih), float(double(v[c]) - (ph - phi[q.s]) * ih)};
+      },
+      [&](int c, F2 y) {
+        u[c] = y.a;
+        v[c] = y.b;
+      });
   __syncthreads();
   if (tid == 0) {
     double* st = T.stats;
@@ -361,10 +448,13 @@ __global__ __launch_bounds__(kFT) void k_flow2d(const hfm_flow2d_task* __restric
   }
   for (int step = 0; step < nsteps; ++step) {
     float umax = 0.0f, vmax = 0.0f;
-    for (int c = threadIdx.x; c < NN; c += kFT) {
-      umax = fmaxf(umax, fabsf(T.u[c]));
-      vmax = fmaxf(vmax, fabsf(T.v[c]));
-    }
+    const gf* u = (const gf*)T.u;
+    const gf* v = (const gf*)T.v;
+    cells<F2>(NN, [&](int c) { return F2{fabsf(u[c]), fabsf(v[c])}; },
+              [&](int, F2 a) {
+                umax = fmaxf(umax, a.a);
+                vmax = fmaxf(vmax, a.b);
+              });
     umax = wg_max(umax, redf);
     vmax = wg_max(vmax, redf);
     const float q = (umax + vmax) * P.cfl_coef;

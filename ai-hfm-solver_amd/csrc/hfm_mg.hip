@@ -25,18 +25,27 @@ constexpr int kCoarseChunk = kBlock * 4;
 
 template <typename C>
 __device__ __forceinline__ float ldc(const void* p, int64_t i) {
-  return to_f(static_cast<const C*>(p)[i]);
+  return ldg(gin<C>(p) + i);
+}
+template <typename C>
+__device__ __forceinline__ float ldc(const HFM_GLOBAL C* p, int64_t i) {
+  return ldg(p + i);
 }
 
 // ------------------------------------------------------------------ 8-wide row access
-__device__ __forceinline__ void ld8(const float* p, float v[8]) {
-  const float4 a = reinterpret_cast<const float4*>(p)[0];
-  const float4 b = reinterpret_cast<const float4*>(p)[1];
+// Native vector types (16- and 8-byte accesses through global pointers).
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef double d2v __attribute__((ext_vector_type(2)));
+typedef unsigned int u4v __attribute__((ext_vector_type(4)));
+typedef unsigned int u2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void ld8(const HFM_GLOBAL float* p, float v[8]) {
+  const f4v a = ((const HFM_GLOBAL f4v*)p)[0];
+  const f4v b = ((const HFM_GLOBAL f4v*)p)[1];
   v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
   v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
 }
-__device__ __forceinline__ void ld8(const bf16* p, float v[8]) {
-  const uint4 u = *reinterpret_cast<const uint4*>(p);
+__device__ __forceinline__ void ld8(const HFM_GLOBAL bf16* p, float v[8]) {
+  const u4v u = *(const HFM_GLOBAL u4v*)p;
   const uint32_t w[4] = {u.x, u.y, u.z, u.w};
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
@@ -44,59 +53,55 @@ __device__ __forceinline__ void ld8(const bf16* p, float v[8]) {
     v[2 * q + 1] = __uint_as_float(w[q] & 0xFFFF0000u);
   }
 }
-__device__ __forceinline__ void ld4(const float* p, float v[4]) {
-  const float4 a = *reinterpret_cast<const float4*>(p);
+__device__ __forceinline__ void ld4(const HFM_GLOBAL float* p, float v[4]) {
+  const f4v a = *(const HFM_GLOBAL f4v*)p;
   v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
 }
-__device__ __forceinline__ void ld4(const bf16* p, float v[4]) {
-  const uint2 u = *reinterpret_cast<const uint2*>(p);
+__device__ __forceinline__ void ld4(const HFM_GLOBAL bf16* p, float v[4]) {
+  const u2v u = *(const HFM_GLOBAL u2v*)p;
   v[0] = __uint_as_float(u.x << 16); v[1] = __uint_as_float(u.x & 0xFFFF0000u);
   v[2] = __uint_as_float(u.y << 16); v[3] = __uint_as_float(u.y & 0xFFFF0000u);
 }
-__device__ __forceinline__ void st8(float* p, const float v[8]) {
-  reinterpret_cast<float4*>(p)[0] = make_float4(v[0], v[1], v[2], v[3]);
-  reinterpret_cast<float4*>(p)[1] = make_float4(v[4], v[5], v[6], v[7]);
+__device__ __forceinline__ void st8(HFM_GLOBAL float* p, const float v[8]) {
+  ((HFM_GLOBAL f4v*)p)[0] = f4v{v[0], v[1], v[2], v[3]};
+  ((HFM_GLOBAL f4v*)p)[1] = f4v{v[4], v[5], v[6], v[7]};
 }
 __device__ __forceinline__ uint32_t pack2(float a, float b) {
   return uint32_t(from_f<bf16>(a).v) | (uint32_t(from_f<bf16>(b).v) << 16);
 }
-__device__ __forceinline__ void st8(bf16* p, const float v[8]) {
-  *reinterpret_cast<uint4*>(p) =
-      make_uint4(pack2(v[0], v[1]), pack2(v[2], v[3]), pack2(v[4], v[5]), pack2(v[6], v[7]));
+__device__ __forceinline__ void st8(HFM_GLOBAL bf16* p, const float v[8]) {
+  *(HFM_GLOBAL u4v*)p = u4v{pack2(v[0], v[1]), pack2(v[2], v[3]), pack2(v[4], v[5]), pack2(v[6], v[7])};
 }
-__device__ __forceinline__ void st4(float* p, const float v[4]) {
-  *reinterpret_cast<float4*>(p) = make_float4(v[0], v[1], v[2], v[3]);
+__device__ __forceinline__ void st4(HFM_GLOBAL float* p, const float v[4]) {
+  *(HFM_GLOBAL f4v*)p = f4v{v[0], v[1], v[2], v[3]};
 }
-__device__ __forceinline__ void st4(bf16* p, const float v[4]) {
-  *reinterpret_cast<uint2*>(p) = make_uint2(pack2(v[0], v[1]), pack2(v[2], v[3]));
+__device__ __forceinline__ void st4(HFM_GLOBAL bf16* p, const float v[4]) {
+  *(HFM_GLOBAL u2v*)p = u2v{pack2(v[0], v[1]), pack2(v[2], v[3])};
 }
-__device__ __forceinline__ void ld8d(const double* p, double v[8]) {
+__device__ __forceinline__ void ld8d(const HFM_GLOBAL double* p, double v[8]) {
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
-    const double2 a = reinterpret_cast<const double2*>(p)[q];
+    const d2v a = ((const HFM_GLOBAL d2v*)p)[q];
     v[2 * q] = a.x;
     v[2 * q + 1] = a.y;
   }
 }
-__device__ __forceinline__ void st8d(double* p, const double v[8]) {
+__device__ __forceinline__ void st8d(HFM_GLOBAL double* p, const double v[8]) {
 #pragma unroll
-  for (int q = 0; q < 4; ++q) reinterpret_cast<double2*>(p)[q] = make_double2(v[2 * q], v[2 * q + 1]);
+  for (int q = 0; q < 4; ++q) ((HFM_GLOBAL d2v*)p)[q] = d2v{v[2 * q], v[2 * q + 1]};
 }
 
 // Streaming (non-temporal) variants for arrays read or written exactly once
 // per launch: they are not kept in the caches that serve the stencil reuse.
-typedef float f4v __attribute__((ext_vector_type(4)));
-typedef double d2v __attribute__((ext_vector_type(2)));
-typedef unsigned int u4v __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ void ld8_nt(const float* p, float v[8]) {
-  const f4v a = __builtin_nontemporal_load(reinterpret_cast<const f4v*>(p));
-  const f4v b = __builtin_nontemporal_load(reinterpret_cast<const f4v*>(p) + 1);
+__device__ __forceinline__ void ld8_nt(const HFM_GLOBAL float* p, float v[8]) {
+  const f4v a = __builtin_nontemporal_load((const HFM_GLOBAL f4v*)p);
+  const f4v b = __builtin_nontemporal_load((const HFM_GLOBAL f4v*)p + 1);
   v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
   v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
 }
-__device__ __forceinline__ void ld8_nt(const bf16* p, float v[8]) {
-  const u4v u = __builtin_nontemporal_load(reinterpret_cast<const u4v*>(p));
+__device__ __forceinline__ void ld8_nt(const HFM_GLOBAL bf16* p, float v[8]) {
+  const u4v u = __builtin_nontemporal_load((const HFM_GLOBAL u4v*)p);
   const uint32_t w[4] = {u.x, u.y, u.z, u.w};
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
@@ -104,27 +109,27 @@ __device__ __forceinline__ void ld8_nt(const bf16* p, float v[8]) {
     v[2 * q + 1] = __uint_as_float(w[q] & 0xFFFF0000u);
   }
 }
-__device__ __forceinline__ void st8_nt(float* p, const float v[8]) {
-  __builtin_nontemporal_store(f4v{v[0], v[1], v[2], v[3]}, reinterpret_cast<f4v*>(p));
-  __builtin_nontemporal_store(f4v{v[4], v[5], v[6], v[7]}, reinterpret_cast<f4v*>(p) + 1);
+__device__ __forceinline__ void st8_nt(HFM_GLOBAL float* p, const float v[8]) {
+  __builtin_nontemporal_store(f4v{v[0], v[1], v[2], v[3]}, (HFM_GLOBAL f4v*)p);
+  __builtin_nontemporal_store(f4v{v[4], v[5], v[6], v[7]}, (HFM_GLOBAL f4v*)p + 1);
 }
-__device__ __forceinline__ void st8_nt(bf16* p, const float v[8]) {
+__device__ __forceinline__ void st8_nt(HFM_GLOBAL bf16* p, const float v[8]) {
   __builtin_nontemporal_store(u4v{pack2(v[0], v[1]), pack2(v[2], v[3]), pack2(v[4], v[5]),
                                   pack2(v[6], v[7])},
-                              reinterpret_cast<u4v*>(p));
+                              (HFM_GLOBAL u4v*)p);
 }
-__device__ __forceinline__ void ld8d_nt(const double* p, double v[8]) {
+__device__ __forceinline__ void ld8d_nt(const HFM_GLOBAL double* p, double v[8]) {
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
-    const d2v a = __builtin_nontemporal_load(reinterpret_cast<const d2v*>(p) + q);
+    const d2v a = __builtin_nontemporal_load((const HFM_GLOBAL d2v*)p + q);
     v[2 * q] = a.x;
     v[2 * q + 1] = a.y;
   }
 }
-__device__ __forceinline__ void st8d_nt(double* p, const double v[8]) {
+__device__ __forceinline__ void st8d_nt(HFM_GLOBAL double* p, const double v[8]) {
 #pragma unroll
   for (int q = 0; q < 4; ++q)
-    __builtin_nontemporal_store(d2v{v[2 * q], v[2 * q + 1]}, reinterpret_cast<d2v*>(p) + q);
+    __builtin_nontemporal_store(d2v{v[2 * q], v[2 * q + 1]}, (HFM_GLOBAL d2v*)p + q);
 }
 
 // ------------------------------------------------------------------ scalar stencil row
@@ -174,9 +179,9 @@ template <typename C>
 struct RunCoef {
   float cw[8], ce[8], cs[8], cn[8], cd[8], cu[8];
   __device__ __forceinline__ void load(const hfm_mg_task& t, const Run& r, bool valid) {
-    const C* cx = static_cast<const C*>(t.c[0]);
-    const C* cy = static_cast<const C*>(t.c[1]);
-    const C* cz = static_cast<const C*>(t.c[2]);
+    const HFM_GLOBAL C* cx = gin<C>(t.c[0]);
+    const HFM_GLOBAL C* cy = gin<C>(t.c[1]);
+    const HFM_GLOBAL C* cz = gin<C>(t.c[2]);
     const int lane = threadIdx.x & 63;
 #pragma unroll
     for (int m = 0; m < 8; ++m) cw[m] = cs[m] = cn[m] = cd[m] = cu[m] = 0.0f;
@@ -187,10 +192,10 @@ struct RunCoef {
       else if (lane == 63) eface = ldc<C>(cx, r.n0 + kRun);
       ld8(cy + r.n0, cs);
       if (r.j + 1 < t.H) ld8(cy + r.n0 + t.W, cn);
-      else ld8(static_cast<const C*>(t.c[4]) + r.k * t.W + r.i0, cn);
+      else ld8(gin<C>(t.c[4]) + r.k * t.W + r.i0, cn);
       ld8(cz + r.n0, cd);
       if (r.k + 1 < t.D) ld8(cz + r.n0 + t.H * t.W, cu);
-      else ld8(static_cast<const C*>(t.c[5]) + r.j * t.W + r.i0, cu);
+      else ld8(gin<C>(t.c[5]) + r.j * t.W + r.i0, cu);
     }
 #pragma unroll
     for (int m = 0; m < 7; ++m) ce[m] = cw[m + 1];
@@ -208,7 +213,7 @@ struct RunCoef {
 // row-aligned position, one(n, k, j, i) a single value.
 template <typename T>
 struct SrcPlain {
-  const T* x;
+  const HFM_GLOBAL T* x;
   template <typename A>
   __device__ __forceinline__ void run(int n, int, int, int, A v[8]) const {
     float f[8];
@@ -216,14 +221,14 @@ struct SrcPlain {
 #pragma unroll
     for (int m = 0; m < 8; ++m) v[m] = (A)f[m];
   }
-  __device__ __forceinline__ float one(int n, int, int, int) const { return to_f(x[n]); }
+  __device__ __forceinline__ float one(int n, int, int, int) const { return ldg(x + n); }
 };
 
 // x := xin + alpha * P ec (nearest prolongation of the coarse correction)
 template <typename T>
 struct SrcCorrect {
-  const T* x;
-  const T* ec;
+  const HFM_GLOBAL T* x;
+  const HFM_GLOBAL T* ec;
   int Hc, Wc;
   float alpha;
   template <typename A>
@@ -235,15 +240,15 @@ struct SrcCorrect {
     for (int m = 0; m < 8; ++m) v[m] = (A)(f[m] + alpha * e[m >> 1]);
   }
   __device__ __forceinline__ float one(int n, int k, int j, int i) const {
-    return to_f(x[n]) + alpha * to_f(ec[((k >> 1) * Hc + (j >> 1)) * Wc + (i >> 1)]);
+    return ldg(x + n) + alpha * ldg(ec + ((k >> 1) * Hc + (j >> 1)) * Wc + (i >> 1));
   }
 };
 
 // p := T(z + beta p_old) (one fused multiply-add in FP32; p_old unread when beta == 0)
 template <typename T>
 struct SrcDir {
-  const T* z;
-  const T* pold;
+  const HFM_GLOBAL T* z;
+  const HFM_GLOBAL T* pold;
   float beta;
   __device__ __forceinline__ float make(float zv, float pv) const {
     return round_to<T>(beta == 0.0f ? zv : fmaf(beta, pv, zv));
@@ -257,7 +262,7 @@ struct SrcDir {
     for (int m = 0; m < 8; ++m) v[m] = (A)make(zf[m], beta != 0.0f ? pf[m] : 0.0f);
   }
   __device__ __forceinline__ float one(int n, int, int, int) const {
-    return make(to_f(z[n]), beta != 0.0f ? to_f(pold[n]) : 0.0f);
+    return make(ldg(z + n), beta != 0.0f ? ldg(pold + n) : 0.0f);
   }
 };
 
@@ -344,10 +349,10 @@ __global__ __launch_bounds__(kBlock) void k_sweep(const hfm_mg_task* __restrict_
   const int chunk = xcd_chunk(g - t.blk0, t.nblk);
   const int N = t.D * t.H * t.W;
   const int base = chunk * kCellChunk;
-  const T* xin = static_cast<const T*>(t.xin);
-  const T* b = static_cast<const T*>(t.b);
-  const T* ec = static_cast<const T*>(t.ec);
-  O* out = static_cast<O*>(t.xout);
+  const HFM_GLOBAL T* xin = gin<T>(t.xin);
+  const HFM_GLOBAL T* b = gin<T>(t.b);
+  const HFM_GLOBAL T* ec = gin<T>(t.ec);
+  HFM_GLOBAL O* out = gout<O>(t.xout);
   double acc = 0.0;
   if (vec_task(t)) {
     const int n0 = base + threadIdx.x * kRun;
@@ -384,22 +389,22 @@ __global__ __launch_bounds__(kBlock) void k_sweep(const hfm_mg_task* __restrict_
       if (n >= N) break;
       int i, j, k;
       decompose(t, n, i, j, k);
-      const float bn = to_f(b[n]);
+      const float bn = ldg(b + n);
       float r;
       if (MODE == HFM_SWEEP_ZERO) {
         float diag;
         face_row<float, T>(t, n, i, j, k, [](int, int, int, int) { return 0.0f; }, diag);
         r = omega * bn / diag;
       } else if (MODE == HFM_SWEEP_PLAIN) {
-        auto xat = [&](int m, int, int, int) { return to_f(xin[m]); };
+        auto xat = [&](int m, int, int, int) { return ldg(xin + m); };
         float diag;
         const float off = face_row<float, T>(t, n, i, j, k, xat, diag);
-        const float xc = to_f(xin[n]);
+        const float xc = ldg(xin + n);
         r = xc + omega * (bn - (diag * xc - off)) / diag;
       } else {
         const int Hc = t.Hc, Wc = t.Wc;
         auto xat = [&](int m, int kk, int jj, int ii) {
-          return to_f(xin[m]) + alpha * to_f(ec[((kk >> 1) * Hc + (jj >> 1)) * Wc + (ii >> 1)]);
+          return ldg(xin + m) + alpha * ldg(ec + ((kk >> 1) * Hc + (jj >> 1)) * Wc + (ii >> 1));
         };
         float diag;
         const float off = face_row<float, T>(t, n, i, j, k, xat, diag);
@@ -407,7 +412,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep(const hfm_mg_task* __restrict_
         r = xc + omega * (bn - (diag * xc - off)) / diag;
       }
       const O rs = from_f<O>(r);
-      out[n] = rs;
+      put(out + n, rs);
       if (DOT) acc += double(bn) * double(to_f(rs));
     }
   }
@@ -431,9 +436,9 @@ __global__ __launch_bounds__(kBlock) void k_resid_restrict(const hfm_mg_task* __
   const int chunk = xcd_chunk(g - t.blk0, t.nblk);
   const int Nc = t.Dc * t.Hc * t.Wc;
   const int base = chunk * kCoarseChunk;
-  const T* x = static_cast<const T*>(t.xin);
-  const T* b = static_cast<const T*>(t.b);
-  T* bc = static_cast<T*>(t.xout);
+  const HFM_GLOBAL T* x = gin<T>(t.xin);
+  const HFM_GLOBAL T* b = gin<T>(t.b);
+  HFM_GLOBAL T* bc = gout<T>(t.xout);
   const int W = t.W, H = t.H, D = t.D;
   if (vec_task(t)) {
     const int nc0 = base + threadIdx.x * 4;
@@ -479,7 +484,7 @@ __global__ __launch_bounds__(kBlock) void k_resid_restrict(const hfm_mg_task* __
     return;
   }
   if (base >= Nc) return;
-  auto xat = [&](int m, int, int, int) { return to_f(x[m]); };
+  auto xat = [&](int m, int, int, int) { return ldg(x + m); };
   for (int s = 0; s < 4; ++s) {
     const int nc = base + s * kBlock + threadIdx.x;
     if (nc >= Nc) break;
@@ -500,11 +505,11 @@ __global__ __launch_bounds__(kBlock) void k_resid_restrict(const hfm_mg_task* __
           const int n = (k * H + j) * W + i;
           float diag;
           const float off = face_row<float, T>(t, n, i, j, k, xat, diag);
-          sum += to_f(b[n]) - (diag * to_f(x[n]) - off);
+          sum += ldg(b + n) - (diag * ldg(x + n) - off);
         }
       }
     }
-    bc[nc] = from_f<T>(sum * 0.125f);
+    put(bc + nc, from_f<T>(sum * 0.125f));
   }
 }
 
@@ -516,16 +521,16 @@ __global__ __launch_bounds__(64) void k_coarse_solve(const hfm_mg_task* __restri
   if (!sc[t.inst].active) return;
   const int N = t.D * t.H * t.W;
   if (N > 64) return;  // host rejects such levels; never index past the LDS tile
-  const double* ainv = static_cast<const double*>(t.ec);
-  const T* b = static_cast<const T*>(t.b);
+  const HFM_GLOBAL double* ainv = gin<double>(t.ec);
+  const HFM_GLOBAL T* b = gin<T>(t.b);
   __shared__ double bs[64];
-  if (threadIdx.x < N) bs[threadIdx.x] = double(to_f(b[threadIdx.x]));
+  if (threadIdx.x < N) bs[threadIdx.x] = double(ldg(b + threadIdx.x));
   __syncthreads();
   const int i = threadIdx.x;
   if (i < N) {
     double e = 0.0;
     for (int j = 0; j < N; ++j) e += ainv[j * N + i] * bs[j];
-    static_cast<T*>(t.xout)[i] = from_f<T>(float(e));
+    put(gout<T>(t.xout) + i, from_f<T>(float(e)));
   }
 }
 
@@ -544,21 +549,24 @@ __global__ __launch_bounds__(kBlock) void k_fcg_init(const hfm_mg_task* __restri
   const int N = t.D * t.H * t.W;
   const int base = chunk * kCellChunk;
   double acc = 0.0;
-  T* rlow = static_cast<T*>(t.rlow);
-  T* x0 = static_cast<T*>(t.xout);
+  HFM_GLOBAL T* rlow = gout<T>(t.rlow);
+  HFM_GLOBAL T* x0 = gout<T>(t.xout);
+  const HFM_GLOBAL float* b32 = gin<float>(t.b32);
+  HFM_GLOBAL double* x64 = gout<double>(t.x64);
+  HFM_GLOBAL double* r64 = gout<double>(t.r64);
   for (int s = 0; s < kRun; ++s) {
     const int n = base + s * kBlock + threadIdx.x;
     if (n >= N) break;
-    const double bv = double(t.b32[n]);
-    t.x64[n] = 0.0;
-    t.r64[n] = bv;
+    const double bv = double(b32[n]);
+    x64[n] = 0.0;
+    r64[n] = bv;
     const T rt = from_f<T>(float(bv));
-    rlow[n] = rt;
+    put(rlow + n, rt);
     acc += bv * bv;
     if (SWEEP) {
       int i, j, k;
       decompose(t, n, i, j, k);
-      x0[n] = from_f<T>(omega * to_f(rt) / diag_rounded<T>(t, n, i, j, k));
+      put(x0 + n, from_f<T>(omega * to_f(rt) / diag_rounded<T>(t, n, i, j, k)));
     }
   }
   const double s = block_sum(acc);
@@ -580,8 +588,8 @@ __global__ __launch_bounds__(kBlock) void k_fcg_dir(const hfm_mg_task* __restric
   const int chunk = xcd_chunk(g - t.blk0, t.nblk);
   const int N = t.D * t.H * t.W;
   const int base = chunk * kCellChunk;
-  const SrcDir<T> src{static_cast<const T*>(t.b), static_cast<const T*>(t.xin), float(S.beta)};
-  T* pnew = static_cast<T*>(t.xout);
+  const SrcDir<T> src{gin<T>(t.b), gin<T>(t.xin), float(S.beta)};
+  HFM_GLOBAL T* pnew = gout<T>(t.xout);
   double acc = 0.0;
   if (vec_task(t)) {
     const int n0 = base + threadIdx.x * kRun;
@@ -607,7 +615,7 @@ __global__ __launch_bounds__(kBlock) void k_fcg_dir(const hfm_mg_task* __restric
       float diag;
       const float off = face_row<float, float>(t, n, i, j, k, pat, diag);
       const float p = pat(n, k, j, i);
-      pnew[n] = from_f<T>(p);
+      put(pnew + n, from_f<T>(p));
       acc += double(p * (diag * p - off));
     }
   }
@@ -633,9 +641,11 @@ __global__ __launch_bounds__(kBlock) void k_fcg_update(const hfm_mg_task* __rest
   const int N = t.D * t.H * t.W;
   const int base = chunk * kCellChunk;
   const double alpha = S.alpha;
-  const SrcPlain<P> src{static_cast<const P*>(t.xin)};
-  T* rlow = static_cast<T*>(t.rlow);
-  T* x0 = static_cast<T*>(t.xout);
+  const SrcPlain<P> src{gin<P>(t.xin)};
+  HFM_GLOBAL T* rlow = gout<T>(t.rlow);
+  HFM_GLOBAL T* x0 = gout<T>(t.xout);
+  HFM_GLOBAL double* x64 = gout<double>(t.x64);
+  HFM_GLOBAL double* r64 = gout<double>(t.r64);
   double acc = 0.0;
   if (vec_task(t)) {
     const int n0 = base + threadIdx.x * kRun;
@@ -649,8 +659,8 @@ __global__ __launch_bounds__(kBlock) void k_fcg_update(const hfm_mg_task* __rest
     if (valid) {
       double xv[8], rv[8];
       float rl[8];
-      ld8d(t.x64 + n0, xv);
-      ld8d(t.r64 + n0, rv);
+      ld8d(x64 + n0, xv);
+      ld8d(r64 + n0, rv);
 #pragma unroll
       for (int m = 0; m < 8; ++m) {
         xv[m] += alpha * p[m];
@@ -658,8 +668,8 @@ __global__ __launch_bounds__(kBlock) void k_fcg_update(const hfm_mg_task* __rest
         rl[m] = float(rv[m]);
         acc += rv[m] * rv[m];
       }
-      st8d(t.x64 + n0, xv);
-      st8d(t.r64 + n0, rv);
+      st8d(x64 + n0, xv);
+      st8d(r64 + n0, rv);
       st8(rlow + n0, rl);
       if (SWEEP) {
         float o[8];
@@ -683,13 +693,13 @@ __global__ __launch_bounds__(kBlock) void k_fcg_update(const hfm_mg_task* __rest
       const double off = face_row<double, float>(t, n, i, j, k, pat, diag);
       const double pn = pat(n, k, j, i);
       const double q = diag * pn - off;
-      t.x64[n] += alpha * pn;
-      const double rr = t.r64[n] - alpha * q;
-      t.r64[n] = rr;
+      x64[n] += alpha * pn;
+      const double rr = r64[n] - alpha * q;
+      r64[n] = rr;
       const T rt = from_f<T>(float(rr));
-      rlow[n] = rt;
+      put(rlow + n, rt);
       acc += rr * rr;
-      if (SWEEP) x0[n] = from_f<T>(omega * to_f(rt) / diag_rounded<T>(t, n, i, j, k));
+      if (SWEEP) put(x0 + n, from_f<T>(omega * to_f(rt) / diag_rounded<T>(t, n, i, j, k)));
     }
   }
   const double s = block_sum(acc);

@@ -8,11 +8,15 @@ arrival.  Each epoch the loop
   3. rebuilds the launch plan over every resident request when the set
      changed and initialises the newly admitted ones,
   4. runs ``iters_per_epoch`` FCG iterations for all residents in shared
-     launches (requests that arrive meanwhile join at the next epoch),
-  5. retires converged (or failed) requests and frees their memory.
-Every request's arithmetic is independent of the others (per-instance
-kernels and fixed-order reductions), so a result is bitwise the same alone
-and in any mix.
+     launches (requests that arrive meanwhile join at the next epoch) and
+     queues an asynchronous copy of the solver scalars to pinned memory,
+  5. then, while the GPU runs that epoch, reads the previous epoch's scalars
+     and retires the requests that had converged (or failed) by then.
+A retired request's kernels in the epoch in flight exit at once (its slot
+is inactive), so its memory and slot are reused safely by later, stream-
+ordered work.  Every request's arithmetic is independent of the others
+(per-instance kernels and fixed-order reductions), so a result is bitwise
+the same alone and in any mix.
 
 The reference has no serving layer; this is the request loop around the
 multigrid solve of ART/BGS_multigrid_v1.ipynb#c42's operator.
@@ -80,6 +84,9 @@ class SolveServer:
         self.parity = 0
         self.epochs = 0
         self.peak_used = 0.0
+        nb = self.pool.sc.numel()
+        self._host_sc = [torch.empty(nb, dtype=torch.uint8, pin_memory=True) for _ in range(2)]
+        self._inflight = None      # (event, host buffer, residents) of the last launched epoch
 
     # ------------------------------------------------------------------ intake
     def submit(self, rid: int, sx, sy, sz, b, t_arrival: Optional[float] = None) -> None:
@@ -90,9 +97,6 @@ class SolveServer:
             raise ValueError(f"request {rid}: {need / 1e9:.1f} GB exceeds the HBM budget")
         self.queue.append(Request(rid, shape, sx, sy, sz, b,
                                   self.clock() if t_arrival is None else t_arrival))
-
-    def pending(self) -> int:
-        return len(self.queue) + len(self.resident)
 
     # ------------------------------------------------------------------ epoch
     def _admit(self) -> List[Instance]:
@@ -113,7 +117,8 @@ class SolveServer:
         return new
 
     def step(self) -> List[Result]:
-        """One epoch; returns the requests that finished in it."""
+        """Launch one epoch, then retire what the previous epoch finished;
+        returns those requests."""
         s = L.stream_handle(self.device)
         new = self._admit()
         if new:
@@ -124,18 +129,32 @@ class SolveServer:
         if new or self.plan is None:
             insts = [v[1] for v in self.resident.values()]
             self.plan = Plan(self.pool, insts) if insts else None
-        if self.plan is None:
+        prev, self._inflight = self._inflight, None
+        if self.plan is not None:
+            if new:
+                self.plan.init(s)
+            for _ in range(self.iters_per_epoch):
+                self.plan.iteration(s, self.parity)
+                self.parity ^= 1
+            self.epochs += 1
+            buf = self._host_sc[self.epochs % 2]
+            buf.copy_(self.pool.sc, non_blocking=True)
+            ev = torch.cuda.Event()
+            ev.record(torch.cuda.current_stream(self.device))
+            self._inflight = (ev, buf, dict(self.resident))
+        if prev is None:
             return []
-        if new:
-            self.plan.init(s)
-        for _ in range(self.iters_per_epoch):
-            self.plan.iteration(s, self.parity)
-            self.parity ^= 1
-        self.epochs += 1
-        sc = self.pool.scalars()           # synchronises with the epoch's launches
+        return self._retire(*prev)
+
+    def _retire(self, ev, buf, members) -> List[Result]:
+        """Retire the members of an epoch whose scalars (in ``buf``) say they stopped."""
+        ev.synchronize()
+        sc = buf.numpy().view(L.SCALARS_DTYPE)
         now = self.clock()
         done = []
-        for rid, (req, inst, need, t_adm) in list(self.resident.items()):
+        for rid, (req, inst, need, t_adm) in members.items():
+            if rid not in self.resident:
+                continue
             rec = sc[inst.slot]
             if int(rec["active"]) == 1:
                 continue
@@ -148,6 +167,9 @@ class SolveServer:
         if done:
             self.plan = None      # rebuilt over the remaining residents next epoch
         return done
+
+    def pending(self) -> int:
+        return len(self.queue) + len(self.resident)
 
     def drain(self) -> List[Result]:
         out = []

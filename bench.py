@@ -118,35 +118,62 @@ def fp32_one_at_a_time(specs, dev, torch):
 def serving(args, dev, torch):
     """Poisson stream of anisotropic solves, edges 32..256 log-uniform (multiples of
     16), sigma_ln = 1, one dipole: continuous batching (BF16 storage, 200 GB budget)
-    vs the plain FP32 path serving one request at a time."""
+    vs the plain FP32 path serving one request at a time.  Two offered loads: the
+    stated arrival rate (latency under that load) and a saturating rate (every
+    request arrives within ~0.1 s: each server's capacity in solves/s)."""
     from hfm.mg import Params
     from hfm.server import SolveServer, latency_stats, poisson_stream, run_stream
     from hfm.workloads import device_fields
-    stream = poisson_stream(args.serve_requests, args.serve_rate, args.seed + 1)
     mk = lambda shape, seed: device_fields(shape, 1.0, 1, seed, dev)  # noqa: E731
     out = {}
-    for name, params, cap in (("batched_bf16", Params(storage="bf16", tol=1e-8), 4096),
-                              ("fp32_one_at_a_time", Params(storage="f32", tol=1e-8), 1)):
-        srv = SolveServer(params, hbm_budget=200e9, capacity=cap, device=dev)
-        warm = SolveServer(params, hbm_budget=200e9, capacity=cap, device=dev)
-        warm.submit(-1, *mk((64, 64, 64), 1))
-        warm.drain()
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        res = run_stream(srv, stream, mk)
-        torch.cuda.synchronize()
-        wall = time.perf_counter() - t0
-        bad = [r for r in res if r.status != "converged" or r.rel_residual > 1e-8]
-        if bad:
-            raise SystemExit(f"serving {name}: {len(bad)} requests failed")
-        st = latency_stats(res)
-        out[name] = {"solves_per_s": round(len(res) / wall, 2), "p50_ms": round(st["p50_ms"], 2),
-                     "p99_ms": round(st["p99_ms"], 2), "requests": len(res),
-                     "peak_resident_GB": round(srv.peak_used / 1e9, 2), "epochs": srv.epochs}
-        log(f"bench: serving {name}: {out[name]}")
-    out["arrival_rate_per_s"] = args.serve_rate
-    out["speedup_solves_per_s"] = round(out["batched_bf16"]["solves_per_s"] /
-                                        out["fp32_one_at_a_time"]["solves_per_s"], 2)
+    servers = (("batched_bf16", Params(storage="bf16", tol=1e-8), 4096),
+               ("fp32_one_at_a_time", Params(storage="f32", tol=1e-8), 1))
+    for load, rate in (("at_rate", args.serve_rate), ("saturated", args.serve_sat_rate)):
+        stream = poisson_stream(args.serve_requests, rate, args.seed + 1)
+        res_load = {}
+        for name, params, cap in servers:
+            srv = SolveServer(params, hbm_budget=200e9, capacity=cap, device=dev)
+            warm = SolveServer(params, hbm_budget=200e9, capacity=cap, device=dev)
+            warm.submit(-1, *mk((64, 64, 64), 1))
+            warm.drain()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            res = run_stream(srv, stream, mk)
+            torch.cuda.synchronize()
+            wall = time.perf_counter() - t0
+            bad = [r for r in res if r.status != "converged" or r.rel_residual > 1e-8]
+            if bad:
+                raise SystemExit(f"serving {name}: {len(bad)} requests failed")
+            st = latency_stats(res)
+            res_load[name] = {"solves_per_s": round(len(res) / wall, 2),
+                              "p50_ms": round(st["p50_ms"], 2), "p99_ms": round(st["p99_ms"], 2),
+                              "requests": len(res), "peak_resident_GB": round(srv.peak_used / 1e9, 2),
+                              "epochs": srv.epochs}
+            log(f"bench: serving {load} {name}: {res_load[name]}")
+        res_load["arrival_rate_per_s"] = rate
+        res_load["speedup_solves_per_s"] = round(res_load["batched_bf16"]["solves_per_s"] /
+                                                 res_load["fp32_one_at_a_time"]["solves_per_s"], 2)
+        out[load] = res_load
+    return out
+
+
+def batched_fp32(specs, dev, torch):
+    """The same 512-instance batch with FP32 storage (the configuration's plain run)."""
+    from hfm.mg import MGBatch
+    mg = MGBatch([s[0] for s in specs], storage="f32", tol=1e-8, maxit=300, device=dev)
+    load_batch(mg, specs, dev)
+    one_step(mg)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    info = one_step(mg)
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    if not np.all(info["status"] == 1):
+        raise SystemExit("bench: FP32 batch did not converge")
+    out = {"solves_per_s": round(len(specs) / t, 2), "ms_per_step": round(t * 1e3, 2),
+           "iters_mean": round(float(info["iter"].mean()), 2), "iters_max": int(info["iter"].max()),
+           "resident_GiB": round(mg.nbytes() / 2 ** 30, 2)}
+    log(f"bench: batched fp32 {out}")
     return out
 
 
@@ -235,7 +262,8 @@ def kolmogorov(args, dev, torch):
         "ms_per_step": round(t1 / args.flow_steps * 1e3, 4),
         "steps_per_s": round(args.flow_steps / t1, 1),
         "cell_steps_per_s": round(N * N * args.flow_steps / t1, 1),
-        "substeps": int(st["substeps"]), "cg_iters_per_substep": round(st["iters_sum"] / st["substeps"], 3),
+        "substeps": int(st["substeps"]),
+        "cg_iters_per_substep": round(float(st["iters_sum"] / st["substeps"]), 3),
         "max_rel_residual": float(st["rel_max"]),
         "max_abs_divergence": float(div.abs().max()),
         "kinetic_energy": float(0.5 * (u.double() ** 2 + v.double() ** 2).mean())}
@@ -260,19 +288,27 @@ def kolmogorov(args, dev, torch):
                            "resident_GiB": round(ens.nbytes() / 2 ** 30, 2)}
         del ens
     if not args.no_cpu_baseline:
+        # the oracle's substeps from the run's first and last states (the early
+        # flow is slower and needs fewer substeps), scaled to the run's substeps
         mg = F.build_mg(N)
-        stc = F.FlowState(u0.copy(), v0.copy(), np.zeros((N, N)))
         fu = F.forcing_u(N)
-        t0 = time.perf_counter()
-        n = 0
-        while n < 10 or (time.perf_counter() - t0 < args.cpu_budget / 3 and n < args.flow_steps):
-            F.step(stc, mg, 0.005, 0.01, fu)
-            n += 1
-        tc = time.perf_counter() - t0
-        out["cpu_baseline"] = {"value": round(n / tc, 2), "unit": "steps/s", "cores": 1,
-                               "kind": "port", "sample": f"oracle step (numpy), first {n} "
-                               "steps of the same flow"}
-        out["speedup_vs_cpu"] = round(out["single_flow"]["steps_per_s"] / (n / tc), 1)
+        uf, vf, pf = (t.cpu().numpy() for t in fl.state(0))
+        nsub, tc = 0, 0.0
+        for (us_, vs_, ps_) in ((u0, v0, np.zeros((N, N))), (uf, vf, pf)):
+            stc = F.FlowState(us_.copy(), vs_.copy(), ps_.copy())
+            t0 = time.perf_counter()
+            k = 0
+            while k < 5 or time.perf_counter() - t0 < args.cpu_budget / 6:
+                nsub += F.step(stc, mg, 0.005, 0.01, fu)[0]
+                k += 1
+            tc += time.perf_counter() - t0
+        cpu_sps = args.flow_steps / (int(st["substeps"]) * tc / nsub)
+        out["cpu_baseline"] = {"value": round(cpu_sps, 2), "unit": "steps/s", "cores": 1,
+                               "kind": "port",
+                               "sample": f"oracle (numpy) steps from the run's first and last "
+                                         f"states: {nsub} substeps in {tc:.1f} s, scaled to the "
+                                         f"run's {int(st['substeps'])} substeps"}
+        out["speedup_vs_cpu"] = round(out["single_flow"]["steps_per_s"] / cpu_sps, 1)
     log(f"bench: kolmogorov {out}")
     return out
 
@@ -346,6 +382,8 @@ def main():
     ap.add_argument("--fp32-sample", type=int, default=16)
     ap.add_argument("--serve-requests", type=int, default=300)
     ap.add_argument("--serve-rate", type=float, default=300.0)
+    ap.add_argument("--serve-sat-rate", type=float, default=3000.0)
+    ap.add_argument("--no-fp32-batch", action="store_true")
     ap.add_argument("--profile-kernel", choices=["update", "sweep", "all"], default=None,
                     help="launch only this kernel (after setup) for profiler runs")
     ap.add_argument("--reps", type=int, default=20)
@@ -410,6 +448,12 @@ def main():
                 "fine_sweep": {"kernel": "k_sweep PLAIN", "launch_ms": round(sweep_ms, 4),
                                "achieved": round(sweep_bytes / (sweep_ms * 1e-3) / 1e9, 1)}}
     extra = {}
+    outer_storage = mg.params.outer
+    if not args.no_fp32_batch:
+        del mg
+        torch.cuda.empty_cache()
+        extra["batched_fp32"] = batched_fp32(specs, dev, torch)
+        extra["speedup_mixed_vs_fp32_batched"] = round(value / extra["batched_fp32"]["solves_per_s"], 2)
     if args.fp32_sample > 0:
         sub = specs[:args.fp32_sample]
         sps = fp32_one_at_a_time(sub, dev, torch)
@@ -430,7 +474,7 @@ def main():
         "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(total / args.steps * 1e3, 2), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64", "storage": args.storage,
-        "outer_storage": mg.params.outer,
+        "outer_storage": outer_storage,
         "data": "synthetic",
         "config": {"workload": f"batched uneven anisotropic solves: {len(specs)} instances, edges "
                                "{32,48,64,96,128}^3, log-normal sigma_ln U[0.5,2], 1-4 dipoles, "
