@@ -335,12 +335,75 @@ def profile_iteration(mg, cells, torch):
                       "launches": {k: round(v, 4) for k, v in rows}}), flush=True)
 
 
+def profile_flow2d(args, dev, torch):
+    """The Kolmogorov flow alone (for kernel traces): one 256^2 flow for
+    ``--flow-steps`` steps, then ``--flow-ensemble`` flows for
+    ``--flow-ensemble-steps`` steps; prints one JSON line."""
+    from hfm.flow2d import KolmogorovFlow
+    from oracle import hfm_flow2d as F
+    N, out = 256, {"workload": "kolmogorov_2d"}
+    for name, nflows, steps in (("single_flow", 1, args.flow_steps),
+                                ("ensemble", args.flow_ensemble, args.flow_ensemble_steps)):
+        if nflows < 1 or steps < 1:
+            continue
+        fl = KolmogorovFlow(N=N, nflows=nflows, device=dev)
+        for q in range(nflows):
+            u, v = F.random_modes_ic(N, args.seed + q)
+            fl.set_state(q, torch.from_numpy(u), torch.from_numpy(v))
+        s = torch.cuda.current_stream()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(s)
+        fl.step(steps)
+        e1.record(s)
+        e1.synchronize()
+        st = fl.stats()
+        out[name] = {"flows": nflows, "steps": steps, "launch_ms": round(e0.elapsed_time(e1), 3),
+                     "ms_per_step": round(e0.elapsed_time(e1) / steps, 4),
+                     "substeps": int(st["substeps"].sum()),
+                     "cg_iters": int(st["iters_sum"].sum())}
+        del fl
+    print(json.dumps(out), flush=True)
+
+
+def profile_urban(args, dev, torch):
+    """The urban projection step alone (for kernel traces): ``--urban-steps``
+    steps of the FP32 and the BF16 + FP32-z/p variants; prints one JSON line."""
+    from hfm.timestep import UrbanProjection
+    from hfm.workloads import log_law_profile, urban_blocks
+    sh = (64, 256, 256)
+    mask = urban_blocks(sh, args.seed)
+    sig = torch.from_numpy(np.where(mask, 1e5, 0.0).astype(np.float32)).to(dev)
+    prof = log_law_profile(sh[0])
+    out = {"workload": "urban_projection", "grid": list(sh)}
+    for name, (storage, outer) in {"f32": ("f32", None), "bf16_fp32_zp": ("bf16", "f32")}.items():
+        flow = UrbanProjection(sh, sigma=sig, inflow=prof, dt=0.05, storage=storage, outer=outer,
+                               tol=1e-8, device=dev)
+        u0 = torch.from_numpy(np.broadcast_to(prof[:, None, None], sh).copy()).to(dev)
+        z = torch.zeros(sh, device=dev)
+        state = (u0, z, z.clone(), z.clone(), z.clone())
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        its = []
+        for _ in range(args.urban_steps):
+            state = flow.step(*state)
+            its.append(int(flow.last_info["iter"][0]))
+        torch.cuda.synchronize()
+        out[name] = {"steps": args.urban_steps,
+                     "ms_per_step": round((time.perf_counter() - t0) / args.urban_steps * 1e3, 3),
+                     "mg_iters": its}
+    print(json.dumps(out), flush=True)
+
+
 def profile_kernel(args, dev, torch):
     """Launch only one kernel (all instances active) and a copy of known size,
     for rocprofv3 kernel-trace / PMC runs; prints one JSON line."""
     import numpy as np
     from hfm.mg import MGBatch
     from hfm.workloads import batched_specs
+    if args.profile_kernel == "flow2d":
+        return profile_flow2d(args, dev, torch)
+    if args.profile_kernel == "urban":
+        return profile_urban(args, dev, torch)
     specs = batched_specs(args.instances, args.seed)
     cells = sum(math.prod(s[0]) for s in specs)
     mg = MGBatch([s[0] for s in specs], storage=args.storage, tol=1e-8, device=dev,
@@ -384,8 +447,9 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=300.0)
     ap.add_argument("--serve-sat-rate", type=float, default=3000.0)
     ap.add_argument("--no-fp32-batch", action="store_true")
-    ap.add_argument("--profile-kernel", choices=["update", "sweep", "all"], default=None,
-                    help="launch only this kernel (after setup) for profiler runs")
+    ap.add_argument("--profile-kernel", choices=["update", "sweep", "all", "flow2d", "urban"],
+                    default=None,
+                    help="launch only this kernel (after setup) or workload, for profiler runs")
     ap.add_argument("--reps", type=int, default=20)
     ap.add_argument("--urban-steps", type=int, default=10)
     ap.add_argument("--flow-steps", type=int, default=1000,
