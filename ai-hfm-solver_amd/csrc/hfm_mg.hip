@@ -338,8 +338,9 @@ __device__ __forceinline__ Run run_at(const hfm_mg_task& t, int n0, bool valid) 
 // T: storage of the level's vectors; O: storage of this sweep's output (the
 // fine level's last post-sweep may write the preconditioned residual z in a
 // wider type than the cycle's internals).
-template <typename T, int MODE, bool DOT, typename O>
-__global__ __launch_bounds__(kBlock) void k_sweep(const hfm_mg_task* __restrict__ tasks,
+// B: threads per block (a block covers 8 B cells); DOT launches use kBlock.
+template <typename T, int MODE, bool DOT, typename O, int B>
+__global__ __launch_bounds__(B) void k_sweep(const hfm_mg_task* __restrict__ tasks,
                                                   const uint16_t* __restrict__ blk_task,
                                                   const hfm_mg_scalars* __restrict__ sc,
                                                   float omega, float alpha, double* dots) {
@@ -348,7 +349,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep(const hfm_mg_task* __restrict_
   if (!sc[t.inst].active) return;
   const int chunk = xcd_chunk(g - t.blk0, t.nblk);
   const int N = t.D * t.H * t.W;
-  const int base = chunk * kCellChunk;
+  const int base = chunk * (B * kRun);
   const HFM_GLOBAL T* xin = gin<T>(t.xin);
   const HFM_GLOBAL T* b = gin<T>(t.b);
   const HFM_GLOBAL T* ec = gin<T>(t.ec);
@@ -385,7 +386,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep(const hfm_mg_task* __restrict_
     }
   } else if (base < N) {
     for (int s = 0; s < kRun; ++s) {
-      const int n = base + s * kBlock + threadIdx.x;
+      const int n = base + s * B + threadIdx.x;
       if (n >= N) break;
       int i, j, k;
       decompose(t, n, i, j, k);
@@ -426,8 +427,8 @@ __global__ __launch_bounds__(kBlock) void k_sweep(const hfm_mg_task* __restrict_
 // bc = sum over the (up to 2x2x2) children of (b - A x), over 8.
 // Vector path: a thread owns 4 consecutive coarse cells = a run of 8 fine
 // cells in each of its 2 fine rows x 2 fine planes.
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_resid_restrict(const hfm_mg_task* __restrict__ tasks,
+template <typename T, int B>
+__global__ __launch_bounds__(B) void k_resid_restrict(const hfm_mg_task* __restrict__ tasks,
                                                            const uint16_t* __restrict__ blk_task,
                                                            const hfm_mg_scalars* __restrict__ sc) {
   const int g = blockIdx.x;
@@ -435,7 +436,7 @@ __global__ __launch_bounds__(kBlock) void k_resid_restrict(const hfm_mg_task* __
   if (!sc[t.inst].active) return;
   const int chunk = xcd_chunk(g - t.blk0, t.nblk);
   const int Nc = t.Dc * t.Hc * t.Wc;
-  const int base = chunk * kCoarseChunk;
+  const int base = chunk * (B * 4);
   const HFM_GLOBAL T* x = gin<T>(t.xin);
   const HFM_GLOBAL T* b = gin<T>(t.b);
   HFM_GLOBAL T* bc = gout<T>(t.xout);
@@ -486,7 +487,7 @@ __global__ __launch_bounds__(kBlock) void k_resid_restrict(const hfm_mg_task* __
   if (base >= Nc) return;
   auto xat = [&](int m, int, int, int) { return ldg(x + m); };
   for (int s = 0; s < 4; ++s) {
-    const int nc = base + s * kBlock + threadIdx.x;
+    const int nc = base + s * B + threadIdx.x;
     if (nc >= Nc) break;
     const uint32_t rowc = fdiv(uint32_t(nc), t.magWc);
     const int I = nc - int(rowc) * t.Wc;
@@ -1008,12 +1009,44 @@ int check_table(const void* tasks, const void* blk_task, int64_t nblocks, const 
 
 }  // namespace
 
+namespace {
+// Threads per block of the tunable cell kernels: 128, 256 or 512.
+bool block_ok(int block) { return block == 128 || block == 256 || block == 512; }
+
+template <int B, typename T, typename O>
+void launch_sweep(int mode, bool dot, dim3 grid, hipStream_t s, const hfm_mg_task* tasks,
+                  const uint16_t* blk_task, const hfm_mg_scalars* sc, float omega, float alpha,
+                  double* dots) {
+  const dim3 block{unsigned(B)};
+#define HFM_SWEEP_CASE(M)                                                                       \
+  if (dot)                                                                                      \
+    hipLaunchKernelGGL((k_sweep<T, M, true, O, kBlock>), grid, dim3(kBlock), 0, s, tasks,       \
+                       blk_task, sc, omega, alpha, dots);                                       \
+  else                                                                                          \
+    hipLaunchKernelGGL((k_sweep<T, M, false, O, B>), grid, block, 0, s, tasks, blk_task, sc,     \
+                       omega, alpha, dots);
+  if (mode == HFM_SWEEP_ZERO) {
+    HFM_SWEEP_CASE(HFM_SWEEP_ZERO)
+  } else if (mode == HFM_SWEEP_PLAIN) {
+    HFM_SWEEP_CASE(HFM_SWEEP_PLAIN)
+  } else {
+    HFM_SWEEP_CASE(HFM_SWEEP_CORRECT)
+  }
+#undef HFM_SWEEP_CASE
+}
+}  // namespace
+
 extern "C" int hfm_mg_sweep(const hfm_mg_task* tasks, const uint16_t* blk_task, int64_t nblocks,
                             const hfm_mg_scalars* sc, int store, int out_store, int mode,
-                            float omega, float alpha, double* dots, void* stream) {
+                            float omega, float alpha, int block, double* dots, void* stream) {
   if (int e = check_table(tasks, blk_task, nblocks, sc, "hfm_mg_sweep")) return e;
   if (mode < HFM_SWEEP_ZERO || mode > HFM_SWEEP_CORRECT) {
     set_error("hfm_mg_sweep: bad mode %d", mode);
+    return HFM_EINVAL;
+  }
+  if (!block_ok(block) || (dots && block != kBlock)) {
+    set_error("hfm_mg_sweep: block of %d threads not supported%s", block,
+              dots ? " with dot partials (256 only)" : "");
     return HFM_EINVAL;
   }
   if (nblocks == 0) return HFM_OK;
@@ -1021,35 +1054,36 @@ extern "C" int hfm_mg_sweep(const hfm_mg_task* tasks, const uint16_t* blk_task, 
   return launch_store2(store, out_store, "hfm_mg_sweep", [&](auto tag, auto otag) {
     using T = decltype(tag);
     using O = decltype(otag);
-    const dim3 grid{unsigned(nblocks)}, block{unsigned(kBlock)};
+    const dim3 grid{unsigned(nblocks)};
     const bool dot = dots != nullptr;
-#define HFM_SWEEP_CASE(M)                                                                  \
-  if (dot)                                                                                 \
-    hipLaunchKernelGGL((k_sweep<T, M, true, O>), grid, block, 0, s, tasks, blk_task, sc,    \
-                       omega, alpha, dots);                                                \
-  else                                                                                     \
-    hipLaunchKernelGGL((k_sweep<T, M, false, O>), grid, block, 0, s, tasks, blk_task, sc,   \
-                       omega, alpha, dots);
-    if (mode == HFM_SWEEP_ZERO) {
-      HFM_SWEEP_CASE(HFM_SWEEP_ZERO)
-    } else if (mode == HFM_SWEEP_PLAIN) {
-      HFM_SWEEP_CASE(HFM_SWEEP_PLAIN)
-    } else {
-      HFM_SWEEP_CASE(HFM_SWEEP_CORRECT)
-    }
-#undef HFM_SWEEP_CASE
+    if (block == 128)
+      launch_sweep<128, T, O>(mode, dot, grid, s, tasks, blk_task, sc, omega, alpha, dots);
+    else if (block == 256)
+      launch_sweep<256, T, O>(mode, dot, grid, s, tasks, blk_task, sc, omega, alpha, dots);
+    else
+      launch_sweep<512, T, O>(mode, dot, grid, s, tasks, blk_task, sc, omega, alpha, dots);
   });
 }
 
 extern "C" int hfm_mg_resid_restrict(const hfm_mg_task* tasks, const uint16_t* blk_task,
                                      int64_t nblocks, const hfm_mg_scalars* sc, int store,
-                                     void* stream) {
+                                     int block, void* stream) {
   if (int e = check_table(tasks, blk_task, nblocks, sc, "hfm_mg_resid_restrict")) return e;
+  if (!block_ok(block)) {
+    set_error("hfm_mg_resid_restrict: block of %d threads not supported", block);
+    return HFM_EINVAL;
+  }
   if (nblocks == 0) return HFM_OK;
   return launch_store(store, "hfm_mg_resid_restrict", [&](auto tag) {
     using T = decltype(tag);
-    hipLaunchKernelGGL(k_resid_restrict<T>, dim3(unsigned(nblocks)), dim3(kBlock), 0,
-                       as_stream(stream), tasks, blk_task, sc);
+    const dim3 grid{unsigned(nblocks)};
+    hipStream_t s = as_stream(stream);
+    if (block == 128)
+      hipLaunchKernelGGL((k_resid_restrict<T, 128>), grid, dim3(128), 0, s, tasks, blk_task, sc);
+    else if (block == 256)
+      hipLaunchKernelGGL((k_resid_restrict<T, 256>), grid, dim3(256), 0, s, tasks, blk_task, sc);
+    else
+      hipLaunchKernelGGL((k_resid_restrict<T, 512>), grid, dim3(512), 0, s, tasks, blk_task, sc);
   });
 }
 

@@ -70,8 +70,8 @@ SIGNATURES = {
     "hfm_bc_velocity_f32": (_int, [_vp, _vp, _vp, _vp, _vp, _vp, _i64, _i64, _i64, _f32, _vp,
                                    _vp]),
     "hfm_bluff_body_f32": (_int, [_vp, _vp, _int, _vp, _f32, _i64, _vp]),
-    "hfm_mg_sweep": (_int, [_vp, _vp, _i64, _vp, _int, _int, _int, _f32, _f32, _vp, _vp]),
-    "hfm_mg_resid_restrict": (_int, [_vp, _vp, _i64, _vp, _int, _vp]),
+    "hfm_mg_sweep": (_int, [_vp, _vp, _i64, _vp, _int, _int, _int, _f32, _f32, _int, _vp, _vp]),
+    "hfm_mg_resid_restrict": (_int, [_vp, _vp, _i64, _vp, _int, _int, _vp]),
     "hfm_mg_coarse_solve": (_int, [_vp, _i64, _vp, _int, _vp]),
     "hfm_fcg_init": (_int, [_vp, _vp, _i64, _vp, _int, _vp, _vp]),
     "hfm_fcg_dir": (_int, [_vp, _vp, _i64, _vp, _int, _vp, _vp]),

@@ -34,8 +34,10 @@ import torch
 
 from . import _lib as L
 
-CHUNK = 2048          # cells per block of the cell kernels (256 threads x 8)
-RCHUNK = 1024         # coarse cells per block of the residual + restriction
+BLOCK = 256           # threads per block of the FCG kernels and the default elsewhere
+CHUNK = 8 * BLOCK     # cells per block of the cell kernels (a thread owns 8)
+RCHUNK = 4 * BLOCK    # coarse cells per block of the residual + restriction
+BLOCKS = (128, 256, 512)   # threads per block a sweep / restriction launch may use
 SCHUNK = 256          # cells per block of the batched setup kernels
 ALIGN = 8             # element alignment of sub-array offsets (16-byte rows)
 ELEM = {"f32": 4, "bf16": 2}
@@ -87,6 +89,9 @@ class Params:
     alpha: float = 1.6
     max_coarse_edge: int = 4
     outer: Optional[str] = None   # z and p storage (default: ``storage``)
+    # threads per block of the launches without reductions, keyed
+    # ("sweep", mode, level) / ("restrict", level); default BLOCK (hfm.tune)
+    blocks: Optional[dict] = None
 
     def __post_init__(self):
         if self.storage not in ELEM:
@@ -492,26 +497,30 @@ class Plan:
             return _table(dev, [i._tpl[key] for i in members], [size_of(i) for i in members],
                           chunk)
 
+        blocks = p.blocks or {}
         self.ops_down, self.ops_up = [], []
         for lev in range(nlev - 1):
             cells = lambda i, lev=lev: math.prod(i.levels[lev])  # noqa: E731
             for s in range(p.nu1):
                 if lev == 0 and s == 0:
                     continue    # computed by the FCG update (and init) kernels
-                tab = op(("down", lev, s), CHUNK, cells)
+                mode = "zero" if s == 0 else "plain"
+                b = blocks.get(("sweep", mode, lev), BLOCK)
+                tab = op(("down", lev, s), 8 * b, cells)
                 if tab:
-                    self.ops_down.append(("sweep", L.SWEEP["zero" if s == 0 else "plain"],
-                                          tab[0], tab[1], False, lev))
-            tab = op(("restrict", lev), RCHUNK, lambda i, lev=lev: math.prod(i.levels[lev + 1]))
+                    self.ops_down.append(("sweep", L.SWEEP[mode], tab[0], tab[1], False, lev, b))
+            b = blocks.get(("restrict", lev), BLOCK)
+            tab = op(("restrict", lev), 4 * b, lambda i, lev=lev: math.prod(i.levels[lev + 1]))
             if tab:
-                self.ops_down.append(("restrict", 0, tab[0], tab[1], False, lev))
+                self.ops_down.append(("restrict", 0, tab[0], tab[1], False, lev, b))
             ups = []
             for s in range(p.nu2):
-                tab = op(("up", lev, s), CHUNK, cells)
+                mode = "correct" if s == 0 else "plain"
+                dot = lev == 0 and s == p.nu2 - 1
+                b = BLOCK if dot else blocks.get(("sweep", mode, lev), BLOCK)
+                tab = op(("up", lev, s), 8 * b, cells)
                 if tab:
-                    dot = lev == 0 and s == p.nu2 - 1
-                    ups.append(("sweep", L.SWEEP["correct" if s == 0 else "plain"],
-                                tab[0], tab[1], dot, lev))
+                    ups.append(("sweep", L.SWEEP[mode], tab[0], tab[1], dot, lev, b))
             self.ops_up.insert(0, ups)
         self.coarse_tasks = torch.from_numpy(
             np.array([i._tpl[("coarse",)] for i in insts], dtype=L.TASK_DTYPE).view(np.uint8)
@@ -523,17 +532,17 @@ class Plan:
         self._graphs = {}
 
     def _launch(self, op, s):
-        kind, mode, tasks, lay, dot = op[:5]
+        kind, mode, tasks, lay, dot, _, block = op
         pool, p = self.pool, self.pool.params
         if kind == "sweep":
             # the sweep carrying the r.z dot is the fine level's last: it writes z
             out = pool.outer_id if dot else pool.store_id
             L.call("hfm_mg_sweep", tasks.data_ptr(), lay.blk_task.data_ptr(), lay.nblocks,
-                   pool.sc.data_ptr(), pool.store_id, out, mode, p.omega, p.alpha,
+                   pool.sc.data_ptr(), pool.store_id, out, mode, p.omega, p.alpha, block,
                    pool.dots.data_ptr() if dot else 0, s)
         else:
             L.call("hfm_mg_resid_restrict", tasks.data_ptr(), lay.blk_task.data_ptr(),
-                   lay.nblocks, pool.sc.data_ptr(), pool.store_id, s)
+                   lay.nblocks, pool.sc.data_ptr(), pool.store_id, block, s)
 
     def init(self, s):
         """x = 0, r = b for the plan's staged instances, then finalize INIT."""
@@ -602,8 +611,9 @@ class MGBatch:
                  maxit: int = 200, nu1: int = 2, nu2: int = 2, omega: float = 2.0 / 3.0,
                  alpha: float = 1.6, max_coarse_edge: int = 4, device=None,
                  iters_per_graph: int = 4, use_graph: bool = True, shrink: float = 0.5,
-                 outer: Optional[str] = None):
-        self.params = Params(storage, tol, maxit, nu1, nu2, omega, alpha, max_coarse_edge, outer)
+                 outer: Optional[str] = None, blocks: Optional[dict] = None):
+        self.params = Params(storage, tol, maxit, nu1, nu2, omega, alpha, max_coarse_edge, outer,
+                             blocks)
         shapes = [check_shape(s, max_coarse_edge)[0] for s in shapes]
         if not shapes:
             raise ValueError("empty batch")
@@ -656,6 +666,17 @@ class MGBatch:
         """(Re)initialise every instance: x = 0, r = b."""
         self.pool.stage(self.insts)
         self.plan.init(L.stream_handle(self.device))
+
+    def set_blocks(self, blocks: Optional[dict]):
+        """Use these per-launch block sizes (see ``hfm.tune``); rebuilds the plans."""
+        for k, b in (blocks or {}).items():
+            if b not in BLOCKS:
+                raise ValueError(f"block size {b} for {k} is not one of {BLOCKS}")
+        self.params.blocks = dict(blocks) if blocks else None
+        for inst in self.insts:
+            inst._tpl = None
+        self.plan = Plan(self.pool, self.insts)
+        self._subplans = {}
 
     def _subplan(self, live):
         key = tuple(i.slot for i in live)

@@ -1,0 +1,138 @@
+"""Launch-shape autotuner for the batched multigrid plans.
+
+The sweeps without a reduction (every V-cycle sweep but the fine level's
+last, which carries the r.z partials) and the residual + restriction launches
+accept 128, 256 or 512 threads per block; a block covers 8 (sweeps) or 4
+(restriction, coarse) cells per thread, so the choice changes how many cells
+each block streams and how many blocks each XCD schedules, not what a cell
+computes: results are bitwise identical for every choice.  ``tune_blocks``
+times each candidate for every such launch of a batch's plan with HIP events
+(all instances active, on the batch's own data) and keeps the fastest;
+``TuneCache`` stores the choices per device, storage type and batch size
+class in a JSON file so a later batch of the same class starts tuned.
+
+The reference has no tuner (its layers are single Keras convolutions); this
+is the launch-shape half of the batched solver's tuning.
+"""
+from __future__ import annotations
+
+import json
+import math
+import os
+from typing import Dict, Optional, Tuple
+
+import torch
+
+from . import _lib as L
+from .mg import BLOCK, BLOCKS, MGBatch, Plan
+
+Key = Tuple
+
+
+def _time_ms(fn, reps: int) -> float:
+    s = torch.cuda.current_stream()
+    fn()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(s)
+    for _ in range(reps):
+        fn()
+    e1.record(s)
+    e1.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def _op_key(op) -> Optional[Key]:
+    kind, mode, _, _, dot, lev, _ = op
+    if dot:
+        return None
+    if kind == "restrict":
+        return ("restrict", lev)
+    return ("sweep", {v: k for k, v in L.SWEEP.items()}[mode], lev)
+
+
+def tune_blocks(mg: MGBatch, reps: int = 5, candidates=BLOCKS, min_ms: float = 0.02) -> Dict[Key, int]:
+    """Choose the threads per block of every tunable launch of ``mg``'s plan.
+
+    Every candidate plan launches each op ``reps`` times after one warm
+    launch; ops faster than ``min_ms`` with the default keep the default (their
+    time is launch latency, not streaming).  The batch's data must be set up
+    (``setup``); the solver state is re-initialised first (every instance
+    active), the launches only rewrite V-cycle scratch vectors, and
+    ``restart``/``solve`` re-initialise it again.  Returns the choice table and
+    installs it with ``mg.set_blocks``.
+    """
+    mg._need_gpu()
+    mg.restart()
+    s = L.stream_handle(mg.device)
+    times: Dict[Key, Dict[int, float]] = {}
+    for b in candidates:
+        trial = {}
+        base = Plan(mg.pool, mg.insts)           # default table, for the key list
+        for op in base.ops_down + [o for ups in base.ops_up for o in ups]:
+            k = _op_key(op)
+            if k is not None:
+                trial[k] = b
+        mg.params.blocks = trial
+        plan = Plan(mg.pool, mg.insts)
+        for op in plan.ops_down + [o for ups in plan.ops_up for o in ups]:
+            k = _op_key(op)
+            if k is None:
+                continue
+            times.setdefault(k, {})[b] = _time_ms(lambda op=op: plan._launch(op, s), reps)
+    choice = {}
+    for k, t in times.items():
+        best = min(t, key=t.get)
+        if t.get(BLOCK, math.inf) < min_ms or t[best] >= t.get(BLOCK, math.inf):
+            best = BLOCK
+        choice[k] = best
+    mg.set_blocks(choice)
+    mg.tune_times = times
+    return choice
+
+
+def batch_class(mg: MGBatch) -> str:
+    """Size class of a batch: storage, outer type and log2 of its fine cells."""
+    cells = sum(i.N for i in mg.insts)
+    return f"{mg.params.storage}/{mg.params.outer}/2^{int(round(math.log2(max(cells, 1))))}"
+
+
+class TuneCache:
+    """Block-size tables keyed by device name and batch class, in a JSON file."""
+
+    def __init__(self, path: Optional[str] = None):
+        self.path = path or os.environ.get(
+            "HFM_TUNE_CACHE", os.path.join(os.path.expanduser("~"), ".cache", "hfm", "tune.json"))
+        try:
+            with open(self.path) as f:
+                self.data = json.load(f)
+        except (OSError, ValueError):
+            self.data = {}
+
+    @staticmethod
+    def _dev(mg: MGBatch) -> str:
+        return torch.cuda.get_device_name(mg.device)
+
+    def get(self, mg: MGBatch) -> Optional[Dict[Key, int]]:
+        rec = self.data.get(self._dev(mg), {}).get(batch_class(mg))
+        if rec is None:
+            return None
+        return {tuple(json.loads(k)): int(v) for k, v in rec.items()}
+
+    def put(self, mg: MGBatch, choice: Dict[Key, int]):
+        self.data.setdefault(self._dev(mg), {})[batch_class(mg)] = {
+            json.dumps(list(k)): int(v) for k, v in choice.items()}
+        os.makedirs(os.path.dirname(self.path) or ".", exist_ok=True)
+        tmp = self.path + ".tmp"
+        with open(tmp, "w") as f:
+            json.dump(self.data, f, indent=1, sort_keys=True)
+        os.replace(tmp, self.path)
+
+    def tuned(self, mg: MGBatch, reps: int = 5) -> Dict[Key, int]:
+        """The cached table for ``mg``'s class if any (installed), else tune and store."""
+        choice = self.get(mg)
+        if choice is not None:
+            mg.set_blocks(choice)
+            return choice
+        choice = tune_blocks(mg, reps)
+        self.put(mg, choice)
+        return choice

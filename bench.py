@@ -413,7 +413,17 @@ def profile_kernel(args, dev, torch):
     mg.setup()
     mg.restart()
     if args.profile_kernel == "all":
-        return profile_iteration(mg, cells, torch)
+        profile_iteration(mg, cells, torch)
+        if not args.no_tune:
+            from hfm.tune import tune_blocks
+            choice = tune_blocks(mg)
+            print(json.dumps({"tune_ms": {"/".join(map(str, k)): {str(b): round(t, 4) for b, t in v.items()}
+                                          for k, v in sorted(mg.tune_times.items())},
+                              "choice": {"/".join(map(str, k)): v for k, v in sorted(choice.items())}}),
+                  flush=True)
+            mg.restart()
+            profile_iteration(mg, cells, torch)
+        return None
     fn = {"update": lambda: mg.launch_update(0), "sweep": mg.launch_fine_sweep}[args.profile_kernel]
     per_cell = {"update": mg.update_bytes_per_cell(), "sweep": mg.sweep_bytes_per_cell()}
     ms = kernel_time_ms(fn, args.reps, torch)
@@ -447,6 +457,8 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=300.0)
     ap.add_argument("--serve-sat-rate", type=float, default=3000.0)
     ap.add_argument("--no-fp32-batch", action="store_true")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep 256-thread blocks (skip the launch-shape tuner)")
     ap.add_argument("--profile-kernel", choices=["update", "sweep", "all", "flow2d", "urban"],
                     default=None,
                     help="launch only this kernel (after setup) or workload, for profiler runs")
@@ -477,6 +489,17 @@ def main():
     mg = MGBatch(shapes, storage=args.storage, tol=1e-8, maxit=300, device=dev, iters_per_graph=4,
                  outer=args.outer)
     load_batch(mg, specs, dev)
+    tuning = None
+    if not args.no_tune:
+        from hfm.tune import tune_blocks
+        mg.build_faces()
+        mg.setup()
+        t0 = time.perf_counter()
+        choice = tune_blocks(mg)
+        tuning = {"tune_s": round(time.perf_counter() - t0, 2),
+                  "blocks": {"/".join(map(str, k)): v for k, v in sorted(choice.items())
+                             if v != 256}}
+        log(f"bench: launch tuning {tuning}")
     torch.cuda.synchronize()
     log(f"bench: resident {mg.nbytes() / 2**30:.1f} GiB; warmup {args.warmup}")
     for _ in range(args.warmup):
@@ -548,7 +571,7 @@ def main():
         "max_rel_residual": float(info["rel"].max()),
         "latency_ms": {"p50": round(float(np.percentile(times, 50)) * 1e3, 2),
                        "p99": round(float(np.percentile(times, 99)) * 1e3, 2)},
-        "roofline": roofline, "cpu_baseline": cpu, **extra,
+        "roofline": roofline, "cpu_baseline": cpu, "launch_tuning": tuning, **extra,
     }
     print(json.dumps(out), flush=True)
 

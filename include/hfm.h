@@ -202,18 +202,20 @@ typedef struct hfm_mg_scalars {
  * ec; `out_store` for xout: the fine level's last post-sweep may write the
  * preconditioned residual z wider than the cycle's internals).  Slots: c =
  * this level's coefficients, xin (unused for ZERO), b, xout, ec (coarse
- * correction for CORRECT, extents Dc/Hc/Wc).  dots != NULL: partial
- * sum(b*out) per block into dots[slot0 + local block], out rounded to
- * out_store. */
+ * correction for CORRECT, extents Dc/Hc/Wc).  `block`: threads per block
+ * (128, 256 or 512; a block covers 8 * block cells of its task, the work
+ * table's layout must use that chunk).  dots != NULL: partial sum(b*out) per
+ * block into dots[slot0 + local block], out rounded to out_store; block must
+ * then be 256 (the FCG kernels' chunk of 2048 cells). */
 int hfm_mg_sweep(const hfm_mg_task* tasks, const uint16_t* blk_task, int64_t nblocks,
                  const hfm_mg_scalars* sc, int store, int out_store, int mode, float omega,
-                 float alpha, double* dots, void* stream);
+                 float alpha, int block, double* dots, void* stream);
 
 /* bc = R (b - A x): one coarse cell per thread over its <= 8 children,
  * R = P^T/8.  Slots: c, xin = x, b, xout = bc (coarse, extents Dc/Hc/Wc).
- * Blocks cover the coarse grid. */
+ * Blocks of `block` threads (128, 256, 512) cover 4 * block coarse cells. */
 int hfm_mg_resid_restrict(const hfm_mg_task* tasks, const uint16_t* blk_task, int64_t nblocks,
-                          const hfm_mg_scalars* sc, int store, void* stream);
+                          const hfm_mg_scalars* sc, int store, int block, void* stream);
 
 /* Coarsest solve e = Ainv b (Ainv FP64 dense, N = D*H*W <= 64): one block
  * per task.  Slots: b, xout = e, ec = Ainv. */

@@ -100,3 +100,20 @@ def test_graph_and_eager_agree(dev):
     # a second solve on the same resident hierarchy gives the same answer
     i3 = mg1.solve()
     assert int(i3["iter"][0]) == int(i1["iter"][0])
+
+
+def test_launch_shapes_do_not_change_results(dev):
+    """Every block size of the tunable launches gives bitwise the same solve."""
+    from hfm.tune import tune_blocks
+    specs = [((32, 32, 32), 31, 1.0, 1), ((48, 40, 64), 32, 1.5, 2), ((20, 12, 36), 33, 0.7, 1)]
+    ref, _ = _batch(dev, specs, "bf16")
+    iref = ref.solve()
+    mg, _ = _batch(dev, specs, "bf16")
+    choice = tune_blocks(mg, reps=2)
+    assert choice and set(choice.values()) <= {128, 256, 512}
+    for table in (choice, {k: 128 for k in choice}, {k: 512 for k in choice}):
+        mg.set_blocks(table)
+        info = mg.solve()
+        assert list(info["iter"]) == list(iref["iter"])
+        for i in range(len(specs)):
+            assert bool((mg.solution(i) == ref.solution(i)).all())

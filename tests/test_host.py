@@ -86,12 +86,19 @@ def _owned_by(inst, ptr, nbytes):
                for b in bufs)
 
 
-def test_work_tables_cover_every_cell_once():
+@pytest.mark.parametrize("tuned", [False, True])
+def test_work_tables_cover_every_cell_once(tuned):
     torch = pytest.importorskip("torch")
     from hfm import _lib as L
-    from hfm.mg import ALIGN, CHUNK, RCHUNK, MGBatch
+    from hfm.mg import ALIGN, CHUNK, MGBatch
     shapes = [(32, 32, 32), (48, 48, 48), (13, 20, 7), (96, 40, 33)]
     mg = MGBatch(shapes, storage="bf16", device="cpu")
+    if tuned:   # launch shapes as the tuner may set them
+        blocks = {("sweep", "plain", 0): 512, ("sweep", "correct", 0): 128,
+                  ("restrict", 0): 128, ("restrict", 1): 512, ("sweep", "zero", 1): 512}
+        mg.set_blocks(blocks)
+        with pytest.raises(ValueError):
+            mg.set_blocks({("restrict", 0): 384})
     plan = mg.plan
     by_slot = {inst.slot: inst for inst in mg.insts}
     fl = plan.flay
@@ -117,12 +124,16 @@ def test_work_tables_cover_every_cell_once():
         assert a1 <= b0
     assert dot_ranges[-1][1] <= mg.pool.dots.numel()
     ops = plan.ops_down + [op for ups in plan.ops_up for op in ups]
-    for kind, mode, tt, lay, dot, lev in ops:
+    for kind, mode, tt, lay, dot, lev, block in ops:
+        key = ("restrict", lev) if kind == "restrict" else \
+            ("sweep", {0: "zero", 1: "plain", 2: "correct"}[mode], lev)
+        assert block == (256 if dot or not tuned else blocks.get(key, 256))
+        assert lay.blk_task.numel() == lay.nblocks
         for t in tt.numpy().view(L.TASK_DTYPE):
             inst = by_slot[int(t["inst"])]
             assert inst.levels[lev] == (t["D"], t["H"], t["W"])
             N = int(t["D"]) * int(t["H"]) * int(t["W"])
-            chunk = RCHUNK if kind == "restrict" else CHUNK
+            chunk = 4 * block if kind == "restrict" else 8 * block
             cells = int(t["Dc"]) * int(t["Hc"]) * int(t["Wc"]) if kind == "restrict" else N
             assert t["nblk"] * chunk >= cells and t["nblk"] % 8 == 0 and t["blk0"] % 8 == 0
             for ptr in list(t["c"]) + [t["xin"], t["xout"], t["b"], t["ec"]]:
