@@ -174,29 +174,28 @@ template <typename A>
 __device__ __forceinline__ A shfl_down1(A v) { return __shfl_down(v, 1, 64); }
 
 // Face coefficients of a run: west face cw[m], east ce[m], south cs, north cn,
-// down cd, up cu.  Every lane executes the shuffle; `valid` gates the loads.
+// down cd, up cu.  Every address is in bounds (lanes past the end of a task
+// carry run 0), so all rows are loaded without branches and waited for
+// together; a boundary selects the boundary-face array instead.
 template <typename C>
 struct RunCoef {
   float cw[8], ce[8], cs[8], cn[8], cd[8], cu[8];
-  __device__ __forceinline__ void load(const hfm_mg_task& t, const Run& r, bool valid) {
+  __device__ __forceinline__ void load(const hfm_mg_task& t, const Run& r) {
     const HFM_GLOBAL C* cx = gin<C>(t.c[0]);
     const HFM_GLOBAL C* cy = gin<C>(t.c[1]);
     const HFM_GLOBAL C* cz = gin<C>(t.c[2]);
-    const int lane = threadIdx.x & 63;
-#pragma unroll
-    for (int m = 0; m < 8; ++m) cw[m] = cs[m] = cn[m] = cd[m] = cu[m] = 0.0f;
-    if (valid) ld8(cx + r.n0, cw);
-    float eface = shfl_down1(cw[0]);
-    if (valid) {
-      if (r.i0 + kRun == t.W) eface = ldc<C>(t.c[3], r.k * t.H + r.j);
-      else if (lane == 63) eface = ldc<C>(cx, r.n0 + kRun);
-      ld8(cy + r.n0, cs);
-      if (r.j + 1 < t.H) ld8(cy + r.n0 + t.W, cn);
-      else ld8(gin<C>(t.c[4]) + r.k * t.W + r.i0, cn);
-      ld8(cz + r.n0, cd);
-      if (r.k + 1 < t.D) ld8(cz + r.n0 + t.H * t.W, cu);
-      else ld8(gin<C>(t.c[5]) + r.j * t.W + r.i0, cu);
-    }
+    const HFM_GLOBAL C* pn =
+        r.j + 1 < t.H ? cy + r.n0 + t.W : gin<C>(t.c[4]) + r.k * t.W + r.i0;
+    const HFM_GLOBAL C* pu =
+        r.k + 1 < t.D ? cz + r.n0 + t.H * t.W : gin<C>(t.c[5]) + r.j * t.W + r.i0;
+    const HFM_GLOBAL C* pe =
+        r.i0 + kRun == t.W ? gin<C>(t.c[3]) + r.k * t.H + r.j : cx + r.n0 + kRun;
+    ld8(cx + r.n0, cw);
+    ld8(cy + r.n0, cs);
+    ld8(pn, cn);
+    ld8(cz + r.n0, cd);
+    ld8(pu, cu);
+    const float eface = ldg(pe);
 #pragma unroll
     for (int m = 0; m < 7; ++m) ce[m] = cw[m + 1];
     ce[7] = eface;
@@ -214,13 +213,7 @@ struct RunCoef {
 template <typename T>
 struct SrcPlain {
   const HFM_GLOBAL T* x;
-  template <typename A>
-  __device__ __forceinline__ void run(int n, int, int, int, A v[8]) const {
-    float f[8];
-    ld8(x + n, f);
-#pragma unroll
-    for (int m = 0; m < 8; ++m) v[m] = (A)f[m];
-  }
+  __device__ __forceinline__ void run(int n, int, int, int, float v[8]) const { ld8(x + n, v); }
   __device__ __forceinline__ float one(int n, int, int, int) const { return ldg(x + n); }
 };
 
@@ -231,13 +224,12 @@ struct SrcCorrect {
   const HFM_GLOBAL T* ec;
   int Hc, Wc;
   float alpha;
-  template <typename A>
-  __device__ __forceinline__ void run(int n, int k, int j, int i0, A v[8]) const {
+  __device__ __forceinline__ void run(int n, int k, int j, int i0, float v[8]) const {
     float f[8], e[4];
     ld8(x + n, f);
     ld4(ec + ((k >> 1) * Hc + (j >> 1)) * Wc + (i0 >> 1), e);
 #pragma unroll
-    for (int m = 0; m < 8; ++m) v[m] = (A)(f[m] + alpha * e[m >> 1]);
+    for (int m = 0; m < 8; ++m) v[m] = f[m] + alpha * e[m >> 1];
   }
   __device__ __forceinline__ float one(int n, int k, int j, int i) const {
     return ldg(x + n) + alpha * ldg(ec + ((k >> 1) * Hc + (j >> 1)) * Wc + (i >> 1));
@@ -253,16 +245,17 @@ struct SrcDir {
   __device__ __forceinline__ float make(float zv, float pv) const {
     return round_to<T>(beta == 0.0f ? zv : fmaf(beta, pv, zv));
   }
-  template <typename A>
-  __device__ __forceinline__ void run(int n, int, int, int, A v[8]) const {
+  // p_old is loaded unconditionally (a buffer of the right size; its value
+  // is discarded when beta == 0) so the loads issue together
+  __device__ __forceinline__ void run(int n, int, int, int, float v[8]) const {
     float zf[8], pf[8];
     ld8(z + n, zf);
-    if (beta != 0.0f) ld8(pold + n, pf);
+    ld8(pold + n, pf);
 #pragma unroll
-    for (int m = 0; m < 8; ++m) v[m] = (A)make(zf[m], beta != 0.0f ? pf[m] : 0.0f);
+    for (int m = 0; m < 8; ++m) v[m] = make(zf[m], pf[m]);
   }
   __device__ __forceinline__ float one(int n, int, int, int) const {
-    return make(ldg(z + n), beta != 0.0f ? ldg(pold + n) : 0.0f);
+    return make(ldg(z + n), ldg(pold + n));
   }
 };
 
@@ -281,56 +274,51 @@ __device__ __forceinline__ float diag_rounded(const hfm_mg_task& t, int n, int i
 }
 
 // xc = x on the run, off = sum of face * neighbour (w, e, s, n, d, u order).
+// The five rows and the two row-end values are loaded first, from in-bounds
+// addresses (a missing neighbour re-reads the run and is zeroed after).
 template <typename A, typename C, typename SRC>
-__device__ __forceinline__ void run_offdiag(const hfm_mg_task& t, const Run& r, bool valid,
+__device__ __forceinline__ void run_offdiag(const hfm_mg_task& t, const Run& r,
                                             const RunCoef<C>& q, const SRC& src, A xc[8],
                                             A off[8]) {
-  const int lane = threadIdx.x & 63;
+  const int W = t.W, HW = t.H * t.W;
+  const bool west = r.i0 > 0, east = r.i0 + kRun < W;
+  const bool south = r.j > 0, north = r.j + 1 < t.H, down = r.k > 0, up = r.k + 1 < t.D;
+  float xf[8], vs[8], vn[8], vd[8], vu[8];
+  src.run(r.n0, r.k, r.j, r.i0, xf);
+  src.run(south ? r.n0 - W : r.n0, r.k, south ? r.j - 1 : r.j, r.i0, vs);
+  src.run(north ? r.n0 + W : r.n0, r.k, north ? r.j + 1 : r.j, r.i0, vn);
+  src.run(down ? r.n0 - HW : r.n0, down ? r.k - 1 : r.k, r.j, r.i0, vd);
+  src.run(up ? r.n0 + HW : r.n0, up ? r.k + 1 : r.k, r.j, r.i0, vu);
+  // the x-neighbours of the run's ends, read directly by every lane (the
+  // adjacent lanes' rows are in the same cache lines)
+  const float wl = src.one(west ? r.n0 - 1 : r.n0, r.k, r.j, west ? r.i0 - 1 : r.i0);
+  const float el = src.one(east ? r.n0 + kRun : r.n0, r.k, r.j, east ? r.i0 + kRun : r.i0);
 #pragma unroll
-  for (int m = 0; m < 8; ++m) xc[m] = (A)0;
-  if (valid) src.run(r.n0, r.k, r.j, r.i0, xc);
-  A wv = shfl_up1(xc[7]);
-  A ev = shfl_down1(xc[0]);
-  if (valid) {
-    if (r.i0 == 0) wv = (A)0;
-    else if (lane == 0) wv = (A)src.one(r.n0 - 1, r.k, r.j, r.i0 - 1);
-    if (r.i0 + kRun == t.W) ev = (A)0;
-    else if (lane == 63) ev = (A)src.one(r.n0 + kRun, r.k, r.j, r.i0 + kRun);
-  }
+  for (int m = 0; m < 8; ++m) xc[m] = (A)xf[m];
+  const A wv = west ? (A)wl : (A)0;
+  const A ev = east ? (A)el : (A)0;
 #pragma unroll
   for (int m = 0; m < 8; ++m) off[m] = (A)q.cw[m] * (m ? xc[m - 1] : wv);
 #pragma unroll
   for (int m = 0; m < 8; ++m) off[m] += (A)q.ce[m] * (m < 7 ? xc[m + 1] : ev);
-  A v[8];
-  const int W = t.W, HW = t.H * t.W;
 #pragma unroll
-  for (int m = 0; m < 8; ++m) v[m] = (A)0;
-  if (valid && r.j > 0) src.run(r.n0 - W, r.k, r.j - 1, r.i0, v);
+  for (int m = 0; m < 8; ++m) off[m] += (A)q.cs[m] * (south ? (A)vs[m] : (A)0);
 #pragma unroll
-  for (int m = 0; m < 8; ++m) off[m] += (A)q.cs[m] * v[m];
+  for (int m = 0; m < 8; ++m) off[m] += (A)q.cn[m] * (north ? (A)vn[m] : (A)0);
 #pragma unroll
-  for (int m = 0; m < 8; ++m) v[m] = (A)0;
-  if (valid && r.j + 1 < t.H) src.run(r.n0 + W, r.k, r.j + 1, r.i0, v);
+  for (int m = 0; m < 8; ++m) off[m] += (A)q.cd[m] * (down ? (A)vd[m] : (A)0);
 #pragma unroll
-  for (int m = 0; m < 8; ++m) off[m] += (A)q.cn[m] * v[m];
-#pragma unroll
-  for (int m = 0; m < 8; ++m) v[m] = (A)0;
-  if (valid && r.k > 0) src.run(r.n0 - HW, r.k - 1, r.j, r.i0, v);
-#pragma unroll
-  for (int m = 0; m < 8; ++m) off[m] += (A)q.cd[m] * v[m];
-#pragma unroll
-  for (int m = 0; m < 8; ++m) v[m] = (A)0;
-  if (valid && r.k + 1 < t.D) src.run(r.n0 + HW, r.k + 1, r.j, r.i0, v);
-#pragma unroll
-  for (int m = 0; m < 8; ++m) off[m] += (A)q.cu[m] * v[m];
+  for (int m = 0; m < 8; ++m) off[m] += (A)q.cu[m] * (up ? (A)vu[m] : (A)0);
 }
 
 __device__ __forceinline__ bool vec_task(const hfm_mg_task& t) { return (t.W & (kRun - 1)) == 0; }
 
+// The run starting at n0, or run 0 for a lane past the end of its task (its
+// loads stay in bounds; its results are not stored).
 __device__ __forceinline__ Run run_at(const hfm_mg_task& t, int n0, bool valid) {
   Run r;
-  r.n0 = n0;
-  decompose(t, valid ? n0 : 0, r.i0, r.j, r.k);
+  r.n0 = valid ? n0 : 0;
+  decompose(t, r.n0, r.i0, r.j, r.k);
   return r;
 }
 
@@ -360,19 +348,19 @@ __global__ __launch_bounds__(B) void k_sweep(const hfm_mg_task* __restrict__ tas
     const bool valid = n0 < N;
     const Run r = run_at(t, n0, valid);
     RunCoef<T> q;
-    q.load(t, r, valid);
+    q.load(t, r);
     float d[8], bv[8], o[8];
+    ld8_nt(b + r.n0, bv);
     q.diag(d);
-    if (valid) ld8_nt(b + n0, bv);
     if (MODE == HFM_SWEEP_ZERO) {
 #pragma unroll
       for (int m = 0; m < 8; ++m) o[m] = omega * bv[m] / d[m];
     } else {
       float xc[8], off[8];
       if (MODE == HFM_SWEEP_PLAIN) {
-        run_offdiag<float>(t, r, valid, q, SrcPlain<T>{xin}, xc, off);
+        run_offdiag<float>(t, r, q, SrcPlain<T>{xin}, xc, off);
       } else {
-        run_offdiag<float>(t, r, valid, q, SrcCorrect<T>{xin, ec, t.Hc, t.Wc, alpha}, xc, off);
+        run_offdiag<float>(t, r, q, SrcCorrect<T>{xin, ec, t.Hc, t.Wc, alpha}, xc, off);
       }
 #pragma unroll
       for (int m = 0; m < 8; ++m) o[m] = xc[m] + omega * (bv[m] - (d[m] * xc[m] - off[m])) / d[m];
@@ -452,26 +440,30 @@ __global__ __launch_bounds__(B) void k_resid_restrict(const hfm_mg_task* __restr
       J = int(rowc) - K * t.Hc;
     }
     float s[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
     for (int a = 0; a < 2; ++a) {
+#pragma unroll
       for (int bb = 0; bb < 2; ++bb) {
         const int k = 2 * K + a, j = 2 * J + bb;
         const bool valid = cvalid && k < D && j < H;
-        Run r;
-        r.i0 = 2 * I0;
-        r.j = j;
-        r.k = k;
-        r.n0 = (k * H + j) * W + r.i0;
+        Run r;  // a missing child row (odd extent, tail lane) reads run 0, adds nothing
+        r.i0 = valid ? 2 * I0 : 0;
+        r.j = valid ? j : 0;
+        r.k = valid ? k : 0;
+        r.n0 = (r.k * H + r.j) * W + r.i0;
         RunCoef<T> q;
-        q.load(t, r, valid);
+        q.load(t, r);
         float d[8], xc[8], off[8], bv[8];
+        ld8(b + r.n0, bv);
         q.diag(d);
-        run_offdiag<float>(t, r, valid, q, SrcPlain<T>{x}, xc, off);
-        if (valid) {
-          ld8(b + r.n0, bv);
+        run_offdiag<float>(t, r, q, SrcPlain<T>{x}, xc, off);
 #pragma unroll
-          for (int m = 0; m < 4; ++m) {
-            s[m] += bv[2 * m] - (d[2 * m] * xc[2 * m] - off[2 * m]);
-            s[m] += bv[2 * m + 1] - (d[2 * m + 1] * xc[2 * m + 1] - off[2 * m + 1]);
+        for (int m = 0; m < 4; ++m) {
+          const float r2 = bv[2 * m] - (d[2 * m] * xc[2 * m] - off[2 * m]);
+          const float r3 = bv[2 * m + 1] - (d[2 * m + 1] * xc[2 * m + 1] - off[2 * m + 1]);
+          if (valid) {
+            s[m] += r2;
+            s[m] += r3;
           }
         }
       }
@@ -597,10 +589,10 @@ __global__ __launch_bounds__(kBlock) void k_fcg_dir(const hfm_mg_task* __restric
     const bool valid = n0 < N;
     const Run r = run_at(t, n0, valid);
     RunCoef<float> q;
-    q.load(t, r, valid);
+    q.load(t, r);
     float d[8], p[8], off[8];
     q.diag(d);
-    run_offdiag<float>(t, r, valid, q, src, p, off);
+    run_offdiag<float>(t, r, q, src, p, off);
     if (valid) {
 #pragma unroll
       for (int m = 0; m < 8; ++m) acc += double(p[m] * (d[m] * p[m] - off[m]));
@@ -653,15 +645,15 @@ __global__ __launch_bounds__(kBlock) void k_fcg_update(const hfm_mg_task* __rest
     const bool valid = n0 < N;
     const Run r = run_at(t, n0, valid);
     RunCoef<float> q;
-    q.load(t, r, valid);
+    q.load(t, r);
+    double xv[8], rv[8];
+    ld8d(x64 + r.n0, xv);
+    ld8d(r64 + r.n0, rv);
     double d[8], p[8], off[8];
     q.diag(d);
-    run_offdiag<double>(t, r, valid, q, src, p, off);
+    run_offdiag<double>(t, r, q, src, p, off);
     if (valid) {
-      double xv[8], rv[8];
       float rl[8];
-      ld8d(x64 + n0, xv);
-      ld8d(r64 + n0, rv);
 #pragma unroll
       for (int m = 0; m < 8; ++m) {
         xv[m] += alpha * p[m];
