@@ -322,6 +322,63 @@ __device__ __forceinline__ Run run_at(const hfm_mg_task& t, int n0, bool valid) 
   return r;
 }
 
+// ------------------------------------------------------------------ per-cell forms
+// The sweep of cell n (index (k, j, i)) before rounding: ZERO omega b/d, PLAIN
+// x + omega (b - A x)/d, CORRECT the same with x := xin + alpha P ec.
+template <typename T, int MODE>
+__device__ __forceinline__ float sweep_one(const hfm_mg_task& t, int n, int i, int j, int k,
+                                           const HFM_GLOBAL T* xin, const HFM_GLOBAL T* ec,
+                                           float bn, float omega, float alpha) {
+  float diag;
+  if (MODE == HFM_SWEEP_ZERO) {
+    face_row<float, T>(t, n, i, j, k, [](int, int, int, int) { return 0.0f; }, diag);
+    return omega * bn / diag;
+  } else if (MODE == HFM_SWEEP_PLAIN) {
+    auto xat = [&](int m, int, int, int) { return ldg(xin + m); };
+    const float off = face_row<float, T>(t, n, i, j, k, xat, diag);
+    const float xc = ldg(xin + n);
+    return xc + omega * (bn - (diag * xc - off)) / diag;
+  } else {
+    const int Hc = t.Hc, Wc = t.Wc;
+    auto xat = [&](int m, int kk, int jj, int ii) {
+      return ldg(xin + m) + alpha * ldg(ec + ((kk >> 1) * Hc + (jj >> 1)) * Wc + (ii >> 1));
+    };
+    const float off = face_row<float, T>(t, n, i, j, k, xat, diag);
+    const float xc = xat(n, k, j, i);
+    return xc + omega * (bn - (diag * xc - off)) / diag;
+  }
+}
+
+// Coarse cell nc of R (b - A x): the sum over its (up to 2x2x2) children, / 8.
+template <typename T>
+__device__ __forceinline__ float restrict_one(const hfm_mg_task& t, int nc,
+                                              const HFM_GLOBAL T* x, const HFM_GLOBAL T* b) {
+  const int W = t.W, H = t.H, D = t.D;
+  const uint32_t rowc = fdiv(uint32_t(nc), t.magWc);
+  const int I = nc - int(rowc) * t.Wc;
+  const int K = int(fdiv(rowc, t.magHc));
+  const int J = int(rowc) - K * t.Hc;
+  auto xat = [&](int m, int, int, int) { return ldg(x + m); };
+  float sum = 0.0f;
+  for (int a = 0; a < 2; ++a) {
+    const int k = 2 * K + a;
+    if (k >= D) break;
+    for (int bb = 0; bb < 2; ++bb) {
+      const int j = 2 * J + bb;
+      if (j >= H) break;
+      for (int c = 0; c < 2; ++c) {
+        const int i = 2 * I + c;
+        if (i >= W) break;
+        const int n = (k * H + j) * W + i;
+        float diag;
+        const float off = face_row<float, T>(t, n, i, j, k, xat, diag);
+        sum += ldg(b + n) - (diag * ldg(x + n) - off);
+      }
+    }
+  }
+  return sum * 0.125f;
+}
+
 // ------------------------------------------------------------------ sweeps
 // T: storage of the level's vectors; O: storage of this sweep's output (the
 // fine level's last post-sweep may write the preconditioned residual z in a
@@ -379,27 +436,7 @@ __global__ __launch_bounds__(B) void k_sweep(const hfm_mg_task* __restrict__ tas
       int i, j, k;
       decompose(t, n, i, j, k);
       const float bn = ldg(b + n);
-      float r;
-      if (MODE == HFM_SWEEP_ZERO) {
-        float diag;
-        face_row<float, T>(t, n, i, j, k, [](int, int, int, int) { return 0.0f; }, diag);
-        r = omega * bn / diag;
-      } else if (MODE == HFM_SWEEP_PLAIN) {
-        auto xat = [&](int m, int, int, int) { return ldg(xin + m); };
-        float diag;
-        const float off = face_row<float, T>(t, n, i, j, k, xat, diag);
-        const float xc = ldg(xin + n);
-        r = xc + omega * (bn - (diag * xc - off)) / diag;
-      } else {
-        const int Hc = t.Hc, Wc = t.Wc;
-        auto xat = [&](int m, int kk, int jj, int ii) {
-          return ldg(xin + m) + alpha * ldg(ec + ((kk >> 1) * Hc + (jj >> 1)) * Wc + (ii >> 1));
-        };
-        float diag;
-        const float off = face_row<float, T>(t, n, i, j, k, xat, diag);
-        const float xc = xat(n, k, j, i);
-        r = xc + omega * (bn - (diag * xc - off)) / diag;
-      }
+      const float r = sweep_one<T, MODE>(t, n, i, j, k, xin, ec, bn, omega, alpha);
       const O rs = from_f<O>(r);
       put(out + n, rs);
       if (DOT) acc += double(bn) * double(to_f(rs));
@@ -477,32 +514,10 @@ __global__ __launch_bounds__(B) void k_resid_restrict(const hfm_mg_task* __restr
     return;
   }
   if (base >= Nc) return;
-  auto xat = [&](int m, int, int, int) { return ldg(x + m); };
   for (int s = 0; s < 4; ++s) {
     const int nc = base + s * B + threadIdx.x;
     if (nc >= Nc) break;
-    const uint32_t rowc = fdiv(uint32_t(nc), t.magWc);
-    const int I = nc - int(rowc) * t.Wc;
-    const int K = int(fdiv(rowc, t.magHc));
-    const int J = int(rowc) - K * t.Hc;
-    float sum = 0.0f;
-    for (int a = 0; a < 2; ++a) {
-      const int k = 2 * K + a;
-      if (k >= D) break;
-      for (int bb = 0; bb < 2; ++bb) {
-        const int j = 2 * J + bb;
-        if (j >= H) break;
-        for (int c = 0; c < 2; ++c) {
-          const int i = 2 * I + c;
-          if (i >= W) break;
-          const int n = (k * H + j) * W + i;
-          float diag;
-          const float off = face_row<float, T>(t, n, i, j, k, xat, diag);
-          sum += ldg(b + n) - (diag * ldg(x + n) - off);
-        }
-      }
-    }
-    put(bc + nc, from_f<T>(sum * 0.125f));
+    put(bc + nc, from_f<T>(restrict_one<T>(t, nc, x, b)));
   }
 }
 
@@ -524,6 +539,101 @@ __global__ __launch_bounds__(64) void k_coarse_solve(const hfm_mg_task* __restri
     double e = 0.0;
     for (int j = 0; j < N; ++j) e += ainv[j * N + i] * bs[j];
     put(gout<T>(t.xout) + i, from_f<T>(float(e)));
+  }
+}
+
+// ------------------------------------------------------------------ coarse-level cycle
+// One workgroup per instance runs the V-cycle's tail: every level from the
+// chain's first (its rhs B already restricted into place) to the coarsest and
+// back, with barriers between passes, so the small levels cost one launch
+// instead of ~5 per level.  Record q of an instance's chain describes level
+// lc + q: D/H/W, Dc/Hc/Wc + magics of the next level, c = storage faces,
+// xin = X, xout = Y (the sweep ping-pong), b = B, ec = the next level's B (the
+// restriction target), or, for the coarsest, the dense FP64 inverse.  The
+// buffers and the per-cell arithmetic are those of the per-level launches
+// (sweep k writes (X, Y)[k % 2]; the level's result ends in
+// (X, Y)[(nu1 + nu2 - 1) % 2], the coarsest's in X).
+constexpr int kCycleThreads = 1024;
+
+template <typename T>
+__global__ __launch_bounds__(kCycleThreads) void k_coarse_cycle(
+    const hfm_mg_task* __restrict__ recs, const int* __restrict__ chain,
+    const hfm_mg_scalars* __restrict__ sc, int nu1, int nu2, float omega, float alpha) {
+  const int first = chain[2 * blockIdx.x], count = chain[2 * blockIdx.x + 1];
+  if (count < 1 || !sc[recs[first].inst].active) return;
+  const int tid = threadIdx.x;
+  // xin and ec are read-only for the per-level kernels; here X and the next
+  // level's B are written as well
+  auto X = [&](int q) { return gout<T>(const_cast<void*>(recs[first + q].xin)); };
+  auto Y = [&](int q) { return gout<T>(recs[first + q].xout); };
+  auto buf = [&](int q, int kk) { return (kk & 1) ? Y(q) : X(q); };
+  const int last = count - 1;
+  // down
+  for (int q = 0; q < last; ++q) {
+    const hfm_mg_task& t = recs[first + q];
+    const int N = t.D * t.H * t.W;
+    const HFM_GLOBAL T* b = gin<T>(t.b);
+    for (int kk = 0; kk < nu1; ++kk) {
+      const HFM_GLOBAL T* xin = buf(q, kk - 1);
+      HFM_GLOBAL T* out = buf(q, kk);
+      for (int n = tid; n < N; n += kCycleThreads) {
+        int i, j, k;
+        decompose(t, n, i, j, k);
+        const float bn = ldg(b + n);
+        const float r = kk == 0 ? sweep_one<T, HFM_SWEEP_ZERO>(t, n, i, j, k, xin, nullptr, bn,
+                                                               omega, alpha)
+                                : sweep_one<T, HFM_SWEEP_PLAIN>(t, n, i, j, k, xin, nullptr, bn,
+                                                                omega, alpha);
+        put(out + n, from_f<T>(r));
+      }
+      __syncthreads();
+    }
+    const HFM_GLOBAL T* x = buf(q, nu1 - 1);
+    HFM_GLOBAL T* bc = gout<T>(const_cast<void*>(t.ec));
+    const int Nc = t.Dc * t.Hc * t.Wc;
+    for (int nc = tid; nc < Nc; nc += kCycleThreads) put(bc + nc, from_f<T>(restrict_one<T>(t, nc, x, b)));
+    __syncthreads();
+  }
+  // coarsest: x = Ainv b
+  {
+    const hfm_mg_task& t = recs[first + last];
+    const int N = t.D * t.H * t.W;
+    if (N <= 64) {
+      const HFM_GLOBAL double* ainv = gin<double>(t.ec);
+      const HFM_GLOBAL T* b = gin<T>(t.b);
+      __shared__ double bs[64];
+      if (tid < N) bs[tid] = double(ldg(b + tid));
+      __syncthreads();
+      if (tid < N) {
+        double e = 0.0;
+        for (int jj = 0; jj < N; ++jj) e += ainv[jj * N + tid] * bs[jj];
+        put(X(last) + tid, from_f<T>(float(e)));
+      }
+    }
+    __syncthreads();
+  }
+  // up
+  for (int q = last - 1; q >= 0; --q) {
+    const hfm_mg_task& t = recs[first + q];
+    const int N = t.D * t.H * t.W;
+    const HFM_GLOBAL T* b = gin<T>(t.b);
+    const HFM_GLOBAL T* ec = q + 1 == last ? X(last) : buf(q + 1, nu1 + nu2 - 1);
+    for (int s2 = 0; s2 < nu2; ++s2) {
+      const int kk = nu1 + s2;
+      const HFM_GLOBAL T* xin = buf(q, kk - 1);
+      HFM_GLOBAL T* out = buf(q, kk);
+      for (int n = tid; n < N; n += kCycleThreads) {
+        int i, j, k;
+        decompose(t, n, i, j, k);
+        const float bn = ldg(b + n);
+        const float r = s2 == 0 ? sweep_one<T, HFM_SWEEP_CORRECT>(t, n, i, j, k, xin, ec, bn,
+                                                                  omega, alpha)
+                                : sweep_one<T, HFM_SWEEP_PLAIN>(t, n, i, j, k, xin, nullptr, bn,
+                                                                omega, alpha);
+        put(out + n, from_f<T>(r));
+      }
+      __syncthreads();
+    }
   }
 }
 
@@ -1090,6 +1200,21 @@ extern "C" int hfm_mg_coarse_solve(const hfm_mg_task* tasks, int64_t ntasks,
     using T = decltype(tag);
     hipLaunchKernelGGL(k_coarse_solve<T>, dim3(unsigned(ntasks)), dim3(64), 0,
                        as_stream(stream), tasks, sc);
+  });
+}
+
+extern "C" int hfm_mg_coarse_cycle(const hfm_mg_task* recs, const int32_t* chain,
+                                   int64_t ninst, const hfm_mg_scalars* sc, int store, int nu1,
+                                   int nu2, float omega, float alpha, void* stream) {
+  if (!recs || !chain || !sc || ninst < 0 || ninst > INT32_MAX || nu1 < 1 || nu2 < 1) {
+    set_error("hfm_mg_coarse_cycle: bad arguments");
+    return HFM_EINVAL;
+  }
+  if (ninst == 0) return HFM_OK;
+  return launch_store(store, "hfm_mg_coarse_cycle", [&](auto tag) {
+    using T = decltype(tag);
+    hipLaunchKernelGGL(k_coarse_cycle<T>, dim3(unsigned(ninst)), dim3(kCycleThreads), 0,
+                       as_stream(stream), recs, chain, sc, nu1, nu2, omega, alpha);
   });
 }
 

@@ -73,6 +73,7 @@ SIGNATURES = {
     "hfm_mg_sweep": (_int, [_vp, _vp, _i64, _vp, _int, _int, _int, _f32, _f32, _int, _vp, _vp]),
     "hfm_mg_resid_restrict": (_int, [_vp, _vp, _i64, _vp, _int, _int, _vp]),
     "hfm_mg_coarse_solve": (_int, [_vp, _i64, _vp, _int, _vp]),
+    "hfm_mg_coarse_cycle": (_int, [_vp, _vp, _i64, _vp, _int, _int, _int, _f32, _f32, _vp]),
     "hfm_fcg_init": (_int, [_vp, _vp, _i64, _vp, _int, _vp, _vp]),
     "hfm_fcg_dir": (_int, [_vp, _vp, _i64, _vp, _int, _vp, _vp]),
     "hfm_fcg_update": (_int, [_vp, _vp, _i64, _vp, _int, _int, _vp, _vp]),

@@ -92,6 +92,9 @@ class Params:
     # threads per block of the launches without reductions, keyed
     # ("sweep", mode, level) / ("restrict", level); default BLOCK (hfm.tune)
     blocks: Optional[dict] = None
+    # levels (below the finest) of at most this many cells run in one
+    # workgroup per instance (hfm_mg_coarse_cycle); the coarsest always does
+    coarse_cells: int = 4096
 
     def __post_init__(self):
         if self.storage not in ELEM:
@@ -334,6 +337,27 @@ def _record(inst: Instance, lev: int, coarse=None):
     return t
 
 
+def chain_start(levels, coarse_cells: int) -> int:
+    """First level of an instance's one-launch coarse cycle (at least 1)."""
+    for l in range(1, len(levels)):
+        if math.prod(levels[l]) <= coarse_cells:
+            return l
+    return len(levels) - 1
+
+
+def chain_records(inst: "Instance", lc: int) -> list:
+    """Records of levels lc .. coarsest for hfm_mg_coarse_cycle."""
+    recs = []
+    L_ = len(inst.levels)
+    for l in range(lc, L_):
+        t = _record(inst, l, coarse=inst.levels[l + 1] if l + 1 < L_ else None)
+        t["c"] = inst.faces(l, True)
+        t["xin"], t["xout"], t["b"] = inst.X[l].data_ptr(), inst.Y[l].data_ptr(), inst.B[l].data_ptr()
+        t["ec"] = inst.B[l + 1].data_ptr() if l + 1 < L_ else inst.ainv.data_ptr()
+        recs.append(t)
+    return recs
+
+
 def _table(device, recs, sizes, chunk):
     """Device task table + block layout for the records (one per member)."""
     arr = np.array(recs, dtype=L.TASK_DTYPE)
@@ -489,9 +513,11 @@ class Plan:
                 inst._tpl = instance_templates(inst, p)
         nlev = max(len(i.levels) for i in insts)
         self.nlev = nlev
+        lcs = {i.slot: chain_start(i.levels, p.coarse_cells) for i in insts}
 
-        def op(key, chunk, size_of):
-            members = [i for i in insts if key in i._tpl]
+        def op(key, chunk, size_of, lev=None):
+            # per-level launches stop where an instance's coarse cycle starts
+            members = [i for i in insts if key in i._tpl and (lev is None or lev < lcs[i.slot])]
             if not members:
                 return None
             return _table(dev, [i._tpl[key] for i in members], [size_of(i) for i in members],
@@ -506,11 +532,12 @@ class Plan:
                     continue    # computed by the FCG update (and init) kernels
                 mode = "zero" if s == 0 else "plain"
                 b = blocks.get(("sweep", mode, lev), BLOCK)
-                tab = op(("down", lev, s), 8 * b, cells)
+                tab = op(("down", lev, s), 8 * b, cells, lev)
                 if tab:
                     self.ops_down.append(("sweep", L.SWEEP[mode], tab[0], tab[1], False, lev, b))
             b = blocks.get(("restrict", lev), BLOCK)
-            tab = op(("restrict", lev), 4 * b, lambda i, lev=lev: math.prod(i.levels[lev + 1]))
+            tab = op(("restrict", lev), 4 * b, lambda i, lev=lev: math.prod(i.levels[lev + 1]),
+                     lev)
             if tab:
                 self.ops_down.append(("restrict", 0, tab[0], tab[1], False, lev, b))
             ups = []
@@ -518,13 +545,22 @@ class Plan:
                 mode = "correct" if s == 0 else "plain"
                 dot = lev == 0 and s == p.nu2 - 1
                 b = BLOCK if dot else blocks.get(("sweep", mode, lev), BLOCK)
-                tab = op(("up", lev, s), 8 * b, cells)
+                tab = op(("up", lev, s), 8 * b, cells, lev)
                 if tab:
                     ups.append(("sweep", L.SWEEP[mode], tab[0], tab[1], dot, lev, b))
             self.ops_up.insert(0, ups)
         self.coarse_tasks = torch.from_numpy(
             np.array([i._tpl[("coarse",)] for i in insts], dtype=L.TASK_DTYPE).view(np.uint8)
             .copy()).to(dev)
+        recs, chain = [], []
+        for i in insts:
+            r = chain_records(i, lcs[i.slot])
+            chain += [len(recs), len(r)]
+            recs += r
+        self.chain_recs = torch.from_numpy(np.array(recs, dtype=L.TASK_DTYPE).view(np.uint8)
+                                           .copy()).to(dev)
+        self.chain = torch.tensor(chain, dtype=torch.int32).to(dev)
+        self.chain_levels = sorted(set(lcs.values()))
         fine = lambda i: i.N  # noqa: E731
         self.init_tasks, self.flay = op(("init",), CHUNK, fine)
         self.dir_tasks = [op(("dir", par), CHUNK, fine)[0] for par in range(2)]
@@ -556,11 +592,17 @@ class Plan:
         """z = V(rlow): one V-cycle over every instance of the plan."""
         for op in self.ops_down:
             self._launch(op, s)
-        L.call("hfm_mg_coarse_solve", self.coarse_tasks.data_ptr(), len(self.insts),
-               self.pool.sc.data_ptr(), self.pool.store_id, s)
+        self.launch_coarse_cycle(s)
         for ups in self.ops_up:
             for op in ups:
                 self._launch(op, s)
+
+    def launch_coarse_cycle(self, s):
+        """The V-cycle below each instance's chain start, one workgroup per instance."""
+        p = self.pool.params
+        L.call("hfm_mg_coarse_cycle", self.chain_recs.data_ptr(), self.chain.data_ptr(),
+               len(self.insts), self.pool.sc.data_ptr(), self.pool.store_id, p.nu1, p.nu2,
+               p.omega, p.alpha, s)
 
     def launch_dir(self, parity, s):
         pool, fl = self.pool, self.flay
@@ -611,9 +653,10 @@ class MGBatch:
                  maxit: int = 200, nu1: int = 2, nu2: int = 2, omega: float = 2.0 / 3.0,
                  alpha: float = 1.6, max_coarse_edge: int = 4, device=None,
                  iters_per_graph: int = 4, use_graph: bool = True, shrink: float = 0.5,
-                 outer: Optional[str] = None, blocks: Optional[dict] = None):
+                 outer: Optional[str] = None, blocks: Optional[dict] = None,
+                 coarse_cells: int = 4096):
         self.params = Params(storage, tol, maxit, nu1, nu2, omega, alpha, max_coarse_edge, outer,
-                             blocks)
+                             blocks, coarse_cells)
         shapes = [check_shape(s, max_coarse_edge)[0] for s in shapes]
         if not shapes:
             raise ValueError("empty batch")
@@ -673,8 +716,16 @@ class MGBatch:
             if b not in BLOCKS:
                 raise ValueError(f"block size {b} for {k} is not one of {BLOCKS}")
         self.params.blocks = dict(blocks) if blocks else None
-        for inst in self.insts:
-            inst._tpl = None
+        self._replan()
+
+    def set_coarse_cells(self, n: int):
+        """Run levels of at most ``n`` cells in the one-launch coarse cycle; rebuilds the plans."""
+        if int(n) < 0:
+            raise ValueError("coarse_cells must be >= 0")
+        self.params.coarse_cells = int(n)
+        self._replan()
+
+    def _replan(self):
         self.plan = Plan(self.pool, self.insts)
         self._subplans = {}
 

@@ -155,7 +155,7 @@ class UrbanProjection(UrbanFlow):
 
     def __init__(self, shape, sigma=None, inflow=None, dt: float = 0.4, dx: float = 1.0,
                  Re: float = 1 / 20, storage: str = "bf16", tol: float = 1e-8,
-                 lscalar: bool = True, device=None, outer=None):
+                 lscalar: bool = True, device=None, outer=None, tune: bool = True):
         from .mg import MGBatch
         super().__init__(shape, sigma=sigma, dt=dt, dx=dx, Re=Re, ub=1.0, multi_itr=0,
                          lscalar=lscalar, device=device)
@@ -173,6 +173,10 @@ class UrbanProjection(UrbanFlow):
                               device=self.device, outer=outer)
         self.solver.set_faces(0, faces, torch.zeros(self.shape))
         self.solver.setup()
+        self.tuning = None
+        if tune:   # launch shapes and coarse-cycle threshold for this grid (~0.3 s)
+            from .tune import tune as _tune
+            self.tuning = _tune(self.solver)
         self.last_info = None
 
     def _bcv(self, u, v, w):

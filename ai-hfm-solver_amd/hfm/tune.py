@@ -1,5 +1,12 @@
 """Launch-shape autotuner for the batched multigrid plans.
 
+Two choices, both timed on the batch's own data with HIP events:
+
+* the level size below which an instance's V-cycle tail runs in one
+  workgroup-per-instance launch (``hfm_mg_coarse_cycle``) instead of one
+  launch per pass and level (``tune_coarse``, whole V-cycles timed);
+* the threads per block of the remaining per-level launches (``tune_blocks``).
+
 The sweeps without a reduction (every V-cycle sweep but the fine level's
 last, which carries the r.z partials) and the residual + restriction launches
 accept 128, 256 or 512 threads per block; a block covers 8 (sweeps) or 4
@@ -90,6 +97,30 @@ def tune_blocks(mg: MGBatch, reps: int = 5, candidates=BLOCKS, min_ms: float = 0
     return choice
 
 
+def tune_coarse(mg: MGBatch, reps: int = 5, candidates=(64, 512, 4096, 32768)) -> int:
+    """Choose the size below which a level joins the one-launch coarse cycle by
+    timing whole V-cycles (all instances active) for each candidate."""
+    mg._need_gpu()
+    mg.restart()
+    s = L.stream_handle(mg.device)
+    t = {}
+    for n in candidates:
+        mg.params.coarse_cells = int(n)
+        plan = Plan(mg.pool, mg.insts)
+        t[n] = _time_ms(lambda: plan.precondition(s), reps)
+    best = min(t, key=t.get)
+    mg.set_coarse_cells(best)
+    mg.coarse_times = t
+    return best
+
+
+def tune(mg: MGBatch, reps: int = 5) -> dict:
+    """The coarse-cycle threshold, then the block sizes of the remaining launches."""
+    n = tune_coarse(mg, reps)
+    blocks = tune_blocks(mg, reps)
+    return {"coarse_cells": n, "blocks": blocks}
+
+
 def batch_class(mg: MGBatch) -> str:
     """Size class of a batch: storage, outer type and log2 of its fine cells."""
     cells = sum(i.N for i in mg.insts)
@@ -112,27 +143,30 @@ class TuneCache:
     def _dev(mg: MGBatch) -> str:
         return torch.cuda.get_device_name(mg.device)
 
-    def get(self, mg: MGBatch) -> Optional[Dict[Key, int]]:
+    def get(self, mg: MGBatch) -> Optional[dict]:
         rec = self.data.get(self._dev(mg), {}).get(batch_class(mg))
         if rec is None:
             return None
-        return {tuple(json.loads(k)): int(v) for k, v in rec.items()}
+        return {"coarse_cells": int(rec["coarse_cells"]),
+                "blocks": {tuple(json.loads(k)): int(v) for k, v in rec["blocks"].items()}}
 
-    def put(self, mg: MGBatch, choice: Dict[Key, int]):
+    def put(self, mg: MGBatch, choice: dict):
         self.data.setdefault(self._dev(mg), {})[batch_class(mg)] = {
-            json.dumps(list(k)): int(v) for k, v in choice.items()}
+            "coarse_cells": int(choice["coarse_cells"]),
+            "blocks": {json.dumps(list(k)): int(v) for k, v in choice["blocks"].items()}}
         os.makedirs(os.path.dirname(self.path) or ".", exist_ok=True)
         tmp = self.path + ".tmp"
         with open(tmp, "w") as f:
             json.dump(self.data, f, indent=1, sort_keys=True)
         os.replace(tmp, self.path)
 
-    def tuned(self, mg: MGBatch, reps: int = 5) -> Dict[Key, int]:
-        """The cached table for ``mg``'s class if any (installed), else tune and store."""
+    def tuned(self, mg: MGBatch, reps: int = 5) -> dict:
+        """The cached choice for ``mg``'s class if any (installed), else tune and store."""
         choice = self.get(mg)
         if choice is not None:
-            mg.set_blocks(choice)
+            mg.params.coarse_cells = choice["coarse_cells"]
+            mg.set_blocks(choice["blocks"])
             return choice
-        choice = tune_blocks(mg, reps)
+        choice = tune(mg, reps)
         self.put(mg, choice)
         return choice

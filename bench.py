@@ -324,9 +324,8 @@ def profile_iteration(mg, cells, torch):
         kind, mode, lev = op[0], op[1], op[5]
         label = f"L{lev} " + ("restrict" if kind == "restrict" else "sweep " + names[mode])
         rows.append((label, kernel_time_ms(lambda op=op: plan._launch(op, s), 5, torch)))
-    rows.append(("coarse solve", kernel_time_ms(
-        lambda: L.call("hfm_mg_coarse_solve", plan.coarse_tasks.data_ptr(), len(plan.insts),
-                       pool.sc.data_ptr(), pool.store_id, s), 5, torch)))
+    rows.append((f"coarse cycle (levels >= {plan.chain_levels})",
+                 kernel_time_ms(lambda: plan.launch_coarse_cycle(s), 5, torch)))
     rows.append(("fcg dir", kernel_time_ms(lambda: plan.launch_dir(0, s), 5, torch)))
     rows.append(("fcg update", kernel_time_ms(lambda: plan.launch_update(0, s), 5, torch)))
     rows.append(("finalize", kernel_time_ms(lambda: pool.finalize("check", s), 5, torch)))
@@ -415,11 +414,13 @@ def profile_kernel(args, dev, torch):
     if args.profile_kernel == "all":
         profile_iteration(mg, cells, torch)
         if not args.no_tune:
-            from hfm.tune import tune_blocks
-            choice = tune_blocks(mg)
-            print(json.dumps({"tune_ms": {"/".join(map(str, k)): {str(b): round(t, 4) for b, t in v.items()}
+            from hfm.tune import tune
+            choice = tune(mg)
+            print(json.dumps({"coarse_cycle_ms": {str(k): round(v, 4) for k, v in mg.coarse_times.items()},
+                              "coarse_cells": choice["coarse_cells"],
+                              "tune_ms": {"/".join(map(str, k)): {str(b): round(t, 4) for b, t in v.items()}
                                           for k, v in sorted(mg.tune_times.items())},
-                              "choice": {"/".join(map(str, k)): v for k, v in sorted(choice.items())}}),
+                              "blocks": {"/".join(map(str, k)): v for k, v in sorted(choice["blocks"].items())}}),
                   flush=True)
             mg.restart()
             profile_iteration(mg, cells, torch)
@@ -491,13 +492,15 @@ def main():
     load_batch(mg, specs, dev)
     tuning = None
     if not args.no_tune:
-        from hfm.tune import tune_blocks
+        from hfm.tune import tune
         mg.build_faces()
         mg.setup()
         t0 = time.perf_counter()
-        choice = tune_blocks(mg)
+        choice = tune(mg)
         tuning = {"tune_s": round(time.perf_counter() - t0, 2),
-                  "blocks": {"/".join(map(str, k)): v for k, v in sorted(choice.items())
+                  "coarse_cells": choice["coarse_cells"],
+                  "coarse_cycle_ms": {str(k): round(v, 4) for k, v in mg.coarse_times.items()},
+                  "blocks": {"/".join(map(str, k)): v for k, v in sorted(choice["blocks"].items())
                              if v != 256}}
         log(f"bench: launch tuning {tuning}")
     torch.cuda.synchronize()

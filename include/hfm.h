@@ -222,6 +222,21 @@ int hfm_mg_resid_restrict(const hfm_mg_task* tasks, const uint16_t* blk_task, in
 int hfm_mg_coarse_solve(const hfm_mg_task* tasks, int64_t ntasks, const hfm_mg_scalars* sc,
                         int store, void* stream);
 
+/* The V-cycle's tail in one launch: one 1024-thread workgroup per instance
+ * runs V(nu1, nu2) from level lc (whose rhs B the caller's restriction wrote)
+ * down to the coarsest (dense inverse) and back up, leaving level lc's
+ * correction where the per-level launches would.  chain[2 g], chain[2 g + 1]:
+ * the first record and the number of levels of instance g (device int32s);
+ * record q: level lc + q with D/H/W, the next level's Dc/Hc/Wc and magics,
+ * c = storage faces, xin = X, xout = Y, b = B, ec = the next level's B (the
+ * coarsest: its FP64 inverse, N <= 64).  Per-cell arithmetic identical to
+ * hfm_mg_sweep / hfm_mg_resid_restrict / hfm_mg_coarse_solve (their per-cell
+ * path).  Replaces the per-level launches the sawtooth cycle of
+ * ART/Anisotropic_3d_Multigrid_7point.ipynb:402-454 makes on small levels. */
+int hfm_mg_coarse_cycle(const hfm_mg_task* recs, const int32_t* chain, int64_t ninst,
+                        const hfm_mg_scalars* sc, int store, int nu1, int nu2, float omega,
+                        float alpha, void* stream);
+
 /* FCG kernels on the fine level (coefficients c in FP32).  T = `store` (the
  * V-cycle's storage: rlow and the fused first sweep), P = `pstore` (z and the
  * search direction p; hfm_fcg_dir's `store` argument is P).

@@ -103,17 +103,21 @@ def test_graph_and_eager_agree(dev):
 
 
 def test_launch_shapes_do_not_change_results(dev):
-    """Every block size of the tunable launches gives bitwise the same solve."""
-    from hfm.tune import tune_blocks
+    """Every block size of the tunable launches and every coarse-cycle
+    threshold gives bitwise the same solve."""
+    from hfm.tune import tune
     specs = [((32, 32, 32), 31, 1.0, 1), ((48, 40, 64), 32, 1.5, 2), ((20, 12, 36), 33, 0.7, 1)]
-    ref, _ = _batch(dev, specs, "bf16")
+    ref, _ = _batch(dev, specs, "bf16", coarse_cells=64)     # coarsest level only
     iref = ref.solve()
     mg, _ = _batch(dev, specs, "bf16")
-    choice = tune_blocks(mg, reps=2)
-    assert choice and set(choice.values()) <= {128, 256, 512}
-    for table in (choice, {k: 128 for k in choice}, {k: 512 for k in choice}):
-        mg.set_blocks(table)
-        info = mg.solve()
-        assert list(info["iter"]) == list(iref["iter"])
-        for i in range(len(specs)):
-            assert bool((mg.solution(i) == ref.solution(i)).all())
+    choice = tune(mg, reps=2)
+    blocks = choice["blocks"]
+    assert blocks and set(blocks.values()) <= {128, 256, 512}
+    for cells in (choice["coarse_cells"], 64, 4096, 10 ** 6):
+        mg.set_coarse_cells(cells)
+        for table in (blocks, {k: 128 for k in blocks}, {k: 512 for k in blocks}):
+            mg.set_blocks(table)
+            info = mg.solve()
+            assert list(info["iter"]) == list(iref["iter"]), (cells, table)
+            for i in range(len(specs)):
+                assert bool((mg.solution(i) == ref.solution(i)).all()), (cells, i)

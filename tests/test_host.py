@@ -153,6 +153,41 @@ def test_work_tables_cover_every_cell_once(tuned):
         assert t["D"] * t["H"] * t["W"] <= 64
 
 
+@pytest.mark.parametrize("coarse_cells", [64, 4096, 10 ** 6])
+def test_coarse_cycle_chains_cover_the_tail_of_every_hierarchy(coarse_cells):
+    pytest.importorskip("torch")
+    from hfm import _lib as L
+    from hfm.mg import MGBatch, chain_start
+    shapes = [(32, 32, 32), (48, 48, 48), (13, 20, 7), (96, 40, 33), (128, 128, 128)]
+    mg = MGBatch(shapes, storage="bf16", device="cpu", coarse_cells=coarse_cells)
+    plan = mg.plan
+    chain = plan.chain.numpy().reshape(-1, 2)
+    recs = plan.chain_recs.numpy().view(L.TASK_DTYPE)
+    assert len(chain) == len(mg.insts)
+    ops = plan.ops_down + [op for ups in plan.ops_up for op in ups]
+    for g, inst in enumerate(mg.insts):
+        lc = chain_start(inst.levels, coarse_cells)
+        assert 1 <= lc <= len(inst.levels) - 1
+        assert all(inst.levels[l][0] * inst.levels[l][1] * inst.levels[l][2] > coarse_cells
+                   for l in range(1, lc))
+        first, count = chain[g]
+        assert count == len(inst.levels) - lc
+        for q in range(count):
+            t, l = recs[first + q], lc + q
+            assert (t["D"], t["H"], t["W"]) == inst.levels[l] and t["inst"] == inst.slot
+            assert t["xin"] == inst.X[l].data_ptr() and t["xout"] == inst.Y[l].data_ptr()
+            assert t["b"] == inst.B[l].data_ptr()
+            if l + 1 < len(inst.levels):
+                assert (t["Dc"], t["Hc"], t["Wc"]) == inst.levels[l + 1]
+                assert t["ec"] == inst.B[l + 1].data_ptr()
+            else:
+                assert t["ec"] == inst.ainv.data_ptr()
+        # the per-level launches stop at the chain: none touches a level >= lc
+        for kind, mode, tt, lay, dot, lev, block in ops:
+            slots = set(int(x) for x in tt.numpy().view(L.TASK_DTYPE)["inst"])
+            assert (inst.slot in slots) == (lev < lc), (kind, lev, lc)
+
+
 def test_plan_rejects_bad_shapes():
     pytest.importorskip("torch")
     from hfm.mg import MGBatch
