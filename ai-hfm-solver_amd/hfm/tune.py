@@ -97,12 +97,17 @@ def tune_blocks(mg: MGBatch, reps: int = 5, candidates=BLOCKS, min_ms: float = 0
     return choice
 
 
-def tune_coarse(mg: MGBatch, reps: int = 5, candidates=(64, 512, 4096, 32768)) -> int:
+def tune_coarse(mg: MGBatch, reps: int = 5, candidates=None, max_cells: int = 1 << 17) -> int:
     """Choose the size below which a level joins the one-launch coarse cycle by
-    timing whole V-cycles (all instances active) for each candidate."""
+    timing whole V-cycles (all instances active) for each candidate: by
+    default every distinct level size of the batch below the finest, up to
+    ``max_cells`` (one workgroup streams such a level in a few microseconds)."""
     mg._need_gpu()
     mg.restart()
     s = L.stream_handle(mg.device)
+    if candidates is None:
+        sizes = {math.prod(lv) for i in mg.insts for lv in i.levels[1:]}
+        candidates = sorted(n for n in sizes if n <= max_cells)
     t = {}
     for n in candidates:
         mg.params.coarse_cells = int(n)
