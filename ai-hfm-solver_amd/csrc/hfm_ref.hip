@@ -50,18 +50,42 @@ struct Grid3 {
   }
 };
 
-// sum_t K[t] x(k+a-1, j+b-1, i+c-1) over the non-zero taps, row-major t = 9a+3b+c
+// The taps of `mask` (row-major t = 9a+3b+c) of the 3x3x3 neighbourhood of
+// (k, j, i) with zero ghosts.  Every load is issued from an in-bounds address
+// (a tap outside the grid re-reads the centre and is zeroed) before any is
+// used, so the loads of a cell are in flight together.
+template <typename XF>
+__device__ __forceinline__ void load_nbhd(uint32_t mask, const Grid3& g, int k, int j, int i,
+                                          XF xat, float v[27]) {
+#pragma unroll
+  for (int t = 0; t < 27; ++t) {
+    if (!((mask >> t) & 1u)) {
+      v[t] = 0.0f;
+      continue;
+    }
+    const int kk = k + t / 9 - 1, jj = j + (t / 3) % 3 - 1, ii = i + t % 3 - 1;
+    const bool in = g.in(kk, jj, ii);
+    const float x = xat(in ? kk : k, in ? jj : j, in ? ii : i);
+    v[t] = in ? x : 0.0f;
+  }
+}
+
+// sum_t K[t] v[t] over the taps of `mask`, in tap order
+__device__ __forceinline__ float wsum(const Weights& K, uint32_t mask, const float v[27]) {
+  float acc = 0.0f;
+#pragma unroll
+  for (int t = 0; t < 27; ++t)
+    if ((mask >> t) & 1u) acc += K.w[t] * v[t];
+  return acc;
+}
+
+// sum_t K[t] x(k+a-1, j+b-1, i+c-1) over the non-zero taps (zero ghosts)
 template <typename XF>
 __device__ __forceinline__ float conv_at(const Weights& K, uint32_t mask, const Grid3& g, int k,
                                          int j, int i, XF xat) {
-  float acc = 0.0f;
-#pragma unroll
-  for (int t = 0; t < 27; ++t) {
-    if (!((mask >> t) & 1u)) continue;
-    const int kk = k + t / 9 - 1, jj = j + (t / 3) % 3 - 1, ii = i + t % 3 - 1;
-    if (g.in(kk, jj, ii)) acc += K.w[t] * xat(kk, jj, ii);
-  }
-  return acc;
+  float v[27];
+  load_nbhd(mask, g, k, j, i, xat, v);
+  return wsum(K, mask, v);
 }
 
 // six-direction operator sum_dir m_dir (x - x_nb), order x-, x+, y-, y+, z-, z+
@@ -75,7 +99,12 @@ __device__ __forceinline__ float var6_at(const Var6& V, const Grid3& g, int k, i
                                          XF xat) {
   const int64_t n = g.idx(k, j, i);
   const float xc = xat(k, j, i);
-  auto nb = [&](int kk, int jj, int ii) { return g.in(kk, jj, ii) ? xat(kk, jj, ii) : 0.0f; };
+  // a neighbour outside the grid re-reads the centre (in bounds) and is zeroed
+  auto nb = [&](int kk, int jj, int ii) {
+    const bool in = g.in(kk, jj, ii);
+    const float x = xat(in ? kk : k, in ? jj : j, in ? ii : i);
+    return in ? x : 0.0f;
+  };
   float acc = 0.0f;
   acc += (xc - nb(k, j, i - 1)) * V.m[0][n];
   acc += (xc - nb(k, j, i + 1)) * V.m[1][n];
@@ -215,16 +244,19 @@ __global__ __launch_bounds__(kBX* kBY) void k_advect27_v(
   const float* phi[3] = {p0, p1, p2};
   const float* base[3] = {b0, b1, b2};
   float* out[3] = {o0, o1, o2};
+  const uint32_t all = K.mask[0] | K.mask[1] | K.mask[2] | K.mask[3];
 #pragma unroll
   for (int q = 0; q < NF; ++q) {
     const float* x = phi[q];
-    auto xat = [&](int kk, int jj, int ii) { return x[g.idx(kk, jj, ii)]; };
-    const float dif = conv_at(K.k[0], K.mask[0], g, k, j, i, xat);
-    const float ax = conv_at(K.k[1], K.mask[1], g, k, j, i, xat);
-    const float ay = conv_at(K.k[2], K.mask[2], g, k, j, i, xat);
-    const float az = conv_at(K.k[3], K.mask[3], g, k, j, i, xat);
+    float v[27];   // one neighbourhood load serves the four stencils
+    load_nbhd(all, g, k, j, i, [&](int kk, int jj, int ii) { return x[g.idx(kk, jj, ii)]; }, v);
+    const float bq = base[q][n];
+    const float dif = wsum(K.k[0], K.mask[0], v);
+    const float ax = wsum(K.k[1], K.mask[1], v);
+    const float ay = wsum(K.k[2], K.mask[2], v);
+    const float az = wsum(K.k[3], K.mask[3], v);
     const float a = ((dif - uc * ax) - vc * ay) - wc * az;
-    out[q][n] = f * a + base[q][n];
+    out[q][n] = f * a + bq;
   }
 }
 

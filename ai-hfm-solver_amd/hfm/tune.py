@@ -36,16 +36,35 @@ from .mg import BLOCK, BLOCKS, MGBatch, Plan
 Key = Tuple
 
 
-def _time_ms(fn, reps: int) -> float:
-    s = torch.cuda.current_stream()
-    fn()
+def _graph_time_ms(mg: MGBatch, fn, reps: int) -> float:
+    """Average time of ``fn(stream)`` replayed from a HIP graph of ``reps`` calls
+    (how the solver runs it: no host launch gaps)."""
+    dev = mg.device
+    side = torch.cuda.Stream(dev)
+    side.wait_stream(torch.cuda.current_stream(dev))
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=side):
+        s = L.stream_handle(dev)
+        for _ in range(reps):
+            fn(s)
+    torch.cuda.current_stream(dev).wait_stream(side)
+    g.replay()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record(s)
-    for _ in range(reps):
-        fn()
-    e1.record(s)
+    e0.record()
+    g.replay()
+    e1.record()
     e1.synchronize()
     return e0.elapsed_time(e1) / reps
+
+
+def _activate(mg: MGBatch):
+    """Re-initialise the batch and mark every instance active: an instance
+    whose rhs is zero (e.g. a solver set up before its first rhs) would
+    otherwise be finished at once and its kernels exit without work.  The
+    timed kernels do the same work whatever the data; ``solve`` re-stages."""
+    mg.restart()
+    idx = torch.tensor([i.slot for i in mg.insts], device=mg.device)
+    mg.pool.active_view().index_fill_(0, idx, 1)
 
 
 def _op_key(op) -> Optional[Key]:
@@ -60,17 +79,16 @@ def _op_key(op) -> Optional[Key]:
 def tune_blocks(mg: MGBatch, reps: int = 5, candidates=BLOCKS, min_ms: float = 0.02) -> Dict[Key, int]:
     """Choose the threads per block of every tunable launch of ``mg``'s plan.
 
-    Every candidate plan launches each op ``reps`` times after one warm
-    launch; ops faster than ``min_ms`` with the default keep the default (their
+    Every candidate plan replays each op ``reps`` times from a HIP graph after
+    one warm replay; ops faster than ``min_ms`` with the default keep the default (their
     time is launch latency, not streaming).  The batch's data must be set up
-    (``setup``); the solver state is re-initialised first (every instance
-    active), the launches only rewrite V-cycle scratch vectors, and
+    (``setup``); the solver state is re-initialised first and every instance
+    marked active, the launches only rewrite V-cycle scratch vectors, and
     ``restart``/``solve`` re-initialise it again.  Returns the choice table and
     installs it with ``mg.set_blocks``.
     """
     mg._need_gpu()
-    mg.restart()
-    s = L.stream_handle(mg.device)
+    _activate(mg)
     times: Dict[Key, Dict[int, float]] = {}
     for b in candidates:
         trial = {}
@@ -85,7 +103,8 @@ def tune_blocks(mg: MGBatch, reps: int = 5, candidates=BLOCKS, min_ms: float = 0
             k = _op_key(op)
             if k is None:
                 continue
-            times.setdefault(k, {})[b] = _time_ms(lambda op=op: plan._launch(op, s), reps)
+            times.setdefault(k, {})[b] = _graph_time_ms(mg, lambda s_, op=op: plan._launch(op, s_),
+                                                        reps)
     choice = {}
     for k, t in times.items():
         best = min(t, key=t.get)
@@ -99,12 +118,12 @@ def tune_blocks(mg: MGBatch, reps: int = 5, candidates=BLOCKS, min_ms: float = 0
 
 def tune_coarse(mg: MGBatch, reps: int = 5, candidates=None, max_cells: int = 1 << 17) -> int:
     """Choose the size below which a level joins the one-launch coarse cycle by
-    timing whole V-cycles (all instances active) for each candidate: by
+    timing whole V-cycles (all instances active, replayed from a HIP graph as
+    the solver runs them) for each candidate: by
     default every distinct level size of the batch below the finest, up to
     ``max_cells`` (one workgroup streams such a level in a few microseconds)."""
     mg._need_gpu()
-    mg.restart()
-    s = L.stream_handle(mg.device)
+    _activate(mg)
     if candidates is None:
         sizes = {math.prod(lv) for i in mg.insts for lv in i.levels[1:]}
         candidates = sorted(n for n in sizes if n <= max_cells)
@@ -112,7 +131,7 @@ def tune_coarse(mg: MGBatch, reps: int = 5, candidates=None, max_cells: int = 1 
     for n in candidates:
         mg.params.coarse_cells = int(n)
         plan = Plan(mg.pool, mg.insts)
-        t[n] = _time_ms(lambda: plan.precondition(s), reps)
+        t[n] = _graph_time_ms(mg, lambda s_: plan.precondition(s_), reps)
     best = min(t, key=t.get)
     mg.set_coarse_cells(best)
     mg.coarse_times = t

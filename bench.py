@@ -172,6 +172,8 @@ def batched_fp32(specs, dev, torch):
         raise SystemExit("bench: FP32 batch did not converge")
     out = {"solves_per_s": round(len(specs) / t, 2), "ms_per_step": round(t * 1e3, 2),
            "iters_mean": round(float(info["iter"].mean()), 2), "iters_max": int(info["iter"].max()),
+        "cell_updates_per_s": round(rates["cell_updates_per_s"], 1),
+        "fine_cell_vcycles_per_s": round(rates["fine_cell_vcycles_per_s"], 1),
            "resident_GiB": round(mg.nbytes() / 2 ** 30, 2)}
     log(f"bench: batched fp32 {out}")
     return out
@@ -310,6 +312,71 @@ def kolmogorov(args, dev, torch):
                                          f"run's {int(st['substeps'])} substeps"}
         out["speedup_vs_cpu"] = round(out["single_flow"]["steps_per_s"] / cpu_sps, 1)
     log(f"bench: kolmogorov {out}")
+    return out
+
+
+def reference_context(args, dev, torch):
+    """The two reference timings taken at this build's algorithm and size
+    (hardware differs; context, not the headline metric):
+
+    * 1000 reference sawtooth cycles on 128^3, constant-coefficient 7-point
+      (Dx, Dy, Dz) = (200, 100, 50), 8 levels to 1^3:
+      295.16 s (ART/Anisotropic_3d_Multigrid_7point.ipynb:395, hardware not
+      recorded) -- here layers.SawtoothMG, the same cycle;
+    * the urban step on 512^3 with 5 sawtooth cycles of the 27-point Poisson
+      layer per step: 5.06 s/step (UM/AI_HFM_v3.ipynb:2508, RTX A5000) --
+      here timestep.UrbanFlow, the same step (seeded blocks: the reference's
+      mesh file is not in its repository)."""
+    from hfm import layers as Ly
+    from hfm.timestep import UrbanFlow
+    from hfm.workloads import urban_blocks
+    out = {}
+    if args.ref_cycles > 0:
+        K = np.zeros((3, 3, 3), np.float32)
+        K[0, 1, 1] = K[2, 1, 1] = -50.0
+        K[1, 0, 1] = K[1, 2, 1] = -100.0
+        K[1, 1, 0] = K[1, 1, 2] = -200.0
+        K[1, 1, 1] = 2.0 * (200.0 + 100.0 + 50.0)
+        mg = Ly.SawtoothMG([Ly.FixedConv(K, (n, n, n)) for n in (128, 64, 32, 16, 8, 4, 2, 1)])
+        b = torch.zeros((1, 128, 128, 128, 1), device=dev)
+        b[0, 40, 64, 64, 0], b[0, 88, 64, 64, 0] = 1.0, -1.0
+        x = torch.zeros_like(b)
+        for _ in range(3):
+            x = mg.cycle(x, b)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.ref_cycles):
+            x = mg.cycle(x, b)
+        torch.cuda.synchronize()
+        per = (time.perf_counter() - t0) / args.ref_cycles
+        out["sawtooth_128"] = {"cycles": args.ref_cycles, "ms_per_cycle": round(per * 1e3, 4),
+                               "s_per_1000_cycles": round(per * 1e3, 3),
+                               "fine_cell_cycles_per_s": round(128 ** 3 / per, 1),
+                               "reference_s_per_1000_cycles": 295.16,
+                               "ratio_vs_reference": round(295.16 / (per * 1e3), 1)}
+    if args.ref_urban_steps > 0:
+        sh = (512, 512, 512)
+        mask = urban_blocks(sh, args.seed)
+        sig = torch.from_numpy(np.where(mask, 1e5, 0.0).astype(np.float32)).to(dev)
+        del mask
+        flow = UrbanFlow(sh, sigma=sig, multi_itr=5, device=dev)
+        z = torch.zeros(sh, device=dev)
+        state = (torch.ones(sh, device=dev), z, z.clone(), z.clone(), z.clone())
+        state = flow.step(*state)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.ref_urban_steps):
+            state = flow.step(*state)
+        torch.cuda.synchronize()
+        per = (time.perf_counter() - t0) / args.ref_urban_steps
+        out["urban_512"] = {"steps": args.ref_urban_steps, "mg_cycles_per_step": 5,
+                            "ms_per_step": round(per * 1e3, 2),
+                            "cell_steps_per_s": round(512 ** 3 / per, 1),
+                            "reference_s_per_step": 5.06, "ratio_vs_reference": round(5.06 / per, 1),
+                            "finite": bool(all(torch.isfinite(t).all() for t in state))}
+        del flow, state, sig, z
+        torch.cuda.empty_cache()
+    log(f"bench: reference context {out}")
     return out
 
 
@@ -465,6 +532,10 @@ def main():
                     help="launch only this kernel (after setup) or workload, for profiler runs")
     ap.add_argument("--reps", type=int, default=20)
     ap.add_argument("--urban-steps", type=int, default=10)
+    ap.add_argument("--ref-cycles", type=int, default=200,
+                    help="reference sawtooth cycles timed on 128^3 (0: skip)")
+    ap.add_argument("--ref-urban-steps", type=int, default=3,
+                    help="reference urban steps timed on 512^3 (0: skip)")
     ap.add_argument("--flow-steps", type=int, default=1000,
                     help="Kolmogorov 2D flow steps of the single-flow run (0: skip)")
     ap.add_argument("--flow-ensemble", type=int, default=256)
@@ -522,6 +593,12 @@ def main():
         raise SystemExit(f"bench: {int(np.sum(status != 1))} instances did not converge")
     total = sum(times)
     value = args.steps * len(specs) / total
+    # Sigma over instances of iterations x (nu1 + nu2) x cells of the swept levels
+    its = info["iter"].astype(np.int64)
+    swept = np.array([sum(math.prod(lv) for lv in inst.levels[:-1]) for inst in mg.insts])
+    fine = np.array([inst.N for inst in mg.insts])
+    rates = {"cell_updates_per_s": float(np.sum(its * 4 * swept)) * args.steps / total,
+             "fine_cell_vcycles_per_s": float(np.sum(its * fine)) * args.steps / total}
 
     # dominant kernel: the fine-level FCG update (x += a p, r -= a A p), timed live
     mg.restart()
@@ -555,6 +632,8 @@ def main():
         extra["urban_flow"] = urban(args, dev, torch)
     if args.flow_steps > 0:
         extra["kolmogorov_2d"] = kolmogorov(args, dev, torch)
+    if args.ref_cycles > 0 or args.ref_urban_steps > 0:
+        extra["reference_context"] = reference_context(args, dev, torch)
     cpu = None
     if not args.no_cpu_baseline:
         torch.set_num_threads(1)
@@ -571,6 +650,8 @@ def main():
                                f"{args.storage}-storage V(2,2) preconditioned FP64 CG to 1e-8",
                    "instances": len(specs), "fine_cells": cells, "seed": args.seed},
         "iters_mean": round(float(info["iter"].mean()), 2), "iters_max": int(info["iter"].max()),
+        "cell_updates_per_s": round(rates["cell_updates_per_s"], 1),
+        "fine_cell_vcycles_per_s": round(rates["fine_cell_vcycles_per_s"], 1),
         "max_rel_residual": float(info["rel"].max()),
         "latency_ms": {"p50": round(float(np.percentile(times, 50)) * 1e3, 2),
                        "p99": round(float(np.percentile(times, 99)) * 1e3, 2)},
