@@ -676,9 +676,12 @@ __global__ __launch_bounds__(kBlock) void k_fcg_init(const hfm_mg_task* __restri
   if (threadIdx.x == 0) dots[t.slot0 + chunk] = s;
 }
 
-// p = T(z + beta p_old) (p_old not read when beta == 0); dots = p . (A p) with
-// A p in FP32 (it only sets the step length; the residual update recomputes
-// A p in FP64).
+// p = T(z + beta p_old); dots = p . (A p) in FP32 per cell (it only sets the
+// step length; the residual update recomputes A p in FP64).  The vector path
+// evaluates p . A p as the operator's sum over faces,
+//   sum_cells cx (p - p_w)^2 + cy (p - p_s)^2 + cz (p - p_d)^2
+//             + [high boundary] (bxh | byh | bzh) p^2      (zero ghosts),
+// which needs only the west, south and down neighbours.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_fcg_dir(const hfm_mg_task* __restrict__ tasks,
                                                     const uint16_t* __restrict__ blk_task,
@@ -698,14 +701,33 @@ __global__ __launch_bounds__(kBlock) void k_fcg_dir(const hfm_mg_task* __restric
     const int n0 = base + threadIdx.x * kRun;
     const bool valid = n0 < N;
     const Run r = run_at(t, n0, valid);
-    RunCoef<float> q;
-    q.load(t, r);
-    float d[8], p[8], off[8];
-    q.diag(d);
-    run_offdiag<float>(t, r, q, src, p, off);
+    const int W = t.W, HW = t.H * t.W;
+    const bool west = r.i0 > 0, south = r.j > 0, down = r.k > 0;
+    const bool east_b = r.i0 + kRun == W, north_b = r.j + 1 == t.H, up_b = r.k + 1 == t.D;
+    float p[8], ps[8], pd[8], cx[8], cy[8], cz[8], by[8], bz[8];
+    src.run(r.n0, r.k, r.j, r.i0, p);
+    src.run(south ? r.n0 - W : r.n0, r.k, r.j, r.i0, ps);
+    src.run(down ? r.n0 - HW : r.n0, r.k, r.j, r.i0, pd);
+    const float pw = src.one(west ? r.n0 - 1 : r.n0, r.k, r.j, r.i0);
+    ld8(gin<float>(t.c[0]) + r.n0, cx);
+    ld8(gin<float>(t.c[1]) + r.n0, cy);
+    ld8(gin<float>(t.c[2]) + r.n0, cz);
+    ld8(gin<float>(t.c[4]) + r.k * W + r.i0, by);
+    ld8(gin<float>(t.c[5]) + r.j * W + r.i0, bz);
+    const float bx = ldg(gin<float>(t.c[3]) + r.k * t.H + r.j);
     if (valid) {
 #pragma unroll
-      for (int m = 0; m < 8; ++m) acc += double(p[m] * (d[m] * p[m] - off[m]));
+      for (int m = 0; m < 8; ++m) {
+        const float dw = p[m] - (m ? p[m - 1] : (west ? pw : 0.0f));
+        const float ds = p[m] - (south ? ps[m] : 0.0f);
+        const float dd = p[m] - (down ? pd[m] : 0.0f);
+        float e = cx[m] * dw * dw + cy[m] * ds * ds + cz[m] * dd * dd;
+        const float p2 = p[m] * p[m];
+        if (m == 7 && east_b) e += bx * p2;
+        if (north_b) e += by[m] * p2;
+        if (up_b) e += bz[m] * p2;
+        acc += double(e);
+      }
       st8_nt(pnew + n0, p);
     }
   } else if (base < N) {

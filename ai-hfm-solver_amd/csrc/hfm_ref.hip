@@ -45,21 +45,33 @@ struct Grid3 {
   __device__ __forceinline__ int64_t idx(int k, int j, int i) const {
     return (int64_t(k) * H + j) * W + i;
   }
+  // without short-circuit evaluation (no branches between a cell's loads)
   __device__ __forceinline__ bool in(int k, int j, int i) const {
-    return k >= 0 && k < D && j >= 0 && j < H && i >= 0 && i < W;
+    return (unsigned(k) < unsigned(D)) & (unsigned(j) < unsigned(H)) & (unsigned(i) < unsigned(W));
   }
 };
 
-// The taps of `mask` (row-major t = 9a+3b+c) of the 3x3x3 neighbourhood of
-// (k, j, i) with zero ghosts.  Every load is issued from an in-bounds address
-// (a tap outside the grid re-reads the centre and is zeroed) before any is
-// used, so the loads of a cell are in flight together.
-template <typename XF>
-__device__ __forceinline__ void load_nbhd(uint32_t mask, const Grid3& g, int k, int j, int i,
-                                          XF xat, float v[27]) {
+// Tap sets of the 3x3x3 stencils (row-major t = 9a+3b+c): the 7-point cross
+// or all 27.  A kernel is instantiated per set and the host picks the cross
+// when every non-zero weight lies on it (the other taps have zero weight, so
+// skipping them is exact).
+constexpr uint32_t kCross = (1u << 4) | (1u << 10) | (1u << 12) | (1u << 13) | (1u << 14) |
+                            (1u << 16) | (1u << 22);
+template <int TAPS>
+__device__ constexpr bool tap_on(int t) {
+  return TAPS == 27 || ((kCross >> t) & 1u);
+}
+inline int taps_of(uint32_t mask) { return (mask & ~kCross) ? 27 : 7; }
+
+// The taps of the 3x3x3 neighbourhood of (k, j, i) with zero ghosts.  All are
+// loaded from in-bounds addresses (a tap outside the grid re-reads the centre
+// and is zeroed) without branches, so a cell's loads are in flight together.
+template <int TAPS, typename XF>
+__device__ __forceinline__ void load_nbhd(const Grid3& g, int k, int j, int i, XF xat,
+                                          float v[27]) {
 #pragma unroll
   for (int t = 0; t < 27; ++t) {
-    if (!((mask >> t) & 1u)) {
+    if (!tap_on<TAPS>(t)) {
       v[t] = 0.0f;
       continue;
     }
@@ -70,22 +82,23 @@ __device__ __forceinline__ void load_nbhd(uint32_t mask, const Grid3& g, int k, 
   }
 }
 
-// sum_t K[t] v[t] over the taps of `mask`, in tap order
-__device__ __forceinline__ float wsum(const Weights& K, uint32_t mask, const float v[27]) {
+// sum_t K[t] v[t] over the tap set, in tap order
+template <int TAPS>
+__device__ __forceinline__ float wsum(const Weights& K, const float v[27]) {
   float acc = 0.0f;
 #pragma unroll
   for (int t = 0; t < 27; ++t)
-    if ((mask >> t) & 1u) acc += K.w[t] * v[t];
+    if (tap_on<TAPS>(t)) acc += K.w[t] * v[t];
   return acc;
 }
 
-// sum_t K[t] x(k+a-1, j+b-1, i+c-1) over the non-zero taps (zero ghosts)
-template <typename XF>
-__device__ __forceinline__ float conv_at(const Weights& K, uint32_t mask, const Grid3& g, int k,
-                                         int j, int i, XF xat) {
+// sum_t K[t] x(k+a-1, j+b-1, i+c-1) over the tap set (zero ghosts)
+template <int TAPS, typename XF>
+__device__ __forceinline__ float conv_at(const Weights& K, const Grid3& g, int k, int j, int i,
+                                         XF xat) {
   float v[27];
-  load_nbhd(mask, g, k, j, i, xat, v);
-  return wsum(K, mask, v);
+  load_nbhd<TAPS>(g, k, j, i, xat, v);
+  return wsum<TAPS>(K, v);
 }
 
 // six-direction operator sum_dir m_dir (x - x_nb), order x-, x+, y-, y+, z-, z+
@@ -128,14 +141,16 @@ __device__ __forceinline__ float jacobi_form(float x, float ax, float b, float d
 }
 
 // ------------------------------------------------------------------ kernels
+template <int TAPS>
 __global__ __launch_bounds__(kBX* kBY) void k_conv27(const float* __restrict__ x,
                                                      float* __restrict__ y,
                                                      const float* __restrict__ b, Weights K,
-                                                     uint32_t mask, float d, Grid3 g, int form) {
+                                                     float d, Grid3 g, int form) {
   int k, j, i;
   if (!cell_of_thread(g, k, j, i)) return;
   const int64_t n = g.idx(k, j, i);
-  const float ax = conv_at(K, mask, g, k, j, i, [&](int kk, int jj, int ii) { return x[g.idx(kk, jj, ii)]; });
+  const float ax =
+      conv_at<TAPS>(K, g, k, j, i, [&](int kk, int jj, int ii) { return x[g.idx(kk, jj, ii)]; });
   const int f = form & HFM_FORM_MASK;
   float out = ax;
   if (f == HFM_FORM_RESID) out = ax - b[n];
@@ -181,11 +196,11 @@ __global__ __launch_bounds__(kBX* kBY) void k_prolong_nearest(const float* __res
   y[gf.idx(k, j, i)] = xc[gc.idx(k / fz, j / fy, i / fx)];
 }
 
-template <bool VAR6>
+template <bool VAR6, int TAPS>
 __global__ __launch_bounds__(kBX* kBY) void k_prolong_jacobi(const float* __restrict__ wc,
                                                              const float* __restrict__ r,
                                                              float* __restrict__ y, Weights K,
-                                                             uint32_t mask, float d, Var6 V,
+                                                             float d, Var6 V,
                                                              Grid3 gf, Grid3 gc, int fz, int fy,
                                                              int fx) {
   int k, j, i;
@@ -196,16 +211,16 @@ __global__ __launch_bounds__(kBX* kBY) void k_prolong_jacobi(const float* __rest
   if (VAR6) {
     y[n] = jacobi_form(wp, var6_at(V, gf, k, j, i, wat), r[n], V.tot[n]);
   } else {
-    y[n] = jacobi_form(wp, conv_at(K, mask, gf, k, j, i, wat), r[n], d);
+    y[n] = jacobi_form(wp, conv_at<TAPS>(K, gf, k, j, i, wat), r[n], d);
   }
 }
 
-template <bool VAR6>
+template <bool VAR6, int TAPS>
 __global__ __launch_bounds__(kBX* kBY) void k_correct_jacobi(const float* __restrict__ x,
                                                              const float* __restrict__ w,
                                                              const float* __restrict__ b,
                                                              float* __restrict__ y, Weights K,
-                                                             uint32_t mask, float d, Var6 V,
+                                                             float d, Var6 V,
                                                              Grid3 g, int flags) {
   int k, j, i;
   if (!cell_of_thread(g, k, j, i)) return;
@@ -219,7 +234,7 @@ __global__ __launch_bounds__(kBX* kBY) void k_correct_jacobi(const float* __rest
   if (VAR6) {
     y[n] = jacobi_form(v, var6_at(V, g, k, j, i, vat), b[n], V.tot[n]);
   } else {
-    y[n] = jacobi_form(v, conv_at(K, mask, g, k, j, i, vat), b[n], d);
+    y[n] = jacobi_form(v, conv_at<TAPS>(K, g, k, j, i, vat), b[n], d);
   }
 }
 
@@ -244,17 +259,16 @@ __global__ __launch_bounds__(kBX* kBY) void k_advect27_v(
   const float* phi[3] = {p0, p1, p2};
   const float* base[3] = {b0, b1, b2};
   float* out[3] = {o0, o1, o2};
-  const uint32_t all = K.mask[0] | K.mask[1] | K.mask[2] | K.mask[3];
 #pragma unroll
   for (int q = 0; q < NF; ++q) {
     const float* x = phi[q];
     float v[27];   // one neighbourhood load serves the four stencils
-    load_nbhd(all, g, k, j, i, [&](int kk, int jj, int ii) { return x[g.idx(kk, jj, ii)]; }, v);
+    load_nbhd<27>(g, k, j, i, [&](int kk, int jj, int ii) { return x[g.idx(kk, jj, ii)]; }, v);
     const float bq = base[q][n];
-    const float dif = wsum(K.k[0], K.mask[0], v);
-    const float ax = wsum(K.k[1], K.mask[1], v);
-    const float ay = wsum(K.k[2], K.mask[2], v);
-    const float az = wsum(K.k[3], K.mask[3], v);
+    const float dif = wsum<27>(K.k[0], v);
+    const float ax = wsum<27>(K.k[1], v);
+    const float ay = wsum<27>(K.k[2], v);
+    const float az = wsum<27>(K.k[3], v);
     const float a = ((dif - uc * ax) - vc * ay) - wc * az;
     out[q][n] = f * a + bq;
   }
@@ -278,7 +292,7 @@ __global__ __launch_bounds__(kBX* kBY) void k_conv27_sum_v(const float* x0, cons
   float acc = 0.0f;
   for (int q = 0; q < nf; ++q) {
     const float* xq = xs[q];
-    acc += conv_at(K.k[q], K.mask[q], g, k, j, ic,
+    acc += conv_at<27>(K.k[q], g, k, j, ic,
                    [&](int kk, int jj, int ii) { return xq[g.idx(kk, jj, ii)]; });
   }
   y[g.idx(k, j, i)] = scale * acc;
@@ -429,8 +443,12 @@ extern "C" int hfm_conv27_f32(const float* x, float* y, const float* b, const fl
     return HFM_EINVAL;
   }
   const Grid3 g{int(D), int(H), int(W)};
-  hipLaunchKernelGGL(k_conv27, grid_for(g), dim3(kBX, kBY), 0, as_stream(stream), x, y, b, w,
-                     mask, d, g, form);
+  if (taps_of(mask) == 7)
+    hipLaunchKernelGGL(k_conv27<7>, grid_for(g), dim3(kBX, kBY), 0, as_stream(stream), x, y, b, w,
+                       d, g, form);
+  else
+    hipLaunchKernelGGL(k_conv27<27>, grid_for(g), dim3(kBX, kBY), 0, as_stream(stream), x, y, b,
+                       w, d, g, form);
   return launch_status("hfm_conv27_f32");
 }
 
@@ -496,12 +514,16 @@ extern "C" int hfm_prolong_jacobi_f32(const float* wc, const float* r, float* y,
   }
   const Grid3 gf{int(D), int(H), int(W)};
   const Grid3 gc{int(D / fz), int(H / fy), int(W / fx)};
+  hipStream_t s = as_stream(stream);
   if (var6)
-    hipLaunchKernelGGL(k_prolong_jacobi<true>, grid_for(gf), dim3(kBX, kBY), 0,
-                       as_stream(stream), wc, r, y, w, mask, d, V, gf, gc, fz, fy, fx);
+    hipLaunchKernelGGL((k_prolong_jacobi<true, 7>), grid_for(gf), dim3(kBX, kBY), 0, s, wc, r, y,
+                       w, d, V, gf, gc, fz, fy, fx);
+  else if (taps_of(mask) == 7)
+    hipLaunchKernelGGL((k_prolong_jacobi<false, 7>), grid_for(gf), dim3(kBX, kBY), 0, s, wc, r,
+                       y, w, d, V, gf, gc, fz, fy, fx);
   else
-    hipLaunchKernelGGL(k_prolong_jacobi<false>, grid_for(gf), dim3(kBX, kBY), 0,
-                       as_stream(stream), wc, r, y, w, mask, d, V, gf, gc, fz, fy, fx);
+    hipLaunchKernelGGL((k_prolong_jacobi<false, 27>), grid_for(gf), dim3(kBX, kBY), 0, s, wc, r,
+                       y, w, d, V, gf, gc, fz, fy, fx);
   return launch_status("hfm_prolong_jacobi_f32");
 }
 
@@ -519,12 +541,16 @@ extern "C" int hfm_correct_jacobi_f32(const float* x, const float* wv, const flo
     return HFM_EINVAL;
   }
   const Grid3 g{int(D), int(H), int(W)};
+  hipStream_t s = as_stream(stream);
   if (var6)
-    hipLaunchKernelGGL(k_correct_jacobi<true>, grid_for(g), dim3(kBX, kBY), 0,
-                       as_stream(stream), x, wv, b, y, w, mask, d, V, g, flags);
+    hipLaunchKernelGGL((k_correct_jacobi<true, 7>), grid_for(g), dim3(kBX, kBY), 0, s, x, wv, b,
+                       y, w, d, V, g, flags);
+  else if (taps_of(mask) == 7)
+    hipLaunchKernelGGL((k_correct_jacobi<false, 7>), grid_for(g), dim3(kBX, kBY), 0, s, x, wv,
+                       b, y, w, d, V, g, flags);
   else
-    hipLaunchKernelGGL(k_correct_jacobi<false>, grid_for(g), dim3(kBX, kBY), 0,
-                       as_stream(stream), x, wv, b, y, w, mask, d, V, g, flags);
+    hipLaunchKernelGGL((k_correct_jacobi<false, 27>), grid_for(g), dim3(kBX, kBY), 0, s, x, wv,
+                       b, y, w, d, V, g, flags);
   return launch_status("hfm_correct_jacobi_f32");
 }
 

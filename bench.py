@@ -172,8 +172,6 @@ def batched_fp32(specs, dev, torch):
         raise SystemExit("bench: FP32 batch did not converge")
     out = {"solves_per_s": round(len(specs) / t, 2), "ms_per_step": round(t * 1e3, 2),
            "iters_mean": round(float(info["iter"].mean()), 2), "iters_max": int(info["iter"].max()),
-        "cell_updates_per_s": round(rates["cell_updates_per_s"], 1),
-        "fine_cell_vcycles_per_s": round(rates["fine_cell_vcycles_per_s"], 1),
            "resident_GiB": round(mg.nbytes() / 2 ** 30, 2)}
     log(f"bench: batched fp32 {out}")
     return out
