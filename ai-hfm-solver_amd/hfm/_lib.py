@@ -2,8 +2,10 @@
 
 torch is imported first so that the HIP runtime torch ships is the one the
 library binds to (both carry the soname libamdhip64.so.7); the library then
-launches on torch's streams.  There is no fallback: if the library is missing
-every entry point raises.
+launches on torch's streams.  With HFM_NO_TORCH=1 in the environment torch is
+not imported (the library binds to the system HIP runtime; stream_handle is
+unavailable) -- for torch-free drivers such as counter-collection runs.
+There is no fallback: if the library is missing every entry point raises.
 """
 from __future__ import annotations
 
@@ -12,7 +14,8 @@ import os
 import threading
 
 import numpy as np
-import torch  # noqa: F401  (loads the HIP runtime before libhfm.so)
+if os.environ.get("HFM_NO_TORCH") != "1":
+    import torch  # noqa: F401  (loads the HIP runtime before libhfm.so)
 
 from ._build import LIB_PATH
 

@@ -506,6 +506,80 @@ def profile_kernel(args, dev, torch):
           flush=True)
 
 
+def traffic_driver(args):
+    """Torch-free driver for hardware-counter runs of k_fcg_update: one
+    (--traffic-edge)^3 BF16 instance's update buffers from hipMalloc (contents
+    irrelevant: the kernel's traffic does not depend on them), every block
+    active, ``--reps`` launches, then the same number of device-to-device
+    copies of a known size (the counter calibration).  Prints one JSON line."""
+    import ctypes as C
+    os.environ["HFM_NO_TORCH"] = "1"
+    from hfm import _lib as L
+    lib = L.load()
+    hip = C.CDLL("libamdhip64.so")
+    vp = C.c_void_p
+
+    def malloc(nbytes):
+        p = vp()
+        if hip.hipMalloc(C.byref(p), C.c_size_t(nbytes)) != 0:
+            raise SystemExit("hipMalloc failed")
+        hip.hipMemset(p, 0, C.c_size_t(nbytes))
+        return p.value
+
+    def upload(arr):
+        p = malloc(arr.nbytes)
+        hip.hipMemcpy(vp(p), arr.ctypes.data_as(vp), C.c_size_t(arr.nbytes), 1)  # host->device
+        return p
+
+    e = args.traffic_edge
+    D = H = W = e
+    N = D * H * W
+    nblk = ((N + 2047) // 2048 + 7) // 8 * 8
+    t = np.zeros((), dtype=L.TASK_DTYPE)
+    t["D"], t["H"], t["W"], t["blk0"], t["nblk"] = D, H, W, 0, nblk
+    t["magW"], t["magH"] = L.magic(W), L.magic(H)
+    faces = [malloc(4 * N) for _ in range(3)] + [malloc(4 * D * H), malloc(4 * D * W), malloc(4 * H * W)]
+    t["c"] = faces
+    t["xin"], t["xout"], t["rlow"] = malloc(2 * N), malloc(2 * N), malloc(2 * N)
+    t["x64"], t["r64"] = malloc(8 * N), malloc(8 * N)
+    tasks = upload(np.array([t]).view(np.uint8))
+    blk = upload(np.zeros(nblk, dtype=np.uint16))
+    sc = np.zeros(1, dtype=L.SCALARS_DTYPE)
+    sc["active"], sc["alpha"], sc["nslot"] = 1, 0.5, nblk
+    scp = upload(sc.view(np.uint8))
+    dots = malloc(8 * nblk)
+    hip.hipDeviceSynchronize()
+
+    def timed(fn, reps):
+        ev = [vp(), vp()]
+        for x in ev:
+            hip.hipEventCreate(C.byref(x))
+        fn()
+        hip.hipEventRecord(ev[0], None)
+        for _ in range(reps):
+            fn()
+        hip.hipEventRecord(ev[1], None)
+        hip.hipEventSynchronize(ev[1])
+        ms = C.c_float()
+        hip.hipEventElapsedTime(C.byref(ms), ev[0], ev[1])
+        return ms.value / reps
+
+    upd = lambda: L.call("hfm_fcg_update_sweep", tasks, blk, nblk, scp, L.STORE["bf16"],  # noqa: E731
+                         L.STORE["bf16"], C.c_float(2.0 / 3.0), dots, None)
+    ms = timed(upd, args.reps)
+    nbytes = 1 << 30
+    src, dst = malloc(nbytes), malloc(nbytes)
+    cp = lambda: hip.hipMemcpy(vp(dst), vp(src), C.c_size_t(nbytes), 3)  # noqa: E731 device->device
+    cp_ms = timed(cp, args.reps)
+    alg = N * 50
+    print(json.dumps({"kernel": "k_fcg_update<bf16, true, bf16>", "edge": e, "cells": N,
+                      "launches": args.reps + 1, "alg_bytes_per_launch": alg,
+                      "alg_read_bytes": N * 30, "alg_write_bytes": N * 20,
+                      "launch_ms": round(ms, 4), "achieved_GBs": round(alg / (ms * 1e-3) / 1e9, 1),
+                      "copy_bytes_each_way": nbytes, "copy_launches": args.reps + 1,
+                      "copy_ms": round(cp_ms, 4)}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -538,7 +612,12 @@ def main():
                     help="Kolmogorov 2D flow steps of the single-flow run (0: skip)")
     ap.add_argument("--flow-ensemble", type=int, default=256)
     ap.add_argument("--flow-ensemble-steps", type=int, default=20)
+    ap.add_argument("--traffic", action="store_true",
+                    help="torch-free k_fcg_update driver for hardware-counter runs")
+    ap.add_argument("--traffic-edge", type=int, default=256)
     args = ap.parse_args()
+    if args.traffic:
+        return traffic_driver(args)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the solves are independent and this build runs "
               "on a single MI355X; multi-GPU runs are not part of it", file=sys.stderr)
