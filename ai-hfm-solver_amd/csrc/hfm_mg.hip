@@ -23,6 +23,22 @@ constexpr int kRun = 8;
 constexpr int kCellChunk = kBlock * kRun;
 constexpr int kCoarseChunk = kBlock * 4;
 
+// FP64 solver vectors (x64, r64) of a vector-path task are stored
+// lane-interleaved: in every 512-cell segment (the runs of one wave), cells
+// 8l + 2q and 8l + 2q + 1 of lane l's run sit at segment offsets 128 q + 2 l
+// and 128 q + 2 l + 1, so each of the wave's four 16-byte accesses per vector
+// covers 1 KiB contiguously instead of striding 64 bytes per lane.  The host
+// pads these arrays to a multiple of kIlvSeg cells.  Scalar-path tasks keep
+// the natural order.
+constexpr int kIlvSeg = 64 * kRun;
+__device__ __forceinline__ int ilv_run(int n0) {  // position of run n0's first pair
+  return (n0 & ~(kIlvSeg - 1)) + ((n0 & (kIlvSeg - 1)) >> 2);
+}
+__device__ __forceinline__ int ilv_cell(int n) {
+  const int o = n & (kIlvSeg - 1);
+  return (n & ~(kIlvSeg - 1)) + 128 * ((o & 7) >> 1) + 2 * (o >> 3) + (o & 1);
+}
+
 template <typename C>
 __device__ __forceinline__ float ldc(const void* p, int64_t i) {
   return ldg(gin<C>(p) + i);
@@ -89,6 +105,21 @@ __device__ __forceinline__ void ld8d(const HFM_GLOBAL double* p, double v[8]) {
 __device__ __forceinline__ void st8d(HFM_GLOBAL double* p, const double v[8]) {
 #pragma unroll
   for (int q = 0; q < 4; ++q) ((HFM_GLOBAL d2v*)p)[q] = d2v{v[2 * q], v[2 * q + 1]};
+}
+// The run at n0 of a lane-interleaved FP64 vector (see kIlvSeg).
+__device__ __forceinline__ void ld8d_ilv(const HFM_GLOBAL double* p, int n0, double v[8]) {
+  const HFM_GLOBAL double* b = p + ilv_run(n0);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const d2v a = *(const HFM_GLOBAL d2v*)(b + 128 * q);
+    v[2 * q] = a.x;
+    v[2 * q + 1] = a.y;
+  }
+}
+__device__ __forceinline__ void st8d_ilv(HFM_GLOBAL double* p, int n0, const double v[8]) {
+  HFM_GLOBAL double* b = p + ilv_run(n0);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) *(HFM_GLOBAL d2v*)(b + 128 * q) = d2v{v[2 * q], v[2 * q + 1]};
 }
 
 // Streaming (non-temporal) variants for arrays read or written exactly once
@@ -657,12 +688,14 @@ __global__ __launch_bounds__(kBlock) void k_fcg_init(const hfm_mg_task* __restri
   const HFM_GLOBAL float* b32 = gin<float>(t.b32);
   HFM_GLOBAL double* x64 = gout<double>(t.x64);
   HFM_GLOBAL double* r64 = gout<double>(t.r64);
+  const bool ilv = vec_task(t);
   for (int s = 0; s < kRun; ++s) {
     const int n = base + s * kBlock + threadIdx.x;
     if (n >= N) break;
     const double bv = double(b32[n]);
-    x64[n] = 0.0;
-    r64[n] = bv;
+    const int n64 = ilv ? ilv_cell(n) : n;
+    x64[n64] = 0.0;
+    r64[n64] = bv;
     const T rt = from_f<T>(float(bv));
     put(rlow + n, rt);
     acc += bv * bv;
@@ -779,8 +812,8 @@ __global__ __launch_bounds__(kBlock) void k_fcg_update(const hfm_mg_task* __rest
     RunCoef<float> q;
     q.load(t, r);
     double xv[8], rv[8];
-    ld8d(x64 + r.n0, xv);
-    ld8d(r64 + r.n0, rv);
+    ld8d_ilv(x64, r.n0, xv);
+    ld8d_ilv(r64, r.n0, rv);
     double d[8], p[8], off[8];
     q.diag(d);
     run_offdiag<double>(t, r, q, src, p, off);
@@ -793,8 +826,8 @@ __global__ __launch_bounds__(kBlock) void k_fcg_update(const hfm_mg_task* __rest
         rl[m] = float(rv[m]);
         acc += rv[m] * rv[m];
       }
-      st8d(x64 + n0, xv);
-      st8d(r64 + n0, rv);
+      st8d_ilv(x64, n0, xv);
+      st8d_ilv(r64, n0, rv);
       st8(rlow + n0, rl);
       if (SWEEP) {
         float o[8];

@@ -44,6 +44,38 @@ ELEM = {"f32": 4, "bf16": 2}
 TORCH_STORE = {"f32": torch.float32, "bf16": torch.bfloat16}
 
 
+ILV_SEG = 512        # cells per lane-interleaved segment of the FP64 vectors
+
+
+def ilv_len(n: int) -> int:
+    """Length of an FP64 solver vector (x64, r64) of ``n`` cells: padded to a
+    whole number of interleave segments."""
+    return (n + ILV_SEG - 1) // ILV_SEG * ILV_SEG
+
+
+def deinterleave(v: torch.Tensor, n: int, width: int) -> torch.Tensor:
+    """Natural-order copy of an FP64 solver vector.
+
+    For a grid whose row width is a multiple of 8 (the kernels' vector path)
+    the vector is stored lane-interleaved: in each 512-cell segment, cells
+    8 l + 2 q and 8 l + 2 q + 1 sit at offsets 128 q + 2 l and 128 q + 2 l + 1
+    (a wave's four 16-byte accesses then cover contiguous bytes).  Other
+    widths keep the natural order."""
+    if width % 8:
+        return v[:n].clone()
+    return v.view(-1, 4, 64, 2).transpose(1, 2).reshape(-1)[:n]
+
+
+def interleave(x: torch.Tensor, width: int) -> torch.Tensor:
+    """Inverse of ``deinterleave``: the stored (padded) form of natural-order ``x``."""
+    n = x.numel()
+    out = torch.zeros(ilv_len(n), dtype=x.dtype, device=x.device)
+    out[:n] = x.reshape(-1)
+    if width % 8:
+        return out
+    return out.view(-1, 64, 4, 2).transpose(1, 2).reshape(-1)
+
+
 def level_shapes(shape: Sequence[int], max_coarse_edge: int = 4) -> List[Tuple[int, int, int]]:
     """Halve every axis (rounding up) while any edge exceeds ``max_coarse_edge``."""
     shapes = [tuple(int(s) for s in shape)]
@@ -222,8 +254,8 @@ class Instance:
         self.N = N0
         self.sigma = torch.zeros(3 * _align(N0), dtype=torch.float32, device=dev)
         self.b32 = torch.zeros(N0, dtype=torch.float32, device=dev)
-        self.x64 = torch.zeros(N0, dtype=torch.float64, device=dev)
-        self.r64 = torch.zeros(N0, dtype=torch.float64, device=dev)
+        self.x64 = torch.zeros(ilv_len(N0), dtype=torch.float64, device=dev)
+        self.r64 = torch.zeros(ilv_len(N0), dtype=torch.float64, device=dev)
         so = TORCH_STORE[p.outer]
         self.P = [torch.zeros(N0, dtype=so, device=dev), torch.zeros(N0, dtype=so, device=dev)]
         # z = V(rlow) gets its own buffer when it is wider than the cycle's vectors
@@ -265,7 +297,7 @@ class Instance:
         nc = math.prod(lv[-1])
         eo = ELEM[params.outer]
         z = N0 * eo if params.outer != params.storage else 0
-        return tot + 3 * _align(N0) * 4 + N0 * (4 + 16 + 2 * eo) + z + nc * nc * 8
+        return tot + 3 * _align(N0) * 4 + N0 * (4 + 2 * eo) + 16 * ilv_len(N0) + z + nc * nc * 8
 
     def faces(self, lev: int, low: bool) -> List[int]:
         """Addresses of the six face arrays of a level (storage copy if ``low``)."""
@@ -302,8 +334,8 @@ class Instance:
         self.b32.copy_(b)
 
     def solution(self) -> torch.Tensor:
-        """FP64 solution as a (D,H,W) view."""
-        return self.x64.view(self.shape)
+        """FP64 solution, (D,H,W), natural order (a copy of the solver vector)."""
+        return deinterleave(self.x64, self.N, self.shape[2]).view(self.shape)
 
 
 @dataclasses.dataclass

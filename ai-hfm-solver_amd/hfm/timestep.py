@@ -186,7 +186,7 @@ class UrbanProjection(UrbanFlow):
         inst = self.solver.insts[0]
         inst.b32.copy_(b.reshape(-1))
         self.last_info = self.solver.solve()
-        return inst.x64.view(self.shape).float()
+        return inst.solution().float()
 
     def step(self, u, v, w, p, alpha):
         shape = tuple(u.shape)

@@ -14,7 +14,6 @@ from __future__ import annotations
 from typing import List, Tuple
 
 import numpy as np
-import torch
 
 EDGES = (32, 48, 64, 96, 128)
 
@@ -48,6 +47,7 @@ def dipoles(shape, n_dipoles: int, seed: int) -> List[Tuple[Tuple[int, int, int]
 
 def device_fields(shape, sigma_ln: float, n_dipoles: int, seed: int, device):
     """(sx, sy, sz, b) FP32 tensors on ``device``."""
+    import torch
     g = torch.Generator(device=device)
     g.manual_seed(int(seed))
     sig = []

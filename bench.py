@@ -217,14 +217,14 @@ def urban(args, dev, torch):
                         "mg_iters_mean": round(float(np.mean(its)), 2),
                         "rel_residual": float(flow.last_info["rel"][0])}
         inst = flow.solver.insts[0]
-        finals[name] = (inst.b32.clone(), inst.x64.clone())
+        finals[name] = (inst.b32.clone(), inst.solution().clone())
         if name == "f32":
             ref_solver = UrbanProjection(sh, sigma=sig, inflow=prof, dt=dt, storage="f32",
                                          tol=1e-12, device=dev).solver
     for name, (b, x) in finals.items():
         ref_solver.insts[0].b32.copy_(b)
         ref_solver.solve()
-        xr = ref_solver.insts[0].x64
+        xr = ref_solver.insts[0].solution()
         out[name]["pressure_rel_err_vs_fp64"] = float((x - xr).norm() / xr.norm())
     log(f"bench: urban {out}")
     return out
@@ -385,9 +385,11 @@ def profile_iteration(mg, cells, torch):
     s = L.stream_handle(mg.device)
     names = {0: "zero", 1: "plain", 2: "correct"}
     rows = []
-    for op in plan.ops_down + [o for ups in plan.ops_up for o in ups]:
+    ops = [("down", op) for op in plan.ops_down] + [("up", o) for ups in plan.ops_up for o in ups]
+    for leg, op in ops:
         kind, mode, lev = op[0], op[1], op[5]
-        label = f"L{lev} " + ("restrict" if kind == "restrict" else "sweep " + names[mode])
+        label = f"{leg} L{lev} " + ("restrict" if kind == "restrict" else "sweep " + names[mode])
+        label += " + r.z partials" if op[4] else ""
         rows.append((label, kernel_time_ms(lambda op=op: plan._launch(op, s), 5, torch)))
     rows.append((f"coarse cycle (levels >= {plan.chain_levels})",
                  kernel_time_ms(lambda: plan.launch_coarse_cycle(s), 5, torch)))
@@ -506,48 +508,77 @@ def profile_kernel(args, dev, torch):
           flush=True)
 
 
+# HBM bytes per k_fcg_update launch of the headline batch (512 instances, seed
+# 2024, BF16 storage and search directions), from two rocprofv3 --pmc runs of
+# ``bench.py --traffic`` (profiles/k_fcg_update_traffic_batched512.md):
+# FETCH_SIZE 5288583.5 KiB x 2.000 + WRITE_SIZE 6765008.3 KiB x 1.000, the
+# factors being what a 1 GiB device copy reads on each counter (524299.5 KiB
+# fetched, 1048576.0 KiB written).  1.026 x the algorithmic 17.31 GB.
+UPDATE_TRAFFIC = {"workload": (512, 2024, "bf16", "bf16"),
+                  "bytes": int(round((5288583.5 * 2.0 + 6765008.3 * 1.0) * 1024))}
+
+
 def traffic_driver(args):
-    """Torch-free driver for hardware-counter runs of k_fcg_update: one
-    (--traffic-edge)^3 BF16 instance's update buffers from hipMalloc (contents
-    irrelevant: the kernel's traffic does not depend on them), every block
-    active, ``--reps`` launches, then the same number of device-to-device
-    copies of a known size (the counter calibration).  Prints one JSON line."""
+    """Torch-free driver for hardware-counter runs of the dominant kernel,
+    k_fcg_update (the _sweep form the solver launches, BF16 storage).
+
+    The work tables and buffers of the headline batch (``--instances`` shapes
+    from ``--seed``, the same draw as the solve workload) or, with
+    ``--traffic-edge E``, of one E^3 instance come from hipMalloc; every
+    instance is active.  The kernel's memory traffic does not depend on the
+    buffer contents.  ``--reps`` launches are timed with HIP events, then the
+    same number of device-to-device copies of 1 GiB (the counter calibration).
+    Prints one JSON line."""
     import ctypes as C
     os.environ["HFM_NO_TORCH"] = "1"
     from hfm import _lib as L
-    lib = L.load()
+    from hfm.workloads import batched_specs
+    CHUNK = 8 * 256           # cells per k_fcg_update block (hfm.mg.CHUNK; mg imports torch)
     hip = C.CDLL("libamdhip64.so")
     vp = C.c_void_p
 
     def malloc(nbytes):
         p = vp()
         if hip.hipMalloc(C.byref(p), C.c_size_t(nbytes)) != 0:
-            raise SystemExit("hipMalloc failed")
-        hip.hipMemset(p, 0, C.c_size_t(nbytes))
+            raise SystemExit(f"hipMalloc({nbytes}) failed")
+        if hip.hipMemset(p, 0, C.c_size_t(nbytes)) != 0:
+            raise SystemExit("hipMemset failed")
         return p.value
 
     def upload(arr):
         p = malloc(arr.nbytes)
-        hip.hipMemcpy(vp(p), arr.ctypes.data_as(vp), C.c_size_t(arr.nbytes), 1)  # host->device
+        if hip.hipMemcpy(vp(p), arr.ctypes.data_as(vp), C.c_size_t(arr.nbytes), 1) != 0:
+            raise SystemExit("hipMemcpy failed")
         return p
 
-    e = args.traffic_edge
-    D = H = W = e
-    N = D * H * W
-    nblk = ((N + 2047) // 2048 + 7) // 8 * 8
-    t = np.zeros((), dtype=L.TASK_DTYPE)
-    t["D"], t["H"], t["W"], t["blk0"], t["nblk"] = D, H, W, 0, nblk
-    t["magW"], t["magH"] = L.magic(W), L.magic(H)
-    faces = [malloc(4 * N) for _ in range(3)] + [malloc(4 * D * H), malloc(4 * D * W), malloc(4 * H * W)]
-    t["c"] = faces
-    t["xin"], t["xout"], t["rlow"] = malloc(2 * N), malloc(2 * N), malloc(2 * N)
-    t["x64"], t["r64"] = malloc(8 * N), malloc(8 * N)
-    tasks = upload(np.array([t]).view(np.uint8))
-    blk = upload(np.zeros(nblk, dtype=np.uint16))
-    sc = np.zeros(1, dtype=L.SCALARS_DTYPE)
-    sc["active"], sc["alpha"], sc["nslot"] = 1, 0.5, nblk
+    if args.traffic_edge:
+        shapes = [(args.traffic_edge,) * 3]
+        what = f"one {args.traffic_edge}^3 instance"
+    else:
+        shapes = [sp[0] for sp in batched_specs(args.instances, args.seed)]
+        what = f"{args.instances} instances, edges from seed {args.seed}"
+    n = len(shapes)
+    cells = np.array([D * H * W for (D, H, W) in shapes], dtype=np.int64)
+    nblk = ((cells + CHUNK - 1) // CHUNK + 7) // 8 * 8
+    blk0 = np.concatenate([[0], np.cumsum(nblk)[:-1]])
+    total = int(nblk.sum())
+    t = np.zeros(n, dtype=L.TASK_DTYPE)
+    for q, (D, H, W) in enumerate(shapes):
+        N = D * H * W
+        r = t[q]
+        r["D"], r["H"], r["W"], r["inst"] = D, H, W, q
+        r["blk0"], r["nblk"], r["slot0"] = blk0[q], nblk[q], blk0[q]
+        r["magW"], r["magH"] = L.magic(W), L.magic(H)
+        r["c"] = [malloc(4 * N) for _ in range(3)] + [malloc(4 * D * H), malloc(4 * D * W),
+                                                      malloc(4 * H * W)]
+        r["xin"], r["xout"], r["rlow"] = malloc(2 * N), malloc(2 * N), malloc(2 * N)
+        r["x64"], r["r64"] = malloc(8 * ((N + 511) // 512 * 512)), malloc(8 * ((N + 511) // 512 * 512))
+    tasks = upload(t.view(np.uint8))
+    blk = upload(np.repeat(np.arange(n, dtype=np.uint16), nblk))
+    sc = np.zeros(n, dtype=L.SCALARS_DTYPE)
+    sc["active"], sc["alpha"], sc["slot0"], sc["nslot"] = 1, 0.5, blk0, nblk
     scp = upload(sc.view(np.uint8))
-    dots = malloc(8 * nblk)
+    dots = malloc(8 * total)
     hip.hipDeviceSynchronize()
 
     def timed(fn, reps):
@@ -564,16 +595,17 @@ def traffic_driver(args):
         hip.hipEventElapsedTime(C.byref(ms), ev[0], ev[1])
         return ms.value / reps
 
-    upd = lambda: L.call("hfm_fcg_update_sweep", tasks, blk, nblk, scp, L.STORE["bf16"],  # noqa: E731
+    upd = lambda: L.call("hfm_fcg_update_sweep", tasks, blk, total, scp, L.STORE["bf16"],  # noqa: E731
                          L.STORE["bf16"], C.c_float(2.0 / 3.0), dots, None)
     ms = timed(upd, args.reps)
     nbytes = 1 << 30
     src, dst = malloc(nbytes), malloc(nbytes)
     cp = lambda: hip.hipMemcpy(vp(dst), vp(src), C.c_size_t(nbytes), 3)  # noqa: E731 device->device
     cp_ms = timed(cp, args.reps)
-    alg = N * 50
-    print(json.dumps({"kernel": "k_fcg_update<bf16, true, bf16>", "edge": e, "cells": N,
-                      "launches": args.reps + 1, "alg_bytes_per_launch": alg,
+    N = int(cells.sum())
+    alg = N * 50              # bf16 MGBatch.update_bytes_per_cell(): 2*2 + 2 + 12 + 32
+    print(json.dumps({"kernel": "k_fcg_update<bf16, true, bf16>", "workload": what, "cells": N,
+                      "blocks": total, "launches": args.reps + 1, "alg_bytes_per_launch": alg,
                       "alg_read_bytes": N * 30, "alg_write_bytes": N * 20,
                       "launch_ms": round(ms, 4), "achieved_GBs": round(alg / (ms * 1e-3) / 1e9, 1),
                       "copy_bytes_each_way": nbytes, "copy_launches": args.reps + 1,
@@ -614,7 +646,7 @@ def main():
     ap.add_argument("--flow-ensemble-steps", type=int, default=20)
     ap.add_argument("--traffic", action="store_true",
                     help="torch-free k_fcg_update driver for hardware-counter runs")
-    ap.add_argument("--traffic-edge", type=int, default=256)
+    ap.add_argument("--traffic-edge", type=int, default=0)
     args = ap.parse_args()
     if args.traffic:
         return traffic_driver(args)
@@ -685,9 +717,12 @@ def main():
     achieved = upd_bytes / (upd_ms * 1e-3) / 1e9
     sweep_ms = kernel_time_ms(lambda: mg.launch_fine_sweep(), 20, torch)
     sweep_bytes = fine_cells * mg.sweep_bytes_per_cell()
+    traffic = None
+    if (args.instances, args.seed, mg.params.storage, mg.params.outer) == UPDATE_TRAFFIC["workload"]:
+        traffic = UPDATE_TRAFFIC["bytes"]
     roofline = {"bound": "hbm", "kernel": "k_fcg_update", "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": None, "launch_ms": round(upd_ms, 4),
+                "traffic": traffic, "launch_ms": round(upd_ms, 4),
                 "alg_bytes_per_launch": int(upd_bytes),
                 "fine_sweep": {"kernel": "k_sweep PLAIN", "launch_ms": round(sweep_ms, 4),
                                "achieved": round(sweep_bytes / (sweep_ms * 1e-3) / 1e9, 1)}}

@@ -59,9 +59,11 @@ def test_coarse_inverse(dev):
 
 
 @pytest.mark.parametrize("storage,outer", [("f32", None), ("bf16", None), ("bf16", "f32")])
-@pytest.mark.parametrize("shape", [(32, 32, 32), (20, 12, 36)])
+@pytest.mark.parametrize("shape", [(32, 32, 32), (20, 12, 36), (20, 13, 16)])
 def test_solve_matches_oracle(dev, storage, outer, shape):
-    # W = 32: 8-wide row kernels on the fine levels; W = 36: per-cell kernels.
+    # W = 32: 8-wide row kernels on the fine levels; W = 36: per-cell kernels;
+    # (20, 13, 16): 8-wide rows with a partly filled last 512-cell segment of
+    # the lane-interleaved FP64 vectors.
     # outer = "f32": BF16 V-cycle with z and p kept in FP32.
     spec = (shape, 7, 1.0, 1)
     mg, probs = _batch(dev, [spec], storage, tol=1e-8, maxit=100, outer=outer)

@@ -107,7 +107,7 @@ def test_urban_projection_pressure_solve(dev, storage):
     op = O.harmonic_faces(ones, ones, ones)
     op.cx[:, :, 0] = 0; op.cy[:, 0, :] = 0; op.cz[0] = 0; op.byh[:] = 0; op.bzh[:] = 0
     x64 = O.solve_fp64_direct(op, b)
-    x = inst.x64.cpu().numpy().reshape(sh)
+    x = inst.solution().cpu().numpy()
     assert rel(x, x64) < 1e-6
     # the inflow profile sits on the x faces
     assert rel(H(state[0])[2, 5, 0], log_law_profile(sh[0])[2]) < 1e-6

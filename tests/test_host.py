@@ -195,3 +195,34 @@ def test_plan_rejects_bad_shapes():
         MGBatch([(4, 4, 4)], device="cpu")
     with pytest.raises(ValueError):
         MGBatch([(32, 32, 32)], storage="f16", device="cpu")
+
+
+def _ilv_cell(n):
+    """Position of cell n in a lane-interleaved FP64 vector, as the kernels'
+    ilv_cell computes it (512-cell segments, lane l owns cells 8l..8l+7)."""
+    seg, o = n & ~511, n & 511
+    return seg + 128 * ((o & 7) >> 1) + 2 * (o >> 3) + (o & 1)
+
+
+@pytest.mark.parametrize("n,width", [(512, 8), (4160, 16), (32768, 32), (2080, 7), (1, 8)])
+def test_fp64_vector_interleave(n, width):
+    torch = pytest.importorskip("torch")
+    from hfm.mg import ILV_SEG, deinterleave, ilv_len, interleave
+    x = torch.arange(n, dtype=torch.float64) + 1.0
+    st = interleave(x, width)
+    assert st.numel() == ilv_len(n) and ilv_len(n) % ILV_SEG == 0 and ilv_len(n) - n < ILV_SEG
+    assert torch.equal(deinterleave(st, n, width), x)
+    if width % 8:
+        assert torch.equal(st[:n], x)
+        return
+    # the position every kernel uses for cell m
+    pos = torch.tensor([_ilv_cell(m) for m in range(n)])
+    assert torch.equal(st[pos], x)
+    # lane l's run: four 16-byte pairs at segment offsets 128 q + 2 l
+    for seg in range(0, ilv_len(n), ILV_SEG):
+        for lane in (0, 1, 37, 63):
+            for q in range(4):
+                for b in range(2):
+                    m = seg + 8 * lane + 2 * q + b
+                    want = float(m + 1) if m < n else 0.0
+                    assert float(st[seg + 128 * q + 2 * lane + b]) == want
