@@ -781,8 +781,15 @@ __global__ __launch_bounds__(kBlock) void k_fcg_dir(const hfm_mg_task* __restric
   if (threadIdx.x == 0) dots[t.slot0 + chunk] = s;
 }
 
-// x64 += alpha p; r64 -= alpha A p; rlow = T(r64); dots = r . r.
+// r64 -= alpha A p; rlow = T(r64); dots = r . r; x64 every other iteration.
 // p is stored as P (the search-direction type), rlow and xout as T.
+// The solution update is deferred: with S.xlag == 0 x64 is not touched (the
+// finalize CHECK then records alpha_prev and xlag = 1); with S.xlag == 1
+// x64 = fma(alpha, p, fma(alpha_prev, p_prev, x64)) with p_prev in ec (the
+// other search-direction buffer), the two updates in their own order, so the
+// result is bitwise that of updating x64 every iteration while x64 moves
+// through HBM on half of the iterations.  k_fcg_flush applies a pending
+// update once an instance stops.
 // SWEEP: also the next preconditioner's first (zero initial guess) fine-level
 // sweep, xout = omega rlow / d with d the diagonal of the storage-rounded faces
 // (identical to a separate HFM_SWEEP_ZERO launch).
@@ -799,7 +806,10 @@ __global__ __launch_bounds__(kBlock) void k_fcg_update(const hfm_mg_task* __rest
   const int N = t.D * t.H * t.W;
   const int base = chunk * kCellChunk;
   const double alpha = S.alpha;
+  const bool lag = S.xlag != 0;
+  const double alpha_prev = S.alpha_prev;
   const SrcPlain<P> src{gin<P>(t.xin)};
+  const SrcPlain<P> prev{gin<P>(t.ec)};
   HFM_GLOBAL T* rlow = gout<T>(t.rlow);
   HFM_GLOBAL T* x0 = gout<T>(t.xout);
   HFM_GLOBAL double* x64 = gout<double>(t.x64);
@@ -811,9 +821,13 @@ __global__ __launch_bounds__(kBlock) void k_fcg_update(const hfm_mg_task* __rest
     const Run r = run_at(t, n0, valid);
     RunCoef<float> q;
     q.load(t, r);
-    double xv[8], rv[8];
-    ld8d_ilv(x64, r.n0, xv);
+    double rv[8], xv[8];
+    float pp[8];
     ld8d_ilv(r64, r.n0, rv);
+    if (lag) {
+      ld8d_ilv(x64, r.n0, xv);
+      prev.run(r.n0, 0, 0, 0, pp);
+    }
     double d[8], p[8], off[8];
     q.diag(d);
     run_offdiag<double>(t, r, q, src, p, off);
@@ -821,14 +835,17 @@ __global__ __launch_bounds__(kBlock) void k_fcg_update(const hfm_mg_task* __rest
       float rl[8];
 #pragma unroll
       for (int m = 0; m < 8; ++m) {
-        xv[m] += alpha * p[m];
         rv[m] = rv[m] - alpha * (d[m] * p[m] - off[m]);
         rl[m] = float(rv[m]);
         acc += rv[m] * rv[m];
       }
-      st8d_ilv(x64, n0, xv);
       st8d_ilv(r64, n0, rv);
       st8(rlow + n0, rl);
+      if (lag) {
+#pragma unroll
+        for (int m = 0; m < 8; ++m) xv[m] = fma(alpha, p[m], fma(alpha_prev, double(pp[m]), xv[m]));
+        st8d_ilv(x64, n0, xv);
+      }
       if (SWEEP) {
         float o[8];
 #pragma unroll
@@ -851,7 +868,7 @@ __global__ __launch_bounds__(kBlock) void k_fcg_update(const hfm_mg_task* __rest
       const double off = face_row<double, float>(t, n, i, j, k, pat, diag);
       const double pn = pat(n, k, j, i);
       const double q = diag * pn - off;
-      x64[n] += alpha * pn;
+      if (lag) x64[n] = fma(alpha, pn, fma(alpha_prev, double(prev.one(n, 0, 0, 0)), x64[n]));
       const double rr = r64[n] - alpha * q;
       r64[n] = rr;
       const T rt = from_f<T>(float(rr));
@@ -864,11 +881,55 @@ __global__ __launch_bounds__(kBlock) void k_fcg_update(const hfm_mg_task* __rest
   if (threadIdx.x == 0) dots[t.slot0 + chunk] = s;
 }
 
+// The pending half of the deferred solution update of instances that have
+// stopped: x64 = fma(alpha_prev, p_prev, x64), p_prev in xin (xlag 1: the
+// direction of the last update) or ec (xlag 2: the one before, when the
+// instance stopped between its direction and its update).  Same tasks as
+// k_fcg_update of the launch that preceded the stopping CHECK.
+template <typename P>
+__global__ __launch_bounds__(kBlock) void k_fcg_flush(const hfm_mg_task* __restrict__ tasks,
+                                                      const uint16_t* __restrict__ blk_task,
+                                                      const hfm_mg_scalars* __restrict__ sc) {
+  const int g = blockIdx.x;
+  const hfm_mg_task& t = tasks[blk_task[g]];
+  const hfm_mg_scalars& S = sc[t.inst];
+  if (S.active || S.xlag == 0) return;
+  const int chunk = xcd_chunk(g - t.blk0, t.nblk);
+  const int N = t.D * t.H * t.W;
+  const int base = chunk * kCellChunk;
+  const double ap = S.alpha_prev;
+  const SrcPlain<P> prev{gin<P>(S.xlag == 1 ? t.xin : t.ec)};
+  HFM_GLOBAL double* x64 = gout<double>(t.x64);
+  if (vec_task(t)) {
+    const int n0 = base + threadIdx.x * kRun;
+    if (n0 < N) {
+      double xv[8];
+      float pp[8];
+      ld8d_ilv(x64, n0, xv);
+      prev.run(n0, 0, 0, 0, pp);
+#pragma unroll
+      for (int m = 0; m < 8; ++m) xv[m] = fma(ap, double(pp[m]), xv[m]);
+      st8d_ilv(x64, n0, xv);
+    }
+  } else {
+    for (int s = 0; s < kRun; ++s) {
+      const int n = base + s * kBlock + threadIdx.x;
+      if (n >= N) break;
+      x64[n] = fma(ap, double(prev.one(n, 0, 0, 0)), x64[n]);
+    }
+  }
+}
+
 // ------------------------------------------------------------------ per-instance scalars
 __global__ __launch_bounds__(kBlock) void k_fcg_finalize(hfm_mg_scalars* sc, const double* dots,
                                                          int kind) {
   hfm_mg_scalars& S = sc[blockIdx.x];
-  if (!S.active) return;
+  if (!S.active) {
+    // a stopped instance's pending solution update was applied by the flush
+    // after its last CHECK; BETA opens the next iteration
+    if (kind == HFM_FIN_BETA && threadIdx.x == 0) S.xlag = 0;
+    return;
+  }
   // INIT applies to slots staged by the host (iter == -1), the other steps to
   // initialised slots; a pool can then admit instances while others iterate.
   if ((kind == HFM_FIN_INIT) != (S.iter == -1)) return;
@@ -884,6 +945,8 @@ __global__ __launch_bounds__(kBlock) void k_fcg_finalize(hfm_mg_scalars* sc, con
       S.iter = 0;
       S.rz = 0.0;
       S.beta = 0.0;
+      S.xlag = 0;
+      S.alpha_prev = 0.0;
       if (s == 0.0) {  // x = 0 solves b = 0
         S.rel = 0.0;
         S.status = HFM_MG_CONVERGED;
@@ -899,6 +962,7 @@ __global__ __launch_bounds__(kBlock) void k_fcg_finalize(hfm_mg_scalars* sc, con
       if (!isfinite(s)) {
         S.status = HFM_MG_BREAKDOWN;
         S.active = 0;
+        if (S.xlag) S.xlag = 2;  // pending p is now the update's previous direction
       }
       break;
     case HFM_FIN_ALPHA:
@@ -906,6 +970,7 @@ __global__ __launch_bounds__(kBlock) void k_fcg_finalize(hfm_mg_scalars* sc, con
       if (!(s > 0.0) || !isfinite(s)) {
         S.status = HFM_MG_BREAKDOWN;
         S.active = 0;
+        if (S.xlag) S.xlag = 2;
       } else {
         S.alpha = S.rz / s;
       }
@@ -914,6 +979,10 @@ __global__ __launch_bounds__(kBlock) void k_fcg_finalize(hfm_mg_scalars* sc, con
       S.rr = s;
       S.rel = sqrt(s / S.bnorm2);
       S.iter += 1;
+      // the update just run applied x's pending half and its own (xlag 1)
+      // or left its own pending (xlag 0)
+      S.xlag = S.xlag ? 0 : 1;
+      if (S.xlag) S.alpha_prev = S.alpha;
       if (S.rel <= S.tol) {
         S.status = HFM_MG_CONVERGED;
         S.active = 0;
@@ -1321,6 +1390,17 @@ extern "C" int hfm_fcg_dir(const hfm_mg_task* tasks, const uint16_t* blk_task, i
                          as_stream(stream), tasks, blk_task, sc, HFM_OMEGA_VAL, dots);          \
     });                                                                                         \
   }
+
+extern "C" int hfm_fcg_flush(const hfm_mg_task* tasks, const uint16_t* blk_task, int64_t nblocks,
+                             const hfm_mg_scalars* sc, int pstore, void* stream) {
+  if (int e = check_table(tasks, blk_task, nblocks, sc, "hfm_fcg_flush")) return e;
+  if (nblocks == 0) return HFM_OK;
+  return launch_store(pstore, "hfm_fcg_flush", [&](auto tag) {
+    using P = decltype(tag);
+    hipLaunchKernelGGL((k_fcg_flush<P>), dim3(unsigned(nblocks)), dim3(kBlock), 0,
+                       as_stream(stream), tasks, blk_task, sc);
+  });
+}
 
 #define HFM_OMEGA_ARG
 #define HFM_OMEGA_VAL 1.0f

@@ -46,7 +46,8 @@ SCALARS_DTYPE = np.dtype([
     ("bnorm2", "<f8"), ("rz", "<f8"), ("alpha", "<f8"), ("beta", "<f8"),
     ("rr", "<f8"), ("rel", "<f8"), ("tol", "<f8"), ("pap", "<f8"),
     ("iter", "<i4"), ("maxit", "<i4"), ("active", "<i4"), ("status", "<i4"),
-    ("slot0", "<i4"), ("nslot", "<i4"), ("pad0", "<i4"), ("pad1", "<i4"),
+    ("slot0", "<i4"), ("nslot", "<i4"), ("xlag", "<i4"), ("pad1", "<i4"),
+    ("alpha_prev", "<f8"),
 ])
 
 _vp = C.c_void_p
@@ -82,6 +83,7 @@ SIGNATURES = {
     "hfm_fcg_update": (_int, [_vp, _vp, _i64, _vp, _int, _int, _vp, _vp]),
     "hfm_fcg_init_sweep": (_int, [_vp, _vp, _i64, _vp, _int, _f32, _vp, _vp]),
     "hfm_fcg_update_sweep": (_int, [_vp, _vp, _i64, _vp, _int, _int, _f32, _vp, _vp]),
+    "hfm_fcg_flush": (_int, [_vp, _vp, _i64, _vp, _int, _vp]),
     "hfm_fcg_finalize": (_int, [_vp, _i64, _vp, _int, _vp]),
     "hfm_mg_faces": (_int, [_vp, _vp, _vp, _vp, _i64, _i64, _i64, _f32, _vp]),
     "hfm_mg_galerkin": (_int, [_vp, _vp, _i64, _i64, _i64, _vp]),

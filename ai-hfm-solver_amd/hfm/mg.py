@@ -209,6 +209,9 @@ class Pool:
     def active_view(self):
         return self.sc.view(torch.int32).view(self.capacity, -1)[:, 18]
 
+    def xlag_view(self):
+        return self.sc.view(torch.int32).view(self.capacity, -1)[:, 22]
+
     def stage(self, insts: Sequence["Instance"]):
         """Mark the instances' slots for (re)initialisation (active = 1, iter = -1)."""
         if not insts:
@@ -457,6 +460,7 @@ def instance_templates(inst: Instance, p: Params) -> dict:
         tpl[("dir", par)] = d
         u = base.copy()
         u["xin"] = inst.P[1 - par].data_ptr()
+        u["ec"] = inst.P[par].data_ptr()      # the previous direction (deferred x update)
         tpl[("update", par)] = u
     return tpl
 
@@ -647,14 +651,21 @@ class Plan:
                fl.nblocks, pool.sc.data_ptr(), pool.store_id, pool.outer_id, pool.params.omega,
                pool.dots.data_ptr(), s)
 
+    def launch_flush(self, parity, s):
+        pool, fl = self.pool, self.flay
+        L.call("hfm_fcg_flush", self.upd_tasks[parity].data_ptr(), fl.blk_task.data_ptr(),
+               fl.nblocks, pool.sc.data_ptr(), pool.outer_id, s)
+
     def iteration(self, s, parity):
-        """One FCG iteration for every active instance of the plan."""
+        """One FCG iteration for every active instance of the plan; instances
+        that stop get the pending half of their deferred solution update."""
         self.precondition(s)
         self.pool.finalize("beta", s)
         self.launch_dir(parity, s)
         self.pool.finalize("alpha", s)
         self.launch_update(parity, s)
         self.pool.finalize("check", s)
+        self.launch_flush(parity, s)
 
     def graph(self, m: int):
         """A HIP graph of ``m`` (rounded up to even) iterations, captured once per plan."""
@@ -817,16 +828,24 @@ class MGBatch:
     def launch_update(self, parity: int = 0):
         self.plan.launch_update(parity, L.stream_handle(self.device))
 
+    def set_xlag(self, lag: int):
+        """Select the update kernel's variant for timing: 1 = with the solution
+        update (every other iteration), 0 = without.  ``restart`` resets it."""
+        idx = torch.tensor([i.slot for i in self.insts], device=self.device)
+        self.pool.xlag_view().index_fill_(0, idx, int(lag))
+
     def launch_fine_sweep(self):
         """The fine-level PLAIN (second pre-) sweep of the V-cycle."""
         op = next(o for o in self.plan.ops_down if o[0] == "sweep" and o[1] == L.SWEEP["plain"])
         self.plan._launch(op, L.stream_handle(self.device))
 
-    def update_bytes_per_cell(self) -> int:
-        """Algorithmic bytes of the fused k_fcg_update per fine cell: p (outer type),
-        3 FP32 faces, x64 and r64 read + written, rlow (T) and the first sweep (T)
-        written."""
-        return 2 * ELEM[self.storage] + ELEM[self.params.outer] + 12 + 32
+    def update_bytes_per_cell(self, lag: int = 1) -> int:
+        """Algorithmic bytes of the fused k_fcg_update per fine cell: p (outer
+        type), 3 FP32 faces, r64 read + written, rlow (T) and the first sweep
+        (T) written; with ``lag`` (every other iteration) also x64 read +
+        written and the previous direction read."""
+        eo = ELEM[self.params.outer]
+        return 2 * ELEM[self.storage] + eo + 12 + 16 + (16 + eo if lag else 0)
 
     def sweep_bytes_per_cell(self) -> int:
         """Algorithmic bytes of a PLAIN sweep per cell: x, b, 3 faces, out (all T)."""

@@ -394,9 +394,17 @@ def profile_iteration(mg, cells, torch):
     rows.append((f"coarse cycle (levels >= {plan.chain_levels})",
                  kernel_time_ms(lambda: plan.launch_coarse_cycle(s), 5, torch)))
     rows.append(("fcg dir", kernel_time_ms(lambda: plan.launch_dir(0, s), 5, torch)))
-    rows.append(("fcg update", kernel_time_ms(lambda: plan.launch_update(0, s), 5, torch)))
+    upd = []
+    for lag in (1, 0):
+        mg.set_xlag(lag)
+        upd.append(kernel_time_ms(lambda: plan.launch_update(0, s), 5, torch))
+    mg.set_xlag(0)
+    rows.append(("fcg update, mean of the two variants", 0.5 * (upd[0] + upd[1])))
+    rows.append(("fcg flush", kernel_time_ms(lambda: plan.launch_flush(0, s), 5, torch)))
     rows.append(("finalize", kernel_time_ms(lambda: pool.finalize("check", s), 5, torch)))
     total = sum(t for _, t in rows)
+    rows += [("  fcg update with x (odd iterations)", upd[0]),
+             ("  fcg update without x (even iterations)", upd[1])]
     print(json.dumps({"fine_cells": cells, "iteration_ms": round(total, 3),
                       "launches": {k: round(v, 4) for k, v in rows}}), flush=True)
 
@@ -492,16 +500,27 @@ def profile_kernel(args, dev, torch):
             mg.restart()
             profile_iteration(mg, cells, torch)
         return None
-    fn = {"update": lambda: mg.launch_update(0), "sweep": mg.launch_fine_sweep}[args.profile_kernel]
-    per_cell = {"update": mg.update_bytes_per_cell(), "sweep": mg.sweep_bytes_per_cell()}
-    ms = kernel_time_ms(fn, args.reps, torch)
+    if args.profile_kernel == "update":
+        # the two variants alternate: with the deferred solution update, without
+        times, per = [], []
+        for lag in (1, 0):
+            mg.set_xlag(lag)
+            times.append(kernel_time_ms(lambda: mg.launch_update(0), args.reps, torch))
+            per.append(mg.update_bytes_per_cell(lag))
+        ms, per_cell = 0.5 * sum(times), {"update": 0.5 * sum(per)}
+        variants = {"with_x_ms": round(times[0], 4), "without_x_ms": round(times[1], 4),
+                    "bytes_per_cell": per}
+    else:
+        variants = {}
+        per_cell = {"sweep": mg.sweep_bytes_per_cell()}
+        ms = kernel_time_ms(mg.launch_fine_sweep, args.reps, torch)
     nbytes = 1 << 30
     src = torch.empty(nbytes // 4, dtype=torch.float32, device=dev).fill_(1.0)
     dst = torch.empty_like(src)
     cp_ms = kernel_time_ms(lambda: dst.copy_(src), 5, torch)
     alg = cells * per_cell[args.profile_kernel]
     print(json.dumps({"kernel": args.profile_kernel, "instances": len(specs), "fine_cells": cells,
-                      "alg_bytes_per_launch": alg, "launch_ms": round(ms, 4),
+                      "alg_bytes_per_launch": alg, "launch_ms": round(ms, 4), **variants,
                       "achieved_GBs": round(alg / (ms * 1e-3) / 1e9, 1), "reps": args.reps + 1,
                       "copy_bytes_each_way": nbytes, "copy_ms": round(cp_ms, 4),
                       "copy_GBs_read_plus_write": round(2 * nbytes / (cp_ms * 1e-3) / 1e9, 1)}),
@@ -509,13 +528,21 @@ def profile_kernel(args, dev, torch):
 
 
 # HBM bytes per k_fcg_update launch of the headline batch (512 instances, seed
-# 2024, BF16 storage and search directions), from two rocprofv3 --pmc runs of
-# ``bench.py --traffic`` (profiles/k_fcg_update_traffic_batched512.md):
-# FETCH_SIZE 5288583.5 KiB x 2.000 + WRITE_SIZE 6765008.3 KiB x 1.000, the
-# factors being what a 1 GiB device copy reads on each counter (524299.5 KiB
-# fetched, 1048576.0 KiB written).  1.026 x the algorithmic 17.31 GB.
-UPDATE_TRAFFIC = {"workload": (512, 2024, "bf16", "bf16"),
-                  "bytes": int(round((5288583.5 * 2.0 + 6765008.3 * 1.0) * 1024))}
+# 2024, BF16 storage and search directions), by variant (with / without the
+# deferred solution update), from rocprofv3 --pmc runs of ``bench.py
+# --traffic --traffic-lag {1,0}`` (profiles/k_fcg_update_traffic_batched512.md):
+# FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 in KiB, the factors being what a
+# 1 GiB device copy reads on each counter (524299.5 KiB fetched, 1048576.0 KiB
+# written).
+UPDATE_TRAFFIC = {"workload": (512, 2024, "bf16", "bf16"), "kib": {1: None, 0: None}}
+
+
+def update_traffic():
+    """Measured HBM bytes of the mean update launch (both variants), or None."""
+    kib = UPDATE_TRAFFIC["kib"]
+    if any(v is None for v in kib.values()):
+        return None
+    return int(round(sum(f * 2.0 + w * 1.0 for f, w in kib.values()) * 1024 / len(kib)))
 
 
 def traffic_driver(args):
@@ -572,11 +599,13 @@ def traffic_driver(args):
         r["c"] = [malloc(4 * N) for _ in range(3)] + [malloc(4 * D * H), malloc(4 * D * W),
                                                       malloc(4 * H * W)]
         r["xin"], r["xout"], r["rlow"] = malloc(2 * N), malloc(2 * N), malloc(2 * N)
+        r["ec"] = malloc(2 * N)
         r["x64"], r["r64"] = malloc(8 * ((N + 511) // 512 * 512)), malloc(8 * ((N + 511) // 512 * 512))
     tasks = upload(t.view(np.uint8))
     blk = upload(np.repeat(np.arange(n, dtype=np.uint16), nblk))
     sc = np.zeros(n, dtype=L.SCALARS_DTYPE)
     sc["active"], sc["alpha"], sc["slot0"], sc["nslot"] = 1, 0.5, blk0, nblk
+    sc["xlag"], sc["alpha_prev"] = args.traffic_lag, 0.25
     scp = upload(sc.view(np.uint8))
     dots = malloc(8 * total)
     hip.hipDeviceSynchronize()
@@ -603,10 +632,14 @@ def traffic_driver(args):
     cp = lambda: hip.hipMemcpy(vp(dst), vp(src), C.c_size_t(nbytes), 3)  # noqa: E731 device->device
     cp_ms = timed(cp, args.reps)
     N = int(cells.sum())
-    alg = N * 50              # bf16 MGBatch.update_bytes_per_cell(): 2*2 + 2 + 12 + 32
+    # bf16 MGBatch.update_bytes_per_cell(lag): p 2, faces 12, r64 16, rlow 2,
+    # first sweep 2; with the solution update also x64 16 and p_prev 2
+    rd, wr = (30 + 2, 20) if args.traffic_lag else (22, 12)
+    alg = N * (rd + wr)
     print(json.dumps({"kernel": "k_fcg_update<bf16, true, bf16>", "workload": what, "cells": N,
-                      "blocks": total, "launches": args.reps + 1, "alg_bytes_per_launch": alg,
-                      "alg_read_bytes": N * 30, "alg_write_bytes": N * 20,
+                      "xlag": args.traffic_lag, "blocks": total, "launches": args.reps + 1,
+                      "alg_bytes_per_launch": alg,
+                      "alg_read_bytes": N * rd, "alg_write_bytes": N * wr,
                       "launch_ms": round(ms, 4), "achieved_GBs": round(alg / (ms * 1e-3) / 1e9, 1),
                       "copy_bytes_each_way": nbytes, "copy_launches": args.reps + 1,
                       "copy_ms": round(cp_ms, 4)}), flush=True)
@@ -647,6 +680,8 @@ def main():
     ap.add_argument("--traffic", action="store_true",
                     help="torch-free k_fcg_update driver for hardware-counter runs")
     ap.add_argument("--traffic-edge", type=int, default=0)
+    ap.add_argument("--traffic-lag", type=int, choices=(0, 1), default=1,
+                    help="update variant: 1 with the deferred solution update, 0 without")
     args = ap.parse_args()
     if args.traffic:
         return traffic_driver(args)
@@ -712,18 +747,28 @@ def main():
     # dominant kernel: the fine-level FCG update (x += a p, r -= a A p), timed live
     mg.restart()
     fine_cells = cells
-    upd_ms = kernel_time_ms(lambda: mg.launch_update(0), 20, torch)
-    upd_bytes = fine_cells * mg.update_bytes_per_cell()
+    # its two variants alternate (the solution update rides on every other
+    # iteration): a launch here is the mean of one of each
+    upd_t = []
+    for lag in (1, 0):
+        mg.set_xlag(lag)
+        upd_t.append(kernel_time_ms(lambda: mg.launch_update(0), 20, torch))
+    upd_ms = 0.5 * sum(upd_t)
+    upd_bytes = fine_cells * 0.5 * (mg.update_bytes_per_cell(1) + mg.update_bytes_per_cell(0))
     achieved = upd_bytes / (upd_ms * 1e-3) / 1e9
     sweep_ms = kernel_time_ms(lambda: mg.launch_fine_sweep(), 20, torch)
     sweep_bytes = fine_cells * mg.sweep_bytes_per_cell()
     traffic = None
     if (args.instances, args.seed, mg.params.storage, mg.params.outer) == UPDATE_TRAFFIC["workload"]:
-        traffic = UPDATE_TRAFFIC["bytes"]
+        traffic = update_traffic()
     roofline = {"bound": "hbm", "kernel": "k_fcg_update", "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": traffic, "launch_ms": round(upd_ms, 4),
                 "alg_bytes_per_launch": int(upd_bytes),
+                "variants": {"with_x": {"launch_ms": round(upd_t[0], 4),
+                                        "bytes_per_cell": mg.update_bytes_per_cell(1)},
+                             "without_x": {"launch_ms": round(upd_t[1], 4),
+                                           "bytes_per_cell": mg.update_bytes_per_cell(0)}},
                 "fine_sweep": {"kernel": "k_sweep PLAIN", "launch_ms": round(sweep_ms, 4),
                                "achieved": round(sweep_bytes / (sweep_ms * 1e-3) / 1e9, 1)}}
     extra = {}

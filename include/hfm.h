@@ -181,7 +181,11 @@ typedef struct hfm_mg_scalars {
   double bnorm2, rz, alpha, beta;
   double rr, rel, tol, pap;
   int32_t iter, maxit, active, status; /* status: 0 running 1 converged 2 maxit 3 breakdown */
-  int32_t slot0, nslot, pad0, pad1;
+  int32_t slot0, nslot;
+  int32_t xlag;       /* x lacks alpha_prev * p_prev: 0 no, 1 p_prev = the update's p (xin),
+                         2 p_prev = the update's previous p (ec); see hfm_fcg_update */
+  int32_t pad1;
+  double alpha_prev;
 } hfm_mg_scalars;
 
 #define HFM_MG_RUNNING 0
@@ -242,7 +246,16 @@ int hfm_mg_coarse_cycle(const hfm_mg_task* recs, const int32_t* chain, int64_t n
  * search direction p; hfm_fcg_dir's `store` argument is P).
  * init:   x64 = 0, r64 = b32, rlow = T(b32), dots = b.b  (staged slots only)
  * dir:    xout = p = P(b + beta xin) (b = z, xin = p_old), dots = p.(A p)
- * update: x64 += alpha p, r64 -= alpha A p, rlow = T(r64), dots = r.r  (xin = p)
+ * update: r64 -= alpha A p, rlow = T(r64), dots = r.r  (xin = p); the
+ *         solution update is deferred to every other iteration: with xlag = 0
+ *         x64 is left alone (HFM_FIN_CHECK then records alpha_prev = alpha,
+ *         xlag = 1), with xlag = 1 x64 += alpha_prev p_prev, then += alpha p
+ *         (p_prev in ec: the other search-direction buffer), in that order,
+ *         so x64 is bitwise what updating every iteration gives.
+ * flush:  for instances no longer active with xlag != 0, x64 += alpha_prev
+ *         p_prev (xlag 1: p_prev in xin, 2: in ec); launched after every
+ *         HFM_FIN_CHECK with the same update tasks (the next HFM_FIN_BETA
+ *         clears xlag of inactive slots).
  * *_sweep variants also write the next V-cycle's first fine-level sweep from
  * a zero guess, xout = omega rlow / d (d = diagonal of the faces rounded to
  * storage), so that sweep is not launched separately. */
@@ -258,6 +271,8 @@ int hfm_fcg_init_sweep(const hfm_mg_task* tasks, const uint16_t* blk_task, int64
 int hfm_fcg_update_sweep(const hfm_mg_task* tasks, const uint16_t* blk_task, int64_t nblocks,
                          const hfm_mg_scalars* sc, int store, int pstore, float omega,
                          double* dots, void* stream);
+int hfm_fcg_flush(const hfm_mg_task* tasks, const uint16_t* blk_task, int64_t nblocks,
+                  const hfm_mg_scalars* sc, int pstore, void* stream);
 
 /* Per-instance fixed-order reduction of dots[slot0 .. slot0+nslot) and the
  * scalar step:  kind 0 INIT (bnorm2), 1 BETA (rz, beta), 2 ALPHA (pap,

@@ -123,3 +123,49 @@ def test_launch_shapes_do_not_change_results(dev):
             assert list(info["iter"]) == list(iref["iter"]), (cells, table)
             for i in range(len(specs)):
                 assert bool((mg.solution(i) == ref.solution(i)).all()), (cells, i)
+
+
+@pytest.mark.parametrize("maxit", [3, 4])
+def test_stop_at_maxit_keeps_every_solution_update(dev, maxit):
+    # x64 moves on every other iteration; an odd stop leaves the last update
+    # to the flush launched after the stopping CHECK.
+    spec = ((32, 24, 40), 8, 1.5, 2)
+    mg, probs = _batch(dev, [spec], "bf16", tol=1e-12, maxit=maxit, iters_per_graph=2)
+    info = mg.solve()
+    assert int(info["status"][0]) == 2 and int(info["iter"][0]) == maxit
+    op, b = probs[0]
+    ref = O.fcg_solve(op, b, tol=1e-12, storage="bf16", maxit=maxit)
+    x = mg.solution(0).cpu().numpy()
+    assert rel(x, ref.x) < 1e-5
+    # the last iteration's update alone is far above that tolerance
+    prev = O.fcg_solve(op, b, tol=1e-12, storage="bf16", maxit=maxit - 1)
+    assert rel(ref.x, prev.x) > 1e-3
+
+
+def test_stop_between_direction_and_update_applies_the_pending_update(dev):
+    # iteration 0 leaves x's update pending; iteration 1 stops at ALPHA (its
+    # p.Ap partials zeroed from the host: a breakdown), so the pending update
+    # sits in the previous-direction buffer and the flush must take it from there
+    from hfm import _lib as L
+    spec = ((32, 32, 32), 5, 1.0, 1)
+    mg, _ = _batch(dev, [spec], "bf16")
+    plan, pool = mg.plan, mg.pool
+    s = L.stream_handle(mg.device)
+    mg.restart()
+    plan.iteration(s, 0)
+    plan.precondition(s)
+    pool.finalize("beta", s)
+    plan.launch_dir(1, s)
+    pool.dots.zero_()
+    pool.finalize("alpha", s)
+    plan.launch_update(1, s)
+    pool.finalize("check", s)
+    plan.launch_flush(1, s)
+    rec = mg.scalars()[0]
+    assert int(rec["status"]) == 3 and int(rec["iter"]) == 1 and int(rec["active"]) == 0
+    x_break = mg.solution(0).clone()
+    one, _ = _batch(dev, [spec], "bf16", maxit=1, iters_per_graph=2)
+    info = one.solve()
+    assert int(info["status"][0]) == 2 and int(info["iter"][0]) == 1
+    assert float(x_break.abs().max()) > 0
+    assert bool((x_break == one.solution(0)).all())

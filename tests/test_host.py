@@ -31,7 +31,7 @@ def test_abi_struct_sizes_match_host_dtypes():
     a, b = L.C.c_int64(), L.C.c_int64()
     assert lib.hfm_struct_sizes(L.C.byref(a), L.C.byref(b)) == 0
     assert a.value == L.TASK_DTYPE.itemsize == 240
-    assert b.value == L.SCALARS_DTYPE.itemsize == 96
+    assert b.value == L.SCALARS_DTYPE.itemsize == 104
 
 
 def test_flow2d_struct_sizes_match_host_dtypes():
