@@ -534,7 +534,8 @@ def profile_kernel(args, dev, torch):
 # FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 in KiB, the factors being what a
 # 1 GiB device copy reads on each counter (524299.5 KiB fetched, 1048576.0 KiB
 # written).
-UPDATE_TRAFFIC = {"workload": (512, 2024, "bf16", "bf16"), "kib": {1: None, 0: None}}
+UPDATE_TRAFFIC = {"workload": (512, 2024, "bf16", "bf16"),
+                  "kib": {1: (5531802.0, 6763086.1), 0: (3833714.8, 4058341.8)}}
 
 
 def update_traffic():
