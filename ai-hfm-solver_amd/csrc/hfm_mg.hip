@@ -745,9 +745,13 @@ __global__ __launch_bounds__(kBlock) void k_fcg_dir(const hfm_mg_task* __restric
     ld8(gin<float>(t.c[0]) + r.n0, cx);
     ld8(gin<float>(t.c[1]) + r.n0, cy);
     ld8(gin<float>(t.c[2]) + r.n0, cz);
-    ld8(gin<float>(t.c[4]) + r.k * W + r.i0, by);
-    ld8(gin<float>(t.c[5]) + r.j * W + r.i0, bz);
-    const float bx = ldg(gin<float>(t.c[3]) + r.k * t.H + r.j);
+    // high-boundary faces: only waves that hold a boundary run load them
+    float bx = 0.0f;
+#pragma unroll
+    for (int m = 0; m < 8; ++m) by[m] = bz[m] = 0.0f;
+    if (__any(north_b)) ld8(gin<float>(t.c[4]) + r.k * W + r.i0, by);
+    if (__any(up_b)) ld8(gin<float>(t.c[5]) + r.j * W + r.i0, bz);
+    if (__any(east_b)) bx = ldg(gin<float>(t.c[3]) + r.k * t.H + r.j);
     if (valid) {
 #pragma unroll
       for (int m = 0; m < 8; ++m) {
