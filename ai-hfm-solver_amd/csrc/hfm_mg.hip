@@ -737,7 +737,7 @@ __global__ __launch_bounds__(kBlock) void k_fcg_dir(const hfm_mg_task* __restric
     const int W = t.W, HW = t.H * t.W;
     const bool west = r.i0 > 0, south = r.j > 0, down = r.k > 0;
     const bool east_b = r.i0 + kRun == W, north_b = r.j + 1 == t.H, up_b = r.k + 1 == t.D;
-    float p[8], ps[8], pd[8], cx[8], cy[8], cz[8], by[8], bz[8];
+    float p[8], ps[8], pd[8], cx[8], cy[8], cz[8];
     src.run(r.n0, r.k, r.j, r.i0, p);
     src.run(south ? r.n0 - W : r.n0, r.k, r.j, r.i0, ps);
     src.run(down ? r.n0 - HW : r.n0, r.k, r.j, r.i0, pd);
@@ -745,26 +745,35 @@ __global__ __launch_bounds__(kBlock) void k_fcg_dir(const hfm_mg_task* __restric
     ld8(gin<float>(t.c[0]) + r.n0, cx);
     ld8(gin<float>(t.c[1]) + r.n0, cy);
     ld8(gin<float>(t.c[2]) + r.n0, cz);
-    // high-boundary faces: only waves that hold a boundary run load them
-    float bx = 0.0f;
-#pragma unroll
-    for (int m = 0; m < 8; ++m) by[m] = bz[m] = 0.0f;
-    if (__any(north_b)) ld8(gin<float>(t.c[4]) + r.k * W + r.i0, by);
-    if (__any(up_b)) ld8(gin<float>(t.c[5]) + r.j * W + r.i0, bz);
-    if (__any(east_b)) bx = ldg(gin<float>(t.c[3]) + r.k * t.H + r.j);
     if (valid) {
 #pragma unroll
       for (int m = 0; m < 8; ++m) {
         const float dw = p[m] - (m ? p[m - 1] : (west ? pw : 0.0f));
         const float ds = p[m] - (south ? ps[m] : 0.0f);
         const float dd = p[m] - (down ? pd[m] : 0.0f);
-        float e = cx[m] * dw * dw + cy[m] * ds * ds + cz[m] * dd * dd;
-        const float p2 = p[m] * p[m];
-        if (m == 7 && east_b) e += bx * p2;
-        if (north_b) e += by[m] * p2;
-        if (up_b) e += bz[m] * p2;
-        acc += double(e);
+        acc += double(cx[m] * dw * dw + cy[m] * ds * ds + cz[m] * dd * dd);
       }
+    }
+    // the high-boundary faces (b p^2 on the last x, y and z layers): only
+    // waves that hold such a run load them
+    const bool edge = valid && (east_b || north_b || up_b);
+    if (__any(edge)) {
+      float by[8], bz[8];
+      ld8(gin<float>(t.c[4]) + r.k * W + r.i0, by);
+      ld8(gin<float>(t.c[5]) + r.j * W + r.i0, bz);
+      const float bx = ldg(gin<float>(t.c[3]) + r.k * t.H + r.j);
+      if (edge) {
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {
+          float e = 0.0f;
+          if (m == 7 && east_b) e += bx;
+          if (north_b) e += by[m];
+          if (up_b) e += bz[m];
+          acc += double(e * (p[m] * p[m]));
+        }
+      }
+    }
+    if (valid) {
       st8_nt(pnew + n0, p);
     }
   } else if (base < N) {
