@@ -135,6 +135,36 @@ class KolmogorovFlow:
         """Advance every flow by ``nsteps`` steps of dt (one launch)."""
         self._launch(nsteps, 0)
 
+    def fields(self) -> dict:
+        """The state a restart needs: u, v and the projection potential as
+        p ((N, N) each for one flow, (nflows, N, N) for an ensemble)."""
+        if self.nflows == 1:
+            return {"u": self.u[0], "v": self.v[0], "p": self.phi[0]}
+        return {"u": self.u, "v": self.v, "p": self.phi}
+
+    def run(self, nsteps: int, writer=None, nrestart: int = 0):
+        """Advance ``nsteps`` steps; with a ``snapshot.SnapshotWriter`` the
+        launches stop at every multiple of its ``n_out`` (counted from step
+        ``nrestart``) and the fields are saved there.  Whole steps per
+        launch, so the result equals ``step(nsteps)``."""
+        if writer is None:
+            self.step(nsteps)
+            return
+        done = 0
+        while done < nsteps:
+            k = min(nsteps - done, writer.n_out - (nrestart + done) % writer.n_out)
+            self.step(k)
+            done += k
+            writer.maybe_save(nrestart + done, **self.fields())
+
+    def restart(self, directory: str, nrestart: int):
+        """Load the fields ``run`` saved at step ``nrestart`` (u, v, p files)."""
+        from .snapshot import load_restart
+        f, _ = load_restart(directory, nrestart, self.dt, names=("u", "v", "p"))
+        u, v, p = (f[n].reshape(self.nflows, self.N, self.N) for n in ("u", "v", "p"))
+        for q in range(self.nflows):
+            self.set_state(q, u[q], v[q], p[q])
+
     def project(self):
         """Project the current velocities (no momentum stage)."""
         self._launch(0, L.FLOW_PROJECT_ONLY)

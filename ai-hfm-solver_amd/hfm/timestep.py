@@ -121,6 +121,18 @@ class UrbanFlow:
         u, v, w, alpha = self._bluff(u, v, w, alpha)
         return tuple(t.reshape(shape) for t in (u, v, w, p, alpha))
 
+    def run(self, u, v, w, p, alpha, ntime: int, writer=None, nrestart: int = 0):
+        """The reference's time loop (UM/AI_HFM_v3.ipynb cell 25): steps
+        itime = 1 .. ntime, each followed by ``writer.maybe_save(itime +
+        nrestart, ...)`` (a ``snapshot.SnapshotWriter``; u, v, w, p and, with
+        the scalar transport, alpha).  Returns the final fields."""
+        for itime in range(1, int(ntime) + 1):
+            u, v, w, p, alpha = self.step(u, v, w, p, alpha)
+            if writer is not None:
+                extra = {"alpha": alpha} if self.lscalar else {}
+                writer.maybe_save(itime + nrestart, u=u, v=v, w=w, p=p, **extra)
+        return u, v, w, p, alpha
+
     def diverged(self, limit: float = 0.2) -> bool:
         """The reference's guard: max |w_fine| of the last cycle above ``limit``."""
         return self.last_w is not None and float(self.last_w.abs().max()) > limit
