@@ -247,15 +247,80 @@ struct Kern4 {
   uint32_t mask[4];
 };
 
-template <int NF>
+// Petrov-Galerkin dissipation (UM/INHALE_demo.ipynb cell 20), in the
+// reference's evaluation order.  Inputs are the damped velocities
+// f~ = f / (1 + dt sigma).  Coefficients, per component f~:
+//   k = min(8*0.25 |1/3 (|u~| + |v~| + |w~|) dif(f~)| / (eps + (|ax| + |ay| + |az|)/3), 0.5)
+//       / (1 + dt sigma)
+// with one 27-point neighbourhood load per component serving the four stencils.
+__global__ __launch_bounds__(kBX* kBY) void k_pg_coef(const float* __restrict__ u,
+                                                      const float* __restrict__ v,
+                                                      const float* __restrict__ w,
+                                                      const float* __restrict__ sigma, float* k0,
+                                                      float* k1, float* k2, Kern4 K, float dt,
+                                                      float eps, Grid3 g) {
+  int k, j, i;
+  if (!cell_of_thread(g, k, j, i)) return;
+  const int64_t n = g.idx(k, j, i);
+  const float speed = (1.0f / 3.0f) * ((fabsf(u[n]) + fabsf(v[n])) + fabsf(w[n]));
+  const float damp = 1.0f + dt * sigma[n];
+  const float* f[3] = {u, v, w};
+  float* out[3] = {k0, k1, k2};
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const float* x = f[q];
+    float nb[27];
+    load_nbhd<27>(g, k, j, i, [&](int kk, int jj, int ii) { return x[g.idx(kk, jj, ii)]; }, nb);
+    const float dif = wsum<27>(K.k[0], nb);
+    const float ax = wsum<27>(K.k[1], nb);
+    const float ay = wsum<27>(K.k[2], nb);
+    const float az = wsum<27>(K.k[3], nb);
+    const float num = 2.0f * fabsf(speed * dif);
+    const float den = eps + ((fabsf(ax) + fabsf(ay)) + fabsf(az)) / 3.0f;
+    out[q][n] = fminf(num / den, 0.5f) / damp;
+  }
+}
+
+// The dissipation flux 3*0.5 (k dif(f~) + dif(f~ k) - f~ dif(k)) per
+// component: the neighbourhoods of f~ and k are loaded once and the product
+// field f~ k is formed tap by tap (zero outside the grid, as its SAME padding).
+__global__ __launch_bounds__(kBX* kBY) void k_pg_flux(const float* f0, const float* f1,
+                                                      const float* f2, const float* c0,
+                                                      const float* c1, const float* c2, float* o0,
+                                                      float* o1, float* o2, Weights Kd, Grid3 g) {
+  int k, j, i;
+  if (!cell_of_thread(g, k, j, i)) return;
+  const int64_t n = g.idx(k, j, i);
+  const float* f[3] = {f0, f1, f2};
+  const float* c[3] = {c0, c1, c2};
+  float* out[3] = {o0, o1, o2};
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const float* x = f[q];
+    const float* y = c[q];
+    float fv[27], kv[27], fk[27];
+    load_nbhd<27>(g, k, j, i, [&](int kk, int jj, int ii) { return x[g.idx(kk, jj, ii)]; }, fv);
+    load_nbhd<27>(g, k, j, i, [&](int kk, int jj, int ii) { return y[g.idx(kk, jj, ii)]; }, kv);
+#pragma unroll
+    for (int t = 0; t < 27; ++t) fk[t] = fv[t] * kv[t];
+    const float dif_f = wsum<27>(Kd, fv);
+    const float dif_fk = wsum<27>(Kd, fk);
+    const float dif_k = wsum<27>(Kd, kv);
+    out[q][n] = 1.5f * ((kv[13] * dif_f + dif_fk) - fv[13] * dif_k);
+  }
+}
+
+template <int NF, bool DIS>
 __global__ __launch_bounds__(kBX* kBY) void k_advect27_v(
     const float* __restrict__ u, const float* __restrict__ v, const float* __restrict__ w,
     const float* p0, const float* p1, const float* p2, const float* b0, const float* b1,
-    const float* b2, float* o0, float* o1, float* o2, Kern4 K, float f, Grid3 g) {
+    const float* b2, float* o0, float* o1, float* o2, Kern4 K, float f, Grid3 g,
+    const float* d0, const float* d1, const float* d2) {
   int k, j, i;
   if (!cell_of_thread(g, k, j, i)) return;
   const int64_t n = g.idx(k, j, i);
   const float uc = u[n], vc = v[n], wc = w[n];
+  const float* dis[3] = {d0, d1, d2};
   const float* phi[3] = {p0, p1, p2};
   const float* base[3] = {b0, b1, b2};
   float* out[3] = {o0, o1, o2};
@@ -265,7 +330,9 @@ __global__ __launch_bounds__(kBX* kBY) void k_advect27_v(
     float v[27];   // one neighbourhood load serves the four stencils
     load_nbhd<27>(g, k, j, i, [&](int kk, int jj, int ii) { return x[g.idx(kk, jj, ii)]; }, v);
     const float bq = base[q][n];
-    const float dif = wsum<27>(K.k[0], v);
+    // DIS: the dissipation term is given (dis0..2 in place of the phi's
+    // diffusion stencil, UM/INHALE_demo.ipynb cell 24)
+    const float dif = DIS ? dis[q][n] : wsum<27>(K.k[0], v);
     const float ax = wsum<27>(K.k[1], v);
     const float ay = wsum<27>(K.k[2], v);
     const float az = wsum<27>(K.k[3], v);
@@ -588,14 +655,109 @@ extern "C" int hfm_advect27_f32(const float* const vel[3], const float* const* p
   const Grid3 g{int(D), int(H), int(W)};
   hipStream_t s = as_stream(stream);
   if (nf == 3)
-    hipLaunchKernelGGL(k_advect27_v<3>, grid_for(g), dim3(kBX, kBY), 0, s, vel[0], vel[1],
-                       vel[2], pp.a[0], pp.a[1], pp.a[2], bb.a[0], bb.a[1], bb.a[2], oo.a[0],
-                       oo.a[1], oo.a[2], kk, f, g);
+    hipLaunchKernelGGL((k_advect27_v<3, false>), grid_for(g), dim3(kBX, kBY), 0, s, vel[0],
+                       vel[1], vel[2], pp.a[0], pp.a[1], pp.a[2], bb.a[0], bb.a[1], bb.a[2],
+                       oo.a[0], oo.a[1], oo.a[2], kk, f, g, nullptr, nullptr, nullptr);
   else
-    hipLaunchKernelGGL(k_advect27_v<1>, grid_for(g), dim3(kBX, kBY), 0, s, vel[0], vel[1],
-                       vel[2], pp.a[0], pp.a[1], pp.a[2], bb.a[0], bb.a[1], bb.a[2], oo.a[0],
-                       oo.a[1], oo.a[2], kk, f, g);
+    hipLaunchKernelGGL((k_advect27_v<1, false>), grid_for(g), dim3(kBX, kBY), 0, s, vel[0],
+                       vel[1], vel[2], pp.a[0], pp.a[1], pp.a[2], bb.a[0], bb.a[1], bb.a[2],
+                       oo.a[0], oo.a[1], oo.a[2], kk, f, g, nullptr, nullptr, nullptr);
   return launch_status("hfm_advect27_f32");
+}
+
+extern "C" int hfm_advect27_dis_f32(const float* const vel[3], const float* const* phi,
+                                    const float* const* base, const float* const* dis,
+                                    float* const* out, int nf, const float K[3][27], float f,
+                                    int64_t D, int64_t H, int64_t W, void* stream) {
+  Kern4 kk{};
+  if (!vel || !vel[0] || !vel[1] || !vel[2] || !phi || !base || !dis || !out || !K ||
+      (nf != 1 && nf != 3) || !dims_ok(D, H, W)) {
+    set_error("hfm_advect27_dis_f32: bad arguments");
+    return HFM_EINVAL;
+  }
+  for (int q = 0; q < 3; ++q) make_weights(K[q], kk.k[q + 1], kk.mask[q + 1]);
+  const float* pp[3] = {};
+  const float* bb[3] = {};
+  const float* dd[3] = {};
+  float* oo[3] = {};
+  for (int q = 0; q < nf; ++q) {
+    if (!phi[q] || !base[q] || !dis[q] || !out[q]) {
+      set_error("hfm_advect27_dis_f32: null field %d", q);
+      return HFM_EINVAL;
+    }
+    for (int r = 0; r < 3; ++r)
+      if (out[q] == vel[r] || out[q] == phi[r < nf ? r : 0] || out[q] == dis[r < nf ? r : 0]) {
+        set_error("hfm_advect27_dis_f32: output %d aliases an input", q);
+        return HFM_EINVAL;
+      }
+    pp[q] = phi[q];
+    bb[q] = base[q];
+    dd[q] = dis[q];
+    oo[q] = out[q];
+  }
+  const Grid3 g{int(D), int(H), int(W)};
+  hipStream_t s = as_stream(stream);
+  if (nf == 3)
+    hipLaunchKernelGGL((k_advect27_v<3, true>), grid_for(g), dim3(kBX, kBY), 0, s, vel[0], vel[1],
+                       vel[2], pp[0], pp[1], pp[2], bb[0], bb[1], bb[2], oo[0], oo[1], oo[2], kk, f,
+                       g, dd[0], dd[1], dd[2]);
+  else
+    hipLaunchKernelGGL((k_advect27_v<1, true>), grid_for(g), dim3(kBX, kBY), 0, s, vel[0], vel[1],
+                       vel[2], pp[0], pp[1], pp[2], bb[0], bb[1], bb[2], oo[0], oo[1], oo[2], kk, f,
+                       g, dd[0], dd[1], dd[2]);
+  return launch_status("hfm_advect27_dis_f32");
+}
+
+extern "C" int hfm_pg_coef_f32(const float* const vt[3], const float* sigma, float* const kout[3],
+                               const float K[4][27], float dt, float eps, int64_t D, int64_t H,
+                               int64_t W, void* stream) {
+  Kern4 kk{};
+  if (!vt || !sigma || !kout || !K || !dims_ok(D, H, W)) {
+    set_error("hfm_pg_coef_f32: bad arguments");
+    return HFM_EINVAL;
+  }
+  for (int q = 0; q < 3; ++q)
+    if (!vt[q] || !kout[q] || kout[q] == sigma) {
+      set_error("hfm_pg_coef_f32: null or aliased field %d", q);
+      return HFM_EINVAL;
+    }
+  for (int q = 0; q < 3; ++q)
+    for (int r = 0; r < 3; ++r)
+      if (kout[q] == vt[r]) {
+        set_error("hfm_pg_coef_f32: output %d aliases an input", q);
+        return HFM_EINVAL;
+      }
+  for (int q = 0; q < 4; ++q) make_weights(K[q], kk.k[q], kk.mask[q]);
+  const Grid3 g{int(D), int(H), int(W)};
+  hipLaunchKernelGGL(k_pg_coef, grid_for(g), dim3(kBX, kBY), 0, as_stream(stream), vt[0], vt[1],
+                     vt[2], sigma, kout[0], kout[1], kout[2], kk, dt, eps, g);
+  return launch_status("hfm_pg_coef_f32");
+}
+
+extern "C" int hfm_pg_flux_f32(const float* const vt[3], const float* const kin[3],
+                               float* const out[3], const float Kdif[27], int64_t D, int64_t H,
+                               int64_t W, void* stream) {
+  Weights w{};
+  uint32_t mask = 0;
+  if (!vt || !kin || !out || !Kdif || !dims_ok(D, H, W) || !make_weights(Kdif, w, mask)) {
+    set_error("hfm_pg_flux_f32: bad arguments");
+    return HFM_EINVAL;
+  }
+  for (int q = 0; q < 3; ++q) {
+    if (!vt[q] || !kin[q] || !out[q]) {
+      set_error("hfm_pg_flux_f32: null field %d", q);
+      return HFM_EINVAL;
+    }
+    for (int r = 0; r < 3; ++r)
+      if (out[q] == vt[r] || out[q] == kin[r]) {
+        set_error("hfm_pg_flux_f32: output %d aliases an input", q);
+        return HFM_EINVAL;
+      }
+  }
+  const Grid3 g{int(D), int(H), int(W)};
+  hipLaunchKernelGGL(k_pg_flux, grid_for(g), dim3(kBX, kBY), 0, as_stream(stream), vt[0], vt[1],
+                     vt[2], kin[0], kin[1], kin[2], out[0], out[1], out[2], w, g);
+  return launch_status("hfm_pg_flux_f32");
 }
 
 extern "C" int hfm_conv27_sum_f32(const float* const* x, float* y, int nf, const float K[3][27],

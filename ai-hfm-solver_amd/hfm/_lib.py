@@ -69,6 +69,10 @@ SIGNATURES = {
     "hfm_correct_jacobi_f32": (_int, [_vp, _vp, _vp, _vp, _vp, _f32, _vp, _vp, _i64, _i64, _i64,
                                       _int, _vp]),
     "hfm_advect27_f32": (_int, [_vp, _vp, _vp, _vp, _int, _vp, _f32, _i64, _i64, _i64, _vp]),
+    "hfm_advect27_dis_f32": (_int, [_vp, _vp, _vp, _vp, _vp, _int, _vp, _f32, _i64, _i64, _i64,
+                                    _vp]),
+    "hfm_pg_coef_f32": (_int, [_vp, _vp, _vp, _vp, _f32, _f32, _i64, _i64, _i64, _vp]),
+    "hfm_pg_flux_f32": (_int, [_vp, _vp, _vp, _vp, _i64, _i64, _i64, _vp]),
     "hfm_conv27_sum_f32": (_int, [_vp, _vp, _int, _vp, _f32, _i64, _i64, _i64, _int, _vp]),
     "hfm_bc_f32": (_int, [_vp, _vp, _i64, _i64, _i64, _int, _vp]),
     "hfm_bc_velocity_f32": (_int, [_vp, _vp, _vp, _vp, _vp, _vp, _i64, _i64, _i64, _f32, _vp,

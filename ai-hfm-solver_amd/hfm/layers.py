@@ -381,8 +381,10 @@ def boundary_condition_velocity(values_u, values_v, values_w, nx=None, ub: float
 
 
 def bluff_body(values_u, values_v, values_w, alpha, sigma, dt: float):
-    """``f / (1 + dt*sigma)`` for u, v, w and the scalar (immersed obstacle damping)."""
-    fields = [_prep(a) for a in (values_u, values_v, values_w, alpha)]
+    """``f / (1 + dt*sigma)`` for u, v, w and the scalar (immersed obstacle
+    damping); ``alpha=None`` damps the velocities only (the INHALE demo's
+    ``bluff_body(u, v, w, sigma)``) and returns three fields."""
+    fields = [_prep(a) for a in (values_u, values_v, values_w, alpha) if a is not None]
     sigma = _prep(sigma, "sigma")
     for f in fields:
         if f.shape != sigma.shape:
@@ -418,6 +420,66 @@ def advect(vel, phis, bases, f: float, kernels):
            L.ptr_array([a.data_ptr() for a in outs]), len(phis), K.ctypes.data_as(L.C.c_void_p),
            float(f), D, H, W, L.stream_handle(vel[0].device))
     return outs
+
+
+def advect_dissipative(vel, phis, bases, dis, f: float, kernels):
+    """``out_q = f * (dis_q - u ax(phi_q) - v ay(phi_q) - w az(phi_q)) + base_q``:
+    the momentum stage of ``UM/INHALE_demo.ipynb`` cell 24 with a given
+    dissipation field (``petrov_galerkin_dissipation``) in place of the
+    diffusion layer; ``kernels`` = (ax, ay, az) weight arrays."""
+    vel = [_prep(a, "vel") for a in vel]
+    phis = [_prep(a, "phi") for a in phis]
+    bases = [_prep(a, "base") for a in bases]
+    dis = [_prep(a, "dis") for a in dis]
+    if len(vel) != 3 or len(phis) not in (1, 3) or len(bases) != len(phis) or len(dis) != len(phis):
+        raise ValueError("advect_dissipative takes 3 velocity fields and 1 or 3 fields with "
+                         "bases and dissipations")
+    shape = vel[0].shape
+    for t in (*vel, *phis, *bases, *dis):
+        if t.shape != shape:
+            raise ValueError("all fields must share one shape")
+    (D, H, W), _ = _spatial(vel[0])
+    outs = [torch.empty_like(a) for a in phis]
+    K = np.ascontiguousarray(np.stack([kernel27(k) for k in kernels]), dtype=np.float32)
+    if K.shape != (3, 27):
+        raise ValueError("kernels = (ax, ay, az)")
+    L.call("hfm_advect27_dis_f32", L.ptr_array([a.data_ptr() for a in vel]),
+           L.ptr_array([a.data_ptr() for a in phis]), L.ptr_array([a.data_ptr() for a in bases]),
+           L.ptr_array([a.data_ptr() for a in dis]), L.ptr_array([a.data_ptr() for a in outs]),
+           len(phis), K.ctypes.data_as(L.C.c_void_p), float(f), D, H, W,
+           L.stream_handle(vel[0].device))
+    return outs
+
+
+def petrov_galerkin_dissipation(values_u, values_v, values_w, eplsion_k, sigma, dt: float,
+                                kernels):
+    """``Petrov_Galerkin_dissipation`` of ``UM/INHALE_demo.ipynb`` (cell 20):
+    returns (k_x, k_y, k_z).  ``kernels`` = (dif, xadv, yadv, zadv) weight arrays
+    (CNN3D_central_2nd_*).  Three launches: the damped velocities
+    f / (1 + dt sigma), the diffusivities k (one neighbourhood load per
+    component for the four stencils), the flux."""
+    u, v, w = (_prep(a, n) for a, n in ((values_u, "u"), (values_v, "v"), (values_w, "w")))
+    sigma = _prep(sigma, "sigma")
+    for t in (v, w, sigma):
+        if t.shape != u.shape:
+            raise ValueError("u, v, w and sigma must share a shape")
+    (D, H, W), _ = _spatial(u)
+    s = L.stream_handle(u.device)
+    vt = [torch.empty_like(u) for _ in range(3)]
+    L.call("hfm_bluff_body_f32", L.ptr_array([a.data_ptr() for a in (u, v, w)]),
+           L.ptr_array([a.data_ptr() for a in vt]), 3, sigma.data_ptr(), float(dt), u.numel(), s)
+    K = np.ascontiguousarray(np.stack([kernel27(k) for k in kernels]), dtype=np.float32)
+    if K.shape != (4, 27):
+        raise ValueError("kernels = (dif, xadv, yadv, zadv)")
+    ks = [torch.empty_like(u) for _ in range(3)]
+    L.call("hfm_pg_coef_f32", L.ptr_array([a.data_ptr() for a in vt]), sigma.data_ptr(),
+           L.ptr_array([a.data_ptr() for a in ks]), K.ctypes.data_as(L.C.c_void_p), float(dt),
+           float(eplsion_k), D, H, W, s)
+    out = [torch.empty_like(u) for _ in range(3)]
+    L.call("hfm_pg_flux_f32", L.ptr_array([a.data_ptr() for a in vt]),
+           L.ptr_array([a.data_ptr() for a in ks]), L.ptr_array([a.data_ptr() for a in out]),
+           K[0].ctypes.data_as(L.C.c_void_p), D, H, W, s)
+    return tuple(out)
 
 
 def conv_sum(xs, kernels, scale: float = 1.0, source_bc: bool = False):

@@ -138,6 +138,59 @@ class UrbanFlow:
         return self.last_w is not None and float(self.last_w.abs().max()) > limit
 
 
+class InhaleFlow(UrbanFlow):
+    """The time step of ``UM/INHALE_demo.ipynb`` (cell 24): the urban step with
+    the Petrov-Galerkin dissipation (cell 20) in place of the diffusion layer
+    in both momentum stages, the bluff body on the velocities only, the scalar
+    transport off by default (``LSCALAR = False``), dt = 0.5, ``eps_k`` =
+    1e-3 (cell 22).  Same BCs and sawtooth pressure cycles as ``UrbanFlow``;
+    the grid may be flat (the demo's nz = nx / 16: coarsest level 1 x r x r)."""
+
+    def __init__(self, shape, sigma: Optional[torch.Tensor] = None, dt: float = 0.5,
+                 dx: float = 1.0, Re: float = 1 / 20, ub: float = 1.0, multi_itr: int = 5,
+                 eps_k: float = 1e-3, lscalar: bool = False, device=None):
+        super().__init__(shape, sigma=sigma, dt=dt, dx=dx, Re=Re, ub=ub, multi_itr=multi_itr,
+                         lscalar=lscalar, device=device)
+        self.eps_k = float(eps_k)
+
+    def _pg(self, u, v, w):
+        return Ly.petrov_galerkin_dissipation(u, v, w, self.eps_k, self.sigma, self.dt,
+                                              self.k_mom)
+
+    def step(self, u, v, w, p, alpha):
+        """One time step; returns (u, v, w, p, alpha) as new tensors."""
+        shape = tuple(u.shape)
+        u, v, w, p, alpha = (t.reshape(self.shape).float().contiguous() for t in (u, v, w, p, alpha))
+        adv = self.k_mom[1:]
+        u, v, w = Ly.boundary_condition_velocity(u, v, w, ub=self.ub)
+        p = Ly.boundary_condition_pressure(p)
+        kx = self._pg(u, v, w)
+        bu, bv, bw = Ly.advect_dissipative((u, v, w), (u, v, w), (u, v, w), kx, 0.5, adv)
+        bu, bv, bw = Ly.boundary_condition_velocity(bu, bv, bw, ub=self.ub)
+        kb = self._pg(bu, bv, bw)
+        u, v, w = Ly.advect_dissipative((bu, bv, bw), (bu, bv, bw), (u, v, w), kb, 1.0, adv)
+        if self.lscalar:
+            (alpha,) = Ly.advect((u, v, w), (alpha,), (alpha,), 1.0, self.k_mom)
+            alpha = Ly.boundary_condition_indicator(alpha)
+        u, v, w = self._bluff(u, v, w, None)
+        u, v, w = self._gradient(u, v, w, p)
+        u, v, w = Ly.boundary_condition_velocity(u, v, w, ub=self.ub)
+        b = Ly.conv_sum((u, v, w), self.k_div, scale=-1.0, source_bc=True)
+        rec = {}
+        for _ in range(self.multi_itr):
+            p = self.mg.cycle(p, b, record=rec)
+        self.last_w = rec.get(0)
+        p = Ly.boundary_condition_pressure(p)
+        u, v, w = self._gradient(u, v, w, p)
+        u, v, w = Ly.boundary_condition_velocity(u, v, w, ub=self.ub)
+        u, v, w = self._bluff(u, v, w, None)
+        return tuple(t.reshape(shape) for t in (u, v, w, p, alpha))
+
+    def diverged(self, limit: float = 0.5) -> bool:
+        """The demo's guard: max |w_fine| of the last cycle above ``nsafe`` = 0.5."""
+        return super().diverged(limit)
+
+
 class AdvectionDiffusion2D:
     """The reference's 2D explicit advection-diffusion step
     (ARTn/Advection_diffusion.ipynb:291-296, ARTs/advection_diffusion_2D.py:131-141):

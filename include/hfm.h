@@ -130,6 +130,28 @@ int hfm_advect27_f32(const float* const vel[3], const float* const* phi, const f
                      float* const* out, int nf, const float K[4][27], float f, int64_t D,
                      int64_t H, int64_t W, void* stream);
 
+/* As hfm_advect27_f32 with the diffusion stencil replaced by a given
+ * dissipation field: out_q = f * (((dis_q - u ax(phi_q)) - v ay(phi_q)) -
+ * w az(phi_q)) + base_q, K = (ax, ay, az) (UM/INHALE_demo.ipynb cell 24, the
+ * momentum stages with the Petrov-Galerkin dissipation k_x, k_y, k_z). */
+int hfm_advect27_dis_f32(const float* const vel[3], const float* const* phi,
+                         const float* const* base, const float* const* dis, float* const* out,
+                         int nf, const float K[3][27], float f, int64_t D, int64_t H, int64_t W,
+                         void* stream);
+
+/* Petrov-Galerkin dissipation (UM/INHALE_demo.ipynb cell 20,
+ * Petrov_Galerkin_dissipation), on the damped velocities vt = f / (1 + dt
+ * sigma) (hfm_bluff_body_f32).  Coefficients per component:
+ *   k_q = min(8*0.25 |1/3 (|u~|+|v~|+|w~|) dif(vt_q)|
+ *             / (eps + (|ax(vt_q)| + |ay(vt_q)| + |az(vt_q)|)/3), 0.5) / (1 + dt sigma)
+ * with K = (dif, ax, ay, az); flux per component:
+ *   out_q = 3*0.5 (k_q dif(vt_q) + dif(vt_q k_q) - vt_q dif(k_q)). */
+int hfm_pg_coef_f32(const float* const vt[3], const float* sigma, float* const kout[3],
+                    const float K[4][27], float dt, float eps, int64_t D, int64_t H, int64_t W,
+                    void* stream);
+int hfm_pg_flux_f32(const float* const vt[3], const float* const kin[3], float* const out[3],
+                    const float Kdif[27], int64_t D, int64_t H, int64_t W, void* stream);
+
 /* y = scale * sum_q K_q (*) x_q over nf <= 3 fields: the divergence source
  * b = -(CNN3D_Su(u) + CNN3D_Sv(v) + CNN3D_Sw(w)) (UM/AI_HFM_v3.ipynb:2568);
  * flags HFM_SUM_SOURCE_BC applies boundary_condition_source. */

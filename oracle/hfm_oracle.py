@@ -37,6 +37,9 @@ Pinning status (see the tests under tests/test_oracle_*.py):
   galerkin_coarsen     brute force P^T A P / 8
   vcycle / solve       FP64 sparse direct solve
   boundary functions   closed forms (plane constants and copies)
+  Petrov-Galerkin      closed forms (constant / linear fields give no
+  dissipation          dissipation; constant k reduces the flux to 3 k dif(f);
+                       odd in the velocity; the cap 0.5/(1 + dt sigma))
 """
 from __future__ import annotations
 
@@ -459,6 +462,90 @@ def urban_step(u, v, w, p, alpha, sigma, dt=0.4, dx=1.0, Re=1 / 20, ub=1.0, mult
     u, v, w = u - C(p, ax), v - C(p, ay), w - C(p, az)
     u, v, w = boundary_condition_velocity(u, v, w, ub=f32(ub))
     u, v, w, alpha = (x / damp for x in (u, v, w, alpha))
+    return u, v, w, p, alpha, rec.get(0)
+
+
+def pg_coefficients(u, v, w, sigma, dt, eps_k, dif, ax, ay, az):
+    """The diffusivities k_u, k_v, k_w of ``Petrov_Galerkin_dissipation``
+    (``UM/INHALE_demo.ipynb`` cell 20, file lines 726-785), FP32:
+
+      f~ = f / (1 + dt sigma)  for f in u, v, w
+      k_f = 8*0.25*|1/3 (|u~| + |v~| + |w~|) dif(f~)|
+            / (eps_k + (|xadv(f~)| + |yadv(f~)| + |zadv(f~)|)/3)
+      k_f = min(k_f, 0.5) / (1 + dt sigma)
+
+    Returns ((u~, v~, w~), (k_u, k_v, k_w))."""
+    f32 = np.float32
+    C = lambda x, K: conv_same(x.astype(f32), K)  # noqa: E731
+    damp = f32(1.0) + f32(dt) * sigma
+    ut, vt, wt = (x / damp for x in (u, v, w))
+    speed = f32(1.0 / 3.0) * ((np.abs(ut) + np.abs(vt)) + np.abs(wt))
+    ks = []
+    for f in (ut, vt, wt):
+        num = f32(2.0) * np.abs(speed * C(f, dif))
+        den = f32(eps_k) + ((np.abs(C(f, ax)) + np.abs(C(f, ay))) + np.abs(C(f, az))) / f32(3.0)
+        ks.append(np.minimum(num / den, f32(0.5)) / damp)
+    return (ut, vt, wt), tuple(ks)
+
+
+def pg_flux(f, k, dif):
+    """``3*0.5*(k dif(f) + dif(f k) - f dif(k))`` of ``Petrov_Galerkin_dissipation``."""
+    f32 = np.float32
+    C = lambda x, K: conv_same(x.astype(f32), K)  # noqa: E731
+    return f32(1.5) * ((k * C(f, dif) + C(f * k, dif)) - f * C(k, dif))
+
+
+def pg_dissipation(u, v, w, sigma, dt, eps_k, dif, ax, ay, az):
+    """(k_x, k_y, k_z) of ``Petrov_Galerkin_dissipation`` (INHALE demo cell 20)."""
+    ft, ks = pg_coefficients(u, v, w, sigma, dt, eps_k, dif, ax, ay, az)
+    return tuple(pg_flux(f, k, dif) for f, k in zip(ft, ks))
+
+
+def inhale_step(u, v, w, p, alpha, sigma, dt=0.5, dx=1.0, Re=1 / 20, ub=1.0, multi_itr=5,
+                eps_k=1e-3, lscalar=False):
+    """One time step of ``UM/INHALE_demo.ipynb`` cell 24: the urban step with
+    the Petrov-Galerkin dissipation in place of the diffusion layer in both
+    momentum stages, the bluff body on the velocities only.  Returns
+    (u, v, w, p, alpha, w_fine) as ``urban_step``."""
+    f32 = np.float32
+    dif = diffusion27_kernel(dt, Re, dx)
+    ax, ay, az = advection_kernels(dt, dx)
+    sx, sy, sz = divergence_kernels(dt, dx)
+    w5 = poisson27_kernel(dx)
+    C = lambda x, K: conv_same(x.astype(f32), K)  # noqa: E731
+
+    def adv(vel, phi, first):
+        uu, vv, ww = vel
+        return ((first - uu * C(phi, ax)) - vv * C(phi, ay)) - ww * C(phi, az)
+
+    u, v, w = boundary_condition_velocity(u, v, w, ub=f32(ub))
+    p = boundary_condition_pressure(p)
+    kx = pg_dissipation(u, v, w, sigma, dt, eps_k, dif, ax, ay, az)
+    a = [adv((u, v, w), phi, k) for phi, k in zip((u, v, w), kx)]
+    bu, bv, bw = (f32(0.5) * ai + phi for ai, phi in zip(a, (u, v, w)))
+    bu, bv, bw = boundary_condition_velocity(bu, bv, bw, ub=f32(ub))
+    kb = pg_dissipation(bu, bv, bw, sigma, dt, eps_k, dif, ax, ay, az)
+    c = [adv((bu, bv, bw), phi, k) for phi, k in zip((bu, bv, bw), kb)]
+    u, v, w = (ci + phi for ci, phi in zip(c, (u, v, w)))
+    if lscalar:
+        alpha = adv((u, v, w), alpha, C(alpha, dif)) + alpha
+        alpha = boundary_condition_indicator(alpha)
+    damp = f32(1.0) + f32(dt) * sigma
+    u, v, w = (x / damp for x in (u, v, w))
+    u, v, w = u - C(p, ax), v - C(p, ay), w - C(p, az)
+    u, v, w = boundary_condition_velocity(u, v, w, ub=f32(ub))
+    b = f32(-1.0) * ((C(u, sx) + C(v, sy)) + C(w, sz))
+    b = boundary_condition_source(b)
+    shapes = urban_levels(p.shape)
+    ops = [lambda x: C(x, w5)] * len(shapes)
+    d = f32(w5[1, 1, 1])
+    rec = {}
+    for _ in range(multi_itr):
+        p = sawtooth_cycle(p, b, ops, [d] * len(shapes), zero_outlet=True, record=rec)
+    p = boundary_condition_pressure(p)
+    u, v, w = u - C(p, ax), v - C(p, ay), w - C(p, az)
+    u, v, w = boundary_condition_velocity(u, v, w, ub=f32(ub))
+    u, v, w = (x / damp for x in (u, v, w))
     return u, v, w, p, alpha, rec.get(0)
 
 
