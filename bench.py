@@ -314,7 +314,7 @@ def kolmogorov(args, dev, torch):
 
 
 def reference_context(args, dev, torch):
-    """The two reference timings taken at this build's algorithm and size
+    """The reference timings taken at this build's algorithm and size
     (hardware differs; context, not the headline metric):
 
     * 1000 reference sawtooth cycles on 128^3, constant-coefficient 7-point
@@ -324,7 +324,11 @@ def reference_context(args, dev, torch):
     * the urban step on 512^3 with 5 sawtooth cycles of the 27-point Poisson
       layer per step: 5.06 s/step (UM/AI_HFM_v3.ipynb:2508, RTX A5000) --
       here timestep.UrbanFlow, the same step (seeded blocks: the reference's
-      mesh file is not in its repository)."""
+      mesh file is not in its repository);
+    * the INHALE demo step (Petrov-Galerkin dissipation) on 256 x 4096 x 4096:
+      4000 steps in 22580.29 s (UM/INHALE_demo.ipynb cell 24 output, hardware
+      not recorded) -- here timestep.InhaleFlow on a 128 x 2048 x 2048 box,
+      compared per cell-step."""
     from hfm import layers as Ly
     from hfm.timestep import UrbanFlow
     from hfm.workloads import urban_blocks
@@ -372,6 +376,35 @@ def reference_context(args, dev, torch):
                             "cell_steps_per_s": round(512 ** 3 / per, 1),
                             "reference_s_per_step": 5.06, "ratio_vs_reference": round(5.06 / per, 1),
                             "finite": bool(all(torch.isfinite(t).all() for t in state))}
+        del flow, state, sig, z
+        torch.cuda.empty_cache()
+    if args.ref_inhale_steps > 0:
+        # the INHALE demo's step (Petrov-Galerkin dissipation) on a quarter-size
+        # box of its 256 x 4096 x 4096 grid (same 1:16 aspect ratio; the full
+        # grid's fields and temporaries exceed one GPU's memory)
+        from hfm.timestep import InhaleFlow
+        sh = (128, 2048, 2048)
+        mask = urban_blocks(sh, args.seed, foot=(16, 48), height=(10, 60))
+        sig = torch.from_numpy(np.where(mask, 1e5, 0.0).astype(np.float32)).to(dev)
+        del mask
+        flow = InhaleFlow(sh, sigma=sig, multi_itr=5, device=dev)
+        z = torch.zeros(sh, device=dev)
+        state = (torch.ones(sh, device=dev), z, z.clone(), z.clone(), z.clone())
+        state = flow.step(*state)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.ref_inhale_steps):
+            state = flow.step(*state)
+        torch.cuda.synchronize()
+        per = (time.perf_counter() - t0) / args.ref_inhale_steps
+        cells = float(np.prod(sh))
+        ref_rate = 256 * 4096 * 4096 / (22580.29 / 4000)
+        out["inhale_128x2048x2048"] = {
+            "steps": args.ref_inhale_steps, "mg_cycles_per_step": 5,
+            "ms_per_step": round(per * 1e3, 2), "cell_steps_per_s": round(cells / per, 1),
+            "reference_cell_steps_per_s": round(ref_rate, 1),
+            "ratio_vs_reference_per_cell": round(cells / per / ref_rate, 2),
+            "finite": bool(all(torch.isfinite(t).all() for t in state[:4]))}
         del flow, state, sig, z
         torch.cuda.empty_cache()
     log(f"bench: reference context {out}")
@@ -672,6 +705,8 @@ def main():
     ap.add_argument("--urban-steps", type=int, default=10)
     ap.add_argument("--ref-cycles", type=int, default=200,
                     help="reference sawtooth cycles timed on 128^3 (0: skip)")
+    ap.add_argument("--ref-inhale-steps", type=int, default=2,
+                    help="INHALE demo steps on 128 x 2048 x 2048 (0: skip)")
     ap.add_argument("--ref-urban-steps", type=int, default=3,
                     help="reference urban steps timed on 512^3 (0: skip)")
     ap.add_argument("--flow-steps", type=int, default=1000,
@@ -790,7 +825,7 @@ def main():
         extra["urban_flow"] = urban(args, dev, torch)
     if args.flow_steps > 0:
         extra["kolmogorov_2d"] = kolmogorov(args, dev, torch)
-    if args.ref_cycles > 0 or args.ref_urban_steps > 0:
+    if args.ref_cycles > 0 or args.ref_urban_steps > 0 or args.ref_inhale_steps > 0:
         extra["reference_context"] = reference_context(args, dev, torch)
     cpu = None
     if not args.no_cpu_baseline:
