@@ -135,6 +135,104 @@ __device__ __forceinline__ bool cell_of_thread(const Grid3& g, int& k, int& j, i
   return i < g.W && j < g.H;
 }
 
+// ------------------------------------------------------------------ column walkers
+// The 3x3x3 kernels on large grids walk z: a thread owns the column (j, i)
+// over kKZ planes and keeps the 3 x 3 (plane, row) window of its column in
+// registers, loading one new plane's three rows per output cell (the loads
+// of the plane after next are in flight while a cell is computed).  The x
+// neighbours come from the adjacent lanes: a wave holds 64 consecutive x
+// cells of one row and outputs the inner 62 (its two end lanes only load
+// the halo).  Per output cell a field costs 3 row loads instead of 27 tap
+// loads, and the taps are the same values in the same order as load_nbhd's.
+constexpr int kKZ = 16;
+constexpr int kOutX = kBX - 2;
+
+struct Col {
+  int i, j, lane, k0, k1;
+  bool in;   // 0 <= i < W: the lane loads its column
+  bool out;  // an inner lane of a column inside the grid: it writes a result
+};
+
+// false: the whole wave lies past the last row
+__device__ __forceinline__ bool col_begin(const Grid3& g, Col& c) {
+  c.lane = threadIdx.x;
+  c.i = int(blockIdx.x) * kOutX + c.lane - 1;
+  c.j = blockIdx.y * kBY + threadIdx.y;
+  c.k0 = blockIdx.z * kKZ;
+  c.k1 = min(c.k0 + kKZ, g.D);
+  c.in = unsigned(c.i) < unsigned(g.W);
+  c.out = c.in && c.lane >= 1 && c.lane <= kOutX;
+  return c.j < g.H;
+}
+
+dim3 grid_col(const Grid3& g) {
+  return dim3(unsigned((g.W + kOutX - 1) / kOutX), unsigned((g.H + kBY - 1) / kBY),
+              unsigned((g.D + kKZ - 1) / kKZ));
+}
+
+struct Win27 {
+  float l[3][3], m[3][3], r[3][3];  // [plane][row], x offsets -1, 0, +1 (zero outside)
+  float sv[3];                      // a fetched plane's rows at this lane's column
+
+  // issue the loads of plane kk of the field xat(kk, jj, ii) (no waiting)
+  template <typename XF>
+  __device__ __forceinline__ void fetch(const Grid3& g, const Col& c, int kk, XF xat) {
+    const bool kin = unsigned(kk) < unsigned(g.D);
+    const int kc = kin ? kk : c.k0;
+    const int ic = c.in ? c.i : 0;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const int jj = c.j - 1 + q;
+      const bool in = kin & (unsigned(jj) < unsigned(g.H)) & c.in;
+      const float x = xat(kc, in ? jj : c.j, ic);
+      sv[q] = in ? x : 0.0f;
+    }
+  }
+  // plane p of the window <- the fetched plane (x neighbours from the adjacent lanes)
+  __device__ __forceinline__ void commit(int p, const Col&) {
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      l[p][q] = __shfl_up(sv[q], 1, 64);    // the end lanes' outer values are unused
+      m[p][q] = sv[q];
+      r[p][q] = __shfl_down(sv[q], 1, 64);
+    }
+  }
+  __device__ __forceinline__ void shift() {
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        l[p][q] = l[p + 1][q];
+        m[p][q] = m[p + 1][q];
+        r[p][q] = r[p + 1][q];
+      }
+  }
+  // the taps of the window's centre plane (load_nbhd order, t = 9a + 3b + c)
+  template <int TAPS>
+  __device__ __forceinline__ void taps(float v[27]) const {
+#pragma unroll
+    for (int t = 0; t < 27; ++t) {
+      const int a = t / 9, b = (t / 3) % 3, cc = t % 3;
+      v[t] = !tap_on<TAPS>(t) ? 0.0f : cc == 0 ? l[a][b] : cc == 1 ? m[a][b] : r[a][b];
+    }
+  }
+  // planes k0 - 1 and k0 in the window, plane k0 + 1 fetched
+  template <typename XF>
+  __device__ __forceinline__ void start(const Grid3& g, const Col& c, XF xat) {
+    fetch(g, c, c.k0 - 1, xat);
+    commit(0, c);
+    fetch(g, c, c.k0, xat);
+    commit(1, c);
+    fetch(g, c, c.k0 + 1, xat);
+  }
+  // at output plane k: plane k + 1 into the window, plane k + 2's loads in flight
+  template <typename XF>
+  __device__ __forceinline__ void advance(const Grid3& g, const Col& c, int k, XF xat) {
+    commit(2, c);
+    fetch(g, c, k + 2, xat);
+  }
+};
+
 // (x - A/d) + b/d, the reference's evaluation order
 __device__ __forceinline__ float jacobi_form(float x, float ax, float b, float d) {
   return (x - ax / d) + b / d;
@@ -146,19 +244,29 @@ __global__ __launch_bounds__(kBX* kBY) void k_conv27(const float* __restrict__ x
                                                      float* __restrict__ y,
                                                      const float* __restrict__ b, Weights K,
                                                      float d, Grid3 g, int form) {
-  int k, j, i;
-  if (!cell_of_thread(g, k, j, i)) return;
-  const int64_t n = g.idx(k, j, i);
-  const float ax =
-      conv_at<TAPS>(K, g, k, j, i, [&](int kk, int jj, int ii) { return x[g.idx(kk, jj, ii)]; });
+  Col c;
+  if (!col_begin(g, c)) return;
+  auto xat = [&](int kk, int jj, int ii) { return x[g.idx(kk, jj, ii)]; };
   const int f = form & HFM_FORM_MASK;
-  float out = ax;
-  if (f == HFM_FORM_RESID) out = ax - b[n];
-  else if (f == HFM_FORM_JACOBI) out = jacobi_form(x[n], ax, b[n], d);
-  else if (f == HFM_FORM_SUBTRACT) out = b[n] - ax;
-  else if (f == HFM_FORM_AXPY) out = b[n] + d * ax;
-  if ((form & HFM_FORM_ZERO_OUTLET) && i == g.W - 1) out = 0.0f;
-  y[n] = out;
+  Win27 win;
+  win.start(g, c, xat);
+  for (int k = c.k0; k < c.k1; ++k) {
+    win.advance(g, c, k, xat);
+    float v[27];
+    win.taps<TAPS>(v);
+    const float ax = wsum<TAPS>(K, v);
+    if (c.out) {
+      const int64_t n = g.idx(k, c.j, c.i);
+      float out = ax;
+      if (f == HFM_FORM_RESID) out = ax - b[n];
+      else if (f == HFM_FORM_JACOBI) out = jacobi_form(win.m[1][1], ax, b[n], d);
+      else if (f == HFM_FORM_SUBTRACT) out = b[n] - ax;
+      else if (f == HFM_FORM_AXPY) out = b[n] + d * ax;
+      if ((form & HFM_FORM_ZERO_OUTLET) && c.i == g.W - 1) out = 0.0f;
+      y[n] = out;
+    }
+    win.shift();
+  }
 }
 
 __global__ __launch_bounds__(kBX* kBY) void k_var6(const float* __restrict__ x,
@@ -215,6 +323,59 @@ __global__ __launch_bounds__(kBX* kBY) void k_prolong_jacobi(const float* __rest
   }
 }
 
+// The same two kernels for the constant 3x3x3 stencil, as column walkers.
+template <int TAPS>
+__global__ __launch_bounds__(kBX* kBY) void k_prolong_jacobi_col(const float* __restrict__ wc,
+                                                                 const float* __restrict__ r,
+                                                                 float* __restrict__ y, Weights K,
+                                                                 float d, Grid3 gf, Grid3 gc,
+                                                                 int fz, int fy, int fx) {
+  Col c;
+  if (!col_begin(gf, c)) return;
+  auto wat = [&](int kk, int jj, int ii) { return wc[gc.idx(kk / fz, jj / fy, ii / fx)]; };
+  Win27 win;
+  win.start(gf, c, wat);
+  for (int k = c.k0; k < c.k1; ++k) {
+    win.advance(gf, c, k, wat);
+    float v[27];
+    win.taps<TAPS>(v);
+    const float ax = wsum<TAPS>(K, v);
+    if (c.out) {
+      const int64_t n = gf.idx(k, c.j, c.i);
+      y[n] = jacobi_form(win.m[1][1], ax, r[n], d);
+    }
+    win.shift();
+  }
+}
+
+template <int TAPS>
+__global__ __launch_bounds__(kBX* kBY) void k_correct_jacobi_col(const float* __restrict__ x,
+                                                                 const float* __restrict__ w,
+                                                                 const float* __restrict__ b,
+                                                                 float* __restrict__ y, Weights K,
+                                                                 float d, Grid3 g, int flags) {
+  Col c;
+  if (!col_begin(g, c)) return;
+  const bool zo = flags & HFM_CORRECT_ZERO_OUTLET;
+  auto vat = [&](int kk, int jj, int ii) {
+    const int64_t m = g.idx(kk, jj, ii);
+    return (zo && ii == g.W - 1) ? 0.0f : x[m] - w[m];
+  };
+  Win27 win;
+  win.start(g, c, vat);
+  for (int k = c.k0; k < c.k1; ++k) {
+    win.advance(g, c, k, vat);
+    float v[27];
+    win.taps<TAPS>(v);
+    const float ax = wsum<TAPS>(K, v);
+    if (c.out) {
+      const int64_t n = g.idx(k, c.j, c.i);
+      y[n] = jacobi_form(win.m[1][1], ax, b[n], d);
+    }
+    win.shift();
+  }
+}
+
 template <bool VAR6, int TAPS>
 __global__ __launch_bounds__(kBX* kBY) void k_correct_jacobi(const float* __restrict__ x,
                                                              const float* __restrict__ w,
@@ -259,25 +420,39 @@ __global__ __launch_bounds__(kBX* kBY) void k_pg_coef(const float* __restrict__ 
                                                       const float* __restrict__ sigma, float* k0,
                                                       float* k1, float* k2, Kern4 K, float dt,
                                                       float eps, Grid3 g) {
-  int k, j, i;
-  if (!cell_of_thread(g, k, j, i)) return;
-  const int64_t n = g.idx(k, j, i);
-  const float speed = (1.0f / 3.0f) * ((fabsf(u[n]) + fabsf(v[n])) + fabsf(w[n]));
-  const float damp = 1.0f + dt * sigma[n];
+  Col c;
+  if (!col_begin(g, c)) return;
   const float* f[3] = {u, v, w};
   float* out[3] = {k0, k1, k2};
+  Win27 win[3];
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
     const float* x = f[q];
-    float nb[27];
-    load_nbhd<27>(g, k, j, i, [&](int kk, int jj, int ii) { return x[g.idx(kk, jj, ii)]; }, nb);
-    const float dif = wsum<27>(K.k[0], nb);
-    const float ax = wsum<27>(K.k[1], nb);
-    const float ay = wsum<27>(K.k[2], nb);
-    const float az = wsum<27>(K.k[3], nb);
-    const float num = 2.0f * fabsf(speed * dif);
-    const float den = eps + ((fabsf(ax) + fabsf(ay)) + fabsf(az)) / 3.0f;
-    out[q][n] = fminf(num / den, 0.5f) / damp;
+    win[q].start(g, c, [&](int kk, int jj, int ii) { return x[g.idx(kk, jj, ii)]; });
+  }
+  for (int k = c.k0; k < c.k1; ++k) {
+    const int64_t n = g.idx(k, c.j, c.in ? c.i : 0);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const float* x = f[q];
+      win[q].advance(g, c, k, [&](int kk, int jj, int ii) { return x[g.idx(kk, jj, ii)]; });
+    }
+    const float speed =
+        (1.0f / 3.0f) * ((fabsf(win[0].m[1][1]) + fabsf(win[1].m[1][1])) + fabsf(win[2].m[1][1]));
+    const float damp = 1.0f + dt * sigma[n];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      float nb[27];
+      win[q].template taps<27>(nb);
+      const float dif = wsum<27>(K.k[0], nb);
+      const float ax = wsum<27>(K.k[1], nb);
+      const float ay = wsum<27>(K.k[2], nb);
+      const float az = wsum<27>(K.k[3], nb);
+      const float num = 2.0f * fabsf(speed * dif);
+      const float den = eps + ((fabsf(ax) + fabsf(ay)) + fabsf(az)) / 3.0f;
+      if (c.out) out[q][n] = fminf(num / den, 0.5f) / damp;
+      win[q].shift();
+    }
   }
 }
 
@@ -288,25 +463,37 @@ __global__ __launch_bounds__(kBX* kBY) void k_pg_flux(const float* f0, const flo
                                                       const float* f2, const float* c0,
                                                       const float* c1, const float* c2, float* o0,
                                                       float* o1, float* o2, Weights Kd, Grid3 g) {
-  int k, j, i;
-  if (!cell_of_thread(g, k, j, i)) return;
-  const int64_t n = g.idx(k, j, i);
+  Col c;
+  if (!col_begin(g, c)) return;
   const float* f[3] = {f0, f1, f2};
-  const float* c[3] = {c0, c1, c2};
+  const float* kf[3] = {c0, c1, c2};
   float* out[3] = {o0, o1, o2};
-#pragma unroll
+  // one component at a time over the column (two windows each)
+#pragma unroll 1
   for (int q = 0; q < 3; ++q) {
     const float* x = f[q];
-    const float* y = c[q];
-    float fv[27], kv[27], fk[27];
-    load_nbhd<27>(g, k, j, i, [&](int kk, int jj, int ii) { return x[g.idx(kk, jj, ii)]; }, fv);
-    load_nbhd<27>(g, k, j, i, [&](int kk, int jj, int ii) { return y[g.idx(kk, jj, ii)]; }, kv);
+    const float* y = kf[q];
+    auto xat = [&](int kk, int jj, int ii) { return x[g.idx(kk, jj, ii)]; };
+    auto yat = [&](int kk, int jj, int ii) { return y[g.idx(kk, jj, ii)]; };
+    Win27 wf, wk;
+    wf.start(g, c, xat);
+    wk.start(g, c, yat);
+    for (int k = c.k0; k < c.k1; ++k) {
+      wf.advance(g, c, k, xat);
+      wk.advance(g, c, k, yat);
+      float fv[27], kv[27], fk[27];
+      wf.taps<27>(fv);
+      wk.taps<27>(kv);
 #pragma unroll
-    for (int t = 0; t < 27; ++t) fk[t] = fv[t] * kv[t];
-    const float dif_f = wsum<27>(Kd, fv);
-    const float dif_fk = wsum<27>(Kd, fk);
-    const float dif_k = wsum<27>(Kd, kv);
-    out[q][n] = 1.5f * ((kv[13] * dif_f + dif_fk) - fv[13] * dif_k);
+      for (int t = 0; t < 27; ++t) fk[t] = fv[t] * kv[t];
+      const float dif_f = wsum<27>(Kd, fv);
+      const float dif_fk = wsum<27>(Kd, fk);
+      const float dif_k = wsum<27>(Kd, kv);
+      if (c.out)
+        out[q][g.idx(k, c.j, c.i)] = 1.5f * ((kv[13] * dif_f + dif_fk) - fv[13] * dif_k);
+      wf.shift();
+      wk.shift();
+    }
   }
 }
 
@@ -316,28 +503,41 @@ __global__ __launch_bounds__(kBX* kBY) void k_advect27_v(
     const float* p0, const float* p1, const float* p2, const float* b0, const float* b1,
     const float* b2, float* o0, float* o1, float* o2, Kern4 K, float f, Grid3 g,
     const float* d0, const float* d1, const float* d2) {
-  int k, j, i;
-  if (!cell_of_thread(g, k, j, i)) return;
-  const int64_t n = g.idx(k, j, i);
-  const float uc = u[n], vc = v[n], wc = w[n];
+  Col c;
+  if (!col_begin(g, c)) return;
   const float* dis[3] = {d0, d1, d2};
   const float* phi[3] = {p0, p1, p2};
   const float* base[3] = {b0, b1, b2};
   float* out[3] = {o0, o1, o2};
+  Win27 win[NF];
 #pragma unroll
   for (int q = 0; q < NF; ++q) {
     const float* x = phi[q];
-    float v[27];   // one neighbourhood load serves the four stencils
-    load_nbhd<27>(g, k, j, i, [&](int kk, int jj, int ii) { return x[g.idx(kk, jj, ii)]; }, v);
-    const float bq = base[q][n];
-    // DIS: the dissipation term is given (dis0..2 in place of the phi's
-    // diffusion stencil, UM/INHALE_demo.ipynb cell 24)
-    const float dif = DIS ? dis[q][n] : wsum<27>(K.k[0], v);
-    const float ax = wsum<27>(K.k[1], v);
-    const float ay = wsum<27>(K.k[2], v);
-    const float az = wsum<27>(K.k[3], v);
-    const float a = ((dif - uc * ax) - vc * ay) - wc * az;
-    out[q][n] = f * a + bq;
+    win[q].start(g, c, [&](int kk, int jj, int ii) { return x[g.idx(kk, jj, ii)]; });
+  }
+  for (int k = c.k0; k < c.k1; ++k) {
+    const int64_t n = g.idx(k, c.j, c.in ? c.i : 0);
+    const float uc = u[n], vc = v[n], wc = w[n];
+#pragma unroll
+    for (int q = 0; q < NF; ++q) {
+      const float* x = phi[q];
+      win[q].advance(g, c, k, [&](int kk, int jj, int ii) { return x[g.idx(kk, jj, ii)]; });
+    }
+#pragma unroll
+    for (int q = 0; q < NF; ++q) {
+      float t[27];   // one neighbourhood serves the four stencils
+      win[q].template taps<27>(t);
+      const float bq = base[q][n];
+      // DIS: the dissipation term is given (dis0..2 in place of the phi's
+      // diffusion stencil, UM/INHALE_demo.ipynb cell 24)
+      const float dif = DIS ? dis[q][n] : wsum<27>(K.k[0], t);
+      const float ax = wsum<27>(K.k[1], t);
+      const float ay = wsum<27>(K.k[2], t);
+      const float az = wsum<27>(K.k[3], t);
+      const float a = ((dif - uc * ax) - vc * ay) - wc * az;
+      if (c.out) out[q][n] = f * a + bq;
+      win[q].shift();
+    }
   }
 }
 
@@ -348,21 +548,40 @@ struct Kern3 {
   uint32_t mask[3];
 };
 
+template <int NF>
 __global__ __launch_bounds__(kBX* kBY) void k_conv27_sum_v(const float* x0, const float* x1,
                                                            const float* x2, float* __restrict__ y,
-                                                           Kern3 K, int nf, float scale, Grid3 g,
+                                                           Kern3 K, float scale, Grid3 g,
                                                            int flags) {
-  int k, j, i;
-  if (!cell_of_thread(g, k, j, i)) return;
-  const int ic = ((flags & HFM_SUM_SOURCE_BC) && i == 0 && g.W > 1) ? 1 : i;
+  Col c;
+  if (!col_begin(g, c)) return;
   const float* xs[3] = {x0, x1, x2};
-  float acc = 0.0f;
-  for (int q = 0; q < nf; ++q) {
+  Win27 win[NF];
+#pragma unroll
+  for (int q = 0; q < NF; ++q) {
     const float* xq = xs[q];
-    acc += conv_at<27>(K.k[q], g, k, j, ic,
-                   [&](int kk, int jj, int ii) { return xq[g.idx(kk, jj, ii)]; });
+    win[q].start(g, c, [&](int kk, int jj, int ii) { return xq[g.idx(kk, jj, ii)]; });
   }
-  y[g.idx(k, j, i)] = scale * acc;
+  // boundary_condition_source: the x = 0 cell takes the x = 1 cell's sum
+  const bool src = (flags & HFM_SUM_SOURCE_BC) && g.W > 1;
+  for (int k = c.k0; k < c.k1; ++k) {
+    float acc = 0.0f;
+#pragma unroll
+    for (int q = 0; q < NF; ++q) {
+      const float* xq = xs[q];
+      win[q].advance(g, c, k, [&](int kk, int jj, int ii) { return xq[g.idx(kk, jj, ii)]; });
+    }
+#pragma unroll
+    for (int q = 0; q < NF; ++q) {
+      float t[27];
+      win[q].template taps<27>(t);
+      acc += wsum<27>(K.k[q], t);
+      win[q].shift();
+    }
+    const float from_right = __shfl_down(acc, 1, 64);
+    if (src && c.i == 0) acc = from_right;
+    if (c.out) y[g.idx(k, c.j, c.i)] = scale * acc;
+  }
 }
 
 // Boundary fills: the value of the last reference assignment covering a cell.
@@ -511,10 +730,10 @@ extern "C" int hfm_conv27_f32(const float* x, float* y, const float* b, const fl
   }
   const Grid3 g{int(D), int(H), int(W)};
   if (taps_of(mask) == 7)
-    hipLaunchKernelGGL(k_conv27<7>, grid_for(g), dim3(kBX, kBY), 0, as_stream(stream), x, y, b, w,
+    hipLaunchKernelGGL(k_conv27<7>, grid_col(g), dim3(kBX, kBY), 0, as_stream(stream), x, y, b, w,
                        d, g, form);
   else
-    hipLaunchKernelGGL(k_conv27<27>, grid_for(g), dim3(kBX, kBY), 0, as_stream(stream), x, y, b,
+    hipLaunchKernelGGL(k_conv27<27>, grid_col(g), dim3(kBX, kBY), 0, as_stream(stream), x, y, b,
                        w, d, g, form);
   return launch_status("hfm_conv27_f32");
 }
@@ -586,11 +805,11 @@ extern "C" int hfm_prolong_jacobi_f32(const float* wc, const float* r, float* y,
     hipLaunchKernelGGL((k_prolong_jacobi<true, 7>), grid_for(gf), dim3(kBX, kBY), 0, s, wc, r, y,
                        w, d, V, gf, gc, fz, fy, fx);
   else if (taps_of(mask) == 7)
-    hipLaunchKernelGGL((k_prolong_jacobi<false, 7>), grid_for(gf), dim3(kBX, kBY), 0, s, wc, r,
-                       y, w, d, V, gf, gc, fz, fy, fx);
+    hipLaunchKernelGGL((k_prolong_jacobi_col<7>), grid_col(gf), dim3(kBX, kBY), 0, s, wc, r, y,
+                       w, d, gf, gc, fz, fy, fx);
   else
-    hipLaunchKernelGGL((k_prolong_jacobi<false, 27>), grid_for(gf), dim3(kBX, kBY), 0, s, wc, r,
-                       y, w, d, V, gf, gc, fz, fy, fx);
+    hipLaunchKernelGGL((k_prolong_jacobi_col<27>), grid_col(gf), dim3(kBX, kBY), 0, s, wc, r, y,
+                       w, d, gf, gc, fz, fy, fx);
   return launch_status("hfm_prolong_jacobi_f32");
 }
 
@@ -613,11 +832,11 @@ extern "C" int hfm_correct_jacobi_f32(const float* x, const float* wv, const flo
     hipLaunchKernelGGL((k_correct_jacobi<true, 7>), grid_for(g), dim3(kBX, kBY), 0, s, x, wv, b,
                        y, w, d, V, g, flags);
   else if (taps_of(mask) == 7)
-    hipLaunchKernelGGL((k_correct_jacobi<false, 7>), grid_for(g), dim3(kBX, kBY), 0, s, x, wv,
-                       b, y, w, d, V, g, flags);
+    hipLaunchKernelGGL((k_correct_jacobi_col<7>), grid_col(g), dim3(kBX, kBY), 0, s, x, wv, b, y,
+                       w, d, g, flags);
   else
-    hipLaunchKernelGGL((k_correct_jacobi<false, 27>), grid_for(g), dim3(kBX, kBY), 0, s, x, wv,
-                       b, y, w, d, V, g, flags);
+    hipLaunchKernelGGL((k_correct_jacobi_col<27>), grid_col(g), dim3(kBX, kBY), 0, s, x, wv, b,
+                       y, w, d, g, flags);
   return launch_status("hfm_correct_jacobi_f32");
 }
 
@@ -655,11 +874,11 @@ extern "C" int hfm_advect27_f32(const float* const vel[3], const float* const* p
   const Grid3 g{int(D), int(H), int(W)};
   hipStream_t s = as_stream(stream);
   if (nf == 3)
-    hipLaunchKernelGGL((k_advect27_v<3, false>), grid_for(g), dim3(kBX, kBY), 0, s, vel[0],
+    hipLaunchKernelGGL((k_advect27_v<3, false>), grid_col(g), dim3(kBX, kBY), 0, s, vel[0],
                        vel[1], vel[2], pp.a[0], pp.a[1], pp.a[2], bb.a[0], bb.a[1], bb.a[2],
                        oo.a[0], oo.a[1], oo.a[2], kk, f, g, nullptr, nullptr, nullptr);
   else
-    hipLaunchKernelGGL((k_advect27_v<1, false>), grid_for(g), dim3(kBX, kBY), 0, s, vel[0],
+    hipLaunchKernelGGL((k_advect27_v<1, false>), grid_col(g), dim3(kBX, kBY), 0, s, vel[0],
                        vel[1], vel[2], pp.a[0], pp.a[1], pp.a[2], bb.a[0], bb.a[1], bb.a[2],
                        oo.a[0], oo.a[1], oo.a[2], kk, f, g, nullptr, nullptr, nullptr);
   return launch_status("hfm_advect27_f32");
@@ -698,11 +917,11 @@ extern "C" int hfm_advect27_dis_f32(const float* const vel[3], const float* cons
   const Grid3 g{int(D), int(H), int(W)};
   hipStream_t s = as_stream(stream);
   if (nf == 3)
-    hipLaunchKernelGGL((k_advect27_v<3, true>), grid_for(g), dim3(kBX, kBY), 0, s, vel[0], vel[1],
+    hipLaunchKernelGGL((k_advect27_v<3, true>), grid_col(g), dim3(kBX, kBY), 0, s, vel[0], vel[1],
                        vel[2], pp[0], pp[1], pp[2], bb[0], bb[1], bb[2], oo[0], oo[1], oo[2], kk, f,
                        g, dd[0], dd[1], dd[2]);
   else
-    hipLaunchKernelGGL((k_advect27_v<1, true>), grid_for(g), dim3(kBX, kBY), 0, s, vel[0], vel[1],
+    hipLaunchKernelGGL((k_advect27_v<1, true>), grid_col(g), dim3(kBX, kBY), 0, s, vel[0], vel[1],
                        vel[2], pp[0], pp[1], pp[2], bb[0], bb[1], bb[2], oo[0], oo[1], oo[2], kk, f,
                        g, dd[0], dd[1], dd[2]);
   return launch_status("hfm_advect27_dis_f32");
@@ -729,7 +948,7 @@ extern "C" int hfm_pg_coef_f32(const float* const vt[3], const float* sigma, flo
       }
   for (int q = 0; q < 4; ++q) make_weights(K[q], kk.k[q], kk.mask[q]);
   const Grid3 g{int(D), int(H), int(W)};
-  hipLaunchKernelGGL(k_pg_coef, grid_for(g), dim3(kBX, kBY), 0, as_stream(stream), vt[0], vt[1],
+  hipLaunchKernelGGL(k_pg_coef, grid_col(g), dim3(kBX, kBY), 0, as_stream(stream), vt[0], vt[1],
                      vt[2], sigma, kout[0], kout[1], kout[2], kk, dt, eps, g);
   return launch_status("hfm_pg_coef_f32");
 }
@@ -755,7 +974,7 @@ extern "C" int hfm_pg_flux_f32(const float* const vt[3], const float* const kin[
       }
   }
   const Grid3 g{int(D), int(H), int(W)};
-  hipLaunchKernelGGL(k_pg_flux, grid_for(g), dim3(kBX, kBY), 0, as_stream(stream), vt[0], vt[1],
+  hipLaunchKernelGGL(k_pg_flux, grid_col(g), dim3(kBX, kBY), 0, as_stream(stream), vt[0], vt[1],
                      vt[2], kin[0], kin[1], kin[2], out[0], out[1], out[2], w, g);
   return launch_status("hfm_pg_flux_f32");
 }
@@ -782,8 +1001,15 @@ extern "C" int hfm_conv27_sum_f32(const float* const* x, float* y, int nf, const
     xs[q] = xs[0];
   }
   const Grid3 g{int(D), int(H), int(W)};
-  hipLaunchKernelGGL(k_conv27_sum_v, grid_for(g), dim3(kBX, kBY), 0, as_stream(stream), xs[0],
-                     xs[1], xs[2], y, kk, nf, scale, g, flags);
+  if (nf == 3)
+    hipLaunchKernelGGL(k_conv27_sum_v<3>, grid_col(g), dim3(kBX, kBY), 0, as_stream(stream),
+                       xs[0], xs[1], xs[2], y, kk, scale, g, flags);
+  else if (nf == 2)
+    hipLaunchKernelGGL(k_conv27_sum_v<2>, grid_col(g), dim3(kBX, kBY), 0, as_stream(stream),
+                       xs[0], xs[1], xs[2], y, kk, scale, g, flags);
+  else
+    hipLaunchKernelGGL(k_conv27_sum_v<1>, grid_col(g), dim3(kBX, kBY), 0, as_stream(stream),
+                       xs[0], xs[1], xs[2], y, kk, scale, g, flags);
   return launch_status("hfm_conv27_sum_f32");
 }
 
