@@ -175,6 +175,63 @@ class AOperatorVar6:
         return self._run(w, _prep(r, "r"), L.FORM["jacobi"])
 
 
+def mesh_shift(mesh: np.ndarray, axis: int) -> np.ndarray:
+    """``mesh_*_p1`` of ART/BGS_multigrid_v1.ipynb (cell 36): the grid-size
+    array shifted by one cell toward lower index along ``axis`` on the
+    interior (``p1[i] = mesh[i + 1]`` for 1 <= i <= n - 2), both boundary
+    cells keeping their own size.  ``mesh`` is the (D, H, W) or
+    (1, D, H, W, 1) array; axis counts over (D, H, W)."""
+    m = np.asarray(mesh)
+    grid = m[0, ..., 0] if m.ndim == 5 else m
+    out = grid.copy()
+    n = grid.shape[axis]
+    if n > 2:
+        dst = [slice(None)] * 3
+        src = [slice(None)] * 3
+        dst[axis] = slice(1, n - 1)
+        src[axis] = slice(2, n)
+        out[tuple(dst)] = grid[tuple(src)]
+    return out.reshape(m.shape)
+
+
+def nonuniform_coefficients(mesh_x, mesh_y, mesh_z, Dx=100.0, Dy=100.0, Dz=100.0) -> Dict:
+    """The fine-level ``A_MAT_*`` arrays of ART/BGS_multigrid_v1.ipynb (cell 38)
+    on a non-uniform mesh: ``A_MAT_xm1 = Dx / mesh_x^2``, ``A_MAT_xp1 = Dx /
+    mesh_x_p1^2`` (likewise y, z) and ``A_MAT_TOT`` their sum, in float32
+    (host arrays of the field's shape; scalars or arrays for Dx, Dy, Dz)."""
+    f32 = np.float32
+    shape = np.shape(mesh_x)
+    mats = {}
+    for name, mesh, D, axis in (("x", mesh_x, Dx, 2), ("y", mesh_y, Dy, 1), ("z", mesh_z, Dz, 0)):
+        mesh = np.asarray(mesh, dtype=np.float64)
+        if mesh.shape != shape:
+            raise ValueError("mesh_x, mesh_y and mesh_z must share one shape")
+        D = np.broadcast_to(np.asarray(D, dtype=np.float64), shape)
+        mats[name + "m1"] = (D / mesh ** 2).astype(f32)
+        mats[name + "p1"] = (D / mesh_shift(mesh, axis) ** 2).astype(f32)
+    tot = mats["xm1"] + mats["ym1"]
+    for name in ("zm1", "xp1", "yp1", "zp1"):
+        tot = tot + mats[name]
+    mats["tot"] = tot
+    return mats
+
+
+def var6_hierarchy(mats: Dict, nlevels: int, device=None) -> list:
+    """``AOperatorVar6`` per level from fine-level coefficient arrays: every
+    coarser level's fields (TOT included) are the 2^d mean restriction of the
+    finer level's (ART/BGS_multigrid_v1.ipynb cell 38, ``CNN3D_res_n`` of each
+    ``A_MAT_*_n``), computed on the GPU."""
+    dev = torch.device(device or "cuda")
+    level = {k: torch.as_tensor(np.asarray(v, np.float32)).to(dev) for k, v in mats.items()}
+    if "tot" not in level:
+        raise ValueError("mats needs the 'tot' array (nonuniform_coefficients makes it)")
+    ops = [AOperatorVar6(level)]
+    for _ in range(1, nlevels):
+        level = {k: restrict_mean(v) for k, v in level.items()}
+        ops.append(AOperatorVar6(level))
+    return ops
+
+
 def _factors(nd):
     return (1,) * (3 - nd) + (2,) * nd
 

@@ -190,3 +190,31 @@ def test_boundary_fills_bitwise(dev):
     outs = layers.bluff_body(ut, vt, wt, ut, T(sig[None, ..., None], dev), dt=0.4)
     ref = O.bluff_body((u,), sig, np.float32(0.4))[0]
     assert rel(H(outs[0])[0, ..., 0], ref) < 1e-7
+
+
+def test_nonuniform_mesh_hierarchy(dev):
+    """ART/BGS_multigrid_v1.ipynb cells 34-38: coefficients from a non-uniform
+    mesh and their mean-restricted hierarchy, applied on every level."""
+    from hfm import layers
+    sh = (8, 8, 16)
+    D, H, W = sh
+    mx, my, mz = (np.ones(sh) for _ in range(3))
+    for i in range(W // 2):
+        mx[:, :, W // 2 + i] = mx[:, :, W // 2 - 1 - i] = 1.0 + 0.02 * i
+    for i in range(H // 2):
+        my[:, H // 2 + i, :] = my[:, H // 2 - 1 - i, :] = 1.0 + 0.4 * i
+    for i in range(D):
+        mz[i] = 1.0 + 0.02 * i
+    mats = layers.nonuniform_coefficients(mx, my, mz)
+    ops = layers.var6_hierarchy(mats, 3, device=dev)
+    m = {k: v.astype(np.float64) for k, v in mats.items()}
+    rng = np.random.default_rng(3)
+    for lev, op in enumerate(ops):
+        x = rng.standard_normal(op.dims).astype(np.float32)
+        got = op(T(x, dev)).cpu().numpy()
+        ref = O.a_operator_var6(x.astype(np.float64), m)
+        assert rel(got, ref) < 1e-6, lev
+        got_j = op.jacobi(T(x, dev), T(x, dev)).cpu().numpy()
+        ref_j = (x - ref / m["tot"]) + x / m["tot"]
+        assert rel(got_j, ref_j) < 1e-6, lev
+        m = O.restrict_var6(m)

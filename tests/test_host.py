@@ -226,3 +226,52 @@ def test_fp64_vector_interleave(n, width):
                     m = seg + 8 * lane + 2 * q + b
                     want = float(m + 1) if m < n else 0.0
                     assert float(st[seg + 128 * q + 2 * lane + b]) == want
+
+
+def _bgs_meshes(shape):
+    """The non-uniform mesh of ART/BGS_multigrid_v1.ipynb cell 34, on a small box:
+    dx grows by 0.02 per cell away from the x centre, dy by 0.4 from the y
+    centre, dz by 0.02 per level."""
+    D, H, W = shape
+    mx, my, mz = (np.zeros(shape) for _ in range(3))
+    for i in range(W // 2):
+        mx[:, :, W // 2 + i] = mx[:, :, W // 2 - 1 - i] = 1.0 + 0.02 * i
+    for i in range(H // 2):
+        my[:, H // 2 + i, :] = my[:, H // 2 - 1 - i, :] = 1.0 + 0.4 * i
+    for i in range(D):
+        mz[i] = 1.0 + 0.02 * i
+    return mx, my, mz
+
+
+def test_mesh_shift_follows_the_reference_loop():
+    pytest.importorskip("torch")
+    from hfm.layers import mesh_shift
+    rng = np.random.default_rng(0)
+    m = rng.uniform(0.5, 2.0, (5, 6, 7))
+    for axis, n in ((2, 7), (1, 6), (0, 5)):
+        ref = m.copy()
+        for i in range(n - 2):          # cell 36: mesh_p1[i + 1] = mesh[i + 2]
+            idx_d = [slice(None)] * 3
+            idx_s = [slice(None)] * 3
+            idx_d[axis], idx_s[axis] = i + 1, i + 2
+            ref[tuple(idx_d)] = m[tuple(idx_s)]
+        np.testing.assert_array_equal(mesh_shift(m, axis), ref)
+    np.testing.assert_array_equal(mesh_shift(m[None, ..., None], 2)[0, ..., 0], mesh_shift(m, 2))
+
+
+def test_nonuniform_coefficients():
+    pytest.importorskip("torch")
+    from hfm.layers import nonuniform_coefficients
+    sh = (4, 6, 8)
+    ones = np.ones(sh)
+    mats = nonuniform_coefficients(ones, ones, ones, 100.0, 50.0, 25.0)
+    for k, v in (("xm1", 100), ("xp1", 100), ("ym1", 50), ("yp1", 50), ("zm1", 25), ("zp1", 25)):
+        assert mats[k].dtype == np.float32 and (mats[k] == v).all()
+    assert (mats["tot"] == 350).all()
+    mx, my, mz = _bgs_meshes(sh)
+    mats = nonuniform_coefficients(mx, my, mz)
+    # x = 5: dx = 1.02 there, the shifted size is the next cell's 1.04
+    assert mats["xm1"][0, 0, 5] == np.float32(100 / 1.02 ** 2)
+    assert mats["xp1"][0, 0, 5] == np.float32(100 / 1.04 ** 2)
+    assert mats["xp1"][0, 0, 7] == np.float32(100 / 1.06 ** 2)   # boundary keeps its own
+    assert mats["zp1"][1, 0, 0] == np.float32(100 / 1.04 ** 2)
