@@ -511,6 +511,11 @@ def profile_kernel(args, dev, torch):
         return profile_flow2d(args, dev, torch)
     if args.profile_kernel == "urban":
         return profile_urban(args, dev, torch)
+    if args.profile_kernel == "reference":
+        # the reference-context runs selected by --ref-cycles / --ref-urban-steps
+        # / --ref-inhale-steps, alone
+        print(json.dumps(reference_context(args, dev, torch)), flush=True)
+        return None
     specs = batched_specs(args.instances, args.seed)
     cells = sum(math.prod(s[0]) for s in specs)
     mg = MGBatch([s[0] for s in specs], storage=args.storage, tol=1e-8, device=dev,
@@ -698,7 +703,8 @@ def main():
     ap.add_argument("--no-fp32-batch", action="store_true")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep 256-thread blocks (skip the launch-shape tuner)")
-    ap.add_argument("--profile-kernel", choices=["update", "sweep", "all", "flow2d", "urban"],
+    ap.add_argument("--profile-kernel",
+                    choices=["update", "sweep", "all", "flow2d", "urban", "reference"],
                     default=None,
                     help="launch only this kernel (after setup) or workload, for profiler runs")
     ap.add_argument("--reps", type=int, default=20)
