@@ -146,18 +146,21 @@ __device__ __forceinline__ bool cell_of_thread(const Grid3& g, int& k, int& j, i
 // loads, and the taps are the same values in the same order as load_nbhd's.
 constexpr int kKZ = 16;
 constexpr int kOutX = kBX - 2;
+// Output rows per thread: RY = 2 for the one-field kernels (a window of 4
+// rows serves two outputs), 1 where several fields' windows are live.
 
 struct Col {
-  int i, j, lane, k0, k1;
+  int i, j, lane, k0, k1;  // j: the first of the thread's RY rows
   bool in;   // 0 <= i < W: the lane loads its column
-  bool out;  // an inner lane of a column inside the grid: it writes a result
+  bool out;  // an inner lane of a column inside the grid: it writes results
 };
 
 // false: the whole wave lies past the last row
+template <int RY>
 __device__ __forceinline__ bool col_begin(const Grid3& g, Col& c) {
   c.lane = threadIdx.x;
   c.i = int(blockIdx.x) * kOutX + c.lane - 1;
-  c.j = blockIdx.y * kBY + threadIdx.y;
+  c.j = (int(blockIdx.y) * kBY + int(threadIdx.y)) * RY;
   c.k0 = blockIdx.z * kKZ;
   c.k1 = min(c.k0 + kKZ, g.D);
   c.in = unsigned(c.i) < unsigned(g.W);
@@ -165,14 +168,17 @@ __device__ __forceinline__ bool col_begin(const Grid3& g, Col& c) {
   return c.j < g.H;
 }
 
+template <int RY>
 dim3 grid_col(const Grid3& g) {
-  return dim3(unsigned((g.W + kOutX - 1) / kOutX), unsigned((g.H + kBY - 1) / kBY),
+  return dim3(unsigned((g.W + kOutX - 1) / kOutX), unsigned((g.H + kBY * RY - 1) / (kBY * RY)),
               unsigned((g.D + kKZ - 1) / kKZ));
 }
 
+template <int RY>
 struct Win27 {
-  float l[3][3], m[3][3], r[3][3];  // [plane][row], x offsets -1, 0, +1 (zero outside)
-  float sv[3];                      // a fetched plane's rows at this lane's column
+  static constexpr int R = RY + 2;
+  float l[3][R], m[3][R], r[3][R];  // [plane][row], x offsets -1, 0, +1 (zero outside)
+  float sv[R];                      // a fetched plane's rows at this lane's column
 
   // issue the loads of plane kk of the field xat(kk, jj, ii) (no waiting)
   template <typename XF>
@@ -181,7 +187,7 @@ struct Win27 {
     const int kc = kin ? kk : c.k0;
     const int ic = c.in ? c.i : 0;
 #pragma unroll
-    for (int q = 0; q < 3; ++q) {
+    for (int q = 0; q < R; ++q) {
       const int jj = c.j - 1 + q;
       const bool in = kin & (unsigned(jj) < unsigned(g.H)) & c.in;
       const float x = xat(kc, in ? jj : c.j, ic);
@@ -191,7 +197,7 @@ struct Win27 {
   // plane p of the window <- the fetched plane (x neighbours from the adjacent lanes)
   __device__ __forceinline__ void commit(int p, const Col&) {
 #pragma unroll
-    for (int q = 0; q < 3; ++q) {
+    for (int q = 0; q < R; ++q) {
       l[p][q] = __shfl_up(sv[q], 1, 64);    // the end lanes' outer values are unused
       m[p][q] = sv[q];
       r[p][q] = __shfl_down(sv[q], 1, 64);
@@ -201,21 +207,22 @@ struct Win27 {
 #pragma unroll
     for (int p = 0; p < 2; ++p)
 #pragma unroll
-      for (int q = 0; q < 3; ++q) {
+      for (int q = 0; q < R; ++q) {
         l[p][q] = l[p + 1][q];
         m[p][q] = m[p + 1][q];
         r[p][q] = r[p + 1][q];
       }
   }
-  // the taps of the window's centre plane (load_nbhd order, t = 9a + 3b + c)
+  // the taps of output row j + ry at the centre plane (load_nbhd order, t = 9a + 3b + c)
   template <int TAPS>
-  __device__ __forceinline__ void taps(float v[27]) const {
+  __device__ __forceinline__ void taps(float v[27], int ry) const {
 #pragma unroll
     for (int t = 0; t < 27; ++t) {
-      const int a = t / 9, b = (t / 3) % 3, cc = t % 3;
+      const int a = t / 9, b = (t / 3) % 3 + ry, cc = t % 3;
       v[t] = !tap_on<TAPS>(t) ? 0.0f : cc == 0 ? l[a][b] : cc == 1 ? m[a][b] : r[a][b];
     }
   }
+  __device__ __forceinline__ float centre(int ry) const { return m[1][1 + ry]; }
   // planes k0 - 1 and k0 in the window, plane k0 + 1 fetched
   template <typename XF>
   __device__ __forceinline__ void start(const Grid3& g, const Col& c, XF xat) {
@@ -245,25 +252,28 @@ __global__ __launch_bounds__(kBX* kBY) void k_conv27(const float* __restrict__ x
                                                      const float* __restrict__ b, Weights K,
                                                      float d, Grid3 g, int form) {
   Col c;
-  if (!col_begin(g, c)) return;
+  if (!col_begin<2>(g, c)) return;
   auto xat = [&](int kk, int jj, int ii) { return x[g.idx(kk, jj, ii)]; };
   const int f = form & HFM_FORM_MASK;
-  Win27 win;
+  Win27<2> win;
   win.start(g, c, xat);
   for (int k = c.k0; k < c.k1; ++k) {
     win.advance(g, c, k, xat);
-    float v[27];
-    win.taps<TAPS>(v);
-    const float ax = wsum<TAPS>(K, v);
-    if (c.out) {
-      const int64_t n = g.idx(k, c.j, c.i);
-      float out = ax;
-      if (f == HFM_FORM_RESID) out = ax - b[n];
-      else if (f == HFM_FORM_JACOBI) out = jacobi_form(win.m[1][1], ax, b[n], d);
-      else if (f == HFM_FORM_SUBTRACT) out = b[n] - ax;
-      else if (f == HFM_FORM_AXPY) out = b[n] + d * ax;
-      if ((form & HFM_FORM_ZERO_OUTLET) && c.i == g.W - 1) out = 0.0f;
-      y[n] = out;
+#pragma unroll
+    for (int ry = 0; ry < 2; ++ry) {
+      float v[27];
+      win.taps<TAPS>(v, ry);
+      const float ax = wsum<TAPS>(K, v);
+      if (c.out && c.j + ry < g.H) {
+        const int64_t n = g.idx(k, c.j + ry, c.i);
+        float out = ax;
+        if (f == HFM_FORM_RESID) out = ax - b[n];
+        else if (f == HFM_FORM_JACOBI) out = jacobi_form(win.centre(ry), ax, b[n], d);
+        else if (f == HFM_FORM_SUBTRACT) out = b[n] - ax;
+        else if (f == HFM_FORM_AXPY) out = b[n] + d * ax;
+        if ((form & HFM_FORM_ZERO_OUTLET) && c.i == g.W - 1) out = 0.0f;
+        y[n] = out;
+      }
     }
     win.shift();
   }
@@ -331,18 +341,21 @@ __global__ __launch_bounds__(kBX* kBY) void k_prolong_jacobi_col(const float* __
                                                                  float d, Grid3 gf, Grid3 gc,
                                                                  int fz, int fy, int fx) {
   Col c;
-  if (!col_begin(gf, c)) return;
+  if (!col_begin<2>(gf, c)) return;
   auto wat = [&](int kk, int jj, int ii) { return wc[gc.idx(kk / fz, jj / fy, ii / fx)]; };
-  Win27 win;
+  Win27<2> win;
   win.start(gf, c, wat);
   for (int k = c.k0; k < c.k1; ++k) {
     win.advance(gf, c, k, wat);
-    float v[27];
-    win.taps<TAPS>(v);
-    const float ax = wsum<TAPS>(K, v);
-    if (c.out) {
-      const int64_t n = gf.idx(k, c.j, c.i);
-      y[n] = jacobi_form(win.m[1][1], ax, r[n], d);
+#pragma unroll
+    for (int ry = 0; ry < 2; ++ry) {
+      float v[27];
+      win.taps<TAPS>(v, ry);
+      const float ax = wsum<TAPS>(K, v);
+      if (c.out && c.j + ry < gf.H) {
+        const int64_t n = gf.idx(k, c.j + ry, c.i);
+        y[n] = jacobi_form(win.centre(ry), ax, r[n], d);
+      }
     }
     win.shift();
   }
@@ -355,22 +368,25 @@ __global__ __launch_bounds__(kBX* kBY) void k_correct_jacobi_col(const float* __
                                                                  float* __restrict__ y, Weights K,
                                                                  float d, Grid3 g, int flags) {
   Col c;
-  if (!col_begin(g, c)) return;
+  if (!col_begin<2>(g, c)) return;
   const bool zo = flags & HFM_CORRECT_ZERO_OUTLET;
   auto vat = [&](int kk, int jj, int ii) {
     const int64_t m = g.idx(kk, jj, ii);
     return (zo && ii == g.W - 1) ? 0.0f : x[m] - w[m];
   };
-  Win27 win;
+  Win27<2> win;
   win.start(g, c, vat);
   for (int k = c.k0; k < c.k1; ++k) {
     win.advance(g, c, k, vat);
-    float v[27];
-    win.taps<TAPS>(v);
-    const float ax = wsum<TAPS>(K, v);
-    if (c.out) {
-      const int64_t n = g.idx(k, c.j, c.i);
-      y[n] = jacobi_form(win.m[1][1], ax, b[n], d);
+#pragma unroll
+    for (int ry = 0; ry < 2; ++ry) {
+      float v[27];
+      win.taps<TAPS>(v, ry);
+      const float ax = wsum<TAPS>(K, v);
+      if (c.out && c.j + ry < g.H) {
+        const int64_t n = g.idx(k, c.j + ry, c.i);
+        y[n] = jacobi_form(win.centre(ry), ax, b[n], d);
+      }
     }
     win.shift();
   }
@@ -421,10 +437,10 @@ __global__ __launch_bounds__(kBX* kBY) void k_pg_coef(const float* __restrict__ 
                                                       float* k1, float* k2, Kern4 K, float dt,
                                                       float eps, Grid3 g) {
   Col c;
-  if (!col_begin(g, c)) return;
+  if (!col_begin<1>(g, c)) return;
   const float* f[3] = {u, v, w};
   float* out[3] = {k0, k1, k2};
-  Win27 win[3];
+  Win27<1> win[3];
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
     const float* x = f[q];
@@ -438,12 +454,12 @@ __global__ __launch_bounds__(kBX* kBY) void k_pg_coef(const float* __restrict__ 
       win[q].advance(g, c, k, [&](int kk, int jj, int ii) { return x[g.idx(kk, jj, ii)]; });
     }
     const float speed =
-        (1.0f / 3.0f) * ((fabsf(win[0].m[1][1]) + fabsf(win[1].m[1][1])) + fabsf(win[2].m[1][1]));
+        (1.0f / 3.0f) * ((fabsf(win[0].centre(0)) + fabsf(win[1].centre(0))) + fabsf(win[2].centre(0)));
     const float damp = 1.0f + dt * sigma[n];
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
       float nb[27];
-      win[q].template taps<27>(nb);
+      win[q].template taps<27>(nb, 0);
       const float dif = wsum<27>(K.k[0], nb);
       const float ax = wsum<27>(K.k[1], nb);
       const float ay = wsum<27>(K.k[2], nb);
@@ -464,7 +480,7 @@ __global__ __launch_bounds__(kBX* kBY) void k_pg_flux(const float* f0, const flo
                                                       const float* c1, const float* c2, float* o0,
                                                       float* o1, float* o2, Weights Kd, Grid3 g) {
   Col c;
-  if (!col_begin(g, c)) return;
+  if (!col_begin<1>(g, c)) return;
   const float* f[3] = {f0, f1, f2};
   const float* kf[3] = {c0, c1, c2};
   float* out[3] = {o0, o1, o2};
@@ -475,15 +491,15 @@ __global__ __launch_bounds__(kBX* kBY) void k_pg_flux(const float* f0, const flo
     const float* y = kf[q];
     auto xat = [&](int kk, int jj, int ii) { return x[g.idx(kk, jj, ii)]; };
     auto yat = [&](int kk, int jj, int ii) { return y[g.idx(kk, jj, ii)]; };
-    Win27 wf, wk;
+    Win27<1> wf, wk;
     wf.start(g, c, xat);
     wk.start(g, c, yat);
     for (int k = c.k0; k < c.k1; ++k) {
       wf.advance(g, c, k, xat);
       wk.advance(g, c, k, yat);
       float fv[27], kv[27], fk[27];
-      wf.taps<27>(fv);
-      wk.taps<27>(kv);
+      wf.taps<27>(fv, 0);
+      wk.taps<27>(kv, 0);
 #pragma unroll
       for (int t = 0; t < 27; ++t) fk[t] = fv[t] * kv[t];
       const float dif_f = wsum<27>(Kd, fv);
@@ -504,12 +520,12 @@ __global__ __launch_bounds__(kBX* kBY) void k_advect27_v(
     const float* b2, float* o0, float* o1, float* o2, Kern4 K, float f, Grid3 g,
     const float* d0, const float* d1, const float* d2) {
   Col c;
-  if (!col_begin(g, c)) return;
+  if (!col_begin<1>(g, c)) return;
   const float* dis[3] = {d0, d1, d2};
   const float* phi[3] = {p0, p1, p2};
   const float* base[3] = {b0, b1, b2};
   float* out[3] = {o0, o1, o2};
-  Win27 win[NF];
+  Win27<1> win[NF];
 #pragma unroll
   for (int q = 0; q < NF; ++q) {
     const float* x = phi[q];
@@ -526,7 +542,7 @@ __global__ __launch_bounds__(kBX* kBY) void k_advect27_v(
 #pragma unroll
     for (int q = 0; q < NF; ++q) {
       float t[27];   // one neighbourhood serves the four stencils
-      win[q].template taps<27>(t);
+      win[q].template taps<27>(t, 0);
       const float bq = base[q][n];
       // DIS: the dissipation term is given (dis0..2 in place of the phi's
       // diffusion stencil, UM/INHALE_demo.ipynb cell 24)
@@ -554,9 +570,9 @@ __global__ __launch_bounds__(kBX* kBY) void k_conv27_sum_v(const float* x0, cons
                                                            Kern3 K, float scale, Grid3 g,
                                                            int flags) {
   Col c;
-  if (!col_begin(g, c)) return;
+  if (!col_begin<1>(g, c)) return;
   const float* xs[3] = {x0, x1, x2};
-  Win27 win[NF];
+  Win27<1> win[NF];
 #pragma unroll
   for (int q = 0; q < NF; ++q) {
     const float* xq = xs[q];
@@ -574,7 +590,7 @@ __global__ __launch_bounds__(kBX* kBY) void k_conv27_sum_v(const float* x0, cons
 #pragma unroll
     for (int q = 0; q < NF; ++q) {
       float t[27];
-      win[q].template taps<27>(t);
+      win[q].template taps<27>(t, 0);
       acc += wsum<27>(K.k[q], t);
       win[q].shift();
     }
@@ -730,10 +746,10 @@ extern "C" int hfm_conv27_f32(const float* x, float* y, const float* b, const fl
   }
   const Grid3 g{int(D), int(H), int(W)};
   if (taps_of(mask) == 7)
-    hipLaunchKernelGGL(k_conv27<7>, grid_col(g), dim3(kBX, kBY), 0, as_stream(stream), x, y, b, w,
+    hipLaunchKernelGGL(k_conv27<7>, grid_col<2>(g), dim3(kBX, kBY), 0, as_stream(stream), x, y, b, w,
                        d, g, form);
   else
-    hipLaunchKernelGGL(k_conv27<27>, grid_col(g), dim3(kBX, kBY), 0, as_stream(stream), x, y, b,
+    hipLaunchKernelGGL(k_conv27<27>, grid_col<2>(g), dim3(kBX, kBY), 0, as_stream(stream), x, y, b,
                        w, d, g, form);
   return launch_status("hfm_conv27_f32");
 }
@@ -805,10 +821,10 @@ extern "C" int hfm_prolong_jacobi_f32(const float* wc, const float* r, float* y,
     hipLaunchKernelGGL((k_prolong_jacobi<true, 7>), grid_for(gf), dim3(kBX, kBY), 0, s, wc, r, y,
                        w, d, V, gf, gc, fz, fy, fx);
   else if (taps_of(mask) == 7)
-    hipLaunchKernelGGL((k_prolong_jacobi_col<7>), grid_col(gf), dim3(kBX, kBY), 0, s, wc, r, y,
+    hipLaunchKernelGGL((k_prolong_jacobi_col<7>), grid_col<2>(gf), dim3(kBX, kBY), 0, s, wc, r, y,
                        w, d, gf, gc, fz, fy, fx);
   else
-    hipLaunchKernelGGL((k_prolong_jacobi_col<27>), grid_col(gf), dim3(kBX, kBY), 0, s, wc, r, y,
+    hipLaunchKernelGGL((k_prolong_jacobi_col<27>), grid_col<2>(gf), dim3(kBX, kBY), 0, s, wc, r, y,
                        w, d, gf, gc, fz, fy, fx);
   return launch_status("hfm_prolong_jacobi_f32");
 }
@@ -832,10 +848,10 @@ extern "C" int hfm_correct_jacobi_f32(const float* x, const float* wv, const flo
     hipLaunchKernelGGL((k_correct_jacobi<true, 7>), grid_for(g), dim3(kBX, kBY), 0, s, x, wv, b,
                        y, w, d, V, g, flags);
   else if (taps_of(mask) == 7)
-    hipLaunchKernelGGL((k_correct_jacobi_col<7>), grid_col(g), dim3(kBX, kBY), 0, s, x, wv, b, y,
+    hipLaunchKernelGGL((k_correct_jacobi_col<7>), grid_col<2>(g), dim3(kBX, kBY), 0, s, x, wv, b, y,
                        w, d, g, flags);
   else
-    hipLaunchKernelGGL((k_correct_jacobi_col<27>), grid_col(g), dim3(kBX, kBY), 0, s, x, wv, b,
+    hipLaunchKernelGGL((k_correct_jacobi_col<27>), grid_col<2>(g), dim3(kBX, kBY), 0, s, x, wv, b,
                        y, w, d, g, flags);
   return launch_status("hfm_correct_jacobi_f32");
 }
@@ -874,11 +890,11 @@ extern "C" int hfm_advect27_f32(const float* const vel[3], const float* const* p
   const Grid3 g{int(D), int(H), int(W)};
   hipStream_t s = as_stream(stream);
   if (nf == 3)
-    hipLaunchKernelGGL((k_advect27_v<3, false>), grid_col(g), dim3(kBX, kBY), 0, s, vel[0],
+    hipLaunchKernelGGL((k_advect27_v<3, false>), grid_col<1>(g), dim3(kBX, kBY), 0, s, vel[0],
                        vel[1], vel[2], pp.a[0], pp.a[1], pp.a[2], bb.a[0], bb.a[1], bb.a[2],
                        oo.a[0], oo.a[1], oo.a[2], kk, f, g, nullptr, nullptr, nullptr);
   else
-    hipLaunchKernelGGL((k_advect27_v<1, false>), grid_col(g), dim3(kBX, kBY), 0, s, vel[0],
+    hipLaunchKernelGGL((k_advect27_v<1, false>), grid_col<1>(g), dim3(kBX, kBY), 0, s, vel[0],
                        vel[1], vel[2], pp.a[0], pp.a[1], pp.a[2], bb.a[0], bb.a[1], bb.a[2],
                        oo.a[0], oo.a[1], oo.a[2], kk, f, g, nullptr, nullptr, nullptr);
   return launch_status("hfm_advect27_f32");
@@ -917,11 +933,11 @@ extern "C" int hfm_advect27_dis_f32(const float* const vel[3], const float* cons
   const Grid3 g{int(D), int(H), int(W)};
   hipStream_t s = as_stream(stream);
   if (nf == 3)
-    hipLaunchKernelGGL((k_advect27_v<3, true>), grid_col(g), dim3(kBX, kBY), 0, s, vel[0], vel[1],
+    hipLaunchKernelGGL((k_advect27_v<3, true>), grid_col<1>(g), dim3(kBX, kBY), 0, s, vel[0], vel[1],
                        vel[2], pp[0], pp[1], pp[2], bb[0], bb[1], bb[2], oo[0], oo[1], oo[2], kk, f,
                        g, dd[0], dd[1], dd[2]);
   else
-    hipLaunchKernelGGL((k_advect27_v<1, true>), grid_col(g), dim3(kBX, kBY), 0, s, vel[0], vel[1],
+    hipLaunchKernelGGL((k_advect27_v<1, true>), grid_col<1>(g), dim3(kBX, kBY), 0, s, vel[0], vel[1],
                        vel[2], pp[0], pp[1], pp[2], bb[0], bb[1], bb[2], oo[0], oo[1], oo[2], kk, f,
                        g, dd[0], dd[1], dd[2]);
   return launch_status("hfm_advect27_dis_f32");
@@ -948,7 +964,7 @@ extern "C" int hfm_pg_coef_f32(const float* const vt[3], const float* sigma, flo
       }
   for (int q = 0; q < 4; ++q) make_weights(K[q], kk.k[q], kk.mask[q]);
   const Grid3 g{int(D), int(H), int(W)};
-  hipLaunchKernelGGL(k_pg_coef, grid_col(g), dim3(kBX, kBY), 0, as_stream(stream), vt[0], vt[1],
+  hipLaunchKernelGGL(k_pg_coef, grid_col<1>(g), dim3(kBX, kBY), 0, as_stream(stream), vt[0], vt[1],
                      vt[2], sigma, kout[0], kout[1], kout[2], kk, dt, eps, g);
   return launch_status("hfm_pg_coef_f32");
 }
@@ -974,7 +990,7 @@ extern "C" int hfm_pg_flux_f32(const float* const vt[3], const float* const kin[
       }
   }
   const Grid3 g{int(D), int(H), int(W)};
-  hipLaunchKernelGGL(k_pg_flux, grid_col(g), dim3(kBX, kBY), 0, as_stream(stream), vt[0], vt[1],
+  hipLaunchKernelGGL(k_pg_flux, grid_col<1>(g), dim3(kBX, kBY), 0, as_stream(stream), vt[0], vt[1],
                      vt[2], kin[0], kin[1], kin[2], out[0], out[1], out[2], w, g);
   return launch_status("hfm_pg_flux_f32");
 }
@@ -1002,13 +1018,13 @@ extern "C" int hfm_conv27_sum_f32(const float* const* x, float* y, int nf, const
   }
   const Grid3 g{int(D), int(H), int(W)};
   if (nf == 3)
-    hipLaunchKernelGGL(k_conv27_sum_v<3>, grid_col(g), dim3(kBX, kBY), 0, as_stream(stream),
+    hipLaunchKernelGGL(k_conv27_sum_v<3>, grid_col<1>(g), dim3(kBX, kBY), 0, as_stream(stream),
                        xs[0], xs[1], xs[2], y, kk, scale, g, flags);
   else if (nf == 2)
-    hipLaunchKernelGGL(k_conv27_sum_v<2>, grid_col(g), dim3(kBX, kBY), 0, as_stream(stream),
+    hipLaunchKernelGGL(k_conv27_sum_v<2>, grid_col<1>(g), dim3(kBX, kBY), 0, as_stream(stream),
                        xs[0], xs[1], xs[2], y, kk, scale, g, flags);
   else
-    hipLaunchKernelGGL(k_conv27_sum_v<1>, grid_col(g), dim3(kBX, kBY), 0, as_stream(stream),
+    hipLaunchKernelGGL(k_conv27_sum_v<1>, grid_col<1>(g), dim3(kBX, kBY), 0, as_stream(stream),
                        xs[0], xs[1], xs[2], y, kk, scale, g, flags);
   return launch_status("hfm_conv27_sum_f32");
 }
