@@ -4,11 +4,14 @@
 //
 // The grids are small (256^2 floats = 256 KB per field) and one step is a
 // chain of ~40 dependent passes per CG iteration, so launches, not bytes,
-// bound a kernel-per-pass design.  Here one 1024-thread workgroup owns a flow
-// and runs whole steps: passes are separated by workgroup barriers, every
-// field stays in the CU's L1/L2 working set, scalars (dots, CG coefficients,
-// the stopping test) never leave the workgroup, and a launch over many flows
-// fills the chip with independent workgroups.
+// bound a kernel-per-pass design.  Here a team of 1024-thread workgroups owns
+// a flow and runs whole steps: passes are separated by team barriers, every
+// field stays in the L2 working set, and scalars (dots, CG coefficients, the
+// stopping test) are reduced within the team in a fixed order, so every
+// workgroup takes the same branches.  A team of one workgroup (barriers are
+// workgroup barriers) suits an ensemble, whose flows fill the chip; a single
+// flow spreads over a team of up to 64 co-resident workgroups (a cooperative
+// launch) that synchronise through a counter in the flow's team scratch.
 #include <math.h>
 
 #include "hfm_common.h"
@@ -71,6 +74,121 @@ __device__ __forceinline__ float wg_max(float a, float* redf) {
   return m;
 }
 
+// ------------------------------------------------------------------ teams
+// Team scratch (per flow, zeroed by the host before each launch): the
+// arrival counter, a failure word, then two slots of per-workgroup partial
+// sums (double pairs) and maxima (float pairs) for the team reductions.
+constexpr int kTeamMax = 64;
+constexpr long kSpinLimit = 1L << 26;  // ~1 s of polling: a barrier that never fills
+
+struct Team {
+  static constexpr bool kTeam = true;
+  unsigned* cnt;
+  unsigned* fail;
+  double* part;   // [2][kTeamMax][2]
+  float* partf;   // [2][kTeamMax][2]
+  int G, rank;
+  unsigned nbar;  // barriers passed (thread 0's count)
+  int slot;       // reduction slot to use next
+  __device__ __forceinline__ int gt() const { return rank * kFT + int(threadIdx.x); }
+  __device__ __forceinline__ int stride() const { return G * kFT; }
+  __device__ __forceinline__ bool lead() const { return rank == 0 && threadIdx.x == 0; }
+};
+
+// A flow owned by one workgroup: the same interface with compile-time extents
+// (an ensemble's path keeps its constant strides and plain barriers).
+struct Solo {
+  static constexpr bool kTeam = false;
+  static constexpr int G = 1;
+  static constexpr int rank = 0;
+  __device__ __forceinline__ int gt() const { return int(threadIdx.x); }
+  __device__ __forceinline__ int stride() const { return kFT; }
+  __device__ __forceinline__ bool lead() const { return threadIdx.x == 0; }
+};
+__device__ __forceinline__ void team_sync(Solo&) { __syncthreads(); }
+__device__ __forceinline__ void team_sum2(Solo&, double& a, double& b, double* red) {
+  wg_sum2(a, b, red);
+}
+__device__ __forceinline__ double team_sum(Solo&, double a, double* red) { return wg_sum(a, red); }
+__device__ __forceinline__ void team_max2(Solo&, float& a, float& b, float* redf) {
+  a = wg_max(a, redf);
+  b = wg_max(b, redf);
+}
+
+// Every workgroup of the team has finished the pass before any starts the
+// next: workgroup barrier, then (for G > 1) a device-scope release, one
+// arrival on the counter and an acquire once all G arrivals of this barrier
+// are in.  The wait is bounded: on timeout the failure word is set and the
+// team runs on without waiting (its results are then void and reported).
+__device__ __forceinline__ void team_sync(Team& t) {
+  __syncthreads();
+  if (t.G == 1) return;
+  if (threadIdx.x == 0) {
+    t.nbar += 1;
+    const unsigned target = t.nbar * unsigned(t.G);
+    __threadfence();
+    atomicAdd(t.cnt, 1u);
+    long spins = 0;
+    while (__hip_atomic_load(t.cnt, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target &&
+           __hip_atomic_load(t.fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) {
+      __builtin_amdgcn_s_sleep(1);
+      if (++spins > kSpinLimit) {
+        atomicExch(t.fail, 1u);
+        break;
+      }
+    }
+    __threadfence();
+  }
+  __syncthreads();
+}
+
+// Team sums: workgroup sums, then the G partials summed in rank order by
+// every thread (the same value in every workgroup).
+__device__ __forceinline__ void team_sum2(Team& t, double& a, double& b, double* red) {
+  wg_sum2(a, b, red);
+  if (t.G == 1) return;
+  double* slot = t.part + t.slot * (2 * kTeamMax);
+  t.slot ^= 1;
+  if (threadIdx.x == 0) {
+    slot[2 * t.rank] = a;
+    slot[2 * t.rank + 1] = b;
+  }
+  team_sync(t);
+  double sa = 0.0, sb = 0.0;
+  for (int q = 0; q < t.G; ++q) {
+    sa += __hip_atomic_load(slot + 2 * q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    sb += __hip_atomic_load(slot + 2 * q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  a = sa;
+  b = sb;
+}
+
+__device__ __forceinline__ double team_sum(Team& t, double a, double* red) {
+  double b = 0.0;
+  team_sum2(t, a, b, red);
+  return a;
+}
+
+__device__ __forceinline__ void team_max2(Team& t, float& a, float& b, float* redf) {
+  a = wg_max(a, redf);
+  b = wg_max(b, redf);
+  if (t.G == 1) return;
+  float* slot = t.partf + t.slot * (2 * kTeamMax);
+  t.slot ^= 1;
+  if (threadIdx.x == 0) {
+    slot[2 * t.rank] = a;
+    slot[2 * t.rank + 1] = b;
+  }
+  team_sync(t);
+  float ma = 0.0f, mb = 0.0f;
+  for (int q = 0; q < t.G; ++q) {
+    ma = fmaxf(ma, __hip_atomic_load(slot + 2 * q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    mb = fmaxf(mb, __hip_atomic_load(slot + 2 * q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+  }
+  a = ma;
+  b = mb;
+}
+
 // Periodic neighbour offsets of cell (j, i) on an n x n level (n = 1 << sh).
 struct Nb {
   int c, e, w, n, s;
@@ -86,23 +204,24 @@ struct Nb {
   }
 };
 
-// Cells c = threadIdx.x + k kFT of an nn-cell pass, kU at a time: load(c)
-// for each cell of a group before put(c, value) for any of them, so a
-// thread has kU cells' loads in flight instead of one cell's.
+// Cells c = gt + k G kFT of an nn-cell pass (gt: the thread's index in its
+// team), kU at a time: load(c) for each cell of a group before put(c, value)
+// for any of them, so a thread has kU cells' loads in flight instead of one.
 constexpr int kU = 4;
 
-template <typename V, typename Load, typename Put>
-__device__ __forceinline__ void cells(int nn, Load&& load, Put&& put) {
-  for (int c0 = threadIdx.x; c0 < nn; c0 += kU * kFT) {
+template <typename V, typename TM, typename Load, typename Put>
+__device__ __forceinline__ void cells(const TM& t, int nn, Load&& load, Put&& put) {
+  const int st = t.stride();
+  for (int c0 = t.gt(); c0 < nn; c0 += kU * st) {
     V val[kU];
 #pragma unroll
     for (int k = 0; k < kU; ++k) {
-      const int c = c0 + k * kFT;
+      const int c = c0 + k * st;
       if (c < nn) val[k] = load(c);
     }
 #pragma unroll
     for (int k = 0; k < kU; ++k) {
-      const int c = c0 + k * kFT;
+      const int c = c0 + k * st;
       if (c < nn) put(c, val[k]);
     }
   }
@@ -177,88 +296,122 @@ __device__ __forceinline__ float resid_cell(const gf* __restrict__ x,
   return b[c] - Ax;
 }
 
-// z = V(b0) with b0 = mgb level 0 and the first level-0 sweep (omega b0 / 4)
-// already in mgx level 0.  Returns this thread's part of b0 . z.
-__device__ double vcycle(const hfm_flow2d_task& T, const hfm_flow2d_params& P,
-                         const double* __restrict__ pinv, const Levels& L) {
-  const int tid = threadIdx.x;
+// Passes of one V-cycle level.  CX: the threads that run them (a team or one
+// workgroup), sync(): the barrier between dependent passes.
+template <typename CX, typename SYNC>
+__device__ void vdown_level(const CX& cx, SYNC sync, const hfm_flow2d_task& T,
+                            const hfm_flow2d_params& P, const Levels& L, int l) {
+  const int nu1 = P.nu1;
+  const float omega = P.omega;
+  const int sh = L.sh - l, nn = 1 << (2 * sh), o = L.off(l);
+  const float s = ldexpf(1.0f, -l), d = 4.0f * s;
+  gf* buf[2] = {(gf*)T.mgx + o, (gf*)T.mgy + o};
+  const gf* b = (const gf*)T.mgb + o;
+  if (l > 0) {
+    gf* x0 = buf[0];
+    cells<float>(cx, nn, [&](int c) { return omega * b[c] / d; }, [&](int c, float y) { x0[c] = y; });
+    sync();
+  }
+  for (int k = 1; k < nu1; ++k) {
+    const gf* xin = buf[(k - 1) & 1];
+    gf* out = buf[k & 1];
+    cells<float>(cx, nn, [&](int c) { return sweep_cell(xin, nullptr, 0.0f, b, c, sh, s, omega); },
+                 [&](int c, float y) { out[c] = y; });
+    sync();
+  }
+  // bc = mean over the 2x2 children of (b - A x)
+  const gf* x = buf[(nu1 - 1) & 1];
+  const int shc = sh - 1, ncc = 1 << (2 * shc), mc = (1 << shc) - 1;
+  gf* bc = (gf*)T.mgb + L.off(l + 1);
+  cells<float>(
+      cx, ncc,
+      [&](int C) {
+        const int J = C >> shc, I = C & mc;
+        const int f00 = ((2 * J) << sh) + 2 * I, f10 = f00 + (1 << sh);
+        const float r00 = resid_cell(x, b, f00, sh, s), r01 = resid_cell(x, b, f00 + 1, sh, s);
+        const float r10 = resid_cell(x, b, f10, sh, s), r11 = resid_cell(x, b, f10 + 1, sh, s);
+        return 0.25f * ((r00 + r01) + (r10 + r11));
+      },
+      [&](int C, float y) { bc[C] = y; });
+  sync();
+}
+
+// coarsest: x = pinv b (FP64)
+template <typename CX, typename SYNC>
+__device__ void vcoarsest(const CX& cx, SYNC sync, const hfm_flow2d_task& T,
+                          const double* __restrict__ pinv, const Levels& L) {
+  const int l = L.nlev - 1, nn = 1 << (2 * (L.sh - l)), o = L.off(l);
+  const gf* b = (const gf*)T.mgb + o;
+  gf* x = (gf*)T.mgx + o;
+  for (int r = cx.gt(); r < nn; r += cx.stride()) {
+    const gd* row = (const gd*)pinv + int64_t(r) * nn;
+    double acc = 0.0;
+    for (int c = 0; c < nn; ++c) acc += row[c] * double(b[c]);
+    x[r] = float(acc);
+  }
+  sync();
+}
+
+// returns this thread's part of b0 . z (level 0's last sweep writes z)
+template <typename CX, typename SYNC>
+__device__ double vup_level(const CX& cx, SYNC sync, const hfm_flow2d_task& T,
+                            const hfm_flow2d_params& P, const Levels& L, int l) {
   const int nu1 = P.nu1, nu2 = P.nu2;
   const float omega = P.omega, alpha = P.alpha;
-  // down
-  for (int l = 0; l < L.nlev - 1; ++l) {
-    const int sh = L.sh - l, nn = 1 << (2 * sh), o = L.off(l);
-    const float s = ldexpf(1.0f, -l), d = 4.0f * s;
-    gf* buf[2] = {(gf*)T.mgx + o, (gf*)T.mgy + o};
-    const gf* b = (const gf*)T.mgb + o;
-    if (l > 0) {
-      gf* x0 = buf[0];
-      cells<float>(nn, [&](int c) { return omega * b[c] / d; }, [&](int c, float y) { x0[c] = y; });
-      __syncthreads();
-    }
-    for (int k = 1; k < nu1; ++k) {
-      const gf* xin = buf[(k - 1) & 1];
-      gf* out = buf[k & 1];
-      cells<float>(nn, [&](int c) { return sweep_cell(xin, nullptr, 0.0f, b, c, sh, s, omega); },
-                   [&](int c, float y) { out[c] = y; });
-      __syncthreads();
-    }
-    // bc = mean over the 2x2 children of (b - A x)
-    const gf* x = buf[(nu1 - 1) & 1];
-    const int shc = sh - 1, ncc = 1 << (2 * shc), mc = (1 << shc) - 1;
-    gf* bc = (gf*)T.mgb + L.off(l + 1);
-    cells<float>(
-        ncc,
-        [&](int C) {
-          const int J = C >> shc, I = C & mc;
-          const int f00 = ((2 * J) << sh) + 2 * I, f10 = f00 + (1 << sh);
-          const float r00 = resid_cell(x, b, f00, sh, s), r01 = resid_cell(x, b, f00 + 1, sh, s);
-          const float r10 = resid_cell(x, b, f10, sh, s), r11 = resid_cell(x, b, f10 + 1, sh, s);
-          return 0.25f * ((r00 + r01) + (r10 + r11));
-        },
-        [&](int C, float y) { bc[C] = y; });
-    __syncthreads();
-  }
-  // coarsest: x = pinv b (FP64)
-  {
-    const int l = L.nlev - 1, nn = 1 << (2 * (L.sh - l)), o = L.off(l);
-    const gf* b = (const gf*)T.mgb + o;
-    gf* x = (gf*)T.mgx + o;
-    for (int r = tid; r < nn; r += kFT) {
-      const gd* row = (const gd*)pinv + int64_t(r) * nn;
-      double acc = 0.0;
-      for (int c = 0; c < nn; ++c) acc += row[c] * double(b[c]);
-      x[r] = float(acc);
-    }
-    __syncthreads();
-  }
-  // up
+  const int sh = L.sh - l, nn = 1 << (2 * sh), o = L.off(l), oc = L.off(l + 1);
+  const float s = ldexpf(1.0f, -l);
+  gf* buf[2] = {(gf*)T.mgx + o, (gf*)T.mgy + o};
+  const gf* b = (const gf*)T.mgb + o;
+  const gf* ec = (const gf*)((l + 1 == L.nlev - 1) ? T.mgx + oc
+                                                   : ((nu1 + nu2 - 1) & 1 ? T.mgy : T.mgx) + oc);
   double rz = 0.0;
-  for (int l = L.nlev - 2; l >= 0; --l) {
-    const int sh = L.sh - l, nn = 1 << (2 * sh), o = L.off(l), oc = L.off(l + 1);
-    const float s = ldexpf(1.0f, -l);
-    gf* buf[2] = {(gf*)T.mgx + o, (gf*)T.mgy + o};
-    const gf* b = (const gf*)T.mgb + o;
-    const gf* ec = (const gf*)((l + 1 == L.nlev - 1) ? T.mgx + oc
-                                                     : ((nu1 + nu2 - 1) & 1 ? T.mgy : T.mgx) + oc);
-    for (int k = 0; k < nu2; ++k) {
-      const int K = nu1 + k;
-      const gf* xin = buf[(K - 1) & 1];
-      const bool last0 = (l == 0 && k == nu2 - 1);
-      gf* out = last0 ? (gf*)T.z : buf[K & 1];
-      const gf* e = k == 0 ? ec : nullptr;
-      if (last0) {
-        cells<F2>(nn, [&](int c) { return F2{sweep_cell(xin, e, alpha, b, c, sh, s, omega), b[c]}; },
-                  [&](int c, F2 y) {
-                    out[c] = y.a;
-                    rz += double(y.b) * double(y.a);
-                  });
-      } else {
-        cells<float>(nn, [&](int c) { return sweep_cell(xin, e, alpha, b, c, sh, s, omega); },
-                     [&](int c, float y) { out[c] = y; });
-      }
-      __syncthreads();
+  for (int k = 0; k < nu2; ++k) {
+    const int K = nu1 + k;
+    const gf* xin = buf[(K - 1) & 1];
+    const bool last0 = (l == 0 && k == nu2 - 1);
+    gf* out = last0 ? (gf*)T.z : buf[K & 1];
+    const gf* e = k == 0 ? ec : nullptr;
+    if (last0) {
+      cells<F2>(cx, nn, [&](int c) { return F2{sweep_cell(xin, e, alpha, b, c, sh, s, omega), b[c]}; },
+                [&](int c, F2 y) {
+                  out[c] = y.a;
+                  rz += double(y.b) * double(y.a);
+                });
+    } else {
+      cells<float>(cx, nn, [&](int c) { return sweep_cell(xin, e, alpha, b, c, sh, s, omega); },
+                   [&](int c, float y) { out[c] = y; });
     }
+    sync();
   }
+  return rz;
+}
+
+// Levels of at most kSoloCells cells run on one workgroup of a team (rank 0)
+// between workgroup barriers; the rest of the team waits at one team barrier.
+constexpr int kSoloCells = 4096;
+
+// z = V(b0) with b0 = mgb level 0 and the first level-0 sweep (omega b0 / 4)
+// already in mgx level 0.  Returns this thread's part of b0 . z.
+template <typename TM>
+__device__ double vcycle(TM& tm, const hfm_flow2d_task& T, const hfm_flow2d_params& P,
+                         const double* __restrict__ pinv, const Levels& L) {
+  int ls = 0;  // first level run by one workgroup
+  if (TM::kTeam) {
+    ls = L.nlev - 1;
+    while (ls > 0 && (1 << (2 * (L.sh - ls + 1))) <= kSoloCells) --ls;
+  }
+  auto tsync = [&] { team_sync(tm); };
+  auto wsync = [] { __syncthreads(); };
+  for (int l = 0; l < ls; ++l) vdown_level(tm, tsync, T, P, L, l);
+  double rz = 0.0;
+  if (tm.rank == 0) {
+    Solo one;
+    for (int l = ls; l < L.nlev - 1; ++l) vdown_level(one, wsync, T, P, L, l);
+    vcoarsest(one, wsync, T, pinv, L);
+    for (int l = L.nlev - 2; l >= ls; --l) rz += vup_level(one, wsync, T, P, L, l);
+  }
+  if (TM::kTeam) team_sync(tm);
+  for (int l = ls - 1; l >= 0; --l) rz += vup_level(tm, tsync, T, P, L, l);
   return rz;
 }
 
@@ -270,10 +423,10 @@ __device__ __forceinline__ double apply64(const HFM_GLOBAL X* __restrict__ x, co
 
 // ------------------------------------------------------------------ one substep
 // Returns the HFM_MG_* status of the projection.
-__device__ int substep(const hfm_flow2d_task& T, const hfm_flow2d_params& P,
+template <typename TM>
+__device__ int substep(TM& tm, const hfm_flow2d_task& T, const hfm_flow2d_params& P,
                        const double* __restrict__ pinv, const Levels& L, float dt, bool momentum,
                        double* red) {
-  const int tid = threadIdx.x;
   const int sh = L.sh, NN = 1 << (2 * sh), m = (1 << sh) - 1;
   gf* __restrict__ u = (gf*)T.u;
   gf* __restrict__ v = (gf*)T.v;
@@ -289,7 +442,7 @@ __device__ int substep(const hfm_flow2d_task& T, const hfm_flow2d_params& P,
   if (momentum) {
     const float hdt = 0.5f * dt;
     cells<F2>(
-        NN,
+        tm, NN,
         [&](int c) {
           F2 F;
           momentum_rhs(u, v, c, sh, P.nu_h2, P.inv_h, fu[c >> sh], F.a, F.b);
@@ -299,10 +452,10 @@ __device__ int substep(const hfm_flow2d_task& T, const hfm_flow2d_params& P,
           uh[c] = y.a;
           vh[c] = y.b;
         });
-    __syncthreads();
+    team_sync(tm);
     // u* in place: a cell reads only its own u, v (before any store of its group)
     cells<F2>(
-        NN,
+        tm, NN,
         [&](int c) {
           F2 F;
           momentum_rhs(uh, vh, c, sh, P.nu_h2, P.inv_h, fu[c >> sh], F.a, F.b);
@@ -312,12 +465,12 @@ __device__ int substep(const hfm_flow2d_task& T, const hfm_flow2d_params& P,
           u[c] = y.a;
           v[c] = y.b;
         });
-    __syncthreads();
+    team_sync(tm);
   }
   // divergence source b (into r), its mean
   double sb = 0.0;
   cells<float>(
-      NN,
+      tm, NN,
       [&](int c) {
         const int j = c >> sh, i = c & m;
         return P.h * ((u[c] - u[(j << sh) + ((i + 1) & m)]) + (v[c] - v[(((j + 1) & m) << sh) + i]));
@@ -326,7 +479,7 @@ __device__ int substep(const hfm_flow2d_task& T, const hfm_flow2d_params& P,
         r[c] = double(bf);
         sb += double(bf);
       });
-  const double mean = wg_sum(sb, red) / double(NN);
+  const double mean = team_sum(tm, sb, red) / double(NN);
   // r = (b - mean) - A phi; rl = f32(r); first level-0 sweep
   double bb = 0.0, rr = 0.0;
   const float w4 = P.omega;
@@ -334,7 +487,7 @@ __device__ int substep(const hfm_flow2d_task& T, const hfm_flow2d_params& P,
     double a, b;
   };
   cells<D2>(
-      NN,
+      tm, NN,
       [&](int c) {
         const Nb q(c, sh);
         const double bm = r[c] - mean;
@@ -348,7 +501,7 @@ __device__ int substep(const hfm_flow2d_task& T, const hfm_flow2d_params& P,
         bb += y.a * y.a;
         rr += y.b * y.b;
       });
-  wg_sum2(bb, rr, red);
+  team_sum2(tm, bb, rr, red);
   int status = HFM_MG_RUNNING;
   int it = 0;
   double rel = bb > 0.0 ? sqrt(rr / bb) : 0.0;
@@ -359,21 +512,21 @@ __device__ int substep(const hfm_flow2d_task& T, const hfm_flow2d_params& P,
       status = HFM_MG_MAXIT;
       break;
     }
-    const double rz = wg_sum(vcycle(T, P, pinv, L), red);
+    const double rz = team_sum(tm, vcycle(tm, T, P, pinv, L), red);
     const double beta = it == 0 ? 0.0 : rz / rz_old;
     rz_old = rz;
     const double bd = double(float(beta));
     if (it == 0) {
-      cells<float>(NN, [&](int c) { return z[c]; }, [&](int c, float y) { p[c] = y; });
+      cells<float>(tm, NN, [&](int c) { return z[c]; }, [&](int c, float y) { p[c] = y; });
     } else {
-      cells<float>(NN, [&](int c) { return float(bd * double(p[c]) + double(z[c])); },
+      cells<float>(tm, NN, [&](int c) { return float(bd * double(p[c]) + double(z[c])); },
                    [&](int c, float y) { p[c] = y; });
     }
-    __syncthreads();
+    team_sync(tm);
     double pq = 0.0;
-    cells<double>(NN, [&](int c) { return double(p[c]) * apply64(p, Nb(c, sh)); },
+    cells<double>(tm, NN, [&](int c) { return double(p[c]) * apply64(p, Nb(c, sh)); },
                   [&](int, double y) { pq += y; });
-    pq = wg_sum(pq, red);
+    pq = team_sum(tm, pq, red);
     if (!(pq > 0.0) || !isfinite(pq)) {
       status = HFM_MG_BREAKDOWN;
       break;
@@ -381,7 +534,7 @@ __device__ int substep(const hfm_flow2d_task& T, const hfm_flow2d_params& P,
     const double a = rz / pq;
     rr = 0.0;
     cells<D2>(
-        NN,
+        tm, NN,
         [&](int c) {
           const Nb q(c, sh);
           const double pc = double(p[c]);
@@ -395,7 +548,7 @@ __device__ int substep(const hfm_flow2d_task& T, const hfm_flow2d_params& P,
           mgx[c] = w4 * rl / 4.0f;
           rr += y.b * y.b;
         });
-    rr = wg_sum(rr, red);
+    rr = team_sum(tm, rr, red);
     ++it;
     rel = sqrt(rr / bb);
     if (rel <= P.tol) status = HFM_MG_CONVERGED;
@@ -404,7 +557,7 @@ __device__ int substep(const hfm_flow2d_task& T, const hfm_flow2d_params& P,
   // gradient correction (phi final: the loop ended on a barrier)
   const double ih = double(P.inv_h);
   cells<F2>(
-      NN,
+      tm, NN,
       [&](int c) {
         const Nb q(c, sh);
         const double ph = phi[c];
@@ -414,8 +567,8 @@ __device__ int substep(const hfm_flow2d_task& T, const hfm_flow2d_params& P,
         u[c] = y.a;
         v[c] = y.b;
       });
-  __syncthreads();
-  if (tid == 0) {
+  team_sync(tm);
+  if (tm.lead()) {
     double* st = T.stats;
     st[HFM_FLOW_ST_ITERS] = it;
     st[HFM_FLOW_ST_REL] = rel;
@@ -427,52 +580,97 @@ __device__ int substep(const hfm_flow2d_task& T, const hfm_flow2d_params& P,
   return status;
 }
 
-__global__ __launch_bounds__(kFT) void k_flow2d(const hfm_flow2d_task* __restrict__ tasks,
-                                               hfm_flow2d_params P,
-                                               const double* __restrict__ pinv, int nsteps,
-                                               int flags) {
-  const hfm_flow2d_task& T = tasks[blockIdx.x];
-  __shared__ double red[2 * kFWv];
-  __shared__ float redf[kFWv];
+// A team whose barrier timed out reports it (its fields are void).
+__device__ __forceinline__ void team_finish(const Solo&, const hfm_flow2d_task&) {}
+__device__ __forceinline__ void team_finish(const Team& t, const hfm_flow2d_task& T) {
+  if (t.G > 1 && t.lead() &&
+      __hip_atomic_load(t.fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u)
+    T.stats[HFM_FLOW_ST_STATUS] = HFM_FLOW_TEAM_TIMEOUT;
+}
+
+template <typename TM>
+__device__ __forceinline__ void flow_steps(TM& tm, const hfm_flow2d_task& T,
+                                           const hfm_flow2d_params& P,
+                                           const double* __restrict__ pinv, int nsteps, int flags,
+                                           double* red, float* redf) {
   Levels L;
   L.N = P.N;
   L.sh = 31 - __clz(P.N);
   L.nlev = P.nlev;
   const int NN = P.N * P.N;
-  // a flow whose last projection did not converge stays stopped
+  // a flow whose last projection did not converge stays stopped (read by
+  // every workgroup before the first barrier, hence before any stats update)
   const double st0 = T.stats[HFM_FLOW_ST_STATUS];
   if (st0 != double(HFM_MG_CONVERGED) && st0 != double(HFM_MG_RUNNING)) return;
   if (flags & HFM_FLOW_PROJECT_ONLY) {
-    substep(T, P, pinv, L, 0.0f, false, red);
+    substep(tm, T, P, pinv, L, 0.0f, false, red);
+    team_finish(tm, T);
     return;
   }
   for (int step = 0; step < nsteps; ++step) {
     float umax = 0.0f, vmax = 0.0f;
     const gf* u = (const gf*)T.u;
     const gf* v = (const gf*)T.v;
-    cells<F2>(NN, [&](int c) { return F2{fabsf(u[c]), fabsf(v[c])}; },
+    cells<F2>(tm, NN, [&](int c) { return F2{fabsf(u[c]), fabsf(v[c])}; },
               [&](int, F2 a) {
                 umax = fmaxf(umax, a.a);
                 vmax = fmaxf(vmax, a.b);
               });
-    umax = wg_max(umax, redf);
-    vmax = wg_max(vmax, redf);
+    team_max2(tm, umax, vmax, redf);
     const float q = (umax + vmax) * P.cfl_coef;
     // NaN or too fast: stop this flow
     if (!(q <= float(P.max_sub))) {
-      if (threadIdx.x == 0) T.stats[HFM_FLOW_ST_STATUS] = HFM_FLOW_CFL;
+      if (tm.lead()) T.stats[HFM_FLOW_ST_STATUS] = HFM_FLOW_CFL;
+      team_finish(tm, T);
       return;
     }
     const int nsub = q > 1.0f ? int(ceilf(q)) : 1;
     const float dts = P.dt / float(nsub);
-    for (int k = 0; k < nsub; ++k) {
-      if (substep(T, P, pinv, L, dts, true, red) != HFM_MG_CONVERGED) return;
-    }
-    if (threadIdx.x == 0) {
+    bool ok = true;
+    for (int k = 0; k < nsub && ok; ++k) ok = substep(tm, T, P, pinv, L, dts, true, red) == HFM_MG_CONVERGED;
+    if (!ok) break;
+    if (tm.lead()) {
       T.stats[HFM_FLOW_ST_STEPS] += 1.0;
       T.stats[HFM_FLOW_ST_NSUB] = nsub;
     }
   }
+  team_finish(tm, T);
+}
+
+// One workgroup per flow (an ensemble).
+__global__ __launch_bounds__(kFT) void k_flow2d(const hfm_flow2d_task* __restrict__ tasks,
+                                               hfm_flow2d_params P,
+                                               const double* __restrict__ pinv, int nsteps,
+                                               int flags) {
+  __shared__ double red[2 * kFWv];
+  __shared__ float redf[kFWv];
+  Solo tm;
+  flow_steps(tm, tasks[blockIdx.x], P, pinv, nsteps, flags, red, redf);
+}
+
+// P.team workgroups per flow: workgroup b is rank b % team of flow b / team.
+// (Placing a team on one XCD -- every 8th workgroup of an 8x larger launch --
+// measured slower than the plain round-robin spread.)
+__global__ __launch_bounds__(kFT) void k_flow2d_team(const hfm_flow2d_task* __restrict__ tasks,
+                                                    hfm_flow2d_params P,
+                                                    const double* __restrict__ pinv, int nsteps,
+                                                    int flags) {
+  __shared__ double red[2 * kFWv];
+  __shared__ float redf[kFWv];
+  const int G = P.team;
+  const int m = int(blockIdx.x);
+  const hfm_flow2d_task& T = tasks[m / G];
+  Team tm;
+  tm.G = G;
+  tm.rank = m % G;
+  tm.nbar = 0;
+  tm.slot = 0;
+  unsigned* w = (unsigned*)T.team;
+  tm.cnt = w;
+  tm.fail = w + 1;
+  tm.part = (double*)T.team + 1;
+  tm.partf = (float*)((double*)T.team + 1 + 4 * kTeamMax);
+  flow_steps(tm, T, P, pinv, nsteps, flags, red, redf);
 }
 
 }  // namespace
@@ -500,9 +698,30 @@ extern "C" int hfm_flow2d_steps(const hfm_flow2d_task* tasks, int64_t nflows,
     set_error("hfm_flow2d_steps: grid %d with %d levels is not supported", P.N, P.nlev);
     return HFM_EINVAL;
   }
+  if (P.team < 1 || P.team > kTeamMax || nflows * P.team > INT32_MAX) {
+    set_error("hfm_flow2d_steps: team of %d workgroups (1..%d)", P.team, kTeamMax);
+    return HFM_EINVAL;
+  }
   if (nflows == 0 || (nsteps == 0 && !(flags & HFM_FLOW_PROJECT_ONLY))) return HFM_OK;
-  hipLaunchKernelGGL(k_flow2d, dim3(unsigned(nflows)), dim3(kFT), 0, as_stream(stream), tasks, P,
-                     pinv, nsteps, flags);
+  if (P.team == 1) {
+    hipLaunchKernelGGL(k_flow2d, dim3(unsigned(nflows)), dim3(kFT), 0, as_stream(stream), tasks,
+                       P, pinv, nsteps, flags);
+    return launch_status("hfm_flow2d_steps");
+  }
+  const hfm_flow2d_task* ta = tasks;
+  hfm_flow2d_params pa = P;
+  const double* pi = pinv;
+  int ns = nsteps, fl = flags;
+  void* args[] = {(void*)&ta, (void*)&pa, (void*)&pi, (void*)&ns, (void*)&fl};
+  const hipError_t e =
+      hipLaunchCooperativeKernel((const void*)k_flow2d_team, dim3(unsigned(nflows * P.team)),
+                                 dim3(kFT),
+                                 args, 0, as_stream(stream));
+  if (e != hipSuccess) {
+    set_error("hfm_flow2d_steps: cooperative launch of %lld workgroups failed: %s",
+              (long long)(nflows * P.team), hipGetErrorString(e));
+    return HFM_EDEVICE;
+  }
   return launch_status("hfm_flow2d_steps");
 }
 

@@ -103,18 +103,20 @@ SIGNATURES = {
 FLOW_TASK_DTYPE = np.dtype([
     ("u", "<u8"), ("v", "<u8"), ("uh", "<u8"), ("vh", "<u8"), ("z", "<u8"), ("p", "<u8"),
     ("phi", "<u8"), ("r", "<u8"), ("mgx", "<u8"), ("mgy", "<u8"), ("mgb", "<u8"),
-    ("fu", "<u8"), ("stats", "<u8"),
+    ("fu", "<u8"), ("stats", "<u8"), ("team", "<u8"),
 ])
 
 FLOW_PARAMS_DTYPE = np.dtype([
     ("nu_h2", "<f4"), ("inv_h", "<f4"), ("h", "<f4"), ("dt", "<f4"), ("cfl_coef", "<f4"),
     ("omega", "<f4"), ("alpha", "<f4"),
     ("N", "<i4"), ("nu1", "<i4"), ("nu2", "<i4"), ("nlev", "<i4"), ("maxit", "<i4"),
-    ("max_sub", "<i4"), ("pad", "<i4"), ("tol", "<f8"),
+    ("max_sub", "<i4"), ("team", "<i4"), ("tol", "<f8"),
 ])
 
 FLOW_STATS = ("iters", "rel", "iters_sum", "status", "steps", "substeps", "rel_max", "nsub")
-FLOW_STATUS = {**STATUS, 4: "cfl"}
+FLOW_STATUS = {**STATUS, 4: "cfl", 5: "team_timeout"}
+FLOW_TEAM_BYTES = 4096      # team scratch per flow (zeroed before each launch)
+FLOW_TEAM_MAX = 64
 FLOW_PROJECT_ONLY = 1
 
 _lock = threading.Lock()

@@ -58,7 +58,8 @@ class KolmogorovFlow:
     def __init__(self, N: int = 256, nflows: int = 1, nu: float = 0.01, dt: float = 0.005,
                  amp: float = 1.0, k: int = 4, tol: float = 1e-6, nlev: int = 6,
                  omega: float = 0.8, alpha: float = 1.8, nu1: int = 2, nu2: int = 2,
-                 maxit: int = 100, cfl_max: float = 0.5, max_sub: int = 64, device=None):
+                 maxit: int = 100, cfl_max: float = 0.5, max_sub: int = 64, team=None,
+                 device=None):
         if N < 4 or N & (N - 1):
             raise ValueError("N must be a power of two >= 4")
         nc = N >> (nlev - 1)
@@ -66,6 +67,13 @@ class KolmogorovFlow:
             raise ValueError(f"{nlev} levels do not fit N = {N} (coarsest edge 2..32)")
         if nflows < 1:
             raise ValueError("nflows must be >= 1")
+        # workgroups per flow: an ensemble fills the chip with one per flow; a
+        # single flow spreads over a team (grids of 128^2 and more)
+        if team is None:
+            team = 16 if nflows == 1 and N >= 128 else 1
+        if not 1 <= int(team) <= L.FLOW_TEAM_MAX:
+            raise ValueError(f"team must be in 1..{L.FLOW_TEAM_MAX}")
+        self.team = int(team)
         self.device = torch.device(device or "cuda")
         if self.device.type != "cuda" or not torch.cuda.is_available():
             raise RuntimeError("KolmogorovFlow needs a GPU (the HIP library has no CPU path)")
@@ -80,6 +88,7 @@ class KolmogorovFlow:
         prm["omega"], prm["alpha"] = omega, alpha
         prm["N"], prm["nu1"], prm["nu2"], prm["nlev"] = N, nu1, nu2, nlev
         prm["maxit"], prm["max_sub"], prm["tol"] = maxit, max_sub, tol
+        prm["team"] = self.team
         self._prm = prm
         dev = self.device
         f32, f64 = torch.float32, torch.float64
@@ -98,6 +107,7 @@ class KolmogorovFlow:
         self.mgb = torch.zeros_like(self.mgx)
         self.fu = torch.from_numpy(np.tile(forcing_profile(N, amp, k), (nflows, 1))).to(dev)
         self.stats_buf = torch.zeros((nflows, L.FLOW_STATS.__len__()), dtype=f64, device=dev)
+        self.team_buf = torch.zeros((nflows, L.FLOW_TEAM_BYTES // 8), dtype=f64, device=dev)
         self.pinv = torch.from_numpy(coarsest_pinv(nc, 2.0 ** -(nlev - 1))).to(dev)
         tab = np.zeros(nflows, dtype=L.FLOW_TASK_DTYPE)
         for q in range(nflows):
@@ -109,6 +119,7 @@ class KolmogorovFlow:
                 tab[name][q] = getattr(self, name)[q].data_ptr()
             tab["fu"][q] = self.fu[q].data_ptr()
             tab["stats"][q] = self.stats_buf[q].data_ptr()
+            tab["team"][q] = self.team_buf[q].data_ptr()
         self.tasks = torch.from_numpy(tab.view(np.uint8).copy()).to(dev)
         assert self.u[0].numel() == NN
 
@@ -128,6 +139,8 @@ class KolmogorovFlow:
 
     # ------------------------------------------------------------------ stepping
     def _launch(self, nsteps: int, flags: int):
+        if self.team > 1:
+            self.team_buf.zero_()      # barrier counters start at zero (a fill kernel)
         L.call("hfm_flow2d_steps", self.tasks.data_ptr(), self.nflows, self._prm.ctypes.data,
                self.pinv.data_ptr(), int(nsteps), flags, L.stream_handle(self.device))
 
@@ -184,5 +197,5 @@ class KolmogorovFlow:
 
     def nbytes(self) -> int:
         ts = [self.u, self.v, self.uh, self.vh, self.z, self.p, self.phi, self.r, self.mgx,
-              self.mgy, self.mgb, self.fu, self.stats_buf]
+              self.mgy, self.mgb, self.fu, self.stats_buf, self.team_buf]
         return sum(t.numel() * t.element_size() for t in ts)

@@ -239,7 +239,7 @@ def kolmogorov(args, dev, torch):
     from oracle import hfm_flow2d as F
     N = 256
     out = {"grid": [N, N], "nu": 0.01, "dt": 0.005, "tol": 1e-6, "levels": 6}
-    fl = KolmogorovFlow(N=N, nflows=1, device=dev)
+    fl = KolmogorovFlow(N=N, nflows=1, team=args.flow_team, device=dev)
     u0, v0 = F.random_modes_ic(N, args.seed)
     fl.set_state(0, torch.from_numpy(u0), torch.from_numpy(v0))
     torch.cuda.synchronize()
@@ -258,7 +258,7 @@ def kolmogorov(args, dev, torch):
     div = fl.divergence(0)
     u, v, _ = fl.state(0)
     out["single_flow"] = {
-        "steps": args.flow_steps, "wall_s": round(t1, 3),
+        "steps": args.flow_steps, "workgroups": fl.team, "wall_s": round(t1, 3),
         "ms_per_step": round(t1 / args.flow_steps * 1e3, 4),
         "steps_per_s": round(args.flow_steps / t1, 1),
         "cell_steps_per_s": round(N * N * args.flow_steps / t1, 1),
@@ -453,7 +453,8 @@ def profile_flow2d(args, dev, torch):
                                 ("ensemble", args.flow_ensemble, args.flow_ensemble_steps)):
         if nflows < 1 or steps < 1:
             continue
-        fl = KolmogorovFlow(N=N, nflows=nflows, device=dev)
+        fl = KolmogorovFlow(N=N, nflows=nflows, team=args.flow_team if nflows == 1 else None,
+                            device=dev)
         for q in range(nflows):
             u, v = F.random_modes_ic(N, args.seed + q)
             fl.set_state(q, torch.from_numpy(u), torch.from_numpy(v))
@@ -719,6 +720,8 @@ def main():
                     help="Kolmogorov 2D flow steps of the single-flow run (0: skip)")
     ap.add_argument("--flow-ensemble", type=int, default=256)
     ap.add_argument("--flow-ensemble-steps", type=int, default=20)
+    ap.add_argument("--flow-team", type=int, default=None,
+                    help="workgroups sharing the single flow (default: 16 at 256^2)")
     ap.add_argument("--traffic", action="store_true",
                     help="torch-free k_fcg_update driver for hardware-counter runs")
     ap.add_argument("--traffic-edge", type=int, default=0)

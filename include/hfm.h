@@ -372,6 +372,8 @@ int hfm_mg_coarse_inverse(const hfm_mg_task* tasks, int64_t ntasks, void* stream
 #define HFM_FLOW_ST_REL_MAX 6    /* largest final relative residual            */
 #define HFM_FLOW_ST_NSUB 7       /* substeps of the last step                  */
 #define HFM_FLOW_CFL 4           /* status: more than max_sub substeps needed */
+#define HFM_FLOW_TEAM_TIMEOUT 5  /* status: a team barrier timed out (fields void) */
+#define HFM_FLOW_TEAM_BYTES 4096 /* team scratch per flow */
 #define HFM_FLOW_PROJECT_ONLY 1  /* flag: skip the momentum stages (project u, v) */
 
 typedef struct hfm_flow2d_task {
@@ -388,7 +390,9 @@ typedef struct hfm_flow2d_task {
   float* mgb;
   const float* fu;
   double* stats;
-} hfm_flow2d_task; /* 104 bytes */
+  void* team;     /* team scratch, HFM_FLOW_TEAM_BYTES zeroed before each
+                     launch; used when params.team > 1 */
+} hfm_flow2d_task; /* 112 bytes */
 
 typedef struct hfm_flow2d_params {
   float nu_h2;    /* nu / h^2 */
@@ -404,13 +408,16 @@ typedef struct hfm_flow2d_params {
   int32_t nlev;   /* levels N, N/2, ..., N >> (nlev-1) >= 2 */
   int32_t maxit;
   int32_t max_sub;
-  int32_t pad;
+  int32_t team;   /* workgroups per flow, 1..64 (> 1: one cooperative launch) */
   double tol;
 } hfm_flow2d_params; /* 64 bytes */
 
 /* Advance every flow of `tasks` (device array) by `nsteps` steps (flags:
  * HFM_FLOW_PROJECT_ONLY projects u, v once instead).  A flow whose status is
- * not converged stops stepping.  No reference counterpart: the reference's
+ * not converged stops stepping.  params.team workgroups share each flow
+ * (rank = block % team); with team > 1 the launch is cooperative (all
+ * nflows * team workgroups co-resident, else HFM_EDEVICE) and every task's
+ * team scratch must be zeroed before the call.  No reference counterpart: the reference's
  * time-step drivers are Python loops over Keras layer calls
  * (UM/AI_HFM_v3.ipynb cell 25). */
 int hfm_flow2d_steps(const hfm_flow2d_task* tasks, int64_t nflows, const hfm_flow2d_params* prm,

@@ -97,7 +97,7 @@ def test_flow_is_the_same_alone_and_in_an_ensemble(dev):
         u, v = F.random_modes_ic(N, 10 + q)
         fl.set_state(q, torch.from_numpy(u), torch.from_numpy(v))
     fl.step(10)
-    one = _flow(dev, N, nflows=1)
+    one = _flow(dev, N, nflows=1, team=1)
     u, v = F.random_modes_ic(N, 11)
     one.set_state(0, torch.from_numpy(u), torch.from_numpy(v))
     one.step(10)
@@ -112,3 +112,27 @@ def test_bad_grids_are_rejected(dev):
         KolmogorovFlow(N=96, device=dev)
     with pytest.raises(ValueError):
         KolmogorovFlow(N=256, nlev=9, device=dev)
+
+
+@pytest.mark.parametrize("team", [4, 32])
+def test_team_flow_matches_oracle(dev, team):
+    # one flow spread over a team of workgroups (cooperative launch, counter
+    # barriers): the oracle's steps with the same substep and CG counts, and
+    # the single-workgroup flow to rounding (reductions summed per workgroup)
+    import torch
+    N, steps, seed = 256, 3, 1
+    fl = _flow(dev, N, team=team)
+    u, v = F.random_modes_ic(N, seed)
+    fl.set_state(0, torch.from_numpy(u), torch.from_numpy(v))
+    fl.step(steps)
+    st = fl.stats()
+    ref, nsubs = _oracle_run(N, seed, steps)
+    assert int(st["status"][0]) == 1 and int(st["steps"][0]) == steps
+    assert int(st["substeps"][0]) == sum(nsubs) and int(st["iters_sum"][0]) == sum(ref.iters)
+    ug, vg, phig = (t.cpu().numpy() for t in fl.state(0))
+    assert rel(ug, ref.u) < 1e-5 and rel(vg, ref.v) < 1e-5
+    solo = _flow(dev, N, team=1)
+    solo.set_state(0, torch.from_numpy(u), torch.from_numpy(v))
+    solo.step(steps)
+    for a, b in zip(fl.state(0), solo.state(0)):
+        assert rel(a.cpu().numpy(), b.cpu().numpy()) < 1e-9

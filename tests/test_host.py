@@ -39,7 +39,7 @@ def test_flow2d_struct_sizes_match_host_dtypes():
     lib = L.load()
     a, b = L.C.c_int64(), L.C.c_int64()
     assert lib.hfm_flow2d_struct_sizes(L.C.byref(a), L.C.byref(b)) == 0
-    assert a.value == L.FLOW_TASK_DTYPE.itemsize == 104
+    assert a.value == L.FLOW_TASK_DTYPE.itemsize == 112
     assert b.value == L.FLOW_PARAMS_DTYPE.itemsize == 64
 
 
@@ -57,6 +57,11 @@ def test_flow2d_rejects_bad_parameters():
     prm["nlev"], prm["maxit"] = 6, 0
     with pytest.raises(L.HfmError, match="bad parameters"):
         L.call("hfm_flow2d_steps", 1, 1, prm.ctypes.data, 1, 1, 0, None)
+    prm["maxit"] = 10
+    for team in (0, L.FLOW_TEAM_MAX + 1):                        # workgroups per flow
+        prm["team"] = team
+        with pytest.raises(L.HfmError, match="team"):
+            L.call("hfm_flow2d_steps", 1, 1, prm.ctypes.data, 1, 1, 0, None)
 
 
 def test_error_path_reports_message():
