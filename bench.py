@@ -574,7 +574,7 @@ def profile_kernel(args, dev, torch):
 # 1 GiB device copy reads on each counter (524299.5 KiB fetched, 1048576.0 KiB
 # written).
 UPDATE_TRAFFIC = {"workload": (512, 2024, "bf16", "bf16"),
-                  "kib": {1: (5531802.0, 6763086.1), 0: (3833714.8, 4058341.8)}}
+                  "kib": {1: (5524986.7, 6763025.9), 0: (3833670.3, 4058333.7)}}
 
 
 def update_traffic():
