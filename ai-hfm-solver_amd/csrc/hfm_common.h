@@ -28,12 +28,16 @@ template <>
 __device__ __forceinline__ float from_f<float>(float f) {
   return f;
 }
-// Round to nearest even; finite inputs (NaN payloads are not preserved).
+// Round to nearest even (gfx950's v_cvt_pk_bf16_f32; NaN stays NaN).
 template <>
 __device__ __forceinline__ bf16 from_f<bf16>(float f) {
-  uint32_t u = __float_as_uint(f);
-  u += 0x7FFFu + ((u >> 16) & 1u);
-  return bf16{uint16_t(u >> 16)};
+  return bf16{__builtin_bit_cast(uint16_t, (__bf16)f)};
+}
+// Two values rounded to BF16 and packed (low half = a): one conversion.
+__device__ __forceinline__ uint32_t pack_bf16x2(float a, float b) {
+  typedef float f2 __attribute__((ext_vector_type(2)));
+  typedef __bf16 h2 __attribute__((ext_vector_type(2)));
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector(f2{a, b}, h2));
 }
 // Value after a round trip through storage type T.
 template <typename T>

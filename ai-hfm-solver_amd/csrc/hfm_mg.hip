@@ -9,6 +9,8 @@
 // over the 8 XCDs.
 #include <math.h>
 
+#include <type_traits>
+
 #include "hfm_common.h"
 
 namespace hfm {
@@ -82,9 +84,7 @@ __device__ __forceinline__ void st8(HFM_GLOBAL float* p, const float v[8]) {
   ((HFM_GLOBAL f4v*)p)[0] = f4v{v[0], v[1], v[2], v[3]};
   ((HFM_GLOBAL f4v*)p)[1] = f4v{v[4], v[5], v[6], v[7]};
 }
-__device__ __forceinline__ uint32_t pack2(float a, float b) {
-  return uint32_t(from_f<bf16>(a).v) | (uint32_t(from_f<bf16>(b).v) << 16);
-}
+__device__ __forceinline__ uint32_t pack2(float a, float b) { return pack_bf16x2(a, b); }
 __device__ __forceinline__ void st8(HFM_GLOBAL bf16* p, const float v[8]) {
   *(HFM_GLOBAL u4v*)p = u4v{pack2(v[0], v[1]), pack2(v[2], v[3]), pack2(v[4], v[5]), pack2(v[6], v[7])};
 }
@@ -166,6 +166,19 @@ __device__ __forceinline__ void st8d_nt(HFM_GLOBAL double* p, const double v[8])
     __builtin_nontemporal_store(d2v{v[2 * q], v[2 * q + 1]}, (HFM_GLOBAL d2v*)p + q);
 }
 
+// ------------------------------------------------------------------ stencil arithmetic
+// The products that feed a sum are written as explicit fused multiply-adds:
+// a cell's value then rounds the same in every kernel and path that computes
+// it (a row of a run, a single neighbour, a fused pass), whatever the
+// compiler would contract in each context.
+__device__ __forceinline__ float fma_a(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fma_a(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+// x + alpha * e: the prolonged coarse correction added to x
+__device__ __forceinline__ float correct_val(float x, float e, float alpha) {
+  return __builtin_fmaf(alpha, e, x);
+}
+
 // ------------------------------------------------------------------ scalar stencil row
 // Off-diagonal sum and diagonal of the face operator at cell (k, j, i), index n.
 // xat(m, kk, jj, ii) returns x at a neighbour.  Accumulation in type A.
@@ -181,12 +194,12 @@ __device__ __forceinline__ A face_row(const hfm_mg_task& t, int n, int i, int j,
   const A cu = (A)(k < D - 1 ? ldc<C>(t.c[2], n + HW) : ldc<C>(t.c[5], j * W + i));
   diag = cw + ce + cs + cn + cd + cu;
   A off = (A)0;
-  if (i > 0) off += cw * (A)xat(n - 1, k, j, i - 1);
-  if (i < W - 1) off += ce * (A)xat(n + 1, k, j, i + 1);
-  if (j > 0) off += cs * (A)xat(n - W, k, j - 1, i);
-  if (j < H - 1) off += cn * (A)xat(n + W, k, j + 1, i);
-  if (k > 0) off += cd * (A)xat(n - HW, k - 1, j, i);
-  if (k < D - 1) off += cu * (A)xat(n + HW, k + 1, j, i);
+  if (i > 0) off = fma_a(cw, (A)xat(n - 1, k, j, i - 1), off);
+  if (i < W - 1) off = fma_a(ce, (A)xat(n + 1, k, j, i + 1), off);
+  if (j > 0) off = fma_a(cs, (A)xat(n - W, k, j - 1, i), off);
+  if (j < H - 1) off = fma_a(cn, (A)xat(n + W, k, j + 1, i), off);
+  if (k > 0) off = fma_a(cd, (A)xat(n - HW, k - 1, j, i), off);
+  if (k < D - 1) off = fma_a(cu, (A)xat(n + HW, k + 1, j, i), off);
   return off;
 }
 
@@ -263,10 +276,10 @@ struct SrcCorrect {
     ld8(x + n, f);
     ld4(ec + ((k >> 1) * Hc + (j >> 1)) * Wc + (i0 >> 1), e);
 #pragma unroll
-    for (int m = 0; m < 8; ++m) v[m] = f[m] + alpha * e[m >> 1];
+    for (int m = 0; m < 8; ++m) v[m] = correct_val(f[m], e[m >> 1], alpha);
   }
   __device__ __forceinline__ float one(int n, int k, int j, int i) const {
-    return ldg(x + n) + alpha * ldg(ec + ((k >> 1) * Hc + (j >> 1)) * Wc + (i >> 1));
+    return correct_val(ldg(x + n), ldg(ec + ((k >> 1) * Hc + (j >> 1)) * Wc + (i >> 1)), alpha);
   }
 };
 
@@ -307,6 +320,38 @@ __device__ __forceinline__ float diag_rounded(const hfm_mg_task& t, int n, int i
   return cw + ce + cs + cn + cd + cu;
 }
 
+// off = sum of face * neighbour over a run in w, e, s, n, d, u order; wv / ev
+// are the x-neighbours of the run's ends (0 when missing), a missing s/n/d/u
+// neighbour contributes face * 0.  Every kernel forms the sum through this
+// one expression, so fused and separate passes round identically.
+template <typename A, typename C>
+__device__ __forceinline__ void offdiag_sum(const RunCoef<C>& q, const A xc[8], A wv, A ev,
+                                            const float vs[8], const float vn[8],
+                                            const float vd[8], const float vu[8], bool south,
+                                            bool north, bool down, bool up, A off[8]) {
+#pragma unroll
+  for (int m = 0; m < 8; ++m) off[m] = (A)q.cw[m] * (m ? xc[m - 1] : wv);
+#pragma unroll
+  for (int m = 0; m < 8; ++m) off[m] = fma_a((A)q.ce[m], m < 7 ? xc[m + 1] : ev, off[m]);
+#pragma unroll
+  for (int m = 0; m < 8; ++m) off[m] = fma_a((A)q.cs[m], south ? (A)vs[m] : (A)0, off[m]);
+#pragma unroll
+  for (int m = 0; m < 8; ++m) off[m] = fma_a((A)q.cn[m], north ? (A)vn[m] : (A)0, off[m]);
+#pragma unroll
+  for (int m = 0; m < 8; ++m) off[m] = fma_a((A)q.cd[m], down ? (A)vd[m] : (A)0, off[m]);
+#pragma unroll
+  for (int m = 0; m < 8; ++m) off[m] = fma_a((A)q.cu[m], up ? (A)vu[m] : (A)0, off[m]);
+}
+
+// The weighted-Jacobi value and the residual of one cell from its diagonal,
+// off-diagonal sum, right-hand side and current value.
+__device__ __forceinline__ float resid_val(float xc, float d, float bv, float off) {
+  return bv - __builtin_fmaf(d, xc, -off);
+}
+__device__ __forceinline__ float jacobi_val(float xc, float d, float bv, float off, float omega) {
+  return xc + omega * resid_val(xc, d, bv, off) / d;
+}
+
 // xc = x on the run, off = sum of face * neighbour (w, e, s, n, d, u order).
 // The five rows and the two row-end values are loaded first, from in-bounds
 // addresses (a missing neighbour re-reads the run and is zeroed after).
@@ -331,18 +376,7 @@ __device__ __forceinline__ void run_offdiag(const hfm_mg_task& t, const Run& r,
   for (int m = 0; m < 8; ++m) xc[m] = (A)xf[m];
   const A wv = west ? (A)wl : (A)0;
   const A ev = east ? (A)el : (A)0;
-#pragma unroll
-  for (int m = 0; m < 8; ++m) off[m] = (A)q.cw[m] * (m ? xc[m - 1] : wv);
-#pragma unroll
-  for (int m = 0; m < 8; ++m) off[m] += (A)q.ce[m] * (m < 7 ? xc[m + 1] : ev);
-#pragma unroll
-  for (int m = 0; m < 8; ++m) off[m] += (A)q.cs[m] * (south ? (A)vs[m] : (A)0);
-#pragma unroll
-  for (int m = 0; m < 8; ++m) off[m] += (A)q.cn[m] * (north ? (A)vn[m] : (A)0);
-#pragma unroll
-  for (int m = 0; m < 8; ++m) off[m] += (A)q.cd[m] * (down ? (A)vd[m] : (A)0);
-#pragma unroll
-  for (int m = 0; m < 8; ++m) off[m] += (A)q.cu[m] * (up ? (A)vu[m] : (A)0);
+  offdiag_sum<A>(q, xc, wv, ev, vs, vn, vd, vu, south, north, down, up, off);
 }
 
 __device__ __forceinline__ bool vec_task(const hfm_mg_task& t) { return (t.W & (kRun - 1)) == 0; }
@@ -371,15 +405,16 @@ __device__ __forceinline__ float sweep_one(const hfm_mg_task& t, int n, int i, i
     auto xat = [&](int m, int, int, int) { return ldg(xin + m); };
     const float off = face_row<float, T>(t, n, i, j, k, xat, diag);
     const float xc = ldg(xin + n);
-    return xc + omega * (bn - (diag * xc - off)) / diag;
+    return jacobi_val(xc, diag, bn, off, omega);
   } else {
     const int Hc = t.Hc, Wc = t.Wc;
     auto xat = [&](int m, int kk, int jj, int ii) {
-      return ldg(xin + m) + alpha * ldg(ec + ((kk >> 1) * Hc + (jj >> 1)) * Wc + (ii >> 1));
+      return correct_val(ldg(xin + m), ldg(ec + ((kk >> 1) * Hc + (jj >> 1)) * Wc + (ii >> 1)),
+                         alpha);
     };
     const float off = face_row<float, T>(t, n, i, j, k, xat, diag);
     const float xc = xat(n, k, j, i);
-    return xc + omega * (bn - (diag * xc - off)) / diag;
+    return jacobi_val(xc, diag, bn, off, omega);
   }
 }
 
@@ -406,7 +441,7 @@ __device__ __forceinline__ float restrict_one(const hfm_mg_task& t, int nc,
         const int n = (k * H + j) * W + i;
         float diag;
         const float off = face_row<float, T>(t, n, i, j, k, xat, diag);
-        sum += ldg(b + n) - (diag * ldg(x + n) - off);
+        sum += resid_val(ldg(x + n), diag, ldg(b + n), off);
       }
     }
   }
@@ -454,7 +489,7 @@ __global__ __launch_bounds__(B) void k_sweep(const hfm_mg_task* __restrict__ tas
         run_offdiag<float>(t, r, q, SrcCorrect<T>{xin, ec, t.Hc, t.Wc, alpha}, xc, off);
       }
 #pragma unroll
-      for (int m = 0; m < 8; ++m) o[m] = xc[m] + omega * (bv[m] - (d[m] * xc[m] - off[m])) / d[m];
+      for (int m = 0; m < 8; ++m) o[m] = jacobi_val(xc[m], d[m], bv[m], off[m], omega);
     }
     if (valid) {
       st8_nt(out + n0, o);
@@ -530,8 +565,8 @@ __global__ __launch_bounds__(B) void k_resid_restrict(const hfm_mg_task* __restr
         run_offdiag<float>(t, r, q, SrcPlain<T>{x}, xc, off);
 #pragma unroll
         for (int m = 0; m < 4; ++m) {
-          const float r2 = bv[2 * m] - (d[2 * m] * xc[2 * m] - off[2 * m]);
-          const float r3 = bv[2 * m + 1] - (d[2 * m + 1] * xc[2 * m + 1] - off[2 * m + 1]);
+          const float r2 = resid_val(xc[2 * m], d[2 * m], bv[2 * m], off[2 * m]);
+          const float r3 = resid_val(xc[2 * m + 1], d[2 * m + 1], bv[2 * m + 1], off[2 * m + 1]);
           if (valid) {
             s[m] += r2;
             s[m] += r3;
@@ -552,6 +587,431 @@ __global__ __launch_bounds__(B) void k_resid_restrict(const hfm_mg_task* __restr
     const int nc = base + s * B + threadIdx.x;
     if (nc >= Nc) break;
     put(bc + nc, from_f<T>(restrict_one<T>(t, nc, x, b)));
+  }
+}
+
+// ------------------------------------------------------------------ fused pass pairs
+// Two consecutive passes of a level in one launch; the first pass's result
+// never leaves the chip.  A block owns a tile of one task: `tile_rows` rows
+// of `tile_planes` planes, plus one halo row on each side when the tile does
+// not span the plane and one halo plane on each side.  It walks the tile
+// plane by plane: pass 1 (a Jacobi sweep) on plane kk, then pass 2 on plane
+// kk - 1, which needs pass 1 on planes kk - 2 .. kk.  A thread owns a run of
+// kPairRun x-cells of one tile row.  The plane's pass-1 input, its south
+// faces and pass 1's result go through LDS (in-plane neighbours); z-neighbours
+// stay in registers, and each plane's rows are loaded two planes ahead.  The
+// halo rows and planes are computed by pass 1 only.
+//   SR (!S2): pass 1 PLAIN sweep xin -> xout, pass 2 the residual b - A xout
+//     summed over each coarse cell's (up to 2x2x2) children / 8 -> rlow.
+//   S2:       pass 1 CORRECT sweep (xin + alpha P ec), pass 2 PLAIN sweep -> xout
+//     (type O), with the r.z partials when DOT.
+// Pass 1's result is rounded to T as a stored sweep is, and both passes use
+// the arithmetic of k_sweep / k_resid_restrict, so the outputs equal the
+// separate launches bitwise (the r.z partials are summed per tile instead).
+// Short runs and large blocks keep the per-thread state small: the walk is
+// latency-bound, so it needs waves more than it needs wide rows.
+constexpr int kPairRun = 4;
+constexpr int kPairBlock = 512;
+constexpr int kPairPlane = kPairRun * kPairBlock;  // cells of a tile plane, halo rows included
+
+// kPairRun values as loaded (storage type T), unpacked to float where used,
+// so the rows a thread keeps across planes cost T's width in registers.
+template <typename T>
+struct Raw4;
+template <>
+struct Raw4<float> {
+  f4v a;
+  __device__ __forceinline__ void load(const HFM_GLOBAL float* p) { a = *(const HFM_GLOBAL f4v*)p; }
+  __device__ __forceinline__ void load_nt(const HFM_GLOBAL float* p) {
+    a = __builtin_nontemporal_load((const HFM_GLOBAL f4v*)p);
+  }
+  __device__ __forceinline__ void lds_load(const float* p) { a = *(const f4v*)p; }
+  __device__ __forceinline__ void lds_store(float* p) const { *(f4v*)p = a; }
+  __device__ __forceinline__ void get(float v[4]) const {
+    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+  }
+  __device__ __forceinline__ void set(const float v[4]) { a = f4v{v[0], v[1], v[2], v[3]}; }
+};
+template <>
+struct Raw4<bf16> {
+  u2v u;
+  __device__ __forceinline__ void load(const HFM_GLOBAL bf16* p) { u = *(const HFM_GLOBAL u2v*)p; }
+  __device__ __forceinline__ void load_nt(const HFM_GLOBAL bf16* p) {
+    u = __builtin_nontemporal_load((const HFM_GLOBAL u2v*)p);
+  }
+  __device__ __forceinline__ void lds_load(const bf16* p) { u = *(const u2v*)p; }
+  __device__ __forceinline__ void lds_store(bf16* p) const { *(u2v*)p = u; }
+  __device__ __forceinline__ void get(float v[4]) const {
+    v[0] = __uint_as_float(u.x << 16); v[1] = __uint_as_float(u.x & 0xFFFF0000u);
+    v[2] = __uint_as_float(u.y << 16); v[3] = __uint_as_float(u.y & 0xFFFF0000u);
+  }
+  __device__ __forceinline__ void set(const float v[4]) {  // values of type bf16
+    u = u2v{pack2(v[0], v[1]), pack2(v[2], v[3])};
+  }
+};
+__device__ __forceinline__ void lds_ld4(const float* p, float v[4]) {
+  const f4v a = *(const f4v*)p;
+  v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+}
+__device__ __forceinline__ void lds_st4(float* p, const float v[4]) {
+  *(f4v*)p = f4v{v[0], v[1], v[2], v[3]};
+}
+__device__ __forceinline__ void lds_ld4(const bf16* p, float v[4]) {
+  Raw4<bf16> r;
+  r.lds_load(p);
+  r.get(v);
+}
+__device__ __forceinline__ float lds_ld1(const float* p) { return *p; }
+__device__ __forceinline__ float lds_ld1(const bf16* p) { return to_f(*p); }
+__device__ __forceinline__ void ld2(const HFM_GLOBAL float* p, float v[2]) {
+  typedef float f2v __attribute__((ext_vector_type(2)));
+  const f2v a = *(const HFM_GLOBAL f2v*)p;
+  v[0] = a.x; v[1] = a.y;
+}
+__device__ __forceinline__ void ld2(const HFM_GLOBAL bf16* p, float v[2]) {
+  const uint32_t u = *(const HFM_GLOBAL uint32_t*)p;
+  v[0] = __uint_as_float(u << 16);
+  v[1] = __uint_as_float(u & 0xFFFF0000u);
+}
+__device__ __forceinline__ void st2(HFM_GLOBAL float* p, const float v[2]) {
+  typedef float f2v __attribute__((ext_vector_type(2)));
+  *(HFM_GLOBAL f2v*)p = f2v{v[0], v[1]};
+}
+__device__ __forceinline__ void st2(HFM_GLOBAL bf16* p, const float v[2]) {
+  *(HFM_GLOBAL uint32_t*)p = pack2(v[0], v[1]);
+}
+__device__ __forceinline__ void st4_nt(HFM_GLOBAL float* p, const float v[4]) {
+  __builtin_nontemporal_store(f4v{v[0], v[1], v[2], v[3]}, (HFM_GLOBAL f4v*)p);
+}
+__device__ __forceinline__ void st4_nt(HFM_GLOBAL bf16* p, const float v[4]) {
+  __builtin_nontemporal_store(u2v{pack2(v[0], v[1]), pack2(v[2], v[3])}, (HFM_GLOBAL u2v*)p);
+}
+
+// A run's pass-1 input as loaded: x, or x and the coarse correction below it
+// (x + alpha P ec is formed where it is used, by correct_val as SrcCorrect).
+template <typename T, bool CORR>
+struct XRun {
+  Raw4<T> x;
+  float e[2];
+  __device__ __forceinline__ void load(const HFM_GLOBAL T* xp, const HFM_GLOBAL T* ec, int n, int k,
+                                       int j, int i0, int Hc, int Wc) {
+    x.load(xp + n);
+    if (CORR) ld2(ec + ((k >> 1) * Hc + (j >> 1)) * Wc + (i0 >> 1), e);
+  }
+  __device__ __forceinline__ void get(float v[4], float alpha) const {
+    x.get(v);
+    if (CORR) {
+#pragma unroll
+      for (int m = 0; m < 4; ++m) v[m] = correct_val(v[m], e[m >> 1], alpha);
+    }
+  }
+};
+
+// Diagonal and off-diagonal sum of a run from its face rows (cx: west faces,
+// ef: the east face of the run's last cell, cy / cn: south / north, cd / cu:
+// down / up) and neighbour values, in the order and with the rounding of
+// RunCoef::diag and offdiag_sum; each face row is unpacked where it is used.
+template <typename T>
+__device__ __forceinline__ void stencil_run(const Raw4<T>& cx, float ef, const Raw4<T>& cy,
+                                            const Raw4<T>& cn, const Raw4<T>& cd,
+                                            const Raw4<T>& cu, const float xc[4], float wv,
+                                            float ev, const float vs[4], const float vn[4],
+                                            const float vd[4], const float vu[4], bool south,
+                                            bool north, bool down, bool up, float d[4],
+                                            float off[4]) {
+  constexpr int N = kPairRun;
+  float c[N], ce[N];
+  cx.get(c);
+#pragma unroll
+  for (int m = 0; m < N - 1; ++m) ce[m] = c[m + 1];
+  ce[N - 1] = ef;
+#pragma unroll
+  for (int m = 0; m < N; ++m) {
+    d[m] = c[m] + ce[m];
+    off[m] = c[m] * (m ? xc[m - 1] : wv);
+    off[m] = fma_a(ce[m], m < N - 1 ? xc[m + 1] : ev, off[m]);
+  }
+  cy.get(c);
+#pragma unroll
+  for (int m = 0; m < N; ++m) {
+    d[m] += c[m];
+    off[m] = fma_a(c[m], south ? vs[m] : 0.0f, off[m]);
+  }
+  cn.get(c);
+#pragma unroll
+  for (int m = 0; m < N; ++m) {
+    d[m] += c[m];
+    off[m] = fma_a(c[m], north ? vn[m] : 0.0f, off[m]);
+  }
+  cd.get(c);
+#pragma unroll
+  for (int m = 0; m < N; ++m) {
+    d[m] += c[m];
+    off[m] = fma_a(c[m], down ? vd[m] : 0.0f, off[m]);
+  }
+  cu.get(c);
+#pragma unroll
+  for (int m = 0; m < N; ++m) {
+    d[m] += c[m];
+    off[m] = fma_a(c[m], up ? vu[m] : 0.0f, off[m]);
+  }
+}
+
+// Loads of one plane kk for pass 1, issued two planes ahead of their use.
+template <typename T, bool CORR>
+struct PlaneIn {
+  XRun<T, CORR> xn;    // pass-1 input on plane kk + 1 (own run)
+  XRun<T, CORR> edge;  // pass-1 input of the row past the tile's edge (edge rows only)
+  Raw4<T> cx, cy, cu, b, cn;  // faces of plane kk; cn: a north row not in LDS
+  float ef;
+};
+
+// Sum of v over a kPairBlock-thread block (valid in thread 0).
+__device__ __forceinline__ double pair_block_sum(double v) {
+  __shared__ double part[kPairBlock / 64];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) part[wid] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0) {
+    for (int w = 0; w < kPairBlock / 64; ++w) s += part[w];
+  }
+  return s;
+}
+
+template <typename T, bool S2, typename O, bool DOT>
+__global__ __launch_bounds__(kPairBlock, sizeof(T) == 2 ? 4 : 2) void k_smooth_pair(
+    const hfm_mg_task* __restrict__ tasks, const uint16_t* __restrict__ blk_task,
+    const hfm_mg_scalars* __restrict__ sc, float omega, float alpha, double* dots) {
+  constexpr int N = kPairRun;
+  __shared__ float lx[2][kPairPlane];  // pass-1 input of the plane
+  __shared__ T lc[2][kPairPlane];      // south faces of the plane
+  __shared__ T ly[2][kPairPlane];      // pass-1 result of the plane
+  __shared__ float lr[2][S2 ? 1 : kPairPlane / 2];  // odd rows' residuals (restriction)
+  // Tiles are numbered contiguously over the launch's tasks (a task's tiles
+  // are blk0 .. blk0 + nblk - 1); each XCD takes a contiguous range of them,
+  // so a task's neighbouring tiles (which re-read each other's halo) share an
+  // L2 and tasks with one tile still spread over the XCDs.  The table is
+  // padded to a multiple of 8 entries with 0xFFFF.
+  const int tile = xcd_chunk(blockIdx.x, gridDim.x);
+  const int task = blk_task[tile];
+  if (task == 0xFFFF) return;
+  const hfm_mg_task& t = tasks[task];
+  if (!sc[t.inst].active) return;
+  const int W = t.W, H = t.H, D = t.D, HW = H * W;
+  const int R = t.tile_rows, Z = t.tile_planes;
+  const int ny = (H + R - 1) / R;
+  const int lb = tile - t.blk0;
+  if (DOT && threadIdx.x == 0) {
+    // the finalize sums every dot slot of the instance: the ones past the
+    // tiles hold zero
+    const int nslot = sc[t.inst].nslot;
+    for (int s = t.nblk + lb; s < nslot; s += t.nblk) dots[t.slot0 + s] = 0.0;
+  }
+  const int yb = lb % ny, zb = lb / ny;
+  const int Q = W / N;
+  const int jh = R < H ? 1 : 0;  // halo rows
+  const int Rb = min(R, H - yb * R);
+  const int NR = Rb + 2 * jh;
+  const int r = int(threadIdx.x) / Q;
+  const int i0 = N * (int(threadIdx.x) - r * Q);
+  const int j = yb * R - jh + r;
+  // live: the thread's row is a row of the plane (tile or halo); rows past
+  // the plane's edges and threads past the tile do no work
+  const bool live = r < NR && j >= 0 && j < H;
+  const bool outrow = live && r >= jh && r < jh + Rb;
+  const int jc = min(max(j, 0), H - 1);
+  const int li = live ? r * W + i0 : 0;
+  const bool south = j > 0, north = j + 1 < H, west = i0 > 0, east = i0 + N < W;
+  // the rows next to the tile's first and last rows are not in LDS
+  const bool edge_s = r == 0 && south, edge_n = r + 1 == NR && north;
+  const int je = edge_s ? jc - 1 : (edge_n ? jc + 1 : jc);
+  const int k0 = zb * Z, k1 = min(k0 + Z, D);  // output planes [k0, k1)
+  const int ks = max(k0 - 1, 0);
+  const HFM_GLOBAL T* xin = gin<T>(t.xin);
+  const HFM_GLOBAL T* ec = gin<T>(t.ec);
+  const HFM_GLOBAL T* cx = gin<T>(t.c[0]);
+  const HFM_GLOBAL T* cy = gin<T>(t.c[1]);
+  const HFM_GLOBAL T* cz = gin<T>(t.c[2]);
+  const HFM_GLOBAL T* b = gin<T>(t.b);
+  const int Hc = t.Hc, Wc = t.Wc;
+  auto cell = [&](int k) { return (k * H + jc) * W + i0; };
+  auto load_plane = [&](PlaneIn<T, S2>& pi, int kk) {
+    const int n = cell(kk);
+    const bool up1 = kk + 1 < D;
+    pi.xn.load(xin, ec, up1 ? n + HW : n, up1 ? kk + 1 : kk, jc, i0, Hc, Wc);
+    pi.cx.load(cx + n);
+    pi.cy.load(cy + n);
+    pi.cu.load(up1 ? cz + n + HW : gin<T>(t.c[5]) + jc * W + i0);
+    pi.b.load_nt(b + n);
+    pi.ef = ldg(east ? cx + n + N : gin<T>(t.c[3]) + kk * H + jc);
+    if (edge_s || edge_n) pi.edge.load(xin, ec, n + (je - jc) * W, kk, je, i0, Hc, Wc);
+    if (!north)
+      pi.cn.load(gin<T>(t.c[4]) + kk * W + i0);
+    else if (edge_n)
+      pi.cn.load(cy + n + W);
+  };
+
+  XRun<T, S2> xm, x0;       // pass-1 input on planes kk-1, kk
+  Raw4<T> cd;               // down faces of plane kk
+  PlaneIn<T, S2> cur, nxt;  // loads of planes kk, kk + 1
+  // pass-2 state: faces, b and pass-1 results of plane kk-1 (kp), pass 1 on kp-1
+  Raw4<T> pcx, pcy, pcn, pcd, pcu, pb, ym2, ym1;
+  float pef = 0.0f;
+  float ev0[N];             // an even row's residuals on an even plane
+#pragma unroll
+  for (int m = 0; m < N; ++m) ev0[m] = 0.0f;
+  if (live) {
+    if (ks > 0) xm.load(xin, ec, cell(ks - 1), ks - 1, jc, i0, Hc, Wc);
+    x0.load(xin, ec, cell(ks), ks, jc, i0, Hc, Wc);
+    cd.load(cz + cell(ks));
+    load_plane(cur, ks);
+    if (ks + 1 < D) load_plane(nxt, ks + 1);
+  }
+  double acc = 0.0;
+
+  for (int kk = ks; kk <= k1; ++kk) {
+    const bool p1 = kk < D;  // pass 1 runs on plane kk (uniform)
+    const int buf = kk & 1;
+    PlaneIn<T, S2> nx2;
+    if (live && kk + 2 < D && kk + 2 <= k1) load_plane(nx2, kk + 2);
+    float xc[N];
+    if (p1 && live) {
+      x0.get(xc, alpha);
+      lds_st4(&lx[buf][li], xc);
+      cur.cy.lds_store(&lc[buf][li]);
+    }
+    __syncthreads();
+    Raw4<T> yk, cn;
+    if (p1 && live) {
+      if (north && !edge_n) cn.lds_load(&lc[buf][li + W]);
+      else cn = cur.cn;
+      float vs[N], vn[N], vd[N], vu[N], d[N], off[N];
+      if (r > 0) lds_ld4(&lx[buf][li - W], vs);
+      else cur.edge.get(vs, alpha);
+      if (r + 1 < NR) lds_ld4(&lx[buf][li + W], vn);
+      else cur.edge.get(vn, alpha);
+      xm.get(vd, alpha);
+      cur.xn.get(vu, alpha);
+      const float wv = west ? lx[buf][li - 1] : 0.0f;
+      const float ev = east ? lx[buf][li + N] : 0.0f;
+      stencil_run<T>(cur.cx, cur.ef, cur.cy, cn, cd, cur.cu, xc, wv, ev, vs, vn, vd, vu, south,
+                     north, kk > 0, kk + 1 < D, d, off);
+      float bv[N], y[N];
+      cur.b.get(bv);
+#pragma unroll
+      for (int m = 0; m < N; ++m) y[m] = round_to<T>(jacobi_val(xc[m], d[m], bv[m], off[m], omega));
+      yk.set(y);
+      if (!S2 && outrow && kk >= k0 && kk < k1) st4_nt(gout<T>(t.xout) + cell(kk), y);
+      yk.lds_store(&ly[buf][li]);
+    } else {
+      yk.set(ev0);  // zeros: plane kk is past the last (its pass-1 value is unused)
+    }
+    if (kk - 1 >= k0) {  // pass 2 on plane kp (uniform)
+      const int kp = kk - 1;
+      const int pb2 = kp & 1;
+      float e[N];
+      if (outrow) {
+        float yc[N], vs[N], vn[N], vd[N], vu[N], d[N], off[N], bv[N];
+        ym1.get(yc);
+        ym2.get(vd);
+        yk.get(vu);
+        lds_ld4(&ly[pb2][south ? li - W : li], vs);
+        lds_ld4(&ly[pb2][north ? li + W : li], vn);
+        const float wv = west ? lds_ld1(&ly[pb2][li - 1]) : 0.0f;
+        const float ev = east ? lds_ld1(&ly[pb2][li + N]) : 0.0f;
+        stencil_run<T>(pcx, pef, pcy, pcn, pcd, pcu, yc, wv, ev, vs, vn, vd, vu, south, north,
+                       kp > 0, kp + 1 < D, d, off);
+        pb.get(bv);
+        if constexpr (S2) {
+          float o[N];
+#pragma unroll
+          for (int m = 0; m < N; ++m) o[m] = jacobi_val(yc[m], d[m], bv[m], off[m], omega);
+          st4_nt(gout<O>(t.xout) + cell(kp), o);
+          if (DOT) {
+#pragma unroll
+            for (int m = 0; m < N; ++m) acc += double(bv[m]) * double(round_to<O>(o[m]));
+          }
+        } else {
+#pragma unroll
+          for (int m = 0; m < N; ++m) e[m] = resid_val(yc[m], d[m], bv[m], off[m]);
+        }
+      }
+      if constexpr (!S2) {
+        // rows 2J and 2J+1 of planes 2K and 2K+1 make coarse row (K, J): odd
+        // rows hand their residuals to the even row, which sums the children
+        // in k_resid_restrict's order (plane, row, cell)
+        const int pidx = ((r - jh) >> 1) * W + i0;
+        const bool odd = (j & 1) != 0;
+        if (outrow && odd) lds_st4(&lr[pb2][pidx], e);
+        if (outrow && !odd && !(kp & 1)) {
+#pragma unroll
+          for (int m = 0; m < N; ++m) ev0[m] = e[m];
+        }
+        if ((kp & 1) || kp == D - 1) {
+          __syncthreads();
+          if (outrow && !odd) {
+            const bool pair = north;        // row 2J+1 exists
+            const bool second = (kp & 1);   // plane 2K+1 exists
+            float s2v[N / 2], o0[N];
+#pragma unroll
+            for (int m = 0; m < N / 2; ++m) {
+              s2v[m] = 0.0f;
+              s2v[m] += ev0[2 * m];
+              s2v[m] += ev0[2 * m + 1];
+            }
+            if (pair) {
+              lds_ld4(&lr[0][pidx], o0);
+#pragma unroll
+              for (int m = 0; m < N / 2; ++m) {
+                s2v[m] += o0[2 * m];
+                s2v[m] += o0[2 * m + 1];
+              }
+            }
+            if (second) {
+#pragma unroll
+              for (int m = 0; m < N / 2; ++m) {
+                s2v[m] += e[2 * m];
+                s2v[m] += e[2 * m + 1];
+              }
+              if (pair) {
+                lds_ld4(&lr[1][pidx], o0);
+#pragma unroll
+                for (int m = 0; m < N / 2; ++m) {
+                  s2v[m] += o0[2 * m];
+                  s2v[m] += o0[2 * m + 1];
+                }
+              }
+            }
+            float o[N / 2];
+#pragma unroll
+            for (int m = 0; m < N / 2; ++m) o[m] = s2v[m] * 0.125f;
+            st2(gout<T>(t.rlow) + ((kp >> 1) * Hc + (j >> 1)) * Wc + (i0 >> 1), o);
+          }
+        }
+      }
+    }
+    // advance the walk one plane
+    pcx = cur.cx;
+    pcy = cur.cy;
+    pcn = cn;
+    pcd = cd;
+    pcu = cur.cu;
+    pef = cur.ef;
+    pb = cur.b;
+    ym2 = ym1;
+    ym1 = yk;
+    xm = x0;
+    x0 = cur.xn;
+    cd = cur.cu;
+    cur = nxt;
+    nxt = nx2;
+  }
+  if (DOT) {
+    const double s = pair_block_sum(acc);
+    if (threadIdx.x == 0) dots[t.slot0 + lb] = s;
   }
 }
 
@@ -1326,6 +1786,85 @@ extern "C" int hfm_mg_resid_restrict(const hfm_mg_task* tasks, const uint16_t* b
       hipLaunchKernelGGL((k_resid_restrict<T, 256>), grid, dim3(256), 0, s, tasks, blk_task, sc);
     else
       hipLaunchKernelGGL((k_resid_restrict<T, 512>), grid, dim3(512), 0, s, tasks, blk_task, sc);
+  });
+}
+
+namespace {
+// Planes per tile of the fused pass pairs: a tile computes pass 1 on two
+// halo planes, so longer walks waste less; 32 keeps a few tiles per task.
+constexpr int kTilePlanes = 32;
+}  // namespace
+
+extern "C" int hfm_mg_tile(int64_t D, int64_t H, int64_t W, int32_t* tile_rows,
+                           int32_t* tile_planes, int64_t* ntiles) {
+  if (!tile_rows || !tile_planes || !ntiles || D < 1 || H < 1 || W < 1 ||
+      D * H * W > INT32_MAX) {
+    set_error("hfm_mg_tile: bad arguments");
+    return HFM_EINVAL;
+  }
+  if (W % kRun) {
+    set_error("hfm_mg_tile: W = %lld is not a multiple of %d", (long long)W, kRun);
+    return HFM_EINVAL;
+  }
+  const int64_t Q = W / kPairRun;  // runs per row = threads per row
+  int64_t R, ny;
+  if (H * Q <= kPairBlock) {       // the whole plane fits: no halo rows
+    R = H;
+    ny = 1;
+  } else {
+    const int64_t rmax = (kPairBlock / Q - 2) & ~int64_t(1);  // tile rows + 2 halo rows fit
+    if (rmax < 2) {
+      set_error("hfm_mg_tile: W = %lld is too wide for a tile", (long long)W);
+      return HFM_EINVAL;
+    }
+    ny = (H + rmax - 1) / rmax;
+    R = 2 * ((H + 2 * ny - 1) / (2 * ny));  // balanced, even (restriction pairs)
+  }
+  const int64_t nz = (D + kTilePlanes - 1) / kTilePlanes;
+  const int64_t Z = 2 * ((D + 2 * nz - 1) / (2 * nz));
+  *tile_rows = int32_t(R);
+  *tile_planes = int32_t(Z);
+  *ntiles = ny * ((D + Z - 1) / Z);
+  return HFM_OK;
+}
+
+extern "C" int hfm_mg_smooth_restrict(const hfm_mg_task* tasks, const uint16_t* blk_task,
+                                      int64_t nblocks, const hfm_mg_scalars* sc, int store,
+                                      float omega, void* stream) {
+  if (int e = check_table(tasks, blk_task, nblocks, sc, "hfm_mg_smooth_restrict")) return e;
+  if (nblocks % 8) {  // the tile -> XCD map is a permutation of 8-block groups
+    set_error("hfm_mg_smooth_restrict: %lld blocks is not a multiple of 8", (long long)nblocks);
+    return HFM_EINVAL;
+  }
+  if (nblocks == 0) return HFM_OK;
+  return launch_store(store, "hfm_mg_smooth_restrict", [&](auto tag) {
+    using T = decltype(tag);
+    hipLaunchKernelGGL((k_smooth_pair<T, false, T, false>), dim3(unsigned(nblocks)),
+                       dim3(kPairBlock), 0, as_stream(stream), tasks, blk_task, sc, omega, 0.0f,
+                       nullptr);
+  });
+}
+
+extern "C" int hfm_mg_smooth2(const hfm_mg_task* tasks, const uint16_t* blk_task, int64_t nblocks,
+                              const hfm_mg_scalars* sc, int store, int out_store, float omega,
+                              float alpha, double* dots, void* stream) {
+  if (int e = check_table(tasks, blk_task, nblocks, sc, "hfm_mg_smooth2")) return e;
+  if (nblocks % 8) {  // the tile -> XCD map is a permutation of 8-block groups
+    set_error("hfm_mg_smooth2: %lld blocks is not a multiple of 8", (long long)nblocks);
+    return HFM_EINVAL;
+  }
+  if (nblocks == 0) return HFM_OK;
+  return launch_store2(store, out_store, "hfm_mg_smooth2", [&](auto tag, auto otag) {
+    using T = decltype(tag);
+    using O = decltype(otag);
+    const dim3 grid{unsigned(nblocks)};
+    hipStream_t s = as_stream(stream);
+    if (dots)
+      hipLaunchKernelGGL((k_smooth_pair<T, true, O, true>), grid, dim3(kPairBlock), 0, s, tasks,
+                         blk_task, sc, omega, alpha, dots);
+    else
+      hipLaunchKernelGGL((k_smooth_pair<T, true, O, false>), grid, dim3(kPairBlock), 0, s,
+                         tasks, blk_task, sc, omega, alpha, dots);
   });
 }
 

@@ -33,9 +33,9 @@ STATUS = {0: "running", 1: "converged", 2: "maxit", 3: "breakdown"}
 
 TASK_DTYPE = np.dtype([
     ("D", "<i4"), ("H", "<i4"), ("W", "<i4"), ("inst", "<i4"),
-    ("blk0", "<i4"), ("nblk", "<i4"), ("slot0", "<i4"), ("flags", "<i4"),
+    ("blk0", "<i4"), ("nblk", "<i4"), ("slot0", "<i4"), ("tile_rows", "<i4"),
     ("magW", "<u8"), ("magH", "<u8"),
-    ("Dc", "<i4"), ("Hc", "<i4"), ("Wc", "<i4"), ("pad0", "<i4"),
+    ("Dc", "<i4"), ("Hc", "<i4"), ("Wc", "<i4"), ("tile_planes", "<i4"),
     ("magWc", "<u8"), ("magHc", "<u8"),
     ("c", "<u8", (6,)), ("xin", "<u8"), ("xout", "<u8"), ("b", "<u8"), ("ec", "<u8"),
     ("x64", "<u8"), ("r64", "<u8"), ("rlow", "<u8"), ("b32", "<u8"),
@@ -80,6 +80,10 @@ SIGNATURES = {
     "hfm_bluff_body_f32": (_int, [_vp, _vp, _int, _vp, _f32, _i64, _vp]),
     "hfm_mg_sweep": (_int, [_vp, _vp, _i64, _vp, _int, _int, _int, _f32, _f32, _int, _vp, _vp]),
     "hfm_mg_resid_restrict": (_int, [_vp, _vp, _i64, _vp, _int, _int, _vp]),
+    "hfm_mg_smooth_restrict": (_int, [_vp, _vp, _i64, _vp, _int, _f32, _vp]),
+    "hfm_mg_smooth2": (_int, [_vp, _vp, _i64, _vp, _int, _int, _f32, _f32, _vp, _vp]),
+    "hfm_mg_tile": (_int, [_i64, _i64, _i64, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                           C.POINTER(_i64)]),
     "hfm_mg_coarse_solve": (_int, [_vp, _i64, _vp, _int, _vp]),
     "hfm_mg_coarse_cycle": (_int, [_vp, _vp, _i64, _vp, _int, _int, _int, _f32, _f32, _vp]),
     "hfm_fcg_init": (_int, [_vp, _vp, _i64, _vp, _int, _vp, _vp]),
@@ -186,6 +190,17 @@ def ptr_array(ptrs):
     for q, p in enumerate(ptrs):
         arr[q] = p
     return arr
+
+
+def tile(shape):
+    """(tile_rows, tile_planes, tiles) of the fused pass pairs for a level of
+    extents (D, H, W), or None when the level cannot be tiled (hfm_mg_tile)."""
+    lib = load()
+    R, Z, n = C.c_int32(), C.c_int32(), _i64()
+    D, H, W = (int(v) for v in shape)
+    if lib.hfm_mg_tile(D, H, W, C.byref(R), C.byref(Z), C.byref(n)) != HFM_OK:
+        return None
+    return R.value, Z.value, n.value
 
 
 def magic(d: int) -> int:

@@ -39,6 +39,7 @@ CHUNK = 8 * BLOCK     # cells per block of the cell kernels (a thread owns 8)
 RCHUNK = 4 * BLOCK    # coarse cells per block of the residual + restriction
 BLOCKS = (128, 256, 512)   # threads per block a sweep / restriction launch may use
 SCHUNK = 256          # cells per block of the batched setup kernels
+PAIR_BLOCK = 512      # threads per block of the fused pass pairs (one tile each)
 ALIGN = 8             # element alignment of sub-array offsets (16-byte rows)
 ELEM = {"f32": 4, "bf16": 2}
 TORCH_STORE = {"f32": torch.float32, "bf16": torch.bfloat16}
@@ -127,6 +128,19 @@ class Params:
     # levels (below the finest) of at most this many cells run in one
     # workgroup per instance (hfm_mg_coarse_cycle); the coarsest always does
     coarse_cells: int = 4096
+    # fused pass pairs on levels that tile (hfm_mg_smooth_restrict: the last
+    # pre-sweep with the restriction; hfm_mg_smooth2: the first two
+    # post-sweeps); results are bitwise those of the separate launches.  Only
+    # the finest `fuse_levels` levels: a fused pair walks its tiles plane by
+    # plane, which pays on the fine level and not on the 8x smaller ones
+    # (MI355X, 512-instance batch: L0 1.36 + 1.60 ms fused vs 1.66 + 1.67
+    # separate; L1 0.23 + 0.27 vs 0.23 + 0.21)
+    fuse: bool = True
+    fuse_levels: int = 1
+    # a plan runs the fine level's pairs fused only with at least this many
+    # tiles over its instances (a few tiles leave most CUs idle: one 64 x 256
+    # x 256 instance has 86); its buffers are the same either way
+    fuse_min_tiles: int = 256
 
     def __post_init__(self):
         if self.storage not in ELEM:
@@ -261,8 +275,9 @@ class Instance:
         self.r64 = torch.zeros(ilv_len(N0), dtype=torch.float64, device=dev)
         so = TORCH_STORE[p.outer]
         self.P = [torch.zeros(N0, dtype=so, device=dev), torch.zeros(N0, dtype=so, device=dev)]
-        # z = V(rlow) gets its own buffer when it is wider than the cycle's vectors
-        self.Z = torch.zeros(N0, dtype=so, device=dev) if p.outer != p.storage else None
+        # z = V(rlow) has a buffer of its own (type outer): the fine level's last
+        # post-sweep writes it whether or not its pass pair runs fused
+        self.Z = torch.zeros(N0, dtype=so, device=dev)
         nc = math.prod(self.levels[-1])
         self.ainv = torch.zeros(nc * nc, dtype=torch.float64, device=dev)
         self.nblk = fine_blocks(self.shape)
@@ -281,7 +296,7 @@ class Instance:
 
     def nbytes(self) -> int:
         ts = [*self.face32, *self.X, *self.Y, *self.B, *self.P, self.sigma, self.b32, self.x64,
-              self.r64, self.ainv] + ([self.Z] if self.Z is not None else [])
+              self.r64, self.ainv, self.Z]
         if self.pool.params.storage != "f32":
             ts += self.faceT
         return sum(t.numel() * t.element_size() for t in ts)
@@ -299,8 +314,7 @@ class Instance:
         N0 = math.prod(shape)
         nc = math.prod(lv[-1])
         eo = ELEM[params.outer]
-        z = N0 * eo if params.outer != params.storage else 0
-        return tot + 3 * _align(N0) * 4 + N0 * (4 + 2 * eo) + 16 * ilv_len(N0) + z + nc * nc * 8
+        return tot + 3 * _align(N0) * 4 + N0 * (4 + 3 * eo) + 16 * ilv_len(N0) + nc * nc * 8
 
     def faces(self, lev: int, low: bool) -> List[int]:
         """Addresses of the six face arrays of a level (storage copy if ``low``)."""
@@ -393,6 +407,22 @@ def chain_records(inst: "Instance", lc: int) -> list:
     return recs
 
 
+def _tile_table(device, recs, ntiles):
+    """Task table + tile layout of a fused pass-pair launch: task q owns tiles
+    blk0 .. blk0 + ntiles[q] - 1, numbered contiguously; the tile -> task map
+    is padded with 0xFFFF to a multiple of 8 (hfm_mg_smooth_restrict)."""
+    arr = np.array(recs, dtype=L.TASK_DTYPE)
+    n = np.asarray(ntiles, dtype=np.int64)
+    blk0 = np.concatenate([[0], np.cumsum(n)[:-1]]).astype(np.int64)
+    arr["blk0"], arr["nblk"] = blk0, n
+    total = int(n.sum())
+    padded = _round8(total)
+    bt = np.full(padded, 0xFFFF, dtype=np.uint16)
+    bt[:total] = np.repeat(np.arange(len(n), dtype=np.uint16), n)
+    lay = _Layout(blk0, n, torch.from_numpy(bt.view(np.int16).copy()).to(device), padded)
+    return torch.from_numpy(arr.view(np.uint8).copy()).to(device), lay
+
+
 def _table(device, recs, sizes, chunk):
     """Device task table + block layout for the records (one per member)."""
     arr = np.array(recs, dtype=L.TASK_DTYPE)
@@ -401,17 +431,44 @@ def _table(device, recs, sizes, chunk):
     return torch.from_numpy(arr.view(np.uint8).copy()).to(device), lay
 
 
-def instance_templates(inst: Instance, p: Params) -> dict:
+def fused_levels(inst: Instance, p: Params, lc: int):
+    """Levels (below ``lc``, where the coarse cycle starts) whose pass pairs run
+    fused: ({lev: tile} for smooth + restrict, {lev: tile} for the two
+    post-sweeps).  The fine level's post pair carries the r.z partials, one
+    per tile, so its tiles must fit the instance's dot slots."""
+    sr, s2 = {}, {}
+    if not p.fuse:
+        return sr, s2
+    for lev in range(min(lc, len(inst.levels) - 1, p.fuse_levels)):
+        tl = L.tile(inst.levels[lev])
+        if tl is None:
+            continue
+        if p.nu1 >= 2:
+            sr[lev] = tl
+        if p.nu2 >= 2 and not (lev == 0 and p.nu2 == 2 and tl[2] > inst.nblk):
+            s2[lev] = tl
+    return sr, s2
+
+
+def instance_templates(inst: Instance, p: Params, lc: int) -> dict:
     """Task records of every launch of the FCG iteration that touch ``inst``,
-    keyed by launch; block fields are filled in when a plan is assembled."""
-    pre_k = (p.nu1 - 1) % 2
-    post_k = (p.nu1 + p.nu2 - 1) % 2
+    keyed by launch; block fields are filled in when a plan is assembled.
+    ``lc``: the instance's first coarse-cycle level (levels below it have
+    per-level launches, and may have fused pass pairs)."""
+    sr, s2 = fused_levels(inst, p, lc)
 
     def xy(k, lev):  # sweeps alternate between the X and Y buffers
         return (inst.X, inst.Y)[k % 2][lev].data_ptr()
 
+    def post_out(lev, s):
+        # buffer written by post-sweep s; a fused pair (s = 0, 1) writes the
+        # buffer of s = 0, since its input (pre-sweep nu1 - 1) is the other one
+        return xy(p.nu1 + s - (1 if lev in s2 and s >= 1 else 0), lev)
+
     def E(lev):  # buffer holding a level's correction after its cycle
-        return inst.X[lev].data_ptr() if lev == len(inst.levels) - 1 else xy(post_k, lev)
+        if lev == len(inst.levels) - 1:
+            return inst.X[lev].data_ptr()
+        return post_out(lev, p.nu2 - 1)
 
     tpl = {}
     for lev in range(len(inst.levels) - 1):
@@ -424,22 +481,34 @@ def instance_templates(inst: Instance, p: Params) -> dict:
             tpl[("down", lev, s)] = t
         t = _record(inst, lev, coarse=inst.levels[lev + 1])
         t["c"] = inst.faces(lev, True)
-        t["xin"] = xy(pre_k, lev)
+        t["xin"] = xy(p.nu1 - 1, lev)
         t["b"] = inst.B[lev].data_ptr()
         t["xout"] = inst.B[lev + 1].data_ptr()
         tpl[("restrict", lev)] = t
         for s in range(p.nu2):
-            k = p.nu1 + s
             t = _record(inst, lev, coarse=inst.levels[lev + 1])
             t["c"] = inst.faces(lev, True)
-            t["xin"] = xy(k - 1, lev)
-            t["xout"] = xy(k, lev)
+            t["xin"] = xy(p.nu1 - 1, lev) if s == 0 else post_out(lev, s - 1)
+            t["xout"] = post_out(lev, s)
             t["b"] = inst.B[lev].data_ptr()
             if s == 0:
                 t["ec"] = E(lev + 1)
-            if lev == 0 and s == p.nu2 - 1 and inst.Z is not None:
+            if lev == 0 and s == p.nu2 - 1:
                 t["xout"] = inst.Z.data_ptr()
             tpl[("up", lev, s)] = t
+        if lev in sr:
+            t = tpl[("restrict", lev)].copy()
+            last = tpl[("down", lev, p.nu1 - 1)]
+            t["xin"], t["xout"] = last["xin"], last["xout"]
+            t["rlow"] = inst.B[lev + 1].data_ptr()
+            t["tile_rows"], t["tile_planes"] = sr[lev][:2]
+            tpl[("smooth_restrict", lev)] = t
+        if lev in s2:
+            t = tpl[("up", lev, 0)].copy()
+            t["xout"] = tpl[("up", lev, 1)]["xout"]
+            t["tile_rows"], t["tile_planes"] = s2[lev][:2]
+            tpl[("smooth2", lev)] = t
+    inst._tiles = {lev: tl[2] for lev, tl in {**sr, **s2}.items()}
     lc = len(inst.levels) - 1
     t = _record(inst, lc)
     t["b"] = inst.B[lc].data_ptr()
@@ -451,7 +520,7 @@ def instance_templates(inst: Instance, p: Params) -> dict:
     base["x64"], base["r64"] = inst.x64.data_ptr(), inst.r64.data_ptr()
     base["rlow"] = inst.B[0].data_ptr()
     base["b32"] = inst.b32.data_ptr()
-    base["b"] = xy(post_k, 0) if inst.Z is None else inst.Z.data_ptr()   # z
+    base["b"] = tpl[("up", 0, p.nu2 - 1)]["xout"]   # z: the fine level's last post-sweep
     base["xout"] = xy(0, 0)     # first fine-level pre-sweep, fused into init / update
     tpl[("init",)] = base
     for par in range(2):
@@ -544,44 +613,74 @@ class Plan:
         self.insts = list(insts)
         p = pool.params
         dev = pool.device
+        lcs = {i.slot: chain_start(i.levels, p.coarse_cells) for i in insts}
         for inst in insts:
-            if getattr(inst, "_tpl", None) is None:
-                inst._tpl = instance_templates(inst, p)
+            # records depend on where the coarse cycle starts (fused levels
+            # end there) and on fusion itself
+            key = (lcs[inst.slot], p.fuse, p.fuse_levels, p.nu1, p.nu2)
+            if getattr(inst, "_tpl", None) is None or inst._tpl_key != key:
+                inst._tpl = instance_templates(inst, p, lcs[inst.slot])
+                inst._tpl_key = key
         nlev = max(len(i.levels) for i in insts)
         self.nlev = nlev
-        lcs = {i.slot: chain_start(i.levels, p.coarse_cells) for i in insts}
 
-        def op(key, chunk, size_of, lev=None):
+        def op(key, chunk, size_of, lev=None, keep=None):
             # per-level launches stop where an instance's coarse cycle starts
-            members = [i for i in insts if key in i._tpl and (lev is None or lev < lcs[i.slot])]
+            members = [i for i in insts if key in i._tpl and (lev is None or lev < lcs[i.slot])
+                       and (keep is None or keep(i))]
             if not members:
                 return None
-            return _table(dev, [i._tpl[key] for i in members], [size_of(i) for i in members],
-                          chunk)
+            recs, sizes = [i._tpl[key] for i in members], [size_of(i) for i in members]
+            if chunk is None:   # fused pass pair: one block per tile
+                return _tile_table(dev, recs, sizes)
+            return _table(dev, recs, sizes, chunk)
+
+        # the fine level's records are the same buffers fused or not (z has a
+        # buffer of its own), so a plan with few tiles there runs it unfused;
+        # coarser fused levels (and nu2 > 2) fix their buffers in the records
+        fine_tiles = sum(i._tiles.get(0, 0) for i in insts)
+        self.fuse_fine = fine_tiles >= p.fuse_min_tiles or p.nu2 > 2
+
+        def fused(key):
+            """Members whose pass pair ``key`` runs fused (fused_levels)."""
+            if key[1] == 0 and not self.fuse_fine:
+                return lambda i: False
+            return lambda i: key in i._tpl
 
         blocks = p.blocks or {}
         self.ops_down, self.ops_up = [], []
         for lev in range(nlev - 1):
             cells = lambda i, lev=lev: math.prod(i.levels[lev])  # noqa: E731
+            tiles = lambda i, lev=lev: i._tiles[lev]  # noqa: E731
+            fr, f2 = fused(("smooth_restrict", lev)), fused(("smooth2", lev))
+            not_fr = lambda i, fr=fr: not fr(i)  # noqa: E731
+            not_f2 = lambda i, f2=f2: not f2(i)  # noqa: E731
             for s in range(p.nu1):
                 if lev == 0 and s == 0:
                     continue    # computed by the FCG update (and init) kernels
                 mode = "zero" if s == 0 else "plain"
                 b = blocks.get(("sweep", mode, lev), BLOCK)
-                tab = op(("down", lev, s), 8 * b, cells, lev)
+                tab = op(("down", lev, s), 8 * b, cells, lev, not_fr if s == p.nu1 - 1 else None)
                 if tab:
                     self.ops_down.append(("sweep", L.SWEEP[mode], tab[0], tab[1], False, lev, b))
             b = blocks.get(("restrict", lev), BLOCK)
             tab = op(("restrict", lev), 4 * b, lambda i, lev=lev: math.prod(i.levels[lev + 1]),
-                     lev)
+                     lev, not_fr)
             if tab:
                 self.ops_down.append(("restrict", 0, tab[0], tab[1], False, lev, b))
+            tab = op(("smooth_restrict", lev), None, tiles, lev, fr)
+            if tab:
+                self.ops_down.append(("smooth_restrict", 0, tab[0], tab[1], False, lev, PAIR_BLOCK))
             ups = []
+            tab = op(("smooth2", lev), None, tiles, lev, f2)
+            if tab:
+                ups.append(("smooth2", 0, tab[0], tab[1], lev == 0 and p.nu2 == 2, lev,
+                            PAIR_BLOCK))
             for s in range(p.nu2):
                 mode = "correct" if s == 0 else "plain"
                 dot = lev == 0 and s == p.nu2 - 1
                 b = BLOCK if dot else blocks.get(("sweep", mode, lev), BLOCK)
-                tab = op(("up", lev, s), 8 * b, cells, lev)
+                tab = op(("up", lev, s), 8 * b, cells, lev, not_f2 if s < 2 else None)
                 if tab:
                     ups.append(("sweep", L.SWEEP[mode], tab[0], tab[1], dot, lev, b))
             self.ops_up.insert(0, ups)
@@ -612,6 +711,14 @@ class Plan:
             L.call("hfm_mg_sweep", tasks.data_ptr(), lay.blk_task.data_ptr(), lay.nblocks,
                    pool.sc.data_ptr(), pool.store_id, out, mode, p.omega, p.alpha, block,
                    pool.dots.data_ptr() if dot else 0, s)
+        elif kind == "smooth2":
+            out = pool.outer_id if dot else pool.store_id
+            L.call("hfm_mg_smooth2", tasks.data_ptr(), lay.blk_task.data_ptr(), lay.nblocks,
+                   pool.sc.data_ptr(), pool.store_id, out, p.omega, p.alpha,
+                   pool.dots.data_ptr() if dot else 0, s)
+        elif kind == "smooth_restrict":
+            L.call("hfm_mg_smooth_restrict", tasks.data_ptr(), lay.blk_task.data_ptr(),
+                   lay.nblocks, pool.sc.data_ptr(), pool.store_id, p.omega, s)
         else:
             L.call("hfm_mg_resid_restrict", tasks.data_ptr(), lay.blk_task.data_ptr(),
                    lay.nblocks, pool.sc.data_ptr(), pool.store_id, block, s)
@@ -697,9 +804,10 @@ class MGBatch:
                  alpha: float = 1.6, max_coarse_edge: int = 4, device=None,
                  iters_per_graph: int = 4, use_graph: bool = True, shrink: float = 0.5,
                  outer: Optional[str] = None, blocks: Optional[dict] = None,
-                 coarse_cells: int = 4096):
+                 coarse_cells: int = 4096, fuse: bool = True, fuse_levels: int = 1,
+                 fuse_min_tiles: int = 256):
         self.params = Params(storage, tol, maxit, nu1, nu2, omega, alpha, max_coarse_edge, outer,
-                             blocks, coarse_cells)
+                             blocks, coarse_cells, fuse, fuse_levels, fuse_min_tiles)
         shapes = [check_shape(s, max_coarse_edge)[0] for s in shapes]
         if not shapes:
             raise ValueError("empty batch")
@@ -768,6 +876,17 @@ class MGBatch:
         self.params.coarse_cells = int(n)
         self._replan()
 
+    def set_fuse(self, fuse: bool, levels: Optional[int] = None):
+        """Run the fused pass pairs (default) or every pass as its own launch,
+        on the finest ``levels`` levels; rebuilds the plans.  All give bitwise
+        the same V-cycle."""
+        self.params.fuse = bool(fuse)
+        if levels is not None:
+            self.params.fuse_levels = int(levels)
+        for inst in self.insts:
+            inst._tpl = None
+        self._replan()
+
     def _replan(self):
         self.plan = Plan(self.pool, self.insts)
         self._subplans = {}
@@ -834,10 +953,25 @@ class MGBatch:
         idx = torch.tensor([i.slot for i in self.insts], device=self.device)
         self.pool.xlag_view().index_fill_(0, idx, int(lag))
 
-    def launch_fine_sweep(self):
-        """The fine-level PLAIN (second pre-) sweep of the V-cycle."""
-        op = next(o for o in self.plan.ops_down if o[0] == "sweep" and o[1] == L.SWEEP["plain"])
-        self.plan._launch(op, L.stream_handle(self.device))
+    def _fine_pre_op(self):
+        ops = [o for o in self.plan.ops_down if o[5] == 0]
+        fused = next((o for o in ops if o[0] == "smooth_restrict"), None)
+        return fused or next(o for o in ops if o[0] == "sweep" and o[1] == L.SWEEP["plain"])
+
+    def launch_fine_pre(self):
+        """The fine level's second pre-smoothing launch: the fused sweep +
+        restriction, or the PLAIN sweep when the plan does not fuse."""
+        self.plan._launch(self._fine_pre_op(), L.stream_handle(self.device))
+
+    def fine_pre_kernel(self) -> str:
+        return "k_smooth_pair SR" if self._fine_pre_op()[0] == "smooth_restrict" else \
+            "k_sweep PLAIN"
+
+    def fine_pre_bytes_per_cell(self) -> float:
+        """Algorithmic bytes per fine cell of ``launch_fine_pre``: x, b, 3 faces
+        read and the sweep written (T), plus the coarse rhs (T / 8) when fused."""
+        e = ELEM[self.storage]
+        return 6 * e + (e / 8 if self._fine_pre_op()[0] == "smooth_restrict" else 0)
 
     def update_bytes_per_cell(self, lag: int = 1) -> int:
         """Algorithmic bytes of the fused k_fcg_update per fine cell: p (outer
@@ -847,6 +981,4 @@ class MGBatch:
         eo = ELEM[self.params.outer]
         return 2 * ELEM[self.storage] + eo + 12 + 16 + (16 + eo if lag else 0)
 
-    def sweep_bytes_per_cell(self) -> int:
-        """Algorithmic bytes of a PLAIN sweep per cell: x, b, 3 faces, out (all T)."""
-        return 6 * ELEM[self.storage]
+

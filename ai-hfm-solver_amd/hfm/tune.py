@@ -69,7 +69,7 @@ def _activate(mg: MGBatch):
 
 def _op_key(op) -> Optional[Key]:
     kind, mode, _, _, dot, lev, _ = op
-    if dot:
+    if dot or kind not in ("sweep", "restrict"):   # fused pass pairs have a fixed shape
         return None
     if kind == "restrict":
         return ("restrict", lev)

@@ -421,7 +421,10 @@ def profile_iteration(mg, cells, torch):
     ops = [("down", op) for op in plan.ops_down] + [("up", o) for ups in plan.ops_up for o in ups]
     for leg, op in ops:
         kind, mode, lev = op[0], op[1], op[5]
-        label = f"{leg} L{lev} " + ("restrict" if kind == "restrict" else "sweep " + names[mode])
+        label = f"{leg} L{lev} " + {"restrict": "restrict",
+                                    "smooth_restrict": "sweep plain + restrict (fused)",
+                                    "smooth2": "sweep correct + plain (fused)"}.get(
+                                        kind, "sweep " + names.get(mode, ""))
         label += " + r.z partials" if op[4] else ""
         rows.append((label, kernel_time_ms(lambda op=op: plan._launch(op, s), 5, torch)))
     rows.append((f"coarse cycle (levels >= {plan.chain_levels})",
@@ -520,7 +523,7 @@ def profile_kernel(args, dev, torch):
     specs = batched_specs(args.instances, args.seed)
     cells = sum(math.prod(s[0]) for s in specs)
     mg = MGBatch([s[0] for s in specs], storage=args.storage, tol=1e-8, device=dev,
-                 outer=args.outer)
+                 outer=args.outer, fuse=not args.no_fuse)
     load_batch(mg, specs, dev)
     mg.build_faces()
     mg.setup()
@@ -551,8 +554,9 @@ def profile_kernel(args, dev, torch):
                     "bytes_per_cell": per}
     else:
         variants = {}
-        per_cell = {"sweep": mg.sweep_bytes_per_cell()}
-        ms = kernel_time_ms(mg.launch_fine_sweep, args.reps, torch)
+        per_cell = {"sweep": mg.fine_pre_bytes_per_cell()}
+        variants = {"launch": mg.fine_pre_kernel()}
+        ms = kernel_time_ms(mg.launch_fine_pre, args.reps, torch)
     nbytes = 1 << 30
     src = torch.empty(nbytes // 4, dtype=torch.float32, device=dev).fill_(1.0)
     dst = torch.empty_like(src)
@@ -702,6 +706,8 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=300.0)
     ap.add_argument("--serve-sat-rate", type=float, default=3000.0)
     ap.add_argument("--no-fp32-batch", action="store_true")
+    ap.add_argument("--no-fuse", action="store_true",
+                    help="every V-cycle pass as its own launch (no fused pass pairs)")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep 256-thread blocks (skip the launch-shape tuner)")
     ap.add_argument("--profile-kernel",
@@ -748,7 +754,7 @@ def main():
     cells = sum(math.prod(s) for s in shapes)
     log(f"bench: {len(specs)} instances, {cells / 1e6:.1f} M fine cells, storage {args.storage}")
     mg = MGBatch(shapes, storage=args.storage, tol=1e-8, maxit=300, device=dev, iters_per_graph=4,
-                 outer=args.outer)
+                 outer=args.outer, fuse=not args.no_fuse)
     load_batch(mg, specs, dev)
     tuning = None
     if not args.no_tune:
@@ -801,8 +807,8 @@ def main():
     upd_ms = 0.5 * sum(upd_t)
     upd_bytes = fine_cells * 0.5 * (mg.update_bytes_per_cell(1) + mg.update_bytes_per_cell(0))
     achieved = upd_bytes / (upd_ms * 1e-3) / 1e9
-    sweep_ms = kernel_time_ms(lambda: mg.launch_fine_sweep(), 20, torch)
-    sweep_bytes = fine_cells * mg.sweep_bytes_per_cell()
+    sweep_ms = kernel_time_ms(lambda: mg.launch_fine_pre(), 20, torch)
+    sweep_bytes = fine_cells * mg.fine_pre_bytes_per_cell()
     traffic = None
     if (args.instances, args.seed, mg.params.storage, mg.params.outer) == UPDATE_TRAFFIC["workload"]:
         traffic = update_traffic()
@@ -814,7 +820,7 @@ def main():
                                         "bytes_per_cell": mg.update_bytes_per_cell(1)},
                              "without_x": {"launch_ms": round(upd_t[1], 4),
                                            "bytes_per_cell": mg.update_bytes_per_cell(0)}},
-                "fine_sweep": {"kernel": "k_sweep PLAIN", "launch_ms": round(sweep_ms, 4),
+                "fine_pre": {"kernel": mg.fine_pre_kernel(), "launch_ms": round(sweep_ms, 4),
                                "achieved": round(sweep_bytes / (sweep_ms * 1e-3) / 1e9, 1)}}
     extra = {}
     outer_storage = mg.params.outer

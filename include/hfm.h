@@ -180,9 +180,11 @@ int hfm_bluff_body_f32(const float* const* in, float* const* out, int nfields,
  * pointer slots depends on the kernel (see each entry point). */
 typedef struct hfm_mg_task {
   int32_t D, H, W, inst;        /* extents of this level, instance index       */
-  int32_t blk0, nblk, slot0, flags; /* first block, block count, first dot slot */
+  int32_t blk0, nblk, slot0;    /* first block, block count, first dot slot    */
+  int32_t tile_rows;            /* fused pass pairs: rows per tile (hfm_mg_tile) */
   uint64_t magW, magH;          /* floor(2^64/W)+1, floor(2^64/H)+1 (0 if 1)   */
-  int32_t Dc, Hc, Wc, pad0;     /* coarse extents (restriction / correction)   */
+  int32_t Dc, Hc, Wc;           /* coarse extents (restriction / correction)   */
+  int32_t tile_planes;          /* fused pass pairs: planes per tile            */
   uint64_t magWc, magHc;
   const void* c[6];             /* cx, cy, cz, bxh, byh, bzh                    */
   const void* xin;
@@ -242,6 +244,37 @@ int hfm_mg_sweep(const hfm_mg_task* tasks, const uint16_t* blk_task, int64_t nbl
  * Blocks of `block` threads (128, 256, 512) cover 4 * block coarse cells. */
 int hfm_mg_resid_restrict(const hfm_mg_task* tasks, const uint16_t* blk_task, int64_t nblocks,
                           const hfm_mg_scalars* sc, int store, int block, void* stream);
+
+/* Fused pass pairs of one level (one launch instead of two; the first
+ * pass's result stays on chip).  Each task is cut into tiles of tile_rows
+ * rows x tile_planes planes (fields of the task, from hfm_mg_tile); one
+ * block per tile: task q owns the tiles blk0 .. blk0 + nblk - 1 (nblk = its
+ * tile count), numbered contiguously over the tasks, and blk_task maps a
+ * tile to its task, padded with 0xFFFF to nblocks (a multiple of 8).  Needs
+ * W % 8 == 0.  512 threads per block.
+ *   smooth_restrict: xout = the PLAIN sweep of xin against b, and
+ *     rlow (coarse, extents Dc/Hc/Wc) = R (b - A xout): hfm_mg_sweep (PLAIN)
+ *     then hfm_mg_resid_restrict, bitwise (the residual and its restrictions of
+ *     ART/Anisotropic_3d_Multigrid_7point.ipynb:407-418, with a smoothing
+ *     sweep before).
+ *   smooth2: xout (type out_store) = PLAIN(CORRECT(xin, ec)) against b:
+ *     hfm_mg_sweep (CORRECT) then hfm_mg_sweep (PLAIN), bitwise; with `dots`
+ *     the b.xout partials of the instance's slots, one per tile (the others
+ *     zeroed; the task's tile count must not exceed the instance's nslot).
+ *     The prolongation + Jacobi steps of the same cycle (:420-443), with a
+ *     second sweep after. */
+int hfm_mg_smooth_restrict(const hfm_mg_task* tasks, const uint16_t* blk_task, int64_t nblocks,
+                           const hfm_mg_scalars* sc, int store, float omega, void* stream);
+int hfm_mg_smooth2(const hfm_mg_task* tasks, const uint16_t* blk_task, int64_t nblocks,
+                   const hfm_mg_scalars* sc, int store, int out_store, float omega, float alpha,
+                   double* dots, void* stream);
+
+/* Tile geometry of the fused pass pairs for a level of extents (D, H, W):
+ * rows and planes per tile and the tile count.  Returns HFM_EINVAL (with
+ * hfm_last_error) when the level cannot be tiled (W % 8 != 0, or W too wide
+ * for a 512-thread block to hold four rows). */
+int hfm_mg_tile(int64_t D, int64_t H, int64_t W, int32_t* tile_rows, int32_t* tile_planes,
+                int64_t* ntiles);
 
 /* Coarsest solve e = Ainv b (Ainv FP64 dense, N = D*H*W <= 64): one block
  * per task.  Slots: b, xout = e, ec = Ainv. */

@@ -169,3 +169,51 @@ def test_stop_between_direction_and_update_applies_the_pending_update(dev):
     assert int(info["status"][0]) == 2 and int(info["iter"][0]) == 1
     assert float(x_break.abs().max()) > 0
     assert bool((x_break == one.solution(0)).all())
+
+
+def _buffer(inst, ptr):
+    for t in [*inst.X, *inst.Y, *inst.B, inst.Z]:
+        if t.data_ptr() == int(ptr):
+            return t
+    raise KeyError(ptr)
+
+
+@pytest.mark.parametrize("storage,outer", [("bf16", None), ("f32", None), ("bf16", "f32")])
+def test_fused_pass_pairs_equal_separate_launches(dev, storage, outer):
+    """One V-cycle with the fused pass pairs (smooth + restrict, correct +
+    smooth) equals the separate launches bitwise: tiles spanning the plane,
+    tiles of row blocks (W = 128, 96), odd extents, several plane ranges, and
+    an instance that cannot be tiled (W = 36) in the same batch.  The r.z
+    partials are summed per tile instead of per chunk: equal to rounding."""
+    import torch
+    from hfm import _lib as L
+    specs = [((33, 130, 128), 41, 1.0, 2), ((20, 13, 16), 42, 1.5, 1),
+             ((70, 45, 96), 43, 0.7, 3), ((9, 7, 36), 44, 1.0, 1), ((64, 64, 48), 45, 2.0, 2)]
+    print(f"fused pass pairs vs separate: storage={storage} outer={outer}", flush=True)
+    a, _ = _batch(dev, specs, storage, outer=outer, fuse_levels=8,   # every level that tiles
+                  fuse_min_tiles=0)
+    b, _ = _batch(dev, specs, storage, outer=outer, fuse=False)
+    fused_levels = {op[5] for op in a.plan.ops_down + [o for u in a.plan.ops_up for o in u]
+                    if op[0] in ("smooth_restrict", "smooth2")}
+    assert {0, 1, 2} <= fused_levels
+    s = L.stream_handle(dev)
+    for mg in (a, b):
+        mg.restart()
+        mg.pool.dots.fill_(float("nan"))   # every slot the cycle's dot covers is rewritten
+        mg.plan.precondition(s)
+    torch.cuda.synchronize()
+    for ia, ib in zip(a.insts, b.insts):
+        za = _buffer(ia, ia._tpl[("dir", 0)]["b"])
+        zb = _buffer(ib, ib._tpl[("dir", 0)]["b"])
+        assert torch.equal(za, zb), ia.shape
+        for lev in range(1, len(ia.levels)):
+            assert torch.equal(ia.B[lev], ib.B[lev]), (ia.shape, lev)
+        da = a.pool.dots[ia.dots_off:ia.dots_off + ia.nblk]
+        db = b.pool.dots[ib.dots_off:ib.dots_off + ib.nblk]
+        assert bool(torch.isfinite(da).all())
+        sa, sb = float(da.sum()), float(db.sum())
+        assert abs(sa - sb) <= 1e-12 * abs(sb), (ia.shape, sa, sb)
+    ia, ib = a.solve(), b.solve()
+    assert list(ia["iter"]) == list(ib["iter"]) and all(int(v) == 1 for v in ia["status"])
+    for i in range(len(specs)):
+        assert rel(a.solution(i).cpu().numpy(), b.solution(i).cpu().numpy()) < 1e-6

@@ -84,9 +84,13 @@ def test_fast_division_magic_is_exact():
             assert q == int(n) // d, (n, d)
 
 
+def bt_of(lay):
+    return lay.blk_task.numpy().view(np.uint16)
+
+
 def _owned_by(inst, ptr, nbytes):
     bufs = [*inst.X, *inst.Y, *inst.B, *inst.P, *inst.face32, *inst.faceT, inst.x64, inst.r64,
-            inst.b32, inst.sigma, inst.ainv]
+            inst.b32, inst.sigma, inst.ainv, inst.Z]
     return any(b.data_ptr() <= ptr and ptr + nbytes <= b.data_ptr() + b.numel() * b.element_size()
                for b in bufs)
 
@@ -97,7 +101,7 @@ def test_work_tables_cover_every_cell_once(tuned):
     from hfm import _lib as L
     from hfm.mg import ALIGN, CHUNK, MGBatch
     shapes = [(32, 32, 32), (48, 48, 48), (13, 20, 7), (96, 40, 33)]
-    mg = MGBatch(shapes, storage="bf16", device="cpu")
+    mg = MGBatch(shapes, storage="bf16", device="cpu", fuse_min_tiles=0)
     if tuned:   # launch shapes as the tuner may set them
         blocks = {("sweep", "plain", 0): 512, ("sweep", "correct", 0): 128,
                   ("restrict", 0): 128, ("restrict", 1): 512, ("sweep", "zero", 1): 512}
@@ -129,11 +133,40 @@ def test_work_tables_cover_every_cell_once(tuned):
         assert a1 <= b0
     assert dot_ranges[-1][1] <= mg.pool.dots.numel()
     ops = plan.ops_down + [op for ups in plan.ops_up for op in ups]
+    assert {"smooth_restrict", "smooth2"} <= {op[0] for op in ops}
     for kind, mode, tt, lay, dot, lev, block in ops:
+        assert lay.blk_task.numel() == lay.nblocks
+        if kind in ("smooth_restrict", "smooth2"):
+            # tiles numbered contiguously over the tasks, padded to 8 with 0xFFFF
+            recs = tt.numpy().view(L.TASK_DTYPE)
+            total = int(recs["nblk"].sum())
+            assert lay.nblocks % 8 == 0 and total <= lay.nblocks < total + 8
+            assert np.all(bt_of(lay)[total:] == 0xFFFF)
+            assert list(recs["blk0"]) == list(np.concatenate([[0], np.cumsum(recs["nblk"])[:-1]]))
+            assert block == 512
+            for t in tt.numpy().view(L.TASK_DTYPE):
+                inst = by_slot[int(t["inst"])]
+                shape = inst.levels[lev]
+                assert shape == (t["D"], t["H"], t["W"])
+                rows, planes, ntile = L.tile(shape)
+                assert (t["tile_rows"], t["tile_planes"]) == (rows, planes)
+                assert t["nblk"] == ntile
+                assert np.all(bt_of(lay)[t["blk0"]:t["blk0"] + t["nblk"]] == bt_of(lay)[t["blk0"]])
+                N = int(np.prod(shape))
+                Nc = int(t["Dc"]) * int(t["Hc"]) * int(t["Wc"])
+                assert (t["Dc"], t["Hc"], t["Wc"]) == inst.levels[lev + 1]
+                for f in ("xin", "b", "xout"):
+                    assert _owned_by(inst, int(t[f]), N * 2)
+                # the first pass's input is never overwritten by the pair
+                assert t["xin"] != t["xout"] and t["b"] != t["xout"]
+                coarse = "rlow" if kind == "smooth_restrict" else "ec"
+                assert _owned_by(inst, int(t[coarse]), Nc * 2)
+                if dot:
+                    assert lev == 0 and t["nblk"] <= inst.nblk   # one dot slot per block
+            continue
         key = ("restrict", lev) if kind == "restrict" else \
             ("sweep", {0: "zero", 1: "plain", 2: "correct"}[mode], lev)
         assert block == (256 if dot or not tuned else blocks.get(key, 256))
-        assert lay.blk_task.numel() == lay.nblocks
         for t in tt.numpy().view(L.TASK_DTYPE):
             inst = by_slot[int(t["inst"])]
             assert inst.levels[lev] == (t["D"], t["H"], t["W"])
@@ -187,10 +220,13 @@ def test_coarse_cycle_chains_cover_the_tail_of_every_hierarchy(coarse_cells):
                 assert t["ec"] == inst.B[l + 1].data_ptr()
             else:
                 assert t["ec"] == inst.ainv.data_ptr()
-        # the per-level launches stop at the chain: none touches a level >= lc
+        # the per-level launches stop at the chain: none touches a level >= lc,
+        # every level below it has launches
+        touched = set()
         for kind, mode, tt, lay, dot, lev, block in ops:
-            slots = set(int(x) for x in tt.numpy().view(L.TASK_DTYPE)["inst"])
-            assert (inst.slot in slots) == (lev < lc), (kind, lev, lc)
+            if inst.slot in set(int(x) for x in tt.numpy().view(L.TASK_DTYPE)["inst"]):
+                touched.add(lev)
+        assert touched == set(range(lc)), (touched, lc)
 
 
 def test_plan_rejects_bad_shapes():
@@ -280,3 +316,89 @@ def test_nonuniform_coefficients():
     assert mats["xp1"][0, 0, 5] == np.float32(100 / 1.04 ** 2)
     assert mats["xp1"][0, 0, 7] == np.float32(100 / 1.06 ** 2)   # boundary keeps its own
     assert mats["zp1"][1, 0, 0] == np.float32(100 / 1.04 ** 2)
+
+
+def _symbolic_cycle(mg, plan, inst):
+    """Run ``plan``'s V-cycle symbolically for one instance: each launch reads
+    the expressions held by its input buffers and writes a new expression;
+    returns the expression the direction kernel reads as z."""
+    from hfm import _lib as L
+    from hfm.mg import chain_start
+    val = {inst.B[0].data_ptr(): "r"}
+    tpl = inst._tpl
+    val[int(tpl[("init",)]["xout"])] = "Z0(r)"     # first sweep, fused into the update
+
+    def rd(ptr):
+        assert ptr in val, "read of a buffer no launch wrote"
+        return val[ptr]
+
+    def rec(tt):
+        for t in tt.numpy().view(L.TASK_DTYPE):
+            if int(t["inst"]) == inst.slot:
+                return t
+        return None
+
+    def run(op):
+        kind, mode, tt, _, _, lev, _ = op
+        t = rec(tt)
+        if t is None:
+            return
+        x, b = int(t["xin"]), int(t["b"])
+        if kind in ("smooth_restrict", "smooth2"):
+            side = int(t["rlow"] if kind == "smooth_restrict" else t["ec"])
+            assert len({x, b, side, int(t["xout"])}) == 4, "a fused pair overwrites its input"
+        if kind == "sweep" and mode == 0:
+            val[int(t["xout"])] = f"Z{lev}({rd(b)})"
+        elif kind == "sweep" and mode == 1:
+            val[int(t["xout"])] = f"P{lev}({rd(x)},{rd(b)})"
+        elif kind == "sweep":
+            val[int(t["xout"])] = f"C{lev}({rd(x)},{rd(int(t['ec']))},{rd(b)})"
+        elif kind == "restrict":
+            val[int(t["xout"])] = f"R{lev}({rd(x)},{rd(b)})"
+        elif kind == "smooth_restrict":
+            y = f"P{lev}({rd(x)},{rd(b)})"
+            val[int(t["xout"])] = y
+            val[int(t["rlow"])] = f"R{lev}({y},{rd(b)})"
+        else:   # smooth2
+            y = f"C{lev}({rd(x)},{rd(int(t['ec']))},{rd(b)})"
+            val[int(t["xout"])] = f"P{lev}({y},{rd(b)})"
+
+    for op in plan.ops_down:
+        run(op)
+    p = mg.params
+    lc = chain_start(inst.levels, p.coarse_cells)
+    last = len(inst.levels) - 1
+    out = inst.X[lc] if lc == last or (p.nu1 + p.nu2 - 1) % 2 == 0 else inst.Y[lc]
+    val[out.data_ptr()] = f"V{lc}({rd(inst.B[lc].data_ptr())})"
+    for ups in plan.ops_up:
+        for op in ups:
+            run(op)
+    return rd(int(tpl[("dir", 0)]["b"]))
+
+
+@pytest.mark.parametrize("nu", [(2, 2), (1, 2), (2, 3), (3, 1), (2, 1)])
+@pytest.mark.parametrize("coarse_cells,fuse_levels,min_tiles",
+                         [(64, 8, 0), (4096, 8, 0), (64, 1, 0), (64, 8, 10 ** 9)])
+def test_fused_plan_computes_the_same_cycle(nu, coarse_cells, fuse_levels, min_tiles):
+    """With fused pass pairs every instance's V-cycle is the same composition
+    of passes as with separate launches, and no fused launch writes a buffer
+    it reads."""
+    pytest.importorskip("torch")
+    from hfm.mg import MGBatch
+    shapes = [(32, 32, 32), (33, 130, 128), (13, 20, 7), (20, 13, 16), (40, 40, 36), (64, 48, 96)]
+    kw = dict(storage="bf16", device="cpu", nu1=nu[0], nu2=nu[1], coarse_cells=coarse_cells)
+    a = MGBatch(shapes, fuse_levels=fuse_levels, fuse_min_tiles=min_tiles, **kw)
+    b = MGBatch(shapes, fuse=False, **kw)
+    fused = [(op[0], op[5]) for op in a.plan.ops_down + [o for u in a.plan.ops_up for o in u]
+             if op[0] in ("smooth_restrict", "smooth2")]
+    kinds = {k for k, _ in fused}
+    assert ("smooth_restrict" in kinds) == (nu[0] >= 2)
+    assert ("smooth2" in kinds) == (nu[1] >= 2)
+    # too few tiles: the fine level runs unfused (with records that fuse coarser
+    # levels, and always when nu2 > 2, whose records fix the fine buffers too)
+    assert any(lev == 0 for _, lev in fused) == (min_tiles == 0 or nu[1] > 2)
+    assert not {"smooth_restrict", "smooth2"} & {
+        op[0] for op in b.plan.ops_down + [o for u in b.plan.ops_up for o in u]}
+    for ia, ib in zip(a.insts, b.insts):
+        za, zb = _symbolic_cycle(a, a.plan, ia), _symbolic_cycle(b, b.plan, ib)
+        assert za == zb, (ia.shape, za, zb)
