@@ -599,8 +599,9 @@ __global__ __launch_bounds__(B) void k_resid_restrict(const hfm_mg_task* __restr
 // kk - 1, which needs pass 1 on planes kk - 2 .. kk.  A thread owns a run of
 // kPairRun x-cells of one tile row.  The plane's pass-1 input, its south
 // faces and pass 1's result go through LDS (in-plane neighbours); z-neighbours
-// stay in registers, and each plane's rows are loaded two planes ahead.  The
-// halo rows and planes are computed by pass 1 only.
+// stay in registers, and each plane's rows are loaded a plane ahead (a
+// deeper prefetch costs registers, and the walk is bound by its instructions
+// more than by latency).  The halo rows and planes are computed by pass 1 only.
 //   SR (!S2): pass 1 PLAIN sweep xin -> xout, pass 2 the residual b - A xout
 //     summed over each coarse cell's (up to 2x2x2) children / 8 -> rlow.
 //   S2:       pass 1 CORRECT sweep (xin + alpha P ec), pass 2 PLAIN sweep -> xout
@@ -757,13 +758,15 @@ __device__ __forceinline__ void stencil_run(const Raw4<T>& cx, float ef, const R
   }
 }
 
-// Loads of one plane kk for pass 1, issued two planes ahead of their use.
+// Loads of one plane kk for pass 1, issued a plane ahead of their use.
 template <typename T, bool CORR>
 struct PlaneIn {
   XRun<T, CORR> xn;    // pass-1 input on plane kk + 1 (own run)
   XRun<T, CORR> edge;  // pass-1 input of the row past the tile's edge (edge rows only)
-  Raw4<T> cx, cy, cu, b, cn;  // faces of plane kk; cn: a north row not in LDS
+  Raw4<T> cx, cy, cu, b, cn;  // faces of plane kk; cn: the north faces (from memory
+                              // for edge and boundary rows, else from LDS)
   float ef;
+  Raw4<T> y;                  // pass 1's result on plane kk
 };
 
 // Sum of v over a kPairBlock-thread block (valid in thread 0).
@@ -854,40 +857,42 @@ __global__ __launch_bounds__(kPairBlock, sizeof(T) == 2 ? 4 : 2) void k_smooth_p
       pi.cn.load(cy + n + W);
   };
 
-  XRun<T, S2> xm, x0;       // pass-1 input on planes kk-1, kk
-  Raw4<T> cd;               // down faces of plane kk
-  PlaneIn<T, S2> cur, nxt;  // loads of planes kk, kk + 1
-  // pass-2 state: faces, b and pass-1 results of plane kk-1 (kp), pass 1 on kp-1
-  Raw4<T> pcx, pcy, pcn, pcd, pcu, pb, ym2, ym1;
-  float pef = 0.0f;
+  // The walk keeps three plane records in rotation (previous, current, the
+  // one being loaded); the step is written once and unrolled three times so
+  // that the rotation costs no register moves.
+  PlaneIn<T, S2> pa, pb, pc;
   float ev0[N];             // an even row's residuals on an even plane
 #pragma unroll
   for (int m = 0; m < N; ++m) ev0[m] = 0.0f;
+  pa.y.set(ev0);  // zeros
+  pc.y.set(ev0);
   if (live) {
-    if (ks > 0) xm.load(xin, ec, cell(ks - 1), ks - 1, jc, i0, Hc, Wc);
-    x0.load(xin, ec, cell(ks), ks, jc, i0, Hc, Wc);
-    cd.load(cz + cell(ks));
-    load_plane(cur, ks);
-    if (ks + 1 < D) load_plane(nxt, ks + 1);
+    // pa stands for plane ks - 1 (its x-row is x(ks), its up faces cz(ks)),
+    // pc for ks - 2 (its x-row is x(ks - 1)); pb is plane ks
+    if (ks > 0) pc.xn.load(xin, ec, cell(ks - 1), ks - 1, jc, i0, Hc, Wc);
+    pa.xn.load(xin, ec, cell(ks), ks, jc, i0, Hc, Wc);
+    pa.cu.load(cz + cell(ks));
+    load_plane(pb, ks);
   }
   double acc = 0.0;
 
-  for (int kk = ks; kk <= k1; ++kk) {
+  auto step = [&](int kk, PlaneIn<T, S2>& prv, PlaneIn<T, S2>& cur, PlaneIn<T, S2>& spare) {
     const bool p1 = kk < D;  // pass 1 runs on plane kk (uniform)
     const int buf = kk & 1;
-    PlaneIn<T, S2> nx2;
-    if (live && kk + 2 < D && kk + 2 <= k1) load_plane(nx2, kk + 2);
+    // what the pass pair still needs of plane kk - 2 before its record is reused
+    const XRun<T, S2> xm = spare.xn;   // pass-1 input on plane kk - 1
+    const Raw4<T> pcd = spare.cu;      // down faces of plane kk - 1
+    const Raw4<T> ym2 = spare.y;       // pass 1 on plane kk - 2
+    if (live && kk + 1 < D && kk + 1 <= k1) load_plane(spare, kk + 1);
     float xc[N];
     if (p1 && live) {
-      x0.get(xc, alpha);
+      prv.xn.get(xc, alpha);
       lds_st4(&lx[buf][li], xc);
       cur.cy.lds_store(&lc[buf][li]);
     }
     __syncthreads();
-    Raw4<T> yk, cn;
     if (p1 && live) {
-      if (north && !edge_n) cn.lds_load(&lc[buf][li + W]);
-      else cn = cur.cn;
+      if (north && !edge_n) cur.cn.lds_load(&lc[buf][li + W]);
       float vs[N], vn[N], vd[N], vu[N], d[N], off[N];
       if (r > 0) lds_ld4(&lx[buf][li - W], vs);
       else cur.edge.get(vs, alpha);
@@ -897,17 +902,18 @@ __global__ __launch_bounds__(kPairBlock, sizeof(T) == 2 ? 4 : 2) void k_smooth_p
       cur.xn.get(vu, alpha);
       const float wv = west ? lx[buf][li - 1] : 0.0f;
       const float ev = east ? lx[buf][li + N] : 0.0f;
-      stencil_run<T>(cur.cx, cur.ef, cur.cy, cn, cd, cur.cu, xc, wv, ev, vs, vn, vd, vu, south,
-                     north, kk > 0, kk + 1 < D, d, off);
+      stencil_run<T>(cur.cx, cur.ef, cur.cy, cur.cn, prv.cu, cur.cu, xc, wv, ev, vs, vn, vd, vu,
+                     south, north, kk > 0, kk + 1 < D, d, off);
       float bv[N], y[N];
       cur.b.get(bv);
 #pragma unroll
       for (int m = 0; m < N; ++m) y[m] = round_to<T>(jacobi_val(xc[m], d[m], bv[m], off[m], omega));
-      yk.set(y);
+      cur.y.set(y);
       if (!S2 && outrow && kk >= k0 && kk < k1) st4_nt(gout<T>(t.xout) + cell(kk), y);
-      yk.lds_store(&ly[buf][li]);
+      cur.y.lds_store(&ly[buf][li]);
     } else {
-      yk.set(ev0);  // zeros: plane kk is past the last (its pass-1 value is unused)
+      const float zero[N] = {};
+      cur.y.set(zero);  // plane kk is past the last (its pass-1 value is unused)
     }
     if (kk - 1 >= k0) {  // pass 2 on plane kp (uniform)
       const int kp = kk - 1;
@@ -915,16 +921,16 @@ __global__ __launch_bounds__(kPairBlock, sizeof(T) == 2 ? 4 : 2) void k_smooth_p
       float e[N];
       if (outrow) {
         float yc[N], vs[N], vn[N], vd[N], vu[N], d[N], off[N], bv[N];
-        ym1.get(yc);
+        prv.y.get(yc);
         ym2.get(vd);
-        yk.get(vu);
+        cur.y.get(vu);
         lds_ld4(&ly[pb2][south ? li - W : li], vs);
         lds_ld4(&ly[pb2][north ? li + W : li], vn);
         const float wv = west ? lds_ld1(&ly[pb2][li - 1]) : 0.0f;
         const float ev = east ? lds_ld1(&ly[pb2][li + N]) : 0.0f;
-        stencil_run<T>(pcx, pef, pcy, pcn, pcd, pcu, yc, wv, ev, vs, vn, vd, vu, south, north,
-                       kp > 0, kp + 1 < D, d, off);
-        pb.get(bv);
+        stencil_run<T>(prv.cx, prv.ef, prv.cy, prv.cn, pcd, prv.cu, yc, wv, ev, vs, vn, vd, vu,
+                       south, north, kp > 0, kp + 1 < D, d, off);
+        prv.b.get(bv);
         if constexpr (S2) {
           float o[N];
 #pragma unroll
@@ -993,21 +999,11 @@ __global__ __launch_bounds__(kPairBlock, sizeof(T) == 2 ? 4 : 2) void k_smooth_p
         }
       }
     }
-    // advance the walk one plane
-    pcx = cur.cx;
-    pcy = cur.cy;
-    pcn = cn;
-    pcd = cd;
-    pcu = cur.cu;
-    pef = cur.ef;
-    pb = cur.b;
-    ym2 = ym1;
-    ym1 = yk;
-    xm = x0;
-    x0 = cur.xn;
-    cd = cur.cu;
-    cur = nxt;
-    nxt = nx2;
+  };
+  for (int kk = ks; kk <= k1; kk += 3) {
+    step(kk, pa, pb, pc);
+    if (kk + 1 <= k1) step(kk + 1, pb, pc, pa);
+    if (kk + 2 <= k1) step(kk + 2, pc, pa, pb);
   }
   if (DOT) {
     const double s = pair_block_sum(acc);
