@@ -712,7 +712,8 @@ struct XRun {
 // ef: the east face of the run's last cell, cy / cn: south / north, cd / cu:
 // down / up) and neighbour values, in the order and with the rounding of
 // RunCoef::diag and offdiag_sum; each face row is unpacked where it is used.
-template <typename T>
+// CALC_D = false: d holds the diagonal already (the same faces' sum).
+template <typename T, bool CALC_D = true>
 __device__ __forceinline__ void stencil_run(const Raw4<T>& cx, float ef, const Raw4<T>& cy,
                                             const Raw4<T>& cn, const Raw4<T>& cd,
                                             const Raw4<T>& cu, const float xc[4], float wv,
@@ -728,32 +729,32 @@ __device__ __forceinline__ void stencil_run(const Raw4<T>& cx, float ef, const R
   ce[N - 1] = ef;
 #pragma unroll
   for (int m = 0; m < N; ++m) {
-    d[m] = c[m] + ce[m];
+    if (CALC_D) d[m] = c[m] + ce[m];
     off[m] = c[m] * (m ? xc[m - 1] : wv);
     off[m] = fma_a(ce[m], m < N - 1 ? xc[m + 1] : ev, off[m]);
   }
   cy.get(c);
 #pragma unroll
   for (int m = 0; m < N; ++m) {
-    d[m] += c[m];
+    if (CALC_D) d[m] += c[m];
     off[m] = fma_a(c[m], south ? vs[m] : 0.0f, off[m]);
   }
   cn.get(c);
 #pragma unroll
   for (int m = 0; m < N; ++m) {
-    d[m] += c[m];
+    if (CALC_D) d[m] += c[m];
     off[m] = fma_a(c[m], north ? vn[m] : 0.0f, off[m]);
   }
   cd.get(c);
 #pragma unroll
   for (int m = 0; m < N; ++m) {
-    d[m] += c[m];
+    if (CALC_D) d[m] += c[m];
     off[m] = fma_a(c[m], down ? vd[m] : 0.0f, off[m]);
   }
   cu.get(c);
 #pragma unroll
   for (int m = 0; m < N; ++m) {
-    d[m] += c[m];
+    if (CALC_D) d[m] += c[m];
     off[m] = fma_a(c[m], up ? vu[m] : 0.0f, off[m]);
   }
 }
@@ -767,6 +768,8 @@ struct PlaneIn {
                               // for edge and boundary rows, else from LDS)
   float ef;
   Raw4<T> y;                  // pass 1's result on plane kk
+  float d[kPairRun];          // the diagonal on plane kk (kept for pass 2 where
+                              // registers allow: smooth + restrict)
 };
 
 // Sum of v over a kPairBlock-thread block (valid in thread 0).
@@ -904,6 +907,10 @@ __global__ __launch_bounds__(kPairBlock, sizeof(T) == 2 ? 4 : 2) void k_smooth_p
       const float ev = east ? lx[buf][li + N] : 0.0f;
       stencil_run<T>(cur.cx, cur.ef, cur.cy, cur.cn, prv.cu, cur.cu, xc, wv, ev, vs, vn, vd, vu,
                      south, north, kk > 0, kk + 1 < D, d, off);
+      if constexpr (!S2) {
+#pragma unroll
+        for (int m = 0; m < N; ++m) cur.d[m] = d[m];
+      }
       float bv[N], y[N];
       cur.b.get(bv);
 #pragma unroll
@@ -928,8 +935,12 @@ __global__ __launch_bounds__(kPairBlock, sizeof(T) == 2 ? 4 : 2) void k_smooth_p
         lds_ld4(&ly[pb2][north ? li + W : li], vn);
         const float wv = west ? lds_ld1(&ly[pb2][li - 1]) : 0.0f;
         const float ev = east ? lds_ld1(&ly[pb2][li + N]) : 0.0f;
-        stencil_run<T>(prv.cx, prv.ef, prv.cy, prv.cn, pcd, prv.cu, yc, wv, ev, vs, vn, vd, vu,
-                       south, north, kp > 0, kp + 1 < D, d, off);
+        if constexpr (!S2) {
+#pragma unroll
+          for (int m = 0; m < N; ++m) d[m] = prv.d[m];
+        }
+        stencil_run<T, S2>(prv.cx, prv.ef, prv.cy, prv.cn, pcd, prv.cu, yc, wv, ev, vs, vn, vd,
+                           vu, south, north, kp > 0, kp + 1 < D, d, off);
         prv.b.get(bv);
         if constexpr (S2) {
           float o[N];
