@@ -137,9 +137,14 @@ class Params:
     # separate; L1 0.23 + 0.27 vs 0.23 + 0.21)
     fuse: bool = True
     fuse_levels: int = 1
-    # a plan runs the fine level's pairs fused only with at least this many
-    # tiles over its instances (a few tiles leave most CUs idle: one 64 x 256
-    # x 256 instance has 86); its buffers are the same either way
+    # an MGBatch fuses the fine level only with at least this many tiles over
+    # its instances (a few tiles leave most CUs idle: one 64 x 256 x 256
+    # instance has 86).  The choice is the batch's (its pool's), not each
+    # plan's: every plan of the batch (the subsets as instances finish, the
+    # stream groups) then runs the same kernels and gives bitwise the same
+    # results.  (The fused post pair sums the r.z partials per tile, the
+    # separate sweep per block of cells: an instance in a batch that fuses
+    # and alone in one that does not agrees to rounding in that sum.)
     fuse_min_tiles: int = 256
 
     def __post_init__(self):
@@ -197,6 +202,9 @@ class Pool:
             L.load()
         self.store_id = L.STORE[params.storage]
         self.outer_id = L.STORE[params.outer]
+        # fine-level pass pairs fused for every plan of this pool (the batch or
+        # server that owns the pool may turn it off: see Params.fuse_min_tiles)
+        self.fuse_fine = params.fuse
         self.capacity = capacity
         self.sc = torch.zeros(capacity * L.SCALARS_DTYPE.itemsize, dtype=torch.uint8,
                               device=self.device)
@@ -239,9 +247,13 @@ class Pool:
         idx = torch.tensor([i.slot for i in insts], device=self.device)
         self.sc.view(self.capacity, -1).index_copy_(0, idx, rows)
 
-    def finalize(self, kind: str, s):
-        L.call("hfm_fcg_finalize", self.sc.data_ptr(), self.capacity, self.dots.data_ptr(),
-               L.FIN[kind], s)
+    def finalize(self, kind: str, s, slots: Optional[Tuple[int, int]] = None):
+        """Finalize step ``kind`` for the slots [lo, hi) (default: all): a plan
+        whose launches run beside another plan's on a second stream touches
+        only its own slots."""
+        lo, hi = slots if slots is not None else (0, self.capacity)
+        L.call("hfm_fcg_finalize", self.sc.data_ptr() + lo * L.SCALARS_DTYPE.itemsize, hi - lo,
+               self.dots.data_ptr(), L.FIN[kind], s)
 
 
 class Instance:
@@ -440,6 +452,8 @@ def fused_levels(inst: Instance, p: Params, lc: int):
     if not p.fuse:
         return sr, s2
     for lev in range(min(lc, len(inst.levels) - 1, p.fuse_levels)):
+        if lev == 0 and not inst.pool.fuse_fine:
+            continue
         tl = L.tile(inst.levels[lev])
         if tl is None:
             continue
@@ -606,18 +620,23 @@ class Plan:
     when the set changes costs a few numpy concatenations and uploads.
     """
 
-    def __init__(self, pool: Pool, insts: Sequence[Instance]):
+    def __init__(self, pool: Pool, insts: Sequence[Instance],
+                 slots: Optional[Tuple[int, int]] = None):
         if not insts:
             raise ValueError("empty plan")
         self.pool = pool
         self.insts = list(insts)
+        # finalize range: every slot of the plan's instances (default: the pool)
+        self.slots = slots
+        if slots is not None and not all(slots[0] <= i.slot < slots[1] for i in insts):
+            raise ValueError("plan instances outside its slot range")
         p = pool.params
         dev = pool.device
         lcs = {i.slot: chain_start(i.levels, p.coarse_cells) for i in insts}
         for inst in insts:
             # records depend on where the coarse cycle starts (fused levels
             # end there) and on fusion itself
-            key = (lcs[inst.slot], p.fuse, p.fuse_levels, p.nu1, p.nu2)
+            key = (lcs[inst.slot], p.fuse, p.fuse_levels, pool.fuse_fine, p.nu1, p.nu2)
             if getattr(inst, "_tpl", None) is None or inst._tpl_key != key:
                 inst._tpl = instance_templates(inst, p, lcs[inst.slot])
                 inst._tpl_key = key
@@ -635,16 +654,8 @@ class Plan:
                 return _tile_table(dev, recs, sizes)
             return _table(dev, recs, sizes, chunk)
 
-        # the fine level's records are the same buffers fused or not (z has a
-        # buffer of its own), so a plan with few tiles there runs it unfused;
-        # coarser fused levels (and nu2 > 2) fix their buffers in the records
-        fine_tiles = sum(i._tiles.get(0, 0) for i in insts)
-        self.fuse_fine = fine_tiles >= p.fuse_min_tiles or p.nu2 > 2
-
         def fused(key):
             """Members whose pass pair ``key`` runs fused (fused_levels)."""
-            if key[1] == 0 and not self.fuse_fine:
-                return lambda i: False
             return lambda i: key in i._tpl
 
         blocks = p.blocks or {}
@@ -729,7 +740,7 @@ class Plan:
         L.call("hfm_fcg_init_sweep", self.init_tasks.data_ptr(), fl.blk_task.data_ptr(),
                fl.nblocks, pool.sc.data_ptr(), pool.store_id, pool.params.omega,
                pool.dots.data_ptr(), s)
-        pool.finalize("init", s)
+        pool.finalize("init", s, self.slots)
 
     def precondition(self, s):
         """z = V(rlow): one V-cycle over every instance of the plan."""
@@ -767,11 +778,11 @@ class Plan:
         """One FCG iteration for every active instance of the plan; instances
         that stop get the pending half of their deferred solution update."""
         self.precondition(s)
-        self.pool.finalize("beta", s)
+        self.pool.finalize("beta", s, self.slots)
         self.launch_dir(parity, s)
-        self.pool.finalize("alpha", s)
+        self.pool.finalize("alpha", s, self.slots)
         self.launch_update(parity, s)
-        self.pool.finalize("check", s)
+        self.pool.finalize("check", s, self.slots)
         self.launch_flush(parity, s)
 
     def graph(self, m: int):
@@ -805,13 +816,14 @@ class MGBatch:
                  iters_per_graph: int = 4, use_graph: bool = True, shrink: float = 0.5,
                  outer: Optional[str] = None, blocks: Optional[dict] = None,
                  coarse_cells: int = 4096, fuse: bool = True, fuse_levels: int = 1,
-                 fuse_min_tiles: int = 256):
+                 fuse_min_tiles: int = 256, streams: int = 1):
         self.params = Params(storage, tol, maxit, nu1, nu2, omega, alpha, max_coarse_edge, outer,
                              blocks, coarse_cells, fuse, fuse_levels, fuse_min_tiles)
         shapes = [check_shape(s, max_coarse_edge)[0] for s in shapes]
         if not shapes:
             raise ValueError("empty batch")
         self.pool = Pool(len(shapes), sum(fine_blocks(s) for s in shapes), self.params, device)
+        self.pool.fuse_fine = self._fuse_fine(shapes)
         self.device = self.pool.device
         self.plan_only = self.pool.plan_only
         self.storage = storage
@@ -825,6 +837,28 @@ class MGBatch:
         self.shrink = float(shrink)
         self._subplans = {}
         self.last_info = None
+        self.set_streams(streams)
+
+    def set_streams(self, n: int):
+        """Solve the batch as ``n`` (1 or 2) groups of instances, each with its
+        own plan on its own stream.  The groups' launches then run beside each
+        other, so one group's compute-bound V-cycle passes can overlap the
+        other's bandwidth-bound FCG passes; results are bitwise the same."""
+        if n not in (1, 2):
+            raise ValueError("streams must be 1 or 2")
+        self.streams = n
+        self._groups = None
+        if n == 2 and len(self.insts) >= 2:
+            cells = np.cumsum([i.N for i in self.insts])
+            h = int(np.searchsorted(cells, cells[-1] / 2)) + 1
+            h = min(max(h, 1), len(self.insts) - 1)
+            groups = [self.insts[:h], self.insts[h:]]
+            self._groups = [(g, (min(i.slot for i in g), max(i.slot for i in g) + 1))
+                            for g in groups]
+            lo0, hi0 = self._groups[0][1]
+            lo1, hi1 = self._groups[1][1]
+            if not (hi0 <= lo1 or hi1 <= lo0):     # slots of the groups interleave
+                self._groups = None
 
     def _need_gpu(self):
         if self.plan_only:
@@ -876,13 +910,25 @@ class MGBatch:
         self.params.coarse_cells = int(n)
         self._replan()
 
-    def set_fuse(self, fuse: bool, levels: Optional[int] = None):
+    def _fuse_fine(self, shapes) -> bool:
+        p = self.params
+        if not p.fuse:
+            return False
+        tiles = [L.tile(level_shapes(s, p.max_coarse_edge)[0]) for s in shapes]
+        return sum(t[2] for t in tiles if t is not None) >= p.fuse_min_tiles
+
+    def set_fuse(self, fuse: bool, levels: Optional[int] = None,
+                 min_tiles: Optional[int] = None):
         """Run the fused pass pairs (default) or every pass as its own launch,
-        on the finest ``levels`` levels; rebuilds the plans.  All give bitwise
-        the same V-cycle."""
+        on the finest ``levels`` levels, the fine one when the batch has at
+        least ``min_tiles`` tiles there; rebuilds the plans.  All give the same
+        V-cycle (the r.z sum in another order when the fine pair changes)."""
         self.params.fuse = bool(fuse)
         if levels is not None:
             self.params.fuse_levels = int(levels)
+        if min_tiles is not None:
+            self.params.fuse_min_tiles = int(min_tiles)
+        self.pool.fuse_fine = self._fuse_fine(self.shapes)
         for inst in self.insts:
             inst._tpl = None
         self._replan()
@@ -891,13 +937,53 @@ class MGBatch:
         self.plan = Plan(self.pool, self.insts)
         self._subplans = {}
 
-    def _subplan(self, live):
-        key = tuple(i.slot for i in live)
+    def _subplan(self, live, slots=None):
+        key = (tuple(i.slot for i in live), slots)
         plan = self._subplans.get(key)
         if plan is None:
-            plan = Plan(self.pool, live)
+            plan = Plan(self.pool, live, slots)
             self._subplans[key] = plan
         return plan
+
+    def _solve_groups(self):
+        """``solve`` with the batch split in two groups on two streams."""
+        s = L.stream_handle(self.device)
+        self.restart()
+        act = self.pool.active_view()
+        cur = torch.cuda.current_stream(self.device)
+        if getattr(self, "_streams", None) is None:
+            self._streams = [torch.cuda.Stream(self.device) for _ in self._groups]
+        plans = [self._subplan(g, r) for g, r in self._groups]
+        its, maxit = 0, self.params.maxit
+        m = self.iters_per_graph + self.iters_per_graph % 2
+        while its < maxit and any(p is not None for p in plans):
+            for plan, st in zip(plans, self._streams):
+                if plan is None:
+                    continue
+                st.wait_stream(cur)
+                with torch.cuda.stream(st):
+                    if self.use_graph:
+                        g, m = plan.graph(m)
+                        g.replay()
+                    else:
+                        hs = L.stream_handle(self.device)
+                        for q in range(m):
+                            plan.iteration(hs, q % 2)
+            for st in self._streams:
+                cur.wait_stream(st)
+            its += m
+            flags = act.cpu().numpy()
+            for q, plan in enumerate(plans):
+                if plan is None:
+                    continue
+                live = [i for i in plan.insts if flags[i.slot]]
+                if not live:
+                    plans[q] = None
+                elif len(live) <= self.shrink * len(plan.insts):
+                    plans[q] = self._subplan(live, self._groups[q][1])
+        info = self.scalars()
+        self.last_info = info
+        return info
 
     def solve(self):
         """Run FCG to ``tol`` (or ``maxit``) for every instance; returns the scalar records.
@@ -908,6 +994,8 @@ class MGBatch:
         continue on a plan of the active ones only (no empty blocks).
         """
         self._need_gpu()
+        if self.streams == 2 and self._groups is not None:
+            return self._solve_groups()
         s = L.stream_handle(self.device)
         self.restart()
         act = self.pool.active_view()

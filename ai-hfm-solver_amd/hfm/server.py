@@ -16,7 +16,9 @@ A retired request's kernels in the epoch in flight exit at once (its slot
 is inactive), so its memory and slot are reused safely by later, stream-
 ordered work.  Every request's arithmetic is independent of the others
 (per-instance kernels and fixed-order reductions), so a result is bitwise
-the same alone and in any mix.
+the same alone and in any mix.  Whether the fine level's pass pairs run
+fused (``Params.fuse``) is the server's choice for every plan it builds, so
+it does not depend on the mix either.
 
 The reference has no serving layer; this is the request loop around the
 multigrid solve of ART/BGS_multigrid_v1.ipynb#c42's operator.

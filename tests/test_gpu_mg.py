@@ -217,3 +217,20 @@ def test_fused_pass_pairs_equal_separate_launches(dev, storage, outer):
     assert list(ia["iter"]) == list(ib["iter"]) and all(int(v) == 1 for v in ia["status"])
     for i in range(len(specs)):
         assert rel(a.solution(i).cpu().numpy(), b.solution(i).cpu().numpy()) < 1e-6
+
+
+def test_two_stream_groups_equal_one_plan_bitwise(dev):
+    """The batch solved as two instance groups on two streams (each group's
+    finalize touching only its own scalar slots) gives bitwise the one-plan
+    solve."""
+    specs = [((32, 32, 32), 51, 1.0, 1), ((48, 40, 64), 52, 1.5, 2), ((20, 12, 36), 53, 0.7, 1),
+             ((40, 48, 32), 54, 2.0, 3), ((24, 24, 24), 55, 1.0, 1)]
+    one, _ = _batch(dev, specs, "bf16")
+    two, _ = _batch(dev, specs, "bf16", streams=2)
+    assert two._groups is not None
+    i1, i2 = one.solve(), two.solve()
+    assert list(i1["iter"]) == list(i2["iter"]) and all(int(v) == 1 for v in i2["status"])
+    for i in range(len(specs)):
+        assert bool((one.solution(i) == two.solution(i)).all()), i
+    i3 = two.solve()        # again, on the cached group plans and graphs
+    assert list(i3["iter"]) == list(i1["iter"])

@@ -394,9 +394,9 @@ def test_fused_plan_computes_the_same_cycle(nu, coarse_cells, fuse_levels, min_t
     kinds = {k for k, _ in fused}
     assert ("smooth_restrict" in kinds) == (nu[0] >= 2)
     assert ("smooth2" in kinds) == (nu[1] >= 2)
-    # too few tiles: the fine level runs unfused (with records that fuse coarser
-    # levels, and always when nu2 > 2, whose records fix the fine buffers too)
-    assert any(lev == 0 for _, lev in fused) == (min_tiles == 0 or nu[1] > 2)
+    # too few tiles in the batch: the fine level runs unfused in every plan
+    assert any(lev == 0 for _, lev in fused) == (min_tiles == 0)
+    assert a.pool.fuse_fine == (min_tiles == 0)
     assert not {"smooth_restrict", "smooth2"} & {
         op[0] for op in b.plan.ops_down + [o for u in b.plan.ops_up for o in u]}
     for ia, ib in zip(a.insts, b.insts):
