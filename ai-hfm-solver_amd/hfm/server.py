@@ -17,8 +17,9 @@ is inactive), so its memory and slot are reused safely by later, stream-
 ordered work.  Every request's arithmetic is independent of the others
 (per-instance kernels and fixed-order reductions), so a result is bitwise
 the same alone and in any mix.  Whether the fine level's pass pairs run
-fused (``Params.fuse``) is the server's choice for every plan it builds, so
-it does not depend on the mix either.
+fused (``Params.fuse``, and a capacity of at least FUSE_MIN_CAPACITY slots)
+is the server's choice for every plan it builds, so it does not depend on
+the mix either.
 
 The reference has no serving layer; this is the request loop around the
 multigrid solve of ART/BGS_multigrid_v1.ipynb#c42's operator.
@@ -65,6 +66,9 @@ class Result:
         return self.t_done - self.t_arrival
 
 
+FUSE_MIN_CAPACITY = 64     # solver slots from which a server fuses the fine level
+
+
 class SolveServer:
     """HBM-budgeted continuous batching of multigrid solves on one GPU."""
 
@@ -73,6 +77,9 @@ class SolveServer:
                  device=None, clock=time.perf_counter):
         self.params = params or Params()
         self.pool = Pool(capacity, dot_slots, self.params, device)
+        # fused fine-level pass pairs only for a server that batches: with a
+        # few residents their tiles leave most CUs idle
+        self.pool.fuse_fine = self.params.fuse and capacity >= FUSE_MIN_CAPACITY
         if self.pool.plan_only:
             raise RuntimeError("SolveServer needs a GPU")
         self.device = self.pool.device

@@ -706,7 +706,7 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=300.0)
     ap.add_argument("--serve-sat-rate", type=float, default=3000.0)
     ap.add_argument("--no-fp32-batch", action="store_true")
-    ap.add_argument("--streams", type=int, default=1, choices=[1, 2],
+    ap.add_argument("--streams", type=int, default=2, choices=[1, 2],
                     help="solve the batch as this many instance groups on their own streams")
     ap.add_argument("--no-fuse", action="store_true",
                     help="every V-cycle pass as its own launch (no fused pass pairs)")

@@ -225,7 +225,7 @@ def test_two_stream_groups_equal_one_plan_bitwise(dev):
     solve."""
     specs = [((32, 32, 32), 51, 1.0, 1), ((48, 40, 64), 52, 1.5, 2), ((20, 12, 36), 53, 0.7, 1),
              ((40, 48, 32), 54, 2.0, 3), ((24, 24, 24), 55, 1.0, 1)]
-    one, _ = _batch(dev, specs, "bf16")
+    one, _ = _batch(dev, specs, "bf16", streams=1)
     two, _ = _batch(dev, specs, "bf16", streams=2)
     assert two._groups is not None
     i1, i2 = one.solve(), two.solve()
