@@ -402,3 +402,37 @@ def test_fused_plan_computes_the_same_cycle(nu, coarse_cells, fuse_levels, min_t
     for ia, ib in zip(a.insts, b.insts):
         za, zb = _symbolic_cycle(a, a.plan, ia), _symbolic_cycle(b, b.plan, ib)
         assert za == zb, (ia.shape, za, zb)
+
+
+@pytest.mark.parametrize("shape", [(32, 32, 32), (48, 96, 48), (33, 130, 128), (5, 300, 512),
+                                   (1, 1, 8), (70, 45, 96), (128, 128, 128), (7, 13, 16)])
+def test_fused_tile_geometry(shape):
+    """hfm_mg_tile: a tile plane (rows plus halo rows) fits a 512-thread block
+    of 4-cell runs, row blocks are even (coarse row pairs stay in one tile),
+    the tiles cover the level, and planes per tile are even and at most 32."""
+    from hfm import _lib as L
+    D, H, W = shape
+    rows, planes, ntiles = L.tile(shape)
+    q = W // 4
+    ny = -(-H // rows)
+    if rows >= H:
+        assert ny == 1 and H * q <= 512
+    else:
+        assert rows % 2 == 0 and (rows + 2) * q <= 512
+    assert planes % 2 == 0 and planes <= 32 + 1 and planes >= min(D, 2)
+    assert ntiles == ny * -(-D // planes)
+    assert (ny - 1) * rows < H <= ny * rows
+
+
+def test_fused_entry_points_reject_bad_tables():
+    from hfm import _lib as L
+    # W % 8 != 0; a row too wide for a tile plus halo rows in one block
+    for shape in [(8, 8, 36), (8, 8, 7), (4, 4, 1024), (4, 4, 2048)]:
+        assert L.tile(shape) is None
+    # tables are checked before anything touches the (here fake) pointers
+    with pytest.raises(L.HfmError, match="multiple of 8"):
+        L.call("hfm_mg_smooth_restrict", 16, 16, 3, 16, 1, 0.5, None)
+    with pytest.raises(L.HfmError, match="multiple of 8"):
+        L.call("hfm_mg_smooth2", 16, 16, 12, 16, 1, 1, 0.5, 1.6, None, None)
+    with pytest.raises(L.HfmError, match="unknown storage"):
+        L.call("hfm_mg_smooth2", 16, 16, 8, 16, 1, 7, 0.5, 1.6, None, None)
