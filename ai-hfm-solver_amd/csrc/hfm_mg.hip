@@ -1517,13 +1517,61 @@ __global__ __launch_bounds__(kBlock) void k_faces_batched(const hfm_mg_task* __r
                                                           float inv_h2) {
   const int g = blockIdx.x;
   const hfm_mg_task& t = tasks[blk_task[g]];
-  const int64_t n = int64_t(g - t.blk0) * kBlock + threadIdx.x;
-  if (n >= int64_t(t.D) * t.H * t.W) return;
+  const int D = t.D, H = t.H, W = t.W;
+  const int64_t N = int64_t(D) * H * W, HW = int64_t(H) * W;
+  const int64_t base = int64_t(g - t.blk0) * kCellChunk;
   float* const o[6] = {static_cast<float*>(t.o[0]), static_cast<float*>(t.o[1]),
                        static_cast<float*>(t.o[2]), static_cast<float*>(t.o[3]),
                        static_cast<float*>(t.o[4]), static_cast<float*>(t.o[5])};
-  faces_cell(n, static_cast<const float*>(t.xin), static_cast<const float*>(t.b),
-             static_cast<const float*>(t.ec), o, t.D, t.H, t.W, inv_h2);
+  const float* sx = static_cast<const float*>(t.xin);
+  const float* sy = static_cast<const float*>(t.b);
+  const float* sz = static_cast<const float*>(t.ec);
+  if (W % kRun == 0) {
+    // a run of 8 cells of one row: 16-byte rows of every array, the
+    // arithmetic of faces_cell per cell
+    const int64_t n0 = base + threadIdx.x * kRun;
+    if (n0 >= N) return;
+    const int i0 = int(n0 % W);
+    const int j = int((n0 / W) % H);
+    const int k = int(n0 / HW);
+    const double s2 = double(inv_h2);
+    auto hm = [](double a, double b) { return 2.0 * a * b / (a + b); };
+    float ax[kRun], ay[kRun], az[kRun], py[kRun], pz[kRun];
+    ld8(gin<float>(sx) + n0, ax);
+    ld8(gin<float>(sy) + n0, ay);
+    ld8(gin<float>(sz) + n0, az);
+    ld8(gin<float>(sy) + (j > 0 ? n0 - W : n0), py);
+    ld8(gin<float>(sz) + (k > 0 ? n0 - HW : n0), pz);
+    const float pw = ldg(gin<float>(sx) + (i0 > 0 ? n0 - 1 : n0));
+    float ox[kRun], oy[kRun], oz[kRun];
+#pragma unroll
+    for (int m = 0; m < kRun; ++m) {
+      const double a = ax[m];
+      ox[m] = float((i0 + m == 0 ? a : hm(a, double(m ? ax[m - 1] : pw))) * s2);
+      oy[m] = float((j == 0 ? double(ay[m]) : hm(double(ay[m]), double(py[m]))) * s2);
+      oz[m] = float((k == 0 ? double(az[m]) : hm(double(az[m]), double(pz[m]))) * s2);
+    }
+    st8(gout<float>(o[0]) + n0, ox);
+    st8(gout<float>(o[1]) + n0, oy);
+    st8(gout<float>(o[2]) + n0, oz);
+    if (i0 + kRun == W) o[3][int64_t(k) * H + j] = float(double(ax[kRun - 1]) * s2);
+    if (j == H - 1) {
+#pragma unroll
+      for (int m = 0; m < kRun; ++m) ay[m] = float(double(ay[m]) * s2);
+      st8(gout<float>(o[4]) + int64_t(k) * W + i0, ay);
+    }
+    if (k == D - 1) {
+#pragma unroll
+      for (int m = 0; m < kRun; ++m) az[m] = float(double(az[m]) * s2);
+      st8(gout<float>(o[5]) + int64_t(j) * W + i0, az);
+    }
+    return;
+  }
+  for (int q = 0; q < kRun; ++q) {
+    const int64_t n = base + q * kBlock + threadIdx.x;
+    if (n >= N) return;
+    faces_cell(n, sx, sy, sz, o, D, H, W, inv_h2);
+  }
 }
 
 // Coarse faces of coarse cell n: (pair over the first listed axis, then the
@@ -1609,13 +1657,26 @@ __global__ __launch_bounds__(kBlock) void k_convert(const float* __restrict__ sr
 }
 
 // Batched: task xin = FP32 source, xout = destination (T), D = element count.
+// A thread converts 8 consecutive elements (16-byte rows when both arrays
+// are 16-byte aligned, else element by element); a block covers kCellChunk.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_convert_batched(const hfm_mg_task* __restrict__ tasks,
                                                             const uint16_t* __restrict__ blk_task) {
   const int g = blockIdx.x;
   const hfm_mg_task& t = tasks[blk_task[g]];
-  const int64_t q = int64_t(g - t.blk0) * kBlock + threadIdx.x;
-  if (q < t.D) static_cast<T*>(t.xout)[q] = from_f<T>(static_cast<const float*>(t.xin)[q]);
+  const int64_t n = t.D;
+  const int64_t q0 = (int64_t(g - t.blk0) * kBlock + threadIdx.x) * kRun;
+  if (q0 >= n) return;
+  const HFM_GLOBAL float* src = gin<float>(t.xin) + q0;
+  HFM_GLOBAL T* dst = gout<T>(t.xout) + q0;
+  if (q0 + kRun <= n && ((reinterpret_cast<uintptr_t>(src) | reinterpret_cast<uintptr_t>(dst)) &
+                         15) == 0) {
+    float v[kRun];
+    ld8_nt(src, v);
+    st8(dst, v);
+  } else {
+    for (int64_t q = 0; q < kRun && q0 + q < n; ++q) put(dst + q, from_f<T>(ldg(src + q)));
+  }
 }
 
 // Dense FP64 inverse of the coarsest face operator by Gauss-Jordan elimination

@@ -563,7 +563,7 @@ class SetupPlan:
             t["xin"], t["b"], t["ec"] = base, base + N8, base + 2 * N8
             t["o"] = inst.faces(0, False)
             recs.append(t)
-        self.faces = _table(dev, recs, [i.N for i in insts], SCHUNK)
+        self.faces = _table(dev, recs, [i.N for i in insts], CHUNK)
         self.galerkin = []
         nlev = max(len(i.levels) for i in insts)
         for lev in range(nlev - 1):
@@ -586,7 +586,7 @@ class SetupPlan:
                     t["xin"], t["xout"] = inst.face32[lev].data_ptr(), inst.faceT[lev].data_ptr()
                     recs.append(t)
                     sizes.append(inst.face32[lev].numel())
-            self.convert = _table(dev, recs, sizes, SCHUNK)
+            self.convert = _table(dev, recs, sizes, CHUNK)   # 8 elements per thread
         recs = []
         for inst in insts:
             lc = len(inst.levels) - 1

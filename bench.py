@@ -706,6 +706,10 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=300.0)
     ap.add_argument("--serve-sat-rate", type=float, default=3000.0)
     ap.add_argument("--no-fp32-batch", action="store_true")
+    ap.add_argument("--iters-per-graph", type=int, default=4,
+                    help="FCG iterations per HIP-graph replay (active flags read between)")
+    ap.add_argument("--shrink", type=float, default=0.5,
+                    help="re-plan over the active instances once at most this fraction remain")
     ap.add_argument("--streams", type=int, default=2, choices=[1, 2],
                     help="solve the batch as this many instance groups on their own streams")
     ap.add_argument("--no-fuse", action="store_true",
@@ -755,8 +759,9 @@ def main():
     shapes = [s[0] for s in specs]
     cells = sum(math.prod(s) for s in shapes)
     log(f"bench: {len(specs)} instances, {cells / 1e6:.1f} M fine cells, storage {args.storage}")
-    mg = MGBatch(shapes, storage=args.storage, tol=1e-8, maxit=300, device=dev, iters_per_graph=4,
-                 outer=args.outer, fuse=not args.no_fuse, streams=args.streams)
+    mg = MGBatch(shapes, storage=args.storage, tol=1e-8, maxit=300, device=dev,
+                 iters_per_graph=args.iters_per_graph, shrink=args.shrink, outer=args.outer,
+                 fuse=not args.no_fuse, streams=args.streams)
     load_batch(mg, specs, dev)
     tuning = None
     if not args.no_tune:

@@ -353,19 +353,20 @@ int hfm_mg_galerkin(const float* const fine[6], float* const coarse[6], int64_t 
                     int64_t W, void* stream);
 int hfm_mg_convert(const float* src, void* dst, int64_t n, int store, void* stream);
 
-/* Batched setup over a work table with 256 cells per block (blk_task as for
- * the cell kernels, tasks need not be padded to 8 blocks):
+/* Batched setup over a work table (blk_task as for the cell kernels, tasks
+ * need not be padded to 8 blocks):
  * faces_batched:    per task xin = sx, b = sy, ec = sz (FP32 cell
  *                   conductivities of extents D/H/W), o[0..5] = output faces;
- *                   blocks cover the cells.
+ *                   a block covers 2048 cells (8 per thread; 16-byte rows when
+ *                   W % 8 == 0, so the arrays must be 16-byte aligned then).
  * galerkin_batched: per task D/H/W = fine extents, c[0..5] = fine FP32 faces,
- *                   o[0..5] = coarse FP32 faces; blocks cover the coarse cells. */
+ *                   o[0..5] = coarse FP32 faces; 256 coarse cells per block. */
 int hfm_mg_faces_batched(const hfm_mg_task* tasks, const uint16_t* blk_task, int64_t nblocks,
                          float inv_h2, void* stream);
 int hfm_mg_galerkin_batched(const hfm_mg_task* tasks, const uint16_t* blk_task, int64_t nblocks,
                             void* stream);
 /* convert_batched:  per task xin = FP32 source, xout = destination of storage
- *                   `store`, D = element count; blocks cover the elements. */
+ *                   `store`, D = element count; a block covers 2048 elements. */
 int hfm_mg_convert_batched(const hfm_mg_task* tasks, const uint16_t* blk_task, int64_t nblocks,
                            int store, void* stream);
 int hfm_mg_coarse_inverse(const hfm_mg_task* tasks, int64_t ntasks, void* stream);

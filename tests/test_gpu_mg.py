@@ -234,3 +234,24 @@ def test_two_stream_groups_equal_one_plan_bitwise(dev):
         assert bool((one.solution(i) == two.solution(i)).all()), i
     i3 = two.solve()        # again, on the cached group plans and graphs
     assert list(i3["iter"]) == list(i1["iter"])
+
+
+def test_batched_setup_kernels_bitwise(dev):
+    """The batched faces kernel (8-cell rows when W % 8 == 0, per cell
+    otherwise) gives bitwise the per-instance kernel's faces, and the batched
+    storage conversion gives torch's round-to-nearest BF16 of every level."""
+    import torch
+    specs = [((20, 13, 16), 61, 1.0, 1), ((17, 9, 12), 62, 2.0, 2), ((32, 24, 40), 63, 0.5, 1),
+             ((9, 8, 8), 64, 1.5, 1)]
+    print("batched setup kernels vs per-instance faces", flush=True)
+    mg, _ = _batch(dev, specs, "bf16")         # faces from hfm_mg_faces, one instance at a time
+    single = [inst.face32[0].clone() for inst in mg.insts]
+    for inst in mg.insts:
+        inst.face32[0].zero_()                 # faces are positive: a missed write shows
+    mg.build_faces()                           # the same staged conductivities, batched
+    mg.setup()
+    torch.cuda.synchronize()
+    for inst, ref in zip(mg.insts, single):
+        assert torch.equal(inst.face32[0], ref), inst.shape
+        for lev in range(len(inst.levels)):
+            assert torch.equal(inst.faceT[lev], inst.face32[lev].to(torch.bfloat16)), (inst.shape, lev)
