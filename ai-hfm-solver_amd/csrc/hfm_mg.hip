@@ -94,18 +94,6 @@ __device__ __forceinline__ void st4(HFM_GLOBAL float* p, const float v[4]) {
 __device__ __forceinline__ void st4(HFM_GLOBAL bf16* p, const float v[4]) {
   *(HFM_GLOBAL u2v*)p = u2v{pack2(v[0], v[1]), pack2(v[2], v[3])};
 }
-__device__ __forceinline__ void ld8d(const HFM_GLOBAL double* p, double v[8]) {
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    const d2v a = ((const HFM_GLOBAL d2v*)p)[q];
-    v[2 * q] = a.x;
-    v[2 * q + 1] = a.y;
-  }
-}
-__device__ __forceinline__ void st8d(HFM_GLOBAL double* p, const double v[8]) {
-#pragma unroll
-  for (int q = 0; q < 4; ++q) ((HFM_GLOBAL d2v*)p)[q] = d2v{v[2 * q], v[2 * q + 1]};
-}
 // The run at n0 of a lane-interleaved FP64 vector (see kIlvSeg).  These
 // vectors are read once per launch: non-temporal loads keep them from
 // displacing the stencil rows other lanes still re-read from the caches
@@ -151,19 +139,6 @@ __device__ __forceinline__ void st8_nt(HFM_GLOBAL bf16* p, const float v[8]) {
   __builtin_nontemporal_store(u4v{pack2(v[0], v[1]), pack2(v[2], v[3]), pack2(v[4], v[5]),
                                   pack2(v[6], v[7])},
                               (HFM_GLOBAL u4v*)p);
-}
-__device__ __forceinline__ void ld8d_nt(const HFM_GLOBAL double* p, double v[8]) {
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    const d2v a = __builtin_nontemporal_load((const HFM_GLOBAL d2v*)p + q);
-    v[2 * q] = a.x;
-    v[2 * q + 1] = a.y;
-  }
-}
-__device__ __forceinline__ void st8d_nt(HFM_GLOBAL double* p, const double v[8]) {
-#pragma unroll
-  for (int q = 0; q < 4; ++q)
-    __builtin_nontemporal_store(d2v{v[2 * q], v[2 * q + 1]}, (HFM_GLOBAL d2v*)p + q);
 }
 
 // ------------------------------------------------------------------ stencil arithmetic
@@ -214,11 +189,6 @@ __device__ __forceinline__ void decompose(const hfm_mg_task& t, int n, int& i, i
 struct Run {
   int n0, i0, j, k;
 };
-
-template <typename A>
-__device__ __forceinline__ A shfl_up1(A v) { return __shfl_up(v, 1, 64); }
-template <typename A>
-__device__ __forceinline__ A shfl_down1(A v) { return __shfl_down(v, 1, 64); }
 
 // Face coefficients of a run: west face cw[m], east ce[m], south cs, north cn,
 // down cd, up cu.  Every address is in bounds (lanes past the end of a task
