@@ -8,6 +8,8 @@
 #include <stdarg.h>
 #include <stdio.h>
 
+#include <algorithm>
+
 #include "hfm_common.h"
 
 namespace hfm {
@@ -688,6 +690,91 @@ __global__ __launch_bounds__(kBX* kBY) void k_bc_velocity(const float* __restric
   wo[n] = c;
 }
 
+// In-place velocity fill.  The values boundary cells copy from the rows and
+// plane next to them (u at j = 1 and j = H-2, u and v at k = D-2) are
+// gathered before any cell is written; then each boundary cell is written
+// once, by one thread, with k_bc_velocity's rule, so the result is bitwise
+// the out-of-place fill's.  scratch: [u(k,1,i) | u(k,H-2,i) | u(D-2,j,i) |
+// v(D-2,j,i)], 2 D W + 2 H W floats.
+__global__ __launch_bounds__(256) void k_bc_velocity_gather(const float* __restrict__ u,
+                                                            const float* __restrict__ v,
+                                                            float* __restrict__ scratch, Grid3 g) {
+  const int64_t DW = int64_t(g.D) * g.W, HW = int64_t(g.H) * g.W;
+  const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t < DW) {
+    const int k = int(t / g.W), i = int(t % g.W);
+    scratch[t] = u[g.idx(k, 1, i)];
+    scratch[DW + t] = u[g.idx(k, g.H - 2, i)];
+  }
+  if (t < HW) {
+    const int j = int(t / g.W), i = int(t % g.W);
+    scratch[2 * DW + t] = u[g.idx(g.D - 2, j, i)];
+    scratch[2 * DW + HW + t] = v[g.idx(g.D - 2, j, i)];
+  }
+}
+
+// Boundary cells, each once: the two x faces (all k, j), then the two y
+// faces without their x-edge cells, then the two z faces without edges.
+__global__ __launch_bounds__(256) void k_bc_velocity_inplace(float* u, float* v, float* w,
+                                                             const float* __restrict__ scratch,
+                                                             Grid3 g, float ub,
+                                                             const float* __restrict__ inflow) {
+  const int D = g.D, H = g.H, W = g.W;
+  const int64_t nx = 2 * int64_t(D) * H, ny = 2 * int64_t(D) * (W - 2);
+  const int64_t nz = 2 * int64_t(H - 2) * (W - 2);
+  int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  int k, j, i;
+  if (t < nx) {
+    i = (t & 1) ? W - 1 : 0;
+    t >>= 1;
+    k = int(t / H);
+    j = int(t % H);
+  } else if ((t -= nx) < ny) {
+    j = (t & 1) ? H - 1 : 0;
+    t >>= 1;
+    k = int(t / (W - 2));
+    i = 1 + int(t % (W - 2));
+  } else if ((t -= ny) < nz) {
+    k = (t & 1) ? D - 1 : 0;
+    t >>= 1;
+    j = 1 + int(t / (W - 2));
+    i = 1 + int(t % (W - 2));
+  } else {
+    return;
+  }
+  const int64_t DW = int64_t(D) * W, HW = int64_t(H) * W;
+  const int64_t n = g.idx(k, j, i);
+  float a = u[n], b = v[n], c = w[n];
+  if (i == 0 || i == W - 1) {
+    a = inflow ? inflow[k] : ub;
+    b = 0.0f;
+    c = 0.0f;
+  }
+  if (j == 0) {
+    a = scratch[int64_t(k) * W + i];
+    b = 0.0f;
+    c = 0.0f;
+  }
+  if (j == H - 1) {
+    a = scratch[DW + int64_t(k) * W + i];
+    b = 0.0f;
+    c = 0.0f;
+  }
+  if (k == 0) {
+    a = 0.0f;
+    b = 0.0f;
+    c = 0.0f;
+  }
+  if (k == D - 1) {
+    a = scratch[2 * DW + int64_t(j) * W + i];
+    b = scratch[2 * DW + HW + int64_t(j) * W + i];
+    c = 0.0f;
+  }
+  u[n] = a;
+  v[n] = b;
+  w[n] = c;
+}
+
 __global__ __launch_bounds__(256) void k_bluff(const float* __restrict__ in, float* out,
                                                const float* __restrict__ sigma, float dt,
                                                int64_t n) {
@@ -1055,6 +1142,24 @@ extern "C" int hfm_bc_f32(const float* in, float* out, int64_t D, int64_t H, int
   const Grid3 g{int(D), int(H), int(W)};
   hipLaunchKernelGGL(k_bc, grid_for(g), dim3(kBX, kBY), 0, as_stream(stream), in, out, g, kind);
   return launch_status("hfm_bc_f32");
+}
+
+extern "C" int hfm_bc_velocity_inplace_f32(float* u, float* v, float* w, float* scratch,
+                                           int64_t D, int64_t H, int64_t W, float ub,
+                                           const float* inflow, void* stream) {
+  if (!u || !v || !w || !scratch || !dims_ok(D, H, W) || D < 2 || H < 2 || W < 2) {
+    set_error("hfm_bc_velocity_inplace_f32: bad arguments");
+    return HFM_EINVAL;
+  }
+  const Grid3 g{int(D), int(H), int(W)};
+  hipStream_t s = as_stream(stream);
+  const int64_t ng = std::max(D * W, H * W);
+  hipLaunchKernelGGL(k_bc_velocity_gather, dim3(unsigned((ng + 255) / 256)), dim3(256), 0, s, u, v,
+                     scratch, g);
+  const int64_t nb = 2 * D * H + 2 * D * (W - 2) + 2 * (H - 2) * (W - 2);
+  hipLaunchKernelGGL(k_bc_velocity_inplace, dim3(unsigned((nb + 255) / 256)), dim3(256), 0, s, u,
+                     v, w, scratch, g, ub, inflow);
+  return launch_status("hfm_bc_velocity_inplace_f32");
 }
 
 extern "C" int hfm_bc_velocity_f32(const float* u, const float* v, const float* w, float* uo,

@@ -437,6 +437,32 @@ def boundary_condition_velocity(values_u, values_v, values_w, nx=None, ub: float
     return uo, vo, wo
 
 
+def boundary_condition_velocity_(u, v, w, ub: float = 1.0, inflow=None, scratch=None):
+    """``boundary_condition_velocity`` in place: writes only the boundary planes
+    of u, v, w (bitwise the out-of-place result) and returns them.  For a
+    driver's own intermediates: the reference's function returns new tensors.
+    ``scratch``: 2 D W + 2 H W floats on the device (allocated when omitted)."""
+    for a, n in ((u, "u"), (v, "v"), (w, "w")):
+        if not (isinstance(a, torch.Tensor) and a.device.type == "cuda"
+                and a.dtype == torch.float32 and a.is_contiguous()):
+            raise ValueError(f"{n} must be a contiguous float32 GPU tensor to be filled in place")
+    (D, H, W), _ = _spatial(u)
+    if v.shape != u.shape or w.shape != u.shape:
+        raise ValueError("u, v, w must share a shape")
+    if inflow is not None:
+        inflow = _prep(inflow, "inflow")
+        if inflow.numel() != D:
+            raise ValueError("inflow profile needs one value per height")
+    need = 2 * D * W + 2 * H * W
+    if scratch is None:
+        scratch = torch.empty(need, dtype=torch.float32, device=u.device)
+    elif scratch.numel() < need or scratch.dtype != torch.float32 or not scratch.is_contiguous():
+        raise ValueError(f"scratch needs {need} contiguous float32 values")
+    L.call("hfm_bc_velocity_inplace_f32", u.data_ptr(), v.data_ptr(), w.data_ptr(),
+           scratch.data_ptr(), D, H, W, float(ub), L.ptr(inflow), L.stream_handle(u.device))
+    return u, v, w
+
+
 def bluff_body(values_u, values_v, values_w, alpha, sigma, dt: float):
     """``f / (1 + dt*sigma)`` for u, v, w and the scalar (immersed obstacle
     damping); ``alpha=None`` damps the velocities only (the INHALE demo's

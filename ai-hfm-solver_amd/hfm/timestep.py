@@ -91,6 +91,19 @@ class UrbanFlow:
     def _bluff(self, u, v, w, alpha):
         return Ly.bluff_body(u, v, w, alpha, self.sigma, dt=self.dt)
 
+    def _inflow(self):
+        return None
+
+    def _bcv_(self, u, v, w):
+        """The velocity fill in place, on the step's own intermediates (it writes
+        only the boundary planes; the step's inputs get the copying fill)."""
+        D, H, W = self.shape
+        n = 2 * D * W + 2 * H * W
+        if getattr(self, "_bc_scratch", None) is None or self._bc_scratch.device != u.device:
+            self._bc_scratch = torch.empty(n, dtype=torch.float32, device=u.device)
+        return Ly.boundary_condition_velocity_(u, v, w, ub=self.ub, inflow=self._inflow(),
+                                               scratch=self._bc_scratch)
+
     def _gradient(self, u, v, w, p):
         return (self.grad[0].subtract_from(u, p), self.grad[1].subtract_from(v, p),
                 self.grad[2].subtract_from(w, p))
@@ -102,14 +115,14 @@ class UrbanFlow:
         u, v, w = Ly.boundary_condition_velocity(u, v, w, ub=self.ub)
         p = Ly.boundary_condition_pressure(p)
         bu, bv, bw = Ly.advect((u, v, w), (u, v, w), (u, v, w), 0.5, self.k_mom)
-        bu, bv, bw = Ly.boundary_condition_velocity(bu, bv, bw, ub=self.ub)
+        bu, bv, bw = self._bcv_(bu, bv, bw)
         u, v, w = Ly.advect((bu, bv, bw), (bu, bv, bw), (u, v, w), 1.0, self.k_mom)
         if self.lscalar:
             (alpha,) = Ly.advect((u, v, w), (alpha,), (alpha,), 1.0, self.k_mom)
             alpha = Ly.boundary_condition_indicator(alpha)
         u, v, w, alpha = self._bluff(u, v, w, alpha)
         u, v, w = self._gradient(u, v, w, p)
-        u, v, w = Ly.boundary_condition_velocity(u, v, w, ub=self.ub)
+        u, v, w = self._bcv_(u, v, w)
         b = Ly.conv_sum((u, v, w), self.k_div, scale=-1.0, source_bc=True)
         rec = {}
         for _ in range(self.multi_itr):
@@ -117,7 +130,7 @@ class UrbanFlow:
         self.last_w = rec.get(0)
         p = Ly.boundary_condition_pressure(p)
         u, v, w = self._gradient(u, v, w, p)
-        u, v, w = Ly.boundary_condition_velocity(u, v, w, ub=self.ub)
+        u, v, w = self._bcv_(u, v, w)
         u, v, w, alpha = self._bluff(u, v, w, alpha)
         return tuple(t.reshape(shape) for t in (u, v, w, p, alpha))
 
@@ -166,7 +179,7 @@ class InhaleFlow(UrbanFlow):
         p = Ly.boundary_condition_pressure(p)
         kx = self._pg(u, v, w)
         bu, bv, bw = Ly.advect_dissipative((u, v, w), (u, v, w), (u, v, w), kx, 0.5, adv)
-        bu, bv, bw = Ly.boundary_condition_velocity(bu, bv, bw, ub=self.ub)
+        bu, bv, bw = self._bcv_(bu, bv, bw)
         kb = self._pg(bu, bv, bw)
         u, v, w = Ly.advect_dissipative((bu, bv, bw), (bu, bv, bw), (u, v, w), kb, 1.0, adv)
         if self.lscalar:
@@ -174,7 +187,7 @@ class InhaleFlow(UrbanFlow):
             alpha = Ly.boundary_condition_indicator(alpha)
         u, v, w = self._bluff(u, v, w, None)
         u, v, w = self._gradient(u, v, w, p)
-        u, v, w = Ly.boundary_condition_velocity(u, v, w, ub=self.ub)
+        u, v, w = self._bcv_(u, v, w)
         b = Ly.conv_sum((u, v, w), self.k_div, scale=-1.0, source_bc=True)
         rec = {}
         for _ in range(self.multi_itr):
@@ -182,7 +195,7 @@ class InhaleFlow(UrbanFlow):
         self.last_w = rec.get(0)
         p = Ly.boundary_condition_pressure(p)
         u, v, w = self._gradient(u, v, w, p)
-        u, v, w = Ly.boundary_condition_velocity(u, v, w, ub=self.ub)
+        u, v, w = self._bcv_(u, v, w)
         u, v, w = self._bluff(u, v, w, None)
         return tuple(t.reshape(shape) for t in (u, v, w, p, alpha))
 
@@ -247,6 +260,9 @@ class UrbanProjection(UrbanFlow):
     def _bcv(self, u, v, w):
         return Ly.boundary_condition_velocity(u, v, w, ub=self.ub, inflow=self.inflow)
 
+    def _inflow(self):
+        return self.inflow
+
     def pressure(self, b):
         inst = self.solver.insts[0]
         inst.b32.copy_(b.reshape(-1))
@@ -259,18 +275,18 @@ class UrbanProjection(UrbanFlow):
         u, v, w = self._bcv(u, v, w)
         p = Ly.boundary_condition_pressure(p)
         bu, bv, bw = Ly.advect((u, v, w), (u, v, w), (u, v, w), 0.5, self.k_mom)
-        bu, bv, bw = self._bcv(bu, bv, bw)
+        bu, bv, bw = self._bcv_(bu, bv, bw)
         u, v, w = Ly.advect((bu, bv, bw), (bu, bv, bw), (u, v, w), 1.0, self.k_mom)
         if self.lscalar:
             (alpha,) = Ly.advect((u, v, w), (alpha,), (alpha,), 1.0, self.k_mom)
             alpha = Ly.boundary_condition_indicator(alpha)
         u, v, w, alpha = self._bluff(u, v, w, alpha)
         u, v, w = self._gradient(u, v, w, p)
-        u, v, w = self._bcv(u, v, w)
+        u, v, w = self._bcv_(u, v, w)
         b = Ly.conv_sum((u, v, w), self.k_div, scale=-1.0, source_bc=True)
         p = self.pressure(b)
         p = Ly.boundary_condition_pressure(p)
         u, v, w = self._gradient(u, v, w, p)
-        u, v, w = self._bcv(u, v, w)
+        u, v, w = self._bcv_(u, v, w)
         u, v, w, alpha = self._bluff(u, v, w, alpha)
         return tuple(t.reshape(shape) for t in (u, v, w, p, alpha))

@@ -170,6 +170,12 @@ int hfm_bc_velocity_f32(const float* u, const float* v, const float* w, float* u
                         float* wo, int64_t D, int64_t H, int64_t W, float ub, const float* inflow,
                         void* stream);
 
+/* The same fill in place: only the boundary planes are written (bitwise the
+ * out-of-place result).  scratch: 2*D*W + 2*H*W floats of device memory (the
+ * pre-fill rows and plane that boundary cells copy). */
+int hfm_bc_velocity_inplace_f32(float* u, float* v, float* w, float* scratch, int64_t D, int64_t H,
+                                int64_t W, float ub, const float* inflow, void* stream);
+
 /* out_i = in_i / (1 + dt*sigma) for n fields (in place allowed). */
 int hfm_bluff_body_f32(const float* const* in, float* const* out, int nfields,
                        const float* sigma, float dt, int64_t ncells, void* stream);

@@ -218,3 +218,27 @@ def test_nonuniform_mesh_hierarchy(dev):
         ref_j = (x - ref / m["tot"]) + x / m["tot"]
         assert rel(got_j, ref_j) < 1e-6, lev
         m = O.restrict_var6(m)
+
+
+@pytest.mark.parametrize("shape", [(6, 7, 9), (2, 2, 2), (3, 2, 5), (2, 5, 2), (17, 33, 64)])
+@pytest.mark.parametrize("profile", [False, True])
+def test_velocity_fill_in_place_bitwise(dev, shape, profile):
+    """The in-place velocity fill (boundary planes only, pre-fill rows and
+    plane gathered first) equals the copying fill and the oracle bitwise,
+    with a constant inflow and with a height profile."""
+    import torch
+    from hfm import layers
+    rng = np.random.default_rng(sum(shape) + profile)
+    u, v, w = (rng.standard_normal(shape).astype(np.float32) for _ in range(3))
+    inflow = rng.standard_normal(shape[0]).astype(np.float32) if profile else None
+    ut, vt, wt = (torch.from_numpy(a).to(dev) for a in (u, v, w))
+    it = torch.from_numpy(inflow).to(dev) if profile else None
+    ref = layers.boundary_condition_velocity(ut, vt, wt, ub=1.25, inflow=it)
+    got = layers.boundary_condition_velocity_(ut, vt, wt, ub=1.25, inflow=it)
+    assert got[0] is ut and got[1] is vt and got[2] is wt
+    torch.cuda.synchronize()
+    for g, r in zip(got, ref):
+        assert torch.equal(g, r)
+    if not profile:
+        for g, o in zip(got, O.boundary_condition_velocity(u, v, w, ub=np.float32(1.25))):
+            np.testing.assert_array_equal(g.cpu().numpy(), o)
