@@ -602,6 +602,7 @@ struct Raw4<float> {
     v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
   }
   __device__ __forceinline__ void set(const float v[4]) { a = f4v{v[0], v[1], v[2], v[3]}; }
+  __device__ __forceinline__ void zero() { a = f4v{0.0f, 0.0f, 0.0f, 0.0f}; }
 };
 template <>
 struct Raw4<bf16> {
@@ -619,6 +620,7 @@ struct Raw4<bf16> {
   __device__ __forceinline__ void set(const float v[4]) {  // values of type bf16
     u = u2v{pack2(v[0], v[1]), pack2(v[2], v[3])};
   }
+  __device__ __forceinline__ void zero() { u = u2v{0u, 0u}; }
 };
 __device__ __forceinline__ void lds_ld4(const float* p, float v[4]) {
   const f4v a = *(const f4v*)p;
@@ -668,6 +670,11 @@ struct XRun {
                                        int j, int i0, int Hc, int Wc) {
     x.load(xp + n);
     if (CORR) ld2(ec + ((k >> 1) * Hc + (j >> 1)) * Wc + (i0 >> 1), e);
+  }
+  // a plane past the grid's ends: get() then yields +0 (x + alpha * 0 = +0)
+  __device__ __forceinline__ void zero() {
+    x.zero();
+    e[0] = e[1] = 0.0f;
   }
   __device__ __forceinline__ void get(float v[4], float alpha) const {
     x.get(v);
@@ -818,6 +825,7 @@ __global__ __launch_bounds__(kPairBlock, sizeof(T) == 2 ? 4 : 2) void k_smooth_p
     const int n = cell(kk);
     const bool up1 = kk + 1 < D;
     pi.xn.load(xin, ec, up1 ? n + HW : n, up1 ? kk + 1 : kk, jc, i0, Hc, Wc);
+    if (!up1) pi.xn.zero();
     pi.cx.load(cx + n);
     pi.cy.load(cy + n);
     pi.cu.load(up1 ? cz + n + HW : gin<T>(t.c[5]) + jc * W + i0);
@@ -832,7 +840,10 @@ __global__ __launch_bounds__(kPairBlock, sizeof(T) == 2 ? 4 : 2) void k_smooth_p
 
   // The walk keeps three plane records in rotation (previous, current, the
   // one being loaded); the step is written once and unrolled three times so
-  // that the rotation costs no register moves.
+  // that the rotation costs no register moves.  The z-neighbours past the
+  // grid's ends are held as +0 in the records (the x-row below plane 0 and
+  // above plane D - 1, the pass-1 results of planes -1 and D), so both passes
+  // take them unmasked: the +0 that the masked form substitutes.
   PlaneIn<T, S2> pa, pb, pc;
   float ev0[N];             // an even row's residuals on an even plane
 #pragma unroll
@@ -842,7 +853,10 @@ __global__ __launch_bounds__(kPairBlock, sizeof(T) == 2 ? 4 : 2) void k_smooth_p
   if (live) {
     // pa stands for plane ks - 1 (its x-row is x(ks), its up faces cz(ks)),
     // pc for ks - 2 (its x-row is x(ks - 1)); pb is plane ks
-    if (ks > 0) pc.xn.load(xin, ec, cell(ks - 1), ks - 1, jc, i0, Hc, Wc);
+    if (ks > 0)
+      pc.xn.load(xin, ec, cell(ks - 1), ks - 1, jc, i0, Hc, Wc);
+    else
+      pc.xn.zero();
     pa.xn.load(xin, ec, cell(ks), ks, jc, i0, Hc, Wc);
     pa.cu.load(cz + cell(ks));
     load_plane(pb, ks);
@@ -876,7 +890,7 @@ __global__ __launch_bounds__(kPairBlock, sizeof(T) == 2 ? 4 : 2) void k_smooth_p
       const float wv = west ? lx[buf][li - 1] : 0.0f;
       const float ev = east ? lx[buf][li + N] : 0.0f;
       stencil_run<T>(cur.cx, cur.ef, cur.cy, cur.cn, prv.cu, cur.cu, xc, wv, ev, vs, vn, vd, vu,
-                     south, north, kk > 0, kk + 1 < D, d, off);
+                     south, north, true, true, d, off);
       if constexpr (!S2) {
 #pragma unroll
         for (int m = 0; m < N; ++m) cur.d[m] = d[m];
@@ -910,7 +924,7 @@ __global__ __launch_bounds__(kPairBlock, sizeof(T) == 2 ? 4 : 2) void k_smooth_p
           for (int m = 0; m < N; ++m) d[m] = prv.d[m];
         }
         stencil_run<T, S2>(prv.cx, prv.ef, prv.cy, prv.cn, pcd, prv.cu, yc, wv, ev, vs, vn, vd,
-                           vu, south, north, kp > 0, kp + 1 < D, d, off);
+                           vu, south, north, true, true, d, off);
         prv.b.get(bv);
         if constexpr (S2) {
           float o[N];
