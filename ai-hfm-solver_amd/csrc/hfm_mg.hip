@@ -603,6 +603,9 @@ struct Raw4<float> {
   }
   __device__ __forceinline__ void set(const float v[4]) { a = f4v{v[0], v[1], v[2], v[3]}; }
   __device__ __forceinline__ void zero() { a = f4v{0.0f, 0.0f, 0.0f, 0.0f}; }
+  __device__ __forceinline__ void store_nt(HFM_GLOBAL float* p) const {
+    __builtin_nontemporal_store(a, (HFM_GLOBAL f4v*)p);
+  }
 };
 template <>
 struct Raw4<bf16> {
@@ -617,10 +620,13 @@ struct Raw4<bf16> {
     v[0] = __uint_as_float(u.x << 16); v[1] = __uint_as_float(u.x & 0xFFFF0000u);
     v[2] = __uint_as_float(u.y << 16); v[3] = __uint_as_float(u.y & 0xFFFF0000u);
   }
-  __device__ __forceinline__ void set(const float v[4]) {  // values of type bf16
+  __device__ __forceinline__ void set(const float v[4]) {  // rounded to BF16 here
     u = u2v{pack2(v[0], v[1]), pack2(v[2], v[3])};
   }
   __device__ __forceinline__ void zero() { u = u2v{0u, 0u}; }
+  __device__ __forceinline__ void store_nt(HFM_GLOBAL bf16* p) const {
+    __builtin_nontemporal_store(u, (HFM_GLOBAL u2v*)p);
+  }
 };
 __device__ __forceinline__ void lds_ld4(const float* p, float v[4]) {
   const f4v a = *(const f4v*)p;
@@ -769,9 +775,13 @@ __global__ __launch_bounds__(kPairBlock, sizeof(T) == 2 ? 4 : 2) void k_smooth_p
     const hfm_mg_task* __restrict__ tasks, const uint16_t* __restrict__ blk_task,
     const hfm_mg_scalars* __restrict__ sc, float omega, float alpha, double* dots) {
   constexpr int N = kPairRun;
-  __shared__ float lx[2][kPairPlane];  // pass-1 input of the plane
-  __shared__ T lc[2][kPairPlane];      // south faces of the plane
-  __shared__ T ly[2][kPairPlane];      // pass-1 result of the plane
+  // lx / ly end in a run of zeros (at kZero), which a thread reads in place
+  // of a neighbour past the grid's edge: the masked form's +0, without a
+  // select per cell
+  constexpr int kZero = kPairPlane;
+  __shared__ float lx[2][kPairPlane + N];  // pass-1 input of the plane
+  __shared__ T lc[2][kPairPlane];          // south faces of the plane
+  __shared__ T ly[2][kPairPlane + N];      // pass-1 result of the plane
   __shared__ float lr[2][S2 ? 1 : kPairPlane / 2];  // odd rows' residuals (restriction)
   // Tiles are numbered contiguously over the launch's tasks (a task's tiles
   // are blk0 .. blk0 + nblk - 1); each XCD takes a contiguous range of them,
@@ -811,6 +821,16 @@ __global__ __launch_bounds__(kPairBlock, sizeof(T) == 2 ? 4 : 2) void k_smooth_p
   // the rows next to the tile's first and last rows are not in LDS
   const bool edge_s = r == 0 && south, edge_n = r + 1 == NR && north;
   const int je = edge_s ? jc - 1 : (edge_n ? jc + 1 : jc);
+  // LDS offsets of the in-plane neighbours (rows r - 1 / r + 1, the cells
+  // before / after the run), or of the zeros where the grid ends there
+  const int os = south ? li - W : kZero, on = north ? li + W : kZero;
+  const int ow = west ? li - 1 : kZero, oe = east ? li + N : kZero;
+  if (threadIdx.x < N) {  // read after the walk's first barrier
+    lx[0][kZero + threadIdx.x] = 0.0f;
+    lx[1][kZero + threadIdx.x] = 0.0f;
+    ly[0][kZero + threadIdx.x] = from_f<T>(0.0f);
+    ly[1][kZero + threadIdx.x] = from_f<T>(0.0f);
+  }
   const int k0 = zb * Z, k1 = min(k0 + Z, D);  // output planes [k0, k1)
   const int ks = max(k0 - 1, 0);
   const HFM_GLOBAL T* xin = gin<T>(t.xin);
@@ -881,16 +901,17 @@ __global__ __launch_bounds__(kPairBlock, sizeof(T) == 2 ? 4 : 2) void k_smooth_p
     if (p1 && live) {
       if (north && !edge_n) cur.cn.lds_load(&lc[buf][li + W]);
       float vs[N], vn[N], vd[N], vu[N], d[N], off[N];
-      if (r > 0) lds_ld4(&lx[buf][li - W], vs);
-      else cur.edge.get(vs, alpha);
-      if (r + 1 < NR) lds_ld4(&lx[buf][li + W], vn);
-      else cur.edge.get(vn, alpha);
+      // the rows past the tile's first and last rows come from memory
+      if (edge_s) cur.edge.get(vs, alpha);
+      else lds_ld4(&lx[buf][os], vs);
+      if (edge_n) cur.edge.get(vn, alpha);
+      else lds_ld4(&lx[buf][on], vn);
       xm.get(vd, alpha);
       cur.xn.get(vu, alpha);
-      const float wv = west ? lx[buf][li - 1] : 0.0f;
-      const float ev = east ? lx[buf][li + N] : 0.0f;
+      const float wv = lx[buf][ow];
+      const float ev = lx[buf][oe];
       stencil_run<T>(cur.cx, cur.ef, cur.cy, cur.cn, prv.cu, cur.cu, xc, wv, ev, vs, vn, vd, vu,
-                     south, north, true, true, d, off);
+                     true, true, true, true, d, off);
       if constexpr (!S2) {
 #pragma unroll
         for (int m = 0; m < N; ++m) cur.d[m] = d[m];
@@ -898,7 +919,9 @@ __global__ __launch_bounds__(kPairBlock, sizeof(T) == 2 ? 4 : 2) void k_smooth_p
       float bv[N], y[N];
       cur.b.get(bv);
 #pragma unroll
-      for (int m = 0; m < N; ++m) y[m] = round_to<T>(jacobi_val(xc[m], d[m], bv[m], off[m], omega));
+      for (int m = 0; m < N; ++m) y[m] = jacobi_val(xc[m], d[m], bv[m], off[m], omega);
+      // y is rounded to T where it is packed (the record, the store): one
+      // conversion per two cells, no round trip through a float
       cur.y.set(y);
       if (!S2 && outrow && kk >= k0 && kk < k1) st4_nt(gout<T>(t.xout) + cell(kk), y);
       cur.y.lds_store(&ly[buf][li]);
@@ -915,25 +938,28 @@ __global__ __launch_bounds__(kPairBlock, sizeof(T) == 2 ? 4 : 2) void k_smooth_p
         prv.y.get(yc);
         ym2.get(vd);
         cur.y.get(vu);
-        lds_ld4(&ly[pb2][south ? li - W : li], vs);
-        lds_ld4(&ly[pb2][north ? li + W : li], vn);
-        const float wv = west ? lds_ld1(&ly[pb2][li - 1]) : 0.0f;
-        const float ev = east ? lds_ld1(&ly[pb2][li + N]) : 0.0f;
+        lds_ld4(&ly[pb2][os], vs);
+        lds_ld4(&ly[pb2][on], vn);
+        const float wv = lds_ld1(&ly[pb2][ow]);
+        const float ev = lds_ld1(&ly[pb2][oe]);
         if constexpr (!S2) {
 #pragma unroll
           for (int m = 0; m < N; ++m) d[m] = prv.d[m];
         }
         stencil_run<T, S2>(prv.cx, prv.ef, prv.cy, prv.cn, pcd, prv.cu, yc, wv, ev, vs, vn, vd,
-                           vu, south, north, true, true, d, off);
+                           vu, true, true, true, true, d, off);
         prv.b.get(bv);
         if constexpr (S2) {
           float o[N];
 #pragma unroll
           for (int m = 0; m < N; ++m) o[m] = jacobi_val(yc[m], d[m], bv[m], off[m], omega);
-          st4_nt(gout<O>(t.xout) + cell(kp), o);
+          Raw4<O> ro;
+          ro.set(o);  // z as stored; the dot takes the stored values
+          ro.store_nt(gout<O>(t.xout) + cell(kp));
           if (DOT) {
+            ro.get(o);
 #pragma unroll
-            for (int m = 0; m < N; ++m) acc += double(bv[m]) * double(round_to<O>(o[m]));
+            for (int m = 0; m < N; ++m) acc += double(bv[m]) * double(o[m]);
           }
         } else {
 #pragma unroll
