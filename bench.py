@@ -591,7 +591,9 @@ def update_traffic():
 
 def traffic_driver(args):
     """Torch-free driver for hardware-counter runs of the dominant kernel,
-    k_fcg_update (the _sweep form the solver launches, BF16 storage).
+    k_fcg_update (the _sweep form the solver launches, BF16 storage), or with
+    ``--traffic-kernel sr`` of the fine level's fused pre-smoothing launch
+    k_smooth_pair<bf16, false> (Jacobi sweep + residual restriction).
 
     The work tables and buffers of the headline batch (``--instances`` shapes
     from ``--seed``, the same draw as the solve workload) or, with
@@ -630,6 +632,8 @@ def traffic_driver(args):
         what = f"{args.instances} instances, edges from seed {args.seed}"
     n = len(shapes)
     cells = np.array([D * H * W for (D, H, W) in shapes], dtype=np.int64)
+    if args.traffic_kernel == "sr":
+        return _traffic_sr(args, shapes, cells, what, malloc, upload, hip)
     nblk = ((cells + CHUNK - 1) // CHUNK + 7) // 8 * 8
     blk0 = np.concatenate([[0], np.cumsum(nblk)[:-1]])
     total = int(nblk.sum())
@@ -654,27 +658,11 @@ def traffic_driver(args):
     dots = malloc(8 * total)
     hip.hipDeviceSynchronize()
 
-    def timed(fn, reps):
-        ev = [vp(), vp()]
-        for x in ev:
-            hip.hipEventCreate(C.byref(x))
-        fn()
-        hip.hipEventRecord(ev[0], None)
-        for _ in range(reps):
-            fn()
-        hip.hipEventRecord(ev[1], None)
-        hip.hipEventSynchronize(ev[1])
-        ms = C.c_float()
-        hip.hipEventElapsedTime(C.byref(ms), ev[0], ev[1])
-        return ms.value / reps
-
+    timed = lambda fn, reps: _hip_timed(hip, fn, reps)  # noqa: E731
     upd = lambda: L.call("hfm_fcg_update_sweep", tasks, blk, total, scp, L.STORE["bf16"],  # noqa: E731
                          L.STORE["bf16"], C.c_float(2.0 / 3.0), dots, None)
     ms = timed(upd, args.reps)
-    nbytes = 1 << 30
-    src, dst = malloc(nbytes), malloc(nbytes)
-    cp = lambda: hip.hipMemcpy(vp(dst), vp(src), C.c_size_t(nbytes), 3)  # noqa: E731 device->device
-    cp_ms = timed(cp, args.reps)
+    nbytes, cp_ms = _copy_reference(hip, malloc, args.reps)
     N = int(cells.sum())
     # bf16 MGBatch.update_bytes_per_cell(lag): p 2, faces 12, r64 16, rlow 2,
     # first sweep 2; with the solution update also x64 16 and p_prev 2
@@ -685,6 +673,87 @@ def traffic_driver(args):
                       "alg_bytes_per_launch": alg,
                       "alg_read_bytes": N * rd, "alg_write_bytes": N * wr,
                       "launch_ms": round(ms, 4), "achieved_GBs": round(alg / (ms * 1e-3) / 1e9, 1),
+                      "copy_bytes_each_way": nbytes, "copy_launches": args.reps + 1,
+                      "copy_ms": round(cp_ms, 4)}), flush=True)
+
+
+def _hip_timed(hip, fn, reps):
+    """Mean ms of ``reps`` calls of fn after one warm call (HIP events on the
+    null stream)."""
+    import ctypes as C
+    ev = [C.c_void_p(), C.c_void_p()]
+    for x in ev:
+        hip.hipEventCreate(C.byref(x))
+    fn()
+    hip.hipEventRecord(ev[0], None)
+    for _ in range(reps):
+        fn()
+    hip.hipEventRecord(ev[1], None)
+    hip.hipEventSynchronize(ev[1])
+    ms = C.c_float()
+    hip.hipEventElapsedTime(C.byref(ms), ev[0], ev[1])
+    return ms.value / reps
+
+
+def _copy_reference(hip, malloc, reps):
+    """(bytes, mean ms) of a 1 GiB device-to-device copy: the counters'
+    calibration."""
+    import ctypes as C
+    nbytes = 1 << 30
+    src, dst = malloc(nbytes), malloc(nbytes)
+    cp = lambda: hip.hipMemcpy(C.c_void_p(dst), C.c_void_p(src), C.c_size_t(nbytes), 3)  # noqa: E731
+    return nbytes, _hip_timed(hip, cp, reps)
+
+
+def _traffic_sr(args, shapes, cells, what, malloc, upload, hip):
+    """The fused pre-smoothing launch of the fine level over the batch: x1 =
+    Jacobi(x0, r) and the coarse right-hand side of r - A x1, one tile per
+    512-thread block (tile geometry from hfm_mg_tile), BF16 storage."""
+    import ctypes as C
+    from hfm import _lib as L
+    tiles = [L.tile(s) for s in shapes]
+    if any(tl is None for tl in tiles):
+        raise SystemExit("a level of the workload does not tile")
+    n = len(shapes)
+    ntile = np.array([tl[2] for tl in tiles], dtype=np.int64)
+    blk0 = np.concatenate([[0], np.cumsum(ntile)[:-1]])
+    total = int(ntile.sum())
+    padded = (total + 7) // 8 * 8
+    t = np.zeros(n, dtype=L.TASK_DTYPE)
+    for q, ((D, H, W), (R, Z, nt)) in enumerate(zip(shapes, tiles)):
+        N = D * H * W
+        Dc, Hc, Wc = (D + 1) // 2, (H + 1) // 2, (W + 1) // 2
+        r = t[q]
+        r["D"], r["H"], r["W"], r["inst"] = D, H, W, q
+        r["blk0"], r["nblk"], r["tile_rows"], r["tile_planes"] = blk0[q], nt, R, Z
+        r["magW"], r["magH"] = L.magic(W), L.magic(H)
+        r["Dc"], r["Hc"], r["Wc"] = Dc, Hc, Wc
+        r["magWc"], r["magHc"] = L.magic(Wc), L.magic(Hc)
+        r["c"] = [malloc(2 * N) for _ in range(3)] + [malloc(2 * D * H), malloc(2 * D * W),
+                                                      malloc(2 * H * W)]
+        r["xin"], r["xout"], r["b"] = malloc(2 * N), malloc(2 * N), malloc(2 * N)
+        r["rlow"] = malloc(2 * Dc * Hc * Wc)
+    tasks = upload(t.view(np.uint8))
+    bt = np.full(padded, 0xFFFF, dtype=np.uint16)
+    bt[:total] = np.repeat(np.arange(n, dtype=np.uint16), ntile)
+    blk = upload(bt)
+    sc = np.zeros(n, dtype=L.SCALARS_DTYPE)
+    sc["active"] = 1
+    scp = upload(sc.view(np.uint8))
+    hip.hipDeviceSynchronize()
+    sr = lambda: L.call("hfm_mg_smooth_restrict", tasks, blk, padded, scp,  # noqa: E731
+                        L.STORE["bf16"], C.c_float(2.0 / 3.0), None)
+    ms = _hip_timed(hip, sr, args.reps)
+    nbytes, cp_ms = _copy_reference(hip, malloc, args.reps)
+    N = int(cells.sum())
+    # x0, r, three faces read; x1 written (2 B each); the coarse right-hand
+    # side written (2 B per 8 cells)
+    rd, wr = 10 * N, 2 * N + N // 4
+    print(json.dumps({"kernel": "k_smooth_pair<bf16, false, bf16, false>", "workload": what,
+                      "cells": N, "tiles": total, "blocks": padded, "launches": args.reps + 1,
+                      "alg_bytes_per_launch": rd + wr, "alg_read_bytes": rd, "alg_write_bytes": wr,
+                      "launch_ms": round(ms, 4),
+                      "achieved_GBs": round((rd + wr) / (ms * 1e-3) / 1e9, 1),
                       "copy_bytes_each_way": nbytes, "copy_launches": args.reps + 1,
                       "copy_ms": round(cp_ms, 4)}), flush=True)
 
@@ -739,6 +808,8 @@ def main():
     ap.add_argument("--traffic-edge", type=int, default=0)
     ap.add_argument("--traffic-lag", type=int, choices=(0, 1), default=1,
                     help="update variant: 1 with the deferred solution update, 0 without")
+    ap.add_argument("--traffic-kernel", choices=("update", "sr"), default="update",
+                    help="update: k_fcg_update; sr: the fine level's fused pre-smoothing launch")
     args = ap.parse_args()
     if args.traffic:
         return traffic_driver(args)
