@@ -860,10 +860,12 @@ __global__ __launch_bounds__(kPairBlock, sizeof(T) == 2 ? 4 : 2) void k_smooth_p
 
   // The walk keeps three plane records in rotation (previous, current, the
   // one being loaded); the step is written once and unrolled three times so
-  // that the rotation costs no register moves.  The z-neighbours past the
-  // grid's ends are held as +0 in the records (the x-row below plane 0 and
-  // above plane D - 1, the pass-1 results of planes -1 and D), so both passes
-  // take them unmasked: the +0 that the masked form substitutes.
+  // that the rotation costs no register moves.  Pass 1 takes its down
+  // neighbours from the other LDS buffer, which holds its own run of the
+  // plane below.  The z-neighbours past the grid's ends are +0 (the x-row
+  // below plane 0 in LDS, the x-row above plane D - 1 and the pass-1 results
+  // of planes -1 and D in the records), so both passes take them unmasked:
+  // the +0 that the masked form substitutes.
   PlaneIn<T, S2> pa, pb, pc;
   float ev0[N];             // an even row's residuals on an even plane
 #pragma unroll
@@ -871,12 +873,20 @@ __global__ __launch_bounds__(kPairBlock, sizeof(T) == 2 ? 4 : 2) void k_smooth_p
   pa.y.set(ev0);  // zeros
   pc.y.set(ev0);
   if (live) {
-    // pa stands for plane ks - 1 (its x-row is x(ks), its up faces cz(ks)),
-    // pc for ks - 2 (its x-row is x(ks - 1)); pb is plane ks
-    if (ks > 0)
-      pc.xn.load(xin, ec, cell(ks - 1), ks - 1, jc, i0, Hc, Wc);
-    else
-      pc.xn.zero();
+    // pa stands for plane ks - 1 (its x-row is x(ks), its up faces cz(ks));
+    // pb is plane ks.  The pass-1 input on plane ks - 1 goes to the LDS
+    // buffer that step ks reads its down neighbours from (a thread reads
+    // only its own run there, so no barrier orders it).
+    float xd[N];
+    if (ks > 0) {
+      XRun<T, S2> xr;
+      xr.load(xin, ec, cell(ks - 1), ks - 1, jc, i0, Hc, Wc);
+      xr.get(xd, alpha);
+    } else {
+#pragma unroll
+      for (int m = 0; m < N; ++m) xd[m] = 0.0f;
+    }
+    lds_st4(&lx[(ks & 1) ^ 1][li], xd);
     pa.xn.load(xin, ec, cell(ks), ks, jc, i0, Hc, Wc);
     pa.cu.load(cz + cell(ks));
     load_plane(pb, ks);
@@ -887,7 +897,6 @@ __global__ __launch_bounds__(kPairBlock, sizeof(T) == 2 ? 4 : 2) void k_smooth_p
     const bool p1 = kk < D;  // pass 1 runs on plane kk (uniform)
     const int buf = kk & 1;
     // what the pass pair still needs of plane kk - 2 before its record is reused
-    const XRun<T, S2> xm = spare.xn;   // pass-1 input on plane kk - 1
     const Raw4<T> pcd = spare.cu;      // down faces of plane kk - 1
     const Raw4<T> ym2 = spare.y;       // pass 1 on plane kk - 2
     if (live && kk + 1 < D && kk + 1 <= k1) load_plane(spare, kk + 1);
@@ -906,7 +915,7 @@ __global__ __launch_bounds__(kPairBlock, sizeof(T) == 2 ? 4 : 2) void k_smooth_p
       else lds_ld4(&lx[buf][os], vs);
       if (edge_n) cur.edge.get(vn, alpha);
       else lds_ld4(&lx[buf][on], vn);
-      xm.get(vd, alpha);
+      lds_ld4(&lx[buf ^ 1][li], vd);  // this run on plane kk - 1
       cur.xn.get(vu, alpha);
       const float wv = lx[buf][ow];
       const float ev = lx[buf][oe];
