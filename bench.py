@@ -523,7 +523,7 @@ def profile_kernel(args, dev, torch):
     specs = batched_specs(args.instances, args.seed)
     cells = sum(math.prod(s[0]) for s in specs)
     mg = MGBatch([s[0] for s in specs], storage=args.storage, tol=1e-8, device=dev,
-                 outer=args.outer, fuse=not args.no_fuse)
+                 outer=args.outer, fuse=not args.no_fuse, **_fuse_kw(args))
     load_batch(mg, specs, dev)
     mg.build_faces()
     mg.setup()
@@ -677,6 +677,10 @@ def traffic_driver(args):
                       "copy_ms": round(cp_ms, 4)}), flush=True)
 
 
+def _fuse_kw(args):
+    return {} if args.fuse_levels is None else {"fuse_levels": args.fuse_levels}
+
+
 def _hip_timed(hip, fn, reps):
     """Mean ms of ``reps`` calls of fn after one warm call (HIP events on the
     null stream)."""
@@ -783,6 +787,8 @@ def main():
                     help="solve the batch as this many instance groups on their own streams")
     ap.add_argument("--no-fuse", action="store_true",
                     help="every V-cycle pass as its own launch (no fused pass pairs)")
+    ap.add_argument("--fuse-levels", type=int, default=None,
+                    help="finest levels that run fused pass pairs (default: the solver's)")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep 256-thread blocks (skip the launch-shape tuner)")
     ap.add_argument("--profile-kernel",
@@ -832,7 +838,7 @@ def main():
     log(f"bench: {len(specs)} instances, {cells / 1e6:.1f} M fine cells, storage {args.storage}")
     mg = MGBatch(shapes, storage=args.storage, tol=1e-8, maxit=300, device=dev,
                  iters_per_graph=args.iters_per_graph, shrink=args.shrink, outer=args.outer,
-                 fuse=not args.no_fuse, streams=args.streams)
+                 fuse=not args.no_fuse, streams=args.streams, **_fuse_kw(args))
     load_batch(mg, specs, dev)
     tuning = None
     if not args.no_tune:
