@@ -593,7 +593,8 @@ def traffic_driver(args):
     """Torch-free driver for hardware-counter runs of the dominant kernel,
     k_fcg_update (the _sweep form the solver launches, BF16 storage), or with
     ``--traffic-kernel sr`` of the fine level's fused pre-smoothing launch
-    k_smooth_pair<bf16, false> (Jacobi sweep + residual restriction).
+    k_smooth_pair<bf16, false> (Jacobi sweep + residual restriction), or with
+    ``--traffic-kernel dir`` of the direction pass k_fcg_dir<bf16>.
 
     The work tables and buffers of the headline batch (``--instances`` shapes
     from ``--seed``, the same draw as the solve workload) or, with
@@ -649,16 +650,34 @@ def traffic_driver(args):
         r["xin"], r["xout"], r["rlow"] = malloc(2 * N), malloc(2 * N), malloc(2 * N)
         r["ec"] = malloc(2 * N)
         r["x64"], r["r64"] = malloc(8 * ((N + 511) // 512 * 512)), malloc(8 * ((N + 511) // 512 * 512))
+        if args.traffic_kernel == "dir":
+            r["b"] = malloc(2 * N)   # z; the old direction is xin, the new one xout
     tasks = upload(t.view(np.uint8))
     blk = upload(np.repeat(np.arange(n, dtype=np.uint16), nblk))
     sc = np.zeros(n, dtype=L.SCALARS_DTYPE)
     sc["active"], sc["alpha"], sc["slot0"], sc["nslot"] = 1, 0.5, blk0, nblk
     sc["xlag"], sc["alpha_prev"] = args.traffic_lag, 0.25
+    sc["beta"] = 0.5
     scp = upload(sc.view(np.uint8))
     dots = malloc(8 * total)
     hip.hipDeviceSynchronize()
 
     timed = lambda fn, reps: _hip_timed(hip, fn, reps)  # noqa: E731
+    if args.traffic_kernel == "dir":
+        fdir = lambda: L.call("hfm_fcg_dir", tasks, blk, total, scp, L.STORE["bf16"],  # noqa: E731
+                              dots, None)
+        ms = timed(fdir, args.reps)
+        nbytes, cp_ms = _copy_reference(hip, malloc, args.reps)
+        N = int(cells.sum())
+        rd, wr = 16 * N, 2 * N    # z, p_old, three FP32 faces; p written
+        print(json.dumps({"kernel": "k_fcg_dir<bf16>", "workload": what, "cells": N,
+                          "blocks": total, "launches": args.reps + 1,
+                          "alg_bytes_per_launch": rd + wr, "alg_read_bytes": rd,
+                          "alg_write_bytes": wr, "launch_ms": round(ms, 4),
+                          "achieved_GBs": round((rd + wr) / (ms * 1e-3) / 1e9, 1),
+                          "copy_bytes_each_way": nbytes, "copy_launches": args.reps + 1,
+                          "copy_ms": round(cp_ms, 4)}), flush=True)
+        return
     upd = lambda: L.call("hfm_fcg_update_sweep", tasks, blk, total, scp, L.STORE["bf16"],  # noqa: E731
                          L.STORE["bf16"], C.c_float(2.0 / 3.0), dots, None)
     ms = timed(upd, args.reps)
@@ -814,8 +833,9 @@ def main():
     ap.add_argument("--traffic-edge", type=int, default=0)
     ap.add_argument("--traffic-lag", type=int, choices=(0, 1), default=1,
                     help="update variant: 1 with the deferred solution update, 0 without")
-    ap.add_argument("--traffic-kernel", choices=("update", "sr"), default="update",
-                    help="update: k_fcg_update; sr: the fine level's fused pre-smoothing launch")
+    ap.add_argument("--traffic-kernel", choices=("update", "sr", "dir"), default="update",
+                    help="update: k_fcg_update; sr: the fine level's fused pre-smoothing "
+                         "launch; dir: k_fcg_dir")
     args = ap.parse_args()
     if args.traffic:
         return traffic_driver(args)
