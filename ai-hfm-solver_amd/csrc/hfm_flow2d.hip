@@ -62,15 +62,19 @@ __device__ __forceinline__ double wg_sum(double a, double* red) {
   return a;
 }
 
+// max that keeps a NaN (fmaxf drops it): a NaN velocity must reach the
+// stopping test of the step
+__device__ __forceinline__ float max_nan(float a, float b) { return (a != a || a > b) ? a : b; }
+
 __device__ __forceinline__ float wg_max(float a, float* redf) {
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) a = fmaxf(a, __shfl_down(a, off, 64));
+  for (int off = 32; off > 0; off >>= 1) a = max_nan(a, __shfl_down(a, off, 64));
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   __syncthreads();
   if (lane == 0) redf[wid] = a;
   __syncthreads();
   float m = redf[0];
-  for (int w = 1; w < kFWv; ++w) m = fmaxf(m, redf[w]);
+  for (int w = 1; w < kFWv; ++w) m = max_nan(m, redf[w]);
   return m;
 }
 
@@ -79,7 +83,7 @@ __device__ __forceinline__ float wg_max(float a, float* redf) {
 // arrival counter, a failure word, then two slots of per-workgroup partial
 // sums (double pairs) and maxima (float pairs) for the team reductions.
 constexpr int kTeamMax = 64;
-constexpr long kSpinLimit = 1L << 26;  // ~1 s of polling: a barrier that never fills
+constexpr double kBarrierTimeoutS = 1.0;  // wall-clock bound of one team barrier wait
 
 struct Team {
   static constexpr bool kTeam = true;
@@ -90,6 +94,7 @@ struct Team {
   int G, rank;
   unsigned nbar;  // barriers passed (thread 0's count)
   int slot;       // reduction slot to use next
+  uint64_t wait_ticks;  // barrier timeout in ticks of the constant-rate wall clock
   __device__ __forceinline__ int gt() const { return rank * kFT + int(threadIdx.x); }
   __device__ __forceinline__ int stride() const { return G * kFT; }
   __device__ __forceinline__ bool lead() const { return rank == 0 && threadIdx.x == 0; }
@@ -128,11 +133,11 @@ __device__ __forceinline__ void team_sync(Team& t) {
     const unsigned target = t.nbar * unsigned(t.G);
     __threadfence();
     atomicAdd(t.cnt, 1u);
-    long spins = 0;
+    const uint64_t t0 = wall_clock64();
     while (__hip_atomic_load(t.cnt, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target &&
            __hip_atomic_load(t.fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) {
       __builtin_amdgcn_s_sleep(1);
-      if (++spins > kSpinLimit) {
+      if (wall_clock64() - t0 > t.wait_ticks) {
         atomicExch(t.fail, 1u);
         break;
       }
@@ -182,8 +187,8 @@ __device__ __forceinline__ void team_max2(Team& t, float& a, float& b, float* re
   team_sync(t);
   float ma = 0.0f, mb = 0.0f;
   for (int q = 0; q < t.G; ++q) {
-    ma = fmaxf(ma, __hip_atomic_load(slot + 2 * q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-    mb = fmaxf(mb, __hip_atomic_load(slot + 2 * q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    ma = max_nan(ma, __hip_atomic_load(slot + 2 * q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    mb = max_nan(mb, __hip_atomic_load(slot + 2 * q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
   }
   a = ma;
   b = mb;
@@ -505,7 +510,14 @@ __device__ int substep(TM& tm, const hfm_flow2d_task& T, const hfm_flow2d_params
   int status = HFM_MG_RUNNING;
   int it = 0;
   double rel = bb > 0.0 ? sqrt(rr / bb) : 0.0;
-  if (bb == 0.0 || rel <= P.tol) status = HFM_MG_CONVERGED;
+  // a non-finite right-hand side or residual (NaN or Inf fields) is a
+  // breakdown, never "converged with zero iterations"
+  if (!isfinite(bb) || !isfinite(rr)) {
+    status = HFM_MG_BREAKDOWN;
+    rel = rr / bb;
+  } else if (bb == 0.0 || rel <= P.tol) {
+    status = HFM_MG_CONVERGED;
+  }
   double rz_old = 0.0;
   while (status == HFM_MG_RUNNING) {
     if (it >= P.maxit) {
@@ -613,14 +625,14 @@ __device__ __forceinline__ void flow_steps(TM& tm, const hfm_flow2d_task& T,
     const gf* v = (const gf*)T.v;
     cells<F2>(tm, NN, [&](int c) { return F2{fabsf(u[c]), fabsf(v[c])}; },
               [&](int, F2 a) {
-                umax = fmaxf(umax, a.a);
-                vmax = fmaxf(vmax, a.b);
+                umax = max_nan(umax, a.a);
+                vmax = max_nan(vmax, a.b);
               });
     team_max2(tm, umax, vmax, redf);
     const float q = (umax + vmax) * P.cfl_coef;
-    // NaN or too fast: stop this flow
+    // NaN (a breakdown) or too fast (more substeps than allowed): stop this flow
     if (!(q <= float(P.max_sub))) {
-      if (tm.lead()) T.stats[HFM_FLOW_ST_STATUS] = HFM_FLOW_CFL;
+      if (tm.lead()) T.stats[HFM_FLOW_ST_STATUS] = q != q ? HFM_MG_BREAKDOWN : HFM_FLOW_CFL;
       team_finish(tm, T);
       return;
     }
@@ -654,7 +666,7 @@ __global__ __launch_bounds__(kFT) void k_flow2d(const hfm_flow2d_task* __restric
 __global__ __launch_bounds__(kFT) void k_flow2d_team(const hfm_flow2d_task* __restrict__ tasks,
                                                     hfm_flow2d_params P,
                                                     const double* __restrict__ pinv, int nsteps,
-                                                    int flags) {
+                                                    int flags, uint64_t wait_ticks) {
   __shared__ double red[2 * kFWv];
   __shared__ float redf[kFWv];
   const int G = P.team;
@@ -665,6 +677,7 @@ __global__ __launch_bounds__(kFT) void k_flow2d_team(const hfm_flow2d_task* __re
   tm.rank = m % G;
   tm.nbar = 0;
   tm.slot = 0;
+  tm.wait_ticks = wait_ticks;
   unsigned* w = (unsigned*)T.team;
   tm.cnt = w;
   tm.fail = w + 1;
@@ -708,11 +721,20 @@ extern "C" int hfm_flow2d_steps(const hfm_flow2d_task* tasks, int64_t nflows,
                        P, pinv, nsteps, flags);
     return launch_status("hfm_flow2d_steps");
   }
+  // the barrier timeout in ticks of the device's constant-rate wall clock
+  int dev = 0, rate_khz = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess ||
+      rate_khz <= 0) {
+    set_error("hfm_flow2d_steps: cannot read the device wall-clock rate");
+    return HFM_EDEVICE;
+  }
   const hfm_flow2d_task* ta = tasks;
   hfm_flow2d_params pa = P;
   const double* pi = pinv;
   int ns = nsteps, fl = flags;
-  void* args[] = {(void*)&ta, (void*)&pa, (void*)&pi, (void*)&ns, (void*)&fl};
+  uint64_t wt = uint64_t(double(rate_khz) * 1000.0 * kBarrierTimeoutS);
+  void* args[] = {(void*)&ta, (void*)&pa, (void*)&pi, (void*)&ns, (void*)&fl, (void*)&wt};
   const hipError_t e =
       hipLaunchCooperativeKernel((const void*)k_flow2d_team, dim3(unsigned(nflows * P.team)),
                                  dim3(kFT),

@@ -363,9 +363,22 @@ __global__ __launch_bounds__(kBX* kBY) void k_prolong_jacobi_col(const float* __
   }
 }
 
-template <int TAPS>
+// The correction w is a field of the fine grid, or (PROL) the coarse field
+// wc read through the nearest prolongation: w(k, j, i) = wc(k >> sz, j >> sy,
+// i >> sx), which is bitwise the prolonged field without writing it.
+struct CorrW {
+  const float* w;
+  Grid3 gc;
+  int sz, sy, sx;
+  template <bool PROL>
+  __device__ __forceinline__ float at(const Grid3& g, int kk, int jj, int ii) const {
+    return PROL ? w[gc.idx(kk >> sz, jj >> sy, ii >> sx)] : w[g.idx(kk, jj, ii)];
+  }
+};
+
+template <int TAPS, bool PROL>
 __global__ __launch_bounds__(kBX* kBY) void k_correct_jacobi_col(const float* __restrict__ x,
-                                                                 const float* __restrict__ w,
+                                                                 CorrW cw,
                                                                  const float* __restrict__ b,
                                                                  float* __restrict__ y, Weights K,
                                                                  float d, Grid3 g, int flags) {
@@ -374,7 +387,7 @@ __global__ __launch_bounds__(kBX* kBY) void k_correct_jacobi_col(const float* __
   const bool zo = flags & HFM_CORRECT_ZERO_OUTLET;
   auto vat = [&](int kk, int jj, int ii) {
     const int64_t m = g.idx(kk, jj, ii);
-    return (zo && ii == g.W - 1) ? 0.0f : x[m] - w[m];
+    return (zo && ii == g.W - 1) ? 0.0f : x[m] - cw.template at<PROL>(g, kk, jj, ii);
   };
   Win27<2> win;
   win.start(g, c, vat);
@@ -394,9 +407,9 @@ __global__ __launch_bounds__(kBX* kBY) void k_correct_jacobi_col(const float* __
   }
 }
 
-template <bool VAR6, int TAPS>
+template <bool VAR6, int TAPS, bool PROL>
 __global__ __launch_bounds__(kBX* kBY) void k_correct_jacobi(const float* __restrict__ x,
-                                                             const float* __restrict__ w,
+                                                             CorrW cw,
                                                              const float* __restrict__ b,
                                                              float* __restrict__ y, Weights K,
                                                              float d, Var6 V,
@@ -407,7 +420,7 @@ __global__ __launch_bounds__(kBX* kBY) void k_correct_jacobi(const float* __rest
   const bool zo = flags & HFM_CORRECT_ZERO_OUTLET;
   auto vat = [&](int kk, int jj, int ii) {
     const int64_t m = g.idx(kk, jj, ii);
-    return (zo && ii == g.W - 1) ? 0.0f : x[m] - w[m];
+    return (zo && ii == g.W - 1) ? 0.0f : x[m] - cw.template at<PROL>(g, kk, jj, ii);
   };
   const float v = vat(k, j, i);
   if (VAR6) {
@@ -929,31 +942,69 @@ extern "C" int hfm_prolong_jacobi_f32(const float* wc, const float* r, float* y,
   return launch_status("hfm_prolong_jacobi_f32");
 }
 
-extern "C" int hfm_correct_jacobi_f32(const float* x, const float* wv, const float* b, float* y,
-                                      const float K[27], float d, const float* const m[6],
-                                      const float* tot, int64_t D, int64_t H, int64_t W, int flags,
-                                      void* stream) {
+namespace {
+int correct_jacobi(const char* what, const float* x, const CorrW& cw, bool prol, const float* b,
+                   float* y, const float K[27], float d, const float* const m[6],
+                   const float* tot, int64_t D, int64_t H, int64_t W, int flags, void* stream) {
   Weights w{};
   uint32_t mask = 0;
   Var6 V{};
   const bool var6 = m != nullptr;
-  if (!x || !wv || !b || !y || y == x || y == wv || !dims_ok(D, H, W) ||
+  if (!x || !cw.w || !b || !y || y == x || y == cw.w || !dims_ok(D, H, W) ||
+      (flags & ~HFM_CORRECT_ZERO_OUTLET) ||
       (var6 ? !make_var6(m, tot, V, true) : !make_weights(K, w, mask))) {
-    set_error("hfm_correct_jacobi_f32: bad arguments");
+    set_error("%s: bad arguments", what);
     return HFM_EINVAL;
   }
   const Grid3 g{int(D), int(H), int(W)};
   hipStream_t s = as_stream(stream);
-  if (var6)
-    hipLaunchKernelGGL((k_correct_jacobi<true, 7>), grid_for(g), dim3(kBX, kBY), 0, s, x, wv, b,
-                       y, w, d, V, g, flags);
-  else if (taps_of(mask) == 7)
-    hipLaunchKernelGGL((k_correct_jacobi_col<7>), grid_col<2>(g), dim3(kBX, kBY), 0, s, x, wv, b, y,
-                       w, d, g, flags);
-  else
-    hipLaunchKernelGGL((k_correct_jacobi_col<27>), grid_col<2>(g), dim3(kBX, kBY), 0, s, x, wv, b,
-                       y, w, d, g, flags);
-  return launch_status("hfm_correct_jacobi_f32");
+  if (prol) {
+    if (var6)
+      hipLaunchKernelGGL((k_correct_jacobi<true, 7, true>), grid_for(g), dim3(kBX, kBY), 0, s, x,
+                         cw, b, y, w, d, V, g, flags);
+    else if (taps_of(mask) == 7)
+      hipLaunchKernelGGL((k_correct_jacobi_col<7, true>), grid_col<2>(g), dim3(kBX, kBY), 0, s, x,
+                         cw, b, y, w, d, g, flags);
+    else
+      hipLaunchKernelGGL((k_correct_jacobi_col<27, true>), grid_col<2>(g), dim3(kBX, kBY), 0, s,
+                         x, cw, b, y, w, d, g, flags);
+  } else {
+    if (var6)
+      hipLaunchKernelGGL((k_correct_jacobi<true, 7, false>), grid_for(g), dim3(kBX, kBY), 0, s, x,
+                         cw, b, y, w, d, V, g, flags);
+    else if (taps_of(mask) == 7)
+      hipLaunchKernelGGL((k_correct_jacobi_col<7, false>), grid_col<2>(g), dim3(kBX, kBY), 0, s,
+                         x, cw, b, y, w, d, g, flags);
+    else
+      hipLaunchKernelGGL((k_correct_jacobi_col<27, false>), grid_col<2>(g), dim3(kBX, kBY), 0, s,
+                         x, cw, b, y, w, d, g, flags);
+  }
+  return launch_status(what);
+}
+}  // namespace
+
+extern "C" int hfm_correct_jacobi_f32(const float* x, const float* wv, const float* b, float* y,
+                                      const float K[27], float d, const float* const m[6],
+                                      const float* tot, int64_t D, int64_t H, int64_t W, int flags,
+                                      void* stream) {
+  CorrW cw{wv, Grid3{int(D), int(H), int(W)}, 0, 0, 0};
+  return correct_jacobi("hfm_correct_jacobi_f32", x, cw, false, b, y, K, d, m, tot, D, H, W,
+                        flags, stream);
+}
+
+extern "C" int hfm_prolong_correct_jacobi_f32(const float* x, const float* wc, const float* b,
+                                              float* y, const float K[27], float d,
+                                              const float* const m[6], const float* tot,
+                                              int64_t D, int64_t H, int64_t W, int fz, int fy,
+                                              int fx, int flags, void* stream) {
+  if (!factors_ok(fz, fy, fx) || !dims_ok(D, H, W) || D % fz || H % fy || W % fx) {
+    set_error("hfm_prolong_correct_jacobi_f32: bad arguments (D,H,W)=(%lld,%lld,%lld) f=(%d,%d,%d)",
+              (long long)D, (long long)H, (long long)W, fz, fy, fx);
+    return HFM_EINVAL;
+  }
+  CorrW cw{wc, Grid3{int(D / fz), int(H / fy), int(W / fx)}, fz - 1, fy - 1, fx - 1};
+  return correct_jacobi("hfm_prolong_correct_jacobi_f32", x, cw, true, b, y, K, d, m, tot, D, H,
+                        W, flags, stream);
 }
 
 extern "C" int hfm_advect27_f32(const float* const vel[3], const float* const* phi,

@@ -68,6 +68,8 @@ SIGNATURES = {
                                       _int, _int, _int, _vp]),
     "hfm_correct_jacobi_f32": (_int, [_vp, _vp, _vp, _vp, _vp, _f32, _vp, _vp, _i64, _i64, _i64,
                                       _int, _vp]),
+    "hfm_prolong_correct_jacobi_f32": (_int, [_vp, _vp, _vp, _vp, _vp, _f32, _vp, _vp, _i64, _i64,
+                                              _i64, _int, _int, _int, _int, _vp]),
     "hfm_advect27_f32": (_int, [_vp, _vp, _vp, _vp, _int, _vp, _f32, _i64, _i64, _i64, _vp]),
     "hfm_advect27_dis_f32": (_int, [_vp, _vp, _vp, _vp, _vp, _int, _vp, _f32, _i64, _i64, _i64,
                                     _vp]),

@@ -276,19 +276,30 @@ class SawtoothMG:
     ``ops[l]`` is a FixedConv (constant weights, divisor = its centre or
     ``divisors[l]``) or an AOperatorVar6 for level l (fine first).  One call
     of ``cycle`` does: r = A x - b; restrict r to every level; from w = 0 do
-    ``coarsest_sweeps`` sweeps on the coarsest level; at each finer level
-    prolong and do one sweep against that level's r; ``x -= w``; one fine
-    sweep on A x = b; optional boundary fill ``bc``.  Prolongation+sweep and
-    correction+sweep are single fused kernels.
+    ``coarsest_sweeps`` sweeps (default ``j_itr``) on the coarsest level; at
+    each finer level prolong and do ``j_itr`` sweeps against that level's r;
+    ``x -= w``; one fine sweep on A x = b; optional boundary fill ``bc``.
+
+    ``fine_sweep=False`` is the urban drivers' form: the finest level takes
+    the prolonged coarse correction with no sweep of its own
+    (UM/AI_HFM_v3.ipynb:2610-2611, UM/INHALE_demo.ipynb:13454-13458,
+    ARTs/naiver_stokes_3D.py:696-699).  Prolongation + first sweep,
+    correction + fine sweep and (without the fine-level sweep) prolongation +
+    correction + fine sweep are single fused kernels.
     """
 
-    def __init__(self, ops: List, divisors: Optional[List[float]] = None, coarsest_sweeps: int = 1,
-                 bc: Optional[Callable] = None, zero_outlet: bool = False):
+    def __init__(self, ops: List, divisors: Optional[List[float]] = None,
+                 coarsest_sweeps: Optional[int] = None, bc: Optional[Callable] = None,
+                 zero_outlet: bool = False, fine_sweep: bool = True, j_itr: int = 1):
         self.ops = ops
         self.divisors = divisors
-        self.coarsest_sweeps = int(coarsest_sweeps)
+        self.j_itr = int(j_itr)
+        if self.j_itr < 1:
+            raise ValueError("j_itr must be at least 1")
+        self.coarsest_sweeps = self.j_itr if coarsest_sweeps is None else int(coarsest_sweeps)
         self.bc = bc
         self.zero_outlet = bool(zero_outlet)
+        self.fine_sweep = bool(fine_sweep)
 
     def _d(self, lev):
         op = self.ops[lev]
@@ -298,60 +309,77 @@ class SawtoothMG:
             return float(np.float32(self.divisors[lev]))
         return op.center
 
-    def _prolong_jacobi(self, lev, wc, r):
+    def _sweep(self, lev, w, r):
         op = self.ops[lev]
+        return op.jacobi(w, r) if isinstance(op, AOperatorVar6) else op.jacobi(w, r, self._d(lev))
+
+    def _op_args(self, lev):
+        op = self.ops[lev]
+        if isinstance(op, AOperatorVar6):
+            return None, 0.0, op._arr, op.tot.data_ptr()
+        return _kptr(op.K), self._d(lev), None, None
+
+    def _prolong_jacobi(self, lev, wc, r):
         r = _prep(r, "r")
         (D, H, W), nd = _spatial(r)
         fz, fy, fx = _factors(nd)
         if wc.numel() * fz * fy * fx != D * H * W:
             raise ValueError(f"coarse correction of {wc.numel()} cells does not match level {(D, H, W)}")
         y = torch.empty_like(r)
-        s = L.stream_handle(r.device)
-        if isinstance(op, AOperatorVar6):
-            L.call("hfm_prolong_jacobi_f32", wc.data_ptr(), r.data_ptr(), y.data_ptr(), None, 0.0,
-                   op._arr, op.tot.data_ptr(), D, H, W, fz, fy, fx, s)
-        else:
-            L.call("hfm_prolong_jacobi_f32", wc.data_ptr(), r.data_ptr(), y.data_ptr(),
-                   _kptr(op.K), self._d(lev), None, None, D, H, W, fz, fy, fx, s)
+        K, d, m, tot = self._op_args(lev)
+        L.call("hfm_prolong_jacobi_f32", wc.data_ptr(), r.data_ptr(), y.data_ptr(), K, d, m, tot,
+               D, H, W, fz, fy, fx, L.stream_handle(r.device))
         return y
 
-    def _correct_jacobi(self, x, w, b):
-        op = self.ops[0]
+    def _correct_jacobi(self, x, w, b, prolonged: bool):
+        """J(x - w) against b; with ``prolonged`` w is the next level's
+        correction, applied through the nearest prolongation in the kernel."""
         flags = L.CORRECT_ZERO_OUTLET if self.zero_outlet else 0
-        (D, H, W), _ = _spatial(x)
-        if w.shape != x.shape or b.shape != x.shape:
-            raise ValueError("x, w and b must share a shape")
+        (D, H, W), nd = _spatial(x)
+        if b.shape != x.shape:
+            raise ValueError("x and b must share a shape")
         y = torch.empty_like(x)
         s = L.stream_handle(x.device)
-        if isinstance(op, AOperatorVar6):
-            L.call("hfm_correct_jacobi_f32", x.data_ptr(), w.data_ptr(), b.data_ptr(), y.data_ptr(),
-                   None, 0.0, op._arr, op.tot.data_ptr(), D, H, W, flags, s)
+        K, d, m, tot = self._op_args(0)
+        if prolonged:
+            fz, fy, fx = _factors(nd)
+            if w.numel() * fz * fy * fx != D * H * W:
+                raise ValueError(f"coarse correction of {w.numel()} cells does not match {(D, H, W)}")
+            L.call("hfm_prolong_correct_jacobi_f32", x.data_ptr(), w.data_ptr(), b.data_ptr(),
+                   y.data_ptr(), K, d, m, tot, D, H, W, fz, fy, fx, flags, s)
         else:
+            if w.shape != x.shape:
+                raise ValueError("x, w and b must share a shape")
             L.call("hfm_correct_jacobi_f32", x.data_ptr(), w.data_ptr(), b.data_ptr(), y.data_ptr(),
-                   _kptr(op.K), self._d(0), None, None, D, H, W, flags, s)
+                   K, d, m, tot, D, H, W, flags, s)
         return y
 
     def cycle(self, values, b, record: Optional[dict] = None):
         """One cycle; with ``zero_outlet`` the fine residual and the corrected
-        field are zeroed on the x = W-1 plane (UM/AI_HFM_v3.ipynb:2574,2611-2612)."""
+        field are zeroed on the x = W-1 plane (UM/AI_HFM_v3.ipynb:2574,2611-2612).
+        ``record[l]`` receives level l's correction; without the fine-level
+        sweep the finest correction (the prolonged ``record[1]``) is never
+        formed, and ``record[0]`` is left unset."""
         values, b = _prep(values, "values"), _prep(b, "b")
         r = self.ops[0].residual(values, b, zero_outlet=self.zero_outlet) if isinstance(
             self.ops[0], FixedConv) else self.ops[0].residual(values, b)
+        nlev = len(self.ops)
         rs = [r]
-        for _ in range(1, len(self.ops)):
+        for _ in range(1, nlev):
             rs.append(restrict_mean(rs[-1]))
         w = torch.zeros_like(rs[-1])
-        last = self.ops[-1]
         for _ in range(self.coarsest_sweeps):
-            w = last.jacobi(w, rs[-1]) if isinstance(last, AOperatorVar6) else \
-                last.jacobi(w, rs[-1], self._d(len(self.ops) - 1))
+            w = self._sweep(nlev - 1, w, rs[-1])
         if record is not None:
-            record[len(self.ops) - 1] = w
-        for lev in range(len(self.ops) - 2, -1, -1):
+            record[nlev - 1] = w
+        low = 0 if (self.fine_sweep or nlev == 1) else 1
+        for lev in range(nlev - 2, low - 1, -1):
             w = self._prolong_jacobi(lev, w, rs[lev])
+            for _ in range(self.j_itr - 1):
+                w = self._sweep(lev, w, rs[lev])
             if record is not None:
                 record[lev] = w
-        values = self._correct_jacobi(values, w, b)
+        values = self._correct_jacobi(values, w, b, prolonged=low == 1)
         if self.bc is not None:
             values = self.bc(values)
         return values

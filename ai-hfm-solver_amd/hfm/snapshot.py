@@ -105,26 +105,28 @@ class SnapshotWriter:
             return False
         if self._error is not None:
             raise RuntimeError("snapshot writer failed") from self._error
-        pool = self._free.get()        # waits while `depth` snapshots are in flight
-        staged = []
-        devices = set()
-        for name, x in fields.items():
-            g = _grid_view(x).detach()
-            key = (name, tuple(g.shape), g.dtype)
-            host = pool.get(key)
-            if host is None:
-                host = torch.empty(g.shape, dtype=g.dtype, pin_memory=g.is_cuda)
-                pool[key] = host
-            host.copy_(g, non_blocking=g.is_cuda)
-            staged.append((name, host))
-            if g.is_cuda:
-                devices.add(g.device)
+        grids = [(name, _grid_view(x).detach()) for name, x in fields.items()]
+        devices = {g.device for _, g in grids if g.is_cuda}
         if len(devices) > 1:
             raise ValueError("snapshot fields must share one device")
-        ev = None
-        if devices:
-            ev = torch.cuda.Event()
-            ev.record(torch.cuda.current_stream(devices.pop()))
+        pool = self._free.get()        # waits while `depth` snapshots are in flight
+        try:
+            staged = []
+            for name, g in grids:
+                key = (name, tuple(g.shape), g.dtype)
+                host = pool.get(key)
+                if host is None:
+                    host = torch.empty(g.shape, dtype=g.dtype, pin_memory=g.is_cuda)
+                    pool[key] = host
+                host.copy_(g, non_blocking=g.is_cuda)
+                staged.append((name, host))
+            ev = None
+            if devices:
+                ev = torch.cuda.Event()
+                ev.record(torch.cuda.current_stream(devices.pop()))
+        except BaseException:
+            self._free.put(pool)       # the pool goes back: later snapshots must not block
+            raise
         self._work.put((int(itime), ev, staged, pool))
         return True
 

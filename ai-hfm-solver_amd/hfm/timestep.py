@@ -11,7 +11,8 @@ script twin ARTs/naiver_stokes_3D.py:571-727) step for step:
   -> bluff body  f / (1 + dt sigma)
   -> u -= pu(p), v -= pv(p), w -= pw(p); velocity BCs
   -> b = -(Su(u) + Sv(v) + Sw(w)); source BC
-  -> multi_itr sawtooth cycles of the 27-point Poisson layer (outlet zeroed)
+  -> multi_itr sawtooth cycles of the 27-point Poisson layer (outlet zeroed;
+     the prolonged fine correction w_512 is applied with no fine-level sweep)
   -> pressure BCs; gradient correction; velocity BCs; bluff body
 
 Each arrow is one fused kernel launch (the three momentum components share
@@ -66,7 +67,11 @@ class UrbanFlow:
 
     def __init__(self, shape, sigma: Optional[torch.Tensor] = None, dt: float = 0.4,
                  dx: float = 1.0, Re: float = 1 / 20, ub: float = 1.0, multi_itr: int = 5,
-                 lscalar: bool = True, device=None):
+                 lscalar: bool = True, device=None, coarsest: int = 1, j_itr: int = 1):
+        """``coarsest``: the pressure hierarchy halves every axis until the
+        shortest is 1 (UM/AI_HFM_v3.ipynb:2571-2584) or 2 (the script twin's
+        2^3 coarsest level, ARTs/naiver_stokes_3D.py:648-662); ``j_itr``:
+        Jacobi sweeps per level below the finest (UM/AI_HFM_v3.ipynb:780)."""
         self.shape = tuple(int(v) for v in shape)
         if len(self.shape) != 3 or min(self.shape) < 2:
             raise ValueError("UrbanFlow needs a (D, H, W) box with every edge >= 2")
@@ -76,17 +81,20 @@ class UrbanFlow:
         self.k_mom = (k["dif"], *k["adv"])
         self.k_div = k["div"]
         self.grad = [Ly.FixedConv(K) for K in k["adv"]]
+        if int(coarsest) not in (1, 2):
+            raise ValueError("coarsest must be 1 or 2")
         shapes = [self.shape]
-        while min(shapes[-1]) > 1:
+        while min(shapes[-1]) > int(coarsest):
             shapes.append(tuple(n // 2 for n in shapes[-1]))
         for s_f, s_c in zip(shapes, shapes[1:]):
             if any(a != 2 * b for a, b in zip(s_f, s_c)):
                 raise ValueError(f"grid {self.shape} does not halve evenly down to its coarsest level")
-        self.mg = Ly.SawtoothMG([Ly.FixedConv(k["poisson"], s) for s in shapes], zero_outlet=True)
+        self.mg = Ly.SawtoothMG([Ly.FixedConv(k["poisson"], s) for s in shapes], zero_outlet=True,
+                                fine_sweep=False, j_itr=j_itr)
         if sigma is None:
             sigma = torch.zeros(self.shape, device=self.device)
         self.sigma = sigma.reshape(self.shape).to(device=self.device, dtype=torch.float32).contiguous()
-        self.last_w = None
+        self._last_wc = None    # the last cycle's correction one level below the finest
 
     def _bluff(self, u, v, w, alpha):
         return Ly.bluff_body(u, v, w, alpha, self.sigma, dt=self.dt)
@@ -124,10 +132,7 @@ class UrbanFlow:
         u, v, w = self._gradient(u, v, w, p)
         u, v, w = self._bcv_(u, v, w)
         b = Ly.conv_sum((u, v, w), self.k_div, scale=-1.0, source_bc=True)
-        rec = {}
-        for _ in range(self.multi_itr):
-            p = self.mg.cycle(p, b, record=rec)
-        self.last_w = rec.get(0)
+        p = self._pressure_cycles(p, b)
         p = Ly.boundary_condition_pressure(p)
         u, v, w = self._gradient(u, v, w, p)
         u, v, w = self._bcv_(u, v, w)
@@ -146,9 +151,27 @@ class UrbanFlow:
                 writer.maybe_save(itime + nrestart, u=u, v=v, w=w, p=p, **extra)
         return u, v, w, p, alpha
 
+    def _pressure_cycles(self, p, b):
+        rec = {}
+        for _ in range(self.multi_itr):
+            p = self.mg.cycle(p, b, record=rec)
+        # the finest correction is the nearest prolongation of level 1's (or,
+        # on a one-level hierarchy, level 0's own)
+        self._last_wc = rec.get(1 if len(self.mg.ops) > 1 else 0)
+        return p
+
+    @property
+    def last_w(self):
+        """The last cycle's fine-level correction ``w_512 = CNN3D_prol_256(w_256)``
+        (formed on request; the step applies it through the fused kernel)."""
+        if self._last_wc is None or len(self.mg.ops) == 1:
+            return self._last_wc
+        return Ly.prolong_nearest(self._last_wc)
+
     def diverged(self, limit: float = 0.2) -> bool:
-        """The reference's guard: max |w_fine| of the last cycle above ``limit``."""
-        return self.last_w is not None and float(self.last_w.abs().max()) > limit
+        """The reference's guard ``np.max(np.abs(w_512)) > 0.2`` (UM/AI_HFM_v3.ipynb:2625):
+        w_512 repeats the coarse correction's values, so its maximum is read there."""
+        return self._last_wc is not None and float(self._last_wc.abs().max()) > limit
 
 
 class InhaleFlow(UrbanFlow):
@@ -161,9 +184,10 @@ class InhaleFlow(UrbanFlow):
 
     def __init__(self, shape, sigma: Optional[torch.Tensor] = None, dt: float = 0.5,
                  dx: float = 1.0, Re: float = 1 / 20, ub: float = 1.0, multi_itr: int = 5,
-                 eps_k: float = 1e-3, lscalar: bool = False, device=None):
+                 eps_k: float = 1e-3, lscalar: bool = False, device=None, coarsest: int = 1,
+                 j_itr: int = 1):
         super().__init__(shape, sigma=sigma, dt=dt, dx=dx, Re=Re, ub=ub, multi_itr=multi_itr,
-                         lscalar=lscalar, device=device)
+                         lscalar=lscalar, device=device, coarsest=coarsest, j_itr=j_itr)
         self.eps_k = float(eps_k)
 
     def _pg(self, u, v, w):
@@ -189,10 +213,7 @@ class InhaleFlow(UrbanFlow):
         u, v, w = self._gradient(u, v, w, p)
         u, v, w = self._bcv_(u, v, w)
         b = Ly.conv_sum((u, v, w), self.k_div, scale=-1.0, source_bc=True)
-        rec = {}
-        for _ in range(self.multi_itr):
-            p = self.mg.cycle(p, b, record=rec)
-        self.last_w = rec.get(0)
+        p = self._pressure_cycles(p, b)
         p = Ly.boundary_condition_pressure(p)
         u, v, w = self._gradient(u, v, w, p)
         u, v, w = self._bcv_(u, v, w)
@@ -200,7 +221,8 @@ class InhaleFlow(UrbanFlow):
         return tuple(t.reshape(shape) for t in (u, v, w, p, alpha))
 
     def diverged(self, limit: float = 0.5) -> bool:
-        """The demo's guard: max |w_fine| of the last cycle above ``nsafe`` = 0.5."""
+        """The demo's guard: max |w_fine| of the last cycle above ``nsafe`` = 0.5
+        (UM/INHALE_demo.ipynb:13478)."""
         return super().diverged(limit)
 
 

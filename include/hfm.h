@@ -120,6 +120,19 @@ int hfm_correct_jacobi_f32(const float* x, const float* w, const float* b, float
                            const float K[27], float d, const float* const m[6], const float* tot,
                            int64_t D, int64_t H, int64_t W, int flags, void* stream);
 
+/* y = J(x - P wc) against b, P = nearest repeat by (fz, fy, fx) of the coarse
+ * correction wc (D/fz, H/fy, W/fx): the urban drivers' fine-level step, which
+ * applies the prolonged correction with no fine-level sweep of it --
+ * "w_512 = CNN3D_prol_256(w_256); values_p = values_p - w_512; outlet = 0;
+ * values_p = values_p - CNN3D_A_512(values_p)/d + b/d" (UM/AI_HFM_v3.ipynb:2610-2613,
+ * UM/INHALE_demo.ipynb:13454-13458, ARTs/naiver_stokes_3D.py:696-703).
+ * Bitwise hfm_prolong_nearest_f32 followed by hfm_correct_jacobi_f32; flags
+ * as there. */
+int hfm_prolong_correct_jacobi_f32(const float* x, const float* wc, const float* b, float* y,
+                                   const float K[27], float d, const float* const m[6],
+                                   const float* tot, int64_t D, int64_t H, int64_t W, int fz,
+                                   int fy, int fx, int flags, void* stream);
+
 /* Explicit advection-diffusion step of nf (1 or 3) fields by the velocity
  * vel = (u, v, w):  out_q = f * (((dif(phi_q) - u ax(phi_q)) - v ay(phi_q)) -
  * w az(phi_q)) + base_q, with K = {dif, ax, ay, az} 27-tap kernels
@@ -397,7 +410,8 @@ int hfm_mg_coarse_inverse(const hfm_mg_task* tasks, int64_t ntasks, void* stream
  *        (the pseudo-inverse of the coarsest level's operator, n_c^2 x n_c^2,
  *        n_c = N >> (nlev-1), n_c^2 <= 1024);
  *   u = u* - (phi - phi_w)/h,  v = v* - (phi - phi_s)/h.
- * Stopping rule per substep: ||r||_2 <= tol ||b||_2.
+ * Stopping rule per substep: ||r||_2 <= tol ||b||_2.  A non-finite velocity,
+ * right-hand side or residual stops the flow with HFM_MG_BREAKDOWN.
  *
  * Buffers per flow (device): u, v, uh, vh, z, p: N*N floats; phi, r: N*N
  * doubles; mgx, mgy, mgb: sum over levels of (N >> l)^2 floats; fu: N floats;
@@ -412,7 +426,8 @@ int hfm_mg_coarse_inverse(const hfm_mg_task* tasks, int64_t ntasks, void* stream
 #define HFM_FLOW_ST_REL_MAX 6    /* largest final relative residual            */
 #define HFM_FLOW_ST_NSUB 7       /* substeps of the last step                  */
 #define HFM_FLOW_CFL 4           /* status: more than max_sub substeps needed */
-#define HFM_FLOW_TEAM_TIMEOUT 5  /* status: a team barrier timed out (fields void) */
+#define HFM_FLOW_TEAM_TIMEOUT 5  /* status: a team barrier waited more than 1 s of
+                                    wall-clock time (fields void) */
 #define HFM_FLOW_TEAM_BYTES 4096 /* team scratch per flow */
 #define HFM_FLOW_PROJECT_ONLY 1  /* flag: skip the momentum stages (project u, v) */
 

@@ -204,21 +204,33 @@ def restrict_var6(mats: dict) -> dict:
 
 
 def sawtooth_cycle(values, b, ops: List[Callable], diags: list, restrict=restrict_mean,
-                   prolong=prolong_nearest, coarsest_sweeps: int = 1, bc=None,
-                   record: Optional[dict] = None, zero_outlet: bool = False):
+                   prolong=prolong_nearest, coarsest_sweeps: Optional[int] = None, bc=None,
+                   record: Optional[dict] = None, zero_outlet: bool = False,
+                   fine_sweep: bool = True, j_itr: int = 1):
     """One reference "sawtooth" multigrid cycle.
 
     ``…7point.ipynb:402-454`` / ``BGS#c42`` / ``ARTn/Parallel_multigrid_2D.ipynb#c20``:
     r = A(x) - b at the finest level, restrict r down the whole hierarchy,
     start from w = 0 at the coarsest level and do ``coarsest_sweeps`` sweeps
-    there (``…_BC_Rec.ipynb:525`` does 2), then at each finer level prolong and
-    do one sweep against that level's restricted residual, then
-    ``x -= w``, one fine sweep on ``A x = b`` and the optional boundary fill.
-    ``ops[l]`` / ``diags[l]`` are the level-l operator and divisor, fine first.
+    there (default ``j_itr``; ``…_BC_Rec.ipynb:525`` does 2), then at each
+    finer level prolong and do ``j_itr`` sweeps against that level's
+    restricted residual, then ``x -= w``, one fine sweep on ``A x = b`` and the
+    optional boundary fill.  ``ops[l]`` / ``diags[l]`` are the level-l
+    operator and divisor, fine first.
+
+    ``fine_sweep=False`` is the urban drivers' form: the finest level's
+    correction is the prolonged coarse correction with no sweep of its own
+    (``UM/AI_HFM_v3.ipynb:2610-2611``: ``w_512 = CNN3D_prol_256(w_256);
+    values_p = values_p - w_512``; ``UM/INHALE_demo.ipynb:13454-13458``, where
+    the fine sweep is commented out; ``ARTs/naiver_stokes_3D.py:696-699``).
     ``zero_outlet``: r[..., nx-1] = 0 after the residual and values[..., nx-1] = 0
     after the correction (``UM/AI_HFM_v3.ipynb:2574,2611-2612``).
+    ``record[l]`` receives level l's correction (``record[0]`` the one that is
+    subtracted from ``values``).
     """
     nlev = len(ops)
+    if coarsest_sweeps is None:
+        coarsest_sweeps = j_itr
     r = ops[0](values) - b
     if zero_outlet:
         r = zero_outlet_plane(r)
@@ -235,7 +247,10 @@ def sawtooth_cycle(values, b, ops: List[Callable], diags: list, restrict=restric
         if record is not None:
             record[("prolonged", lev)] = wp
             record[("A_prolonged", lev)] = ops[lev](wp)
-        w = jacobi_ref(wp, rs[lev], ops[lev], diags[lev])
+        w = wp
+        if lev > 0 or fine_sweep:
+            for _ in range(j_itr):
+                w = jacobi_ref(w, rs[lev], ops[lev], diags[lev])
         if record is not None:
             record[lev] = w
     values = values - w
@@ -406,30 +421,53 @@ def implicit_ad1d_kernel(dt=10.0, dx=1.0, Dx=0.1, ub=1.0):
                      -Dx * dt / dx ** 2])
 
 
-def urban_levels(shape):
-    """Reference-mode pressure hierarchy: halve every axis until the shortest is 1."""
+def urban_levels(shape, coarsest: int = 1):
+    """Reference-mode pressure hierarchy: halve every axis until the shortest
+    is ``coarsest`` (1: ``UM/AI_HFM_v3.ipynb:2571-2584``, 512^3 down to 1^3;
+    2: the script twin's ``w_2 = tf.zeros([1, 2, 2, 2, 1])`` coarsest level,
+    ``ARTs/naiver_stokes_3D.py:648-662``)."""
     shapes = [tuple(shape)]
-    while min(shapes[-1]) > 1:
+    while min(shapes[-1]) > coarsest:
         shapes.append(tuple(n // 2 for n in shapes[-1]))
     return shapes
 
 
+def urban_pressure_cycles(p, b, multi_itr=5, dx=1.0, coarsest: int = 1, j_itr: int = 1,
+                          record: Optional[dict] = None):
+    """The urban drivers' ``multi_itr`` pressure cycles (``UM/AI_HFM_v3.ipynb:2570-2613``):
+    27-point Poisson layer w5 at every level with its centre as divisor, outlet
+    zeroing, ``j_itr`` sweeps per level below the finest and the prolonged
+    correction applied with no fine-level sweep (``fine_sweep=False``)."""
+    f32 = np.float32
+    w5 = poisson27_kernel(dx)
+    shapes = urban_levels(p.shape, coarsest)
+    C = lambda x: conv_same(x.astype(f32), w5)  # noqa: E731
+    ops = [C] * len(shapes)
+    d = f32(w5[1, 1, 1])
+    for _ in range(multi_itr):
+        p = sawtooth_cycle(p, b, ops, [d] * len(shapes), zero_outlet=True, record=record,
+                           fine_sweep=False, j_itr=j_itr)
+    return p
+
+
 def urban_step(u, v, w, p, alpha, sigma, dt=0.4, dx=1.0, Re=1 / 20, ub=1.0, multi_itr=5,
-               lscalar=True):
+               lscalar=True, coarsest: int = 1, j_itr: int = 1):
     """One time step of ``UM/AI_HFM_v3.ipynb`` cell 25 (fields (D,H,W), float32 in/out).
 
     BCs; explicit two-stage momentum with the 27-point diffusion and smoothed
     central advection layers; scalar transport; bluff body; pressure-gradient
     correction; divergence source; ``multi_itr`` sawtooth cycles of the
-    27-point Poisson layer with outlet zeroing; BCs, correction and bluff body
+    27-point Poisson layer with outlet zeroing and no fine-level sweep of the
+    correction (``urban_pressure_cycles``); BCs, correction and bluff body
     again.  Returns (u, v, w, p, alpha, w_fine) with w_fine the last cycle's
-    fine-level correction (the reference's divergence guard reads max|w_512|).
+    fine-level correction, the prolonged ``w_512`` (the reference's divergence
+    guard reads max|w_512|, ``UM/AI_HFM_v3.ipynb:2625``).  ``coarsest`` /
+    ``j_itr``: see ``urban_levels`` / ``sawtooth_cycle``.
     """
     f32 = np.float32
     dif = diffusion27_kernel(dt, Re, dx)
     ax, ay, az = advection_kernels(dt, dx)
     sx, sy, sz = divergence_kernels(dt, dx)
-    w5 = poisson27_kernel(dx)
     C = lambda x, K: conv_same(x.astype(f32), K)  # noqa: E731
 
     def adv(vel, phi):
@@ -452,12 +490,8 @@ def urban_step(u, v, w, p, alpha, sigma, dt=0.4, dx=1.0, Re=1 / 20, ub=1.0, mult
     u, v, w = boundary_condition_velocity(u, v, w, ub=f32(ub))
     b = f32(-1.0) * ((C(u, sx) + C(v, sy)) + C(w, sz))
     b = boundary_condition_source(b)
-    shapes = urban_levels(p.shape)
-    ops = [lambda x: C(x, w5)] * len(shapes)
-    d = f32(w5[1, 1, 1])
     rec = {}
-    for _ in range(multi_itr):
-        p = sawtooth_cycle(p, b, ops, [d] * len(shapes), zero_outlet=True, record=rec)
+    p = urban_pressure_cycles(p, b, multi_itr, dx, coarsest, j_itr, record=rec)
     p = boundary_condition_pressure(p)
     u, v, w = u - C(p, ax), v - C(p, ay), w - C(p, az)
     u, v, w = boundary_condition_velocity(u, v, w, ub=f32(ub))
@@ -502,7 +536,7 @@ def pg_dissipation(u, v, w, sigma, dt, eps_k, dif, ax, ay, az):
 
 
 def inhale_step(u, v, w, p, alpha, sigma, dt=0.5, dx=1.0, Re=1 / 20, ub=1.0, multi_itr=5,
-                eps_k=1e-3, lscalar=False):
+                eps_k=1e-3, lscalar=False, coarsest: int = 1, j_itr: int = 1):
     """One time step of ``UM/INHALE_demo.ipynb`` cell 24: the urban step with
     the Petrov-Galerkin dissipation in place of the diffusion layer in both
     momentum stages, the bluff body on the velocities only.  Returns
@@ -511,7 +545,6 @@ def inhale_step(u, v, w, p, alpha, sigma, dt=0.5, dx=1.0, Re=1 / 20, ub=1.0, mul
     dif = diffusion27_kernel(dt, Re, dx)
     ax, ay, az = advection_kernels(dt, dx)
     sx, sy, sz = divergence_kernels(dt, dx)
-    w5 = poisson27_kernel(dx)
     C = lambda x, K: conv_same(x.astype(f32), K)  # noqa: E731
 
     def adv(vel, phi, first):
@@ -536,12 +569,8 @@ def inhale_step(u, v, w, p, alpha, sigma, dt=0.5, dx=1.0, Re=1 / 20, ub=1.0, mul
     u, v, w = boundary_condition_velocity(u, v, w, ub=f32(ub))
     b = f32(-1.0) * ((C(u, sx) + C(v, sy)) + C(w, sz))
     b = boundary_condition_source(b)
-    shapes = urban_levels(p.shape)
-    ops = [lambda x: C(x, w5)] * len(shapes)
-    d = f32(w5[1, 1, 1])
     rec = {}
-    for _ in range(multi_itr):
-        p = sawtooth_cycle(p, b, ops, [d] * len(shapes), zero_outlet=True, record=rec)
+    p = urban_pressure_cycles(p, b, multi_itr, dx, coarsest, j_itr, record=rec)
     p = boundary_condition_pressure(p)
     u, v, w = u - C(p, ax), v - C(p, ay), w - C(p, az)
     u, v, w = boundary_condition_velocity(u, v, w, ub=f32(ub))

@@ -136,3 +136,38 @@ def test_team_flow_matches_oracle(dev, team):
     solo.step(steps)
     for a, b in zip(fl.state(0), solo.state(0)):
         assert rel(a.cpu().numpy(), b.cpu().numpy()) < 1e-9
+
+
+@pytest.mark.parametrize("team", [1, 4])
+def test_nan_velocity_stops_with_breakdown(dev, team):
+    """A NaN cell stops its flow with "breakdown" before any step (the velocity
+    maximum keeps the NaN), and a NaN reaching the projection is a breakdown too,
+    never a converged solve; the other flow of an ensemble steps as it would alone."""
+    import torch
+    from hfm import _lib as L
+    N = 64
+    nflows = 2 if team == 1 else 1
+    fl = _flow(dev, N, nflows=nflows, team=team)
+    u, v = F.random_modes_ic(N, 3)
+    u_bad = u.copy()
+    u_bad[7, 9] = np.nan
+    fl.set_state(0, torch.from_numpy(u_bad), torch.from_numpy(v))
+    if nflows == 2:
+        fl.set_state(1, torch.from_numpy(u), torch.from_numpy(v))
+    fl.step(3)
+    st = fl.stats()
+    assert L.FLOW_STATUS[int(st["status"][0])] == "breakdown"
+    assert int(st["steps"][0]) == 0
+    if nflows == 2:
+        one = _flow(dev, N, nflows=1, team=1)
+        one.set_state(0, torch.from_numpy(u), torch.from_numpy(v))
+        one.step(3)
+        assert int(st["status"][1]) == 1 and int(st["steps"][1]) == 3
+        for a, b in zip(one.state(0), fl.state(1)):
+            assert bool((a == b).all())
+    # projection only: the NaN reaches the right-hand side
+    fl.set_state(0, torch.from_numpy(u_bad), torch.from_numpy(v))
+    fl.project()
+    st = fl.stats()
+    assert L.FLOW_STATUS[int(st["status"][0])] == "breakdown"
+    assert int(st["iters"][0]) == 0

@@ -66,16 +66,22 @@ def test_dissipative_advection_matches_oracle(dev):
 @pytest.mark.parametrize("sh", [(8, 16, 16), (4, 16, 16)])
 def test_inhale_steps_match_oracle(dev, sh):
     # (4, 16, 16): a flat box as in the demo (nz = nx / ratio; coarsest 1 x 4 x 4)
+    # UM/INHALE_demo.ipynb cell 24: the prolonged fine correction w_256 applied
+    # with no fine-level sweep (:13454-13458), the guard max|w_256| > nsafe
     from hfm.timestep import InhaleFlow
-    u = np.ones(sh, np.float32); v = np.zeros(sh, np.float32); w = np.zeros(sh, np.float32)
-    p = np.zeros(sh, np.float32); a = np.zeros(sh, np.float32)
+    rng = np.random.default_rng(2)
+    u = (1.0 + 0.05 * rng.standard_normal(sh)).astype(np.float32)
+    v, w, p = (0.05 * rng.standard_normal(sh).astype(np.float32) for _ in range(3))
+    a = np.zeros(sh, np.float32)
     sig = np.zeros(sh, np.float32); sig[1:3, 6:9, 6:9] = 1e5
     flow = InhaleFlow(sh, sigma=T(sig, dev), multi_itr=3)
     g = [T(x, dev) for x in (u, v, w, p, a)]
     o = (u, v, w, p, a)
     for _ in range(3):
         g = flow.step(*g)
-        o = O.inhale_step(*o, sig, multi_itr=3)[:5]
+        res = O.inhale_step(*o, sig, multi_itr=3)
+        o = res[:5]
     for gi, oi in zip(g[:4], o[:4]):
         assert rel(H(gi), oi) < 1e-5
+    assert rel(H(flow.last_w), res[5]) < 1e-5
     assert not flow.diverged()

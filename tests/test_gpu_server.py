@@ -71,3 +71,35 @@ def test_admission_respects_the_hbm_budget(dev):
     assert all(r.status == "converged" for r in res)
     with pytest.raises(ValueError):
         srv.submit(99, *_fields(dev, (96, 96, 96), 1))
+
+
+def test_failed_request_fails_alone(dev):
+    """A request with a NaN right-hand side, and one with a NaN conductivity, stop with
+    "breakdown" on their own; the requests beside them in the mix converge bitwise as
+    they do alone (UM/AI_HFM_v3.ipynb:2625-2627 is the reference's only guard: it ends
+    the whole run)."""
+    from hfm.mg import Params
+    from hfm.server import SolveServer
+    params = Params(storage="bf16", tol=1e-8)
+    alone = {}
+    for rid in (0, 3):
+        srv = SolveServer(params, hbm_budget=8e9, capacity=16, device=dev)
+        srv.submit(rid, *_fields(dev, *REQS[rid][:2], *REQS[rid][2:]))
+        (r,) = srv.drain()
+        alone[rid] = (r.iters, r.x.clone())
+    srv = SolveServer(params, hbm_budget=8e9, capacity=16, iters_per_epoch=3, device=dev)
+    srv.submit(0, *_fields(dev, *REQS[0][:2], *REQS[0][2:]))
+    sx, sy, sz, b = _fields(dev, *REQS[1][:2], *REQS[1][2:])
+    b[5, 6, 7] = float("nan")
+    srv.submit(10, sx, sy, sz, b)
+    sx, sy, sz, b = _fields(dev, *REQS[2][:2], *REQS[2][2:])
+    sy[3, 4, 5] = float("nan")
+    srv.submit(11, sx, sy, sz, b)
+    srv.submit(3, *_fields(dev, *REQS[3][:2], *REQS[3][2:]))
+    by = {r.rid: r for r in srv.drain()}
+    assert sorted(by) == [0, 3, 10, 11]
+    assert by[10].status == "breakdown" and by[11].status == "breakdown"
+    for rid in (0, 3):
+        assert by[rid].status == "converged"
+        assert by[rid].iters == alone[rid][0]
+        assert bool((by[rid].x == alone[rid][1]).all())

@@ -47,21 +47,71 @@ def test_fused_advection_matches_layers(dev):
     assert rel(H(r), O.zero_outlet_plane(C(a, k["poisson"]) - u)) < 1e-6
 
 
-def test_urban_steps_match_oracle(dev):
+def _urban_state(sh, seed):
+    rng = np.random.default_rng(seed)
+    u = (1.0 + 0.05 * rng.standard_normal(sh)).astype(np.float32)
+    v, w, p = (0.05 * rng.standard_normal(sh).astype(np.float32) for _ in range(3))
+    a = np.zeros(sh, np.float32); a[:4] = 1
+    sig = np.zeros(sh, np.float32); sig[1:5, 6:9, 6:9] = 1e5
+    return (u, v, w, p, a), sig
+
+
+@pytest.mark.parametrize("coarsest,j_itr", [(1, 1), (2, 1), (1, 2)])
+def test_urban_steps_match_oracle(dev, coarsest, j_itr):
+    """The reference's urban step (UM/AI_HFM_v3.ipynb cell 25): pressure cycles with
+    the prolonged fine correction applied unsmoothed; the divergence guard reads it."""
     from hfm.timestep import UrbanFlow
     sh = (8, 16, 16)
-    u = np.ones(sh, np.float32); v = np.zeros(sh, np.float32); w = np.zeros(sh, np.float32)
-    p = np.zeros(sh, np.float32); a = np.zeros(sh, np.float32); a[:4] = 1
-    sig = np.zeros(sh, np.float32); sig[1:5, 6:9, 6:9] = 1e5
-    flow = UrbanFlow(sh, sigma=T(sig, dev), multi_itr=3)
-    g = [T(x, dev) for x in (u, v, w, p, a)]
-    o = (u, v, w, p, a)
+    o, sig = _urban_state(sh, 1)
+    flow = UrbanFlow(sh, sigma=T(sig, dev), multi_itr=3, coarsest=coarsest, j_itr=j_itr)
+    g = [T(x, dev) for x in o]
     for _ in range(3):
         g = flow.step(*g)
-        o = O.urban_step(*o, sig, multi_itr=3)[:5]
+        res = O.urban_step(*o, sig, multi_itr=3, coarsest=coarsest, j_itr=j_itr)
+        o = res[:5]
     for gi, oi in zip(g, o):
         assert rel(H(gi), oi) < 1e-5
+    w_fine = res[5]
+    assert rel(H(flow.last_w), w_fine) < 1e-5
+    assert flow.diverged(limit=0.0) == bool(np.abs(w_fine).max() > 0.0)
     assert not flow.diverged()
+    lim = float(np.abs(w_fine).max())
+    assert flow.diverged(limit=0.5 * lim) and not flow.diverged(limit=2.0 * lim)
+
+
+@pytest.mark.parametrize("kind", ["k27", "k7", "var6", "2d"])
+@pytest.mark.parametrize("outlet", [False, True])
+def test_prolong_correct_jacobi_is_prolong_then_correct(dev, kind, outlet):
+    """hfm_prolong_correct_jacobi_f32 = hfm_prolong_nearest_f32 + hfm_correct_jacobi_f32, bitwise."""
+    import torch
+    from hfm import _lib as L, layers
+    rng = np.random.default_rng(21)
+    sh = (1, 12, 20) if kind == "2d" else (6, 10, 24)
+    fz = 1 if kind == "2d" else 2
+    csh = (sh[0] // fz, sh[1] // 2, sh[2] // 2)
+    x, b = (T(rng.standard_normal(sh), dev) for _ in range(2))
+    wc = T(rng.standard_normal(csh), dev)
+    if kind == "var6":
+        mats = {k: T(rng.uniform(0.5, 2.0, sh), dev) for k in layers.VAR6_DIRS}
+        op = layers.AOperatorVar6(mats)
+        K, d, m, tot = None, 0.0, op._arr, op.tot.data_ptr()
+    else:
+        Kw = O.poisson27_kernel() if kind != "k7" else O.a7_const_kernel()
+        if kind == "2d":
+            Kw = O.implicit_ad2d_kernel()
+        K27 = layers.kernel27(Kw)
+        K, d, m, tot = K27.ctypes.data_as(L.C.c_void_p), float(K27[13]), None, None
+    flags = L.CORRECT_ZERO_OUTLET if outlet else 0
+    s = L.stream_handle(dev)
+    wf = torch.empty(sh, device=dev)
+    L.call("hfm_prolong_nearest_f32", wc.data_ptr(), wf.data_ptr(), *csh, fz, 2, 2, s)
+    y1, y2 = torch.empty(sh, device=dev), torch.empty(sh, device=dev)
+    L.call("hfm_correct_jacobi_f32", x.data_ptr(), wf.data_ptr(), b.data_ptr(), y1.data_ptr(),
+           K, d, m, tot, *sh, flags, s)
+    L.call("hfm_prolong_correct_jacobi_f32", x.data_ptr(), wc.data_ptr(), b.data_ptr(),
+           y2.data_ptr(), K, d, m, tot, *sh, fz, 2, 2, flags, s)
+    torch.cuda.synchronize()
+    assert torch.equal(y1, y2)
 
 
 @pytest.mark.parametrize("which", ["UPWIND", "CENTRAL"])

@@ -69,3 +69,26 @@ def test_writer_rejects_a_zero_period(tmp_path):
     from hfm.snapshot import SnapshotWriter
     with pytest.raises(ValueError):
         SnapshotWriter(str(tmp_path), n_out=0)
+
+
+@pytest.mark.timeout(60)
+def test_writer_keeps_its_pools_after_staging_errors(tmp_path, monkeypatch):
+    """A snapshot that fails while staging hands its staging pool back, so later
+    snapshots never block (with depth 1, a lost pool would hang the next call)."""
+    from hfm import snapshot
+
+    class FailingAlloc:
+        def __getattr__(self, name):
+            return getattr(torch, name)
+
+        def empty(self, *a, **k):
+            raise RuntimeError("staging buffer allocation failed")
+
+    with snapshot.SnapshotWriter(str(tmp_path), n_out=1, depth=1) as wr:
+        monkeypatch.setattr(snapshot, "torch", FailingAlloc())
+        for _ in range(3):
+            with pytest.raises(RuntimeError, match="staging buffer"):
+                wr.maybe_save(1, **_fields((2, 3, 4), five_d=False))
+        monkeypatch.setattr(snapshot, "torch", torch)
+        assert wr.maybe_save(2, **_fields((2, 3, 4), five_d=False))
+    assert sorted(os.listdir(tmp_path)) == [f"{n}2.npy" for n in "puvw"]
