@@ -9,6 +9,7 @@
 #include <stdio.h>
 
 #include <algorithm>
+#include <mutex>
 
 #include "hfm_common.h"
 
@@ -1195,13 +1196,15 @@ extern "C" int hfm_bc_f32(const float* in, float* out, int64_t D, int64_t H, int
   return launch_status("hfm_bc_f32");
 }
 
-extern "C" int hfm_bc_velocity_inplace_f32(float* u, float* v, float* w, float* scratch,
-                                           int64_t D, int64_t H, int64_t W, float ub,
-                                           const float* inflow, void* stream) {
-  if (!u || !v || !w || !scratch || !dims_ok(D, H, W) || D < 2 || H < 2 || W < 2) {
-    set_error("hfm_bc_velocity_inplace_f32: bad arguments");
-    return HFM_EINVAL;
-  }
+// The no-allocation fill's scratch: one copy of this array per device, held
+// until the process ends.  Host code never passes the symbol itself to a
+// kernel; it passes the device address hipGetSymbolAddress returns, looked
+// up once per device (with that device current) and kept per device.
+static __device__ __align__(256) float scratch[HFM_BC_NOALLOC_SCRATCH_FLOATS];
+
+namespace {
+int bc_velocity_inplace(const char* what, float* u, float* v, float* w, float* scratch, int64_t D,
+                        int64_t H, int64_t W, float ub, const float* inflow, void* stream) {
   const Grid3 g{int(D), int(H), int(W)};
   hipStream_t s = as_stream(stream);
   const int64_t ng = std::max(D * W, H * W);
@@ -1210,7 +1213,67 @@ extern "C" int hfm_bc_velocity_inplace_f32(float* u, float* v, float* w, float* 
   const int64_t nb = 2 * D * H + 2 * D * (W - 2) + 2 * (H - 2) * (W - 2);
   hipLaunchKernelGGL(k_bc_velocity_inplace, dim3(unsigned((nb + 255) / 256)), dim3(256), 0, s, u,
                      v, w, scratch, g, ub, inflow);
-  return launch_status("hfm_bc_velocity_inplace_f32");
+  return launch_status(what);
+}
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+float* g_scratch_dev[kMaxDevices] = {};
+
+int scratch_of_current_device(const char* what, float** out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+    set_error("%s: no usable current device (%s)", what, hipGetErrorString(e));
+    return HFM_EDEVICE;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* ptr = nullptr;
+    e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+      set_error("%s: hipGetSymbolAddress of the scratch array failed on device %d: %s", what,
+                dev, hipGetErrorString(e));
+      return HFM_EDEVICE;
+    }
+    g_scratch_dev[dev] = static_cast<float*>(ptr);
+  }
+  *out = g_scratch_dev[dev];
+  return HFM_OK;
+}
+}  // namespace
+
+extern "C" int hfm_bc_velocity_inplace_f32(float* u, float* v, float* w, float* scratch,
+                                           int64_t D, int64_t H, int64_t W, float ub,
+                                           const float* inflow, void* stream) {
+  if (!u || !v || !w || !scratch || !dims_ok(D, H, W) || D < 2 || H < 2 || W < 2) {
+    set_error("hfm_bc_velocity_inplace_f32: bad arguments");
+    return HFM_EINVAL;
+  }
+  return bc_velocity_inplace("hfm_bc_velocity_inplace_f32", u, v, w, scratch, D, H, W, ub, inflow,
+                             stream);
+}
+
+extern "C" int hfm_bc_velocity_inplace_noalloc_f32(float* u, float* v, float* w, int64_t D,
+                                                   int64_t H, int64_t W, float ub,
+                                                   const float* inflow, void* stream) {
+  const char* what = "hfm_bc_velocity_inplace_noalloc_f32";
+  if (!u || !v || !w || !dims_ok(D, H, W) || D < 2 || H < 2 || W < 2) {
+    set_error("%s: bad arguments", what);
+    return HFM_EINVAL;
+  }
+  const int64_t need = 2 * D * W + 2 * H * W;
+  if (need > HFM_BC_NOALLOC_SCRATCH_FLOATS) {
+    set_error("%s: a (%lld, %lld, %lld) grid needs %lld scratch floats, more than the %lld this "
+              "entry point holds; use hfm_bc_velocity_inplace_f32 with a workspace",
+              what, (long long)D, (long long)H, (long long)W, (long long)need,
+              (long long)HFM_BC_NOALLOC_SCRATCH_FLOATS);
+    return HFM_EINVAL;
+  }
+  float* ptr = nullptr;
+  const int rc = scratch_of_current_device(what, &ptr);
+  if (rc != HFM_OK) return rc;
+  return bc_velocity_inplace(what, u, v, w, ptr, D, H, W, ub, inflow, stream);
 }
 
 extern "C" int hfm_bc_velocity_f32(const float* u, const float* v, const float* w, float* uo,

@@ -80,6 +80,8 @@ SIGNATURES = {
     "hfm_bc_velocity_f32": (_int, [_vp, _vp, _vp, _vp, _vp, _vp, _i64, _i64, _i64, _f32, _vp,
                                    _vp]),
     "hfm_bc_velocity_inplace_f32": (_int, [_vp, _vp, _vp, _vp, _i64, _i64, _i64, _f32, _vp, _vp]),
+    "hfm_bc_velocity_inplace_noalloc_f32": (_int, [_vp, _vp, _vp, _i64, _i64, _i64, _f32, _vp,
+                                                   _vp]),
     "hfm_bluff_body_f32": (_int, [_vp, _vp, _int, _vp, _f32, _i64, _vp]),
     "hfm_mg_sweep": (_int, [_vp, _vp, _i64, _vp, _int, _int, _int, _f32, _f32, _int, _vp, _vp]),
     "hfm_mg_resid_restrict": (_int, [_vp, _vp, _i64, _vp, _int, _int, _vp]),
@@ -125,6 +127,7 @@ FLOW_STATUS = {**STATUS, 4: "cfl", 5: "team_timeout"}
 FLOW_TEAM_BYTES = 4096      # team scratch per flow (zeroed before each launch)
 FLOW_TEAM_MAX = 64
 FLOW_PROJECT_ONLY = 1
+BC_NOALLOC_SCRATCH_FLOATS = 1 << 20   # scratch of hfm_bc_velocity_inplace_noalloc_f32
 
 _lock = threading.Lock()
 _lib = None

@@ -189,6 +189,19 @@ int hfm_bc_velocity_f32(const float* u, const float* v, const float* w, float* u
 int hfm_bc_velocity_inplace_f32(float* u, float* v, float* w, float* scratch, int64_t D, int64_t H,
                                 int64_t W, float ub, const float* inflow, void* stream);
 
+/* hfm_bc_velocity_inplace_f32 without a workspace, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops).
+ * No counterpart in the reference.  The pre-fill rows go to a scratch array
+ * of HFM_BC_NOALLOC_SCRATCH_FLOATS floats (4 MiB: grids up to 512^3) that the
+ * library holds on each device it runs on until the process ends; grids
+ * needing more (2 D W + 2 H W > the maximum) return HFM_EINVAL.  It calls no
+ * allocator.  Every call on one device shares that device's array: calls
+ * that overlap on one GPU (different streams, or host threads) race.  The
+ * stream must belong to the current device. */
+#define HFM_BC_NOALLOC_SCRATCH_FLOATS (1 << 20)
+int hfm_bc_velocity_inplace_noalloc_f32(float* u, float* v, float* w, int64_t D, int64_t H,
+                                        int64_t W, float ub, const float* inflow, void* stream);
+
 /* out_i = in_i / (1 + dt*sigma) for n fields (in place allowed). */
 int hfm_bluff_body_f32(const float* const* in, float* const* out, int nfields,
                        const float* sigma, float dt, int64_t ncells, void* stream);
