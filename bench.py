@@ -36,11 +36,17 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def load_batch(mg, specs, dev):
-    from hfm.workloads import device_fields
-    for i, (shape, sig, nd, seed) in enumerate(specs):
-        sx, sy, sz, b = device_fields(shape, sig, nd, seed, dev)
-        mg.stage_sigma(i, sx, sy, sz, b)
+def load_batch(mg, specs, dev, chunk=64):
+    """Stage every instance's fields: drawn on the host from numpy's PCG64
+    (hfm.workloads.host_fields, the oracle's draw order), 16 threads, ``chunk``
+    instances at a time, then copied to the device."""
+    import torch
+    from hfm.workloads import host_fields_many
+    for c0 in range(0, len(specs), chunk):
+        part = specs[c0:c0 + chunk]
+        for i, arrs in enumerate(host_fields_many(part, workers=16)):
+            sx, sy, sz, b = (torch.from_numpy(a).to(dev) for a in arrs)
+            mg.stage_sigma(c0 + i, sx, sy, sz, b)
 
 
 def one_step(mg):
@@ -63,46 +69,73 @@ def kernel_time_ms(fn, reps, torch):
     return e0.elapsed_time(e1) / reps
 
 
-def cpu_baseline(specs, iters, budget_s):
-    """The oracle's solve (numpy: FP64 CG + FP32-storage V(2,2)) on host cores.
-
-    Instances are taken in workload order (edges <= 64 only, so one solve
-    stays a few seconds) until the budget is spent; the measured rate in
-    fine-cell-iterations/s is scaled to the whole workload's
-    sum(cells x iterations) (iterations of the same algorithm as counted on
-    the GPU) to give solves/s for the same 512-instance job."""
+def _cpu_solve(spec):
+    """One oracle solve (worker process): the instance regenerated from its seed
+    (the same PCG64 draw as the GPU's inputs), numpy FCG with FP32-storage V(2,2)."""
     from oracle import hfm_oracle as O
-    t_used, work, done, edges = 0.0, 0, 0, []
-    for (shape, sig, nd, seed) in specs:
-        if shape[0] > 64:
-            continue
-        sx, sy, sz, b = O.make_anisotropic_fields(shape, seed, sig, nd)
-        op = O.harmonic_faces(sx, sy, sz)
-        t0 = time.perf_counter()
-        res = O.fcg_solve(op, b, tol=1e-8, storage="f32")
-        t_used += time.perf_counter() - t0
-        assert res.status == "converged"
-        work += math.prod(shape) * res.iters
-        done += 1
-        edges.append(shape[0])
-        if t_used >= budget_s:
-            break
-    rate = work / t_used
+    shape, sig, nd, seed = spec
+    sx, sy, sz, b = O.make_anisotropic_fields(shape, seed, sig, nd)
+    op = O.harmonic_faces(sx, sy, sz)
+    t0 = time.perf_counter()
+    res = O.fcg_solve(op, b, tol=1e-8, storage="f32")
+    return {"t": time.perf_counter() - t0, "iters": res.iters, "status": res.status,
+            "x": res.x.astype(np.float64)}
+
+
+def cpu_baseline(specs, iters, nsample, workers, gpu_x):
+    """The oracle's solve (numpy FCG + FP32-storage V(2,2), unfused array ops) on
+    ``workers`` host processes: the first ``nsample`` instances of the workload,
+    all to completion; the rate in fine-cell-iterations/s (wall clock) is
+    scaled to the whole workload's sum(cells x iterations) to give solves/s
+    for the same 512-instance job.  The GPU solutions of those instances
+    (``gpu_x``) are compared with the oracle's."""
+    import multiprocessing as mp
+    sample = specs[:nsample]
+    saved = {k: os.environ.get(k) for k in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS")}
+    for k in saved:
+        os.environ[k] = "1"          # one thread per worker process
+    try:
+        with mp.get_context("spawn").Pool(workers) as pool:
+            t0 = time.perf_counter()
+            outs = pool.map(_cpu_solve, sample, chunksize=1)
+            wall = time.perf_counter() - t0
+    finally:
+        for k, v in saved.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+    if any(o["status"] != "converged" for o in outs):
+        raise SystemExit("bench: an oracle solve of the CPU baseline did not converge")
+    work = sum(math.prod(sp[0]) * o["iters"] for sp, o in zip(sample, outs))
+    rate = work / wall
     total = sum(math.prod(s[0]) * int(k) for s, k in zip(specs, iters))
-    return {"value": round(len(specs) * rate / total, 3), "unit": "solves/s", "cores": 1,
+    diffs, it_diff = [], []
+    for q, o in enumerate(outs):
+        if q in gpu_x:
+            xg, kg = gpu_x[q]
+            diffs.append(float(np.linalg.norm(xg - o["x"]) / np.linalg.norm(o["x"])))
+            it_diff.append(int(kg) - int(o["iters"]))
+    edges = sorted({sp[0][0] for sp in sample})
+    return {"value": round(len(specs) * rate / total, 3), "unit": "solves/s", "cores": workers,
             "kind": "port",
-            "sample": f"{done} solves (edges {sorted(set(edges))}) of the workload in order, "
-                      f"{t_used:.1f} s, {rate / 1e6:.2f} M fine-cell-iterations/s, scaled to the "
-                      f"workload's {total / 1e9:.2f} G cell-iterations"}
+            "sample": f"the oracle's numpy FCG (FP32-storage V(2,2)) on the first {len(sample)} "
+                      f"instances of the workload (edges {edges}), {workers} worker processes, "
+                      f"{wall:.1f} s wall, {rate / 1e6:.2f} M fine-cell-iterations/s, scaled to "
+                      f"the workload's {total / 1e9:.2f} G cell-iterations",
+            # the GPU solved these instances with the bench's storage, the oracle
+            # here with FP32 storage: both to 1e-8 on the same FP32 operator
+            "gpu_vs_oracle_f32_max_rel_diff": max(diffs) if diffs else None,
+            "iters_gpu_minus_oracle_f32": sorted(set(it_diff))}
 
 
 def fp32_one_at_a_time(specs, dev, torch):
-    """The plain FP32 path, one instance at a time, on a sample (solves/s)."""
+    """The plain FP32 path, one instance at a time, on a sample: (solves/s, mean cells)."""
     from hfm.mg import MGBatch
-    from hfm.workloads import device_fields
+    from hfm.workloads import host_fields
     t_tot = 0.0
     for (shape, sig, nd, seed) in specs:
-        sx, sy, sz, b = device_fields(shape, sig, nd, seed, dev)
+        sx, sy, sz, b = (torch.from_numpy(a).to(dev) for a in host_fields(shape, sig, nd, seed))
         mg = MGBatch([shape], storage="f32", tol=1e-8, device=dev)
         mg.stage_sigma(0, sx, sy, sz, b)
         one_step(mg)   # warm: graph capture
@@ -113,6 +146,46 @@ def fp32_one_at_a_time(specs, dev, torch):
         t_tot += time.perf_counter() - t0
         assert int(info["status"][0]) == 1
     return len(specs) / t_tot
+
+
+def true_residuals(mg, torch):
+    """Per instance, ||b - A x||_2 / ||b||_2 in FP64 with A the stored FP32 face
+    operator, computed here with torch array ops (independent of the solver's
+    kernels and of its recursive residual), and the largest relative
+    difference between the stored faces and an FP64 harmonic mean of the staged
+    conductivities (boundary face = the cell's own value) rounded to FP32."""
+    rels, face_dev = [], 0.0
+    for inst in mg.insts:
+        D, H, W = inst.shape
+        N = D * H * W
+        f, o = inst.face32[0], inst.face_off[0]
+        cx, cy, cz = (f[o[q]:o[q] + N].view(D, H, W).double() for q in range(3))
+        bxh = f[o[3]:o[3] + D * H].view(D, H).double()
+        byh = f[o[4]:o[4] + D * W].view(D, W).double()
+        bzh = f[o[5]:o[5] + H * W].view(H, W).double()
+        ex = torch.cat([cx[:, :, 1:], bxh[:, :, None]], 2)
+        ey = torch.cat([cy[:, 1:, :], byh[:, None, :]], 1)
+        ez = torch.cat([cz[1:], bzh[None]], 0)
+        x = inst.solution().double()
+        b = inst.b32.view(D, H, W).double()
+        ax = (cx + ex + cy + ey + cz + ez) * x
+        ax[:, :, 1:] -= cx[:, :, 1:] * x[:, :, :-1]
+        ax[:, :, :-1] -= ex[:, :, :-1] * x[:, :, 1:]
+        ax[:, 1:, :] -= cy[:, 1:, :] * x[:, :-1, :]
+        ax[:, :-1, :] -= ey[:, :-1, :] * x[:, 1:, :]
+        ax[1:] -= cz[1:] * x[:-1]
+        ax[:-1] -= ez[:-1] * x[1:]
+        rels.append(float((b - ax).norm() / b.norm()))
+        N8 = inst.sigma.numel() // 3
+        sg = [inst.sigma[q * N8:q * N8 + N].view(D, H, W).double() for q in range(3)]
+        hm = lambda a, c: 2.0 * a * c / (a + c)  # noqa: E731
+        ref = (torch.cat([sg[0][:, :, :1], hm(sg[0][:, :, 1:], sg[0][:, :, :-1])], 2),
+               torch.cat([sg[1][:, :1, :], hm(sg[1][:, 1:, :], sg[1][:, :-1, :])], 1),
+               torch.cat([sg[2][:1], hm(sg[2][1:], sg[2][:-1])], 0))
+        for got, want in zip((cx, cy, cz), ref):
+            want32 = want.float().double()
+            face_dev = max(face_dev, float(((got - want32).abs() / want32.abs()).max()))
+    return np.array(rels), face_dev
 
 
 def serving(args, dev, torch):
@@ -236,11 +309,11 @@ def kolmogorov(args, dev, torch):
     ensemble of ``--flow-ensemble`` flows in one launch, and the oracle's step
     on one host core over a bounded sample."""
     from hfm.flow2d import KolmogorovFlow
-    from oracle import hfm_flow2d as F
+    from hfm.workloads import random_modes_ic
     N = 256
     out = {"grid": [N, N], "nu": 0.01, "dt": 0.005, "tol": 1e-6, "levels": 6}
     fl = KolmogorovFlow(N=N, nflows=1, team=args.flow_team, device=dev)
-    u0, v0 = F.random_modes_ic(N, args.seed)
+    u0, v0 = random_modes_ic(N, args.seed)
     fl.set_state(0, torch.from_numpy(u0), torch.from_numpy(v0))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -270,7 +343,7 @@ def kolmogorov(args, dev, torch):
     if args.flow_ensemble > 0:
         ens = KolmogorovFlow(N=N, nflows=args.flow_ensemble, device=dev)
         for q in range(args.flow_ensemble):
-            u, v = F.random_modes_ic(N, args.seed + q)
+            u, v = random_modes_ic(N, args.seed + q)
             ens.set_state(q, torch.from_numpy(u), torch.from_numpy(v))
         ens.step(1)
         torch.cuda.synchronize()
@@ -290,6 +363,7 @@ def kolmogorov(args, dev, torch):
     if not args.no_cpu_baseline:
         # the oracle's substeps from the run's first and last states (the early
         # flow is slower and needs fewer substeps), scaled to the run's substeps
+        from oracle import hfm_flow2d as F
         mg = F.build_mg(N)
         fu = F.forcing_u(N)
         uf, vf, pf = (t.cpu().numpy() for t in fl.state(0))
@@ -450,7 +524,7 @@ def profile_flow2d(args, dev, torch):
     ``--flow-steps`` steps, then ``--flow-ensemble`` flows for
     ``--flow-ensemble-steps`` steps; prints one JSON line."""
     from hfm.flow2d import KolmogorovFlow
-    from oracle import hfm_flow2d as F
+    from hfm.workloads import random_modes_ic
     N, out = 256, {"workload": "kolmogorov_2d"}
     for name, nflows, steps in (("single_flow", 1, args.flow_steps),
                                 ("ensemble", args.flow_ensemble, args.flow_ensemble_steps)):
@@ -459,7 +533,7 @@ def profile_flow2d(args, dev, torch):
         fl = KolmogorovFlow(N=N, nflows=nflows, team=args.flow_team if nflows == 1 else None,
                             device=dev)
         for q in range(nflows):
-            u, v = F.random_modes_ic(N, args.seed + q)
+            u, v = random_modes_ic(N, args.seed + q)
             fl.set_state(q, torch.from_numpy(u), torch.from_numpy(v))
         s = torch.cuda.current_stream()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -793,7 +867,11 @@ def main():
                     help="storage of z and the search direction (default: --storage)")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--fp32-sample", type=int, default=16)
+    ap.add_argument("--fp32-sample", type=int, default=64)
+    ap.add_argument("--cpu-sample", type=int, default=32,
+                    help="instances (workload order) the CPU baseline solves")
+    ap.add_argument("--cpu-workers", type=int, default=16,
+                    help="host processes of the CPU baseline (at most os.cpu_count())")
     ap.add_argument("--serve-requests", type=int, default=300)
     ap.add_argument("--serve-rate", type=float, default=300.0)
     ap.add_argument("--serve-sat-rate", type=float, default=3000.0)
@@ -890,6 +968,12 @@ def main():
     status = info["status"]
     if not np.all(status == 1):
         raise SystemExit(f"bench: {int(np.sum(status != 1))} instances did not converge")
+    # independent FP64 check of every result (after the timed region)
+    true_rel, face_dev = true_residuals(mg, torch)
+    log(f"bench: true FP64 residual max {true_rel.max():.3e}, stored faces vs FP64 harmonic "
+        f"{face_dev:.2e}")
+    gpu_x = {q: (mg.solution(q).cpu().numpy(), int(info["iter"][q]))
+             for q in range(min(args.cpu_sample, len(specs)))} if not args.no_cpu_baseline else {}
     total = sum(times)
     value = args.steps * len(specs) / total
     # Sigma over instances of iterations x (nu1 + nu2) x cells of the swept levels
@@ -934,10 +1018,20 @@ def main():
         extra["batched_fp32"] = batched_fp32(specs, dev, torch)
         extra["speedup_mixed_vs_fp32_batched"] = round(value / extra["batched_fp32"]["solves_per_s"], 2)
     if args.fp32_sample > 0:
-        sub = specs[:args.fp32_sample]
+        # a systematic sample (every k-th instance of the randomly ordered
+        # workload), so its size mix follows the batch's
+        k = max(1, len(specs) // args.fp32_sample)
+        sub = specs[::k][:args.fp32_sample]
         sps = fp32_one_at_a_time(sub, dev, torch)
-        extra["fp32_one_at_a_time_solves_per_s"] = round(sps, 2)
-        extra["speedup_vs_fp32_one_at_a_time"] = round(value / sps, 2)
+        sub_cells = float(np.mean([math.prod(sp[0]) for sp in sub]))
+        bat_cells = cells / len(specs)
+        # per-cell normalisation to the batch's mean size
+        sps_norm = sps * sub_cells / bat_cells
+        extra["fp32_one_at_a_time"] = {
+            "sample": f"every {k}th instance ({len(sub)})", "solves_per_s": round(sps, 2),
+            "sample_mean_cells": round(sub_cells), "batch_mean_cells": round(bat_cells),
+            "solves_per_s_at_batch_mean_size": round(sps_norm, 2)}
+        extra["speedup_vs_fp32_one_at_a_time"] = round(value / sps_norm, 2)
     if args.serve_requests > 0:
         extra["serving"] = serving(args, dev, torch)
     if args.urban_steps > 0:
@@ -948,8 +1042,8 @@ def main():
         extra["reference_context"] = reference_context(args, dev, torch)
     cpu = None
     if not args.no_cpu_baseline:
-        torch.set_num_threads(1)
-        cpu = cpu_baseline(specs, info["iter"], args.cpu_budget)
+        workers = max(1, min(os.cpu_count() or 1, args.cpu_workers))
+        cpu = cpu_baseline(specs, info["iter"], args.cpu_sample, workers, gpu_x)
     out = {
         "metric": METRIC, "value": round(value, 2), "unit": "solves/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup,
@@ -965,6 +1059,8 @@ def main():
         "cell_updates_per_s": round(rates["cell_updates_per_s"], 1),
         "fine_cell_vcycles_per_s": round(rates["fine_cell_vcycles_per_s"], 1),
         "max_rel_residual": float(info["rel"].max()),
+        "max_true_rel_residual_fp64": float(true_rel.max()),
+        "faces_max_rel_diff_vs_fp64_harmonic": face_dev,
         "latency_ms": {"p50": round(float(np.percentile(times, 50)) * 1e3, 2),
                        "p99": round(float(np.percentile(times, 99)) * 1e3, 2)},
         "roofline": roofline, "cpu_baseline": cpu, "launch_tuning": tuning, **extra,

@@ -436,3 +436,27 @@ def test_fused_entry_points_reject_bad_tables():
         L.call("hfm_mg_smooth2", 16, 16, 12, 16, 1, 1, 0.5, 1.6, None, None)
     with pytest.raises(L.HfmError, match="unknown storage"):
         L.call("hfm_mg_smooth2", 16, 16, 8, 16, 1, 7, 0.5, 1.6, None, None)
+
+
+def test_host_fields_are_the_oracles_draw():
+    """The bench workload's inputs (hfm.workloads.host_fields) are the oracle's
+    make_anisotropic_fields bit for bit, so the oracle can regenerate any instance."""
+    from hfm.workloads import batched_specs, host_fields, host_fields_many
+    from oracle import hfm_oracle as O
+    specs = [((8, 10, 12), 0.7, 2, 5), ((16, 16, 16), 1.9, 4, 99)] + \
+        [(s, sig, nd, seed) for (s, sig, nd, seed) in batched_specs(3, 2024) if s[0] <= 48]
+    many = host_fields_many(specs, workers=3)
+    for sp, got2 in zip(specs, many):
+        ref = O.make_anisotropic_fields(sp[0], sp[3], sp[1], sp[2])
+        for a, b, c in zip(host_fields(*sp), ref, got2):
+            assert a.dtype == np.float32
+            np.testing.assert_array_equal(a, b)
+            np.testing.assert_array_equal(c, b)
+
+
+def test_random_modes_ic_is_the_oracles():
+    from hfm.workloads import random_modes_ic
+    from oracle import hfm_flow2d as F
+    for N, seed in ((32, 3), (64, 2024)):
+        for a, b in zip(random_modes_ic(N, seed), F.random_modes_ic(N, seed)):
+            np.testing.assert_array_equal(a, b)

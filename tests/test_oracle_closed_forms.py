@@ -234,3 +234,30 @@ def test_bf16_rounding_is_nearest_even():
     x = np.array([1.0, 1.0 + 2 ** -8, 1.0 + 3 * 2 ** -8, 1.0 + 2 ** -9, -2.5, 3.0e38], np.float32)
     r = O.round_bf16(x)
     np.testing.assert_array_equal(r[:5], [1.0, 1.0, 1.0 + 2 ** -7 * 2, 1.0, -2.5])
+
+
+@pytest.mark.parametrize("shape,seed,sig", [((12, 10, 8), 1, 1.0), ((16, 16, 16), 2, 2.0),
+                                            ((9, 20, 14), 3, 0.5)])
+def test_fp64_pcg_matches_direct_solve(shape, seed, sig):
+    """The FP64 reference of the large-instance fixtures (solve_fp64_pcg) against
+    scipy's sparse direct solve."""
+    sx, sy, sz, b = O.make_anisotropic_fields(shape, seed, sig, 2)
+    op = O.harmonic_faces(sx, sy, sz).astype("f32")
+    x = O.solve_fp64_pcg(op, b, tol=1e-13)
+    xd = O.solve_fp64_direct(op, b)
+    assert np.linalg.norm(x - xd) / np.linalg.norm(xd) < 1e-10
+    r = b - O.face_apply(op, x)
+    assert np.linalg.norm(r) / np.linalg.norm(b) < 1e-12
+
+
+def test_fp64_reference_fixtures_are_consistent():
+    """The committed FP64 reference samples: each solve's recorded residual is at
+    the FP64 level and its sampled cell indices are distinct and inside the grid
+    it names."""
+    import os
+    gold = np.load(os.path.join(os.path.dirname(__file__), "golden", "fp64_reference_solves.npz"))
+    for q in range(2):
+        assert gold[f"case{q}_relres"][0] < 1e-12
+        D, H, W, seed, sig, nd = gold[f"case{q}_spec"]
+        assert gold[f"case{q}_idx"].max() < D * H * W
+        assert np.all(np.diff(gold[f"case{q}_idx"]) > 0)
