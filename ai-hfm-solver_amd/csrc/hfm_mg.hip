@@ -313,13 +313,22 @@ __device__ __forceinline__ void offdiag_sum(const RunCoef<C>& q, const A xc[8], 
   for (int m = 0; m < 8; ++m) off[m] = fma_a((A)q.cu[m], up ? (A)vu[m] : (A)0, off[m]);
 }
 
+// omega v / d of the weighted-Jacobi sweeps, formed as (omega v) times the
+// hardware reciprocal of d (v_rcp_f32, within 1 ulp) instead of the IEEE
+// division sequence (about ten instructions).  Every sweep of every kernel
+// forms its quotient through this helper, so fused and separate passes and
+// the one-launch coarse cycle stay bitwise equal.
+__device__ __forceinline__ float wdiv(float omega, float v, float d) {
+  return (omega * v) * __builtin_amdgcn_rcpf(d);
+}
+
 // The weighted-Jacobi value and the residual of one cell from its diagonal,
 // off-diagonal sum, right-hand side and current value.
 __device__ __forceinline__ float resid_val(float xc, float d, float bv, float off) {
   return bv - __builtin_fmaf(d, xc, -off);
 }
 __device__ __forceinline__ float jacobi_val(float xc, float d, float bv, float off, float omega) {
-  return xc + omega * resid_val(xc, d, bv, off) / d;
+  return xc + wdiv(omega, resid_val(xc, d, bv, off), d);
 }
 
 // xc = x on the run, off = sum of face * neighbour (w, e, s, n, d, u order).
@@ -370,7 +379,7 @@ __device__ __forceinline__ float sweep_one(const hfm_mg_task& t, int n, int i, i
   float diag;
   if (MODE == HFM_SWEEP_ZERO) {
     face_row<float, T>(t, n, i, j, k, [](int, int, int, int) { return 0.0f; }, diag);
-    return omega * bn / diag;
+    return wdiv(omega, bn, diag);
   } else if (MODE == HFM_SWEEP_PLAIN) {
     auto xat = [&](int m, int, int, int) { return ldg(xin + m); };
     const float off = face_row<float, T>(t, n, i, j, k, xat, diag);
@@ -450,7 +459,7 @@ __global__ __launch_bounds__(B) void k_sweep(const hfm_mg_task* __restrict__ tas
     q.diag(d);
     if (MODE == HFM_SWEEP_ZERO) {
 #pragma unroll
-      for (int m = 0; m < 8; ++m) o[m] = omega * bv[m] / d[m];
+      for (int m = 0; m < 8; ++m) o[m] = wdiv(omega, bv[m], d[m]);
     } else {
       float xc[8], off[8];
       if (MODE == HFM_SWEEP_PLAIN) {
@@ -1191,7 +1200,7 @@ __global__ __launch_bounds__(kBlock) void k_fcg_init(const hfm_mg_task* __restri
     if (SWEEP) {
       int i, j, k;
       decompose(t, n, i, j, k);
-      put(x0 + n, from_f<T>(omega * to_f(rt) / diag_rounded<T>(t, n, i, j, k)));
+      put(x0 + n, from_f<T>(wdiv(omega, to_f(rt), diag_rounded<T>(t, n, i, j, k))));
     }
   }
   const double s = block_sum(acc);
@@ -1354,7 +1363,7 @@ __global__ __launch_bounds__(kBlock) void k_fcg_update(const hfm_mg_task* __rest
         for (int m = 0; m < 8; ++m) {
           const float dT = round_to<T>(q.cw[m]) + round_to<T>(q.ce[m]) + round_to<T>(q.cs[m]) +
                            round_to<T>(q.cn[m]) + round_to<T>(q.cd[m]) + round_to<T>(q.cu[m]);
-          o[m] = omega * round_to<T>(rl[m]) / dT;
+          o[m] = wdiv(omega, round_to<T>(rl[m]), dT);
         }
         st8_nt(x0 + n0, o);
       }
@@ -1376,7 +1385,7 @@ __global__ __launch_bounds__(kBlock) void k_fcg_update(const hfm_mg_task* __rest
       const T rt = from_f<T>(float(rr));
       put(rlow + n, rt);
       acc += rr * rr;
-      if (SWEEP) put(x0 + n, from_f<T>(omega * to_f(rt) / diag_rounded<T>(t, n, i, j, k)));
+      if (SWEEP) put(x0 + n, from_f<T>(wdiv(omega, to_f(rt), diag_rounded<T>(t, n, i, j, k))));
     }
   }
   const double s = block_sum(acc);
