@@ -1468,7 +1468,7 @@ __global__ __launch_bounds__(kBlock) void k_fcg_finalize(hfm_mg_scalars* sc, con
       }
       break;
     case HFM_FIN_BETA:
-      S.beta = (S.iter == 0) ? 0.0 : s / S.rz;
+      S.beta = (S.iter == 0 || S.method == HFM_METHOD_REFINE) ? 0.0 : s / S.rz;
       S.rz = s;
       if (!isfinite(s)) {
         S.status = HFM_MG_BREAKDOWN;
@@ -1478,12 +1478,12 @@ __global__ __launch_bounds__(kBlock) void k_fcg_finalize(hfm_mg_scalars* sc, con
       break;
     case HFM_FIN_ALPHA:
       S.pap = s;
-      if (!(s > 0.0) || !isfinite(s)) {
+      if (S.method == HFM_METHOD_REFINE ? !isfinite(s) : (!(s > 0.0) || !isfinite(s))) {
         S.status = HFM_MG_BREAKDOWN;
         S.active = 0;
         if (S.xlag) S.xlag = 2;
       } else {
-        S.alpha = S.rz / s;
+        S.alpha = S.method == HFM_METHOD_REFINE ? 1.0 : S.rz / s;
       }
       break;
     case HFM_FIN_CHECK:

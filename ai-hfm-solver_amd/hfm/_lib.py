@@ -30,6 +30,7 @@ BC = {"top_copy": 0, "pressure": 1, "source": 2, "indicator": 3, "zero_outlet": 
 SWEEP = {"zero": 0, "plain": 1, "correct": 2}
 FIN = {"init": 0, "beta": 1, "alpha": 2, "check": 3}
 STATUS = {0: "running", 1: "converged", 2: "maxit", 3: "breakdown"}
+METHOD = {"fcg": 0, "refine": 1}
 
 TASK_DTYPE = np.dtype([
     ("D", "<i4"), ("H", "<i4"), ("W", "<i4"), ("inst", "<i4"),
@@ -46,7 +47,7 @@ SCALARS_DTYPE = np.dtype([
     ("bnorm2", "<f8"), ("rz", "<f8"), ("alpha", "<f8"), ("beta", "<f8"),
     ("rr", "<f8"), ("rel", "<f8"), ("tol", "<f8"), ("pap", "<f8"),
     ("iter", "<i4"), ("maxit", "<i4"), ("active", "<i4"), ("status", "<i4"),
-    ("slot0", "<i4"), ("nslot", "<i4"), ("xlag", "<i4"), ("pad1", "<i4"),
+    ("slot0", "<i4"), ("nslot", "<i4"), ("xlag", "<i4"), ("method", "<i4"),
     ("alpha_prev", "<f8"),
 ])
 

@@ -146,6 +146,11 @@ class Params:
     # separate sweep per block of cells: an instance in a batch that fuses
     # and alone in one that does not agrees to rounding in that sum.)
     fuse_min_tiles: int = 256
+    # outer iteration: "fcg" (flexible CG preconditioned by one V-cycle) or
+    # "refine" (FP64 defect correction x += V(r): the stationary V-cycle
+    # iteration with an FP64 residual and solution, ART/Anisotropic_3d_Multigrid_
+    # 7point.ipynb:402-454's cycle loop stopped at a tolerance)
+    method: str = "fcg"
 
     def __post_init__(self):
         if self.storage not in ELEM:
@@ -156,6 +161,8 @@ class Params:
             raise ValueError(f"outer must be one of {sorted(ELEM)}")
         if self.nu1 < 1 or self.nu2 < 1:
             raise ValueError("nu1 and nu2 must be >= 1")
+        if self.method not in L.METHOD:
+            raise ValueError(f"method must be one of {sorted(L.METHOD)}")
 
 
 class _Arena:
@@ -241,6 +248,7 @@ class Pool:
         p = self.params
         rec = np.zeros(len(insts), dtype=L.SCALARS_DTYPE)
         rec["tol"], rec["maxit"], rec["active"], rec["iter"] = p.tol, p.maxit, 1, -1
+        rec["method"] = L.METHOD[p.method]
         rec["slot0"] = [i.dots_off for i in insts]
         rec["nslot"] = [i.nblk for i in insts]
         rows = torch.from_numpy(rec.view(np.uint8).reshape(len(insts), -1).copy()).to(self.device)
@@ -816,9 +824,9 @@ class MGBatch:
                  iters_per_graph: int = 4, use_graph: bool = True, shrink: float = 0.5,
                  outer: Optional[str] = None, blocks: Optional[dict] = None,
                  coarse_cells: int = 4096, fuse: bool = True, fuse_levels: int = 1,
-                 fuse_min_tiles: int = 256, streams: int = 2):
+                 fuse_min_tiles: int = 256, streams: int = 2, method: str = "fcg"):
         self.params = Params(storage, tol, maxit, nu1, nu2, omega, alpha, max_coarse_edge, outer,
-                             blocks, coarse_cells, fuse, fuse_levels, fuse_min_tiles)
+                             blocks, coarse_cells, fuse, fuse_levels, fuse_min_tiles, method)
         shapes = [check_shape(s, max_coarse_edge)[0] for s in shapes]
         if not shapes:
             raise ValueError("empty batch")

@@ -594,7 +594,11 @@ def profile_kernel(args, dev, torch):
         # / --ref-inhale-steps, alone
         print(json.dumps(reference_context(args, dev, torch)), flush=True)
         return None
-    specs = batched_specs(args.instances, args.seed)
+    if args.single_shape:
+        shape = tuple(int(v) for v in args.single_shape.split(","))
+        specs = [(shape, 1.0, 1, args.seed)]
+    else:
+        specs = batched_specs(args.instances, args.seed)
     cells = sum(math.prod(s[0]) for s in specs)
     mg = MGBatch([s[0] for s in specs], storage=args.storage, tol=1e-8, device=dev,
                  outer=args.outer, fuse=not args.no_fuse, **_fuse_kw(args))
@@ -892,6 +896,9 @@ def main():
                     choices=["update", "sweep", "all", "flow2d", "urban", "reference"],
                     default=None,
                     help="launch only this kernel (after setup) or workload, for profiler runs")
+    ap.add_argument("--single-shape", default=None,
+                    help="D,H,W: profile one instance of this shape (sigma_ln 1, one dipole) "
+                         "instead of the batch")
     ap.add_argument("--reps", type=int, default=20)
     ap.add_argument("--urban-steps", type=int, default=10)
     ap.add_argument("--ref-cycles", type=int, default=200,

@@ -240,9 +240,16 @@ typedef struct hfm_mg_scalars {
   int32_t slot0, nslot;
   int32_t xlag;       /* x lacks alpha_prev * p_prev: 0 no, 1 p_prev = the update's p (xin),
                          2 p_prev = the update's previous p (ec); see hfm_fcg_update */
-  int32_t pad1;
+  int32_t method;     /* HFM_METHOD_FCG or HFM_METHOD_REFINE */
   double alpha_prev;
 } hfm_mg_scalars;
+
+/* Outer iterations (hfm_mg_scalars.method).  REFINE is the FP64 defect
+ * correction x += V(r), r -= A V(r) (iterative refinement with one V-cycle
+ * per step): the same launches as FCG, with beta = 0 and alpha = 1 set by
+ * hfm_fcg_finalize (the direction pass then copies z into p). */
+#define HFM_METHOD_FCG 0
+#define HFM_METHOD_REFINE 1
 
 #define HFM_MG_RUNNING 0
 #define HFM_MG_CONVERGED 1
@@ -363,7 +370,9 @@ int hfm_fcg_flush(const hfm_mg_task* tasks, const uint16_t* blk_task, int64_t nb
 
 /* Per-instance fixed-order reduction of dots[slot0 .. slot0+nslot) and the
  * scalar step:  kind 0 INIT (bnorm2), 1 BETA (rz, beta), 2 ALPHA (pap,
- * alpha), 3 CHECK (rr, rel, iter, status).  One block per instance. */
+ * alpha), 3 CHECK (rr, rel, iter, status).  One block per instance.  With
+ * method REFINE, BETA sets beta = 0 and ALPHA alpha = 1 (pap is only checked
+ * to be finite). */
 #define HFM_FIN_INIT 0
 #define HFM_FIN_BETA 1
 #define HFM_FIN_ALPHA 2

@@ -282,3 +282,26 @@ def test_large_instances_against_fp64_references(dev, storage):
         assert abs(np.linalg.norm(x) / gold[f"case{q}_norm"][0] - 1.0) < 1e-6
         r = b - O.face_apply(op.astype("f32"), x)
         assert np.linalg.norm(r) / np.linalg.norm(b) <= 1e-8
+
+
+@pytest.mark.parametrize("shape,storage,tol,alpha", [
+    ((32, 32, 32), "f32", 1e-6, 1.0),    # the plain V(2,2) method of the 64^3 configuration
+    ((64, 64, 64), "f32", 1e-6, 1.6),
+    ((48, 40, 32), "bf16", 1e-8, 1.6),   # FP64 defect correction around BF16 V-cycles
+])
+def test_defect_correction_matches_oracle(dev, shape, storage, tol, alpha):
+    """method="refine": x += V(r), r -= A V(r) in FP64 around storage-precision
+    V(2,2) cycles, against the oracle's solve_refined (one V-cycle per outer
+    step): the same number of cycles, the same solution."""
+    spec = (shape, 0, 1.0, 1)
+    mg, probs = _batch(dev, [spec], storage, tol=tol, maxit=400, alpha=alpha, method="refine")
+    info = mg.solve()
+    op, b = probs[0]
+    ref = O.solve_refined(op, b, tol=tol, storage=storage, inner_cycles=1, alpha=alpha,
+                          max_outer=400)
+    assert int(info["status"][0]) == 1 and ref.converged
+    assert int(info["iter"][0]) == ref.outer_iters
+    x = mg.solution(0).cpu().numpy()
+    assert rel(x, ref.x) < 1e-5
+    r = b - O.face_apply(op.astype("f32"), x)
+    assert np.linalg.norm(r) / np.linalg.norm(b) <= tol

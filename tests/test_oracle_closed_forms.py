@@ -261,3 +261,37 @@ def test_fp64_reference_fixtures_are_consistent():
         D, H, W, seed, sig, nd = gold[f"case{q}_spec"]
         assert gold[f"case{q}_idx"].max() < D * H * W
         assert np.all(np.diff(gold[f"case{q}_idx"]) > 0)
+
+
+def test_solve_refined_is_the_stationary_iteration():
+    """Defect correction with one V-cycle per step (solve_refined, inner_cycles=1) in
+    FP64 is the stationary iteration e <- (I - M A) e of the V-cycle's matrix M
+    (assembled column by column from unit right-hand sides): its residual history
+    is ||A (I - M A)^k x*|| / ||b|| with x* the sparse direct solution."""
+    sx, sy, sz, b = O.make_anisotropic_fields((8, 6, 4), 4, 1.0, 1)
+    op = O.harmonic_faces(sx, sy, sz).astype("f32")   # the outer residual's FP32 faces
+    b = b.astype(np.float64)
+    h = O.build_hierarchy(op, "f64")
+    h.ops = [op] + h.ops[1:]
+    n = b.size
+    M = np.zeros((n, n))
+    for q in range(n):
+        e = np.zeros(n); e[q] = 1.0
+        M[:, q] = O.vcycle(h, e.reshape(b.shape), alpha=1.6).ravel()
+    A = O.face_sparse(op).toarray()
+    xs = np.linalg.solve(A, b.ravel())
+    G = np.eye(n) - M @ A
+    pred, err = [], xs.copy()
+    for _ in range(12):
+        pred.append(np.linalg.norm(A @ err) / np.linalg.norm(b))
+        err = G @ err
+    res = O.solve_refined(op, b, tol=1e-30, storage="f64", inner_cycles=1, alpha=1.6,
+                          max_outer=11, hierarchy=h)
+    np.testing.assert_allclose(res.rel_history[:12], pred, rtol=1e-9, atol=1e-15)
+    # and it converges to the direct solution at the contraction rate of G
+    rho = max(abs(np.linalg.eigvals(G)))
+    assert rho < 0.5
+    res = O.solve_refined(op, b, tol=1e-11, storage="f64", inner_cycles=1, alpha=1.6,
+                          max_outer=100, hierarchy=h)
+    assert res.converged
+    assert np.linalg.norm(res.x.ravel() - xs) / np.linalg.norm(xs) < 1e-9
