@@ -248,6 +248,13 @@ __device__ __forceinline__ float jacobi_form(float x, float ax, float b, float d
   return (x - ax / d) + b / d;
 }
 
+// (x - c A) + c b: the smoother whose 1/diag is a layer weight c
+// (ARTn/Multigrid_cnn.ipynb:126-398, "w - bias(A(w)) + bias(r)" with the bias
+// layer's centre weight 1/107), each product rounded as the layer's output is
+__device__ __forceinline__ float jacobi_mul_form(float x, float ax, float b, float c) {
+  return __fadd_rn(__fsub_rn(x, __fmul_rn(c, ax)), __fmul_rn(c, b));
+}
+
 // ------------------------------------------------------------------ kernels
 template <int TAPS>
 __global__ __launch_bounds__(kBX* kBY) void k_conv27(const float* __restrict__ x,
@@ -274,6 +281,7 @@ __global__ __launch_bounds__(kBX* kBY) void k_conv27(const float* __restrict__ x
         else if (f == HFM_FORM_JACOBI) out = jacobi_form(win.centre(ry), ax, b[n], d);
         else if (f == HFM_FORM_SUBTRACT) out = b[n] - ax;
         else if (f == HFM_FORM_AXPY) out = b[n] + d * ax;
+        else if (f == HFM_FORM_JACOBI_MUL) out = jacobi_mul_form(win.centre(ry), ax, b[n], d);
         if ((form & HFM_FORM_ZERO_OUTLET) && c.i == g.W - 1) out = 0.0f;
         y[n] = out;
       }
@@ -315,6 +323,50 @@ __global__ __launch_bounds__(kBX* kBY) void k_prolong_nearest(const float* __res
   int k, j, i;
   if (!cell_of_thread(gf, k, j, i)) return;
   y[gf.idx(k, j, i)] = xc[gc.idx(k / fz, j / fy, i / fx)];
+}
+
+// UpSampling by (fz, fy, fx) then the all-ones convolution of window size 2
+// on every upsampled axis with 'SAME' padding, which for an even window pads
+// one zero after the end only: y(k, j, i) = sum over k..k+fz-1, j..j+fy-1,
+// i..i+fx-1 (inside the grid) of the upsampled field
+// (ARTn/Multigrid_cnn.ipynb:342-398).  Terms are summed in window order.
+__global__ __launch_bounds__(kBX* kBY) void k_prolong_upsum(const float* __restrict__ xc,
+                                                            float* __restrict__ y, Grid3 gf,
+                                                            Grid3 gc, int fz, int fy, int fx) {
+  int k, j, i;
+  if (!cell_of_thread(gf, k, j, i)) return;
+  float acc = 0.0f;
+  for (int a = 0; a < fz; ++a) {
+    const int kk = k + a;
+    if (kk >= gf.D) break;
+    for (int b = 0; b < fy; ++b) {
+      const int jj = j + b;
+      if (jj >= gf.H) break;
+      for (int c = 0; c < fx; ++c) {
+        const int ii = i + c;
+        if (ii >= gf.W) break;
+        acc += xc[gc.idx(kk / fz, jj / fy, ii / fx)];
+      }
+    }
+  }
+  y[gf.idx(k, j, i)] = acc;
+}
+
+// Fixed-order FP64 sum of n floats by one workgroup: thread t sums t, t +
+// 1024, ... in order, then the 1024 partials are added pairwise in a fixed
+// tree.
+__global__ __launch_bounds__(1024) void k_sum_f64(const float* __restrict__ x, int64_t n,
+                                                  double* __restrict__ out) {
+  __shared__ double part[1024];
+  double acc = 0.0;
+  for (int64_t q = threadIdx.x; q < n; q += 1024) acc += double(x[q]);
+  part[threadIdx.x] = acc;
+  __syncthreads();
+  for (int h = 512; h > 0; h >>= 1) {
+    if (int(threadIdx.x) < h) part[threadIdx.x] += part[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = part[0];
 }
 
 template <bool VAR6, int TAPS>
@@ -401,7 +453,8 @@ __global__ __launch_bounds__(kBX* kBY) void k_correct_jacobi_col(const float* __
       const float ax = wsum<TAPS>(K, v);
       if (c.out && c.j + ry < g.H) {
         const int64_t n = g.idx(k, c.j + ry, c.i);
-        y[n] = jacobi_form(win.centre(ry), ax, b[n], d);
+        y[n] = (flags & HFM_CORRECT_RECIPROCAL) ? jacobi_mul_form(win.centre(ry), ax, b[n], d)
+                                                : jacobi_form(win.centre(ry), ax, b[n], d);
       }
     }
     win.shift();
@@ -853,7 +906,8 @@ extern "C" int hfm_conv27_f32(const float* x, float* y, const float* b, const fl
   uint32_t mask;
   const int fm = form & HFM_FORM_MASK;
   if (!x || !y || !dims_ok(D, H, W) || !make_weights(K, w, mask) || x == y ||
-      fm < HFM_FORM_APPLY || fm > HFM_FORM_AXPY || (form & ~(HFM_FORM_MASK | HFM_FORM_ZERO_OUTLET)) ||
+      fm < HFM_FORM_APPLY || fm > HFM_FORM_JACOBI_MUL ||
+      (form & ~(HFM_FORM_MASK | HFM_FORM_ZERO_OUTLET)) ||
       (fm != HFM_FORM_APPLY && !b)) {
     set_error("hfm_conv27_f32: bad arguments");
     return HFM_EINVAL;
@@ -915,6 +969,28 @@ extern "C" int hfm_prolong_nearest_f32(const float* xc, float* y, int64_t Dc, in
   return launch_status("hfm_prolong_nearest_f32");
 }
 
+extern "C" int hfm_prolong_upsum_f32(const float* xc, float* y, int64_t Dc, int64_t Hc,
+                                     int64_t Wc, int fz, int fy, int fx, void* stream) {
+  if (!xc || !y || xc == y || !factors_ok(fz, fy, fx) || !dims_ok(Dc * fz, Hc * fy, Wc * fx)) {
+    set_error("hfm_prolong_upsum_f32: bad arguments");
+    return HFM_EINVAL;
+  }
+  const Grid3 gc{int(Dc), int(Hc), int(Wc)};
+  const Grid3 gf{int(Dc * fz), int(Hc * fy), int(Wc * fx)};
+  hipLaunchKernelGGL(k_prolong_upsum, grid_for(gf), dim3(kBX, kBY), 0, as_stream(stream), xc, y,
+                     gf, gc, fz, fy, fx);
+  return launch_status("hfm_prolong_upsum_f32");
+}
+
+extern "C" int hfm_sum_f64(const float* x, int64_t n, double* out, void* stream) {
+  if (!x || !out || n < 0) {
+    set_error("hfm_sum_f64: bad arguments");
+    return HFM_EINVAL;
+  }
+  hipLaunchKernelGGL(k_sum_f64, dim3(1), dim3(1024), 0, as_stream(stream), x, n, out);
+  return launch_status("hfm_sum_f64");
+}
+
 extern "C" int hfm_prolong_jacobi_f32(const float* wc, const float* r, float* y,
                                       const float K[27], float d, const float* const m[6],
                                       const float* tot, int64_t D, int64_t H, int64_t W, int fz,
@@ -952,7 +1028,8 @@ int correct_jacobi(const char* what, const float* x, const CorrW& cw, bool prol,
   Var6 V{};
   const bool var6 = m != nullptr;
   if (!x || !cw.w || !b || !y || y == x || y == cw.w || !dims_ok(D, H, W) ||
-      (flags & ~HFM_CORRECT_ZERO_OUTLET) ||
+      (flags & ~(HFM_CORRECT_ZERO_OUTLET | HFM_CORRECT_RECIPROCAL)) ||
+      (var6 && (flags & HFM_CORRECT_RECIPROCAL)) ||
       (var6 ? !make_var6(m, tot, V, true) : !make_weights(K, w, mask))) {
     set_error("%s: bad arguments", what);
     return HFM_EINVAL;

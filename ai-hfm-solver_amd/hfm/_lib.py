@@ -22,9 +22,10 @@ from ._build import LIB_PATH
 HFM_OK = 0
 
 STORE = {"f32": 0, "bf16": 1}
-FORM = {"apply": 0, "resid": 1, "jacobi": 2, "subtract": 3, "axpy": 4}
+FORM = {"apply": 0, "resid": 1, "jacobi": 2, "subtract": 3, "axpy": 4, "jacobi_mul": 5}
 FORM_ZERO_OUTLET = 16
 CORRECT_ZERO_OUTLET = 1
+CORRECT_RECIPROCAL = 2
 SUM_SOURCE_BC = 1
 BC = {"top_copy": 0, "pressure": 1, "source": 2, "indicator": 3, "zero_outlet": 4}
 SWEEP = {"zero": 0, "plain": 1, "correct": 2}
@@ -65,6 +66,8 @@ SIGNATURES = {
     "hfm_var6_f32": (_int, [_vp, _vp, _vp, _vp, _vp, _i64, _i64, _i64, _int, _vp]),
     "hfm_restrict_mean_f32": (_int, [_vp, _vp, _i64, _i64, _i64, _int, _int, _int, _vp]),
     "hfm_prolong_nearest_f32": (_int, [_vp, _vp, _i64, _i64, _i64, _int, _int, _int, _vp]),
+    "hfm_prolong_upsum_f32": (_int, [_vp, _vp, _i64, _i64, _i64, _int, _int, _int, _vp]),
+    "hfm_sum_f64": (_int, [_vp, _i64, _vp, _vp]),
     "hfm_prolong_jacobi_f32": (_int, [_vp, _vp, _vp, _vp, _f32, _vp, _vp, _i64, _i64, _i64,
                                       _int, _int, _int, _vp]),
     "hfm_correct_jacobi_f32": (_int, [_vp, _vp, _vp, _vp, _vp, _f32, _vp, _vp, _i64, _i64, _i64,

@@ -385,6 +385,91 @@ class SawtoothMG:
         return values
 
 
+def prolong_upsum(x):
+    """``UpSampling2D(2)`` + ``Conv2D(2x2 of ones, padding='SAME')`` of
+    ARTn/Multigrid_cnn.ipynb:342-398 (any 1-3 spatial axes): output i sums the
+    upsampled cells i and i + 1 of every axis, zero past the end."""
+    x = _prep(x)
+    (D, H, W), nd = _spatial(x)
+    fz, fy, fx = _factors(nd)
+    out_shape = list(x.shape)
+    off = 1 if x.dim() > nd else 0
+    for a in range(nd):
+        out_shape[off + a] *= 2
+    y = torch.empty(out_shape, dtype=torch.float32, device=x.device)
+    L.call("hfm_prolong_upsum_f32", x.data_ptr(), y.data_ptr(), D, H, W, fz, fy, fx,
+           L.stream_handle(x.device))
+    return y
+
+
+def field_sum(x) -> torch.Tensor:
+    """Sum of a field's values in FP64 (fixed order) as a one-element device tensor."""
+    x = _prep(x)
+    out = torch.empty(1, dtype=torch.float64, device=x.device)
+    L.call("hfm_sum_f64", x.data_ptr(), x.numel(), out.data_ptr(), L.stream_handle(x.device))
+    return out
+
+
+class MultigridCNN:
+    """The 2D multigrid of ARTn/Multigrid_cnn.ipynb (cells 2-21).
+
+    One fixed 3x3 operator ``A`` at every level (``w3``), a smoother whose
+    divisor is a layer weight, ``w - bias(A(w)) + bias(r)`` with ``bias`` the
+    centre weight ``c`` (1/107), 2x2 mean restriction down to 1 x 1, and the
+    upsampling prolongation ``prolong_upsum``.  A cycle: pre-sweep; r = A x - b;
+    restrict; from w = 0, ``j_itr`` sweeps at the coarsest level and after each
+    prolongation below the finest; at the finest, the prolongation and one
+    sweep against r; x -= w; post-sweep.  ``last_norm`` is the notebook's
+    residual monitor ``sum(r) / (nx ny)`` of the last cycle (a device tensor).
+    """
+
+    def __init__(self, weights=None, bias: float = 1.0 / 107.0, j_itr: int = 2):
+        K = np.array([[0.0, -1.5, 0.0], [-51.5, 107.0, -1.5], [0.0, -51.5, 0.0]]) \
+            if weights is None else weights
+        self.A = FixedConv(K)
+        self.c = float(np.float32(bias))
+        self.j_itr = int(j_itr)
+        self._rsum, self._n = None, 0
+
+    def _sweep(self, w, r):
+        return self.A._form(w, r, "jacobi_mul", False, self.c)
+
+    @property
+    def last_norm(self) -> Optional[float]:
+        """sum(r) / (nx ny) of the last cycle's residual (reads it back)."""
+        return None if self._rsum is None else float(self._rsum) / self._n
+
+    def cycle(self, values, b):
+        values, b = _prep(values, "values"), _prep(b, "b")
+        if b.shape != values.shape:
+            raise ValueError("values and b must share a shape")
+        values = self._sweep(values, b)
+        r = self.A.residual(values, b)
+        rs = [r]
+        while True:
+            dims, nd = _spatial(rs[-1])
+            if min(dims[3 - nd:]) <= 1:
+                break
+            rs.append(restrict_mean(rs[-1]))
+        w = torch.zeros_like(rs[-1])
+        for _ in range(self.j_itr):
+            w = self._sweep(w, rs[-1])
+        for lev in range(len(rs) - 2, 0, -1):
+            w = prolong_upsum(w)
+            for _ in range(self.j_itr):
+                w = self._sweep(w, rs[lev])
+        if len(rs) > 1:
+            w = self._sweep(prolong_upsum(w), r)
+        # values - w and the post-sweep in one pass
+        (D, H, W), _ = _spatial(values)
+        out = torch.empty_like(values)
+        L.call("hfm_correct_jacobi_f32", values.data_ptr(), w.data_ptr(), b.data_ptr(),
+               out.data_ptr(), _kptr(self.A.K), self.c, None, None, D, H, W,
+               L.CORRECT_RECIPROCAL, L.stream_handle(values.device))
+        self._rsum, self._n = field_sum(r), r.numel()
+        return out
+
+
 def level_layers(K, fine_shape: Sequence[int], nlevels: int, prefix: str = "CNN3D") -> dict:
     """The reference's per-level layer registry (UM/AI_HFM_v3.ipynb:587-623):
     ``{prefix}_A_<tag>``, ``{prefix}_res_<tag>``, ``{prefix}_prol_<tag>`` with

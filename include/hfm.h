@@ -63,11 +63,14 @@ extern "C" {
 #define HFM_FORM_JACOBI 2   /* y = x - A(x)/d + b/d            */
 #define HFM_FORM_SUBTRACT 3 /* y = b - A x                     */
 #define HFM_FORM_AXPY 4     /* y = b + d A x  (explicit stages) */
+#define HFM_FORM_JACOBI_MUL 5 /* y = (x - d A x) + d b: d the smoother's reciprocal weight */
 #define HFM_FORM_MASK 15
 #define HFM_FORM_ZERO_OUTLET 16 /* flag: y = 0 on the x = W-1 plane */
 
 /* flags of hfm_correct_jacobi_f32 / hfm_conv27_sum_f32 */
 #define HFM_CORRECT_ZERO_OUTLET 1 /* (x - w) = 0 on the x = W-1 plane before the sweep */
+#define HFM_CORRECT_RECIPROCAL 2  /* the sweep in the HFM_FORM_JACOBI_MUL form: d is the
+                                     reciprocal weight (constant kernel only) */
 #define HFM_SUM_SOURCE_BC 1       /* x = 0 plane takes the value of the x = 1 plane   */
 
 /* boundary fills for hfm_bc_f32 */
@@ -105,6 +108,17 @@ int hfm_restrict_mean_f32(const float* x, float* y, int64_t D, int64_t H, int64_
 /* y (Dc*fz, Hc*fy, Wc*fx) = nearest-neighbour repeat of x (Dc, Hc, Wc). */
 int hfm_prolong_nearest_f32(const float* xc, float* y, int64_t Dc, int64_t Hc, int64_t Wc,
                             int fz, int fy, int fx, void* stream);
+
+/* y (Dc*fz, Hc*fy, Wc*fx) = UpSampling by (fz, fy, fx) followed by the
+ * all-ones window-2 convolution with 'SAME' padding on each upsampled axis
+ * (zero after the end): the prolongation of ARTn/Multigrid_cnn.ipynb:342-398
+ * (UpSampling2D then Conv2D(kernel 2x2 of ones, padding='SAME')). */
+int hfm_prolong_upsum_f32(const float* xc, float* y, int64_t Dc, int64_t Hc, int64_t Wc, int fz,
+                          int fy, int fx, void* stream);
+
+/* out[0] = sum of the n floats of x in FP64, fixed order (one workgroup): the
+ * residual monitor sum(r) / (nx ny) of ARTn/Multigrid_cnn.ipynb:605. */
+int hfm_sum_f64(const float* x, int64_t n, double* out, void* stream);
 
 /* y = J(P wc) against r: P = nearest repeat by (fz,fy,fx) onto (D,H,W).
  * Operator: constant kernel K with divisor d (m == NULL) or the var6 fields

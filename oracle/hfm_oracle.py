@@ -30,7 +30,14 @@ Pinning status (see the tests under tests/test_oracle_*.py):
   conv_same            golden (3D weight layout known answers), closed form,
                        library identity (torch conv3d)
   restrict/prolong     closed forms, R(P(x)) = x
-  sawtooth_cycle       golden vectors (1D w_32/w_4, 2D w_2/w_4)
+  sawtooth_cycle       golden vectors (1D w_32/w_4, 2D w_2/w_4); the urban form
+                       (no fine-level sweep) by a dense matrix restatement
+  urban / INHALE steps dense cycle restatement, y-mirror equivariance, zero state
+  2D explicit update   kernel sums, exact mass / drift / variance moments
+  Multigrid_cnn cycle  dense matrix restatement (window-sum prolongation with
+                       TensorFlow's after-the-end SAME padding); prolongation
+                       against torch (interpolate + pad + conv2d)
+  defect correction    the stationary iteration matrix (I - M A)^k
   var6 operator        reduces to the constant operator, dense brute force
   27-point layers      closed forms (linear fields, x^2)
   face operator        scipy.sparse assembly, sine eigenmodes, symmetry
@@ -419,6 +426,90 @@ def implicit_ad1d_kernel(dt=10.0, dx=1.0, Dx=0.1, ub=1.0):
     return np.array([-ub * dt / dx - Dx * dt / dx ** 2,
                      1 + 2 * Dx * dt / dx ** 2 + ub * dt / dx,
                      -Dx * dt / dx ** 2])
+
+
+# --- 2D multigrid of ARTn/Multigrid_cnn.ipynb (cells 2-21)
+
+MGCNN_A = np.array([[0.0, -1.5, 0.0], [-51.5, 107.0, -1.5], [0.0, -51.5, 0.0]])
+MGCNN_BIAS = float(np.float32(1.0 / 107.0))   # the bias layer's centre weight as stored
+
+
+def prolong_upsum(x: np.ndarray, factors: Optional[Sequence[int]] = None) -> np.ndarray:
+    """``UpSampling2D(2)`` then ``Conv2D(kernel 2x2 of ones, padding='SAME')``
+    (``ARTn/Multigrid_cnn.ipynb:342-398``).  TensorFlow's SAME padding of an
+    even window of 2 puts its one padding cell after the end, so output i sums
+    upsampled cells i and i + 1 (0 past the end) on every upsampled axis."""
+    f = tuple(factors) if factors is not None else (2,) * np.ndim(x)
+    y = prolong_nearest(np.asarray(x, dtype=np.float64), f)
+    for ax, k in enumerate(f):
+        if k == 2:
+            nxt = np.zeros_like(y)
+            src = [slice(None)] * y.ndim
+            dst = [slice(None)] * y.ndim
+            src[ax] = slice(1, None)
+            dst[ax] = slice(0, -1)
+            nxt[tuple(dst)] = y[tuple(src)]
+            y = y + nxt
+    return y
+
+
+def mgcnn_initial_condition(n: int = 128) -> np.ndarray:
+    """Cells 14-16: a Gaussian (gamma = 10) times 0.8 on the interior plus a
+    0.2 "blob" lattice (every 4th cell of every 4th row, and the cells off
+    that lattice's columns in the other rows)."""
+    x = -63.5 + 128.0 / n * np.arange(n)
+    T = np.zeros((n, n))
+    g2 = 2 * 10.0 ** 2
+    for i in range(1, n - 1):
+        for j in range(1, n - 1):
+            T[i, j] = math.exp(-((x[i]) ** 2 + (x[j]) ** 2) / g2)
+    blob = np.zeros((n, n))
+    for i in range(n):
+        for j in range(n):
+            if (i % 4 == 0 and j % 4 == 0) or (i % 4 != 0 and j % 4 != 0):
+                blob[i, j] = 0.2
+    return T * 0.8 + blob
+
+
+def mgcnn_cycle(values, b, K=MGCNN_A, c=MGCNN_BIAS, j_itr: int = 2,
+                record: Optional[dict] = None):
+    """One cycle of ``ARTn/Multigrid_cnn.ipynb`` cell 21 (2D, levels down to 1 x 1).
+
+    The smoother is ``w - bias(A(w)) + bias(r)`` with ``bias`` the layer that
+    multiplies by its centre weight c = 1/107.  Order: a pre-sweep on A x = b;
+    r = A x - b; r restricted (2x2 mean) to every level; from w = 0, ``j_itr``
+    sweeps at the coarsest level; at each finer level but the finest, the
+    upsampling prolongation (``prolong_upsum``) and ``j_itr`` sweeps; at the
+    finest, the prolongation and one sweep against r; x -= w; a post-sweep
+    on A x = b.  Returns (x, norm) with the notebook's residual monitor norm
+    = sum(r) / (nx ny) of the cycle's r (``:605``)."""
+    jac = lambda w, r: w - c * conv_same(w, K) + c * r  # noqa: E731
+    values = np.asarray(values, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    values = jac(values, b)
+    r = conv_same(values, K) - b
+    rs = [r]
+    while min(rs[-1].shape) > 1:
+        rs.append(restrict_mean(rs[-1]))
+    nlev = len(rs)
+    w = np.zeros_like(rs[-1])
+    for _ in range(j_itr):
+        w = jac(w, rs[-1])
+    if record is not None:
+        record[nlev - 1] = w
+    for lev in range(nlev - 2, 0, -1):
+        w = prolong_upsum(w)
+        for _ in range(j_itr):
+            w = jac(w, rs[lev])
+        if record is not None:
+            record[lev] = w
+    if nlev > 1:
+        w = jac(prolong_upsum(w), r)
+    if record is not None:
+        record[0] = w
+    values = values - w
+    values = jac(values, b)
+    return values, float(r.sum()) / r.size
 
 
 def urban_levels(shape, coarsest: int = 1):

@@ -242,3 +242,34 @@ def test_velocity_fill_in_place_bitwise(dev, shape, profile):
     if not profile:
         for g, o in zip(got, O.boundary_condition_velocity(u, v, w, ub=np.float32(1.25))):
             np.testing.assert_array_equal(g.cpu().numpy(), o)
+
+
+def test_prolong_upsum_and_field_sum(dev):
+    from hfm import layers
+    rng = np.random.default_rng(31)
+    for shape in ((6, 10), (3, 4, 5), (7,)):
+        c = rng.standard_normal(shape).astype(np.float32)
+        g = H(layers.prolong_upsum(T(c[None, ..., None], dev)))[0, ..., 0]
+        assert rel(g, O.prolong_upsum(c.astype(np.float64))) < 1e-7
+    x = rng.standard_normal((33, 70)).astype(np.float32)
+    s = float(layers.field_sum(T(x, dev)))
+    assert abs(s - x.astype(np.float64).sum()) < 1e-12 * np.abs(x).sum()
+
+
+def test_multigrid_cnn_cycles_match_oracle(dev):
+    """ARTn/Multigrid_cnn.ipynb cell 21 on its 128^2 start (cells 14-16), b = the
+    start: three cycles and their residual monitors sum(r)/(nx ny) against the
+    oracle.  (The cycle as written grows the residual by about 30x per cycle: the
+    4x window-sum prolongation over-corrects this advection-dominated operator on
+    every level; the oracle does the same.)"""
+    from hfm import layers
+    v0 = O.mgcnn_initial_condition(128)
+    mg = layers.MultigridCNN()
+    xt = T(v0[None, ..., None], dev)
+    bt = xt.clone()
+    xo = v0
+    for _ in range(3):
+        xt = mg.cycle(xt, bt)
+        xo, norm = O.mgcnn_cycle(xo, v0)
+        assert rel(H(xt)[0, ..., 0], xo) < 1e-5
+        assert abs(mg.last_norm - norm) <= 1e-5 * abs(norm)

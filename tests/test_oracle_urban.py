@@ -266,3 +266,100 @@ def test_ad2d_step_moments(name):
     # the upwind kernel carries the fixed velocity u dt/dx = 0.1 along +x and rows
     if name == "AD2D_UPWIND":
         np.testing.assert_allclose(m1, [-0.1, 0.1], atol=1e-15)
+
+
+# ---------------------------------------------------------------- Multigrid_cnn 2D cycle
+
+def test_prolong_upsum_is_upsampling_then_same_conv():
+    """prolong_upsum = torch nearest upsampling, one zero of padding after the end of
+    each axis, and a 2x2 all-ones cross-correlation (TensorFlow's SAME padding of an
+    even window); a constant coarse field gives 4 c inside, 2 c on the last row or
+    column and c in the last corner."""
+    torch = pytest.importorskip("torch")
+    F = torch.nn.functional
+    rng = np.random.default_rng(4)
+    c = rng.standard_normal((5, 7))
+    t = torch.from_numpy(c)[None, None]
+    up = F.interpolate(t, scale_factor=2, mode="nearest")
+    ref = F.conv2d(F.pad(up, (0, 1, 0, 1)), torch.ones(1, 1, 2, 2, dtype=torch.float64))[0, 0]
+    np.testing.assert_allclose(O.prolong_upsum(c), ref.numpy(), rtol=1e-14, atol=1e-14)
+    y = O.prolong_upsum(np.full((3, 3), 2.0))
+    assert np.all(y[:-1, :-1] == 8.0) and np.all(y[-1, :-1] == 4.0) and np.all(y[:-1, -1] == 4.0)
+    assert y[-1, -1] == 2.0
+
+
+def _upsum_1d(n):
+    """(2n x n): nearest repeat then the window {i, i+1} sum, zero past the end."""
+    S = np.zeros((2 * n, 2 * n))
+    for i in range(2 * n):
+        S[i, i] = 1.0
+        if i + 1 < 2 * n:
+            S[i, i + 1] = 1.0
+    return S @ repeat_1d(n)
+
+
+def dense_conv2d_matrix(shape, K):
+    H, W = shape
+    A = np.zeros((H * W, H * W))
+    for j in range(H):
+        for i in range(W):
+            for a in range(3):
+                for b in range(3):
+                    jj, ii = j + a - 1, i + b - 1
+                    if 0 <= jj < H and 0 <= ii < W:
+                        A[j * W + i, jj * W + ii] += K[a, b]
+    return A
+
+
+@pytest.mark.parametrize("shape", [(4, 4), (8, 4)])
+def test_mgcnn_cycle_matches_dense_restatement(shape):
+    """ARTn/Multigrid_cnn.ipynb cell 21 as matrices: pre-sweep, r = A x - b, 2x2 mean
+    restriction to every level, j_itr = 2 sweeps from zero at the coarsest and after
+    each prolongation below the finest, one sweep at the finest, x -= w, post-sweep;
+    the smoother w - c A w + c r with c = 1/107."""
+    rng = np.random.default_rng(15)
+    x, b = rng.standard_normal(shape), rng.standard_normal(shape)
+    K, c = O.MGCNN_A, O.MGCNN_BIAS
+    shapes = [shape]
+    while min(shapes[-1]) > 1:
+        shapes.append(tuple(n // 2 for n in shapes[-1]))
+    A = [dense_conv2d_matrix(s, K) for s in shapes]
+    R = [np.kron(mean_1d(s[0]), mean_1d(s[1])) for s in shapes[:-1]]
+    P = [np.kron(_upsum_1d(s[0]), _upsum_1d(s[1])) for s in shapes[1:]]
+    J = lambda lev, w, r: w - c * (A[lev] @ w) + c * r  # noqa: E731
+    xv, bv = x.ravel(), b.ravel()
+    xv = J(0, xv, bv)
+    rs = [A[0] @ xv - bv]
+    for lev in range(len(shapes) - 1):
+        rs.append(R[lev] @ rs[-1])
+    L = len(shapes) - 1
+    w = np.zeros(rs[L].size)
+    for _ in range(2):
+        w = J(L, w, rs[L])
+    for lev in range(L - 1, 0, -1):
+        w = P[lev] @ w
+        for _ in range(2):
+            w = J(lev, w, rs[lev])
+    w = J(0, P[0] @ w, rs[0])
+    ref = J(0, xv - w, bv).reshape(shape)
+    got, norm = O.mgcnn_cycle(x, b)
+    assert rel(got, ref) < 1e-12
+    assert abs(norm - rs[0].sum() / rs[0].size) < 1e-12
+    # padding before the start instead of after the end is another prolongation
+    orig = O.prolong_upsum
+    try:
+        O.prolong_upsum = lambda w: np.flip(orig(np.flip(w)))
+        other, _ = O.mgcnn_cycle(x, b)
+    finally:
+        O.prolong_upsum = orig
+    assert rel(other, ref) > 1e-3
+
+
+def test_mgcnn_initial_condition():
+    """Cells 14-16: the interior Gaussian times 0.8 peaks near the centre, the 0.2
+    lattice covers 10 of every 16 cells."""
+    v = O.mgcnn_initial_condition(128)
+    blob = v[0]                          # row 0: the Gaussian is zero on the edge rows
+    assert np.isclose(blob[::4], 0.2).all() and np.isclose(blob[1::4], 0.0).all()
+    assert abs((v > 0.19).mean() - 10 / 16) < 0.05
+    assert 0.95 < v.max() < 1.0 + 1e-12
