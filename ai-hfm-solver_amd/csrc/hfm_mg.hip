@@ -1891,13 +1891,14 @@ namespace {
 constexpr int kTilePlanes = 32;
 }  // namespace
 
-extern "C" int hfm_mg_tile(int64_t D, int64_t H, int64_t W, int32_t* tile_rows,
-                           int32_t* tile_planes, int64_t* ntiles) {
+extern "C" int hfm_mg_tile(int64_t D, int64_t H, int64_t W, int32_t max_planes,
+                           int32_t* tile_rows, int32_t* tile_planes, int64_t* ntiles) {
   if (!tile_rows || !tile_planes || !ntiles || D < 1 || H < 1 || W < 1 ||
-      D * H * W > INT32_MAX) {
+      D * H * W > INT32_MAX || max_planes == 1 || max_planes < 0 || max_planes > 4096) {
     set_error("hfm_mg_tile: bad arguments");
     return HFM_EINVAL;
   }
+  const int64_t zmax = max_planes ? max_planes : kTilePlanes;
   if (W % kRun) {
     set_error("hfm_mg_tile: W = %lld is not a multiple of %d", (long long)W, kRun);
     return HFM_EINVAL;
@@ -1916,7 +1917,7 @@ extern "C" int hfm_mg_tile(int64_t D, int64_t H, int64_t W, int32_t* tile_rows,
     ny = (H + rmax - 1) / rmax;
     R = 2 * ((H + 2 * ny - 1) / (2 * ny));  // balanced, even (restriction pairs)
   }
-  const int64_t nz = (D + kTilePlanes - 1) / kTilePlanes;
+  const int64_t nz = (D + zmax - 1) / zmax;
   const int64_t Z = 2 * ((D + 2 * nz - 1) / (2 * nz));
   *tile_rows = int32_t(R);
   *tile_planes = int32_t(Z);

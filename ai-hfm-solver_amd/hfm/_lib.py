@@ -91,8 +91,8 @@ SIGNATURES = {
     "hfm_mg_resid_restrict": (_int, [_vp, _vp, _i64, _vp, _int, _int, _vp]),
     "hfm_mg_smooth_restrict": (_int, [_vp, _vp, _i64, _vp, _int, _f32, _vp]),
     "hfm_mg_smooth2": (_int, [_vp, _vp, _i64, _vp, _int, _int, _f32, _f32, _vp, _vp]),
-    "hfm_mg_tile": (_int, [_i64, _i64, _i64, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
-                           C.POINTER(_i64)]),
+    "hfm_mg_tile": (_int, [_i64, _i64, _i64, C.c_int32, C.POINTER(C.c_int32),
+                           C.POINTER(C.c_int32), C.POINTER(_i64)]),
     "hfm_mg_coarse_solve": (_int, [_vp, _i64, _vp, _int, _vp]),
     "hfm_mg_coarse_cycle": (_int, [_vp, _vp, _i64, _vp, _int, _int, _int, _f32, _f32, _vp]),
     "hfm_fcg_init": (_int, [_vp, _vp, _i64, _vp, _int, _vp, _vp]),
@@ -202,13 +202,14 @@ def ptr_array(ptrs):
     return arr
 
 
-def tile(shape):
+def tile(shape, max_planes: int = 0):
     """(tile_rows, tile_planes, tiles) of the fused pass pairs for a level of
-    extents (D, H, W), or None when the level cannot be tiled (hfm_mg_tile)."""
+    extents (D, H, W) with at most ``max_planes`` planes per tile (0: the
+    library's 32), or None when the level cannot be tiled (hfm_mg_tile)."""
     lib = load()
     R, Z, n = C.c_int32(), C.c_int32(), _i64()
     D, H, W = (int(v) for v in shape)
-    if lib.hfm_mg_tile(D, H, W, C.byref(R), C.byref(Z), C.byref(n)) != HFM_OK:
+    if lib.hfm_mg_tile(D, H, W, int(max_planes), C.byref(R), C.byref(Z), C.byref(n)) != HFM_OK:
         return None
     return R.value, Z.value, n.value
 

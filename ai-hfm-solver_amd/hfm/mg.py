@@ -151,6 +151,9 @@ class Params:
     # iteration with an FP64 residual and solution, ART/Anisotropic_3d_Multigrid_
     # 7point.ipynb:402-454's cycle loop stopped at a tolerance)
     method: str = "fcg"
+    # planes per tile of the fused pass pairs (at most; hfm_mg_tile): fewer
+    # planes give more tiles (blocks) at the cost of more halo planes
+    tile_planes: int = 32
 
     def __post_init__(self):
         if self.storage not in ELEM:
@@ -163,6 +166,8 @@ class Params:
             raise ValueError("nu1 and nu2 must be >= 1")
         if self.method not in L.METHOD:
             raise ValueError(f"method must be one of {sorted(L.METHOD)}")
+        if self.tile_planes < 2:
+            raise ValueError("tile_planes must be >= 2")
 
 
 class _Arena:
@@ -462,7 +467,7 @@ def fused_levels(inst: Instance, p: Params, lc: int):
     for lev in range(min(lc, len(inst.levels) - 1, p.fuse_levels)):
         if lev == 0 and not inst.pool.fuse_fine:
             continue
-        tl = L.tile(inst.levels[lev])
+        tl = L.tile(inst.levels[lev], p.tile_planes)
         if tl is None:
             continue
         if p.nu1 >= 2:
@@ -644,7 +649,8 @@ class Plan:
         for inst in insts:
             # records depend on where the coarse cycle starts (fused levels
             # end there) and on fusion itself
-            key = (lcs[inst.slot], p.fuse, p.fuse_levels, pool.fuse_fine, p.nu1, p.nu2)
+            key = (lcs[inst.slot], p.fuse, p.fuse_levels, pool.fuse_fine, p.nu1, p.nu2,
+                   p.tile_planes)
             if getattr(inst, "_tpl", None) is None or inst._tpl_key != key:
                 inst._tpl = instance_templates(inst, p, lcs[inst.slot])
                 inst._tpl_key = key
@@ -824,9 +830,11 @@ class MGBatch:
                  iters_per_graph: int = 4, use_graph: bool = True, shrink: float = 0.5,
                  outer: Optional[str] = None, blocks: Optional[dict] = None,
                  coarse_cells: int = 4096, fuse: bool = True, fuse_levels: int = 1,
-                 fuse_min_tiles: int = 256, streams: int = 2, method: str = "fcg"):
+                 fuse_min_tiles: int = 256, streams: int = 2, method: str = "fcg",
+                 tile_planes: int = 32):
         self.params = Params(storage, tol, maxit, nu1, nu2, omega, alpha, max_coarse_edge, outer,
-                             blocks, coarse_cells, fuse, fuse_levels, fuse_min_tiles, method)
+                             blocks, coarse_cells, fuse, fuse_levels, fuse_min_tiles, method,
+                             tile_planes)
         shapes = [check_shape(s, max_coarse_edge)[0] for s in shapes]
         if not shapes:
             raise ValueError("empty batch")
@@ -923,8 +931,30 @@ class MGBatch:
         p = self.params
         if not p.fuse:
             return False
-        tiles = [L.tile(level_shapes(s, p.max_coarse_edge)[0]) for s in shapes]
+        tiles = [L.tile(level_shapes(s, p.max_coarse_edge)[0], p.tile_planes) for s in shapes]
         return sum(t[2] for t in tiles if t is not None) >= p.fuse_min_tiles
+
+    def set_tile_planes(self, n: int):
+        """At most ``n`` planes per tile of the fused pass pairs; rebuilds the
+        plans (the pairs' outputs do not depend on the tiling; the fine post
+        pair's r.z partials are summed per tile)."""
+        if int(n) < 2:
+            raise ValueError("tile_planes must be >= 2")
+        self.params.tile_planes = int(n)
+        self.pool.fuse_fine = self._fuse_fine(self.shapes)
+        for inst in self.insts:
+            inst._tpl = None
+        self._replan()
+
+    def set_schedule(self, nu1: int, nu2: int, alpha: float):
+        """The V-cycle's sweeps before / after the coarse correction and the
+        correction's over-relaxation factor; rebuilds the plans."""
+        if int(nu1) < 1 or int(nu2) < 1:
+            raise ValueError("nu1 and nu2 must be >= 1")
+        self.params.nu1, self.params.nu2, self.params.alpha = int(nu1), int(nu2), float(alpha)
+        for inst in self.insts:
+            inst._tpl = None
+        self._replan()
 
     def set_fuse(self, fuse: bool, levels: Optional[int] = None,
                  min_tiles: Optional[int] = None):

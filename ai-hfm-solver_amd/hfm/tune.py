@@ -1,11 +1,16 @@
-"""Launch-shape autotuner for the batched multigrid plans.
+"""Autotuner for the batched multigrid plans: launch shapes, tiles, schedules.
 
-Two choices, both timed on the batch's own data with HIP events:
+Timed on the batch's own data with HIP events:
 
 * the level size below which an instance's V-cycle tail runs in one
   workgroup-per-instance launch (``hfm_mg_coarse_cycle``) instead of one
   launch per pass and level (``tune_coarse``, whole V-cycles timed);
-* the threads per block of the remaining per-level launches (``tune_blocks``).
+* the threads per block of the remaining per-level launches (``tune_blocks``);
+* the planes per tile of the fused pass pairs (``tune_tiles``, whole V-cycles
+  timed): more planes walk fewer halo planes, fewer planes give more blocks;
+* the sweep schedule (nu1, nu2) and the coarse-correction factor alpha
+  (``tune_schedule``, the objective being the time of the whole solve to its
+  tolerance: a schedule trades iterations against the cost of one).
 
 The sweeps without a reduction (every V-cycle sweep but the fine level's
 last, which carries the r.z partials) and the residual + restriction launches
@@ -138,11 +143,69 @@ def tune_coarse(mg: MGBatch, reps: int = 5, candidates=None, max_cells: int = 1 
     return best
 
 
-def tune(mg: MGBatch, reps: int = 5) -> dict:
-    """The coarse-cycle threshold, then the block sizes of the remaining launches."""
+TILE_PLANES = (8, 16, 32, 64)
+SCHEDULES = ((1, 1, 1.6), (1, 2, 1.6), (2, 1, 1.6), (2, 2, 1.0), (2, 2, 1.3), (2, 2, 1.6),
+             (2, 3, 1.6), (3, 3, 1.6))
+
+
+def tune_tiles(mg: MGBatch, reps: int = 5, candidates=TILE_PLANES) -> int:
+    """Choose the planes per tile of the fused pass pairs by timing whole
+    V-cycles (all instances active, graph-replayed).  The pairs compute the
+    same values for every tiling; only the fine post pair's r.z partials are
+    summed per tile."""
+    mg._need_gpu()
+    t = {}
+    for z in candidates:
+        mg.set_tile_planes(int(z))
+        _activate(mg)
+        plan = mg.plan
+        t[int(z)] = _graph_time_ms(mg, lambda s_: plan.precondition(s_), reps)
+    best = min(t, key=t.get)
+    mg.set_tile_planes(best)
+    mg.tile_times = t
+    return best
+
+
+def tune_schedule(mg: MGBatch, candidates=SCHEDULES) -> Tuple[int, int, float]:
+    """Choose (nu1, nu2, alpha) by the time of the batch's whole solve to its
+    tolerance (a warm solve captures each schedule's graphs first); schedules
+    whose solve leaves an instance unconverged are discarded.  Installs and
+    returns the fastest."""
+    mg._need_gpu()
+    dev = mg.device
+    t = {}
+    for sch in candidates:
+        mg.set_schedule(*sch)
+        info = mg.solve()
+        if not bool((info["status"] == 1).all()):
+            t[tuple(sch)] = math.inf
+            continue
+        torch.cuda.synchronize(dev)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        info = mg.solve()
+        e1.record()
+        e1.synchronize()
+        t[tuple(sch)] = e0.elapsed_time(e1) if bool((info["status"] == 1).all()) else math.inf
+    best = min(t, key=t.get)
+    if not math.isfinite(t[best]):
+        raise RuntimeError("no schedule converged")
+    mg.set_schedule(*best)
+    mg.schedule_times = t
+    return best
+
+
+def tune(mg: MGBatch, reps: int = 5, schedule: bool = False) -> dict:
+    """The coarse-cycle threshold, the block sizes of the remaining launches,
+    the fused pairs' planes per tile and, with ``schedule``, the sweep
+    schedule."""
     n = tune_coarse(mg, reps)
     blocks = tune_blocks(mg, reps)
-    return {"coarse_cells": n, "blocks": blocks}
+    planes = tune_tiles(mg, reps)
+    out = {"coarse_cells": n, "blocks": blocks, "tile_planes": planes}
+    if schedule:
+        out["schedule"] = list(tune_schedule(mg))
+    return out
 
 
 def batch_class(mg: MGBatch) -> str:
@@ -171,13 +234,24 @@ class TuneCache:
         rec = self.data.get(self._dev(mg), {}).get(batch_class(mg))
         if rec is None:
             return None
-        return {"coarse_cells": int(rec["coarse_cells"]),
-                "blocks": {tuple(json.loads(k)): int(v) for k, v in rec["blocks"].items()}}
+        out = {"coarse_cells": int(rec["coarse_cells"]),
+               "blocks": {tuple(json.loads(k)): int(v) for k, v in rec["blocks"].items()}}
+        if "tile_planes" in rec:
+            out["tile_planes"] = int(rec["tile_planes"])
+        if "schedule" in rec:
+            nu1, nu2, alpha = rec["schedule"]
+            out["schedule"] = [int(nu1), int(nu2), float(alpha)]
+        return out
 
     def put(self, mg: MGBatch, choice: dict):
-        self.data.setdefault(self._dev(mg), {})[batch_class(mg)] = {
-            "coarse_cells": int(choice["coarse_cells"]),
-            "blocks": {json.dumps(list(k)): int(v) for k, v in choice["blocks"].items()}}
+        rec = {"coarse_cells": int(choice["coarse_cells"]),
+               "blocks": {json.dumps(list(k)): int(v) for k, v in choice["blocks"].items()}}
+        if "tile_planes" in choice:
+            rec["tile_planes"] = int(choice["tile_planes"])
+        if "schedule" in choice:
+            rec["schedule"] = [int(choice["schedule"][0]), int(choice["schedule"][1]),
+                               float(choice["schedule"][2])]
+        self.data.setdefault(self._dev(mg), {})[batch_class(mg)] = rec
         os.makedirs(os.path.dirname(self.path) or ".", exist_ok=True)
         tmp = self.path + ".tmp"
         with open(tmp, "w") as f:
@@ -190,6 +264,10 @@ class TuneCache:
         if choice is not None:
             mg.params.coarse_cells = choice["coarse_cells"]
             mg.set_blocks(choice["blocks"])
+            if "tile_planes" in choice:
+                mg.set_tile_planes(choice["tile_planes"])
+            if "schedule" in choice:
+                mg.set_schedule(*choice["schedule"])
             return choice
         choice = tune(mg, reps)
         self.put(mg, choice)

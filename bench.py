@@ -892,6 +892,9 @@ def main():
                     help="finest levels that run fused pass pairs (default: the solver's)")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep 256-thread blocks (skip the launch-shape tuner)")
+    ap.add_argument("--tune-schedule", type=int, default=1,
+                    help="1: also rank sweep schedules by time to 1e-8 (reported beside "
+                         "the V(2,2) headline)")
     ap.add_argument("--profile-kernel",
                     choices=["update", "sweep", "all", "flow2d", "urban", "reference"],
                     default=None,
@@ -955,6 +958,9 @@ def main():
         tuning = {"tune_s": round(time.perf_counter() - t0, 2),
                   "coarse_cells": choice["coarse_cells"],
                   "coarse_cycle_ms": {str(k): round(v, 4) for k, v in mg.coarse_times.items()},
+                  "tile_planes": choice["tile_planes"],
+                  "vcycle_ms_by_tile_planes": {str(k): round(v, 4)
+                                               for k, v in mg.tile_times.items()},
                   "blocks": {"/".join(map(str, k)): v for k, v in sorted(choice["blocks"].items())
                              if v != 256}}
         log(f"bench: launch tuning {tuning}")
@@ -1019,6 +1025,28 @@ def main():
                                "achieved": round(sweep_bytes / (sweep_ms * 1e-3) / 1e9, 1)}}
     extra = {}
     outer_storage = mg.params.outer
+    if args.tune_schedule:
+        # the sweep schedule (nu1, nu2, alpha) ranked by the batch's time to 1e-8;
+        # the headline keeps the configuration's V(2,2)
+        from hfm.tune import tune_schedule
+        t0 = time.perf_counter()
+        best = tune_schedule(mg)
+        tsec = time.perf_counter() - t0
+        one_step(mg)                    # warm: the new plans' graphs
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        inf2 = one_step(mg)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter() - t0
+        extra["schedule_tuning"] = {
+            "best": list(best), "tune_s": round(tsec, 2),
+            "solve_ms_by_schedule": {f"V({a},{b}) alpha={c}": (round(v, 2) if math.isfinite(v)
+                                                               else None)
+                                     for (a, b, c), v in mg.schedule_times.items()},
+            "best_solves_per_s": round(len(specs) / t1, 2),
+            "best_iters_mean": round(float(inf2["iter"].mean()), 2),
+            "converged": bool((inf2["status"] == 1).all())}
+        log(f"bench: schedule tuning {extra['schedule_tuning']}")
     if not args.no_fp32_batch:
         del mg
         torch.cuda.empty_cache()

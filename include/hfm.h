@@ -323,11 +323,13 @@ int hfm_mg_smooth2(const hfm_mg_task* tasks, const uint16_t* blk_task, int64_t n
                    double* dots, void* stream);
 
 /* Tile geometry of the fused pass pairs for a level of extents (D, H, W):
- * rows and planes per tile and the tile count.  Returns HFM_EINVAL (with
- * hfm_last_error) when the level cannot be tiled (W % 8 != 0, or W too wide
- * for a 512-thread block to hold four rows). */
-int hfm_mg_tile(int64_t D, int64_t H, int64_t W, int32_t* tile_rows, int32_t* tile_planes,
-                int64_t* ntiles);
+ * rows and planes per tile and the tile count.  Tiles hold at most
+ * max_planes planes (rounded to balanced, even counts; 0 = 32), as many rows
+ * as a 512-thread block holds.  Returns HFM_EINVAL (with hfm_last_error) when
+ * the level cannot be tiled (W % 8 != 0, or W too wide for a 512-thread block
+ * to hold four rows). */
+int hfm_mg_tile(int64_t D, int64_t H, int64_t W, int32_t max_planes, int32_t* tile_rows,
+                int32_t* tile_planes, int64_t* ntiles);
 
 /* Coarsest solve e = Ainv b (Ainv FP64 dense, N = D*H*W <= 64): one block
  * per task.  Slots: b, xout = e, ec = Ainv. */

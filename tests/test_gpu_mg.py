@@ -305,3 +305,56 @@ def test_defect_correction_matches_oracle(dev, shape, storage, tol, alpha):
     assert rel(x, ref.x) < 1e-5
     r = b - O.face_apply(op.astype("f32"), x)
     assert np.linalg.norm(r) / np.linalg.norm(b) <= tol
+
+
+def test_every_tile_plane_choice_gives_the_same_cycle(dev):
+    """The fused pass pairs with 8, 16, 32 or 64 planes per tile: z and every
+    coarse right-hand side bitwise the same, the r.z partials' sum to rounding
+    (they are summed per tile), the solves the same iterations."""
+    import torch
+    from hfm import _lib as L
+    specs = [((64, 64, 64), 61, 1.0, 2), ((48, 96, 96), 62, 1.5, 1), ((33, 30, 32), 63, 0.7, 3)]
+    s = L.stream_handle(dev)
+    runs = {}
+    for z in (8, 16, 32, 64):
+        mg, _ = _batch(dev, specs, "bf16", fuse_min_tiles=0, tile_planes=z)
+        assert any(op[0] == "smooth2" for u in mg.plan.ops_up for op in u)
+        mg.restart()
+        mg.plan.precondition(s)
+        torch.cuda.synchronize()
+        zs = [_buffer(i, i._tpl[("dir", 0)]["b"]).clone() for i in mg.insts]
+        bs = [[i.B[lev].clone() for lev in range(1, len(i.levels))] for i in mg.insts]
+        dsum = [float(mg.pool.dots[i.dots_off:i.dots_off + i.nblk].sum()) for i in mg.insts]
+        info = mg.solve()
+        runs[z] = (zs, bs, dsum, list(info["iter"]), [mg.solution(q).clone() for q in range(3)])
+    ref = runs[32]
+    for z, (zs, bs, dsum, its, xs) in runs.items():
+        for a, b in zip(zs, ref[0]):
+            assert torch.equal(a, b), z
+        for la, lb in zip(bs, ref[1]):
+            for a, b in zip(la, lb):
+                assert torch.equal(a, b), z
+        for a, b in zip(dsum, ref[2]):
+            assert abs(a - b) <= 1e-12 * abs(b)
+        assert its == ref[3]
+        for a, b in zip(xs, ref[4]):
+            assert rel(a.cpu().numpy(), b.cpu().numpy()) < 1e-9
+
+
+def test_every_schedule_converges_to_the_true_residual(dev):
+    """tune_schedule's candidates (nu1, nu2, alpha) each reach 1e-8 on the FP64
+    true residual of every instance (the tuner only ranks them by time)."""
+    from hfm.tune import SCHEDULES, tune_schedule
+    specs = [((32, 32, 32), 71, 2.0, 2), ((48, 40, 64), 72, 1.0, 1), ((20, 12, 36), 73, 0.5, 3)]
+    mg, probs = _batch(dev, specs, "bf16", tol=1e-8, maxit=300)
+    for sch in SCHEDULES:
+        mg.set_schedule(*sch)
+        info = mg.solve()
+        assert bool((info["status"] == 1).all()), sch
+        for q, (op, b) in enumerate(probs):
+            x = mg.solution(q).cpu().numpy()
+            r = b - O.face_apply(op.astype("f32"), x)
+            assert np.linalg.norm(r) / np.linalg.norm(b) <= 1e-8, (sch, q)
+    best = tune_schedule(mg)
+    assert tuple(best) in {tuple(s) for s in SCHEDULES}
+    assert (mg.params.nu1, mg.params.nu2, mg.params.alpha) == tuple(best)

@@ -422,6 +422,10 @@ def test_fused_tile_geometry(shape):
     assert planes % 2 == 0 and planes <= 32 + 1 and planes >= min(D, 2)
     assert ntiles == ny * -(-D // planes)
     assert (ny - 1) * rows < H <= ny * rows
+    for zmax in (2, 8, 16, 64):
+        r2, p2, n2 = L.tile(shape, zmax)
+        assert r2 == rows and p2 % 2 == 0 and p2 <= zmax + 1 and p2 >= min(D, 2)
+        assert n2 == ny * -(-D // p2)
 
 
 def test_fused_entry_points_reject_bad_tables():
