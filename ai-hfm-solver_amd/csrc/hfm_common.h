@@ -81,19 +81,33 @@ constexpr int kBlock = 256;
 constexpr int kWaves = kBlock / 64;
 
 // Deterministic block sum: fixed shuffle tree per wave, waves summed in order.
-__device__ __forceinline__ double block_sum(double v) {
-  __shared__ double part[kWaves];
+// `half` selects the kBlock-thread half of a 2 kBlock-thread workgroup that
+// runs two blocks' work side by side (the persistent kernel); both halves
+// call it together (its barriers are the workgroup's).
+__device__ __forceinline__ double block_sum(double v, int half = 0) {
+  __shared__ double part[2][kWaves];
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  if (lane == 0) part[wid] = v;
+  const int lane = threadIdx.x & 63, wid = (threadIdx.x >> 6) & (kWaves - 1);
+  if (lane == 0) part[half][wid] = v;
   __syncthreads();
   double s = 0.0;
-  if (threadIdx.x == 0) {
-    for (int w = 0; w < kWaves; ++w) s += part[w];
+  if ((threadIdx.x & (kBlock - 1)) == 0) {
+    for (int w = 0; w < kWaves; ++w) s += part[half][w];
   }
   __syncthreads();
-  return s;  // valid in thread 0
+  return s;  // valid in the half's thread 0
+}
+
+// A block's coordinates: virtual block g of a launch of nblocks blocks and the
+// thread's index in it.  A kernel launch passes blockIdx / gridDim /
+// threadIdx; the persistent kernel passes the work item it runs (half = which
+// kBlock-thread half of its workgroup runs a kBlock-thread block).
+struct Blk {
+  int g, nblocks, tid, half;
+};
+__device__ __forceinline__ Blk launch_blk() {
+  return Blk{int(blockIdx.x), int(gridDim.x), int(threadIdx.x), 0};
 }
 
 // ---------------------------------------------------------------- work table

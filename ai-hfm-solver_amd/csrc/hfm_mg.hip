@@ -432,14 +432,19 @@ __device__ __forceinline__ float restrict_one(const hfm_mg_task& t, int nc,
 // fine level's last post-sweep may write the preconditioned residual z in a
 // wider type than the cycle's internals).
 // B: threads per block (a block covers 8 B cells); DOT launches use kBlock.
-template <typename T, int MODE, bool DOT, typename O, int B>
-__global__ __launch_bounds__(B) void k_sweep(const hfm_mg_task* __restrict__ tasks,
-                                                  const uint16_t* __restrict__ blk_task,
-                                                  const hfm_mg_scalars* __restrict__ sc,
-                                                  float omega, float alpha, double* dots) {
-  const int g = blockIdx.x;
+// PB: the body runs as half of a persistent workgroup, whose other half
+// shares its barriers: an early exit still takes part in them.
+template <typename T, int MODE, bool DOT, typename O, int B, bool PB = false>
+__device__ __forceinline__ void sweep_body(const hfm_mg_task* __restrict__ tasks,
+                                           const uint16_t* __restrict__ blk_task,
+                                           const hfm_mg_scalars* __restrict__ sc, float omega,
+                                           float alpha, double* dots, const Blk& bk) {
+  const int g = bk.g;
   const hfm_mg_task& t = tasks[blk_task[g]];
-  if (!sc[t.inst].active) return;
+  if (!sc[t.inst].active) {
+    if (DOT && PB) block_sum(0.0, bk.half);  // the workgroup's barriers
+    return;
+  }
   const int chunk = xcd_chunk(g - t.blk0, t.nblk);
   const int N = t.D * t.H * t.W;
   const int base = chunk * (B * kRun);
@@ -449,7 +454,7 @@ __global__ __launch_bounds__(B) void k_sweep(const hfm_mg_task* __restrict__ tas
   HFM_GLOBAL O* out = gout<O>(t.xout);
   double acc = 0.0;
   if (vec_task(t)) {
-    const int n0 = base + threadIdx.x * kRun;
+    const int n0 = base + bk.tid * kRun;
     const bool valid = n0 < N;
     const Run r = run_at(t, n0, valid);
     RunCoef<T> q;
@@ -479,7 +484,7 @@ __global__ __launch_bounds__(B) void k_sweep(const hfm_mg_task* __restrict__ tas
     }
   } else if (base < N) {
     for (int s = 0; s < kRun; ++s) {
-      const int n = base + s * B + threadIdx.x;
+      const int n = base + s * B + bk.tid;
       if (n >= N) break;
       int i, j, k;
       decompose(t, n, i, j, k);
@@ -491,9 +496,17 @@ __global__ __launch_bounds__(B) void k_sweep(const hfm_mg_task* __restrict__ tas
     }
   }
   if (DOT) {
-    const double s = block_sum(acc);
-    if (threadIdx.x == 0) dots[t.slot0 + chunk] = s;
+    const double s = block_sum(acc, bk.half);
+    if (bk.tid == 0) dots[t.slot0 + chunk] = s;
   }
+}
+
+template <typename T, int MODE, bool DOT, typename O, int B>
+__global__ __launch_bounds__(B) void k_sweep(const hfm_mg_task* __restrict__ tasks,
+                                             const uint16_t* __restrict__ blk_task,
+                                             const hfm_mg_scalars* __restrict__ sc, float omega,
+                                             float alpha, double* dots) {
+  sweep_body<T, MODE, DOT, O, B>(tasks, blk_task, sc, omega, alpha, dots, launch_blk());
 }
 
 // ------------------------------------------------------------------ residual + restriction
@@ -501,10 +514,11 @@ __global__ __launch_bounds__(B) void k_sweep(const hfm_mg_task* __restrict__ tas
 // Vector path: a thread owns 4 consecutive coarse cells = a run of 8 fine
 // cells in each of its 2 fine rows x 2 fine planes.
 template <typename T, int B>
-__global__ __launch_bounds__(B) void k_resid_restrict(const hfm_mg_task* __restrict__ tasks,
-                                                           const uint16_t* __restrict__ blk_task,
-                                                           const hfm_mg_scalars* __restrict__ sc) {
-  const int g = blockIdx.x;
+__device__ __forceinline__ void resid_restrict_body(const hfm_mg_task* __restrict__ tasks,
+                                                    const uint16_t* __restrict__ blk_task,
+                                                    const hfm_mg_scalars* __restrict__ sc,
+                                                    const Blk& bk) {
+  const int g = bk.g;
   const hfm_mg_task& t = tasks[blk_task[g]];
   if (!sc[t.inst].active) return;
   const int chunk = xcd_chunk(g - t.blk0, t.nblk);
@@ -515,7 +529,7 @@ __global__ __launch_bounds__(B) void k_resid_restrict(const hfm_mg_task* __restr
   HFM_GLOBAL T* bc = gout<T>(t.xout);
   const int W = t.W, H = t.H, D = t.D;
   if (vec_task(t)) {
-    const int nc0 = base + threadIdx.x * 4;
+    const int nc0 = base + bk.tid * 4;
     const bool cvalid = nc0 < Nc;
     int I0 = 0, J = 0, K = 0;
     if (cvalid) {
@@ -563,10 +577,17 @@ __global__ __launch_bounds__(B) void k_resid_restrict(const hfm_mg_task* __restr
   }
   if (base >= Nc) return;
   for (int s = 0; s < 4; ++s) {
-    const int nc = base + s * B + threadIdx.x;
+    const int nc = base + s * B + bk.tid;
     if (nc >= Nc) break;
     put(bc + nc, from_f<T>(restrict_one<T>(t, nc, x, b)));
   }
+}
+
+template <typename T, int B>
+__global__ __launch_bounds__(B) void k_resid_restrict(const hfm_mg_task* __restrict__ tasks,
+                                                      const uint16_t* __restrict__ blk_task,
+                                                      const hfm_mg_scalars* __restrict__ sc) {
+  resid_restrict_body<T, B>(tasks, blk_task, sc, launch_blk());
 }
 
 // ------------------------------------------------------------------ fused pass pairs
@@ -779,25 +800,35 @@ __device__ __forceinline__ double pair_block_sum(double v) {
   return s;
 }
 
+// The pair's LDS: lx / ly end in a run of zeros (at kPairPlane), which a
+// thread reads in place of a neighbour past the grid's edge: the masked
+// form's +0, without a select per cell.
+template <typename T, bool S2>
+struct PairLds {
+  float lx[2][kPairPlane + kPairRun];  // pass-1 input of the plane
+  T lc[2][kPairPlane];                 // south faces of the plane
+  T ly[2][kPairPlane + kPairRun];      // pass-1 result of the plane
+  float lr[2][S2 ? 1 : kPairPlane / 2];  // odd rows' residuals (restriction)
+};
+
 template <typename T, bool S2, typename O, bool DOT>
-__global__ __launch_bounds__(kPairBlock, sizeof(T) == 2 ? 4 : 2) void k_smooth_pair(
-    const hfm_mg_task* __restrict__ tasks, const uint16_t* __restrict__ blk_task,
-    const hfm_mg_scalars* __restrict__ sc, float omega, float alpha, double* dots) {
+__device__ __forceinline__ void smooth_pair_body(const hfm_mg_task* __restrict__ tasks,
+                                                 const uint16_t* __restrict__ blk_task,
+                                                 const hfm_mg_scalars* __restrict__ sc,
+                                                 float omega, float alpha, double* dots,
+                                                 PairLds<T, S2>& lds, const Blk& bk) {
   constexpr int N = kPairRun;
-  // lx / ly end in a run of zeros (at kZero), which a thread reads in place
-  // of a neighbour past the grid's edge: the masked form's +0, without a
-  // select per cell
   constexpr int kZero = kPairPlane;
-  __shared__ float lx[2][kPairPlane + N];  // pass-1 input of the plane
-  __shared__ T lc[2][kPairPlane];          // south faces of the plane
-  __shared__ T ly[2][kPairPlane + N];      // pass-1 result of the plane
-  __shared__ float lr[2][S2 ? 1 : kPairPlane / 2];  // odd rows' residuals (restriction)
+  auto& lx = lds.lx;
+  auto& lc = lds.lc;
+  auto& ly = lds.ly;
+  auto& lr = lds.lr;
   // Tiles are numbered contiguously over the launch's tasks (a task's tiles
   // are blk0 .. blk0 + nblk - 1); each XCD takes a contiguous range of them,
   // so a task's neighbouring tiles (which re-read each other's halo) share an
   // L2 and tasks with one tile still spread over the XCDs.  The table is
   // padded to a multiple of 8 entries with 0xFFFF.
-  const int tile = xcd_chunk(blockIdx.x, gridDim.x);
+  const int tile = xcd_chunk(bk.g, bk.nblocks);
   const int task = blk_task[tile];
   if (task == 0xFFFF) return;
   const hfm_mg_task& t = tasks[task];
@@ -806,7 +837,7 @@ __global__ __launch_bounds__(kPairBlock, sizeof(T) == 2 ? 4 : 2) void k_smooth_p
   const int R = t.tile_rows, Z = t.tile_planes;
   const int ny = (H + R - 1) / R;
   const int lb = tile - t.blk0;
-  if (DOT && threadIdx.x == 0) {
+  if (DOT && bk.tid == 0) {
     // the finalize sums every dot slot of the instance: the ones past the
     // tiles hold zero
     const int nslot = sc[t.inst].nslot;
@@ -817,8 +848,8 @@ __global__ __launch_bounds__(kPairBlock, sizeof(T) == 2 ? 4 : 2) void k_smooth_p
   const int jh = R < H ? 1 : 0;  // halo rows
   const int Rb = min(R, H - yb * R);
   const int NR = Rb + 2 * jh;
-  const int r = int(threadIdx.x) / Q;
-  const int i0 = N * (int(threadIdx.x) - r * Q);
+  const int r = int(bk.tid) / Q;
+  const int i0 = N * (int(bk.tid) - r * Q);
   const int j = yb * R - jh + r;
   // live: the thread's row is a row of the plane (tile or halo); rows past
   // the plane's edges and threads past the tile do no work
@@ -834,11 +865,11 @@ __global__ __launch_bounds__(kPairBlock, sizeof(T) == 2 ? 4 : 2) void k_smooth_p
   // before / after the run), or of the zeros where the grid ends there
   const int os = south ? li - W : kZero, on = north ? li + W : kZero;
   const int ow = west ? li - 1 : kZero, oe = east ? li + N : kZero;
-  if (threadIdx.x < N) {  // read after the walk's first barrier
-    lx[0][kZero + threadIdx.x] = 0.0f;
-    lx[1][kZero + threadIdx.x] = 0.0f;
-    ly[0][kZero + threadIdx.x] = from_f<T>(0.0f);
-    ly[1][kZero + threadIdx.x] = from_f<T>(0.0f);
+  if (bk.tid < N) {  // read after the walk's first barrier
+    lx[0][kZero + bk.tid] = 0.0f;
+    lx[1][kZero + bk.tid] = 0.0f;
+    ly[0][kZero + bk.tid] = from_f<T>(0.0f);
+    ly[1][kZero + bk.tid] = from_f<T>(0.0f);
   }
   const int k0 = zb * Z, k1 = min(k0 + Z, D);  // output planes [k0, k1)
   const int ks = max(k0 - 1, 0);
@@ -1046,8 +1077,16 @@ __global__ __launch_bounds__(kPairBlock, sizeof(T) == 2 ? 4 : 2) void k_smooth_p
   }
   if (DOT) {
     const double s = pair_block_sum(acc);
-    if (threadIdx.x == 0) dots[t.slot0 + lb] = s;
+    if (bk.tid == 0) dots[t.slot0 + lb] = s;
   }
+}
+
+template <typename T, bool S2, typename O, bool DOT>
+__global__ __launch_bounds__(kPairBlock, sizeof(T) == 2 ? 4 : 2) void k_smooth_pair(
+    const hfm_mg_task* __restrict__ tasks, const uint16_t* __restrict__ blk_task,
+    const hfm_mg_scalars* __restrict__ sc, float omega, float alpha, double* dots) {
+  __shared__ PairLds<T, S2> lds;
+  smooth_pair_body<T, S2, O, DOT>(tasks, blk_task, sc, omega, alpha, dots, lds, launch_blk());
 }
 
 // ------------------------------------------------------------------ coarsest solve
@@ -1084,13 +1123,17 @@ __global__ __launch_bounds__(64) void k_coarse_solve(const hfm_mg_task* __restri
 // (X, Y)[(nu1 + nu2 - 1) % 2], the coarsest's in X).
 constexpr int kCycleThreads = 1024;
 
-template <typename T>
-__global__ __launch_bounds__(kCycleThreads) void k_coarse_cycle(
-    const hfm_mg_task* __restrict__ recs, const int* __restrict__ chain,
-    const hfm_mg_scalars* __restrict__ sc, int nu1, int nu2, float omega, float alpha) {
-  const int first = chain[2 * blockIdx.x], count = chain[2 * blockIdx.x + 1];
+// NT threads (a whole workgroup) run instance bk.g's tail; the per-cell loops
+// stride by NT, so the result does not depend on it.
+template <typename T, int NT>
+__device__ __forceinline__ void coarse_cycle_body(const hfm_mg_task* __restrict__ recs,
+                                                  const int* __restrict__ chain,
+                                                  const hfm_mg_scalars* __restrict__ sc, int nu1,
+                                                  int nu2, float omega, float alpha,
+                                                  const Blk& bk) {
+  const int first = chain[2 * bk.g], count = chain[2 * bk.g + 1];
   if (count < 1 || !sc[recs[first].inst].active) return;
-  const int tid = threadIdx.x;
+  const int tid = bk.tid;
   // xin and ec are read-only for the per-level kernels; here X and the next
   // level's B are written as well
   auto X = [&](int q) { return gout<T>(const_cast<void*>(recs[first + q].xin)); };
@@ -1105,7 +1148,7 @@ __global__ __launch_bounds__(kCycleThreads) void k_coarse_cycle(
     for (int kk = 0; kk < nu1; ++kk) {
       const HFM_GLOBAL T* xin = buf(q, kk - 1);
       HFM_GLOBAL T* out = buf(q, kk);
-      for (int n = tid; n < N; n += kCycleThreads) {
+      for (int n = tid; n < N; n += NT) {
         int i, j, k;
         decompose(t, n, i, j, k);
         const float bn = ldg(b + n);
@@ -1120,7 +1163,7 @@ __global__ __launch_bounds__(kCycleThreads) void k_coarse_cycle(
     const HFM_GLOBAL T* x = buf(q, nu1 - 1);
     HFM_GLOBAL T* bc = gout<T>(const_cast<void*>(t.ec));
     const int Nc = t.Dc * t.Hc * t.Wc;
-    for (int nc = tid; nc < Nc; nc += kCycleThreads) put(bc + nc, from_f<T>(restrict_one<T>(t, nc, x, b)));
+    for (int nc = tid; nc < Nc; nc += NT) put(bc + nc, from_f<T>(restrict_one<T>(t, nc, x, b)));
     __syncthreads();
   }
   // coarsest: x = Ainv b
@@ -1151,7 +1194,7 @@ __global__ __launch_bounds__(kCycleThreads) void k_coarse_cycle(
       const int kk = nu1 + s2;
       const HFM_GLOBAL T* xin = buf(q, kk - 1);
       HFM_GLOBAL T* out = buf(q, kk);
-      for (int n = tid; n < N; n += kCycleThreads) {
+      for (int n = tid; n < N; n += NT) {
         int i, j, k;
         decompose(t, n, i, j, k);
         const float bn = ldg(b + n);
@@ -1164,6 +1207,13 @@ __global__ __launch_bounds__(kCycleThreads) void k_coarse_cycle(
       __syncthreads();
     }
   }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kCycleThreads) void k_coarse_cycle(
+    const hfm_mg_task* __restrict__ recs, const int* __restrict__ chain,
+    const hfm_mg_scalars* __restrict__ sc, int nu1, int nu2, float omega, float alpha) {
+  coarse_cycle_body<T, kCycleThreads>(recs, chain, sc, nu1, nu2, omega, alpha, launch_blk());
 }
 
 // ------------------------------------------------------------------ FCG passes (fine level)
@@ -1213,15 +1263,18 @@ __global__ __launch_bounds__(kBlock) void k_fcg_init(const hfm_mg_task* __restri
 //   sum_cells cx (p - p_w)^2 + cy (p - p_s)^2 + cz (p - p_d)^2
 //             + [high boundary] (bxh | byh | bzh) p^2      (zero ghosts),
 // which needs only the west, south and down neighbours.
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_fcg_dir(const hfm_mg_task* __restrict__ tasks,
-                                                    const uint16_t* __restrict__ blk_task,
-                                                    const hfm_mg_scalars* __restrict__ sc,
-                                                    double* dots) {
-  const int g = blockIdx.x;
+template <typename T, bool PB = false>
+__device__ __forceinline__ void fcg_dir_body(const hfm_mg_task* __restrict__ tasks,
+                                             const uint16_t* __restrict__ blk_task,
+                                             const hfm_mg_scalars* __restrict__ sc, double* dots,
+                                             const Blk& bk) {
+  const int g = bk.g;
   const hfm_mg_task& t = tasks[blk_task[g]];
   const hfm_mg_scalars& S = sc[t.inst];
-  if (!S.active) return;
+  if (!S.active) {
+    if (PB) block_sum(0.0, bk.half);  // the workgroup's barriers
+    return;
+  }
   const int chunk = xcd_chunk(g - t.blk0, t.nblk);
   const int N = t.D * t.H * t.W;
   const int base = chunk * kCellChunk;
@@ -1229,7 +1282,7 @@ __global__ __launch_bounds__(kBlock) void k_fcg_dir(const hfm_mg_task* __restric
   HFM_GLOBAL T* pnew = gout<T>(t.xout);
   double acc = 0.0;
   if (vec_task(t)) {
-    const int n0 = base + threadIdx.x * kRun;
+    const int n0 = base + bk.tid * kRun;
     const bool valid = n0 < N;
     const Run r = run_at(t, n0, valid);
     const int W = t.W, HW = t.H * t.W;
@@ -1277,7 +1330,7 @@ __global__ __launch_bounds__(kBlock) void k_fcg_dir(const hfm_mg_task* __restric
   } else if (base < N) {
     auto pat = [&](int m, int, int, int) -> float { return src.one(m, 0, 0, 0); };
     for (int s = 0; s < kRun; ++s) {
-      const int n = base + s * kBlock + threadIdx.x;
+      const int n = base + s * kBlock + bk.tid;
       if (n >= N) break;
       int i, j, k;
       decompose(t, n, i, j, k);
@@ -1288,8 +1341,16 @@ __global__ __launch_bounds__(kBlock) void k_fcg_dir(const hfm_mg_task* __restric
       acc += double(p * (diag * p - off));
     }
   }
-  const double s = block_sum(acc);
-  if (threadIdx.x == 0) dots[t.slot0 + chunk] = s;
+  const double s = block_sum(acc, bk.half);
+  if (bk.tid == 0) dots[t.slot0 + chunk] = s;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_fcg_dir(const hfm_mg_task* __restrict__ tasks,
+                                                    const uint16_t* __restrict__ blk_task,
+                                                    const hfm_mg_scalars* __restrict__ sc,
+                                                    double* dots) {
+  fcg_dir_body<T>(tasks, blk_task, sc, dots, launch_blk());
 }
 
 // r64 -= alpha A p; rlow = T(r64); dots = r . r; x64 every other iteration.
@@ -1304,6 +1365,100 @@ __global__ __launch_bounds__(kBlock) void k_fcg_dir(const hfm_mg_task* __restric
 // SWEEP: also the next preconditioner's first (zero initial guess) fine-level
 // sweep, xout = omega rlow / d with d the diagonal of the storage-rounded faces
 // (identical to a separate HFM_SWEEP_ZERO launch).
+template <typename T, bool SWEEP, typename P, bool PB = false>
+__device__ __forceinline__ void fcg_update_body(const hfm_mg_task* __restrict__ tasks,
+                                                const uint16_t* __restrict__ blk_task,
+                                                const hfm_mg_scalars* __restrict__ sc,
+                                                float omega, double* dots, const Blk& bk) {
+  const int g = bk.g;
+  const hfm_mg_task& t = tasks[blk_task[g]];
+  const hfm_mg_scalars& S = sc[t.inst];
+  if (!S.active) {
+    if (PB) block_sum(0.0, bk.half);  // the workgroup's barriers
+    return;
+  }
+  const int chunk = xcd_chunk(g - t.blk0, t.nblk);
+  const int N = t.D * t.H * t.W;
+  const int base = chunk * kCellChunk;
+  const double alpha = S.alpha;
+  const bool lag = S.xlag != 0;
+  const double alpha_prev = S.alpha_prev;
+  const SrcPlain<P> src{gin<P>(t.xin)};
+  const SrcPlain<P> prev{gin<P>(t.ec)};
+  HFM_GLOBAL T* rlow = gout<T>(t.rlow);
+  HFM_GLOBAL T* x0 = gout<T>(t.xout);
+  HFM_GLOBAL double* x64 = gout<double>(t.x64);
+  HFM_GLOBAL double* r64 = gout<double>(t.r64);
+  double acc = 0.0;
+  if (vec_task(t)) {
+    const int n0 = base + bk.tid * kRun;
+    const bool valid = n0 < N;
+    const Run r = run_at(t, n0, valid);
+    RunCoef<float> q;
+    q.load(t, r);
+    double rv[8], xv[8];
+    float pp[8];
+    ld8d_ilv(r64, r.n0, rv);
+    if (lag) {
+      ld8d_ilv(x64, r.n0, xv);
+      prev.run(r.n0, 0, 0, 0, pp);
+    }
+    double d[8], p[8], off[8];
+    q.diag(d);
+    run_offdiag<double>(t, r, q, src, p, off);
+    if (valid) {
+      float rl[8];
+#pragma unroll
+      for (int m = 0; m < 8; ++m) {
+        rv[m] = rv[m] - alpha * (d[m] * p[m] - off[m]);
+        rl[m] = float(rv[m]);
+        acc += rv[m] * rv[m];
+      }
+      st8d_ilv(r64, n0, rv);
+      st8(rlow + n0, rl);
+      if (lag) {
+#pragma unroll
+        for (int m = 0; m < 8; ++m) xv[m] = fma(alpha, p[m], fma(alpha_prev, double(pp[m]), xv[m]));
+        st8d_ilv(x64, n0, xv);
+      }
+      if (SWEEP) {
+        float o[8];
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {
+          const float dT = round_to<T>(q.cw[m]) + round_to<T>(q.ce[m]) + round_to<T>(q.cs[m]) +
+                           round_to<T>(q.cn[m]) + round_to<T>(q.cd[m]) + round_to<T>(q.cu[m]);
+          o[m] = wdiv(omega, round_to<T>(rl[m]), dT);
+        }
+        st8_nt(x0 + n0, o);
+      }
+    }
+  } else if (base < N) {
+    auto pat = [&](int m, int, int, int) -> double { return double(src.one(m, 0, 0, 0)); };
+    for (int s = 0; s < kRun; ++s) {
+      const int n = base + s * kBlock + bk.tid;
+      if (n >= N) break;
+      int i, j, k;
+      decompose(t, n, i, j, k);
+      double diag;
+      const double off = face_row<double, float>(t, n, i, j, k, pat, diag);
+      const double pn = pat(n, k, j, i);
+      const double q = diag * pn - off;
+      if (lag) x64[n] = fma(alpha, pn, fma(alpha_prev, double(prev.one(n, 0, 0, 0)), x64[n]));
+      const double rr = r64[n] - alpha * q;
+      r64[n] = rr;
+      const T rt = from_f<T>(float(rr));
+      put(rlow + n, rt);
+      acc += rr * rr;
+      if (SWEEP) put(x0 + n, from_f<T>(wdiv(omega, to_f(rt), diag_rounded<T>(t, n, i, j, k))));
+    }
+  }
+  const double s = block_sum(acc, bk.half);
+  if (bk.tid == 0) dots[t.slot0 + chunk] = s;
+}
+
+// The launch keeps the body written out in the kernel itself (a copy of
+// fcg_update_body with PB = false): compiled through the shared inline body
+// it takes 194 VGPRs instead of 152, one wave per SIMD less.
 template <typename T, bool SWEEP, typename P>
 __global__ __launch_bounds__(kBlock) void k_fcg_update(const hfm_mg_task* __restrict__ tasks,
                                                        const uint16_t* __restrict__ blk_task,
@@ -1398,10 +1553,11 @@ __global__ __launch_bounds__(kBlock) void k_fcg_update(const hfm_mg_task* __rest
 // instance stopped between its direction and its update).  Same tasks as
 // k_fcg_update of the launch that preceded the stopping CHECK.
 template <typename P>
-__global__ __launch_bounds__(kBlock) void k_fcg_flush(const hfm_mg_task* __restrict__ tasks,
-                                                      const uint16_t* __restrict__ blk_task,
-                                                      const hfm_mg_scalars* __restrict__ sc) {
-  const int g = blockIdx.x;
+__device__ __forceinline__ void fcg_flush_body(const hfm_mg_task* __restrict__ tasks,
+                                               const uint16_t* __restrict__ blk_task,
+                                               const hfm_mg_scalars* __restrict__ sc,
+                                               const Blk& bk) {
+  const int g = bk.g;
   const hfm_mg_task& t = tasks[blk_task[g]];
   const hfm_mg_scalars& S = sc[t.inst];
   if (S.active || S.xlag == 0) return;
@@ -1412,7 +1568,7 @@ __global__ __launch_bounds__(kBlock) void k_fcg_flush(const hfm_mg_task* __restr
   const SrcPlain<P> prev{gin<P>(S.xlag == 1 ? t.xin : t.ec)};
   HFM_GLOBAL double* x64 = gout<double>(t.x64);
   if (vec_task(t)) {
-    const int n0 = base + threadIdx.x * kRun;
+    const int n0 = base + bk.tid * kRun;
     if (n0 < N) {
       double xv[8];
       float pp[8];
@@ -1424,30 +1580,42 @@ __global__ __launch_bounds__(kBlock) void k_fcg_flush(const hfm_mg_task* __restr
     }
   } else {
     for (int s = 0; s < kRun; ++s) {
-      const int n = base + s * kBlock + threadIdx.x;
+      const int n = base + s * kBlock + bk.tid;
       if (n >= N) break;
       x64[n] = fma(ap, double(prev.one(n, 0, 0, 0)), x64[n]);
     }
   }
 }
 
+template <typename P>
+__global__ __launch_bounds__(kBlock) void k_fcg_flush(const hfm_mg_task* __restrict__ tasks,
+                                                      const uint16_t* __restrict__ blk_task,
+                                                      const hfm_mg_scalars* __restrict__ sc) {
+  fcg_flush_body<P>(tasks, blk_task, sc, launch_blk());
+}
+
 // ------------------------------------------------------------------ per-instance scalars
-__global__ __launch_bounds__(kBlock) void k_fcg_finalize(hfm_mg_scalars* sc, const double* dots,
-                                                         int kind) {
-  hfm_mg_scalars& S = sc[blockIdx.x];
+template <bool PB = false>
+__device__ __forceinline__ void fcg_finalize_body(hfm_mg_scalars* sc, const double* dots,
+                                                  int kind, const Blk& bk) {
+  hfm_mg_scalars& S = sc[bk.g];
   if (!S.active) {
     // a stopped instance's pending solution update was applied by the flush
     // after its last CHECK; BETA opens the next iteration
-    if (kind == HFM_FIN_BETA && threadIdx.x == 0) S.xlag = 0;
+    if (kind == HFM_FIN_BETA && bk.tid == 0) S.xlag = 0;
+    if (PB) block_sum(0.0, bk.half);  // the workgroup's barriers
     return;
   }
   // INIT applies to slots staged by the host (iter == -1), the other steps to
   // initialised slots; a pool can then admit instances while others iterate.
-  if ((kind == HFM_FIN_INIT) != (S.iter == -1)) return;
+  if ((kind == HFM_FIN_INIT) != (S.iter == -1)) {
+    if (PB) block_sum(0.0, bk.half);
+    return;
+  }
   double acc = 0.0;
-  for (int q = threadIdx.x; q < S.nslot; q += kBlock) acc += dots[S.slot0 + q];
-  const double s = block_sum(acc);
-  if (threadIdx.x != 0) return;
+  for (int q = bk.tid; q < S.nslot; q += kBlock) acc += dots[S.slot0 + q];
+  const double s = block_sum(acc, bk.half);
+  if (bk.tid != 0) return;
   switch (kind) {
     case HFM_FIN_INIT:
       S.bnorm2 = s;
@@ -1506,6 +1674,11 @@ __global__ __launch_bounds__(kBlock) void k_fcg_finalize(hfm_mg_scalars* sc, con
       }
       break;
   }
+}
+
+__global__ __launch_bounds__(kBlock) void k_fcg_finalize(hfm_mg_scalars* sc, const double* dots,
+                                                         int kind) {
+  fcg_finalize_body(sc, dots, kind, launch_blk());
 }
 
 // ------------------------------------------------------------------ setup kernels
@@ -1755,6 +1928,223 @@ __global__ __launch_bounds__(kBlock) void k_coarse_inverse(const hfm_mg_task* __
   }
   double* out = static_cast<double*>(t.xout);
   for (int e = threadIdx.x; e < N * N; e += kBlock) out[e] = M[e / N][N + e % N];
+}
+
+
+// ------------------------------------------------------------------ persistent iteration
+// One launch runs a program of phases (hfm_phase): each phase is one of the
+// launches above -- its task table, block map and arguments -- and its work
+// items are that launch's blocks (for the finalize and the coarse cycle: its
+// instances).  A workgroup of 512 threads runs an item of a 512-thread body
+// (the fused pass pairs, the coarse cycle) or two items of a 256-thread body
+// side by side, each half with its own block index; the bodies are the
+// launches' own, so every value and every partial sum is the launch's.
+//
+// Order: for each instance the phases run in program order.  Item i of
+// phase p for instance s waits until every item of phase prev[p][s] (the
+// instance's previous phase) for s has finished: a counter per (phase,
+// instance), raised by the item's workgroup after the item (release at agent
+// scope), read by the waiting one (acquire); there is no grid barrier, so one
+// instance's passes overlap another's.  Every workgroup takes its items in
+// program order and the grid is co-resident (a cooperative launch), so the
+// earliest unfinished item can always proceed.  Every wait is bounded by
+// wall-clock time: on a timeout the failure word is set, every workgroup
+// stops at its next item, and the host reports the program as failed.
+//
+// Items on XCD x: workgroup w runs on XCD w % 8 (round-robin dispatch), and
+// the j-th of an XCD's workgroups takes a contiguous range of the phase's
+// items g with g % 8 == x -- the items a launch would have sent to that XCD,
+// so neighbouring chunks of a level stay in one L2.
+constexpr int kPersistThreads = 512;
+constexpr int kPairLdsBytes = sizeof(PairLds<float, false>) > sizeof(PairLds<float, true>)
+                                  ? int(sizeof(PairLds<float, false>))
+                                  : int(sizeof(PairLds<float, true>));
+
+// the item's instance slot, or -1 for a padding item
+__device__ __forceinline__ int item_slot(const hfm_phase& ph, int g) {
+  if (ph.kind == HFM_PH_FINALIZE) return ph.slot0 + g;
+  if (ph.kind == HFM_PH_COARSE_CYCLE) {
+    const int first = ph.chain[2 * g], count = ph.chain[2 * g + 1];
+    return count > 0 ? ph.tasks[first].inst : -1;
+  }
+  // a fused pair's block g runs tile xcd_chunk(g) (smooth_pair_body)
+  const bool tiles = ph.kind == HFM_PH_SMOOTH_RESTRICT || ph.kind == HFM_PH_SMOOTH2;
+  const int q = ph.blk_task[tiles ? xcd_chunk(g, ph.nitems) : g];
+  return q == 0xFFFF ? -1 : ph.tasks[q].inst;
+}
+
+// thread 0 of a (half-)block: wait for the item's previous phase; false on failure
+__device__ __forceinline__ bool persist_wait(const hfm_persist_ctl& ctl, int p, int slot,
+                                             uint64_t ticks) {
+  if (slot < 0) return true;
+  const int q = ctl.prev[p * ctl.nslot + slot];
+  if (q < 0) return true;
+  const uint32_t target = uint32_t(ctl.need[q * ctl.nslot + slot]);
+  const uint32_t* c = ctl.cnt + q * ctl.nslot + slot;
+  // poll with relaxed loads; one acquire fence (a cache invalidation) once met
+  if (__hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+    const uint64_t t0 = wall_clock64();
+    for (;;) {
+      __builtin_amdgcn_s_sleep(2);
+      if (__hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target) break;
+      if (__hip_atomic_load(ctl.fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u)
+        return false;
+      if (wall_clock64() - t0 > ticks) {
+        atomicExch(ctl.fail, 1u);
+        return false;
+      }
+    }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  return true;
+}
+
+__device__ __forceinline__ void persist_signal(const hfm_persist_ctl& ctl, int p, int slot,
+                                               int count) {
+  if (slot < 0 || count <= 0) return;
+  __threadfence();
+  __hip_atomic_fetch_add(ctl.cnt + p * ctl.nslot + slot, uint32_t(count), __ATOMIC_RELEASE,
+                         __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// the 256-thread bodies; `g < 0`: this half has no item (it takes part in
+// the body's barriers only)
+template <typename T, typename P>
+__device__ __forceinline__ void persist_half(const hfm_phase& ph, int g, int half,
+                                             hfm_mg_scalars* sc) {
+  const Blk bk{g, ph.nitems, int(threadIdx.x) & (kBlock - 1), half};
+  const bool none = g < 0;
+  switch (ph.kind) {
+    case HFM_PH_SWEEP:
+      if (ph.dot) {
+        if (none) { block_sum(0.0, half); break; }
+        switch (ph.mode) {
+          case HFM_SWEEP_ZERO:
+            sweep_body<T, HFM_SWEEP_ZERO, true, P, kBlock, true>(ph.tasks, ph.blk_task, sc, ph.omega, ph.alpha, ph.dots, bk); break;
+          case HFM_SWEEP_PLAIN:
+            sweep_body<T, HFM_SWEEP_PLAIN, true, P, kBlock, true>(ph.tasks, ph.blk_task, sc, ph.omega, ph.alpha, ph.dots, bk); break;
+          default:
+            sweep_body<T, HFM_SWEEP_CORRECT, true, P, kBlock, true>(ph.tasks, ph.blk_task, sc, ph.omega, ph.alpha, ph.dots, bk); break;
+        }
+      } else if (!none) {
+        switch (ph.mode) {
+          case HFM_SWEEP_ZERO:
+            sweep_body<T, HFM_SWEEP_ZERO, false, T, kBlock, true>(ph.tasks, ph.blk_task, sc, ph.omega, ph.alpha, ph.dots, bk); break;
+          case HFM_SWEEP_PLAIN:
+            sweep_body<T, HFM_SWEEP_PLAIN, false, T, kBlock, true>(ph.tasks, ph.blk_task, sc, ph.omega, ph.alpha, ph.dots, bk); break;
+          default:
+            sweep_body<T, HFM_SWEEP_CORRECT, false, T, kBlock, true>(ph.tasks, ph.blk_task, sc, ph.omega, ph.alpha, ph.dots, bk); break;
+        }
+      }
+      break;
+    case HFM_PH_RESTRICT:
+      if (!none) resid_restrict_body<T, kBlock>(ph.tasks, ph.blk_task, sc, bk);
+      break;
+    case HFM_PH_DIR:
+      if (none) block_sum(0.0, half);
+      else fcg_dir_body<P, true>(ph.tasks, ph.blk_task, sc, ph.dots, bk);
+      break;
+    case HFM_PH_UPDATE:
+      if (none) block_sum(0.0, half);
+      else fcg_update_body<T, true, P, true>(ph.tasks, ph.blk_task, sc, ph.omega, ph.dots, bk);
+      break;
+    case HFM_PH_FLUSH:
+      if (!none) fcg_flush_body<P>(ph.tasks, ph.blk_task, sc, bk);
+      break;
+    case HFM_PH_FINALIZE:
+      if (none) block_sum(0.0, half);
+      else fcg_finalize_body<true>(sc + ph.slot0, ph.dots, ph.mode, bk);
+      break;
+    default:
+      break;
+  }
+}
+
+template <typename T, typename P>
+__global__ __launch_bounds__(kPersistThreads) void k_persistent(const hfm_phase* __restrict__ prog,
+                                                                int nphase, hfm_persist_ctl ctl,
+                                                                hfm_mg_scalars* sc,
+                                                                uint64_t ticks) {
+  __shared__ __align__(16) unsigned char lds[kPairLdsBytes];
+  __shared__ int s_go[2];
+  const int G = int(gridDim.x);
+  const int w = int(blockIdx.x);
+  const int x = w & 7, j = w >> 3, per = G >> 3;
+  const int tid = int(threadIdx.x), half = tid >> 8;
+  const bool lead = (tid & (kBlock - 1)) == 0;  // thread 0 of a half
+  for (int p = 0; p < nphase; ++p) {
+    const hfm_phase ph = prog[p];
+    const int n8 = ph.nitems > x ? (ph.nitems - x + 7) / 8 : 0;  // items with g % 8 == x
+    const int e0 = int(int64_t(n8) * j / per), e1 = int(int64_t(n8) * (j + 1) / per);
+    const bool whole = ph.kind == HFM_PH_SMOOTH_RESTRICT || ph.kind == HFM_PH_SMOOTH2 ||
+                       ph.kind == HFM_PH_COARSE_CYCLE;
+    // Per half (the same values in every thread): the slot whose finished
+    // items this workgroup has not signalled yet and their count, and the
+    // slot whose dependency is known to be met.  Consecutive items mostly
+    // belong to one instance: they then take no wait and no signal of their
+    // own (the count is raised once, when the slot changes or the phase ends).
+    int pend[2] = {-1, -1}, npend[2] = {0, 0}, okslot[2] = {-1, -1};
+    for (int e = e0; e < e1; e += whole ? 1 : 2) {
+      const int g0 = x + 8 * e;
+      const int g1 = whole ? -1 : (e + 1 < e1 ? g0 + 8 : -1);
+      const int sl[2] = {item_slot(ph, g0), g1 >= 0 ? item_slot(ph, g1) : -1};
+      const int nh = whole ? 1 : 2;
+      bool fast = true;
+      for (int h = 0; h < nh; ++h)
+        fast = fast && (sl[h] < 0 || (sl[h] == pend[h] && sl[h] == okslot[h]));
+      if (!fast || whole) {
+        // the previous items are done by every thread before their signal and
+        // before this item touches the LDS
+        __syncthreads();
+        if (lead && (whole ? tid == 0 : true) && half < nh) {
+          bool ok = true;
+          if (sl[half] != pend[half] && npend[half] > 0) {
+            persist_signal(ctl, p, pend[half], npend[half]);
+          }
+          if (sl[half] >= 0 && sl[half] != okslot[half])
+            ok = persist_wait(ctl, p, sl[half], ticks);
+          s_go[half] = ok ? 1 : 0;
+        }
+        __syncthreads();
+        const bool go = s_go[0] != 0 && (nh == 1 || s_go[1] != 0);
+        if (!go) return;  // a wait failed: every workgroup leaves (uniformly here)
+        for (int h = 0; h < nh; ++h) {
+          if (sl[h] != pend[h]) {
+            pend[h] = sl[h];
+            npend[h] = 0;
+          }
+          okslot[h] = sl[h];
+        }
+      }
+      if (whole) {
+        const Blk bk{g0, ph.nitems, tid, 0};
+        if (ph.kind == HFM_PH_COARSE_CYCLE) {
+          coarse_cycle_body<T, kPersistThreads>(ph.tasks, ph.chain, sc, ph.nu1, ph.nu2,
+                                                ph.omega, ph.alpha, bk);
+        } else if (ph.kind == HFM_PH_SMOOTH_RESTRICT) {
+          smooth_pair_body<T, false, T, false>(ph.tasks, ph.blk_task, sc, ph.omega, ph.alpha,
+                                               ph.dots,
+                                               *reinterpret_cast<PairLds<T, false>*>(lds), bk);
+        } else if (ph.dot) {
+          smooth_pair_body<T, true, P, true>(ph.tasks, ph.blk_task, sc, ph.omega, ph.alpha,
+                                             ph.dots,
+                                             *reinterpret_cast<PairLds<T, true>*>(lds), bk);
+        } else {
+          smooth_pair_body<T, true, T, false>(ph.tasks, ph.blk_task, sc, ph.omega, ph.alpha,
+                                              ph.dots,
+                                              *reinterpret_cast<PairLds<T, true>*>(lds), bk);
+        }
+      } else {
+        persist_half<T, P>(ph, half == 0 ? g0 : g1, half, sc);
+      }
+      for (int h = 0; h < nh; ++h)
+        if (sl[h] >= 0) npend[h] += 1;
+    }
+    // the phase's last signals, once every thread is done
+    __syncthreads();
+    if (lead && npend[half] > 0 && (whole ? tid == 0 : true))
+      persist_signal(ctl, p, pend[half], npend[half]);
+  }
 }
 
 }  // namespace hfm
@@ -2180,4 +2570,83 @@ extern "C" int hfm_mg_coarse_inverse(const hfm_mg_task* tasks, int64_t ntasks, v
   hipLaunchKernelGGL(k_coarse_inverse, dim3(unsigned(ntasks)), dim3(kBlock), 0,
                      as_stream(stream), tasks);
   return launch_status("hfm_mg_coarse_inverse");
+}
+
+namespace {
+template <typename T, typename P>
+const void* persistent_kernel() {
+  return reinterpret_cast<const void*>(&k_persistent<T, P>);
+}
+
+int persistent_fn(int store, int pstore, const void** fn) {
+  if (store == HFM_STORE_F32 && pstore == HFM_STORE_F32) *fn = persistent_kernel<float, float>();
+  else if (store == HFM_STORE_BF16 && pstore == HFM_STORE_BF16) *fn = persistent_kernel<bf16, bf16>();
+  else if (store == HFM_STORE_BF16 && pstore == HFM_STORE_F32) *fn = persistent_kernel<bf16, float>();
+  else if (store == HFM_STORE_F32 && pstore == HFM_STORE_BF16) *fn = persistent_kernel<float, bf16>();
+  else return HFM_EINVAL;
+  return HFM_OK;
+}
+}  // namespace
+
+extern "C" int hfm_mg_persistent_workgroups(int store, int pstore, int32_t* n) {
+  const void* fn = nullptr;
+  if (!n || persistent_fn(store, pstore, &fn) != HFM_OK) {
+    set_error("hfm_mg_persistent_workgroups: bad arguments");
+    return HFM_EINVAL;
+  }
+  int dev = 0, cus = 0, per = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, kPersistThreads, 0) != hipSuccess ||
+      per < 1) {
+    set_error("hfm_mg_persistent_workgroups: occupancy query failed");
+    return HFM_EDEVICE;
+  }
+  *n = int32_t((int64_t(cus) * per) / 8 * 8);
+  return HFM_OK;
+}
+
+extern "C" int hfm_mg_persistent(const hfm_phase* prog, int32_t nphase, const hfm_persist_ctl* ctl,
+                                 hfm_mg_scalars* sc, int store, int pstore, int32_t workgroups,
+                                 double timeout_s, void* stream) {
+  const void* fn = nullptr;
+  if (!prog || !ctl || !sc || nphase < 0 || !ctl->cnt || !ctl->need || !ctl->prev ||
+      !ctl->fail || ctl->nslot < 1 || workgroups < 8 || workgroups % 8 ||
+      !(timeout_s > 0.0) || persistent_fn(store, pstore, &fn) != HFM_OK) {
+    set_error("hfm_mg_persistent: bad arguments");
+    return HFM_EINVAL;
+  }
+  if (nphase == 0) return HFM_OK;
+  int dev = 0, rate_khz = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess ||
+      rate_khz <= 0) {
+    set_error("hfm_mg_persistent: cannot read the device wall-clock rate");
+    return HFM_EDEVICE;
+  }
+  const hfm_phase* pr = prog;
+  int np = nphase;
+  hfm_persist_ctl c = *ctl;
+  hfm_mg_scalars* s = sc;
+  uint64_t ticks = uint64_t(double(rate_khz) * 1000.0 * timeout_s);
+  void* args[] = {(void*)&pr, (void*)&np, (void*)&c, (void*)&s, (void*)&ticks};
+  const hipError_t e = hipLaunchCooperativeKernel(fn, dim3(unsigned(workgroups)),
+                                                  dim3(kPersistThreads), args, 0,
+                                                  as_stream(stream));
+  if (e != hipSuccess) {
+    set_error("hfm_mg_persistent: cooperative launch of %d workgroups failed: %s", workgroups,
+              hipGetErrorString(e));
+    return HFM_EDEVICE;
+  }
+  return launch_status("hfm_mg_persistent");
+}
+
+extern "C" int hfm_mg_persist_struct_sizes(int64_t* phase_size, int64_t* ctl_size) {
+  if (!phase_size || !ctl_size) {
+    set_error("hfm_mg_persist_struct_sizes: null output");
+    return HFM_EINVAL;
+  }
+  *phase_size = int64_t(sizeof(hfm_phase));
+  *ctl_size = int64_t(sizeof(hfm_persist_ctl));
+  return HFM_OK;
 }

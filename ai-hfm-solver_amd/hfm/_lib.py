@@ -109,9 +109,28 @@ SIGNATURES = {
     "hfm_mg_galerkin_batched": (_int, [_vp, _vp, _i64, _vp]),
     "hfm_mg_convert_batched": (_int, [_vp, _vp, _i64, _int, _vp]),
     "hfm_mg_coarse_inverse": (_int, [_vp, _i64, _vp]),
+    "hfm_mg_persistent": (_int, [_vp, C.c_int32, _vp, _vp, _int, _int, C.c_int32, C.c_double,
+                                 _vp]),
+    "hfm_mg_persistent_workgroups": (_int, [_int, _int, C.POINTER(C.c_int32)]),
+    "hfm_mg_persist_struct_sizes": (_int, [C.POINTER(_i64), C.POINTER(_i64)]),
     "hfm_flow2d_steps": (_int, [_vp, _i64, _vp, _vp, _int, _int, _vp]),
     "hfm_flow2d_struct_sizes": (_int, [C.POINTER(_i64), C.POINTER(_i64)]),
 }
+
+PHASE_DTYPE = np.dtype([
+    ("kind", "<i4"), ("mode", "<i4"), ("dot", "<i4"), ("nitems", "<i4"),
+    ("nu1", "<i4"), ("nu2", "<i4"), ("slot0", "<i4"), ("pad", "<i4"),
+    ("omega", "<f4"), ("alpha", "<f4"),
+    ("tasks", "<u8"), ("blk_task", "<u8"), ("chain", "<u8"), ("dots", "<u8"),
+])
+PH = {"sweep": 0, "restrict": 1, "smooth_restrict": 2, "smooth2": 3, "coarse_cycle": 4,
+      "finalize": 5, "dir": 6, "update": 7, "flush": 8}
+
+
+class PersistCtl(C.Structure):
+    _fields_ = [("cnt", C.c_void_p), ("need", C.c_void_p), ("prev", C.c_void_p),
+                ("fail", C.c_void_p), ("nslot", C.c_int32), ("pad", C.c_int32)]
+
 
 FLOW_TASK_DTYPE = np.dtype([
     ("u", "<u8"), ("v", "<u8"), ("uh", "<u8"), ("vh", "<u8"), ("z", "<u8"), ("p", "<u8"),
@@ -171,6 +190,11 @@ def load() -> C.CDLL:
         if ts.value != FLOW_TASK_DTYPE.itemsize or ss.value != FLOW_PARAMS_DTYPE.itemsize:
             raise HfmError(f"ABI mismatch: flow task {ts.value} vs {FLOW_TASK_DTYPE.itemsize}, "
                            f"params {ss.value} vs {FLOW_PARAMS_DTYPE.itemsize}")
+        if lib.hfm_mg_persist_struct_sizes(C.byref(ts), C.byref(ss)) != HFM_OK:
+            raise HfmError("hfm_mg_persist_struct_sizes failed")
+        if ts.value != PHASE_DTYPE.itemsize or ss.value != C.sizeof(PersistCtl):
+            raise HfmError(f"ABI mismatch: phase {ts.value} vs {PHASE_DTYPE.itemsize}, "
+                           f"persist ctl {ss.value} vs {C.sizeof(PersistCtl)}")
         _lib = lib
         return lib
 

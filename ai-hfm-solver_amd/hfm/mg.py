@@ -816,6 +816,117 @@ class Plan:
         return self._graphs[m], m
 
 
+class PersistentProgram:
+    """``m`` FCG iterations of a plan as one hfm_mg_persistent launch.
+
+    The phases are the plan's launches in the order ``Plan.iteration`` makes
+    them (V-cycle down, coarse cycle, up, finalize BETA, direction, ALPHA,
+    update, CHECK, flush), with the same tables; the cell kernels run with
+    256-thread blocks (block size changes no value: see ``hfm.tune``).  The
+    per-(phase, slot) item counts and each phase's predecessor per slot are
+    computed here from the tables, once per program."""
+
+    def __init__(self, plan: "Plan", m: int):
+        pool = plan.pool
+        p = pool.params
+        self.pool, self.plan, self.m = pool, plan, int(m)
+        dev = pool.device
+        lo, hi = plan.slots if plan.slots is not None else (0, pool.capacity)
+        nslot = pool.capacity
+        dots = pool.dots.data_ptr()
+        rows, item_slots = [], []
+
+        def host_slots(tasks, lay, tiles=False):
+            # the instance slot of each item (block) g; a fused pair's block g
+            # runs tile (g % 8) (n / 8) + g / 8 (the kernels' XCD map)
+            inst = tasks.cpu().numpy().view(L.TASK_DTYPE)["inst"].astype(np.int64)
+            bt = lay.blk_task.cpu().numpy().view(np.uint16).astype(np.int64)
+            if tiles:
+                g = np.arange(len(bt))
+                bt = bt[(g & 7) * (len(bt) >> 3) + (g >> 3)]
+            return np.where(bt == 0xFFFF, -1, inst[np.minimum(bt, len(inst) - 1)])
+
+        def add(kind, nitems, slots, tasks=0, blk=0, mode=0, dot=0, chain=0, slot0=0):
+            r = np.zeros((), dtype=L.PHASE_DTYPE)
+            r["kind"], r["mode"], r["dot"], r["nitems"] = L.PH[kind], mode, int(bool(dot)), nitems
+            r["nu1"], r["nu2"], r["slot0"] = p.nu1, p.nu2, slot0
+            r["omega"], r["alpha"] = p.omega, p.alpha
+            r["tasks"], r["blk_task"], r["chain"], r["dots"] = tasks, blk, chain, dots
+            rows.append(r)
+            item_slots.append(slots)
+
+        def add_op(op):
+            kind, mode, tasks, lay, dot, _, _ = op
+            tiles = kind in ("smooth_restrict", "smooth2")
+            add(kind, lay.nblocks, host_slots(tasks, lay, tiles), tasks.data_ptr(),
+                lay.blk_task.data_ptr(), mode=mode, dot=dot)
+
+        fin_slots = np.arange(lo, hi, dtype=np.int64)
+        chain = plan.chain.cpu().numpy().reshape(-1, 2)
+        crec = plan.chain_recs.cpu().numpy().view(L.TASK_DTYPE)["inst"].astype(np.int64)
+        coarse_slots = np.where(chain[:, 1] > 0, crec[np.minimum(chain[:, 0], len(crec) - 1)], -1)
+        fine = [host_slots(plan.dir_tasks[q], plan.flay) for q in range(2)]
+        for it in range(self.m):
+            par = it % 2
+            for op in plan.ops_down:
+                add_op(op)
+            add("coarse_cycle", len(plan.insts), coarse_slots, plan.chain_recs.data_ptr(),
+                chain=plan.chain.data_ptr())
+            for ups in plan.ops_up:
+                for op in ups:
+                    add_op(op)
+            for kind, fin, tasks in (("finalize", "beta", None), ("dir", None, plan.dir_tasks[par]),
+                                     ("finalize", "alpha", None),
+                                     ("update", None, plan.upd_tasks[par]),
+                                     ("finalize", "check", None),
+                                     ("flush", None, plan.upd_tasks[par])):
+                if kind == "finalize":
+                    add("finalize", hi - lo, fin_slots, mode=L.FIN[fin], slot0=lo)
+                else:
+                    add(kind, plan.flay.nblocks, fine[par], tasks.data_ptr(),
+                        plan.flay.blk_task.data_ptr())
+        self.nphase = len(rows)
+        self.item_slots = item_slots
+        need = np.zeros((self.nphase, nslot), dtype=np.int32)
+        for q, sl in enumerate(item_slots):
+            sl = sl[sl >= 0]
+            need[q] = np.bincount(sl, minlength=nslot)[:nslot]
+        prev = np.full((self.nphase, nslot), -1, dtype=np.int32)
+        last = np.full(nslot, -1, dtype=np.int32)
+        for q in range(self.nphase):
+            prev[q] = last
+            last = np.where(need[q] > 0, q, last)
+        self.prog = torch.from_numpy(np.array(rows, dtype=L.PHASE_DTYPE).view(np.uint8)
+                                     .copy()).to(dev)
+        self.need = torch.from_numpy(need).to(dev)
+        self.prev = torch.from_numpy(prev).to(dev)
+        self.cnt = torch.zeros((self.nphase, nslot), dtype=torch.int32, device=dev)
+        self.fail = torch.zeros(1, dtype=torch.int32, device=dev)
+        self.ctl = L.PersistCtl(self.cnt.data_ptr(), self.need.data_ptr(), self.prev.data_ptr(),
+                                self.fail.data_ptr(), nslot, 0)
+        self.items = int(sum(int(r["nitems"]) for r in rows))
+
+    def launch(self, s, workgroups: int, timeout_s: float = 2.0):
+        """Run the program on stream ``s`` (the counters and the failure word
+        are cleared first, with fill kernels)."""
+        self.cnt.zero_()
+        self.fail.zero_()
+        pool = self.pool
+        L.call("hfm_mg_persistent", self.prog.data_ptr(), self.nphase, L.C.byref(self.ctl),
+               pool.sc.data_ptr(), pool.store_id, pool.outer_id, int(workgroups),
+               float(timeout_s), s)
+
+    def failed(self) -> bool:
+        return int(self.fail.item()) != 0
+
+
+def persistent_workgroups(pool: "Pool") -> int:
+    """Resident workgroups of the persistent kernel for the pool's storage types."""
+    n = L.C.c_int32()
+    L.call("hfm_mg_persistent_workgroups", pool.store_id, pool.outer_id, L.C.byref(n))
+    return int(n.value)
+
+
 class MGBatch:
     """A fixed batch of instances of given shapes sharing every launch.
 
@@ -1024,15 +1135,19 @@ class MGBatch:
         self.last_info = info
         return info
 
-    def solve(self):
+    def solve(self, persistent: bool = False):
         """Run FCG to ``tol`` (or ``maxit``) for every instance; returns the scalar records.
 
-        Iterations run ``iters_per_graph`` at a time (one HIP-graph replay);
-        between replays the active flags are read and, once at most
-        ``shrink`` of the plan's instances are still iterating, the launches
-        continue on a plan of the active ones only (no empty blocks).
+        Iterations run ``iters_per_graph`` at a time (one HIP-graph replay, or
+        with ``persistent`` one hfm_mg_persistent launch of a program of
+        those iterations); between them the active flags are read and, once at
+        most ``shrink`` of the plan's instances are still iterating, the
+        launches continue on a plan of the active ones only (no empty blocks).
+        Both modes give bitwise the same results.
         """
         self._need_gpu()
+        if persistent:
+            return self._solve_persistent()
         if self.streams == 2 and self._groups is not None:
             return self._solve_groups()
         s = L.stream_handle(self.device)
@@ -1056,6 +1171,47 @@ class MGBatch:
                 break
             if len(live) <= self.shrink * len(plan.insts):
                 plan = self._subplan(live)
+        info = self.scalars()
+        self.last_info = info
+        return info
+
+    def _persistent(self, live) -> PersistentProgram:
+        """The persistent program of ``iters_per_graph`` iterations over the
+        instances ``live`` (cached), on a plan with 256-thread cell blocks."""
+        m = self.iters_per_graph + self.iters_per_graph % 2
+        key = ("persistent", tuple(i.slot for i in live), m)
+        prog = self._subplans.get(key)
+        if prog is None:
+            blocks, self.params.blocks = self.params.blocks, None
+            try:
+                plan = Plan(self.pool, live)
+            finally:
+                self.params.blocks = blocks
+            prog = PersistentProgram(plan, m)
+            self._subplans[key] = prog
+        return prog
+
+    def _solve_persistent(self):
+        s = L.stream_handle(self.device)
+        if getattr(self, "_pwg", None) is None:
+            self._pwg = persistent_workgroups(self.pool)
+        self.restart()
+        act = self.pool.active_view()
+        its, maxit = 0, self.params.maxit
+        live = list(self.insts)
+        prog = self._persistent(live)
+        while its < maxit:
+            prog.launch(s, self._pwg)
+            its += prog.m
+            flags = act.cpu().numpy()
+            if prog.failed():
+                raise L.HfmError("hfm_mg_persistent: a dependency wait timed out")
+            now = [i for i in live if flags[i.slot]]
+            if not now:
+                break
+            if len(now) <= self.shrink * len(live):
+                live = now
+                prog = self._persistent(live)
         info = self.scalars()
         self.last_info = info
         return info

@@ -49,11 +49,11 @@ def load_batch(mg, specs, dev, chunk=64):
             mg.stage_sigma(c0 + i, sx, sy, sz, b)
 
 
-def one_step(mg):
+def one_step(mg, persistent=False):
     """faces from sigma -> hierarchy -> solve, for every instance."""
     mg.build_faces()
     mg.setup()
-    return mg.solve()
+    return mg.solve(persistent=persistent)
 
 
 def kernel_time_ms(fn, reps, torch):
@@ -892,6 +892,9 @@ def main():
                     help="finest levels that run fused pass pairs (default: the solver's)")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep 256-thread blocks (skip the launch-shape tuner)")
+    ap.add_argument("--persistent-steps", type=int, default=3,
+                    help="also time the batch with one persistent launch per iters_per_graph "
+                         "iterations (0: skip)")
     ap.add_argument("--tune-schedule", type=int, default=1,
                     help="1: also rank sweep schedules by time to 1e-8 (reported beside "
                          "the V(2,2) headline)")
@@ -996,6 +999,8 @@ def main():
     rates = {"cell_updates_per_s": float(np.sum(its * 4 * swept)) * args.steps / total,
              "fine_cell_vcycles_per_s": float(np.sum(its * fine)) * args.steps / total}
 
+    probe_q = (0, len(specs) // 2, len(specs) - 1)
+    xs0 = [mg.solution(q).clone() for q in probe_q] if args.persistent_steps > 0 else []
     # dominant kernel: the fine-level FCG update (x += a p, r -= a A p), timed live
     mg.restart()
     fine_cells = cells
@@ -1025,6 +1030,30 @@ def main():
                                "achieved": round(sweep_bytes / (sweep_ms * 1e-3) / 1e9, 1)}}
     extra = {}
     outer_storage = mg.params.outer
+    if args.persistent_steps > 0:
+        # the same batch, each group of iters_per_graph iterations one
+        # persistent launch (per-instance dependency counters) instead of
+        # graph-replayed launches; bitwise the same results
+        from hfm.mg import persistent_workgroups
+        info_p = one_step(mg, persistent=True)          # warm: programs built
+        torch.cuda.synchronize()
+        pt = []
+        for _ in range(args.persistent_steps):
+            t0 = time.perf_counter()
+            info_p = one_step(mg, persistent=True)
+            torch.cuda.synchronize()
+            pt.append(time.perf_counter() - t0)
+        same = all(bool((mg.solution(q) == x).all())
+                   for q, x in zip(probe_q, xs0))
+        extra["persistent"] = {
+            "solves_per_s": round(len(specs) * len(pt) / sum(pt), 2),
+            "ms_per_step": round(sum(pt) / len(pt) * 1e3, 2),
+            "latency_ms": {"p50": round(float(np.percentile(pt, 50)) * 1e3, 2),
+                           "p99": round(float(np.percentile(pt, 99)) * 1e3, 2)},
+            "workgroups": persistent_workgroups(mg.pool),
+            "iters_equal": bool(np.array_equal(info_p["iter"], info["iter"])),
+            "solutions_bitwise_equal": same}
+        log(f"bench: persistent {extra['persistent']}")
     if args.tune_schedule:
         # the sweep schedule (nu1, nu2, alpha) ranked by the batch's time to 1e-8;
         # the headline keeps the configuration's V(2,2)

@@ -428,6 +428,59 @@ int hfm_mg_convert_batched(const hfm_mg_task* tasks, const uint16_t* blk_task, i
                            int store, void* stream);
 int hfm_mg_coarse_inverse(const hfm_mg_task* tasks, int64_t ntasks, void* stream);
 
+/* ---------------- persistent iteration ----------------
+ * hfm_mg_persistent runs a program of phases in one cooperative launch: a
+ * phase is one of the launches above (its table, block map and arguments;
+ * the cell kernels with 256-thread blocks), its work items that launch's
+ * blocks (for HFM_PH_FINALIZE the scalar slots slot0 .. slot0 + nitems - 1,
+ * for HFM_PH_COARSE_CYCLE the chain's instances).  For each instance slot the
+ * phases run in program order: an item of phase p for slot s starts once all
+ * need[q][s] items of phase q = prev[p][s] for s have finished (prev -1: no
+ * wait), so the results are bitwise those of the launches run one after the
+ * other.  Arrays are [nphase][nslot] in device memory; cnt must be zero at the
+ * launch.  A wait longer than timeout_s (wall clock) sets *fail and ends the
+ * launch early (results void).  workgroups: a multiple of 8, at most
+ * hfm_mg_persistent_workgroups (every workgroup must be resident).  No
+ * reference counterpart: the reference's multigrid cycles are host loops of
+ * layer calls (ART/Anisotropic_3d_Multigrid_7point.ipynb:402-454). */
+#define HFM_PH_SWEEP 0           /* hfm_mg_sweep (mode, dot; out = pstore if dot)   */
+#define HFM_PH_RESTRICT 1        /* hfm_mg_resid_restrict                          */
+#define HFM_PH_SMOOTH_RESTRICT 2 /* hfm_mg_smooth_restrict                         */
+#define HFM_PH_SMOOTH2 3         /* hfm_mg_smooth2 (dot; out = pstore if dot)       */
+#define HFM_PH_COARSE_CYCLE 4    /* hfm_mg_coarse_cycle (tasks = records, chain)    */
+#define HFM_PH_FINALIZE 5        /* hfm_fcg_finalize (mode = kind)                  */
+#define HFM_PH_DIR 6             /* hfm_fcg_dir                                     */
+#define HFM_PH_UPDATE 7          /* hfm_fcg_update_sweep                            */
+#define HFM_PH_FLUSH 8           /* hfm_fcg_flush                                   */
+
+typedef struct hfm_phase {
+  int32_t kind, mode, dot, nitems;
+  int32_t nu1, nu2, slot0, pad;
+  float omega, alpha;
+  const hfm_mg_task* tasks;
+  const uint16_t* blk_task;
+  const int32_t* chain;
+  double* dots;
+} hfm_phase; /* 72 bytes */
+
+typedef struct hfm_persist_ctl {
+  uint32_t* cnt;
+  const int32_t* need;
+  const int32_t* prev;
+  uint32_t* fail;
+  int32_t nslot;
+  int32_t pad;
+} hfm_persist_ctl;
+
+int hfm_mg_persistent(const hfm_phase* prog, int32_t nphase, const hfm_persist_ctl* ctl,
+                      hfm_mg_scalars* sc, int store, int pstore, int32_t workgroups,
+                      double timeout_s, void* stream);
+/* Workgroups of hfm_mg_persistent that are resident at once on the current
+ * device (a multiple of 8). */
+int hfm_mg_persistent_workgroups(int store, int pstore, int32_t* n);
+/* sizeof(hfm_phase) and sizeof(hfm_persist_ctl) as compiled. */
+int hfm_mg_persist_struct_sizes(int64_t* phase_size, int64_t* ctl_size);
+
 /* ---------------- 2D periodic flow (Kolmogorov configuration) ----------------
  * The reference's 2D layers (explicit two-stage stencil update,
  * ARTn/Advection_diffusion.ipynb:291-296; mean restriction / nearest

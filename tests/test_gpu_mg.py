@@ -358,3 +358,31 @@ def test_every_schedule_converges_to_the_true_residual(dev):
     best = tune_schedule(mg)
     assert tuple(best) in {tuple(s) for s in SCHEDULES}
     assert (mg.params.nu1, mg.params.nu2, mg.params.alpha) == tuple(best)
+
+
+@pytest.mark.parametrize("storage,outer,method,fuse", [
+    ("bf16", None, "fcg", True), ("f32", None, "fcg", True), ("bf16", "f32", "fcg", True),
+    ("bf16", None, "refine", False), ("bf16", None, "fcg", False)])
+def test_persistent_solve_equals_the_launch_plan_bitwise(dev, storage, outer, method, fuse):
+    """The persistent launch (one hfm_mg_persistent program per iters_per_graph
+    iterations, per-instance dependency counters) gives bitwise the solutions,
+    iteration counts and statuses of the graph-replayed launches; instances
+    stop at different iterations, so the programs shrink on the way."""
+    print(f"persistent vs launches: {storage} {outer} {method} fuse={fuse}", flush=True)
+    specs = [((64, 64, 64), 81, 1.0, 2), ((48, 96, 32), 82, 2.0, 1), ((20, 12, 36), 83, 0.7, 3),
+             ((32, 32, 32), 84, 0.5, 1), ((96, 40, 48), 85, 1.5, 4), ((9, 7, 16), 86, 1.0, 1)]
+    kw = dict(tol=1e-8, maxit=300, outer=outer, method=method, fuse=fuse, fuse_min_tiles=0)
+    if method == "refine":
+        kw["alpha"] = 1.0 if storage == "f32" else 1.6
+    mg, _ = _batch(dev, specs, storage, **kw)
+    ref = mg.solve()
+    xr = [mg.solution(q).clone() for q in range(len(specs))]
+    got = mg.solve(persistent=True)
+    assert list(got["iter"]) == list(ref["iter"]) and list(got["status"]) == list(ref["status"])
+    assert len(set(int(v) for v in ref["iter"])) > 1
+    for q in range(len(specs)):
+        assert bool((mg.solution(q) == xr[q]).all()), q
+    again = mg.solve(persistent=True)       # the cached programs
+    assert list(again["iter"]) == list(ref["iter"])
+    for q in range(len(specs)):
+        assert bool((mg.solution(q) == xr[q]).all()), q

@@ -464,3 +464,102 @@ def test_random_modes_ic_is_the_oracles():
     for N, seed in ((32, 3), (64, 2024)):
         for a, b in zip(random_modes_ic(N, seed), F.random_modes_ic(N, seed)):
             np.testing.assert_array_equal(a, b)
+
+
+def _simulate_persistent(prog, G):
+    """Run the persistent kernel's schedule on the host: workgroup w takes, per
+    phase, the items g = x + 8 e (x = w % 8) of its contiguous e-range, two at a
+    time for the 256-thread phases, in program order.  An item whose slot
+    differs from the half's previous one first raises the count of the
+    previous slot by the number of its items, then waits for
+    cnt[prev][slot] == need[prev][slot]; the phase's last counts are raised at
+    its end.  Returns the per-(phase, slot) counts, or raises on a deadlock (no
+    workgroup can move)."""
+    from hfm import _lib as L
+    rows = prog.prog.numpy().view(L.PHASE_DTYPE)
+    need, prev = prog.need.numpy(), prog.prev.numpy()
+    whole_kinds = {L.PH["smooth_restrict"], L.PH["smooth2"], L.PH["coarse_cycle"]}
+    ops = []   # per workgroup: ("signal", phase, slot, count) and ("wait", phase, slots)
+    for w in range(G):
+        x, j, per = w & 7, w >> 3, G >> 3
+        seq = []
+        for p, r in enumerate(rows):
+            n = int(r["nitems"])
+            n8 = (n - x + 7) // 8 if n > x else 0
+            e0, e1 = n8 * j // per, n8 * (j + 1) // per
+            whole = int(r["kind"]) in whole_kinds
+            nh = 1 if whole else 2
+            pend, npend, okslot = [-1, -1], [0, 0], [-1, -1]
+            e = e0
+            while e < e1:
+                gs = [x + 8 * e] + ([] if whole else [x + 8 * (e + 1)] if e + 1 < e1 else [-1])
+                sl = [int(prog.item_slots[p][g]) if g >= 0 else -1 for g in gs]
+                fast = all(s < 0 or (s == pend[h] and s == okslot[h]) for h, s in enumerate(sl))
+                if whole or not fast:
+                    for h in range(nh):
+                        if sl[h] != pend[h] and npend[h] > 0:
+                            seq.append(("signal", p, pend[h], npend[h]))
+                    seq.append(("wait", p, [s for h, s in enumerate(sl)
+                                            if s >= 0 and s != okslot[h]]))
+                    for h in range(nh):
+                        if sl[h] != pend[h]:
+                            pend[h], npend[h] = sl[h], 0
+                        okslot[h] = sl[h]
+                for h, s in enumerate(sl):
+                    if s >= 0:
+                        npend[h] += 1
+                e += 1 if whole else 2
+            for h in range(nh):
+                if npend[h] > 0:
+                    seq.append(("signal", p, pend[h], npend[h]))
+        ops.append(seq)
+    cnt = np.zeros_like(need)
+    pos = [0] * G
+    left = sum(len(s) for s in ops)
+    while left:
+        moved = False
+        for w in range(G):
+            while pos[w] < len(ops[w]):
+                op = ops[w][pos[w]]
+                if op[0] == "signal":
+                    cnt[op[1], op[2]] += op[3]
+                else:
+                    p = op[1]
+                    if not all(prev[p, s] < 0 or cnt[prev[p, s], s] >= need[prev[p, s], s]
+                               for s in op[2]):
+                        break
+                pos[w] += 1
+                left -= 1
+                moved = True
+        if not moved:
+            raise AssertionError("persistent schedule deadlocks")
+    return cnt
+
+
+@pytest.mark.parametrize("fuse", [True, False])
+def test_persistent_program_schedule(fuse):
+    """The persistent program built from a plan: every phase's predecessor per slot
+    is an earlier phase, every item belongs to a slot of the plan (or is a
+    padding tile), the phases follow Plan.iteration, and the kernel's static
+    schedule completes every item without a deadlock for several grid sizes."""
+    from hfm import _lib as L
+    from hfm.mg import MGBatch, PersistentProgram
+    shapes = [(64, 64, 64), (48, 96, 32), (20, 12, 36), (32, 32, 32), (9, 7, 16)]
+    mg = MGBatch(shapes, storage="bf16", device="cpu", fuse=fuse, fuse_min_tiles=0)
+    prog = PersistentProgram(mg.plan, 2)
+    rows = prog.prog.numpy().view(L.PHASE_DTYPE)
+    kinds = [int(k) for k in rows["kind"]]
+    per_it = len(kinds) // 2
+    assert kinds[:per_it] == kinds[per_it:]
+    tail = [L.PH[k] for k in ("finalize", "dir", "finalize", "update", "finalize", "flush")]
+    assert kinds[per_it - 6:per_it] == tail
+    assert (L.PH["smooth_restrict"] in kinds) == fuse
+    need, prev = prog.need.numpy(), prog.prev.numpy()
+    for p in range(prog.nphase):
+        assert np.all(prev[p] < p)
+        sl = prog.item_slots[p]
+        assert need[p].sum() == np.count_nonzero(sl >= 0)
+        assert set(sl[sl >= 0]) <= {i.slot for i in mg.insts} | set(range(mg.pool.capacity))
+    for G in (8, 16, 64, 256):
+        cnt = _simulate_persistent(prog, G)
+        assert np.array_equal(cnt, need)
