@@ -2002,7 +2002,8 @@ __device__ __forceinline__ bool persist_wait(const hfm_persist_ctl& ctl, int p, 
 __device__ __forceinline__ void persist_signal(const hfm_persist_ctl& ctl, int p, int slot,
                                                int count) {
   if (slot < 0 || count <= 0) return;
-  __threadfence();
+  // the release writes back this XCD's L2: the items' stores (ordered before
+  // it by the workgroup barrier) reach the other XCDs
   __hip_atomic_fetch_add(ctl.cnt + p * ctl.nslot + slot, uint32_t(count), __ATOMIC_RELEASE,
                          __HIP_MEMORY_SCOPE_AGENT);
 }

@@ -230,6 +230,45 @@ def serving(args, dev, torch):
     return out
 
 
+def small_batches(args, dev, torch, sizes=(1, 8, 32)):
+    """Launch-bound batches: n instances of edges 32..64 solved with graph-replayed
+    launches and with the persistent kernel, the batch's wall time from staged
+    fields (faces, hierarchy, solve) per mode, median of ``reps`` after a warm
+    run.  The two modes give bitwise the same solutions."""
+    from hfm.mg import MGBatch
+    from hfm.workloads import batched_specs
+    out = {}
+    for n in sizes:
+        specs = batched_specs(n, args.seed + 7, edges=(32, 48, 64))
+        mg = MGBatch([s[0] for s in specs], storage=args.storage, tol=1e-8, maxit=300,
+                     device=dev, iters_per_graph=args.iters_per_graph, shrink=args.shrink,
+                     outer=args.outer, fuse=not args.no_fuse, streams=args.streams,
+                     **_fuse_kw(args))
+        load_batch(mg, specs, dev)
+        row = {"cells": int(sum(math.prod(s[0]) for s in specs))}
+        xs = {}
+        for mode in (False, True):
+            one_step(mg, persistent=mode)
+            torch.cuda.synchronize()
+            ts = []
+            for _ in range(7):
+                t0 = time.perf_counter()
+                info = one_step(mg, persistent=mode)
+                torch.cuda.synchronize()
+                ts.append(time.perf_counter() - t0)
+            xs[mode] = [mg.solution(q).clone() for q in range(n)]
+            row["persistent_ms" if mode else "graphs_ms"] = round(float(np.median(ts)) * 1e3, 3)
+            row["iters_max"] = int(info["iter"].max())
+            if not np.all(info["status"] == 1):
+                raise SystemExit(f"small batch {n}: not converged")
+        row["bitwise_equal"] = all(bool((a == b).all()) for a, b in zip(xs[False], xs[True]))
+        row["speedup_persistent"] = round(row["graphs_ms"] / row["persistent_ms"], 2)
+        out[str(n)] = row
+        log(f"bench: small batch {n}: {row}")
+        del mg
+    return out
+
+
 def batched_fp32(specs, dev, torch):
     """The same 512-instance batch with FP32 storage (the configuration's plain run)."""
     from hfm.mg import MGBatch
@@ -1054,6 +1093,7 @@ def main():
             "iters_equal": bool(np.array_equal(info_p["iter"], info["iter"])),
             "solutions_bitwise_equal": same}
         log(f"bench: persistent {extra['persistent']}")
+        extra["small_batches"] = small_batches(args, dev, torch)
     if args.tune_schedule:
         # the sweep schedule (nu1, nu2, alpha) ranked by the batch's time to 1e-8;
         # the headline keeps the configuration's V(2,2)
