@@ -148,6 +148,32 @@ def fp32_one_at_a_time(specs, dev, torch):
     return len(specs) / t_tot
 
 
+def true_rel_residual(inst, b, x, torch):
+    """||b - A x||_2 / ||b||_2 in FP64 with A the instance's stored FP32 face
+    operator (low faces cx/cy/cz, high boundary faces bxh/byh/bzh), with torch
+    array ops apart from the solver's kernels."""
+    D, H, W = inst.shape
+    N = D * H * W
+    f, o = inst.face32[0], inst.face_off[0]
+    cx, cy, cz = (f[o[q]:o[q] + N].view(D, H, W).double() for q in range(3))
+    bxh = f[o[3]:o[3] + D * H].view(D, H).double()
+    byh = f[o[4]:o[4] + D * W].view(D, W).double()
+    bzh = f[o[5]:o[5] + H * W].view(H, W).double()
+    ex = torch.cat([cx[:, :, 1:], bxh[:, :, None]], 2)
+    ey = torch.cat([cy[:, 1:, :], byh[:, None, :]], 1)
+    ez = torch.cat([cz[1:], bzh[None]], 0)
+    x = x.reshape(D, H, W).double()
+    b = b.reshape(-1)[:N].view(D, H, W).double()
+    ax = (cx + ex + cy + ey + cz + ez) * x
+    ax[:, :, 1:] -= cx[:, :, 1:] * x[:, :, :-1]
+    ax[:, :, :-1] -= ex[:, :, :-1] * x[:, :, 1:]
+    ax[:, 1:, :] -= cy[:, 1:, :] * x[:, :-1, :]
+    ax[:, :-1, :] -= ey[:, :-1, :] * x[:, 1:, :]
+    ax[1:] -= cz[1:] * x[:-1]
+    ax[:-1] -= ez[:-1] * x[1:]
+    return float((b - ax).norm() / b.norm())
+
+
 def true_residuals(mg, torch):
     """Per instance, ||b - A x||_2 / ||b||_2 in FP64 with A the stored FP32 face
     operator, computed here with torch array ops (independent of the solver's
@@ -158,24 +184,9 @@ def true_residuals(mg, torch):
     for inst in mg.insts:
         D, H, W = inst.shape
         N = D * H * W
+        rels.append(true_rel_residual(inst, inst.b32, inst.solution(), torch))
         f, o = inst.face32[0], inst.face_off[0]
         cx, cy, cz = (f[o[q]:o[q] + N].view(D, H, W).double() for q in range(3))
-        bxh = f[o[3]:o[3] + D * H].view(D, H).double()
-        byh = f[o[4]:o[4] + D * W].view(D, W).double()
-        bzh = f[o[5]:o[5] + H * W].view(H, W).double()
-        ex = torch.cat([cx[:, :, 1:], bxh[:, :, None]], 2)
-        ey = torch.cat([cy[:, 1:, :], byh[:, None, :]], 1)
-        ez = torch.cat([cz[1:], bzh[None]], 0)
-        x = inst.solution().double()
-        b = inst.b32.view(D, H, W).double()
-        ax = (cx + ex + cy + ey + cz + ez) * x
-        ax[:, :, 1:] -= cx[:, :, 1:] * x[:, :, :-1]
-        ax[:, :, :-1] -= ex[:, :, :-1] * x[:, :, 1:]
-        ax[:, 1:, :] -= cy[:, 1:, :] * x[:, :-1, :]
-        ax[:, :-1, :] -= ey[:, :-1, :] * x[:, 1:, :]
-        ax[1:] -= cz[1:] * x[:-1]
-        ax[:-1] -= ez[:-1] * x[1:]
-        rels.append(float((b - ax).norm() / b.norm()))
         N8 = inst.sigma.numel() // 3
         sg = [inst.sigma[q * N8:q * N8 + N].view(D, H, W).double() for q in range(3)]
         hm = lambda a, c: 2.0 * a * c / (a + c)  # noqa: E731
@@ -330,6 +341,8 @@ def urban(args, dev, torch):
                         "rel_residual": float(flow.last_info["rel"][0])}
         inst = flow.solver.insts[0]
         finals[name] = (inst.b32.clone(), inst.solution().clone())
+        out[name]["true_rel_residual_fp64"] = true_rel_residual(inst, inst.b32,
+                                                                inst.solution(), torch)
         if name == "f32":
             ref_solver = UrbanProjection(sh, sigma=sig, inflow=prof, dt=dt, storage="f32",
                                          tol=1e-12, device=dev).solver
