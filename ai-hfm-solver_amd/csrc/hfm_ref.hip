@@ -255,6 +255,13 @@ __device__ __forceinline__ float jacobi_mul_form(float x, float ax, float b, flo
   return __fadd_rn(__fsub_rn(x, __fmul_rn(c, ax)), __fmul_rn(c, b));
 }
 
+// The obstacle damping f / (1 + dt sigma) (bluff_body, UM/AI_HFM_v3.ipynb cell
+// 19): one expression for every kernel that applies it, so fused and separate
+// passes round alike.
+__device__ __forceinline__ float bluff_apply(float f, float s, float dt) {
+  return f / __builtin_fmaf(dt, s, 1.0f);
+}
+
 // ------------------------------------------------------------------ kernels
 template <int TAPS>
 __global__ __launch_bounds__(kBX* kBY) void k_conv27(const float* __restrict__ x,
@@ -593,7 +600,8 @@ __global__ __launch_bounds__(kBX* kBY) void k_advect27_v(
     const float* __restrict__ u, const float* __restrict__ v, const float* __restrict__ w,
     const float* p0, const float* p1, const float* p2, const float* b0, const float* b1,
     const float* b2, float* o0, float* o1, float* o2, Kern4 K, float f, Grid3 g,
-    const float* d0, const float* d1, const float* d2) {
+    const float* d0, const float* d1, const float* d2, const float* __restrict__ sigma, float dt,
+    int post) {
   Col c;
   if (!col_begin<1>(g, c)) return;
   const float* dis[3] = {d0, d1, d2};
@@ -606,14 +614,37 @@ __global__ __launch_bounds__(kBX* kBY) void k_advect27_v(
     const float* x = phi[q];
     win[q].start(g, c, [&](int kk, int jj, int ii) { return x[g.idx(kk, jj, ii)]; });
   }
+  // the cell's own values (velocity, bases, dissipation, sigma); with three
+  // fields loaded one plane ahead (the last plane's successor re-reads the
+  // last plane), a single field loads them at the plane (faster there)
+  struct Cells {
+    float u, v, w, b[NF], d[NF], s;
+  };
+  auto load_cells = [&](int kk, Cells& cv) {
+    const int64_t m = g.idx(min(kk, g.D - 1), c.j, c.in ? c.i : 0);
+    cv.u = u[m];
+    cv.v = v[m];
+    cv.w = w[m];
+#pragma unroll
+    for (int q = 0; q < NF; ++q) {
+      cv.b[q] = base[q][m];
+      cv.d[q] = DIS ? dis[q][m] : 0.0f;
+    }
+    cv.s = post ? sigma ? sigma[m] : 0.0f : 0.0f;
+  };
+  constexpr bool ahead = NF > 1;
+  Cells cur, nxt;
+  if (ahead) load_cells(c.k0, cur);
   for (int k = c.k0; k < c.k1; ++k) {
     const int64_t n = g.idx(k, c.j, c.in ? c.i : 0);
-    const float uc = u[n], vc = v[n], wc = w[n];
+    if (!ahead) load_cells(k, cur);
+    const float uc = cur.u, vc = cur.v, wc = cur.w;
 #pragma unroll
     for (int q = 0; q < NF; ++q) {
       const float* x = phi[q];
       win[q].advance(g, c, k, [&](int kk, int jj, int ii) { return x[g.idx(kk, jj, ii)]; });
     }
+    if (ahead) load_cells(k + 1, nxt);
     // the cell's combined weights dif - u ax - v ay - w az, shared by the
     // NF fields: half the FMAs of four stencils per field
     float cw[27];
@@ -627,15 +658,27 @@ __global__ __launch_bounds__(kBX* kBY) void k_advect27_v(
     for (int q = 0; q < NF; ++q) {
       float t[27];
       win[q].template taps<27>(t, 0);
-      const float bq = base[q][n];
+      const float bq = cur.b[q];
       // DIS: the dissipation term is given (dis0..2 in place of the phi's
       // diffusion stencil, UM/INHALE_demo.ipynb cell 24)
-      float a = DIS ? dis[q][n] : 0.0f;
+      float a = DIS ? cur.d[q] : 0.0f;
 #pragma unroll
       for (int s = 0; s < 27; ++s) a = __builtin_fmaf(cw[s], t[s], a);
-      if (c.out) out[q][n] = f * a + bq;
+      if (c.out) {
+        float o = f * a + bq;
+        if (post) {
+          // the scalar's fill (every boundary cell 0) and nbluff dampings
+          const bool face = c.i == 0 || c.i == g.W - 1 || c.j == 0 || c.j == g.H - 1 || k == 0 ||
+                            k == g.D - 1;
+          if ((post & HFM_ADV_INDICATOR) && face) o = 0.0f;
+          const int nb = (post >> HFM_ADV_BLUFF_SHIFT) & 3;
+          for (int r = 0; r < nb; ++r) o = bluff_apply(o, cur.s, dt);
+        }
+        out[q][n] = o;
+      }
       win[q].shift();
     }
+    if (ahead) cur = nxt;
   }
 }
 
@@ -679,6 +722,96 @@ __global__ __launch_bounds__(kBX* kBY) void k_conv27_sum_v(const float* x0, cons
     const float from_right = __shfl_down(acc, 1, 64);
     if (src && c.i == 0) acc = from_right;
     if (c.out) y[g.idx(k, c.j, c.i)] = scale * acc;
+  }
+}
+
+// The urban step's pressure gradient for the three velocity components in one
+// walk over p: o_q = f_q' - K_q (*) p' (the SUBTRACT form of k_conv27 per
+// component, the same taps and sums), where
+//   p' = boundary_condition_pressure(p) (HFM_GRAD_P_BC): each tap reads p at the
+//        cell the fill's last assignment copies from (0 at the outlet); pout,
+//        when given, receives p' itself;
+//   f' = bluff(f) (HFM_GRAD_PRE_BLUFF);
+// and with HFM_GRAD_POST_BLUFF o_q is damped as well, except at the cells the
+// velocity fill copies from (u at j = 1, j = H-2 and k = D-2; v at k = D-2):
+// hfm_bc_velocity_bluff_inplace_f32 gathers those undamped and damps them.
+template <int TAPS>
+__global__ __launch_bounds__(kBX* kBY) void k_grad3(const float* __restrict__ p,
+                                                    const float* f0, const float* f1,
+                                                    const float* f2, float* o0, float* o1,
+                                                    float* o2, float* __restrict__ pout, Kern3 K,
+                                                    const float* __restrict__ sigma, float dt,
+                                                    Grid3 g, int flags) {
+  Col c;
+  if (!col_begin<2>(g, c)) return;
+  const bool pbc = flags & HFM_GRAD_P_BC;
+  auto pat = [&](int kk, int jj, int ii) {
+    if (pbc) {
+      // boundary_condition_pressure: the last assignment covering the cell wins
+      // (x = W-1 -> 0, x = 0, y = 0, y = H-1, z = 0, z = D-1 copy their neighbour)
+      int k2 = kk, j2 = jj, i2 = ii;
+      bool zero = false;
+      if (kk == g.D - 1) k2 = g.D - 2;
+      else if (kk == 0) k2 = 1;
+      else if (jj == g.H - 1) j2 = g.H - 2;
+      else if (jj == 0) j2 = 1;
+      else if (ii == 0) i2 = 1;
+      else if (ii == g.W - 1) zero = true;
+      const float x = p[g.idx(k2, j2, i2)];
+      return zero ? 0.0f : x;
+    }
+    return p[g.idx(kk, jj, ii)];
+  };
+  const float* fin[3] = {f0, f1, f2};
+  float* out[3] = {o0, o1, o2};
+  const bool damp = flags & (HFM_GRAD_PRE_BLUFF | HFM_GRAD_POST_BLUFF);
+  // the cells' own values (f, sigma) of a plane, loaded one plane ahead from
+  // in-bounds addresses (a row past the grid re-reads the last row)
+  struct Cells {
+    float f[2][3], s[2];
+  };
+  const int ic = c.in ? c.i : 0;
+  auto load_cells = [&](int kk, Cells& cv) {
+    const int kc = min(kk, g.D - 1);
+#pragma unroll
+    for (int ry = 0; ry < 2; ++ry) {
+      const int64_t n = g.idx(kc, min(c.j + ry, g.H - 1), ic);
+#pragma unroll
+      for (int q = 0; q < 3; ++q) cv.f[ry][q] = fin[q][n];
+      cv.s[ry] = damp ? sigma[n] : 0.0f;
+    }
+  };
+  Win27<2> win;
+  win.start(g, c, pat);
+  Cells cur, nxt;
+  load_cells(c.k0, cur);
+  for (int k = c.k0; k < c.k1; ++k) {
+    win.advance(g, c, k, pat);
+    load_cells(k + 1, nxt);
+#pragma unroll
+    for (int ry = 0; ry < 2; ++ry) {
+      float v[27];
+      win.taps<TAPS>(v, ry);
+      const int j = c.j + ry;
+      if (c.out && j < g.H) {
+        const int64_t n = g.idx(k, j, c.i);
+        const float s = cur.s[ry];
+        const bool src_u = j == 1 || j == g.H - 2 || k == g.D - 2, src_v = k == g.D - 2;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+          const float ax = wsum<TAPS>(K.k[q], v);
+          float fv = cur.f[ry][q];
+          if (flags & HFM_GRAD_PRE_BLUFF) fv = bluff_apply(fv, s, dt);
+          float o = fv - ax;
+          const bool src = q == 0 ? src_u : q == 1 ? src_v : false;
+          if ((flags & HFM_GRAD_POST_BLUFF) && !src) o = bluff_apply(o, s, dt);
+          out[q][n] = o;
+        }
+        if (pout) pout[n] = win.centre(ry);
+      }
+    }
+    cur = nxt;
+    win.shift();
   }
 }
 
@@ -785,7 +918,9 @@ __global__ __launch_bounds__(256) void k_bc_velocity_gather(const float* __restr
 __global__ __launch_bounds__(256) void k_bc_velocity_inplace(float* u, float* v, float* w,
                                                              const float* __restrict__ scratch,
                                                              Grid3 g, float ub,
-                                                             const float* __restrict__ inflow) {
+                                                             const float* __restrict__ inflow,
+                                                             const float* __restrict__ sigma,
+                                                             float dt) {
   const int D = g.D, H = g.H, W = g.W;
   const int64_t nx = 2 * int64_t(D) * H, ny = 2 * int64_t(D) * (W - 2);
   const int64_t nz = 2 * int64_t(H - 2) * (W - 2);
@@ -837,16 +972,49 @@ __global__ __launch_bounds__(256) void k_bc_velocity_inplace(float* u, float* v,
     b = scratch[2 * DW + HW + int64_t(j) * W + i];
     c = 0.0f;
   }
+  if (sigma) {  // the bluff body after the fill
+    const float s = sigma[n];
+    a = bluff_apply(a, s, dt);
+    b = bluff_apply(b, s, dt);
+    c = bluff_apply(c, s, dt);
+  }
   u[n] = a;
   v[n] = b;
   w[n] = c;
+}
+
+// The bluff body at the cells the fill copies from that are not boundary
+// cells themselves, from their gathered (undamped) values, each once: u at
+// j = 1 and j = H-2 for 1 <= k <= D-3, then u and v on the plane k = D-2.
+__global__ __launch_bounds__(256) void k_bc_velocity_src_bluff(float* u, float* v,
+                                                               const float* __restrict__ scratch,
+                                                               const float* __restrict__ sigma,
+                                                               float dt, Grid3 g) {
+  const int D = g.D, H = g.H, W = g.W;
+  const int nrow = H - 2 > 1 ? 2 : 1;  // rows j = 1 and j = H-2 (one row when H = 3)
+  const int64_t ny = int64_t(nrow) * (D - 3) * (W - 2), nz = int64_t(H - 2) * (W - 2);
+  int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t DW = int64_t(D) * W, HW = int64_t(H) * W;
+  if (t < ny) {
+    const int r = int(t % nrow);
+    t /= nrow;
+    const int k = 1 + int(t / (W - 2)), i = 1 + int(t % (W - 2));
+    const int j = r ? H - 2 : 1;
+    const int64_t n = g.idx(k, j, i);
+    u[n] = bluff_apply(scratch[(r ? DW : 0) + int64_t(k) * W + i], sigma[n], dt);
+  } else if ((t -= ny) < nz) {
+    const int j = 1 + int(t / (W - 2)), i = 1 + int(t % (W - 2));
+    const int64_t n = g.idx(D - 2, j, i);
+    u[n] = bluff_apply(scratch[2 * DW + int64_t(j) * W + i], sigma[n], dt);
+    v[n] = bluff_apply(scratch[2 * DW + HW + int64_t(j) * W + i], sigma[n], dt);
+  }
 }
 
 __global__ __launch_bounds__(256) void k_bluff(const float* __restrict__ in, float* out,
                                                const float* __restrict__ sigma, float dt,
                                                int64_t n) {
   const int64_t q = int64_t(blockIdx.x) * 256 + threadIdx.x;
-  if (q < n) out[q] = in[q] / (1.0f + dt * sigma[q]);
+  if (q < n) out[q] = bluff_apply(in[q], sigma[q], dt);
 }
 
 }  // namespace hfm
@@ -1085,14 +1253,16 @@ extern "C" int hfm_prolong_correct_jacobi_f32(const float* x, const float* wc, c
                         W, flags, stream);
 }
 
-extern "C" int hfm_advect27_f32(const float* const vel[3], const float* const* phi,
-                                const float* const* base, float* const* out, int nf,
-                                const float K[4][27], float f, int64_t D, int64_t H, int64_t W,
-                                void* stream) {
+static int advect27(const char* what, const float* const vel[3], const float* const* phi,
+                    const float* const* base, float* const* out, int nf, const float K[4][27],
+                    float f, const float* sigma, float dt, int post, int64_t D, int64_t H,
+                    int64_t W, void* stream) {
   Kern4 kk;
   if (!vel || !vel[0] || !vel[1] || !vel[2] || !phi || !base || !out || !K ||
-      (nf != 1 && nf != 3) || !dims_ok(D, H, W)) {
-    set_error("hfm_advect27_f32: bad arguments");
+      (nf != 1 && nf != 3) || !dims_ok(D, H, W) ||
+      (post & ~(HFM_ADV_INDICATOR | (3 << HFM_ADV_BLUFF_SHIFT))) ||
+      (((post >> HFM_ADV_BLUFF_SHIFT) & 3) && !sigma)) {
+    set_error("%s: bad arguments", what);
     return HFM_EINVAL;
   }
   for (int q = 0; q < 4; ++q) make_weights(K[q], kk.k[q], kk.mask[q]);
@@ -1104,12 +1274,12 @@ extern "C" int hfm_advect27_f32(const float* const vel[3], const float* const* p
   } oo{};
   for (int q = 0; q < nf; ++q) {
     if (!phi[q] || !base[q] || !out[q]) {
-      set_error("hfm_advect27_f32: null field %d", q);
+      set_error("%s: null field %d", what, q);
       return HFM_EINVAL;
     }
     for (int r = 0; r < 3; ++r)
-      if (out[q] == vel[r] || out[q] == phi[r < nf ? r : 0]) {
-        set_error("hfm_advect27_f32: output %d aliases an input", q);
+      if (out[q] == vel[r] || out[q] == phi[r < nf ? r : 0] || (sigma && out[q] == sigma)) {
+        set_error("%s: output %d aliases an input", what, q);
         return HFM_EINVAL;
       }
     pp.a[q] = phi[q];
@@ -1121,12 +1291,30 @@ extern "C" int hfm_advect27_f32(const float* const vel[3], const float* const* p
   if (nf == 3)
     hipLaunchKernelGGL((k_advect27_v<3, false>), grid_col<1>(g), dim3(kBX, kBY), 0, s, vel[0],
                        vel[1], vel[2], pp.a[0], pp.a[1], pp.a[2], bb.a[0], bb.a[1], bb.a[2],
-                       oo.a[0], oo.a[1], oo.a[2], kk, f, g, nullptr, nullptr, nullptr);
+                       oo.a[0], oo.a[1], oo.a[2], kk, f, g, nullptr, nullptr, nullptr, sigma, dt,
+                       post);
   else
     hipLaunchKernelGGL((k_advect27_v<1, false>), grid_col<1>(g), dim3(kBX, kBY), 0, s, vel[0],
                        vel[1], vel[2], pp.a[0], pp.a[1], pp.a[2], bb.a[0], bb.a[1], bb.a[2],
-                       oo.a[0], oo.a[1], oo.a[2], kk, f, g, nullptr, nullptr, nullptr);
-  return launch_status("hfm_advect27_f32");
+                       oo.a[0], oo.a[1], oo.a[2], kk, f, g, nullptr, nullptr, nullptr, sigma, dt,
+                       post);
+  return launch_status(what);
+}
+
+extern "C" int hfm_advect27_f32(const float* const vel[3], const float* const* phi,
+                                const float* const* base, float* const* out, int nf,
+                                const float K[4][27], float f, int64_t D, int64_t H, int64_t W,
+                                void* stream) {
+  return advect27("hfm_advect27_f32", vel, phi, base, out, nf, K, f, nullptr, 0.0f, 0, D, H, W,
+                  stream);
+}
+
+extern "C" int hfm_advect27_post_f32(const float* const vel[3], const float* const* phi,
+                                     const float* const* base, float* const* out, int nf,
+                                     const float K[4][27], float f, const float* sigma, float dt,
+                                     int post, int64_t D, int64_t H, int64_t W, void* stream) {
+  return advect27("hfm_advect27_post_f32", vel, phi, base, out, nf, K, f, sigma, dt, post, D, H,
+                  W, stream);
 }
 
 extern "C" int hfm_advect27_dis_f32(const float* const vel[3], const float* const* phi,
@@ -1164,11 +1352,11 @@ extern "C" int hfm_advect27_dis_f32(const float* const vel[3], const float* cons
   if (nf == 3)
     hipLaunchKernelGGL((k_advect27_v<3, true>), grid_col<1>(g), dim3(kBX, kBY), 0, s, vel[0], vel[1],
                        vel[2], pp[0], pp[1], pp[2], bb[0], bb[1], bb[2], oo[0], oo[1], oo[2], kk, f,
-                       g, dd[0], dd[1], dd[2]);
+                       g, dd[0], dd[1], dd[2], nullptr, 0.0f, 0);
   else
     hipLaunchKernelGGL((k_advect27_v<1, true>), grid_col<1>(g), dim3(kBX, kBY), 0, s, vel[0], vel[1],
                        vel[2], pp[0], pp[1], pp[2], bb[0], bb[1], bb[2], oo[0], oo[1], oo[2], kk, f,
-                       g, dd[0], dd[1], dd[2]);
+                       g, dd[0], dd[1], dd[2], nullptr, 0.0f, 0);
   return launch_status("hfm_advect27_dis_f32");
 }
 
@@ -1258,6 +1446,50 @@ extern "C" int hfm_conv27_sum_f32(const float* const* x, float* y, int nf, const
   return launch_status("hfm_conv27_sum_f32");
 }
 
+extern "C" int hfm_grad3_f32(const float* p, const float* const f[3], float* const o[3],
+                             float* pout, const float K[3][27], const float* sigma, float dt,
+                             int flags, int64_t D, int64_t H, int64_t W, void* stream) {
+  const char* what = "hfm_grad3_f32";
+  Kern3 kk;
+  const int known = HFM_GRAD_P_BC | HFM_GRAD_PRE_BLUFF | HFM_GRAD_POST_BLUFF;
+  if (!p || !f || !o || !K || !dims_ok(D, H, W) || (flags & ~known) ||
+      ((flags & (HFM_GRAD_PRE_BLUFF | HFM_GRAD_POST_BLUFF)) && !sigma) ||
+      ((flags & HFM_GRAD_P_BC) && (D < 2 || H < 2 || W < 2)) || pout == p) {
+    set_error("%s: bad arguments", what);
+    return HFM_EINVAL;
+  }
+  for (int q = 0; q < 3; ++q) {
+    if (!f[q] || !o[q] || o[q] == p || o[q] == pout || (sigma && o[q] == sigma)) {
+      set_error("%s: null or aliased field %d", what, q);
+      return HFM_EINVAL;
+    }
+    if (!make_weights(K[q], kk.k[q], kk.mask[q])) {
+      set_error("%s: bad weights %d", what, q);
+      return HFM_EINVAL;
+    }
+  }
+  // each output may overwrite its own f (a cell reads only its own f value)
+  for (int q = 0; q < 3; ++q)
+    for (int r = 0; r < 3; ++r)
+      if (r != q && o[q] == f[r]) {
+        set_error("%s: output %d aliases field %d", what, q, r);
+        return HFM_EINVAL;
+      }
+  const int taps = taps_of(kk.mask[0]);
+  if (taps_of(kk.mask[1]) != taps || taps_of(kk.mask[2]) != taps) {
+    set_error("%s: the three kernels must share a tap set (all 7-point or all 27-point)", what);
+    return HFM_EINVAL;
+  }
+  const Grid3 g{int(D), int(H), int(W)};
+  if (taps == 7)
+    hipLaunchKernelGGL(k_grad3<7>, grid_col<2>(g), dim3(kBX, kBY), 0, as_stream(stream), p, f[0],
+                       f[1], f[2], o[0], o[1], o[2], pout, kk, sigma, dt, g, flags);
+  else
+    hipLaunchKernelGGL(k_grad3<27>, grid_col<2>(g), dim3(kBX, kBY), 0, as_stream(stream), p, f[0],
+                       f[1], f[2], o[0], o[1], o[2], pout, kk, sigma, dt, g, flags);
+  return launch_status(what);
+}
+
 extern "C" int hfm_bc_f32(const float* in, float* out, int64_t D, int64_t H, int64_t W, int kind,
                           void* stream) {
   const bool needs2 = kind == HFM_BC_TOP_COPY || kind == HFM_BC_PRESSURE || kind == HFM_BC_SOURCE;
@@ -1281,7 +1513,8 @@ static __device__ __align__(256) float scratch[HFM_BC_NOALLOC_SCRATCH_FLOATS];
 
 namespace {
 int bc_velocity_inplace(const char* what, float* u, float* v, float* w, float* scratch, int64_t D,
-                        int64_t H, int64_t W, float ub, const float* inflow, void* stream) {
+                        int64_t H, int64_t W, float ub, const float* inflow, void* stream,
+                        const float* sigma = nullptr, float dt = 0.0f) {
   const Grid3 g{int(D), int(H), int(W)};
   hipStream_t s = as_stream(stream);
   const int64_t ng = std::max(D * W, H * W);
@@ -1289,7 +1522,12 @@ int bc_velocity_inplace(const char* what, float* u, float* v, float* w, float* s
                      scratch, g);
   const int64_t nb = 2 * D * H + 2 * D * (W - 2) + 2 * (H - 2) * (W - 2);
   hipLaunchKernelGGL(k_bc_velocity_inplace, dim3(unsigned((nb + 255) / 256)), dim3(256), 0, s, u,
-                     v, w, scratch, g, ub, inflow);
+                     v, w, scratch, g, ub, inflow, sigma, dt);
+  if (sigma && D >= 3 && H >= 3 && W >= 3) {
+    const int64_t ns = (H - 2 > 1 ? 2 : 1) * (D - 3) * (W - 2) + (H - 2) * (W - 2);
+    hipLaunchKernelGGL(k_bc_velocity_src_bluff, dim3(unsigned((ns + 255) / 256)), dim3(256), 0, s,
+                       u, v, scratch, sigma, dt, g);
+  }
   return launch_status(what);
 }
 
@@ -1329,6 +1567,18 @@ extern "C" int hfm_bc_velocity_inplace_f32(float* u, float* v, float* w, float* 
   }
   return bc_velocity_inplace("hfm_bc_velocity_inplace_f32", u, v, w, scratch, D, H, W, ub, inflow,
                              stream);
+}
+
+extern "C" int hfm_bc_velocity_bluff_inplace_f32(float* u, float* v, float* w, float* scratch,
+                                                 int64_t D, int64_t H, int64_t W, float ub,
+                                                 const float* inflow, const float* sigma,
+                                                 float dt, void* stream) {
+  if (!u || !v || !w || !scratch || !sigma || !dims_ok(D, H, W) || D < 2 || H < 2 || W < 2) {
+    set_error("hfm_bc_velocity_bluff_inplace_f32: bad arguments");
+    return HFM_EINVAL;
+  }
+  return bc_velocity_inplace("hfm_bc_velocity_bluff_inplace_f32", u, v, w, scratch, D, H, W, ub,
+                             inflow, stream, sigma, dt);
 }
 
 extern "C" int hfm_bc_velocity_inplace_noalloc_f32(float* u, float* v, float* w, int64_t D,

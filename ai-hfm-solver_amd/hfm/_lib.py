@@ -75,6 +75,11 @@ SIGNATURES = {
     "hfm_prolong_correct_jacobi_f32": (_int, [_vp, _vp, _vp, _vp, _vp, _f32, _vp, _vp, _i64, _i64,
                                               _i64, _int, _int, _int, _int, _vp]),
     "hfm_advect27_f32": (_int, [_vp, _vp, _vp, _vp, _int, _vp, _f32, _i64, _i64, _i64, _vp]),
+    "hfm_advect27_post_f32": (_int, [_vp, _vp, _vp, _vp, _int, _vp, _f32, _vp, _f32, _int,
+                                     _i64, _i64, _i64, _vp]),
+    "hfm_grad3_f32": (_int, [_vp, _vp, _vp, _vp, _vp, _vp, _f32, _int, _i64, _i64, _i64, _vp]),
+    "hfm_bc_velocity_bluff_inplace_f32": (_int, [_vp, _vp, _vp, _vp, _i64, _i64, _i64, _f32, _vp,
+                                                 _vp, _f32, _vp]),
     "hfm_advect27_dis_f32": (_int, [_vp, _vp, _vp, _vp, _vp, _int, _vp, _f32, _i64, _i64, _i64,
                                     _vp]),
     "hfm_pg_coef_f32": (_int, [_vp, _vp, _vp, _vp, _f32, _f32, _i64, _i64, _i64, _vp]),
@@ -151,6 +156,8 @@ FLOW_TEAM_BYTES = 4096      # team scratch per flow (zeroed before each launch)
 FLOW_TEAM_MAX = 64
 FLOW_PROJECT_ONLY = 1
 BC_NOALLOC_SCRATCH_FLOATS = 1 << 20   # scratch of hfm_bc_velocity_inplace_noalloc_f32
+ADV_INDICATOR, ADV_BLUFF_SHIFT = 1, 4          # hfm_advect27_post_f32
+GRAD_P_BC, GRAD_PRE_BLUFF, GRAD_POST_BLUFF = 1, 2, 4   # hfm_grad3_f32
 
 _lock = threading.Lock()
 _lib = None

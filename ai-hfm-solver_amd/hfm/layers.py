@@ -576,6 +576,35 @@ def boundary_condition_velocity_(u, v, w, ub: float = 1.0, inflow=None, scratch=
     return u, v, w
 
 
+def boundary_condition_velocity_bluff_(u, v, w, sigma, dt: float, ub: float = 1.0, inflow=None,
+                                       scratch=None):
+    """``bluff_body(*boundary_condition_velocity_(u, v, w))`` in place, after
+    ``gradient3(..., post_bluff=True)`` (which leaves the cells the fill copies
+    from undamped): bitwise the plain gradient, the in-place fill and the
+    damping as three passes."""
+    for a, n in ((u, "u"), (v, "v"), (w, "w")):
+        if not (isinstance(a, torch.Tensor) and a.device.type == "cuda"
+                and a.dtype == torch.float32 and a.is_contiguous()):
+            raise ValueError(f"{n} must be a contiguous float32 GPU tensor to be filled in place")
+    (D, H, W), _ = _spatial(u)
+    sigma = _prep(sigma, "sigma")
+    if v.shape != u.shape or w.shape != u.shape or sigma.numel() != u.numel():
+        raise ValueError("u, v, w and sigma must share a shape")
+    if inflow is not None:
+        inflow = _prep(inflow, "inflow")
+        if inflow.numel() != D:
+            raise ValueError("inflow profile needs one value per height")
+    need = 2 * D * W + 2 * H * W
+    if scratch is None:
+        scratch = torch.empty(need, dtype=torch.float32, device=u.device)
+    elif scratch.numel() < need or scratch.dtype != torch.float32 or not scratch.is_contiguous():
+        raise ValueError(f"scratch needs {need} contiguous float32 values")
+    L.call("hfm_bc_velocity_bluff_inplace_f32", u.data_ptr(), v.data_ptr(), w.data_ptr(),
+           scratch.data_ptr(), D, H, W, float(ub), L.ptr(inflow), sigma.data_ptr(), float(dt),
+           L.stream_handle(u.device))
+    return u, v, w
+
+
 def bluff_body(values_u, values_v, values_w, alpha, sigma, dt: float):
     """``f / (1 + dt*sigma)`` for u, v, w and the scalar (immersed obstacle
     damping); ``alpha=None`` damps the velocities only (the INHALE demo's
@@ -596,9 +625,18 @@ def bluff_body(values_u, values_v, values_w, alpha, sigma, dt: float):
 # fused time-step kernels (UM/AI_HFM_v3.ipynb cell 25)
 
 
-def advect(vel, phis, bases, f: float, kernels):
+def advect(vel, phis, bases, f: float, kernels, indicator: bool = False, sigma=None,
+           dt: float = 0.0, nbluff: int = 0):
     """``out_q = f * (dif(phi_q) - u ax(phi_q) - v ay(phi_q) - w az(phi_q)) + base_q``
-    for 1 or 3 fields in one pass; ``kernels`` = (dif, ax, ay, az) weight arrays."""
+    for 1 or 3 fields in one pass; ``kernels`` = (dif, ax, ay, az) weight arrays.
+    ``indicator``: then ``boundary_condition_indicator``; ``nbluff`` (0-2): then
+    that many ``bluff_body`` dampings with ``sigma``, ``dt`` -- in the same pass,
+    bitwise the separate calls."""
+    post = (L.ADV_INDICATOR if indicator else 0) | (int(nbluff) << L.ADV_BLUFF_SHIFT)
+    if not 0 <= int(nbluff) <= 2:
+        raise ValueError("nbluff must be 0, 1 or 2")
+    if nbluff and sigma is None:
+        raise ValueError("damping needs sigma")
     vel = [_prep(a, "vel") for a in vel]
     phis = [_prep(a, "phi") for a in phis]
     bases = [_prep(a, "base") for a in bases]
@@ -611,11 +649,53 @@ def advect(vel, phis, bases, f: float, kernels):
     (D, H, W), _ = _spatial(vel[0])
     outs = [torch.empty_like(a) for a in phis]
     K = np.ascontiguousarray(np.stack([kernel27(k) for k in kernels]), dtype=np.float32)
+    if post:
+        sig = _prep(sigma, "sigma") if sigma is not None else None
+        if sig is not None and sig.numel() != vel[0].numel():
+            raise ValueError("sigma must match the fields")
+        L.call("hfm_advect27_post_f32", L.ptr_array([a.data_ptr() for a in vel]),
+               L.ptr_array([a.data_ptr() for a in phis]),
+               L.ptr_array([a.data_ptr() for a in bases]),
+               L.ptr_array([a.data_ptr() for a in outs]), len(phis),
+               K.ctypes.data_as(L.C.c_void_p), float(f), L.ptr(sig), float(dt), post, D, H, W,
+               L.stream_handle(vel[0].device))
+        return outs
     L.call("hfm_advect27_f32", L.ptr_array([a.data_ptr() for a in vel]),
            L.ptr_array([a.data_ptr() for a in phis]), L.ptr_array([a.data_ptr() for a in bases]),
            L.ptr_array([a.data_ptr() for a in outs]), len(phis), K.ctypes.data_as(L.C.c_void_p),
            float(f), D, H, W, L.stream_handle(vel[0].device))
     return outs
+
+
+def gradient3(p, fields, kernels, sigma=None, dt: float = 0.0, p_bc: bool = False,
+              pre_bluff: bool = False, post_bluff: bool = False, out=None):
+    """``f_q - K_q (*) p`` for the three velocity components in one walk over p
+    (the urban step's ``values_u - CNN3D_Su(values_p)``, ...).  ``p_bc`` reads p
+    through ``boundary_condition_pressure`` and also returns the filled p;
+    ``pre_bluff`` damps f first (``bluff_body`` before the gradient);
+    ``post_bluff`` damps the result except where the velocity fill copies from
+    (finish with ``boundary_condition_velocity_bluff_``).  ``out``: output
+    tensors (may be the fields themselves).  Returns (u, v, w, p_filled or None)."""
+    p = _prep(p, "p")
+    fields = [_prep(a, "f") for a in fields]
+    if len(fields) != 3 or any(f.shape != p.shape for f in fields):
+        raise ValueError("gradient3 takes p and three fields of one shape")
+    if (pre_bluff or post_bluff) and sigma is None:
+        raise ValueError("damping needs sigma")
+    sig = _prep(sigma, "sigma") if sigma is not None else None
+    if sig is not None and sig.numel() != p.numel():
+        raise ValueError("sigma must match the fields")
+    (D, H, W), _ = _spatial(p)
+    outs = [torch.empty_like(f) for f in fields] if out is None else list(out)
+    pout = torch.empty_like(p) if p_bc else None
+    flags = ((L.GRAD_P_BC if p_bc else 0) | (L.GRAD_PRE_BLUFF if pre_bluff else 0)
+             | (L.GRAD_POST_BLUFF if post_bluff else 0))
+    K = np.ascontiguousarray(np.stack([kernel27(k) for k in kernels]), dtype=np.float32)
+    L.call("hfm_grad3_f32", p.data_ptr(), L.ptr_array([f.data_ptr() for f in fields]),
+           L.ptr_array([o.data_ptr() for o in outs]), L.ptr(pout),
+           K.ctypes.data_as(L.C.c_void_p), L.ptr(sig), float(dt), flags, D, H, W,
+           L.stream_handle(p.device))
+    return outs[0], outs[1], outs[2], pout
 
 
 def advect_dissipative(vel, phis, bases, dis, f: float, kernels):

@@ -67,11 +67,15 @@ class UrbanFlow:
 
     def __init__(self, shape, sigma: Optional[torch.Tensor] = None, dt: float = 0.4,
                  dx: float = 1.0, Re: float = 1 / 20, ub: float = 1.0, multi_itr: int = 5,
-                 lscalar: bool = True, device=None, coarsest: int = 1, j_itr: int = 1):
+                 lscalar: bool = True, device=None, coarsest: int = 1, j_itr: int = 1,
+                 fused: bool = True):
         """``coarsest``: the pressure hierarchy halves every axis until the
         shortest is 1 (UM/AI_HFM_v3.ipynb:2571-2584) or 2 (the script twin's
         2^3 coarsest level, ARTs/naiver_stokes_3D.py:648-662); ``j_itr``:
-        Jacobi sweeps per level below the finest (UM/AI_HFM_v3.ipynb:780)."""
+        Jacobi sweeps per level below the finest (UM/AI_HFM_v3.ipynb:780).
+        ``fused``: the pressure fills, obstacle dampings and the scalar's fill
+        ride in the gradient and advection passes (bitwise the layer-by-layer
+        step, which ``fused=False`` runs)."""
         self.shape = tuple(int(v) for v in shape)
         if len(self.shape) != 3 or min(self.shape) < 2:
             raise ValueError("UrbanFlow needs a (D, H, W) box with every edge >= 2")
@@ -80,6 +84,8 @@ class UrbanFlow:
         k = urban_kernels(dt, dx, Re)
         self.k_mom = (k["dif"], *k["adv"])
         self.k_div = k["div"]
+        self.k_grad = k["adv"]
+        self.fused = bool(fused)
         self.grad = [Ly.FixedConv(K) for K in k["adv"]]
         if int(coarsest) not in (1, 2):
             raise ValueError("coarsest must be 1 or 2")
@@ -96,48 +102,107 @@ class UrbanFlow:
         self.sigma = sigma.reshape(self.shape).to(device=self.device, dtype=torch.float32).contiguous()
         self._last_wc = None    # the last cycle's correction one level below the finest
 
+    # the scalar is damped with the velocities (the INHALE demo damps the velocities only)
+    damp_scalar = True
+
     def _bluff(self, u, v, w, alpha):
         return Ly.bluff_body(u, v, w, alpha, self.sigma, dt=self.dt)
+
+    def _bluff_all(self, u, v, w, alpha):
+        if self.damp_scalar:
+            return self._bluff(u, v, w, alpha)
+        return (*self._bluff(u, v, w, None), alpha)
 
     def _inflow(self):
         return None
 
-    def _bcv_(self, u, v, w):
-        """The velocity fill in place, on the step's own intermediates (it writes
-        only the boundary planes; the step's inputs get the copying fill)."""
+    def _bcv(self, u, v, w):
+        return Ly.boundary_condition_velocity(u, v, w, ub=self.ub, inflow=self._inflow())
+
+    def _scratch(self, u):
         D, H, W = self.shape
         n = 2 * D * W + 2 * H * W
         if getattr(self, "_bc_scratch", None) is None or self._bc_scratch.device != u.device:
             self._bc_scratch = torch.empty(n, dtype=torch.float32, device=u.device)
+        return self._bc_scratch
+
+    def _bcv_(self, u, v, w):
+        """The velocity fill in place, on the step's own intermediates (it writes
+        only the boundary planes; the step's inputs get the copying fill)."""
         return Ly.boundary_condition_velocity_(u, v, w, ub=self.ub, inflow=self._inflow(),
-                                               scratch=self._bc_scratch)
+                                               scratch=self._scratch(u))
+
+    def _bcv_bluff_(self, u, v, w):
+        """The in-place fill and the damping, after ``gradient3(post_bluff=True)``."""
+        return Ly.boundary_condition_velocity_bluff_(u, v, w, self.sigma, self.dt, ub=self.ub,
+                                                     inflow=self._inflow(),
+                                                     scratch=self._scratch(u))
 
     def _gradient(self, u, v, w, p):
         return (self.grad[0].subtract_from(u, p), self.grad[1].subtract_from(v, p),
                 self.grad[2].subtract_from(w, p))
 
+    def _momentum(self, u, v, w):
+        """The predictor and corrector (UM/AI_HFM_v3.ipynb cell 25)."""
+        bu, bv, bw = Ly.advect((u, v, w), (u, v, w), (u, v, w), 0.5, self.k_mom)
+        bu, bv, bw = self._bcv_(bu, bv, bw)
+        return Ly.advect((bu, bv, bw), (bu, bv, bw), (u, v, w), 1.0, self.k_mom)
+
+    def _solve_pressure(self, p, b):
+        return self._pressure_cycles(p, b)
+
     def step(self, u, v, w, p, alpha):
         """One time step; returns (u, v, w, p, alpha) as new tensors."""
         shape = tuple(u.shape)
         u, v, w, p, alpha = (t.reshape(self.shape).float().contiguous() for t in (u, v, w, p, alpha))
-        u, v, w = Ly.boundary_condition_velocity(u, v, w, ub=self.ub)
+        u, v, w = self._bcv(u, v, w)
+        out = self._step_fused(u, v, w, p, alpha) if self.fused else \
+            self._step_layers(u, v, w, p, alpha)
+        return tuple(t.reshape(shape) for t in out)
+
+    def _step_layers(self, u, v, w, p, alpha):
+        """The step layer by layer, in the reference's order (after the inputs'
+        velocity fill)."""
         p = Ly.boundary_condition_pressure(p)
-        bu, bv, bw = Ly.advect((u, v, w), (u, v, w), (u, v, w), 0.5, self.k_mom)
-        bu, bv, bw = self._bcv_(bu, bv, bw)
-        u, v, w = Ly.advect((bu, bv, bw), (bu, bv, bw), (u, v, w), 1.0, self.k_mom)
+        u, v, w = self._momentum(u, v, w)
         if self.lscalar:
             (alpha,) = Ly.advect((u, v, w), (alpha,), (alpha,), 1.0, self.k_mom)
             alpha = Ly.boundary_condition_indicator(alpha)
-        u, v, w, alpha = self._bluff(u, v, w, alpha)
+        u, v, w, alpha = self._bluff_all(u, v, w, alpha)
         u, v, w = self._gradient(u, v, w, p)
         u, v, w = self._bcv_(u, v, w)
         b = Ly.conv_sum((u, v, w), self.k_div, scale=-1.0, source_bc=True)
-        p = self._pressure_cycles(p, b)
+        p = self._solve_pressure(p, b)
         p = Ly.boundary_condition_pressure(p)
         u, v, w = self._gradient(u, v, w, p)
         u, v, w = self._bcv_(u, v, w)
-        u, v, w, alpha = self._bluff(u, v, w, alpha)
-        return tuple(t.reshape(shape) for t in (u, v, w, p, alpha))
+        u, v, w, alpha = self._bluff_all(u, v, w, alpha)
+        return u, v, w, p, alpha
+
+    def _step_fused(self, u, v, w, p, alpha):
+        """The same step with the standalone passes folded into their neighbours:
+        the scalar's advection also fills its faces and applies both of the
+        step's dampings (alpha is not used between them); the first gradient
+        reads p through the pressure fill (and writes the filled p) and damps
+        the velocities before subtracting; the second fills p the same way and
+        damps its result, which the in-place velocity fill completes."""
+        u, v, w = self._momentum(u, v, w)
+        nb = 2 if self.damp_scalar else 0
+        if self.lscalar:
+            (alpha,) = Ly.advect((u, v, w), (alpha,), (alpha,), 1.0, self.k_mom, indicator=True,
+                                 sigma=self.sigma, dt=self.dt, nbluff=nb)
+        elif nb:
+            for _ in range(nb):   # the input scalar, damped with the velocities
+                (alpha,) = Ly.bluff_body(alpha, None, None, None, self.sigma, dt=self.dt)
+        u, v, w, p = Ly.gradient3(p, (u, v, w), self.k_grad, self.sigma, self.dt, p_bc=True,
+                                  pre_bluff=True, out=(u, v, w))
+        u, v, w = self._bcv_(u, v, w)
+        b = Ly.conv_sum((u, v, w), self.k_div, scale=-1.0, source_bc=True)
+        p = self._solve_pressure(p, b)
+        u, v, w, p = Ly.gradient3(p, (u, v, w), self.k_grad, self.sigma, self.dt, p_bc=True,
+                                  post_bluff=True, out=(u, v, w))
+        u, v, w = self._bcv_bluff_(u, v, w)
+        return u, v, w, p, alpha
 
     def run(self, u, v, w, p, alpha, ntime: int, writer=None, nrestart: int = 0):
         """The reference's time loop (UM/AI_HFM_v3.ipynb cell 25): steps
@@ -185,40 +250,26 @@ class InhaleFlow(UrbanFlow):
     def __init__(self, shape, sigma: Optional[torch.Tensor] = None, dt: float = 0.5,
                  dx: float = 1.0, Re: float = 1 / 20, ub: float = 1.0, multi_itr: int = 5,
                  eps_k: float = 1e-3, lscalar: bool = False, device=None, coarsest: int = 1,
-                 j_itr: int = 1):
+                 j_itr: int = 1, fused: bool = True):
         super().__init__(shape, sigma=sigma, dt=dt, dx=dx, Re=Re, ub=ub, multi_itr=multi_itr,
-                         lscalar=lscalar, device=device, coarsest=coarsest, j_itr=j_itr)
+                         lscalar=lscalar, device=device, coarsest=coarsest, j_itr=j_itr,
+                         fused=fused)
         self.eps_k = float(eps_k)
+
+    damp_scalar = False
 
     def _pg(self, u, v, w):
         return Ly.petrov_galerkin_dissipation(u, v, w, self.eps_k, self.sigma, self.dt,
                                               self.k_mom)
 
-    def step(self, u, v, w, p, alpha):
-        """One time step; returns (u, v, w, p, alpha) as new tensors."""
-        shape = tuple(u.shape)
-        u, v, w, p, alpha = (t.reshape(self.shape).float().contiguous() for t in (u, v, w, p, alpha))
+    def _momentum(self, u, v, w):
+        """Both momentum stages with the Petrov-Galerkin dissipation (cell 24)."""
         adv = self.k_mom[1:]
-        u, v, w = Ly.boundary_condition_velocity(u, v, w, ub=self.ub)
-        p = Ly.boundary_condition_pressure(p)
         kx = self._pg(u, v, w)
         bu, bv, bw = Ly.advect_dissipative((u, v, w), (u, v, w), (u, v, w), kx, 0.5, adv)
         bu, bv, bw = self._bcv_(bu, bv, bw)
         kb = self._pg(bu, bv, bw)
-        u, v, w = Ly.advect_dissipative((bu, bv, bw), (bu, bv, bw), (u, v, w), kb, 1.0, adv)
-        if self.lscalar:
-            (alpha,) = Ly.advect((u, v, w), (alpha,), (alpha,), 1.0, self.k_mom)
-            alpha = Ly.boundary_condition_indicator(alpha)
-        u, v, w = self._bluff(u, v, w, None)
-        u, v, w = self._gradient(u, v, w, p)
-        u, v, w = self._bcv_(u, v, w)
-        b = Ly.conv_sum((u, v, w), self.k_div, scale=-1.0, source_bc=True)
-        p = self._pressure_cycles(p, b)
-        p = Ly.boundary_condition_pressure(p)
-        u, v, w = self._gradient(u, v, w, p)
-        u, v, w = self._bcv_(u, v, w)
-        u, v, w = self._bluff(u, v, w, None)
-        return tuple(t.reshape(shape) for t in (u, v, w, p, alpha))
+        return Ly.advect_dissipative((bu, bv, bw), (bu, bv, bw), (u, v, w), kb, 1.0, adv)
 
     def diverged(self, limit: float = 0.5) -> bool:
         """The demo's guard: max |w_fine| of the last cycle above ``nsafe`` = 0.5
@@ -255,10 +306,11 @@ class UrbanProjection(UrbanFlow):
 
     def __init__(self, shape, sigma=None, inflow=None, dt: float = 0.4, dx: float = 1.0,
                  Re: float = 1 / 20, storage: str = "bf16", tol: float = 1e-8,
-                 lscalar: bool = True, device=None, outer=None, tune: bool = True):
+                 lscalar: bool = True, device=None, outer=None, tune: bool = True,
+                 fused: bool = True):
         from .mg import MGBatch
         super().__init__(shape, sigma=sigma, dt=dt, dx=dx, Re=Re, ub=1.0, multi_itr=0,
-                         lscalar=lscalar, device=device)
+                         lscalar=lscalar, device=device, fused=fused)
         D, H, W = self.shape
         self.inflow = None if inflow is None else torch.as_tensor(inflow, dtype=torch.float32).to(self.device)
         inv = 1.0 / dx ** 2
@@ -279,9 +331,6 @@ class UrbanProjection(UrbanFlow):
             self.tuning = _tune(self.solver)
         self.last_info = None
 
-    def _bcv(self, u, v, w):
-        return Ly.boundary_condition_velocity(u, v, w, ub=self.ub, inflow=self.inflow)
-
     def _inflow(self):
         return self.inflow
 
@@ -291,24 +340,5 @@ class UrbanProjection(UrbanFlow):
         self.last_info = self.solver.solve()
         return inst.solution().float()
 
-    def step(self, u, v, w, p, alpha):
-        shape = tuple(u.shape)
-        u, v, w, p, alpha = (t.reshape(self.shape).float().contiguous() for t in (u, v, w, p, alpha))
-        u, v, w = self._bcv(u, v, w)
-        p = Ly.boundary_condition_pressure(p)
-        bu, bv, bw = Ly.advect((u, v, w), (u, v, w), (u, v, w), 0.5, self.k_mom)
-        bu, bv, bw = self._bcv_(bu, bv, bw)
-        u, v, w = Ly.advect((bu, bv, bw), (bu, bv, bw), (u, v, w), 1.0, self.k_mom)
-        if self.lscalar:
-            (alpha,) = Ly.advect((u, v, w), (alpha,), (alpha,), 1.0, self.k_mom)
-            alpha = Ly.boundary_condition_indicator(alpha)
-        u, v, w, alpha = self._bluff(u, v, w, alpha)
-        u, v, w = self._gradient(u, v, w, p)
-        u, v, w = self._bcv_(u, v, w)
-        b = Ly.conv_sum((u, v, w), self.k_div, scale=-1.0, source_bc=True)
-        p = self.pressure(b)
-        p = Ly.boundary_condition_pressure(p)
-        u, v, w = self._gradient(u, v, w, p)
-        u, v, w = self._bcv_(u, v, w)
-        u, v, w, alpha = self._bluff(u, v, w, alpha)
-        return tuple(t.reshape(shape) for t in (u, v, w, p, alpha))
+    def _solve_pressure(self, p, b):
+        return self.pressure(b)

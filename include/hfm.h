@@ -157,6 +157,20 @@ int hfm_advect27_f32(const float* const vel[3], const float* const* phi, const f
                      float* const* out, int nf, const float K[4][27], float f, int64_t D,
                      int64_t H, int64_t W, void* stream);
 
+/* hfm_advect27_f32 with the scalar's boundary fill and obstacle damping in
+ * the same pass (the urban step's scalar transport, UM/AI_HFM_v3.ipynb cell 25:
+ * advection, boundary_condition_indicator, then bluff_body before and after
+ * the projection; alpha is not touched in between).  post: HFM_ADV_INDICATOR
+ * (every boundary cell -> 0) | nbluff << HFM_ADV_BLUFF_SHIFT (0..2 dampings
+ * out /= 1 + dt sigma).  Bitwise the separate hfm_advect27_f32, hfm_bc_f32
+ * (HFM_BC_INDICATOR) and nbluff hfm_bluff_body_f32 calls. */
+#define HFM_ADV_INDICATOR 1
+#define HFM_ADV_BLUFF_SHIFT 4
+int hfm_advect27_post_f32(const float* const vel[3], const float* const* phi,
+                          const float* const* base, float* const* out, int nf,
+                          const float K[4][27], float f, const float* sigma, float dt, int post,
+                          int64_t D, int64_t H, int64_t W, void* stream);
+
 /* As hfm_advect27_f32 with the diffusion stencil replaced by a given
  * dissipation field: out_q = f * (((dis_q - u ax(phi_q)) - v ay(phi_q)) -
  * w az(phi_q)) + base_q, K = (ax, ay, az) (UM/INHALE_demo.ipynb cell 24, the
@@ -185,6 +199,24 @@ int hfm_pg_flux_f32(const float* const vt[3], const float* const kin[3], float* 
 int hfm_conv27_sum_f32(const float* const* x, float* y, int nf, const float K[3][27], float scale,
                        int64_t D, int64_t H, int64_t W, int flags, void* stream);
 
+/* The urban step's pressure gradient, u_q -= K_q (*) p for the three velocity
+ * components (UM/AI_HFM_v3.ipynb cell 25: values_u - CNN3D_Su(values_p), ...),
+ * in one walk over p: o_q = f_q' - K_q (*) p' with
+ *   HFM_GRAD_P_BC:       p' = boundary_condition_pressure(p), read through the
+ *                        fill's cell map; pout (nullable, != p) receives p';
+ *   HFM_GRAD_PRE_BLUFF:  f' = f / (1 + dt sigma) (bluff_body before the gradient);
+ *   HFM_GRAD_POST_BLUFF: o_q /= 1 + dt sigma, except at the cells the velocity
+ *                        fill copies from (u: j = 1, j = H-2, k = D-2; v: k = D-2),
+ *                        for hfm_bc_velocity_bluff_inplace_f32 to finish.
+ * o_q may be f_q (in place); the three kernels share a tap set.  Bitwise the
+ * separate hfm_bc_f32, hfm_bluff_body_f32 and hfm_conv27_f32 (SUBTRACT) calls. */
+#define HFM_GRAD_P_BC 1
+#define HFM_GRAD_PRE_BLUFF 2
+#define HFM_GRAD_POST_BLUFF 4
+int hfm_grad3_f32(const float* p, const float* const f[3], float* const o[3], float* pout,
+                  const float K[3][27], const float* sigma, float dt, int flags, int64_t D,
+                  int64_t H, int64_t W, void* stream);
+
 /* out = in with one boundary fill applied (kind = HFM_BC_*).  Copies read
  * the input field, as the reference's tf.Variable copies do; out != in. */
 int hfm_bc_f32(const float* in, float* out, int64_t D, int64_t H, int64_t W, int kind,
@@ -202,6 +234,16 @@ int hfm_bc_velocity_f32(const float* u, const float* v, const float* w, float* u
  * pre-fill rows and plane that boundary cells copy). */
 int hfm_bc_velocity_inplace_f32(float* u, float* v, float* w, float* scratch, int64_t D, int64_t H,
                                 int64_t W, float ub, const float* inflow, void* stream);
+
+/* The in-place fill followed by bluff_body on the three fields (the end of the
+ * urban step, UM/AI_HFM_v3.ipynb cell 25), after hfm_grad3_f32 with
+ * HFM_GRAD_POST_BLUFF: boundary cells get the fill's value damped with their
+ * own sigma, and the cells the fill copies from (left undamped by the
+ * gradient) are damped from their gathered values.  Bitwise
+ * hfm_bc_velocity_inplace_f32 then hfm_bluff_body_f32 after the plain gradient. */
+int hfm_bc_velocity_bluff_inplace_f32(float* u, float* v, float* w, float* scratch, int64_t D,
+                                      int64_t H, int64_t W, float ub, const float* inflow,
+                                      const float* sigma, float dt, void* stream);
 
 /* hfm_bc_velocity_inplace_f32 without a workspace, for callers that cannot
  * allocate device memory (code under HIP graph capture, tight serving loops).

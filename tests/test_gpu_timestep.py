@@ -161,3 +161,101 @@ def test_urban_projection_pressure_solve(dev, storage):
     assert rel(x, x64) < 1e-6
     # the inflow profile sits on the x faces
     assert rel(H(state[0])[2, 5, 0], log_law_profile(sh[0])[2]) < 1e-6
+
+
+def _fields(sh, seed, dev):
+    rng = np.random.default_rng(seed)
+    u, v, w, p, a = (rng.standard_normal(sh).astype(np.float32) for _ in range(5))
+    sig = rng.uniform(0.0, 2.0, sh).astype(np.float32)
+    sig[:, 1:3, 1:3] = 1e5                                  # an obstacle
+    return [T(x, dev) for x in (u, v, w, p, a)], T(sig, dev)
+
+
+@pytest.mark.parametrize("sh", [(8, 16, 16), (3, 3, 5), (5, 3, 8), (4, 7, 70), (2, 4, 6)])
+@pytest.mark.parametrize("post", [False, True])
+@pytest.mark.parametrize("inflow", [False, True])
+def test_gradient3_equals_separate_passes_bitwise(dev, sh, post, inflow):
+    """hfm_grad3_f32: the pressure fill, the damping before the gradient (or, with
+    post, after it and the in-place velocity fill through
+    hfm_bc_velocity_bluff_inplace_f32) and the three gradients in one pass, bitwise
+    the separate layers (shapes with 2 and 3 cells per axis: the fill's source
+    rows coincide with boundary rows or with each other)."""
+    import torch
+    from hfm import layers
+    from hfm.timestep import urban_kernels
+    print(f"grad3 {sh} post={post} inflow={inflow}", flush=True)
+    Kg = urban_kernels(0.4, 1.0, 1 / 20)["adv"]
+    (u, v, w, p, _), sig = _fields(sh, 5, dev)
+    prof = T(np.linspace(0.5, 1.5, sh[0]), dev) if inflow else None
+    dt = 0.4
+    pb = layers.boundary_condition_pressure(p)
+    if post:
+        ur, vr, wr = (layers.FixedConv(K).subtract_from(f, pb) for K, f in zip(Kg, (u, v, w)))
+        layers.boundary_condition_velocity_(ur, vr, wr, ub=1.0, inflow=prof)
+        ur, vr, wr = layers.bluff_body(ur, vr, wr, None, sig, dt=dt)
+    else:
+        ub_, vb_, wb_ = layers.bluff_body(u, v, w, None, sig, dt=dt)
+        ur, vr, wr = (layers.FixedConv(K).subtract_from(f, pb) for K, f in zip(Kg, (ub_, vb_, wb_)))
+    uf, vf, wf = u.clone(), v.clone(), w.clone()
+    uo, vo, wo, pf = layers.gradient3(p, (uf, vf, wf), Kg, sig, dt, p_bc=True, pre_bluff=not post,
+                                      post_bluff=post, out=(uf, vf, wf))
+    if post:
+        layers.boundary_condition_velocity_bluff_(uo, vo, wo, sig, dt, ub=1.0, inflow=prof)
+    torch.cuda.synchronize()
+    assert torch.equal(pf, pb)
+    for got, want in zip((uo, vo, wo), (ur, vr, wr)):
+        assert torch.equal(got, want)
+
+
+@pytest.mark.parametrize("indicator,nbluff", [(True, 2), (True, 0), (False, 1)])
+def test_advect_post_equals_separate_passes_bitwise(dev, indicator, nbluff):
+    """hfm_advect27_post_f32: the scalar's advection, its face fill and the dampings
+    in one pass, bitwise the separate hfm_advect27_f32, hfm_bc_f32 and
+    hfm_bluff_body_f32 calls."""
+    import torch
+    from hfm import layers
+    from hfm.timestep import urban_kernels
+    k = urban_kernels(0.4, 1.0, 1 / 20)
+    K = (k["dif"], *k["adv"])
+    (u, v, w, _, a), sig = _fields((6, 10, 70), 6, dev)
+    (ref,) = layers.advect((u, v, w), (a,), (a,), 1.0, K)
+    if indicator:
+        ref = layers.boundary_condition_indicator(ref)
+    for _ in range(nbluff):
+        (ref,) = layers.bluff_body(ref, None, None, None, sig, dt=0.4)
+    (got,) = layers.advect((u, v, w), (a,), (a,), 1.0, K, indicator=indicator, sigma=sig, dt=0.4,
+                           nbluff=nbluff)
+    torch.cuda.synchronize()
+    assert torch.equal(got, ref)
+
+
+@pytest.mark.parametrize("driver", ["urban", "urban_noscalar", "inhale", "projection"])
+def test_fused_step_equals_layer_step_bitwise(dev, driver):
+    """Two time steps of each driver with the fused passes against the same step
+    layer by layer (the reference's order), bitwise: u, v, w, p and the scalar."""
+    import torch
+    from hfm.timestep import InhaleFlow, UrbanFlow, UrbanProjection
+    print(f"fused step {driver}", flush=True)
+    sh = (8, 16, 32)
+    (u, v, w, p, a), sig = _fields(sh, 7, dev)
+    u = u * 0.05 + 1.0
+    v, w, p = v * 0.05, w * 0.05, p * 0.05
+    a = (a > 0).float()
+    sig = torch.where(sig > 1e4, sig, torch.zeros_like(sig))
+    mk = {"urban": lambda f: UrbanFlow(sh, sigma=sig, multi_itr=2, fused=f),
+          "urban_noscalar": lambda f: UrbanFlow(sh, sigma=sig, multi_itr=2, lscalar=False,
+                                                fused=f),
+          "inhale": lambda f: InhaleFlow(sh, sigma=sig, multi_itr=2, lscalar=True, fused=f),
+          "projection": lambda f: UrbanProjection(sh, sigma=sig, inflow=np.linspace(0.5, 1.0, 8),
+                                                  dt=0.05, storage="f32", tune=False,
+                                                  fused=f)}[driver]
+    outs = []
+    for fused in (False, True):
+        flow = mk(fused)
+        g = [t.clone() for t in (u, v, w, p, a)]
+        for _ in range(2):
+            g = flow.step(*g)
+        torch.cuda.synchronize()
+        outs.append(g)
+    for x, y in zip(*outs):
+        assert torch.equal(x, y)
