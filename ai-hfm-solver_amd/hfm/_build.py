@@ -34,5 +34,23 @@ def build(verbose: bool = False) -> str:
     return LIB_PATH
 
 
+def build_asan_host(out: str, verbose: bool = False) -> str:
+    """The same library with its host code built under AddressSanitizer (the
+    device code as usual, at -O1), for the C-ABI argument-check tests: load it
+    into a process that preloads the compiler's ASan runtime."""
+    srcs = [os.path.join(CSRC, s) for s in SOURCES]
+    flags = [f for f in HIP_FLAGS if f != "-O3"] + [
+        "-O1", "-g", "-Xarch_host", "-fsanitize=address",
+        "-Xarch_host", "-fno-omit-frame-pointer"]
+    cmd = [HIPCC, *flags, f"-I{INCLUDE}", f"-I{CSRC}", "-o", out, *srcs]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError(f"hipcc failed ({res.returncode}) building {out}")
+    return out
+
+
 if __name__ == "__main__":
     print(build(verbose=True))

@@ -202,11 +202,16 @@ class _Arena:
 class Pool:
     """Device-wide solver state: per-slot FP64 scalars and dot-product partials."""
 
-    def __init__(self, capacity: int, dot_slots: int, params: Params, device=None):
+    def __init__(self, capacity: int, dot_slots: int, params: Params, device=None, alloc=None):
+        """``alloc(numel, dtype)`` returns a zeroed device tensor for every
+        buffer of the pool and its instances (default ``torch.zeros``); tests
+        pass one that surrounds each buffer with guard bands."""
         if capacity < 1 or capacity > 65535:
             raise ValueError("pool capacity must be in [1, 65535]")
         self.params = params
         self.device = torch.device(device or "cuda")
+        self.zeros = alloc if alloc is not None else \
+            (lambda n, dtype: torch.zeros(n, dtype=dtype, device=self.device))
         self.plan_only = self.device.type != "cuda"
         if not self.plan_only:
             if not torch.cuda.is_available():
@@ -218,9 +223,8 @@ class Pool:
         # server that owns the pool may turn it off: see Params.fuse_min_tiles)
         self.fuse_fine = params.fuse
         self.capacity = capacity
-        self.sc = torch.zeros(capacity * L.SCALARS_DTYPE.itemsize, dtype=torch.uint8,
-                              device=self.device)
-        self.dots = torch.zeros(max(dot_slots, 1), dtype=torch.float64, device=self.device)
+        self.sc = self.zeros(capacity * L.SCALARS_DTYPE.itemsize, torch.uint8)
+        self.dots = self.zeros(max(dot_slots, 1), torch.float64)
         self._free_slots = list(range(capacity - 1, -1, -1))
         self._dots = _Arena(max(dot_slots, 1))
 
@@ -276,7 +280,8 @@ class Instance:
         self.pool = pool
         p = pool.params
         self.shape, self.levels = check_shape(shape, p.max_coarse_edge)
-        dev, st = pool.device, TORCH_STORE[p.storage]
+        st = TORCH_STORE[p.storage]
+        zeros = pool.zeros
         self.face_off, self.face32, self.faceT = [], [], []
         self.X, self.Y, self.B = [], [], []
         for (D, H, W) in self.levels:
@@ -286,25 +291,25 @@ class Instance:
                 offs.append(tot)
                 tot += _align(sz)
             self.face_off.append(offs)
-            f32 = torch.zeros(tot, dtype=torch.float32, device=dev)
+            f32 = zeros(tot, torch.float32)
             self.face32.append(f32)
-            self.faceT.append(f32 if p.storage == "f32" else torch.zeros(tot, dtype=st, device=dev))
-            self.X.append(torch.zeros(N, dtype=st, device=dev))
-            self.Y.append(torch.zeros(N, dtype=st, device=dev))
-            self.B.append(torch.zeros(N, dtype=st, device=dev))
+            self.faceT.append(f32 if p.storage == "f32" else zeros(tot, st))
+            self.X.append(zeros(N, st))
+            self.Y.append(zeros(N, st))
+            self.B.append(zeros(N, st))
         N0 = math.prod(self.shape)
         self.N = N0
-        self.sigma = torch.zeros(3 * _align(N0), dtype=torch.float32, device=dev)
-        self.b32 = torch.zeros(N0, dtype=torch.float32, device=dev)
-        self.x64 = torch.zeros(ilv_len(N0), dtype=torch.float64, device=dev)
-        self.r64 = torch.zeros(ilv_len(N0), dtype=torch.float64, device=dev)
+        self.sigma = zeros(3 * _align(N0), torch.float32)
+        self.b32 = zeros(N0, torch.float32)
+        self.x64 = zeros(ilv_len(N0), torch.float64)
+        self.r64 = zeros(ilv_len(N0), torch.float64)
         so = TORCH_STORE[p.outer]
-        self.P = [torch.zeros(N0, dtype=so, device=dev), torch.zeros(N0, dtype=so, device=dev)]
+        self.P = [zeros(N0, so), zeros(N0, so)]
         # z = V(rlow) has a buffer of its own (type outer): the fine level's last
         # post-sweep writes it whether or not its pass pair runs fused
-        self.Z = torch.zeros(N0, dtype=so, device=dev)
+        self.Z = zeros(N0, so)
         nc = math.prod(self.levels[-1])
-        self.ainv = torch.zeros(nc * nc, dtype=torch.float64, device=dev)
+        self.ainv = zeros(nc * nc, torch.float64)
         self.nblk = fine_blocks(self.shape)
         self.slot, self.dots_off = pool.acquire(self.nblk)
         self.released = False
@@ -942,14 +947,15 @@ class MGBatch:
                  outer: Optional[str] = None, blocks: Optional[dict] = None,
                  coarse_cells: int = 4096, fuse: bool = True, fuse_levels: int = 1,
                  fuse_min_tiles: int = 256, streams: int = 2, method: str = "fcg",
-                 tile_planes: int = 32):
+                 tile_planes: int = 32, alloc=None):
         self.params = Params(storage, tol, maxit, nu1, nu2, omega, alpha, max_coarse_edge, outer,
                              blocks, coarse_cells, fuse, fuse_levels, fuse_min_tiles, method,
                              tile_planes)
         shapes = [check_shape(s, max_coarse_edge)[0] for s in shapes]
         if not shapes:
             raise ValueError("empty batch")
-        self.pool = Pool(len(shapes), sum(fine_blocks(s) for s in shapes), self.params, device)
+        self.pool = Pool(len(shapes), sum(fine_blocks(s) for s in shapes), self.params, device,
+                         alloc=alloc)
         self.pool.fuse_fine = self._fuse_fine(shapes)
         self.device = self.pool.device
         self.plan_only = self.pool.plan_only

@@ -563,3 +563,24 @@ def test_persistent_program_schedule(fuse):
     for G in (8, 16, 64, 256):
         cnt = _simulate_persistent(prog, G)
         assert np.array_equal(cnt, need)
+
+
+def test_guard_bands_report_a_write_past_either_end():
+    """The guard-band allocator of the GPU bounds tests (on CPU tensors): a byte
+    written just before or just after a buffer is reported, an untouched one
+    is not, and the buffer itself comes back zeroed."""
+    import torch
+    from test_gpu_guard import GUARD, GuardedAlloc
+    alloc = GuardedAlloc("cpu")
+    a = alloc(100, torch.float32)
+    b = alloc(33, torch.bfloat16)
+    assert bool((a == 0).all()) and bool((b == 0).all())
+    a.fill_(7.0)
+    b.fill_(1.0)
+    assert alloc.damaged() == []
+    raw, nb = alloc.bufs[1]
+    raw[GUARD + nb] = 0
+    assert alloc.damaged() == [1]
+    raw[GUARD + nb] = 0xA5
+    alloc.bufs[0][0][GUARD - 1] = 1
+    assert alloc.damaged() == [0]
