@@ -27,6 +27,7 @@ from __future__ import annotations
 
 import dataclasses
 import math
+import time
 from typing import List, Optional, Sequence, Tuple
 
 import numpy as np
@@ -271,6 +272,20 @@ class Pool:
         lo, hi = slots if slots is not None else (0, self.capacity)
         L.call("hfm_fcg_finalize", self.sc.data_ptr() + lo * L.SCALARS_DTYPE.itemsize, hi - lo,
                self.dots.data_ptr(), L.FIN[kind], s)
+
+
+def iteration_bytes(inst, params: Params) -> float:
+    """Algorithmic bytes of one FCG iteration of ``inst``: on every level but the
+    coarsest, nu1 + nu2 sweeps (x, b, 3 faces read, x written) and the residual
+    restriction (x, b, 3 faces read, the coarse rhs written); on the fine level
+    the direction pass (z, p read, 3 FP32 faces, p written) and the update (p,
+    3 FP32 faces, r64 read and written, rlow and the first sweep written; x64
+    and the previous direction on every other iteration: their mean)."""
+    e, eo = ELEM[params.storage], ELEM[params.outer]
+    nu = params.nu1 + params.nu2
+    tot = sum(D * H * W * (nu * 6 * e + 5 * e + e / 8) for (D, H, W) in inst.levels[:-1])
+    upd = 2 * e + eo + 12 + 16 + 0.5 * (16 + eo)
+    return tot + math.prod(inst.levels[0]) * (3 * eo + 12 + upd)
 
 
 class Instance:
@@ -1141,7 +1156,7 @@ class MGBatch:
         self.last_info = info
         return info
 
-    def solve(self, persistent: bool = False):
+    def solve(self, persistent: bool = False, log=None):
         """Run FCG to ``tol`` (or ``maxit``) for every instance; returns the scalar records.
 
         Iterations run ``iters_per_graph`` at a time (one HIP-graph replay, or
@@ -1149,13 +1164,31 @@ class MGBatch:
         those iterations); between them the active flags are read and, once at
         most ``shrink`` of the plan's instances are still iterating, the
         launches continue on a plan of the active ones only (no empty blocks).
-        Both modes give bitwise the same results.
+        Both modes give bitwise the same results.  ``log``: a
+        ``hfm.solvelog.SolveLog`` that receives one JSON line per instance.
         """
         self._need_gpu()
+        t0 = time.perf_counter()
         if persistent:
-            return self._solve_persistent()
-        if self.streams == 2 and self._groups is not None:
-            return self._solve_groups()
+            info = self._solve_persistent()
+        elif self.streams == 2 and self._groups is not None:
+            info = self._solve_groups()
+        else:
+            info = self._solve_plan()
+        if log is not None:
+            from .solvelog import solve_record
+            wall = time.perf_counter() - t0
+            for q, inst in enumerate(self.insts):
+                log.write(solve_record(inst, self.params, info[q], wall, self.ninst,
+                                       self.iteration_bytes(q),
+                                       mode="persistent" if persistent else "graphs"))
+        return info
+
+    def iteration_bytes(self, i: int) -> float:
+        """Algorithmic bytes of one iteration of instance ``i`` (iteration_bytes)."""
+        return iteration_bytes(self.insts[i], self.params)
+
+    def _solve_plan(self):
         s = L.stream_handle(self.device)
         self.restart()
         act = self.pool.active_view()

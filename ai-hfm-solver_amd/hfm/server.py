@@ -35,7 +35,7 @@ import numpy as np
 import torch
 
 from . import _lib as L
-from .mg import Instance, Params, Plan, Pool, SetupPlan, check_shape
+from .mg import Instance, Params, Plan, Pool, SetupPlan, check_shape, iteration_bytes
 
 
 @dataclasses.dataclass
@@ -74,8 +74,11 @@ class SolveServer:
 
     def __init__(self, params: Optional[Params] = None, hbm_budget: float = 200e9,
                  capacity: int = 4096, dot_slots: int = 1 << 22, iters_per_epoch: int = 4,
-                 device=None, clock=time.perf_counter):
+                 device=None, clock=time.perf_counter, log=None):
+        """``log``: a ``hfm.solvelog.SolveLog`` receiving one JSON line per
+        finished request (its times, iterations, status and residual)."""
         self.params = params or Params()
+        self.log = log
         self.pool = Pool(capacity, dot_slots, self.params, device)
         # fused fine-level pass pairs only for a server that batches: with a
         # few residents their tiles leave most CUs idle
@@ -170,6 +173,13 @@ class SolveServer:
             status = L.STATUS.get(int(rec["status"]), "unknown")
             done.append(Result(rid, req.shape, status, int(rec["iter"]), float(rec["rel"]),
                                inst.solution(), req.t_arrival, t_adm, now))
+            if self.log is not None:
+                from .solvelog import solve_record
+                self.log.write(solve_record(
+                    inst, self.params, rec, now - t_adm, len(members),
+                    iteration_bytes(inst, self.params), rid=rid,
+                    t_arrival=req.t_arrival, t_admitted=t_adm, t_done=now,
+                    queue_s=t_adm - req.t_arrival))
             inst.release()
             self.used -= need
             del self.resident[rid]

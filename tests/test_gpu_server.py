@@ -103,3 +103,35 @@ def test_failed_request_fails_alone(dev):
         assert by[rid].status == "converged"
         assert by[rid].iters == alone[rid][0]
         assert bool((by[rid].x == alone[rid][1]).all())
+
+
+def test_solve_log_has_one_line_per_request_and_instance(dev):
+    """The per-solve JSON lines of a server and of a batch: one per finished
+    request (with its id and times) or instance, matching the results."""
+    import io
+    import json
+    from hfm.mg import MGBatch, Params
+    from hfm.server import SolveServer
+    from hfm.solvelog import SolveLog
+    buf = io.StringIO()
+    srv = SolveServer(Params(storage="bf16", tol=1e-8), hbm_budget=8e9, capacity=16,
+                      device=dev, log=SolveLog(buf))
+    for rid in (0, 2, 4):
+        srv.submit(rid, *_fields(dev, *REQS[rid][:2], *REQS[rid][2:]))
+    res = {r.rid: r for r in srv.drain()}
+    recs = [json.loads(x) for x in buf.getvalue().splitlines()]
+    assert sorted(r["rid"] for r in recs) == [0, 2, 4]
+    for r in recs:
+        assert r["iters"] == res[r["rid"]].iters and r["status"] == "converged"
+        assert r["shape"] == list(REQS[r["rid"]][0]) and r["alg_bytes"] > 0
+        assert r["t_done"] >= r["t_admitted"] >= r["t_arrival"]
+    buf = io.StringIO()
+    mg = MGBatch([REQS[1][0], REQS[3][0]], storage="bf16", device=dev)
+    for q, rid in enumerate((1, 3)):
+        sx, sy, sz, b = O.make_anisotropic_fields(*REQS[rid][:2], *REQS[rid][2:])
+        mg.set_problem(q, sx, sy, sz, b)
+    mg.setup()
+    info = mg.solve(log=SolveLog(buf))
+    recs = [json.loads(x) for x in buf.getvalue().splitlines()]
+    assert [r["iters"] for r in recs] == [int(v) for v in info["iter"]]
+    assert all(r["mode"] == "graphs" and r["batch_instances"] == 2 for r in recs)
