@@ -39,6 +39,45 @@ __device__ __forceinline__ uint32_t pack_bf16x2(float a, float b) {
   typedef __bf16 h2 __attribute__((ext_vector_type(2)));
   return __builtin_bit_cast(uint32_t, __builtin_convertvector(f2{a, b}, h2));
 }
+// IEEE half storage (FP16): conversions round to nearest even; values past
+// the range become +-inf, below it subnormal or zero.
+struct f16 {
+  uint16_t v;
+};
+__device__ __forceinline__ float to_f(f16 v) { return float(__builtin_bit_cast(_Float16, v.v)); }
+template <>
+__device__ __forceinline__ f16 from_f<f16>(float f) {
+  return f16{__builtin_bit_cast(uint16_t, (_Float16)f)};
+}
+// Two values rounded to FP16 and packed (low half = a).
+__device__ __forceinline__ uint32_t pack_f16x2(float a, float b) {
+  typedef float f2 __attribute__((ext_vector_type(2)));
+  typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector(f2{a, b}, h2));
+}
+__device__ __forceinline__ float f16_lo(uint32_t w) {
+  return float(__builtin_bit_cast(_Float16, uint16_t(w & 0xFFFFu)));
+}
+__device__ __forceinline__ float f16_hi(uint32_t w) {
+  return float(__builtin_bit_cast(_Float16, uint16_t(w >> 16)));
+}
+
+template <typename T>
+struct is_f16 {
+  static constexpr bool value = false;
+};
+template <>
+struct is_f16<f16> {
+  static constexpr bool value = true;
+};
+// The fine residual as the V-cycle gets it: FP16 storage scales it by 2^e
+// (exact), the other types take it as is.
+template <typename T>
+__device__ __forceinline__ float rscale(float v, int e) {
+  if constexpr (is_f16<T>::value) return __builtin_ldexpf(v, e);
+  return v;
+}
+
 // Value after a round trip through storage type T.
 template <typename T>
 __device__ __forceinline__ float round_to(float f) {
@@ -68,6 +107,14 @@ __device__ __forceinline__ void put(HFM_GLOBAL float* p, bf16 v) { *p = to_f(v);
 __device__ __forceinline__ void put(HFM_GLOBAL bf16* p, bf16 v) { *(HFM_GLOBAL uint16_t*)p = v.v; }
 __device__ __forceinline__ void put(HFM_GLOBAL bf16* p, float v) {
   *(HFM_GLOBAL uint16_t*)p = from_f<bf16>(v).v;
+}
+__device__ __forceinline__ float ldg(const HFM_GLOBAL f16* p) {
+  return to_f(f16{*(const HFM_GLOBAL uint16_t*)p});
+}
+__device__ __forceinline__ void put(HFM_GLOBAL float* p, f16 v) { *p = to_f(v); }
+__device__ __forceinline__ void put(HFM_GLOBAL f16* p, f16 v) { *(HFM_GLOBAL uint16_t*)p = v.v; }
+__device__ __forceinline__ void put(HFM_GLOBAL f16* p, float v) {
+  *(HFM_GLOBAL uint16_t*)p = from_f<f16>(v).v;
 }
 
 // ---------------------------------------------------------------- division

@@ -94,6 +94,29 @@ __device__ __forceinline__ void st4(HFM_GLOBAL float* p, const float v[4]) {
 __device__ __forceinline__ void st4(HFM_GLOBAL bf16* p, const float v[4]) {
   *(HFM_GLOBAL u2v*)p = u2v{pack2(v[0], v[1]), pack2(v[2], v[3])};
 }
+// FP16 storage: the same accesses, converted by the hardware's f16 <-> f32 moves
+__device__ __forceinline__ uint32_t pack2h(float a, float b) { return pack_f16x2(a, b); }
+__device__ __forceinline__ void ld8(const HFM_GLOBAL f16* p, float v[8]) {
+  const u4v u = *(const HFM_GLOBAL u4v*)p;
+  const uint32_t w[4] = {u.x, u.y, u.z, u.w};
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    v[2 * q] = f16_lo(w[q]);
+    v[2 * q + 1] = f16_hi(w[q]);
+  }
+}
+__device__ __forceinline__ void ld4(const HFM_GLOBAL f16* p, float v[4]) {
+  const u2v u = *(const HFM_GLOBAL u2v*)p;
+  v[0] = f16_lo(u.x); v[1] = f16_hi(u.x);
+  v[2] = f16_lo(u.y); v[3] = f16_hi(u.y);
+}
+__device__ __forceinline__ void st8(HFM_GLOBAL f16* p, const float v[8]) {
+  *(HFM_GLOBAL u4v*)p = u4v{pack2h(v[0], v[1]), pack2h(v[2], v[3]), pack2h(v[4], v[5]),
+                            pack2h(v[6], v[7])};
+}
+__device__ __forceinline__ void st4(HFM_GLOBAL f16* p, const float v[4]) {
+  *(HFM_GLOBAL u2v*)p = u2v{pack2h(v[0], v[1]), pack2h(v[2], v[3])};
+}
 // The run at n0 of a lane-interleaved FP64 vector (see kIlvSeg).  These
 // vectors are read once per launch: non-temporal loads keep them from
 // displacing the stencil rows other lanes still re-read from the caches
@@ -138,6 +161,20 @@ __device__ __forceinline__ void st8_nt(HFM_GLOBAL float* p, const float v[8]) {
 __device__ __forceinline__ void st8_nt(HFM_GLOBAL bf16* p, const float v[8]) {
   __builtin_nontemporal_store(u4v{pack2(v[0], v[1]), pack2(v[2], v[3]), pack2(v[4], v[5]),
                                   pack2(v[6], v[7])},
+                              (HFM_GLOBAL u4v*)p);
+}
+__device__ __forceinline__ void ld8_nt(const HFM_GLOBAL f16* p, float v[8]) {
+  const u4v u = __builtin_nontemporal_load((const HFM_GLOBAL u4v*)p);
+  const uint32_t w[4] = {u.x, u.y, u.z, u.w};
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    v[2 * q] = f16_lo(w[q]);
+    v[2 * q + 1] = f16_hi(w[q]);
+  }
+}
+__device__ __forceinline__ void st8_nt(HFM_GLOBAL f16* p, const float v[8]) {
+  __builtin_nontemporal_store(u4v{pack2h(v[0], v[1]), pack2h(v[2], v[3]), pack2h(v[4], v[5]),
+                                  pack2h(v[6], v[7])},
                               (HFM_GLOBAL u4v*)p);
 }
 
@@ -658,6 +695,27 @@ struct Raw4<bf16> {
     __builtin_nontemporal_store(u, (HFM_GLOBAL u2v*)p);
   }
 };
+template <>
+struct Raw4<f16> {
+  u2v u;
+  __device__ __forceinline__ void load(const HFM_GLOBAL f16* p) { u = *(const HFM_GLOBAL u2v*)p; }
+  __device__ __forceinline__ void load_nt(const HFM_GLOBAL f16* p) {
+    u = __builtin_nontemporal_load((const HFM_GLOBAL u2v*)p);
+  }
+  __device__ __forceinline__ void lds_load(const f16* p) { u = *(const u2v*)p; }
+  __device__ __forceinline__ void lds_store(f16* p) const { *(u2v*)p = u; }
+  __device__ __forceinline__ void get(float v[4]) const {
+    v[0] = f16_lo(u.x); v[1] = f16_hi(u.x);
+    v[2] = f16_lo(u.y); v[3] = f16_hi(u.y);
+  }
+  __device__ __forceinline__ void set(const float v[4]) {  // rounded to FP16 here
+    u = u2v{pack2h(v[0], v[1]), pack2h(v[2], v[3])};
+  }
+  __device__ __forceinline__ void zero() { u = u2v{0u, 0u}; }
+  __device__ __forceinline__ void store_nt(HFM_GLOBAL f16* p) const {
+    __builtin_nontemporal_store(u, (HFM_GLOBAL u2v*)p);
+  }
+};
 __device__ __forceinline__ void lds_ld4(const float* p, float v[4]) {
   const f4v a = *(const f4v*)p;
   v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
@@ -672,6 +730,12 @@ __device__ __forceinline__ void lds_ld4(const bf16* p, float v[4]) {
 }
 __device__ __forceinline__ float lds_ld1(const float* p) { return *p; }
 __device__ __forceinline__ float lds_ld1(const bf16* p) { return to_f(*p); }
+__device__ __forceinline__ void lds_ld4(const f16* p, float v[4]) {
+  Raw4<f16> r;
+  r.lds_load(p);
+  r.get(v);
+}
+__device__ __forceinline__ float lds_ld1(const f16* p) { return to_f(*p); }
 __device__ __forceinline__ void ld2(const HFM_GLOBAL float* p, float v[2]) {
   typedef float f2v __attribute__((ext_vector_type(2)));
   const f2v a = *(const HFM_GLOBAL f2v*)p;
@@ -689,11 +753,22 @@ __device__ __forceinline__ void st2(HFM_GLOBAL float* p, const float v[2]) {
 __device__ __forceinline__ void st2(HFM_GLOBAL bf16* p, const float v[2]) {
   *(HFM_GLOBAL uint32_t*)p = pack2(v[0], v[1]);
 }
+__device__ __forceinline__ void ld2(const HFM_GLOBAL f16* p, float v[2]) {
+  const uint32_t u = *(const HFM_GLOBAL uint32_t*)p;
+  v[0] = f16_lo(u);
+  v[1] = f16_hi(u);
+}
+__device__ __forceinline__ void st2(HFM_GLOBAL f16* p, const float v[2]) {
+  *(HFM_GLOBAL uint32_t*)p = pack2h(v[0], v[1]);
+}
 __device__ __forceinline__ void st4_nt(HFM_GLOBAL float* p, const float v[4]) {
   __builtin_nontemporal_store(f4v{v[0], v[1], v[2], v[3]}, (HFM_GLOBAL f4v*)p);
 }
 __device__ __forceinline__ void st4_nt(HFM_GLOBAL bf16* p, const float v[4]) {
   __builtin_nontemporal_store(u2v{pack2(v[0], v[1]), pack2(v[2], v[3])}, (HFM_GLOBAL u2v*)p);
+}
+__device__ __forceinline__ void st4_nt(HFM_GLOBAL f16* p, const float v[4]) {
+  __builtin_nontemporal_store(u2v{pack2h(v[0], v[1]), pack2h(v[2], v[3])}, (HFM_GLOBAL u2v*)p);
 }
 
 // A run's pass-1 input as loaded: x, or x and the coarse correction below it
@@ -1244,7 +1319,7 @@ __global__ __launch_bounds__(kBlock) void k_fcg_init(const hfm_mg_task* __restri
     const int n64 = ilv ? ilv_cell(n) : n;
     x64[n64] = 0.0;
     r64[n64] = bv;
-    const T rt = from_f<T>(float(bv));
+    const T rt = from_f<T>(rscale<T>(float(bv), S.sexp));
     put(rlow + n, rt);
     acc += bv * bv;
     if (SWEEP) {
@@ -1383,6 +1458,7 @@ __device__ __forceinline__ void fcg_update_body(const hfm_mg_task* __restrict__ 
   const double alpha = S.alpha;
   const bool lag = S.xlag != 0;
   const double alpha_prev = S.alpha_prev;
+  const int sexp = S.sexp;  // the fine residual's scale (FP16 storage)
   const SrcPlain<P> src{gin<P>(t.xin)};
   const SrcPlain<P> prev{gin<P>(t.ec)};
   HFM_GLOBAL T* rlow = gout<T>(t.rlow);
@@ -1411,7 +1487,7 @@ __device__ __forceinline__ void fcg_update_body(const hfm_mg_task* __restrict__ 
 #pragma unroll
       for (int m = 0; m < 8; ++m) {
         rv[m] = rv[m] - alpha * (d[m] * p[m] - off[m]);
-        rl[m] = float(rv[m]);
+        rl[m] = rscale<T>(float(rv[m]), sexp);
         acc += rv[m] * rv[m];
       }
       st8d_ilv(r64, n0, rv);
@@ -1446,7 +1522,7 @@ __device__ __forceinline__ void fcg_update_body(const hfm_mg_task* __restrict__ 
       if (lag) x64[n] = fma(alpha, pn, fma(alpha_prev, double(prev.one(n, 0, 0, 0)), x64[n]));
       const double rr = r64[n] - alpha * q;
       r64[n] = rr;
-      const T rt = from_f<T>(float(rr));
+      const T rt = from_f<T>(rscale<T>(float(rr), sexp));
       put(rlow + n, rt);
       acc += rr * rr;
       if (SWEEP) put(x0 + n, from_f<T>(wdiv(omega, to_f(rt), diag_rounded<T>(t, n, i, j, k))));
@@ -1474,6 +1550,7 @@ __global__ __launch_bounds__(kBlock) void k_fcg_update(const hfm_mg_task* __rest
   const double alpha = S.alpha;
   const bool lag = S.xlag != 0;
   const double alpha_prev = S.alpha_prev;
+  const int sexp = S.sexp;  // the fine residual's scale (FP16 storage)
   const SrcPlain<P> src{gin<P>(t.xin)};
   const SrcPlain<P> prev{gin<P>(t.ec)};
   HFM_GLOBAL T* rlow = gout<T>(t.rlow);
@@ -1502,7 +1579,7 @@ __global__ __launch_bounds__(kBlock) void k_fcg_update(const hfm_mg_task* __rest
 #pragma unroll
       for (int m = 0; m < 8; ++m) {
         rv[m] = rv[m] - alpha * (d[m] * p[m] - off[m]);
-        rl[m] = float(rv[m]);
+        rl[m] = rscale<T>(float(rv[m]), sexp);
         acc += rv[m] * rv[m];
       }
       st8d_ilv(r64, n0, rv);
@@ -1537,7 +1614,7 @@ __global__ __launch_bounds__(kBlock) void k_fcg_update(const hfm_mg_task* __rest
       if (lag) x64[n] = fma(alpha, pn, fma(alpha_prev, double(prev.one(n, 0, 0, 0)), x64[n]));
       const double rr = r64[n] - alpha * q;
       r64[n] = rr;
-      const T rt = from_f<T>(float(rr));
+      const T rt = from_f<T>(rscale<T>(float(rr), sexp));
       put(rlow + n, rt);
       acc += rr * rr;
       if (SWEEP) put(x0 + n, from_f<T>(wdiv(omega, to_f(rt), diag_rounded<T>(t, n, i, j, k))));
@@ -1595,6 +1672,14 @@ __global__ __launch_bounds__(kBlock) void k_fcg_flush(const hfm_mg_task* __restr
 }
 
 // ------------------------------------------------------------------ per-instance scalars
+// After a residual of squared norm rr has been written (INIT, CHECK): the z
+// to come was made with 2^sexp; the next residual gets 2^(sbase - ilogb ||r||)
+// (kept when rr is zero or not finite).
+__device__ __forceinline__ void rescale(hfm_mg_scalars& S, double rr) {
+  S.zexp = S.sexp;
+  if (S.scaled && rr > 0.0 && isfinite(rr)) S.sexp = S.sbase - ilogb(sqrt(rr));
+}
+
 template <bool PB = false>
 __device__ __forceinline__ void fcg_finalize_body(hfm_mg_scalars* sc, const double* dots,
                                                   int kind, const Blk& bk) {
@@ -1626,6 +1711,7 @@ __device__ __forceinline__ void fcg_finalize_body(hfm_mg_scalars* sc, const doub
       S.beta = 0.0;
       S.xlag = 0;
       S.alpha_prev = 0.0;
+      rescale(S, s);
       if (s == 0.0) {  // x = 0 solves b = 0
         S.rel = 0.0;
         S.status = HFM_MG_CONVERGED;
@@ -1635,15 +1721,20 @@ __device__ __forceinline__ void fcg_finalize_body(hfm_mg_scalars* sc, const doub
         S.active = 0;
       }
       break;
-    case HFM_FIN_BETA:
-      S.beta = (S.iter == 0 || S.method == HFM_METHOD_REFINE) ? 0.0 : s / S.rz;
-      S.rz = s;
+    case HFM_FIN_BETA: {
+      // r.z of the rescaled z (2^zexp z from 2^zexp r): divided by 2^zexp it
+      // is the r.z of z scaled by 2^zexp once, so that beta, alpha and p all
+      // follow the rescaled z (exact: a power of 2)
+      const double rz = S.zexp ? ldexp(s, -S.zexp) : s;
+      S.beta = (S.iter == 0 || S.method == HFM_METHOD_REFINE) ? 0.0 : rz / S.rz;
+      S.rz = rz;
       if (!isfinite(s)) {
         S.status = HFM_MG_BREAKDOWN;
         S.active = 0;
         if (S.xlag) S.xlag = 2;  // pending p is now the update's previous direction
       }
       break;
+    }
     case HFM_FIN_ALPHA:
       S.pap = s;
       if (S.method == HFM_METHOD_REFINE ? !isfinite(s) : (!(s > 0.0) || !isfinite(s))) {
@@ -1651,13 +1742,15 @@ __device__ __forceinline__ void fcg_finalize_body(hfm_mg_scalars* sc, const doub
         S.active = 0;
         if (S.xlag) S.xlag = 2;
       } else {
-        S.alpha = S.method == HFM_METHOD_REFINE ? 1.0 : S.rz / s;
+        S.alpha = S.method == HFM_METHOD_REFINE ? (S.zexp ? ldexp(1.0, -S.zexp) : 1.0)
+                                                : S.rz / s;
       }
       break;
     case HFM_FIN_CHECK:
       S.rr = s;
       S.rel = sqrt(s / S.bnorm2);
       S.iter += 1;
+      rescale(S, s);
       // the update just run applied x's pending half and its own (xlag 1)
       // or left its own pending (xlag 0)
       S.xlag = S.xlag ? 0 : 1;
@@ -2161,6 +2254,8 @@ int launch_store(int store, const char* what, F&& f) {
     f(float{});
   } else if (store == HFM_STORE_BF16) {
     f(bf16{});
+  } else if (store == HFM_STORE_F16) {
+    f(f16{});
   } else {
     set_error("%s: unknown storage type %d", what, store);
     return HFM_EINVAL;
@@ -2168,18 +2263,29 @@ int launch_store(int store, const char* what, F&& f) {
   return launch_status(what);
 }
 
-// Two storage types (inner, outer): f(T{}, O{}).
+// Two storage types (inner, outer): f(T{}, O{}).  The outer type is FP32 or
+// the inner 16-bit type's kind (BF16 with FP32 or BF16, FP16 with FP16).
 template <typename F>
 int launch_store2(int store, int outer, const char* what, F&& f) {
-  if (outer != HFM_STORE_F32 && outer != HFM_STORE_BF16) {
-    set_error("%s: unknown storage type %d", what, outer);
+  const bool ok = outer == HFM_STORE_F32 ||
+                  (outer == HFM_STORE_BF16 && store != HFM_STORE_F16) ||
+                  (outer == HFM_STORE_F16 && store == HFM_STORE_F16);
+  if (!ok) {
+    if (outer < HFM_STORE_F32 || outer > HFM_STORE_F16)
+      set_error("%s: unknown storage type %d", what, outer);
+    else
+      set_error("%s: storage %d with outer storage %d is not built", what, store, outer);
     return HFM_EINVAL;
   }
   return launch_store(store, what, [&](auto tag) {
-    if (outer == HFM_STORE_F32)
+    using T = decltype(tag);
+    if (outer == HFM_STORE_F32) {
       f(tag, float{});
-    else
+    } else if constexpr (is_f16<T>::value) {
+      f(tag, f16{});
+    } else {
       f(tag, bf16{});
+    }
   });
 }
 
@@ -2584,6 +2690,8 @@ int persistent_fn(int store, int pstore, const void** fn) {
   else if (store == HFM_STORE_BF16 && pstore == HFM_STORE_BF16) *fn = persistent_kernel<bf16, bf16>();
   else if (store == HFM_STORE_BF16 && pstore == HFM_STORE_F32) *fn = persistent_kernel<bf16, float>();
   else if (store == HFM_STORE_F32 && pstore == HFM_STORE_BF16) *fn = persistent_kernel<float, bf16>();
+  else if (store == HFM_STORE_F16 && pstore == HFM_STORE_F16) *fn = persistent_kernel<f16, f16>();
+  else if (store == HFM_STORE_F16 && pstore == HFM_STORE_F32) *fn = persistent_kernel<f16, float>();
   else return HFM_EINVAL;
   return HFM_OK;
 }

@@ -21,7 +21,7 @@ from ._build import LIB_PATH
 
 HFM_OK = 0
 
-STORE = {"f32": 0, "bf16": 1}
+STORE = {"f32": 0, "bf16": 1, "f16": 2}
 FORM = {"apply": 0, "resid": 1, "jacobi": 2, "subtract": 3, "axpy": 4, "jacobi_mul": 5}
 FORM_ZERO_OUTLET = 16
 CORRECT_ZERO_OUTLET = 1
@@ -50,6 +50,7 @@ SCALARS_DTYPE = np.dtype([
     ("iter", "<i4"), ("maxit", "<i4"), ("active", "<i4"), ("status", "<i4"),
     ("slot0", "<i4"), ("nslot", "<i4"), ("xlag", "<i4"), ("method", "<i4"),
     ("alpha_prev", "<f8"),
+    ("sexp", "<i4"), ("zexp", "<i4"), ("sbase", "<i4"), ("scaled", "<i4"),
 ])
 
 _vp = C.c_void_p

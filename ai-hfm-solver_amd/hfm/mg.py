@@ -42,8 +42,10 @@ BLOCKS = (128, 256, 512)   # threads per block a sweep / restriction launch may 
 SCHUNK = 256          # cells per block of the batched setup kernels
 PAIR_BLOCK = 512      # threads per block of the fused pass pairs (one tile each)
 ALIGN = 8             # element alignment of sub-array offsets (16-byte rows)
-ELEM = {"f32": 4, "bf16": 2}
-TORCH_STORE = {"f32": torch.float32, "bf16": torch.bfloat16}
+ELEM = {"f32": 4, "bf16": 2, "f16": 2}
+TORCH_STORE = {"f32": torch.float32, "bf16": torch.bfloat16, "f16": torch.float16}
+# outer (z, p) storage types built for each V-cycle storage type
+OUTER_OF = {"f32": ("f32", "bf16"), "bf16": ("bf16", "f32"), "f16": ("f16", "f32")}
 
 
 ILV_SEG = 512        # cells per lane-interleaved segment of the FP64 vectors
@@ -155,14 +157,19 @@ class Params:
     # planes per tile of the fused pass pairs (at most; hfm_mg_tile): fewer
     # planes give more tiles (blocks) at the cost of more halo planes
     tile_planes: int = 32
+    # FP16 storage: every iteration scales the fine residual the V-cycle gets
+    # by a power of 2 so that its RMS is near 2^f16_rms_exp (the V-cycle's
+    # values then stay inside FP16's range; FCG is invariant to the scale)
+    f16_rms_exp: int = -4
 
     def __post_init__(self):
         if self.storage not in ELEM:
             raise ValueError(f"storage must be one of {sorted(ELEM)}")
         if self.outer is None:
             self.outer = self.storage
-        if self.outer not in ELEM:
-            raise ValueError(f"outer must be one of {sorted(ELEM)}")
+        if self.outer not in OUTER_OF[self.storage]:
+            raise ValueError(f"outer storage for {self.storage} must be one of "
+                             f"{OUTER_OF[self.storage]}")
         if self.nu1 < 1 or self.nu2 < 1:
             raise ValueError("nu1 and nu2 must be >= 1")
         if self.method not in L.METHOD:
@@ -261,6 +268,16 @@ class Pool:
         rec["method"] = L.METHOD[p.method]
         rec["slot0"] = [i.dots_off for i in insts]
         rec["nslot"] = [i.nblk for i in insts]
+        if p.storage == "f16":
+            # the first residual (b) scaled to an RMS near 2^f16_rms_exp; the
+            # finalize keeps it there as the residual falls
+            norms = torch.stack([torch.linalg.vector_norm(i.b32.double()) for i in insts])
+            norms = norms.cpu().numpy()
+            sbase = [p.f16_rms_exp + int(round(0.5 * math.log2(max(i.N, 1)))) for i in insts]
+            rec["sbase"], rec["scaled"] = sbase, 1
+            rec["sexp"] = [sb - (math.frexp(float(bn))[1] - 1)
+                           if bn > 0 and math.isfinite(bn) else sb
+                           for sb, bn in zip(sbase, norms)]
         rows = torch.from_numpy(rec.view(np.uint8).reshape(len(insts), -1).copy()).to(self.device)
         idx = torch.tensor([i.slot for i in insts], device=self.device)
         self.sc.view(self.capacity, -1).index_copy_(0, idx, rows)

@@ -280,10 +280,11 @@ def small_batches(args, dev, torch, sizes=(1, 8, 32)):
     return out
 
 
-def batched_fp32(specs, dev, torch):
-    """The same 512-instance batch with FP32 storage (the configuration's plain run)."""
+def batched_fp32(specs, dev, torch, storage="f32"):
+    """The same 512-instance batch with FP32 storage (the configuration's plain
+    run), or another storage type."""
     from hfm.mg import MGBatch
-    mg = MGBatch([s[0] for s in specs], storage="f32", tol=1e-8, maxit=300, device=dev)
+    mg = MGBatch([s[0] for s in specs], storage=storage, tol=1e-8, maxit=300, device=dev)
     load_batch(mg, specs, dev)
     one_step(mg)
     torch.cuda.synchronize()
@@ -291,13 +292,20 @@ def batched_fp32(specs, dev, torch):
     info = one_step(mg)
     torch.cuda.synchronize()
     t = time.perf_counter() - t0
-    if not np.all(info["status"] == 1):
-        raise SystemExit("bench: FP32 batch did not converge")
     out = {"solves_per_s": round(len(specs) / t, 2), "ms_per_step": round(t * 1e3, 2),
            "iters_mean": round(float(info["iter"].mean()), 2), "iters_max": int(info["iter"].max()),
-           "resident_GiB": round(mg.nbytes() / 2 ** 30, 2)}
-    log(f"bench: batched fp32 {out}")
+           "resident_GiB": round(mg.nbytes() / 2 ** 30, 2),
+           "converged": int(np.sum(info["status"] == 1))}
+    if storage != "f32":
+        rel, _ = true_residuals(mg, torch)
+        out["max_true_rel_residual_fp64"] = float(rel.max())
+    elif not np.all(info["status"] == 1):
+        raise SystemExit("bench: FP32 batch did not converge")
+    log(f"bench: batched {storage} {out}")
     return out
+
+
+L_STATUS = {0: "running", 1: "converged", 2: "maxit", 3: "breakdown"}
 
 
 def urban(args, dev, torch):
@@ -317,7 +325,8 @@ def urban(args, dev, torch):
     out = {"grid": list(sh), "steps": args.urban_steps, "dt": dt,
            "ground_cover": round(float(mask[0].mean()), 3)}
     finals = {}
-    variants = {"bf16": ("bf16", None), "bf16_fp32_zp": ("bf16", "f32"), "f32": ("f32", None)}
+    variants = {"bf16": ("bf16", None), "bf16_fp32_zp": ("bf16", "f32"), "f32": ("f32", None),
+                "f16": ("f16", None), "f16_fp32_zp": ("f16", "f32")}
     for name, (storage, outer) in variants.items():
         flow = UrbanProjection(sh, sigma=sig, inflow=prof, dt=dt, storage=storage, outer=outer,
                                tol=1e-8, device=dev)
@@ -328,12 +337,20 @@ def urban(args, dev, torch):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         its = []
+        failed = None
         for _ in range(args.urban_steps):
             state = flow.step(*state)
             its.append(int(flow.last_info["iter"][0]))
             if int(flow.last_info["status"][0]) != 1:
-                raise SystemExit(f"bench: urban {name} pressure solve failed: {flow.last_info}")
+                failed = L_STATUS[int(flow.last_info["status"][0])]
+                break
         torch.cuda.synchronize()
+        if failed is not None:
+            if storage != "f16":
+                raise SystemExit(f"bench: urban {name} pressure solve failed: {flow.last_info}")
+            out[name] = {"failed": failed, "after_steps": len(its)}
+            log(f"bench: urban {name}: the pressure solve stopped ({failed})")
+            continue
         tstep = (time.perf_counter() - t0) / args.urban_steps
         out[name] = {"ms_per_step": round(tstep * 1e3, 3),
                         "cell_steps_per_s": round(float(np.prod(sh)) / tstep, 1),
@@ -1133,6 +1150,7 @@ def main():
         del mg
         torch.cuda.empty_cache()
         extra["batched_fp32"] = batched_fp32(specs, dev, torch)
+        extra["batched_f16"] = batched_fp32(specs, dev, torch, storage="f16")
         extra["speedup_mixed_vs_fp32_batched"] = round(value / extra["batched_fp32"]["solves_per_s"], 2)
     if args.fp32_sample > 0:
         # a systematic sample (every k-th instance of the randomly ordered

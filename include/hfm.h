@@ -56,6 +56,7 @@ extern "C" {
 /* storage types of config-mode fields */
 #define HFM_STORE_F32 0
 #define HFM_STORE_BF16 1
+#define HFM_STORE_F16 2   /* IEEE half; the solver scales the fine residual (sexp) */
 
 /* reference-mode stencil forms */
 #define HFM_FORM_APPLY 0    /* y = A x                         */
@@ -298,6 +299,14 @@ typedef struct hfm_mg_scalars {
                          2 p_prev = the update's previous p (ec); see hfm_fcg_update */
   int32_t method;     /* HFM_METHOD_FCG or HFM_METHOD_REFINE */
   double alpha_prev;
+  /* Residual scaling (FP16 storage): the fine right-hand side of the V-cycle
+   * is T(2^sexp r); zexp is the exponent the current z = V(.) was made with
+   * (the finalize divides r.z by 2^zexp: FCG is invariant under a rescaled
+   * z, and refinement takes alpha = 2^-zexp).  With `scaled` set, INIT and
+   * CHECK choose the next sexp = sbase - ilogb(||r||), so that the scaled
+   * residual's RMS stays near 2^(sbase - log2(sqrt(N))).  All zero: no
+   * scaling (FP32 / BF16 storage, bitwise the unscaled solver). */
+  int32_t sexp, zexp, sbase, scaled;
 } hfm_mg_scalars;
 
 /* Outer iterations (hfm_mg_scalars.method).  REFINE is the FP64 defect

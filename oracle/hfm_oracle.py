@@ -951,9 +951,15 @@ class FCGResult:
     status: str
 
 
+def _ilogb(v: float) -> int:
+    """floor(log2 v) for v > 0 (C's ilogb)."""
+    return math.frexp(v)[1] - 1
+
+
 def fcg_solve(fine: FaceOp, b, tol=1e-8, storage="f32", maxit=200, nu1=2, nu2=2,
               omega=2.0 / 3.0, alpha=1.6, hierarchy: Optional[Hierarchy] = None,
-              max_coarse_edge: int = 4, outer: Optional[str] = None) -> FCGResult:
+              max_coarse_edge: int = 4, outer: Optional[str] = None,
+              f16_rms_exp: int = -4, scaled: Optional[bool] = None) -> FCGResult:
     """FP64 conjugate gradients preconditioned by one storage-precision V-cycle.
 
     Mirrors the device algorithm step for step (T = storage, P = outer,
@@ -963,6 +969,12 @@ def fcg_solve(fine: FaceOp, b, tol=1e-8, storage="f32", maxit=200, nu1=2, nu2=2,
       p = P(fma32(beta, p, z));  q = A p (FP32 coefficients, FP64 arithmetic);
       a = rz / (p . q);  x += a p;  r -= a q;  rl = T(r);
       stop when ||r|| / ||b|| <= tol.
+    FP16 storage scales the V-cycle's right-hand side by a power of 2:
+    rl = T(2^e r) with e = sbase - ilogb(||r_prev||) (||b|| for the first two),
+    sbase = f16_rms_exp + round(log2 sqrt(N)), and rz = (rl . z) / 2^e: FCG is
+    invariant to the per-iteration scale of z (p, alpha follow it), which only
+    keeps the V-cycle's values inside FP16's range.  ``scaled`` forces the
+    scaling on or off for any storage (default: FP16 only).
     """
     h = hierarchy or build_hierarchy(fine, storage, max_coarse_edge)
     s = h.storage
@@ -973,14 +985,18 @@ def fcg_solve(fine: FaceOp, b, tol=1e-8, storage="f32", maxit=200, nu1=2, nu2=2,
     x = np.zeros_like(b64)
     if bn2 == 0.0:
         return FCGResult(x, [0.0], 0, "converged")
+    scaled = (s == "f16") if scaled is None else bool(scaled)
+    sbase = f16_rms_exp + int(round(0.5 * math.log2(b64.size)))
+    sexp = sbase - _ilogb(math.sqrt(bn2)) if scaled else 0
     r = b64.copy()
-    rl = round_storage(r, s)
+    rl = round_storage(r * 2.0 ** sexp, s)
+    zexp = sexp
     rz = 0.0
     p = None
     hist = [1.0]
     for it in range(maxit):
         z = vcycle(h, rl, nu1, nu2, omega, alpha, out_storage=so)
-        rzn = float(np.sum(rl * z))
+        rzn = float(np.sum(rl * z)) * 2.0 ** -zexp
         beta = 0.0 if it == 0 else rzn / rz
         rz = rzn
         # one FP32 fused multiply-add, then the storage rounding
@@ -993,8 +1009,12 @@ def fcg_solve(fine: FaceOp, b, tol=1e-8, storage="f32", maxit=200, nu1=2, nu2=2,
         a = rz / pap
         x = x + a * p
         r = r - a * q
-        rl = round_storage(r, s)
-        rel = math.sqrt(float(np.sum(r * r)) / bn2)
+        rl = round_storage(r * 2.0 ** sexp, s)
+        rr = float(np.sum(r * r))
+        zexp = sexp
+        if scaled and rr > 0.0 and np.isfinite(rr):
+            sexp = sbase - _ilogb(math.sqrt(rr))
+        rel = math.sqrt(rr / bn2)
         hist.append(rel)
         if rel <= tol:
             return FCGResult(x, hist, it + 1, "converged")

@@ -58,7 +58,8 @@ def test_coarse_inverse(dev):
         np.testing.assert_allclose(got, h.coarse_inv, rtol=1e-10, atol=1e-12 * np.abs(h.coarse_inv).max())
 
 
-@pytest.mark.parametrize("storage,outer", [("f32", None), ("bf16", None), ("bf16", "f32")])
+@pytest.mark.parametrize("storage,outer", [("f32", None), ("bf16", None), ("bf16", "f32"),
+                                           ("f16", None), ("f16", "f32")])
 @pytest.mark.parametrize("shape", [(32, 32, 32), (20, 12, 36), (20, 13, 16)])
 def test_solve_matches_oracle(dev, storage, outer, shape):
     # W = 32: 8-wide row kernels on the fine levels; W = 36: per-cell kernels;
@@ -178,7 +179,8 @@ def _buffer(inst, ptr):
     raise KeyError(ptr)
 
 
-@pytest.mark.parametrize("storage,outer", [("bf16", None), ("f32", None), ("bf16", "f32")])
+@pytest.mark.parametrize("storage,outer", [("bf16", None), ("f32", None), ("bf16", "f32"),
+                                           ("f16", None)])
 def test_fused_pass_pairs_equal_separate_launches(dev, storage, outer):
     """One V-cycle with the fused pass pairs (smooth + restrict, correct +
     smooth) equals the separate launches bitwise: tiles spanning the plane,
@@ -257,7 +259,7 @@ def test_batched_setup_kernels_bitwise(dev):
             assert torch.equal(inst.faceT[lev], inst.face32[lev].to(torch.bfloat16)), (inst.shape, lev)
 
 
-@pytest.mark.parametrize("storage", ["bf16", "f32"])
+@pytest.mark.parametrize("storage", ["bf16", "f32", "f16"])
 def test_large_instances_against_fp64_references(dev, storage):
     """96^3 (sigma_ln 2, 3 dipoles) and 128^3 (sigma_ln 1.5) solved together in one
     batch to 1e-8, against FP64 reference solutions of the same draws
@@ -362,7 +364,7 @@ def test_every_schedule_converges_to_the_true_residual(dev):
 
 @pytest.mark.parametrize("storage,outer,method,fuse", [
     ("bf16", None, "fcg", True), ("f32", None, "fcg", True), ("bf16", "f32", "fcg", True),
-    ("bf16", None, "refine", False), ("bf16", None, "fcg", False)])
+    ("bf16", None, "refine", False), ("bf16", None, "fcg", False), ("f16", None, "fcg", True)])
 def test_persistent_solve_equals_the_launch_plan_bitwise(dev, storage, outer, method, fuse):
     """The persistent launch (one hfm_mg_persistent program per iters_per_graph
     iterations, per-instance dependency counters) gives bitwise the solutions,

@@ -132,7 +132,7 @@ def test_ad2d_steps_match_oracle(dev, which):
     assert abs(H(g).sum() - x.sum()) / x.sum() < 1e-3
 
 
-@pytest.mark.parametrize("storage", ["f32", "bf16"])
+@pytest.mark.parametrize("storage", ["f32", "bf16", "f16"])
 def test_urban_projection_pressure_solve(dev, storage):
     """The projection's pressure solve reaches 1e-8 against the FP64 sparse solution."""
     import torch

@@ -31,7 +31,7 @@ def test_abi_struct_sizes_match_host_dtypes():
     a, b = L.C.c_int64(), L.C.c_int64()
     assert lib.hfm_struct_sizes(L.C.byref(a), L.C.byref(b)) == 0
     assert a.value == L.TASK_DTYPE.itemsize == 240
-    assert b.value == L.SCALARS_DTYPE.itemsize == 104
+    assert b.value == L.SCALARS_DTYPE.itemsize == 120
 
 
 def test_flow2d_struct_sizes_match_host_dtypes():
@@ -235,7 +235,9 @@ def test_plan_rejects_bad_shapes():
     with pytest.raises(ValueError):
         MGBatch([(4, 4, 4)], device="cpu")
     with pytest.raises(ValueError):
-        MGBatch([(32, 32, 32)], storage="f16", device="cpu")
+        MGBatch([(32, 32, 32)], storage="f8", device="cpu")
+    with pytest.raises(ValueError):   # FP16 storage pairs with FP16 or FP32 z / p only
+        MGBatch([(32, 32, 32)], storage="f16", outer="bf16", device="cpu")
 
 
 def _ilv_cell(n):

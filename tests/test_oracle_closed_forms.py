@@ -295,3 +295,30 @@ def test_solve_refined_is_the_stationary_iteration():
                           max_outer=100, hierarchy=h)
     assert res.converged
     assert np.linalg.norm(res.x.ravel() - xs) / np.linalg.norm(xs) < 1e-9
+
+
+def test_fcg_is_invariant_to_the_residual_scale():
+    """The FP16 path's per-iteration power-of-2 scaling of the V-cycle's
+    right-hand side, with r.z divided by the scale, is the same iteration: in
+    FP64 storage the scaled and unscaled solves take the same iterations and
+    agree to rounding (dividing r.z by the square of the scale, or not at all,
+    changes beta and the iterates)."""
+    sx, sy, sz, b = O.make_anisotropic_fields((12, 10, 8), 3, 1.5, 2)
+    op = O.harmonic_faces(sx, sy, sz)
+    a = O.fcg_solve(op, b, tol=1e-10, storage="f64", scaled=False)
+    for e in (-12, -4, 6):
+        c = O.fcg_solve(op, b, tol=1e-10, storage="f64", scaled=True, f16_rms_exp=e)
+        assert c.iters == a.iters and c.status == "converged"
+        assert np.linalg.norm(c.x - a.x) / np.linalg.norm(a.x) < 1e-10
+        np.testing.assert_allclose(c.rel_history, a.rel_history, rtol=1e-7, atol=1e-14)
+
+
+def test_fcg_f16_storage_reaches_the_fp64_solution():
+    sx, sy, sz, b = O.make_anisotropic_fields((16, 12, 16), 4, 1.0, 1)
+    op = O.harmonic_faces(sx, sy, sz)
+    res = O.fcg_solve(op, b, tol=1e-8, storage="f16")
+    assert res.status == "converged"
+    xd = O.solve_fp64_direct(op.astype("f32"), b.astype(np.float32))
+    assert np.linalg.norm(res.x - xd) / np.linalg.norm(xd) < 1e-6
+    bf = O.fcg_solve(op, b, tol=1e-8, storage="bf16")
+    assert res.iters <= bf.iters       # more mantissa than BF16
