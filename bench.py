@@ -241,6 +241,53 @@ def serving(args, dev, torch):
     return out
 
 
+def cpu_baseline_config0(sx, sy, sz, b):
+    """The oracle's stationary cycle on the same 64^3 problem, one host core."""
+    from oracle import hfm_oracle as O
+    t0 = time.perf_counter()
+    ref = O.solve_refined(O.harmonic_faces(sx, sy, sz).astype("f32"), b, tol=1e-6, storage="f32",
+                          inner_cycles=1, alpha=1.0, max_outer=400)
+    return ref, time.perf_counter() - t0
+
+
+def config0(args, dev, torch):
+    """The plain method of the CPU-runnable configuration: one 64^3 solve
+    (sigma_ln 1, one dipole, seed 0), stationary V(2,2) cycles (omega 2/3,
+    nearest prolongation, alpha 1) with an FP32 V-cycle to 1e-6, as FP64 defect
+    correction (method refine); GPU wall time from staged fields (faces,
+    hierarchy, solve; median of 5 after a warm run) and the oracle's restatement
+    of the same cycle on one host core (numpy, its cycle count checked: this
+    leg's CPU baseline)."""
+    from hfm.mg import MGBatch
+    from hfm.workloads import host_fields
+    shape = (64, 64, 64)
+    sx, sy, sz, b = host_fields(shape, 1.0, 1, 0)
+    mg = MGBatch([shape], storage="f32", tol=1e-6, maxit=400, alpha=1.0, method="refine",
+                 device=dev)
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)  # noqa: E731
+    mg.stage_sigma(0, T(sx), T(sy), T(sz), T(b))
+    one_step(mg)
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        info = one_step(mg)
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t0)
+    rel, _ = true_residuals(mg, torch)
+    ref, tcpu = cpu_baseline_config0(sx, sy, sz, b)
+    out = {"grid": list(shape), "method": "FP64 defect correction, FP32 V(2,2), alpha 1",
+           "tol": 1e-6, "ms_per_solve": round(float(np.median(ts)) * 1e3, 3),
+           "vcycles": int(info["iter"][0]), "oracle_vcycles": int(ref.vcycles),
+           "status": int(info["status"][0]), "true_rel_residual_fp64": float(rel[0]),
+           "gpu_vs_oracle_rel_diff": float(np.linalg.norm(mg.solution(0).cpu().numpy() - ref.x)
+                                           / np.linalg.norm(ref.x)),
+           "cpu_oracle_s": round(tcpu, 3), "cpu_cores": 1,
+           "speedup_vs_cpu_oracle": round(tcpu / float(np.median(ts)), 1)}
+    log(f"bench: config0 {out}")
+    return out
+
+
 def small_batches(args, dev, torch, sizes=(1, 8, 32)):
     """Launch-bound batches: n instances of edges 32..64 solved with graph-replayed
     launches and with the persistent kernel, the batch's wall time from staged
@@ -964,6 +1011,9 @@ def main():
     ap.add_argument("--persistent-steps", type=int, default=3,
                     help="also time the batch with one persistent launch per iters_per_graph "
                          "iterations (0: skip)")
+    ap.add_argument("--config0", type=int, default=1,
+                    help="also time the CPU-runnable configuration's plain 64^3 solve (and "
+                         "the oracle's on one host core)")
     ap.add_argument("--tune-schedule", type=int, default=1,
                     help="1: also rank sweep schedules by time to 1e-8 (reported beside "
                          "the V(2,2) headline)")
@@ -1124,6 +1174,8 @@ def main():
             "solutions_bitwise_equal": same}
         log(f"bench: persistent {extra['persistent']}")
         extra["small_batches"] = small_batches(args, dev, torch)
+    if args.config0:
+        extra["config0_plain_64"] = config0(args, dev, torch)
     if args.tune_schedule:
         # the sweep schedule (nu1, nu2, alpha) ranked by the batch's time to 1e-8;
         # the headline keeps the configuration's V(2,2)
