@@ -388,3 +388,83 @@ def test_persistent_solve_equals_the_launch_plan_bitwise(dev, storage, outer, me
     assert list(again["iter"]) == list(ref["iter"])
     for q in range(len(specs)):
         assert bool((mg.solution(q) == xr[q]).all()), q
+
+
+def test_zero_right_hand_side_stops_at_once_and_alone(dev):
+    """An instance with b = 0 converges at iteration 0 with x = 0 exactly; the
+    instances beside it are bitwise what they are without it."""
+    import torch
+    specs = [((32, 32, 32), 21, 1.0, 1), ((24, 40, 16), 22, 1.5, 2)]
+    alone, _ = _batch(dev, specs, "bf16", tol=1e-8, fuse=False)
+    ia = alone.solve()
+    from hfm.mg import MGBatch
+    mg = MGBatch([specs[0][0], (40, 24, 32), specs[1][0]], storage="bf16", device=dev, fuse=False)
+    for q, spec in zip((0, 2), specs):
+        sx, sy, sz, b = O.make_anisotropic_fields(*spec)
+        mg.set_problem(q, sx, sy, sz, b)
+    sx, sy, sz, _ = O.make_anisotropic_fields((40, 24, 32), 23, 1.0, 1)
+    mg.set_problem(1, sx, sy, sz, np.zeros((40, 24, 32), np.float32))
+    mg.setup()
+    info = mg.solve()
+    assert int(info["status"][1]) == 1 and int(info["iter"][1]) == 0
+    assert float(info["rel"][1]) == 0.0
+    assert bool((mg.solution(1) == 0).all())
+    for q, p in ((0, 0), (2, 1)):
+        assert int(info["iter"][q]) == int(ia["iter"][p])
+        assert torch.equal(mg.solution(q), alone.solution(p))
+
+
+@pytest.mark.parametrize("shape", [(5, 64, 64), (8, 8, 256), (128, 16, 9), (6, 6, 6)])
+def test_flat_and_elongated_grids_match_oracle(dev, shape):
+    """Boxes far from cubic (a 5-plane slab, a long bar, odd extents) and the
+    smallest grid with two levels: the oracle's iteration count and its FP64
+    direct solution."""
+    spec = (shape, 31, 1.0, 1)
+    mg, probs = _batch(dev, [spec], "bf16", tol=1e-8, maxit=200)
+    info = mg.solve()
+    op, b = probs[0]
+    ref = O.fcg_solve(op, b, tol=1e-8, storage="bf16", maxit=200)
+    assert info["status"][0] == 1 and ref.status == "converged"
+    assert int(info["iter"][0]) == ref.iters
+    x = mg.solution(0).cpu().numpy()
+    xd = O.solve_fp64_direct(op.astype("f32"), b.astype(np.float32))
+    assert rel(x, xd) < 1e-6
+
+
+def test_largest_single_grid_reaches_the_true_residual(dev):
+    """A 512^3 instance (134 M cells, the largest edge the fused tiles and the
+    32-bit cell indices are sized for) to 1e-8, checked by an FP64 residual of
+    the stored FP32 faces computed with torch array ops on the GPU."""
+    import torch
+    from hfm.mg import MGBatch
+    from hfm.workloads import host_fields
+    shape = (512, 512, 512)
+    print("512^3 solve", flush=True)
+    sx, sy, sz, b = host_fields(shape, 1.0, 2, 5)
+    mg = MGBatch([shape], storage="bf16", tol=1e-8, maxit=100, device=dev)
+    T = lambda a: torch.from_numpy(a).to(dev)  # noqa: E731
+    mg.set_problem(0, T(sx), T(sy), T(sz), T(b))
+    del sx, sy, sz
+    mg.setup()
+    info = mg.solve()
+    assert int(info["status"][0]) == 1
+    inst = mg.insts[0]
+    D, H, W = shape
+    N = D * H * W
+    f, o = inst.face32[0], inst.face_off[0]
+    cx, cy, cz = (f[o[q]:o[q] + N].view(D, H, W).double() for q in range(3))
+    bxh = f[o[3]:o[3] + D * H].view(D, H).double()
+    byh = f[o[4]:o[4] + D * W].view(D, W).double()
+    bzh = f[o[5]:o[5] + H * W].view(H, W).double()
+    x = inst.solution().double()
+    ax = (cx + torch.cat([cx[:, :, 1:], bxh[:, :, None]], 2) + cy +
+          torch.cat([cy[:, 1:, :], byh[:, None, :]], 1) + cz + torch.cat([cz[1:], bzh[None]], 0)) * x
+    ax[:, :, 1:] -= cx[:, :, 1:] * x[:, :, :-1]
+    ax[:, :, :-1] -= cx[:, :, 1:] * x[:, :, 1:]
+    ax[:, 1:, :] -= cy[:, 1:, :] * x[:, :-1, :]
+    ax[:, :-1, :] -= cy[:, 1:, :] * x[:, 1:, :]
+    ax[1:] -= cz[1:] * x[:-1]
+    ax[:-1] -= cz[1:] * x[1:]
+    bt = inst.b32.view(D, H, W).double()
+    relres = float((bt - ax).norm() / bt.norm())
+    assert relres <= 1e-8, relres
