@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <mutex>
+#include <type_traits>
 
 #include "hfm_common.h"
 
@@ -60,11 +61,32 @@ struct Grid3 {
 // skipping them is exact).
 constexpr uint32_t kCross = (1u << 4) | (1u << 10) | (1u << 12) | (1u << 13) | (1u << 14) |
                             (1u << 16) | (1u << 22);
+// Also the 18-tap sets of the first-derivative stencils (the advection and
+// divergence layers: the taps of the centre x column, y row or z plane are
+// zero), TAPS = kTapsX / kTapsY / kTapsZ.
+constexpr int kTapsX = 180, kTapsY = 181, kTapsZ = 182;
+constexpr uint32_t tap_mask(int taps) {
+  uint32_t m = 0;
+  for (int t = 0; t < 27; ++t) {
+    const bool on = taps == 27 ? true
+                    : taps == 7 ? ((kCross >> t) & 1u) != 0
+                    : taps == kTapsX ? t % 3 != 1
+                    : taps == kTapsY ? (t / 3) % 3 != 1
+                                     : t / 9 != 1;
+    if (on) m |= 1u << t;
+  }
+  return m;
+}
 template <int TAPS>
 __device__ constexpr bool tap_on(int t) {
-  return TAPS == 27 || ((kCross >> t) & 1u);
+  return (tap_mask(TAPS) >> t) & 1u;
 }
-inline int taps_of(uint32_t mask) { return (mask & ~kCross) ? 27 : 7; }
+// the smallest tap set holding every non-zero weight
+inline int taps_of(uint32_t mask) {
+  for (int taps : {7, kTapsX, kTapsY, kTapsZ})
+    if (!(mask & ~tap_mask(taps))) return taps;
+  return 27;
+}
 
 // The taps of the 3x3x3 neighbourhood of (k, j, i) with zero ghosts.  All are
 // loaded from in-bounds addresses (a tap outside the grid re-reads the centre
@@ -506,6 +528,7 @@ struct Kern4 {
 //   k = min(8*0.25 |1/3 (|u~| + |v~| + |w~|) dif(f~)| / (eps + (|ax| + |ay| + |az|)/3), 0.5)
 //       / (1 + dt sigma)
 // with one 27-point neighbourhood load per component serving the four stencils.
+template <int TX, int TY, int TZ>
 __global__ __launch_bounds__(kBX* kBY) void k_pg_coef(const float* __restrict__ u,
                                                       const float* __restrict__ v,
                                                       const float* __restrict__ w,
@@ -537,9 +560,9 @@ __global__ __launch_bounds__(kBX* kBY) void k_pg_coef(const float* __restrict__ 
       float nb[27];
       win[q].template taps<27>(nb, 0);
       const float dif = wsum<27>(K.k[0], nb);
-      const float ax = wsum<27>(K.k[1], nb);
-      const float ay = wsum<27>(K.k[2], nb);
-      const float az = wsum<27>(K.k[3], nb);
+      const float ax = wsum<TX>(K.k[1], nb);
+      const float ay = wsum<TY>(K.k[2], nb);
+      const float az = wsum<TZ>(K.k[3], nb);
       const float num = 2.0f * fabsf(speed * dif);
       const float den = eps + ((fabsf(ax) + fabsf(ay)) + fabsf(az)) / 3.0f;
       if (c.out) out[q][n] = fminf(num / den, 0.5f) / damp;
@@ -689,7 +712,7 @@ struct Kern3 {
   uint32_t mask[3];
 };
 
-template <int NF>
+template <int NF, int T0 = 27, int T1 = 27, int T2 = 27>
 __global__ __launch_bounds__(kBX* kBY) void k_conv27_sum_v(const float* x0, const float* x1,
                                                            const float* x2, float* __restrict__ y,
                                                            Kern3 K, float scale, Grid3 g,
@@ -716,7 +739,7 @@ __global__ __launch_bounds__(kBX* kBY) void k_conv27_sum_v(const float* x0, cons
     for (int q = 0; q < NF; ++q) {
       float t[27];
       win[q].template taps<27>(t, 0);
-      acc += wsum<27>(K.k[q], t);
+      acc += q == 0 ? wsum<T0>(K.k[0], t) : q == 1 ? wsum<T1>(K.k[1], t) : wsum<T2>(K.k[2], t);
       win[q].shift();
     }
     const float from_right = __shfl_down(acc, 1, 64);
@@ -735,7 +758,7 @@ __global__ __launch_bounds__(kBX* kBY) void k_conv27_sum_v(const float* x0, cons
 // and with HFM_GRAD_POST_BLUFF o_q is damped as well, except at the cells the
 // velocity fill copies from (u at j = 1, j = H-2 and k = D-2; v at k = D-2):
 // hfm_bc_velocity_bluff_inplace_f32 gathers those undamped and damps them.
-template <int TAPS>
+template <int T0, int T1, int T2>
 __global__ __launch_bounds__(kBX* kBY) void k_grad3(const float* __restrict__ p,
                                                     const float* f0, const float* f1,
                                                     const float* f2, float* o0, float* o1,
@@ -791,7 +814,7 @@ __global__ __launch_bounds__(kBX* kBY) void k_grad3(const float* __restrict__ p,
 #pragma unroll
     for (int ry = 0; ry < 2; ++ry) {
       float v[27];
-      win.taps<TAPS>(v, ry);
+      win.taps<27>(v, ry);
       const int j = c.j + ry;
       if (c.out && j < g.H) {
         const int64_t n = g.idx(k, j, c.i);
@@ -799,7 +822,8 @@ __global__ __launch_bounds__(kBX* kBY) void k_grad3(const float* __restrict__ p,
         const bool src_u = j == 1 || j == g.H - 2 || k == g.D - 2, src_v = k == g.D - 2;
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
-          const float ax = wsum<TAPS>(K.k[q], v);
+          const float ax = q == 0 ? wsum<T0>(K.k[0], v)
+                           : q == 1 ? wsum<T1>(K.k[1], v) : wsum<T2>(K.k[2], v);
           float fv = cur.f[ry][q];
           if (flags & HFM_GRAD_PRE_BLUFF) fv = bluff_apply(fv, s, dt);
           float o = fv - ax;
@@ -1032,6 +1056,16 @@ dim3 grid_for(const Grid3& g) {
   return dim3(unsigned((g.W + kBX - 1) / kBX), unsigned((g.H + kBY - 1) / kBY), unsigned(g.D));
 }
 
+// Tap sets of a three-kernel walker: the x / y / z first-derivative sets in
+// that order, all 7-point, or all 27 (any other mix).
+enum { kSetXYZ, kSet7, kSet27 };
+int tapset3(const uint32_t m[3]) {
+  const int a = taps_of(m[0]), b = taps_of(m[1]), c = taps_of(m[2]);
+  if (a == kTapsX && b == kTapsY && c == kTapsZ) return kSetXYZ;
+  if (a == 7 && b == 7 && c == 7) return kSet7;
+  return kSet27;
+}
+
 bool make_weights(const float* K, Weights& w, uint32_t& mask) {
   if (!K) return false;
   mask = 0;
@@ -1081,12 +1115,17 @@ extern "C" int hfm_conv27_f32(const float* x, float* y, const float* b, const fl
     return HFM_EINVAL;
   }
   const Grid3 g{int(D), int(H), int(W)};
-  if (taps_of(mask) == 7)
-    hipLaunchKernelGGL(k_conv27<7>, grid_col<2>(g), dim3(kBX, kBY), 0, as_stream(stream), x, y, b, w,
-                       d, g, form);
-  else
-    hipLaunchKernelGGL(k_conv27<27>, grid_col<2>(g), dim3(kBX, kBY), 0, as_stream(stream), x, y, b,
-                       w, d, g, form);
+  auto go = [&](auto taps) {
+    hipLaunchKernelGGL(k_conv27<decltype(taps)::value>, grid_col<2>(g), dim3(kBX, kBY), 0,
+                       as_stream(stream), x, y, b, w, d, g, form);
+  };
+  switch (taps_of(mask)) {
+    case 7: go(std::integral_constant<int, 7>{}); break;
+    case kTapsX: go(std::integral_constant<int, kTapsX>{}); break;
+    case kTapsY: go(std::integral_constant<int, kTapsY>{}); break;
+    case kTapsZ: go(std::integral_constant<int, kTapsZ>{}); break;
+    default: go(std::integral_constant<int, 27>{}); break;
+  }
   return launch_status("hfm_conv27_f32");
 }
 
@@ -1381,8 +1420,15 @@ extern "C" int hfm_pg_coef_f32(const float* const vt[3], const float* sigma, flo
       }
   for (int q = 0; q < 4; ++q) make_weights(K[q], kk.k[q], kk.mask[q]);
   const Grid3 g{int(D), int(H), int(W)};
-  hipLaunchKernelGGL(k_pg_coef, grid_col<1>(g), dim3(kBX, kBY), 0, as_stream(stream), vt[0], vt[1],
-                     vt[2], sigma, kout[0], kout[1], kout[2], kk, dt, eps, g);
+  const uint32_t adv[3] = {kk.mask[1], kk.mask[2], kk.mask[3]};
+  if (tapset3(adv) == kSetXYZ)
+    hipLaunchKernelGGL((k_pg_coef<kTapsX, kTapsY, kTapsZ>), grid_col<1>(g), dim3(kBX, kBY), 0,
+                       as_stream(stream), vt[0], vt[1], vt[2], sigma, kout[0], kout[1], kout[2], kk,
+                       dt, eps, g);
+  else
+    hipLaunchKernelGGL((k_pg_coef<27, 27, 27>), grid_col<1>(g), dim3(kBX, kBY), 0,
+                       as_stream(stream), vt[0], vt[1], vt[2], sigma, kout[0], kout[1], kout[2], kk,
+                       dt, eps, g);
   return launch_status("hfm_pg_coef_f32");
 }
 
@@ -1434,7 +1480,11 @@ extern "C" int hfm_conv27_sum_f32(const float* const* x, float* y, int nf, const
     xs[q] = xs[0];
   }
   const Grid3 g{int(D), int(H), int(W)};
-  if (nf == 3)
+  if (nf == 3 && tapset3(kk.mask) == kSetXYZ)
+    hipLaunchKernelGGL((k_conv27_sum_v<3, kTapsX, kTapsY, kTapsZ>), grid_col<1>(g),
+                       dim3(kBX, kBY), 0, as_stream(stream), xs[0], xs[1], xs[2], y, kk, scale, g,
+                       flags);
+  else if (nf == 3)
     hipLaunchKernelGGL(k_conv27_sum_v<3>, grid_col<1>(g), dim3(kBX, kBY), 0, as_stream(stream),
                        xs[0], xs[1], xs[2], y, kk, scale, g, flags);
   else if (nf == 2)
@@ -1475,18 +1525,23 @@ extern "C" int hfm_grad3_f32(const float* p, const float* const f[3], float* con
         set_error("%s: output %d aliases field %d", what, q, r);
         return HFM_EINVAL;
       }
-  const int taps = taps_of(kk.mask[0]);
-  if (taps_of(kk.mask[1]) != taps || taps_of(kk.mask[2]) != taps) {
-    set_error("%s: the three kernels must share a tap set (all 7-point or all 27-point)", what);
-    return HFM_EINVAL;
-  }
   const Grid3 g{int(D), int(H), int(W)};
-  if (taps == 7)
-    hipLaunchKernelGGL(k_grad3<7>, grid_col<2>(g), dim3(kBX, kBY), 0, as_stream(stream), p, f[0],
-                       f[1], f[2], o[0], o[1], o[2], pout, kk, sigma, dt, g, flags);
-  else
-    hipLaunchKernelGGL(k_grad3<27>, grid_col<2>(g), dim3(kBX, kBY), 0, as_stream(stream), p, f[0],
-                       f[1], f[2], o[0], o[1], o[2], pout, kk, sigma, dt, g, flags);
+  switch (tapset3(kk.mask)) {
+    case kSetXYZ:
+      hipLaunchKernelGGL((k_grad3<kTapsX, kTapsY, kTapsZ>), grid_col<2>(g), dim3(kBX, kBY), 0,
+                         as_stream(stream), p, f[0], f[1], f[2], o[0], o[1], o[2], pout, kk, sigma,
+                         dt, g, flags);
+      break;
+    case kSet7:
+      hipLaunchKernelGGL((k_grad3<7, 7, 7>), grid_col<2>(g), dim3(kBX, kBY), 0,
+                         as_stream(stream), p, f[0], f[1], f[2], o[0], o[1], o[2], pout, kk, sigma,
+                         dt, g, flags);
+      break;
+    default:
+      hipLaunchKernelGGL((k_grad3<27, 27, 27>), grid_col<2>(g), dim3(kBX, kBY), 0,
+                         as_stream(stream), p, f[0], f[1], f[2], o[0], o[1], o[2], pout, kk, sigma,
+                         dt, g, flags);
+  }
   return launch_status(what);
 }
 

@@ -729,6 +729,9 @@ def profile_kernel(args, dev, torch):
             choice = tune(mg)
             print(json.dumps({"coarse_cycle_ms": {str(k): round(v, 4) for k, v in mg.coarse_times.items()},
                               "coarse_cells": choice["coarse_cells"],
+                              "tile_planes": choice["tile_planes"],
+                              "vcycle_ms_by_tile_planes": {str(k): round(v, 4)
+                                                           for k, v in mg.tile_times.items()},
                               "tune_ms": {"/".join(map(str, k)): {str(b): round(t, 4) for b, t in v.items()}
                                           for k, v in sorted(mg.tune_times.items())},
                               "blocks": {"/".join(map(str, k)): v for k, v in sorted(choice["blocks"].items())}}),
