@@ -82,6 +82,46 @@ def _cpu_solve(spec):
             "x": res.x.astype(np.float64)}
 
 
+def cpu_baseline_torch(specs, iters, nsample, threads, gpu_x):
+    """The baseline plan's CPU restatement: the solve as unfused PyTorch-CPU FP32
+    array operations (the operator as six shifted face products, Jacobi sweeps
+    as separate passes, avg_pool3d restriction, repeat_interleave prolongation;
+    oracle/torch_cpu.py) with ``threads`` torch threads, on the first
+    ``nsample`` instances of the workload; the rate in fine-cell-iterations per
+    second of solve time is scaled to the workload's sum(cells x iterations).
+    The GPU solutions of those instances (``gpu_x``) are compared with it."""
+    import torch
+    from oracle import torch_cpu as TC
+    from hfm.workloads import host_fields
+    sample = specs[:nsample]
+    prev = torch.get_num_threads()
+    try:
+        TC.cell_iterations_per_s(sample[:1], threads, host_fields)     # warm
+        res = TC.cell_iterations_per_s(sample, threads, host_fields)
+    finally:
+        torch.set_num_threads(prev)
+    if not all(o["converged"] for o in res["solves"]):
+        raise SystemExit("bench: a CPU-restatement solve did not converge")
+    total = sum(math.prod(s[0]) * int(k) for s, k in zip(specs, iters))
+    diffs, it_diff = [], []
+    for q, o in enumerate(res["solves"]):
+        if q in gpu_x:
+            xg, kg = gpu_x[q]
+            diffs.append(float(np.linalg.norm(xg - o["x"]) / np.linalg.norm(o["x"])))
+            it_diff.append(int(kg) - int(o["iters"]))
+    edges = sorted({sp[0][0] for sp in sample})
+    return {"value": round(len(specs) * res["rate"] / total, 3), "unit": "solves/s",
+            "cores": threads, "kind": "port",
+            "sample": f"CPU restatement of the solve path as unfused PyTorch-CPU FP32 array ops "
+                      f"(FP64 FCG around an FP32 V(2,2)) with {threads} torch threads, on the "
+                      f"first {len(sample)} instances of the workload (edges {edges}), "
+                      f"{res['seconds']:.1f} s of solves, {res['rate'] / 1e6:.2f} M "
+                      f"fine-cell-iterations/s, scaled to the workload's {total / 1e9:.2f} G "
+                      f"cell-iterations",
+            "gpu_vs_cpu_max_rel_diff": max(diffs) if diffs else None,
+            "iters_gpu_minus_cpu": sorted(set(it_diff))}
+
+
 def cpu_baseline(specs, iters, nsample, workers, gpu_x):
     """The oracle's solve (numpy FCG + FP32-storage V(2,2), unfused array ops) on
     ``workers`` host processes: the first ``nsample`` instances of the workload,
@@ -1233,7 +1273,14 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         workers = max(1, min(os.cpu_count() or 1, args.cpu_workers))
-        cpu = cpu_baseline(specs, info["iter"], args.cpu_sample, workers, gpu_x)
+        # the baseline plan's per-op PyTorch-CPU restatement on all host cores
+        # (the line's cpu_baseline), and the oracle's numpy solve in one worker
+        # process per core beside it
+        cpu = cpu_baseline_torch(specs, info["iter"], args.cpu_sample, workers, gpu_x)
+        log(f"bench: cpu baseline (torch per-op) {cpu}")
+        cpu["numpy_oracle_per_core_processes"] = cpu_baseline(specs, info["iter"],
+                                                              args.cpu_sample, workers, gpu_x)
+        log(f"bench: cpu baseline (numpy oracle) {cpu['numpy_oracle_per_core_processes']}")
     out = {
         "metric": METRIC, "value": round(value, 2), "unit": "solves/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup,
