@@ -708,7 +708,8 @@ def profile_flow2d(args, dev, torch):
 
 def profile_urban(args, dev, torch):
     """The urban projection step alone (for kernel traces): ``--urban-steps``
-    steps of the FP32 and the BF16 + FP32-z/p variants; prints one JSON line."""
+    steps of the FP16 (the default), FP32 and BF16 + FP32-z/p variants;
+    prints one JSON line."""
     from hfm.timestep import UrbanProjection
     from hfm.workloads import log_law_profile, urban_blocks
     sh = (64, 256, 256)
@@ -716,7 +717,8 @@ def profile_urban(args, dev, torch):
     sig = torch.from_numpy(np.where(mask, 1e5, 0.0).astype(np.float32)).to(dev)
     prof = log_law_profile(sh[0])
     out = {"workload": "urban_projection", "grid": list(sh)}
-    for name, (storage, outer) in {"f32": ("f32", None), "bf16_fp32_zp": ("bf16", "f32")}.items():
+    for name, (storage, outer) in {"f16": ("f16", None), "f32": ("f32", None),
+                                   "bf16_fp32_zp": ("bf16", "f32")}.items():
         flow = UrbanProjection(sh, sigma=sig, inflow=prof, dt=0.05, storage=storage, outer=outer,
                                tol=1e-8, device=dev)
         u0 = torch.from_numpy(np.broadcast_to(prof[:, None, None], sh).copy()).to(dev)
