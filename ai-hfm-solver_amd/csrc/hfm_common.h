@@ -111,6 +111,14 @@ __device__ __forceinline__ void put(HFM_GLOBAL bf16* p, float v) {
 __device__ __forceinline__ float ldg(const HFM_GLOBAL f16* p) {
   return to_f(f16{*(const HFM_GLOBAL uint16_t*)p});
 }
+// One stored value as loaded, not converted (for values used well after their load).
+__device__ __forceinline__ float ldraw(const HFM_GLOBAL float* p) { return *p; }
+__device__ __forceinline__ bf16 ldraw(const HFM_GLOBAL bf16* p) {
+  return bf16{*(const HFM_GLOBAL uint16_t*)p};
+}
+__device__ __forceinline__ f16 ldraw(const HFM_GLOBAL f16* p) {
+  return f16{*(const HFM_GLOBAL uint16_t*)p};
+}
 __device__ __forceinline__ void put(HFM_GLOBAL float* p, f16 v) { *p = to_f(v); }
 __device__ __forceinline__ void put(HFM_GLOBAL f16* p, f16 v) { *(HFM_GLOBAL uint16_t*)p = v.v; }
 __device__ __forceinline__ void put(HFM_GLOBAL f16* p, float v) {

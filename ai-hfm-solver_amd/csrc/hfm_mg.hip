@@ -771,27 +771,58 @@ __device__ __forceinline__ void st4_nt(HFM_GLOBAL f16* p, const float v[4]) {
   __builtin_nontemporal_store(u2v{pack2h(v[0], v[1]), pack2h(v[2], v[3])}, (HFM_GLOBAL u2v*)p);
 }
 
+// Two values as loaded (the coarse correction under a run), unpacked where used.
+template <typename T>
+struct Raw2;
+template <>
+struct Raw2<float> {
+  typedef float f2v __attribute__((ext_vector_type(2)));
+  f2v a;
+  __device__ __forceinline__ void load(const HFM_GLOBAL float* p) { a = *(const HFM_GLOBAL f2v*)p; }
+  __device__ __forceinline__ void get(float v[2]) const { v[0] = a.x; v[1] = a.y; }
+  __device__ __forceinline__ void zero() { a = f2v{0.0f, 0.0f}; }
+};
+template <>
+struct Raw2<bf16> {
+  uint32_t u;
+  __device__ __forceinline__ void load(const HFM_GLOBAL bf16* p) { u = *(const HFM_GLOBAL uint32_t*)p; }
+  __device__ __forceinline__ void get(float v[2]) const {
+    v[0] = __uint_as_float(u << 16);
+    v[1] = __uint_as_float(u & 0xFFFF0000u);
+  }
+  __device__ __forceinline__ void zero() { u = 0u; }
+};
+template <>
+struct Raw2<f16> {
+  uint32_t u;
+  __device__ __forceinline__ void load(const HFM_GLOBAL f16* p) { u = *(const HFM_GLOBAL uint32_t*)p; }
+  __device__ __forceinline__ void get(float v[2]) const { v[0] = f16_lo(u); v[1] = f16_hi(u); }
+  __device__ __forceinline__ void zero() { u = 0u; }
+};
+
 // A run's pass-1 input as loaded: x, or x and the coarse correction below it
 // (x + alpha P ec is formed where it is used, by correct_val as SrcCorrect).
 template <typename T, bool CORR>
 struct XRun {
   Raw4<T> x;
-  float e[2];
+  Raw2<T> e;
   __device__ __forceinline__ void load(const HFM_GLOBAL T* xp, const HFM_GLOBAL T* ec, int n, int k,
                                        int j, int i0, int Hc, int Wc) {
     x.load(xp + n);
-    if (CORR) ld2(ec + ((k >> 1) * Hc + (j >> 1)) * Wc + (i0 >> 1), e);
+    if (CORR) e.load(ec + ((k >> 1) * Hc + (j >> 1)) * Wc + (i0 >> 1));
   }
   // a plane past the grid's ends: get() then yields +0 (x + alpha * 0 = +0)
   __device__ __forceinline__ void zero() {
     x.zero();
-    e[0] = e[1] = 0.0f;
+    e.zero();
   }
   __device__ __forceinline__ void get(float v[4], float alpha) const {
     x.get(v);
     if (CORR) {
+      float ev[2];
+      e.get(ev);
 #pragma unroll
-      for (int m = 0; m < 4; ++m) v[m] = correct_val(v[m], e[m >> 1], alpha);
+      for (int m = 0; m < 4; ++m) v[m] = correct_val(v[m], ev[m >> 1], alpha);
     }
   }
 };
@@ -854,10 +885,8 @@ struct PlaneIn {
   XRun<T, CORR> edge;  // pass-1 input of the row past the tile's edge (edge rows only)
   Raw4<T> cx, cy, cu, b, cn;  // faces of plane kk; cn: the north faces (from memory
                               // for edge and boundary rows, else from LDS)
-  float ef;
+  T ef;                       // the east face of the run's last cell, as loaded
   Raw4<T> y;                  // pass 1's result on plane kk
-  float d[kPairRun];          // the diagonal on plane kk (kept for pass 2 where
-                              // registers allow: smooth + restrict)
 };
 
 // Sum of v over a kPairBlock-thread block (valid in thread 0).
@@ -875,15 +904,26 @@ __device__ __forceinline__ double pair_block_sum(double v) {
   return s;
 }
 
-// The pair's LDS: lx / ly end in a run of zeros (at kPairPlane), which a
+// The pair's LDS.  A plane's rows are laid out from row 0, or, in a tile
+// with halo rows, from row 1: rows 0 and NR + 1 then hold the rows just past
+// the tile's first and last rows, which the threads of those rows load and
+// put there.  A tile with halo rows has at most kPairBlock / 4 threads per
+// row, so W <= kPairBlock and the NR + 2 rows take at most kPairPlane +
+// kPairEdge cells.  lx / ly end in a run of zeros (at kPairZero), which a
 // thread reads in place of a neighbour past the grid's edge: the masked
 // form's +0, without a select per cell.
+constexpr int kPairEdge = 2 * kPairBlock;
+constexpr int kPairZero = kPairPlane + kPairEdge;
 template <typename T, bool S2>
 struct PairLds {
-  float lx[2][kPairPlane + kPairRun];  // pass-1 input of the plane
-  T lc[2][kPairPlane];                 // south faces of the plane
-  T ly[2][kPairPlane + kPairRun];      // pass-1 result of the plane
+  float lx[2][kPairZero + kPairRun];  // pass-1 input of the plane
+  float ly[2][kPairZero + kPairRun];  // pass-1 result of the plane, rounded to T
+  float ld[2][kPairPlane + kPairEdge / 2];  // the plane's diagonal, from pass 1 to pass 2:
+                                      // each thread's run at its own lx offset (no barrier,
+                                      // no registers; rows 0 .. NR of the layout)
   float lr[2][S2 ? 1 : kPairPlane / 2];  // odd rows' residuals (restriction)
+  double acc[S2 ? kPairBlock : 1];       // each thread's r.z partial (its own slot: no
+                                         // barrier; in LDS, it costs the walk no registers)
 };
 
 template <typename T, bool S2, typename O, bool DOT>
@@ -893,10 +933,10 @@ __device__ __forceinline__ void smooth_pair_body(const hfm_mg_task* __restrict__
                                                  float omega, float alpha, double* dots,
                                                  PairLds<T, S2>& lds, const Blk& bk) {
   constexpr int N = kPairRun;
-  constexpr int kZero = kPairPlane;
+  constexpr int kZero = kPairZero;
   auto& lx = lds.lx;
-  auto& lc = lds.lc;
   auto& ly = lds.ly;
+  auto& ld = lds.ld;
   auto& lr = lds.lr;
   // Tiles are numbered contiguously over the launch's tasks (a task's tiles
   // are blk0 .. blk0 + nblk - 1); each XCD takes a contiguous range of them,
@@ -931,23 +971,27 @@ __device__ __forceinline__ void smooth_pair_body(const hfm_mg_task* __restrict__
   const bool live = r < NR && j >= 0 && j < H;
   const bool outrow = live && r >= jh && r < jh + Rb;
   const int jc = min(max(j, 0), H - 1);
-  const int li = live ? r * W + i0 : 0;
+  const int li = live ? (r + jh) * W + i0 : 0;
   const bool south = j > 0, north = j + 1 < H, west = i0 > 0, east = i0 + N < W;
-  // the rows next to the tile's first and last rows are not in LDS
+  // the rows next to the tile's first and last rows are not in LDS: the
+  // threads of those rows load them (every other thread loads its own row
+  // in their place, so that every thread issues the same loads)
   const bool edge_s = r == 0 && south, edge_n = r + 1 == NR && north;
   const int je = edge_s ? jc - 1 : (edge_n ? jc + 1 : jc);
   // LDS offsets of the in-plane neighbours (rows r - 1 / r + 1, the cells
-  // before / after the run), or of the zeros where the grid ends there
+  // before / after the run), or of the zeros where the grid ends there or
+  // the neighbour row comes from memory
   const int os = south ? li - W : kZero, on = north ? li + W : kZero;
   const int ow = west ? li - 1 : kZero, oe = east ? li + N : kZero;
   if (bk.tid < N) {  // read after the walk's first barrier
     lx[0][kZero + bk.tid] = 0.0f;
     lx[1][kZero + bk.tid] = 0.0f;
-    ly[0][kZero + bk.tid] = from_f<T>(0.0f);
-    ly[1][kZero + bk.tid] = from_f<T>(0.0f);
+    ly[0][kZero + bk.tid] = 0.0f;
+    ly[1][kZero + bk.tid] = 0.0f;
   }
   const int k0 = zb * Z, k1 = min(k0 + Z, D);  // output planes [k0, k1)
   const int ks = max(k0 - 1, 0);
+  const int kmax = min(k1, D - 1);  // the last plane the walk loads
   const HFM_GLOBAL T* xin = gin<T>(t.xin);
   const HFM_GLOBAL T* ec = gin<T>(t.ec);
   const HFM_GLOBAL T* cx = gin<T>(t.c[0]);
@@ -956,21 +1000,28 @@ __device__ __forceinline__ void smooth_pair_body(const hfm_mg_task* __restrict__
   const HFM_GLOBAL T* b = gin<T>(t.b);
   const int Hc = t.Hc, Wc = t.Wc;
   auto cell = [&](int k) { return (k * H + jc) * W + i0; };
+  const HFM_GLOBAL T* bxh = gin<T>(t.c[3]);
+  const HFM_GLOBAL T* byh = gin<T>(t.c[4]);
+  const HFM_GLOBAL T* bzh = gin<T>(t.c[5]);
+  // The loads of a plane: every thread issues all of them, unconditionally,
+  // from in-bounds addresses (rows outside the tile read the clamped row jc),
+  // and every value goes into its record as loaded.  So the compiler counts
+  // the loads in flight exactly and waits for a plane's loads only where
+  // they are first used, a plane later; a load under a branch, or a
+  // conversion or select at the load, makes it wait for every load at once.
+  // On the last plane the x-row above is read from the plane itself and
+  // taken as +0 where it is used.
   auto load_plane = [&](PlaneIn<T, S2>& pi, int kk) {
     const int n = cell(kk);
     const bool up1 = kk + 1 < D;
     pi.xn.load(xin, ec, up1 ? n + HW : n, up1 ? kk + 1 : kk, jc, i0, Hc, Wc);
-    if (!up1) pi.xn.zero();
     pi.cx.load(cx + n);
     pi.cy.load(cy + n);
-    pi.cu.load(up1 ? cz + n + HW : gin<T>(t.c[5]) + jc * W + i0);
+    pi.cu.load(up1 ? cz + n + HW : bzh + jc * W + i0);
     pi.b.load_nt(b + n);
-    pi.ef = ldg(east ? cx + n + N : gin<T>(t.c[3]) + kk * H + jc);
-    if (edge_s || edge_n) pi.edge.load(xin, ec, n + (je - jc) * W, kk, je, i0, Hc, Wc);
-    if (!north)
-      pi.cn.load(gin<T>(t.c[4]) + kk * W + i0);
-    else if (edge_n)
-      pi.cn.load(cy + n + W);
+    pi.ef = ldraw(east ? cx + n + N : bxh + kk * H + jc);
+    pi.edge.load(xin, ec, n + (je - jc) * W, kk, je, i0, Hc, Wc);
+    pi.cn.load(north ? cy + n + W : byh + kk * W + i0);
   };
 
   // The walk keeps three plane records in rotation (previous, current, the
@@ -987,7 +1038,7 @@ __device__ __forceinline__ void smooth_pair_body(const hfm_mg_task* __restrict__
   for (int m = 0; m < N; ++m) ev0[m] = 0.0f;
   pa.y.set(ev0);  // zeros
   pc.y.set(ev0);
-  if (live) {
+  {
     // pa stands for plane ks - 1 (its x-row is x(ks), its up faces cz(ks));
     // pb is plane ks.  The pass-1 input on plane ks - 1 goes to the LDS
     // buffer that step ks reads its down neighbours from (a thread reads
@@ -1001,12 +1052,12 @@ __device__ __forceinline__ void smooth_pair_body(const hfm_mg_task* __restrict__
 #pragma unroll
       for (int m = 0; m < N; ++m) xd[m] = 0.0f;
     }
-    lds_st4(&lx[(ks & 1) ^ 1][li], xd);
+    if (live) lds_st4(&lx[(ks & 1) ^ 1][li], xd);
     pa.xn.load(xin, ec, cell(ks), ks, jc, i0, Hc, Wc);
     pa.cu.load(cz + cell(ks));
     load_plane(pb, ks);
   }
-  double acc = 0.0;
+  if (DOT) lds.acc[bk.tid] = 0.0;
 
   auto step = [&](int kk, PlaneIn<T, S2>& prv, PlaneIn<T, S2>& cur, PlaneIn<T, S2>& spare) {
     const bool p1 = kk < D;  // pass 1 runs on plane kk (uniform)
@@ -1014,64 +1065,67 @@ __device__ __forceinline__ void smooth_pair_body(const hfm_mg_task* __restrict__
     // what the pass pair still needs of plane kk - 2 before its record is reused
     const Raw4<T> pcd = spare.cu;      // down faces of plane kk - 1
     const Raw4<T> ym2 = spare.y;       // pass 1 on plane kk - 2
-    if (live && kk + 1 < D && kk + 1 <= k1) load_plane(spare, kk + 1);
-    float xc[N];
+    float xc[N], yu[N];
     if (p1 && live) {
       prv.xn.get(xc, alpha);
       lds_st4(&lx[buf][li], xc);
-      cur.cy.lds_store(&lc[buf][li]);
+      if (edge_s || edge_n) {  // the row past the tile's edge, from memory
+        float ve[N];
+        cur.edge.get(ve, alpha);
+        lds_st4(&lx[buf][edge_s ? os : on], ve);
+      }
     }
+    // the next plane (past the tile's or the grid's last one: the last one
+    // again, whose values are not used), after this plane's edge row is used,
+    // so that the two records' edge rows need no registers at the same time
+    load_plane(spare, min(kk + 1, kmax));
     __syncthreads();
     if (p1 && live) {
-      if (north && !edge_n) cur.cn.lds_load(&lc[buf][li + W]);
       float vs[N], vn[N], vd[N], vu[N], d[N], off[N];
-      // the rows past the tile's first and last rows come from memory
-      if (edge_s) cur.edge.get(vs, alpha);
-      else lds_ld4(&lx[buf][os], vs);
-      if (edge_n) cur.edge.get(vn, alpha);
-      else lds_ld4(&lx[buf][on], vn);
+      lds_ld4(&lx[buf][os], vs);
+      lds_ld4(&lx[buf][on], vn);
       lds_ld4(&lx[buf ^ 1][li], vd);  // this run on plane kk - 1
       cur.xn.get(vu, alpha);
+      if (kk + 1 >= D) {  // past the grid's last plane (uniform): +0
+#pragma unroll
+        for (int m = 0; m < N; ++m) vu[m] = 0.0f;
+      }
       const float wv = lx[buf][ow];
       const float ev = lx[buf][oe];
-      stencil_run<T>(cur.cx, cur.ef, cur.cy, cur.cn, prv.cu, cur.cu, xc, wv, ev, vs, vn, vd, vu,
-                     true, true, true, true, d, off);
-      if constexpr (!S2) {
-#pragma unroll
-        for (int m = 0; m < N; ++m) cur.d[m] = d[m];
-      }
+      stencil_run<T>(cur.cx, to_f(cur.ef), cur.cy, cur.cn, prv.cu, cur.cu, xc, wv, ev, vs, vn, vd,
+                     vu, true, true, true, true, d, off);
+      lds_st4(&ld[buf][li], d);  // for pass 2 on this plane, a step later
       float bv[N], y[N];
       cur.b.get(bv);
 #pragma unroll
       for (int m = 0; m < N; ++m) y[m] = jacobi_val(xc[m], d[m], bv[m], off[m], omega);
       // y is rounded to T where it is packed (the record, the store): one
-      // conversion per two cells, no round trip through a float
+      // conversion per two cells; pass 2 takes the rounded values as floats
+      // (from LDS, or yu for this run)
       cur.y.set(y);
-      if (!S2 && outrow && kk >= k0 && kk < k1) st4_nt(gout<T>(t.xout) + cell(kk), y);
-      cur.y.lds_store(&ly[buf][li]);
+      if (!S2 && outrow && kk >= k0 && kk < k1) cur.y.store_nt(gout<T>(t.xout) + cell(kk));
+      cur.y.get(yu);
+      lds_st4(&ly[buf][li], yu);
     } else {
-      const float zero[N] = {};
-      cur.y.set(zero);  // plane kk is past the last (its pass-1 value is unused)
+#pragma unroll
+      for (int m = 0; m < N; ++m) yu[m] = 0.0f;
+      cur.y.set(yu);  // plane kk is past the last (its pass-1 value is unused)
     }
     if (kk - 1 >= k0) {  // pass 2 on plane kp (uniform)
       const int kp = kk - 1;
       const int pb2 = kp & 1;
       float e[N];
       if (outrow) {
-        float yc[N], vs[N], vn[N], vd[N], vu[N], d[N], off[N], bv[N];
-        prv.y.get(yc);
+        float yc[N], vs[N], vn[N], vd[N], d[N], off[N], bv[N];
+        lds_ld4(&ly[pb2][li], yc);
         ym2.get(vd);
-        cur.y.get(vu);
         lds_ld4(&ly[pb2][os], vs);
         lds_ld4(&ly[pb2][on], vn);
-        const float wv = lds_ld1(&ly[pb2][ow]);
-        const float ev = lds_ld1(&ly[pb2][oe]);
-        if constexpr (!S2) {
-#pragma unroll
-          for (int m = 0; m < N; ++m) d[m] = prv.d[m];
-        }
-        stencil_run<T, S2>(prv.cx, prv.ef, prv.cy, prv.cn, pcd, prv.cu, yc, wv, ev, vs, vn, vd,
-                           vu, true, true, true, true, d, off);
+        lds_ld4(&ld[pb2][li], d);
+        const float wv = ly[pb2][ow];
+        const float ev = ly[pb2][oe];
+        stencil_run<T, false>(prv.cx, to_f(prv.ef), prv.cy, prv.cn, pcd, prv.cu, yc, wv, ev, vs,
+                              vn, vd, yu, true, true, true, true, d, off);
         prv.b.get(bv);
         if constexpr (S2) {
           float o[N];
@@ -1082,8 +1136,10 @@ __device__ __forceinline__ void smooth_pair_body(const hfm_mg_task* __restrict__
           ro.store_nt(gout<O>(t.xout) + cell(kp));
           if (DOT) {
             ro.get(o);
+            double acc = lds.acc[bk.tid];
 #pragma unroll
             for (int m = 0; m < N; ++m) acc += double(bv[m]) * double(o[m]);
+            lds.acc[bk.tid] = acc;
           }
         } else {
 #pragma unroll
@@ -1151,7 +1207,7 @@ __device__ __forceinline__ void smooth_pair_body(const hfm_mg_task* __restrict__
     if (kk + 2 <= k1) step(kk + 2, pc, pa, pb);
   }
   if (DOT) {
-    const double s = pair_block_sum(acc);
+    const double s = pair_block_sum(lds.acc[bk.tid]);
     if (bk.tid == 0) dots[t.slot0 + lb] = s;
   }
 }

@@ -831,9 +831,11 @@ def update_traffic():
 def traffic_driver(args):
     """Torch-free driver for hardware-counter runs of the dominant kernel,
     k_fcg_update (the _sweep form the solver launches, BF16 storage), or with
-    ``--traffic-kernel sr`` of the fine level's fused pre-smoothing launch
-    k_smooth_pair<bf16, false> (Jacobi sweep + residual restriction), or with
-    ``--traffic-kernel dir`` of the direction pass k_fcg_dir<bf16>.
+    ``--traffic-kernel sr`` / ``s2`` of the fine level's fused pre-smoothing
+    launch k_smooth_pair<T, false> (Jacobi sweep + residual restriction) /
+    post-smoothing launch k_smooth_pair<T, true> (CORRECT + PLAIN sweeps, r.z
+    partials), T = ``--traffic-storage``, or with ``--traffic-kernel dir`` of
+    the direction pass k_fcg_dir<bf16>.
 
     The work tables and buffers of the headline batch (``--instances`` shapes
     from ``--seed``, the same draw as the solve workload) or, with
@@ -872,8 +874,8 @@ def traffic_driver(args):
         what = f"{args.instances} instances, edges from seed {args.seed}"
     n = len(shapes)
     cells = np.array([D * H * W for (D, H, W) in shapes], dtype=np.int64)
-    if args.traffic_kernel == "sr":
-        return _traffic_sr(args, shapes, cells, what, malloc, upload, hip)
+    if args.traffic_kernel in ("sr", "s2"):
+        return _traffic_pair(args, shapes, cells, what, malloc, upload, hip)
     nblk = ((cells + CHUNK - 1) // CHUNK + 7) // 8 * 8
     blk0 = np.concatenate([[0], np.cumsum(nblk)[:-1]])
     total = int(nblk.sum())
@@ -967,12 +969,17 @@ def _copy_reference(hip, malloc, reps):
     return nbytes, _hip_timed(hip, cp, reps)
 
 
-def _traffic_sr(args, shapes, cells, what, malloc, upload, hip):
-    """The fused pre-smoothing launch of the fine level over the batch: x1 =
-    Jacobi(x0, r) and the coarse right-hand side of r - A x1, one tile per
-    512-thread block (tile geometry from hfm_mg_tile), BF16 storage."""
+def _traffic_pair(args, shapes, cells, what, malloc, upload, hip):
+    """A fused pass pair of the fine level over the batch, one tile per
+    512-thread block (tile geometry from hfm_mg_tile), in ``--traffic-storage``:
+    sr, x1 = Jacobi(x0, r) and the coarse right-hand side of r - A x1; s2, the
+    CORRECT sweep (x0 + alpha P ec) followed by a PLAIN sweep, z stored in the
+    same type, with the r.z partials."""
     import ctypes as C
     from hfm import _lib as L
+    s2 = args.traffic_kernel == "s2"
+    st = args.traffic_storage
+    T = {"bf16": 2, "f16": 2, "f32": 4}[st]
     tiles = [L.tile(s) for s in shapes]
     if any(tl is None for tl in tiles):
         raise SystemExit("a level of the workload does not tile")
@@ -988,30 +995,45 @@ def _traffic_sr(args, shapes, cells, what, malloc, upload, hip):
         r = t[q]
         r["D"], r["H"], r["W"], r["inst"] = D, H, W, q
         r["blk0"], r["nblk"], r["tile_rows"], r["tile_planes"] = blk0[q], nt, R, Z
+        r["slot0"] = blk0[q]
         r["magW"], r["magH"] = L.magic(W), L.magic(H)
         r["Dc"], r["Hc"], r["Wc"] = Dc, Hc, Wc
         r["magWc"], r["magHc"] = L.magic(Wc), L.magic(Hc)
-        r["c"] = [malloc(2 * N) for _ in range(3)] + [malloc(2 * D * H), malloc(2 * D * W),
-                                                      malloc(2 * H * W)]
-        r["xin"], r["xout"], r["b"] = malloc(2 * N), malloc(2 * N), malloc(2 * N)
-        r["rlow"] = malloc(2 * Dc * Hc * Wc)
+        r["c"] = [malloc(T * N) for _ in range(3)] + [malloc(T * D * H), malloc(T * D * W),
+                                                      malloc(T * H * W)]
+        r["xin"], r["xout"], r["b"] = malloc(T * N), malloc(T * N), malloc(T * N)
+        if s2:
+            r["ec"] = malloc(T * Dc * Hc * Wc)
+        else:
+            r["rlow"] = malloc(T * Dc * Hc * Wc)
     tasks = upload(t.view(np.uint8))
     bt = np.full(padded, 0xFFFF, dtype=np.uint16)
     bt[:total] = np.repeat(np.arange(n, dtype=np.uint16), ntile)
     blk = upload(bt)
     sc = np.zeros(n, dtype=L.SCALARS_DTYPE)
-    sc["active"] = 1
+    sc["active"], sc["slot0"], sc["nslot"] = 1, blk0, ntile
     scp = upload(sc.view(np.uint8))
+    dots = malloc(8 * padded)
     hip.hipDeviceSynchronize()
-    sr = lambda: L.call("hfm_mg_smooth_restrict", tasks, blk, padded, scp,  # noqa: E731
-                        L.STORE["bf16"], C.c_float(2.0 / 3.0), None)
-    ms = _hip_timed(hip, sr, args.reps)
+    om = C.c_float(2.0 / 3.0)
+    if s2:
+        fn = lambda: L.call("hfm_mg_smooth2", tasks, blk, padded, scp, L.STORE[st],  # noqa: E731
+                            L.STORE[st], om, C.c_float(1.6), dots, None)
+    else:
+        fn = lambda: L.call("hfm_mg_smooth_restrict", tasks, blk, padded, scp,  # noqa: E731
+                            L.STORE[st], om, None)
+    ms = _hip_timed(hip, fn, args.reps)
     nbytes, cp_ms = _copy_reference(hip, malloc, args.reps)
     N = int(cells.sum())
-    # x0, r, three faces read; x1 written (2 B each); the coarse right-hand
-    # side written (2 B per 8 cells)
-    rd, wr = 10 * N, 2 * N + N // 4
-    print(json.dumps({"kernel": "k_smooth_pair<bf16, false, bf16, false>", "workload": what,
+    # x0, r, three faces read and x1 written (T bytes each); sr: the coarse
+    # right-hand side written, s2: the coarse correction read (T per 8 cells)
+    rd, wr = 5 * T * N, T * N
+    if s2:
+        rd += T * N // 8
+    else:
+        wr += T * N // 8
+    print(json.dumps({"kernel": f"k_smooth_pair<{st}, {str(s2).lower()}, {st}, {str(s2).lower()}>",
+                      "workload": what,
                       "cells": N, "tiles": total, "blocks": padded, "launches": args.reps + 1,
                       "alg_bytes_per_launch": rd + wr, "alg_read_bytes": rd, "alg_write_bytes": wr,
                       "launch_ms": round(ms, 4),
@@ -1027,8 +1049,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--instances", type=int, default=512)
     ap.add_argument("--seed", type=int, default=2024)
-    ap.add_argument("--storage", default="bf16", choices=["bf16", "f32"])
-    ap.add_argument("--outer", default=None, choices=["bf16", "f32"],
+    ap.add_argument("--storage", default="bf16", choices=["bf16", "f16", "f32"])
+    ap.add_argument("--outer", default=None, choices=["bf16", "f16", "f32"],
                     help="storage of z and the search direction (default: --storage)")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -1088,9 +1110,11 @@ def main():
     ap.add_argument("--traffic-edge", type=int, default=0)
     ap.add_argument("--traffic-lag", type=int, choices=(0, 1), default=1,
                     help="update variant: 1 with the deferred solution update, 0 without")
-    ap.add_argument("--traffic-kernel", choices=("update", "sr", "dir"), default="update",
-                    help="update: k_fcg_update; sr: the fine level's fused pre-smoothing "
-                         "launch; dir: k_fcg_dir")
+    ap.add_argument("--traffic-kernel", choices=("update", "sr", "s2", "dir"), default="update",
+                    help="update: k_fcg_update; sr / s2: the fine level's fused pre- / "
+                         "post-smoothing launch; dir: k_fcg_dir")
+    ap.add_argument("--traffic-storage", choices=("bf16", "f16", "f32"), default="bf16",
+                    help="storage type of the fused pass pairs (sr, s2)")
     args = ap.parse_args()
     if args.traffic:
         return traffic_driver(args)
