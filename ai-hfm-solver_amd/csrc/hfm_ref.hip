@@ -199,13 +199,30 @@ dim3 grid_col(const Grid3& g) {
               unsigned((g.D + kKZ - 1) / kKZ));
 }
 
-template <int RY>
+// What a window fetches per cell: the loaded values, kept as loaded until the
+// plane is committed a plane later (a conversion, select or sum at the load
+// would make the walk wait for its loads at once), and whether the cell
+// counts (else it is taken as +0).  Fetch1: the value a; Fetch2: a - b.
+struct Fetch1 {
+  float a;
+  bool keep;
+  __device__ __forceinline__ float value() const { return a; }
+};
+struct Fetch2 {
+  float a, b;
+  bool keep;
+  __device__ __forceinline__ float value() const { return a - b; }
+};
+
+template <int RY, typename FT = Fetch1>
 struct Win27 {
   static constexpr int R = RY + 2;
   float l[3][R], m[3][R], r[3][R];  // [plane][row], x offsets -1, 0, +1 (zero outside)
-  float sv[R];                      // a fetched plane's rows at this lane's column
+  FT sv[R];                         // a fetched plane's rows at this lane's column
 
-  // issue the loads of plane kk of the field xat(kk, jj, ii) (no waiting)
+  // issue the loads of plane kk of the field xat(kk, jj, ii) (no waiting):
+  // every load is issued, from in-bounds addresses, and taken as +0 where the
+  // cell is outside the grid
   template <typename XF>
   __device__ __forceinline__ void fetch(const Grid3& g, const Col& c, int kk, XF xat) {
     const bool kin = unsigned(kk) < unsigned(g.D);
@@ -215,17 +232,18 @@ struct Win27 {
     for (int q = 0; q < R; ++q) {
       const int jj = c.j - 1 + q;
       const bool in = kin & (unsigned(jj) < unsigned(g.H)) & c.in;
-      const float x = xat(kc, in ? jj : c.j, ic);
-      sv[q] = in ? x : 0.0f;
+      sv[q] = xat(kc, in ? jj : c.j, ic);
+      sv[q].keep = sv[q].keep & in;
     }
   }
   // plane p of the window <- the fetched plane (x neighbours from the adjacent lanes)
   __device__ __forceinline__ void commit(int p, const Col&) {
 #pragma unroll
     for (int q = 0; q < R; ++q) {
-      l[p][q] = __shfl_up(sv[q], 1, 64);    // the end lanes' outer values are unused
-      m[p][q] = sv[q];
-      r[p][q] = __shfl_down(sv[q], 1, 64);
+      const float x = sv[q].keep ? sv[q].value() : 0.0f;
+      l[p][q] = __shfl_up(x, 1, 64);    // the end lanes' outer values are unused
+      m[p][q] = x;
+      r[p][q] = __shfl_down(x, 1, 64);
     }
   }
   __device__ __forceinline__ void shift() {
@@ -265,6 +283,20 @@ struct Win27 {
   }
 };
 
+// The thread's own cells of rows c.j .. c.j + RY - 1 at plane k of field p,
+// from in-bounds addresses (a row or plane past the grid re-reads the last
+// one).  The walkers load them a plane ahead, as the window's planes: a value
+// loaded in one step and used in the same step makes the walk wait there for
+// every load in flight, the window's next plane included.
+template <int RY>
+__device__ __forceinline__ void own_cells(const Grid3& g, const Col& c, int k, const float* p,
+                                          float v[RY]) {
+  const int ic = c.in ? c.i : 0;
+  const int kc = min(k, g.D - 1);
+#pragma unroll
+  for (int ry = 0; ry < RY; ++ry) v[ry] = p[g.idx(kc, min(c.j + ry, g.H - 1), ic)];
+}
+
 // (x - A/d) + b/d, the reference's evaluation order
 __device__ __forceinline__ float jacobi_form(float x, float ax, float b, float d) {
   return (x - ax / d) + b / d;
@@ -292,12 +324,16 @@ __global__ __launch_bounds__(kBX* kBY) void k_conv27(const float* __restrict__ x
                                                      float d, Grid3 g, int form) {
   Col c;
   if (!col_begin<2>(g, c)) return;
-  auto xat = [&](int kk, int jj, int ii) { return x[g.idx(kk, jj, ii)]; };
+  auto xat = [&](int kk, int jj, int ii) { return Fetch1{x[g.idx(kk, jj, ii)], true}; };
   const int f = form & HFM_FORM_MASK;
+  const float* bp = f == HFM_FORM_APPLY ? x : b;  // APPLY reads no b (x stands in)
   Win27<2> win;
   win.start(g, c, xat);
+  float bv[2], bn[2];
+  own_cells<2>(g, c, c.k0, bp, bv);
   for (int k = c.k0; k < c.k1; ++k) {
     win.advance(g, c, k, xat);
+    own_cells<2>(g, c, k + 1, bp, bn);
 #pragma unroll
     for (int ry = 0; ry < 2; ++ry) {
       float v[27];
@@ -305,17 +341,20 @@ __global__ __launch_bounds__(kBX* kBY) void k_conv27(const float* __restrict__ x
       const float ax = wsum<TAPS>(K, v);
       if (c.out && c.j + ry < g.H) {
         const int64_t n = g.idx(k, c.j + ry, c.i);
+        const float bc = bv[ry];
         float out = ax;
-        if (f == HFM_FORM_RESID) out = ax - b[n];
-        else if (f == HFM_FORM_JACOBI) out = jacobi_form(win.centre(ry), ax, b[n], d);
-        else if (f == HFM_FORM_SUBTRACT) out = b[n] - ax;
-        else if (f == HFM_FORM_AXPY) out = b[n] + d * ax;
-        else if (f == HFM_FORM_JACOBI_MUL) out = jacobi_mul_form(win.centre(ry), ax, b[n], d);
+        if (f == HFM_FORM_RESID) out = ax - bc;
+        else if (f == HFM_FORM_JACOBI) out = jacobi_form(win.centre(ry), ax, bc, d);
+        else if (f == HFM_FORM_SUBTRACT) out = bc - ax;
+        else if (f == HFM_FORM_AXPY) out = bc + d * ax;
+        else if (f == HFM_FORM_JACOBI_MUL) out = jacobi_mul_form(win.centre(ry), ax, bc, d);
         if ((form & HFM_FORM_ZERO_OUTLET) && c.i == g.W - 1) out = 0.0f;
         y[n] = out;
       }
     }
     win.shift();
+    bv[0] = bn[0];
+    bv[1] = bn[1];
   }
 }
 
@@ -426,11 +465,14 @@ __global__ __launch_bounds__(kBX* kBY) void k_prolong_jacobi_col(const float* __
                                                                  int fz, int fy, int fx) {
   Col c;
   if (!col_begin<2>(gf, c)) return;
-  auto wat = [&](int kk, int jj, int ii) { return wc[gc.idx(kk / fz, jj / fy, ii / fx)]; };
+  auto wat = [&](int kk, int jj, int ii) { return Fetch1{wc[gc.idx(kk / fz, jj / fy, ii / fx)], true}; };
   Win27<2> win;
   win.start(gf, c, wat);
+  float rv[2], rn[2];
+  own_cells<2>(gf, c, c.k0, r, rv);
   for (int k = c.k0; k < c.k1; ++k) {
     win.advance(gf, c, k, wat);
+    own_cells<2>(gf, c, k + 1, r, rn);
 #pragma unroll
     for (int ry = 0; ry < 2; ++ry) {
       float v[27];
@@ -438,10 +480,12 @@ __global__ __launch_bounds__(kBX* kBY) void k_prolong_jacobi_col(const float* __
       const float ax = wsum<TAPS>(K, v);
       if (c.out && c.j + ry < gf.H) {
         const int64_t n = gf.idx(k, c.j + ry, c.i);
-        y[n] = jacobi_form(win.centre(ry), ax, r[n], d);
+        y[n] = jacobi_form(win.centre(ry), ax, rv[ry], d);
       }
     }
     win.shift();
+    rv[0] = rn[0];
+    rv[1] = rn[1];
   }
 }
 
@@ -467,14 +511,18 @@ __global__ __launch_bounds__(kBX* kBY) void k_correct_jacobi_col(const float* __
   Col c;
   if (!col_begin<2>(g, c)) return;
   const bool zo = flags & HFM_CORRECT_ZERO_OUTLET;
+  // x - w, or 0 at the outlet (HFM_CORRECT_ZERO_OUTLET)
   auto vat = [&](int kk, int jj, int ii) {
     const int64_t m = g.idx(kk, jj, ii);
-    return (zo && ii == g.W - 1) ? 0.0f : x[m] - cw.template at<PROL>(g, kk, jj, ii);
+    return Fetch2{x[m], cw.template at<PROL>(g, kk, jj, ii), !(zo && ii == g.W - 1)};
   };
-  Win27<2> win;
+  Win27<2, Fetch2> win;
   win.start(g, c, vat);
+  float bv[2], bn[2];
+  own_cells<2>(g, c, c.k0, b, bv);
   for (int k = c.k0; k < c.k1; ++k) {
     win.advance(g, c, k, vat);
+    own_cells<2>(g, c, k + 1, b, bn);
 #pragma unroll
     for (int ry = 0; ry < 2; ++ry) {
       float v[27];
@@ -482,11 +530,13 @@ __global__ __launch_bounds__(kBX* kBY) void k_correct_jacobi_col(const float* __
       const float ax = wsum<TAPS>(K, v);
       if (c.out && c.j + ry < g.H) {
         const int64_t n = g.idx(k, c.j + ry, c.i);
-        y[n] = (flags & HFM_CORRECT_RECIPROCAL) ? jacobi_mul_form(win.centre(ry), ax, b[n], d)
-                                                : jacobi_form(win.centre(ry), ax, b[n], d);
+        y[n] = (flags & HFM_CORRECT_RECIPROCAL) ? jacobi_mul_form(win.centre(ry), ax, bv[ry], d)
+                                                : jacobi_form(win.centre(ry), ax, bv[ry], d);
       }
     }
     win.shift();
+    bv[0] = bn[0];
+    bv[1] = bn[1];
   }
 }
 
@@ -543,18 +593,22 @@ __global__ __launch_bounds__(kBX* kBY) void k_pg_coef(const float* __restrict__ 
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
     const float* x = f[q];
-    win[q].start(g, c, [&](int kk, int jj, int ii) { return x[g.idx(kk, jj, ii)]; });
+    win[q].start(g, c, [&](int kk, int jj, int ii) { return Fetch1{x[g.idx(kk, jj, ii)], true}; });
   }
+  float sv[1], sn[1];
+  own_cells<1>(g, c, c.k0, sigma, sv);
   for (int k = c.k0; k < c.k1; ++k) {
     const int64_t n = g.idx(k, c.j, c.in ? c.i : 0);
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
       const float* x = f[q];
-      win[q].advance(g, c, k, [&](int kk, int jj, int ii) { return x[g.idx(kk, jj, ii)]; });
+      win[q].advance(g, c, k, [&](int kk, int jj, int ii) { return Fetch1{x[g.idx(kk, jj, ii)], true}; });
     }
+    own_cells<1>(g, c, k + 1, sigma, sn);
     const float speed =
         (1.0f / 3.0f) * ((fabsf(win[0].centre(0)) + fabsf(win[1].centre(0))) + fabsf(win[2].centre(0)));
-    const float damp = 1.0f + dt * sigma[n];
+    const float damp = 1.0f + dt * sv[0];
+    sv[0] = sn[0];
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
       float nb[27];
@@ -588,8 +642,8 @@ __global__ __launch_bounds__(kBX* kBY) void k_pg_flux(const float* f0, const flo
   for (int q = 0; q < 3; ++q) {
     const float* x = f[q];
     const float* y = kf[q];
-    auto xat = [&](int kk, int jj, int ii) { return x[g.idx(kk, jj, ii)]; };
-    auto yat = [&](int kk, int jj, int ii) { return y[g.idx(kk, jj, ii)]; };
+    auto xat = [&](int kk, int jj, int ii) { return Fetch1{x[g.idx(kk, jj, ii)], true}; };
+    auto yat = [&](int kk, int jj, int ii) { return Fetch1{y[g.idx(kk, jj, ii)], true}; };
     Win27<1> wf, wk;
     wf.start(g, c, xat);
     wk.start(g, c, yat);
@@ -635,7 +689,7 @@ __global__ __launch_bounds__(kBX* kBY) void k_advect27_v(
 #pragma unroll
   for (int q = 0; q < NF; ++q) {
     const float* x = phi[q];
-    win[q].start(g, c, [&](int kk, int jj, int ii) { return x[g.idx(kk, jj, ii)]; });
+    win[q].start(g, c, [&](int kk, int jj, int ii) { return Fetch1{x[g.idx(kk, jj, ii)], true}; });
   }
   // the cell's own values (velocity, bases, dissipation, sigma); with three
   // fields loaded one plane ahead (the last plane's successor re-reads the
@@ -643,6 +697,8 @@ __global__ __launch_bounds__(kBX* kBY) void k_advect27_v(
   struct Cells {
     float u, v, w, b[NF], d[NF], s;
   };
+  const bool has_s = post && sigma;
+  const float* sg = has_s ? sigma : u;
   auto load_cells = [&](int kk, Cells& cv) {
     const int64_t m = g.idx(min(kk, g.D - 1), c.j, c.in ? c.i : 0);
     cv.u = u[m];
@@ -653,7 +709,7 @@ __global__ __launch_bounds__(kBX* kBY) void k_advect27_v(
       cv.b[q] = base[q][m];
       cv.d[q] = DIS ? dis[q][m] : 0.0f;
     }
-    cv.s = post ? sigma ? sigma[m] : 0.0f : 0.0f;
+    cv.s = sg[m];  // sigma (or u, unused): every load is issued
   };
   constexpr bool ahead = NF > 1;
   Cells cur, nxt;
@@ -665,7 +721,7 @@ __global__ __launch_bounds__(kBX* kBY) void k_advect27_v(
 #pragma unroll
     for (int q = 0; q < NF; ++q) {
       const float* x = phi[q];
-      win[q].advance(g, c, k, [&](int kk, int jj, int ii) { return x[g.idx(kk, jj, ii)]; });
+      win[q].advance(g, c, k, [&](int kk, int jj, int ii) { return Fetch1{x[g.idx(kk, jj, ii)], true}; });
     }
     if (ahead) load_cells(k + 1, nxt);
     // the cell's combined weights dif - u ax - v ay - w az, shared by the
@@ -695,7 +751,8 @@ __global__ __launch_bounds__(kBX* kBY) void k_advect27_v(
                             k == g.D - 1;
           if ((post & HFM_ADV_INDICATOR) && face) o = 0.0f;
           const int nb = (post >> HFM_ADV_BLUFF_SHIFT) & 3;
-          for (int r = 0; r < nb; ++r) o = bluff_apply(o, cur.s, dt);
+          const float s = has_s ? cur.s : 0.0f;
+          for (int r = 0; r < nb; ++r) o = bluff_apply(o, s, dt);
         }
         out[q][n] = o;
       }
@@ -724,7 +781,7 @@ __global__ __launch_bounds__(kBX* kBY) void k_conv27_sum_v(const float* x0, cons
 #pragma unroll
   for (int q = 0; q < NF; ++q) {
     const float* xq = xs[q];
-    win[q].start(g, c, [&](int kk, int jj, int ii) { return xq[g.idx(kk, jj, ii)]; });
+    win[q].start(g, c, [&](int kk, int jj, int ii) { return Fetch1{xq[g.idx(kk, jj, ii)], true}; });
   }
   // boundary_condition_source: the x = 0 cell takes the x = 1 cell's sum
   const bool src = (flags & HFM_SUM_SOURCE_BC) && g.W > 1;
@@ -733,7 +790,7 @@ __global__ __launch_bounds__(kBX* kBY) void k_conv27_sum_v(const float* x0, cons
 #pragma unroll
     for (int q = 0; q < NF; ++q) {
       const float* xq = xs[q];
-      win[q].advance(g, c, k, [&](int kk, int jj, int ii) { return xq[g.idx(kk, jj, ii)]; });
+      win[q].advance(g, c, k, [&](int kk, int jj, int ii) { return Fetch1{xq[g.idx(kk, jj, ii)], true}; });
     }
 #pragma unroll
     for (int q = 0; q < NF; ++q) {
@@ -769,25 +826,25 @@ __global__ __launch_bounds__(kBX* kBY) void k_grad3(const float* __restrict__ p,
   if (!col_begin<2>(g, c)) return;
   const bool pbc = flags & HFM_GRAD_P_BC;
   auto pat = [&](int kk, int jj, int ii) {
-    if (pbc) {
-      // boundary_condition_pressure: the last assignment covering the cell wins
-      // (x = W-1 -> 0, x = 0, y = 0, y = H-1, z = 0, z = D-1 copy their neighbour)
-      int k2 = kk, j2 = jj, i2 = ii;
-      bool zero = false;
-      if (kk == g.D - 1) k2 = g.D - 2;
-      else if (kk == 0) k2 = 1;
-      else if (jj == g.H - 1) j2 = g.H - 2;
-      else if (jj == 0) j2 = 1;
-      else if (ii == 0) i2 = 1;
-      else if (ii == g.W - 1) zero = true;
-      const float x = p[g.idx(k2, j2, i2)];
-      return zero ? 0.0f : x;
-    }
-    return p[g.idx(kk, jj, ii)];
+    // boundary_condition_pressure: the last assignment covering the cell wins
+    // (x = W-1 -> 0, x = 0, y = 0, y = H-1, z = 0, z = D-1 copy their
+    // neighbour), as selects: a branch between the walk's loads would make it
+    // wait for all of them
+    const bool kt = pbc & (kk == g.D - 1), kb = pbc & (kk == 0);
+    const bool kf = !(kt | kb);
+    const bool jt = kf & pbc & (jj == g.H - 1), jb = kf & pbc & (jj == 0);
+    const bool jf = kf & !(jt | jb);
+    const bool ib = jf & pbc & (ii == 0);
+    const bool zero = jf & pbc & !ib & (ii == g.W - 1);
+    const int k2 = kt ? g.D - 2 : (kb ? 1 : kk);
+    const int j2 = jt ? g.H - 2 : (jb ? 1 : jj);
+    const int i2 = ib ? 1 : ii;
+    return Fetch1{p[g.idx(k2, j2, i2)], !zero};
   };
   const float* fin[3] = {f0, f1, f2};
   float* out[3] = {o0, o1, o2};
   const bool damp = flags & (HFM_GRAD_PRE_BLUFF | HFM_GRAD_POST_BLUFF);
+  const float* sg = damp ? sigma : p;  // sigma, or p (unused): every load is issued
   // the cells' own values (f, sigma) of a plane, loaded one plane ahead from
   // in-bounds addresses (a row past the grid re-reads the last row)
   struct Cells {
@@ -801,7 +858,7 @@ __global__ __launch_bounds__(kBX* kBY) void k_grad3(const float* __restrict__ p,
       const int64_t n = g.idx(kc, min(c.j + ry, g.H - 1), ic);
 #pragma unroll
       for (int q = 0; q < 3; ++q) cv.f[ry][q] = fin[q][n];
-      cv.s[ry] = damp ? sigma[n] : 0.0f;
+      cv.s[ry] = sg[n];
     }
   };
   Win27<2> win;
