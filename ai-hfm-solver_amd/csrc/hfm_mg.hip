@@ -771,6 +771,13 @@ __device__ __forceinline__ void st4_nt(HFM_GLOBAL f16* p, const float v[4]) {
   __builtin_nontemporal_store(u2v{pack2h(v[0], v[1]), pack2h(v[2], v[3])}, (HFM_GLOBAL u2v*)p);
 }
 
+// base + off bytes: a uniform base with a 32-bit offset, which the loads take
+// as their SGPR base and VGPR offset (no 64-bit address arithmetic per load).
+template <typename T>
+__device__ __forceinline__ const HFM_GLOBAL T* byte_at(const HFM_GLOBAL T* base, uint32_t off) {
+  return (const HFM_GLOBAL T*)((const HFM_GLOBAL char*)base + off);
+}
+
 // Two values as loaded (the coarse correction under a run), unpacked where used.
 template <typename T>
 struct Raw2;
@@ -810,6 +817,11 @@ struct XRun {
                                        int j, int i0, int Hc, int Wc) {
     x.load(xp + n);
     if (CORR) e.load(ec + ((k >> 1) * Hc + (j >> 1)) * Wc + (i0 >> 1));
+  }
+  // from the run's and the correction's addresses
+  __device__ __forceinline__ void load_at(const HFM_GLOBAL T* xp, const HFM_GLOBAL T* ep) {
+    x.load(xp);
+    if (CORR) e.load(ep);
   }
   // a plane past the grid's ends: get() then yields +0 (x + alpha * 0 = +0)
   __device__ __forceinline__ void zero() {
@@ -1011,16 +1023,27 @@ __device__ __forceinline__ void smooth_pair_body(const hfm_mg_task* __restrict__
   // conversion or select at the load, makes it wait for every load at once.
   // On the last plane the x-row above is read from the plane itself and
   // taken as +0 where it is used.
+  // byte offsets: of the thread's run in a plane, of a plane, of the edge
+  // row from the thread's row, of the correction under the thread's and the
+  // edge row's run in a coarse plane, of a coarse plane
+  constexpr uint32_t TB = sizeof(T);
+  const uint32_t rowb = uint32_t(jc * W + i0) * TB, HWb = uint32_t(HW) * TB;
+  const uint32_t edgeb = uint32_t((je - jc) * W) * TB;  // modulo 2^32: may step back a row
+  const uint32_t ecown = uint32_t((jc >> 1) * Wc + (i0 >> 1)) * TB;
+  const uint32_t ecedge = uint32_t((je >> 1) * Wc + (i0 >> 1)) * TB;
+  const uint32_t HWcb = uint32_t(Hc * Wc) * TB;
   auto load_plane = [&](PlaneIn<T, S2>& pi, int kk) {
     const int n = cell(kk);
     const bool up1 = kk + 1 < D;
-    pi.xn.load(xin, ec, up1 ? n + HW : n, up1 ? kk + 1 : kk, jc, i0, Hc, Wc);
-    pi.cx.load(cx + n);
-    pi.cy.load(cy + n);
-    pi.cu.load(up1 ? cz + n + HW : bzh + jc * W + i0);
-    pi.b.load_nt(b + n);
+    const uint32_t nb = uint32_t(kk) * HWb + rowb;
+    const int ku = up1 ? kk + 1 : kk;
+    pi.xn.load_at(byte_at(xin, up1 ? nb + HWb : nb), byte_at(ec, uint32_t(ku >> 1) * HWcb + ecown));
+    pi.cx.load(byte_at(cx, nb));
+    pi.cy.load(byte_at(cy, nb));
+    pi.cu.load(byte_at(up1 ? cz : bzh, up1 ? nb + HWb : rowb));
+    pi.b.load_nt(byte_at(b, nb));
     pi.ef = ldraw(east ? cx + n + N : bxh + kk * H + jc);
-    pi.edge.load(xin, ec, n + (je - jc) * W, kk, je, i0, Hc, Wc);
+    pi.edge.load_at(byte_at(xin, nb + edgeb), byte_at(ec, uint32_t(kk >> 1) * HWcb + ecedge));
     pi.cn.load(north ? cy + n + W : byh + kk * W + i0);
   };
 
