@@ -134,12 +134,12 @@ class Params:
     # fused pass pairs on levels that tile (hfm_mg_smooth_restrict: the last
     # pre-sweep with the restriction; hfm_mg_smooth2: the first two
     # post-sweeps); results are bitwise those of the separate launches.  Only
-    # the finest `fuse_levels` levels: a fused pair walks its tiles plane by
-    # plane, which pays on the fine level and not on the 8x smaller ones
-    # (MI355X, 512-instance batch: L0 1.36 + 1.60 ms fused vs 1.66 + 1.67
-    # separate; L1 0.23 + 0.27 vs 0.23 + 0.21)
+    # the finest `fuse_levels` levels, each where the batch has enough tiles
+    # there: a fused pair walks its tiles plane by plane, which pays on the
+    # two finest levels of a large batch (MI355X, 512-instance batch: L0 1.11
+    # + 1.10 ms fused vs 1.66 + 1.67 separate; L1 0.17 + 0.17 vs 0.23 + 0.21)
     fuse: bool = True
-    fuse_levels: int = 1
+    fuse_levels: int = 2
     # an MGBatch fuses the fine level only with at least this many tiles over
     # its instances (a few tiles leave most CUs idle: one 64 x 256 x 256
     # instance has 86).  The choice is the batch's (its pool's), not each
@@ -228,8 +228,10 @@ class Pool:
         self.store_id = L.STORE[params.storage]
         self.outer_id = L.STORE[params.outer]
         # fine-level pass pairs fused for every plan of this pool (the batch or
-        # server that owns the pool may turn it off: see Params.fuse_min_tiles)
+        # server that owns the pool may turn it off: see Params.fuse_min_tiles),
+        # and those of the levels below it (up to Params.fuse_levels)
         self.fuse_fine = params.fuse
+        self.fuse_coarse = params.fuse
         self.capacity = capacity
         self.sc = self.zeros(capacity * L.SCALARS_DTYPE.itemsize, torch.uint8)
         self.dots = self.zeros(max(dot_slots, 1), torch.float64)
@@ -502,7 +504,7 @@ def fused_levels(inst: Instance, p: Params, lc: int):
     if not p.fuse:
         return sr, s2
     for lev in range(min(lc, len(inst.levels) - 1, p.fuse_levels)):
-        if lev == 0 and not inst.pool.fuse_fine:
+        if not (inst.pool.fuse_fine if lev == 0 else inst.pool.fuse_coarse):
             continue
         tl = L.tile(inst.levels[lev], p.tile_planes)
         if tl is None:
@@ -686,8 +688,8 @@ class Plan:
         for inst in insts:
             # records depend on where the coarse cycle starts (fused levels
             # end there) and on fusion itself
-            key = (lcs[inst.slot], p.fuse, p.fuse_levels, pool.fuse_fine, p.nu1, p.nu2,
-                   p.tile_planes)
+            key = (lcs[inst.slot], p.fuse, p.fuse_levels, pool.fuse_fine, pool.fuse_coarse,
+                   p.nu1, p.nu2, p.tile_planes)
             if getattr(inst, "_tpl", None) is None or inst._tpl_key != key:
                 inst._tpl = instance_templates(inst, p, lcs[inst.slot])
                 inst._tpl_key = key
@@ -977,7 +979,7 @@ class MGBatch:
                  alpha: float = 1.6, max_coarse_edge: int = 4, device=None,
                  iters_per_graph: int = 4, use_graph: bool = True, shrink: float = 0.5,
                  outer: Optional[str] = None, blocks: Optional[dict] = None,
-                 coarse_cells: int = 4096, fuse: bool = True, fuse_levels: int = 1,
+                 coarse_cells: int = 4096, fuse: bool = True, fuse_levels: int = 2,
                  fuse_min_tiles: int = 256, streams: int = 2, method: str = "fcg",
                  tile_planes: int = 32, alloc=None):
         self.params = Params(storage, tol, maxit, nu1, nu2, omega, alpha, max_coarse_edge, outer,
@@ -988,7 +990,7 @@ class MGBatch:
             raise ValueError("empty batch")
         self.pool = Pool(len(shapes), sum(fine_blocks(s) for s in shapes), self.params, device,
                          alloc=alloc)
-        self.pool.fuse_fine = self._fuse_fine(shapes)
+        self._set_fusion(shapes)
         self.device = self.pool.device
         self.plan_only = self.pool.plan_only
         self.storage = storage
@@ -1076,12 +1078,22 @@ class MGBatch:
         self.params.coarse_cells = int(n)
         self._replan()
 
-    def _fuse_fine(self, shapes) -> bool:
+    def _fuse_fine(self, shapes, level: int = 0) -> bool:
+        """Whether the batch fuses the pass pairs of ``level``: at least
+        ``fuse_min_tiles`` tiles there over its instances."""
         p = self.params
         if not p.fuse:
             return False
-        tiles = [L.tile(level_shapes(s, p.max_coarse_edge)[0], p.tile_planes) for s in shapes]
+        tiles = []
+        for s in shapes:
+            ls = level_shapes(s, p.max_coarse_edge)
+            if level < len(ls):
+                tiles.append(L.tile(ls[level], p.tile_planes))
         return sum(t[2] for t in tiles if t is not None) >= p.fuse_min_tiles
+
+    def _set_fusion(self, shapes):
+        self.pool.fuse_fine = self._fuse_fine(shapes)
+        self.pool.fuse_coarse = self._fuse_fine(shapes, 1)
 
     def set_tile_planes(self, n: int):
         """At most ``n`` planes per tile of the fused pass pairs; rebuilds the
@@ -1090,7 +1102,7 @@ class MGBatch:
         if int(n) < 2:
             raise ValueError("tile_planes must be >= 2")
         self.params.tile_planes = int(n)
-        self.pool.fuse_fine = self._fuse_fine(self.shapes)
+        self._set_fusion(self.shapes)
         for inst in self.insts:
             inst._tpl = None
         self._replan()
@@ -1116,7 +1128,7 @@ class MGBatch:
             self.params.fuse_levels = int(levels)
         if min_tiles is not None:
             self.params.fuse_min_tiles = int(min_tiles)
-        self.pool.fuse_fine = self._fuse_fine(self.shapes)
+        self._set_fusion(self.shapes)
         for inst in self.insts:
             inst._tpl = None
         self._replan()

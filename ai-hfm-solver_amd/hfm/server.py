@@ -83,6 +83,7 @@ class SolveServer:
         # fused fine-level pass pairs only for a server that batches: with a
         # few residents their tiles leave most CUs idle
         self.pool.fuse_fine = self.params.fuse and capacity >= FUSE_MIN_CAPACITY
+        self.pool.fuse_coarse = self.pool.fuse_fine
         if self.pool.plan_only:
             raise RuntimeError("SolveServer needs a GPU")
         self.device = self.pool.device

@@ -394,11 +394,15 @@ def test_fused_plan_computes_the_same_cycle(nu, coarse_cells, fuse_levels, min_t
     fused = [(op[0], op[5]) for op in a.plan.ops_down + [o for u in a.plan.ops_up for o in u]
              if op[0] in ("smooth_restrict", "smooth2")]
     kinds = {k for k, _ in fused}
-    assert ("smooth_restrict" in kinds) == (nu[0] >= 2)
-    assert ("smooth2" in kinds) == (nu[1] >= 2)
-    # too few tiles in the batch: the fine level runs unfused in every plan
-    assert any(lev == 0 for _, lev in fused) == (min_tiles == 0)
+    if min_tiles == 0:
+        assert ("smooth_restrict" in kinds) == (nu[0] >= 2)
+        assert ("smooth2" in kinds) == (nu[1] >= 2)
+    else:
+        # too few tiles in the batch on every level: no level runs fused
+        assert not fused
+    assert any(lev == 0 for _, lev in fused) == (min_tiles == 0 and (nu[0] >= 2 or nu[1] >= 2))
     assert a.pool.fuse_fine == (min_tiles == 0)
+    assert a.pool.fuse_coarse == (min_tiles == 0)
     assert not {"smooth_restrict", "smooth2"} & {
         op[0] for op in b.plan.ops_down + [o for u in b.plan.ops_up for o in u]}
     for ia, ib in zip(a.insts, b.insts):
