@@ -771,11 +771,18 @@ __device__ __forceinline__ void st4_nt(HFM_GLOBAL f16* p, const float v[4]) {
   __builtin_nontemporal_store(u2v{pack2h(v[0], v[1]), pack2h(v[2], v[3])}, (HFM_GLOBAL u2v*)p);
 }
 
-// base + off bytes: a uniform base with a 32-bit offset, which the loads take
-// as their SGPR base and VGPR offset (no 64-bit address arithmetic per load).
+// Element n of a field: for 2-byte types a uniform base with a 32-bit byte
+// offset, which the loads take as their SGPR base and VGPR offset (no 64-bit
+// address arithmetic per load; a task holds fewer than 2^31 cells, so the
+// offset fits); a 4-byte field may exceed 4 GiB and takes 64-bit addresses.
+// n is taken modulo 2^32: an index computed with a negative step is exact as
+// long as the element it names exists.
 template <typename T>
-__device__ __forceinline__ const HFM_GLOBAL T* byte_at(const HFM_GLOBAL T* base, uint32_t off) {
-  return (const HFM_GLOBAL T*)((const HFM_GLOBAL char*)base + off);
+__device__ __forceinline__ const HFM_GLOBAL T* elem_at(const HFM_GLOBAL T* base, uint32_t n) {
+  if constexpr (sizeof(T) <= 2)
+    return (const HFM_GLOBAL T*)((const HFM_GLOBAL char*)base + n * uint32_t(sizeof(T)));
+  else
+    return base + n;
 }
 
 // Two values as loaded (the coarse correction under a run), unpacked where used.
@@ -1023,27 +1030,27 @@ __device__ __forceinline__ void smooth_pair_body(const hfm_mg_task* __restrict__
   // conversion or select at the load, makes it wait for every load at once.
   // On the last plane the x-row above is read from the plane itself and
   // taken as +0 where it is used.
-  // byte offsets: of the thread's run in a plane, of a plane, of the edge
-  // row from the thread's row, of the correction under the thread's and the
-  // edge row's run in a coarse plane, of a coarse plane
-  constexpr uint32_t TB = sizeof(T);
-  const uint32_t rowb = uint32_t(jc * W + i0) * TB, HWb = uint32_t(HW) * TB;
-  const uint32_t edgeb = uint32_t((je - jc) * W) * TB;  // modulo 2^32: may step back a row
-  const uint32_t ecown = uint32_t((jc >> 1) * Wc + (i0 >> 1)) * TB;
-  const uint32_t ecedge = uint32_t((je >> 1) * Wc + (i0 >> 1)) * TB;
-  const uint32_t HWcb = uint32_t(Hc * Wc) * TB;
+  // element offsets: of the thread's run in a plane, of a plane, of the edge
+  // row from the thread's row (modulo 2^32: it may step back a row), of the
+  // correction under the thread's and the edge row's run in a coarse plane,
+  // of a coarse plane
+  const uint32_t row = uint32_t(jc * W + i0), uHW = uint32_t(HW);
+  const uint32_t edge = uint32_t((je - jc) * W);
+  const uint32_t ecown = uint32_t((jc >> 1) * Wc + (i0 >> 1));
+  const uint32_t ecedge = uint32_t((je >> 1) * Wc + (i0 >> 1));
+  const uint32_t HWc = uint32_t(Hc * Wc);
   auto load_plane = [&](PlaneIn<T, S2>& pi, int kk) {
     const int n = cell(kk);
     const bool up1 = kk + 1 < D;
-    const uint32_t nb = uint32_t(kk) * HWb + rowb;
+    const uint32_t nu = uint32_t(kk) * uHW + row;
     const int ku = up1 ? kk + 1 : kk;
-    pi.xn.load_at(byte_at(xin, up1 ? nb + HWb : nb), byte_at(ec, uint32_t(ku >> 1) * HWcb + ecown));
-    pi.cx.load(byte_at(cx, nb));
-    pi.cy.load(byte_at(cy, nb));
-    pi.cu.load(byte_at(up1 ? cz : bzh, up1 ? nb + HWb : rowb));
-    pi.b.load_nt(byte_at(b, nb));
+    pi.xn.load_at(elem_at(xin, up1 ? nu + uHW : nu), elem_at(ec, uint32_t(ku >> 1) * HWc + ecown));
+    pi.cx.load(elem_at(cx, nu));
+    pi.cy.load(elem_at(cy, nu));
+    pi.cu.load(elem_at(up1 ? cz : bzh, up1 ? nu + uHW : row));
+    pi.b.load_nt(elem_at(b, nu));
     pi.ef = ldraw(east ? cx + n + N : bxh + kk * H + jc);
-    pi.edge.load_at(byte_at(xin, nb + edgeb), byte_at(ec, uint32_t(kk >> 1) * HWcb + ecedge));
+    pi.edge.load_at(elem_at(xin, nu + edge), elem_at(ec, uint32_t(kk >> 1) * HWc + ecedge));
     pi.cn.load(north ? cy + n + W : byh + kk * W + i0);
   };
 
