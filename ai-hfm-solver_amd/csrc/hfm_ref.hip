@@ -49,6 +49,16 @@ struct Grid3 {
   __device__ __forceinline__ int64_t idx(int k, int j, int i) const {
     return (int64_t(k) * H + j) * W + i;
   }
+  // &x(k, j, i) as plane k's base (formed by the scalar unit where k is
+  // wave-uniform, as in the column walkers) plus a 32-bit byte offset in the
+  // plane: a plane holds fewer than 2^30 cells (dims_ok)
+  template <typename T>
+  __device__ __forceinline__ T* cell(T* x, int k, int j, int i) const {
+    using B = std::conditional_t<std::is_const_v<T>, const char, char>;
+    T* plane = x + int64_t(k) * (int64_t(H) * W);
+    return reinterpret_cast<T*>(reinterpret_cast<B*>(plane) +
+                                uint32_t(j * W + i) * uint32_t(sizeof(T)));
+  }
   // without short-circuit evaluation (no branches between a cell's loads)
   __device__ __forceinline__ bool in(int k, int j, int i) const {
     return (unsigned(k) < unsigned(D)) & (unsigned(j) < unsigned(H)) & (unsigned(i) < unsigned(W));
@@ -169,7 +179,7 @@ __device__ __forceinline__ bool cell_of_thread(const Grid3& g, int& k, int& j, i
 // cells of one row and outputs the inner 62 (its two end lanes only load
 // the halo).  Per output cell a field costs 3 row loads instead of 27 tap
 // loads, and the taps are the same values in the same order as load_nbhd's.
-constexpr int kKZ = 16;
+constexpr int kKZ = 18;  // a multiple of three (walk3)
 constexpr int kOutX = kBX - 2;
 // Output rows per thread: RY = 2 for the one-field kernels (a window of 4
 // rows serves two outputs), 1 where several fields' windows are live.
@@ -180,13 +190,39 @@ struct Col {
   bool out;  // an inner lane of a column inside the grid: it writes results
 };
 
+// The walkers' tiles, XCD-aware.  The dispatcher deals consecutive workgroup
+// ids (x fastest) round-robin to the 8 XCDs, each with its own L2, so the
+// linear id L is taken as tile T = (the range of XCD L % 8) + L / 8, and T
+// enumerates the tiles slab by slab: the y rows of tiles are cut into eight
+// slabs, and a slab's tiles go x fastest, then y, then z.  An XCD thus walks
+// one slab over every z chunk (the same work as the others, the short last
+// chunk included), and the tiles it runs at once are a patch of adjacent
+// rows, so the rows a tile shares with its y neighbours (the windows' halo
+// rows) are fetched once into that XCD's L2.
+constexpr unsigned kXCDs = 8;
+__device__ __forceinline__ void xcd_tile(unsigned& bx, unsigned& by, unsigned& bz) {
+  const unsigned nx = gridDim.x, ny = gridDim.y, nz = gridDim.z;
+  const unsigned n = nx * ny * nz;
+  const unsigned L = blockIdx.x + nx * (blockIdx.y + ny * blockIdx.z);
+  const unsigned x = L % kXCDs, per = n / kXCDs, rem = n % kXCDs;
+  const unsigned T = x * per + min(x, rem) + L / kXCDs;
+  const unsigned sy = (ny + kXCDs - 1) / kXCDs;  // rows of tiles per slab
+  const unsigned q = T / (nx * nz * sy), t = T % (nx * nz * sy);
+  const unsigned rows = min(sy, ny - q * sy);     // the last slab may be shorter
+  bx = t % nx;
+  by = q * sy + (t / nx) % rows;
+  bz = (t / nx) / rows;
+}
+
 // false: the whole wave lies past the last row
 template <int RY>
 __device__ __forceinline__ bool col_begin(const Grid3& g, Col& c) {
+  unsigned bx, by, bz;
+  xcd_tile(bx, by, bz);
   c.lane = threadIdx.x;
-  c.i = int(blockIdx.x) * kOutX + c.lane - 1;
-  c.j = (int(blockIdx.y) * kBY + int(threadIdx.y)) * RY;
-  c.k0 = blockIdx.z * kKZ;
+  c.i = int(bx) * kOutX + c.lane - 1;
+  c.j = (int(by) * kBY + int(threadIdx.y)) * RY;
+  c.k0 = int(bz) * kKZ;
   c.k1 = min(c.k0 + kKZ, g.D);
   c.in = unsigned(c.i) < unsigned(g.W);
   c.out = c.in && c.lane >= 1 && c.lane <= kOutX;
@@ -214,16 +250,23 @@ struct Fetch2 {
   __device__ __forceinline__ float value() const { return a - b; }
 };
 
-template <int RY, typename FT = Fetch1>
+template <int RY, typename FT = Fetch1, int PF = 1>
 struct Win27 {
+  static_assert(PF == 1 || PF == 2, "prefetch distance of one or two steps");
   static constexpr int R = RY + 2;
-  float l[3][R], m[3][R], r[3][R];  // [plane][row], x offsets -1, 0, +1 (zero outside)
-  FT sv[R];                         // a fetched plane's rows at this lane's column
+  // three plane slots [slot][row], x offsets -1, 0, +1 (zero outside): at a
+  // step of phase P (walk3) slot (a + P) % 3 holds plane k - 1 + a, so the
+  // planes rotate through the slots without copies
+  float l[3][R], m[3][R], r[3][R];
+  // fetched planes' rows at this lane's column: a plane is fetched PF steps
+  // before the step that commits it (into sv[phase of the fetching step]), so
+  // PF planes' loads are in flight while a cell is computed
+  FT sv[3][R];
 
-  // issue the loads of plane kk of the field xat(kk, jj, ii) (no waiting):
-  // every load is issued, from in-bounds addresses, and taken as +0 where the
-  // cell is outside the grid
-  template <typename XF>
+  // issue the loads of plane kk of the field xat(kk, jj, ii) into sv[B] (no
+  // waiting): every load is issued, from in-bounds addresses, and taken as +0
+  // where the cell is outside the grid
+  template <int B, typename XF>
   __device__ __forceinline__ void fetch(const Grid3& g, const Col& c, int kk, XF xat) {
     const bool kin = unsigned(kk) < unsigned(g.D);
     const int kc = kin ? kk : c.k0;
@@ -231,57 +274,79 @@ struct Win27 {
 #pragma unroll
     for (int q = 0; q < R; ++q) {
       const int jj = c.j - 1 + q;
-      const bool in = kin & (unsigned(jj) < unsigned(g.H)) & c.in;
-      sv[q] = xat(kc, in ? jj : c.j, ic);
-      sv[q].keep = sv[q].keep & in;
+      const bool rin = unsigned(jj) < unsigned(g.H);
+      sv[B][q] = xat(kc, rin ? jj : c.j, ic);  // the row offset is the walk's constant
+      sv[B][q].keep = sv[B][q].keep & (kin & rin & c.in);
     }
   }
-  // plane p of the window <- the fetched plane (x neighbours from the adjacent lanes)
-  __device__ __forceinline__ void commit(int p, const Col&) {
+  // slot S <- the plane fetched into sv[B] (x neighbours from the adjacent lanes)
+  template <int S, int B>
+  __device__ __forceinline__ void commit(const Col&) {
 #pragma unroll
     for (int q = 0; q < R; ++q) {
-      const float x = sv[q].keep ? sv[q].value() : 0.0f;
-      l[p][q] = __shfl_up(x, 1, 64);    // the end lanes' outer values are unused
-      m[p][q] = x;
-      r[p][q] = __shfl_down(x, 1, 64);
+      const float x = sv[B][q].keep ? sv[B][q].value() : 0.0f;
+      l[S][q] = __shfl_up(x, 1, 64);    // the end lanes' outer values are unused
+      m[S][q] = x;
+      r[S][q] = __shfl_down(x, 1, 64);
     }
   }
-  __device__ __forceinline__ void shift() {
-#pragma unroll
-    for (int p = 0; p < 2; ++p)
-#pragma unroll
-      for (int q = 0; q < R; ++q) {
-        l[p][q] = l[p + 1][q];
-        m[p][q] = m[p + 1][q];
-        r[p][q] = r[p + 1][q];
-      }
-  }
-  // the taps of output row j + ry at the centre plane (load_nbhd order, t = 9a + 3b + c)
-  template <int TAPS>
+  // the taps of output row j + ry at the centre plane, at phase P (load_nbhd
+  // order, t = 9a + 3b + c)
+  template <int TAPS, int P>
   __device__ __forceinline__ void taps(float v[27], int ry) const {
 #pragma unroll
     for (int t = 0; t < 27; ++t) {
-      const int a = t / 9, b = (t / 3) % 3 + ry, cc = t % 3;
+      const int a = (t / 9 + P) % 3, b = (t / 3) % 3 + ry, cc = t % 3;
       v[t] = !tap_on<TAPS>(t) ? 0.0f : cc == 0 ? l[a][b] : cc == 1 ? m[a][b] : r[a][b];
     }
   }
-  __device__ __forceinline__ float centre(int ry) const { return m[1][1 + ry]; }
-  // planes k0 - 1 and k0 in the window, plane k0 + 1 fetched
+  template <int P>
+  __device__ __forceinline__ float centre(int ry) const {
+    return m[(1 + P) % 3][1 + ry];
+  }
+  // planes k0 - 1 and k0 in slots 0 and 1; planes k0 + 1 .. k0 + PF fetched
+  // (plane k0 + 1 + d is committed at step d, from the buffer of step d - PF)
   template <typename XF>
   __device__ __forceinline__ void start(const Grid3& g, const Col& c, XF xat) {
-    fetch(g, c, c.k0 - 1, xat);
-    commit(0, c);
-    fetch(g, c, c.k0, xat);
-    commit(1, c);
-    fetch(g, c, c.k0 + 1, xat);
+    fetch<0>(g, c, c.k0 - 1, xat);
+    commit<0, 0>(c);
+    fetch<0>(g, c, c.k0, xat);
+    commit<1, 0>(c);
+    fetch<(3 - PF) % 3>(g, c, c.k0 + 1, xat);
+    if constexpr (PF == 2) fetch<2>(g, c, c.k0 + 2, xat);
   }
-  // at output plane k: plane k + 1 into the window, plane k + 2's loads in flight
-  template <typename XF>
+  // at output plane k (phase P): plane k + 1 into its slot, plane k + 1 + PF's
+  // loads issued
+  template <int P, typename XF>
   __device__ __forceinline__ void advance(const Grid3& g, const Col& c, int k, XF xat) {
-    commit(2, c);
-    fetch(g, c, k + 2, xat);
+    commit<(P + 2) % 3, (P + 3 - PF) % 3>(c);
+    fetch<P>(g, c, k + 1 + PF, xat);
   }
 };
+
+template <int P>
+using Phase = std::integral_constant<int, P>;
+
+// Prefetch distance (steps) of the walkers.  Two steps ahead only pays in
+// k_pg_flux (whose two windows leave registers for a second plane in flight);
+// elsewhere the extra registers cost occupancy and the walk is no faster.
+constexpr int kWalkPF = 1;
+
+// The walk over a column's planes k0 .. k1 - 1, unrolled by three: step(P, k)
+// at phase P = (k - k0) % 3, so the window slots and a walker's per-plane
+// records (indexed by phase) rotate without copies, and no value loaded for
+// a later step is moved (which would wait for it) at the end of a step.  With
+// k1 - k0 not a multiple of three (only the last chunk of a column, kKZ being
+// one) the walk ends with one or two steps past k1: they load from clamped
+// in-bounds addresses and store nothing (a step stores only for k < k1).
+template <typename Step>
+__device__ __forceinline__ void walk3(const Col& c, Step&& step) {
+  for (int k = c.k0; k < c.k1; k += 3) {
+    step(Phase<0>{}, k);
+    step(Phase<1>{}, k + 1);
+    step(Phase<2>{}, k + 2);
+  }
+}
 
 // The thread's own cells of rows c.j .. c.j + RY - 1 at plane k of field p,
 // from in-bounds addresses (a row or plane past the grid re-reads the last
@@ -294,7 +359,7 @@ __device__ __forceinline__ void own_cells(const Grid3& g, const Col& c, int k, c
   const int ic = c.in ? c.i : 0;
   const int kc = min(k, g.D - 1);
 #pragma unroll
-  for (int ry = 0; ry < RY; ++ry) v[ry] = p[g.idx(kc, min(c.j + ry, g.H - 1), ic)];
+  for (int ry = 0; ry < RY; ++ry) v[ry] = *g.cell(p, kc, min(c.j + ry, g.H - 1), ic);
 }
 
 // (x - A/d) + b/d, the reference's evaluation order
@@ -324,38 +389,38 @@ __global__ __launch_bounds__(kBX* kBY) void k_conv27(const float* __restrict__ x
                                                      float d, Grid3 g, int form) {
   Col c;
   if (!col_begin<2>(g, c)) return;
-  auto xat = [&](int kk, int jj, int ii) { return Fetch1{x[g.idx(kk, jj, ii)], true}; };
+  auto xat = [&](int kk, int jj, int ii) { return Fetch1{*g.cell(x, kk, jj, ii), true}; };
   const int f = form & HFM_FORM_MASK;
   const float* bp = f == HFM_FORM_APPLY ? x : b;  // APPLY reads no b (x stands in)
-  Win27<2> win;
+  constexpr int PF = kWalkPF;
+  Win27<2, Fetch1, PF> win;
   win.start(g, c, xat);
-  float bv[2], bn[2];
-  own_cells<2>(g, c, c.k0, bp, bv);
-  for (int k = c.k0; k < c.k1; ++k) {
-    win.advance(g, c, k, xat);
-    own_cells<2>(g, c, k + 1, bp, bn);
+  float bv[3][2];  // b of the plane of phase P in bv[P], loaded PF steps ahead
+  for (int d = 0; d < PF; ++d) own_cells<2>(g, c, c.k0 + d, bp, bv[d]);
+  walk3(c, [&](auto ph, int k) {
+    constexpr int P = decltype(ph)::value;
+    win.template advance<P>(g, c, k, xat);
+    own_cells<2>(g, c, k + PF, bp, bv[(P + PF) % 3]);
 #pragma unroll
     for (int ry = 0; ry < 2; ++ry) {
       float v[27];
-      win.taps<TAPS>(v, ry);
+      win.template taps<TAPS, P>(v, ry);
       const float ax = wsum<TAPS>(K, v);
-      if (c.out && c.j + ry < g.H) {
-        const int64_t n = g.idx(k, c.j + ry, c.i);
-        const float bc = bv[ry];
+      if (c.out && k < c.k1 && c.j + ry < g.H) {
+        float* const yp = g.cell(y, k, c.j + ry, c.i);
+        const float bc = bv[P][ry];
+        const float xc = win.template centre<P>(ry);
         float out = ax;
         if (f == HFM_FORM_RESID) out = ax - bc;
-        else if (f == HFM_FORM_JACOBI) out = jacobi_form(win.centre(ry), ax, bc, d);
+        else if (f == HFM_FORM_JACOBI) out = jacobi_form(xc, ax, bc, d);
         else if (f == HFM_FORM_SUBTRACT) out = bc - ax;
         else if (f == HFM_FORM_AXPY) out = bc + d * ax;
-        else if (f == HFM_FORM_JACOBI_MUL) out = jacobi_mul_form(win.centre(ry), ax, bc, d);
+        else if (f == HFM_FORM_JACOBI_MUL) out = jacobi_mul_form(xc, ax, bc, d);
         if ((form & HFM_FORM_ZERO_OUTLET) && c.i == g.W - 1) out = 0.0f;
-        y[n] = out;
+        *yp = out;
       }
     }
-    win.shift();
-    bv[0] = bn[0];
-    bv[1] = bn[1];
-  }
+  });
 }
 
 __global__ __launch_bounds__(kBX* kBY) void k_var6(const float* __restrict__ x,
@@ -365,7 +430,7 @@ __global__ __launch_bounds__(kBX* kBY) void k_var6(const float* __restrict__ x,
   int k, j, i;
   if (!cell_of_thread(g, k, j, i)) return;
   const int64_t n = g.idx(k, j, i);
-  const float ax = var6_at(V, g, k, j, i, [&](int kk, int jj, int ii) { return x[g.idx(kk, jj, ii)]; });
+  const float ax = var6_at(V, g, k, j, i, [&](int kk, int jj, int ii) { return *g.cell(x, kk, jj, ii); });
   float out = ax;
   if (form == HFM_FORM_RESID) out = ax - b[n];
   else if (form == HFM_FORM_JACOBI) out = jacobi_form(x[n], ax, b[n], V.tot[n]);
@@ -465,28 +530,26 @@ __global__ __launch_bounds__(kBX* kBY) void k_prolong_jacobi_col(const float* __
                                                                  int fz, int fy, int fx) {
   Col c;
   if (!col_begin<2>(gf, c)) return;
-  auto wat = [&](int kk, int jj, int ii) { return Fetch1{wc[gc.idx(kk / fz, jj / fy, ii / fx)], true}; };
-  Win27<2> win;
+  auto wat = [&](int kk, int jj, int ii) { return Fetch1{*gc.cell(wc, kk / fz, jj / fy, ii / fx), true}; };
+  constexpr int PF = kWalkPF;
+  Win27<2, Fetch1, PF> win;
   win.start(gf, c, wat);
-  float rv[2], rn[2];
-  own_cells<2>(gf, c, c.k0, r, rv);
-  for (int k = c.k0; k < c.k1; ++k) {
-    win.advance(gf, c, k, wat);
-    own_cells<2>(gf, c, k + 1, r, rn);
+  float rv[3][2];  // r of the plane of phase P in rv[P], loaded PF steps ahead
+  for (int d = 0; d < PF; ++d) own_cells<2>(gf, c, c.k0 + d, r, rv[d]);
+  walk3(c, [&](auto ph, int k) {
+    constexpr int P = decltype(ph)::value;
+    win.template advance<P>(gf, c, k, wat);
+    own_cells<2>(gf, c, k + PF, r, rv[(P + PF) % 3]);
 #pragma unroll
     for (int ry = 0; ry < 2; ++ry) {
       float v[27];
-      win.taps<TAPS>(v, ry);
+      win.template taps<TAPS, P>(v, ry);
       const float ax = wsum<TAPS>(K, v);
-      if (c.out && c.j + ry < gf.H) {
-        const int64_t n = gf.idx(k, c.j + ry, c.i);
-        y[n] = jacobi_form(win.centre(ry), ax, rv[ry], d);
-      }
+      if (c.out && k < c.k1 && c.j + ry < gf.H)
+        *gf.cell(y, k, c.j + ry, c.i) =
+            jacobi_form(win.template centre<P>(ry), ax, rv[P][ry], d);
     }
-    win.shift();
-    rv[0] = rn[0];
-    rv[1] = rn[1];
-  }
+  });
 }
 
 // The correction w is a field of the fine grid, or (PROL) the coarse field
@@ -498,7 +561,7 @@ struct CorrW {
   int sz, sy, sx;
   template <bool PROL>
   __device__ __forceinline__ float at(const Grid3& g, int kk, int jj, int ii) const {
-    return PROL ? w[gc.idx(kk >> sz, jj >> sy, ii >> sx)] : w[g.idx(kk, jj, ii)];
+    return PROL ? *gc.cell(w, kk >> sz, jj >> sy, ii >> sx) : *g.cell(w, kk, jj, ii);
   }
 };
 
@@ -513,31 +576,31 @@ __global__ __launch_bounds__(kBX* kBY) void k_correct_jacobi_col(const float* __
   const bool zo = flags & HFM_CORRECT_ZERO_OUTLET;
   // x - w, or 0 at the outlet (HFM_CORRECT_ZERO_OUTLET)
   auto vat = [&](int kk, int jj, int ii) {
-    const int64_t m = g.idx(kk, jj, ii);
-    return Fetch2{x[m], cw.template at<PROL>(g, kk, jj, ii), !(zo && ii == g.W - 1)};
+    return Fetch2{*g.cell(x, kk, jj, ii), cw.template at<PROL>(g, kk, jj, ii),
+                  !(zo && ii == g.W - 1)};
   };
-  Win27<2, Fetch2> win;
+  constexpr int PF = kWalkPF;
+  Win27<2, Fetch2, PF> win;
   win.start(g, c, vat);
-  float bv[2], bn[2];
-  own_cells<2>(g, c, c.k0, b, bv);
-  for (int k = c.k0; k < c.k1; ++k) {
-    win.advance(g, c, k, vat);
-    own_cells<2>(g, c, k + 1, b, bn);
+  float bv[3][2];  // b of the plane of phase P in bv[P], loaded PF steps ahead
+  for (int d = 0; d < PF; ++d) own_cells<2>(g, c, c.k0 + d, b, bv[d]);
+  walk3(c, [&](auto ph, int k) {
+    constexpr int P = decltype(ph)::value;
+    win.template advance<P>(g, c, k, vat);
+    own_cells<2>(g, c, k + PF, b, bv[(P + PF) % 3]);
 #pragma unroll
     for (int ry = 0; ry < 2; ++ry) {
       float v[27];
-      win.taps<TAPS>(v, ry);
+      win.template taps<TAPS, P>(v, ry);
       const float ax = wsum<TAPS>(K, v);
-      if (c.out && c.j + ry < g.H) {
-        const int64_t n = g.idx(k, c.j + ry, c.i);
-        y[n] = (flags & HFM_CORRECT_RECIPROCAL) ? jacobi_mul_form(win.centre(ry), ax, bv[ry], d)
-                                                : jacobi_form(win.centre(ry), ax, bv[ry], d);
+      if (c.out && k < c.k1 && c.j + ry < g.H) {
+        const float xc = win.template centre<P>(ry);
+        *g.cell(y, k, c.j + ry, c.i) = (flags & HFM_CORRECT_RECIPROCAL)
+                                           ? jacobi_mul_form(xc, ax, bv[P][ry], d)
+                                           : jacobi_form(xc, ax, bv[P][ry], d);
       }
     }
-    win.shift();
-    bv[0] = bn[0];
-    bv[1] = bn[1];
-  }
+  });
 }
 
 template <bool VAR6, int TAPS, bool PROL>
@@ -589,40 +652,41 @@ __global__ __launch_bounds__(kBX* kBY) void k_pg_coef(const float* __restrict__ 
   if (!col_begin<1>(g, c)) return;
   const float* f[3] = {u, v, w};
   float* out[3] = {k0, k1, k2};
-  Win27<1> win[3];
+  constexpr int PF = kWalkPF;
+  Win27<1, Fetch1, PF> win[3];
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
     const float* x = f[q];
-    win[q].start(g, c, [&](int kk, int jj, int ii) { return Fetch1{x[g.idx(kk, jj, ii)], true}; });
+    win[q].start(g, c, [&](int kk, int jj, int ii) { return Fetch1{*g.cell(x, kk, jj, ii), true}; });
   }
-  float sv[1], sn[1];
-  own_cells<1>(g, c, c.k0, sigma, sv);
-  for (int k = c.k0; k < c.k1; ++k) {
-    const int64_t n = g.idx(k, c.j, c.in ? c.i : 0);
+  float sv[3][1];  // sigma of the plane of phase P in sv[P], loaded PF steps ahead
+  for (int d = 0; d < PF; ++d) own_cells<1>(g, c, c.k0 + d, sigma, sv[d]);
+  walk3(c, [&](auto ph, int k) {
+    constexpr int P = decltype(ph)::value;
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
       const float* x = f[q];
-      win[q].advance(g, c, k, [&](int kk, int jj, int ii) { return Fetch1{x[g.idx(kk, jj, ii)], true}; });
+      win[q].template advance<P>(
+          g, c, k, [&](int kk, int jj, int ii) { return Fetch1{*g.cell(x, kk, jj, ii), true}; });
     }
-    own_cells<1>(g, c, k + 1, sigma, sn);
-    const float speed =
-        (1.0f / 3.0f) * ((fabsf(win[0].centre(0)) + fabsf(win[1].centre(0))) + fabsf(win[2].centre(0)));
-    const float damp = 1.0f + dt * sv[0];
-    sv[0] = sn[0];
+    own_cells<1>(g, c, k + PF, sigma, sv[(P + PF) % 3]);
+    const float speed = (1.0f / 3.0f) * ((fabsf(win[0].template centre<P>(0)) +
+                                          fabsf(win[1].template centre<P>(0))) +
+                                         fabsf(win[2].template centre<P>(0)));
+    const float damp = 1.0f + dt * sv[P][0];
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
       float nb[27];
-      win[q].template taps<27>(nb, 0);
+      win[q].template taps<27, P>(nb, 0);
       const float dif = wsum<27>(K.k[0], nb);
       const float ax = wsum<TX>(K.k[1], nb);
       const float ay = wsum<TY>(K.k[2], nb);
       const float az = wsum<TZ>(K.k[3], nb);
       const float num = 2.0f * fabsf(speed * dif);
       const float den = eps + ((fabsf(ax) + fabsf(ay)) + fabsf(az)) / 3.0f;
-      if (c.out) out[q][n] = fminf(num / den, 0.5f) / damp;
-      win[q].shift();
+      if (c.out && k < c.k1) *g.cell(out[q], k, c.j, c.i) = fminf(num / den, 0.5f) / damp;
     }
-  }
+  });
 }
 
 // The dissipation flux 3*0.5 (k dif(f~) + dif(f~ k) - f~ dif(k)) per
@@ -642,27 +706,26 @@ __global__ __launch_bounds__(kBX* kBY) void k_pg_flux(const float* f0, const flo
   for (int q = 0; q < 3; ++q) {
     const float* x = f[q];
     const float* y = kf[q];
-    auto xat = [&](int kk, int jj, int ii) { return Fetch1{x[g.idx(kk, jj, ii)], true}; };
-    auto yat = [&](int kk, int jj, int ii) { return Fetch1{y[g.idx(kk, jj, ii)], true}; };
-    Win27<1> wf, wk;
+    auto xat = [&](int kk, int jj, int ii) { return Fetch1{*g.cell(x, kk, jj, ii), true}; };
+    auto yat = [&](int kk, int jj, int ii) { return Fetch1{*g.cell(y, kk, jj, ii), true}; };
+    Win27<1, Fetch1, 2> wf, wk;
     wf.start(g, c, xat);
     wk.start(g, c, yat);
-    for (int k = c.k0; k < c.k1; ++k) {
-      wf.advance(g, c, k, xat);
-      wk.advance(g, c, k, yat);
+    walk3(c, [&](auto ph, int k) {
+      constexpr int P = decltype(ph)::value;
+      wf.template advance<P>(g, c, k, xat);
+      wk.template advance<P>(g, c, k, yat);
       float fv[27], kv[27], fk[27];
-      wf.taps<27>(fv, 0);
-      wk.taps<27>(kv, 0);
+      wf.template taps<27, P>(fv, 0);
+      wk.template taps<27, P>(kv, 0);
 #pragma unroll
       for (int t = 0; t < 27; ++t) fk[t] = fv[t] * kv[t];
       const float dif_f = wsum<27>(Kd, fv);
       const float dif_fk = wsum<27>(Kd, fk);
       const float dif_k = wsum<27>(Kd, kv);
-      if (c.out)
-        out[q][g.idx(k, c.j, c.i)] = 1.5f * ((kv[13] * dif_f + dif_fk) - fv[13] * dif_k);
-      wf.shift();
-      wk.shift();
-    }
+      if (c.out && k < c.k1)
+        *g.cell(out[q], k, c.j, c.i) = 1.5f * ((kv[13] * dif_f + dif_fk) - fv[13] * dif_k);
+    });
   }
 }
 
@@ -672,7 +735,10 @@ __global__ __launch_bounds__(kBX* kBY) void k_pg_flux(const float* f0, const flo
 // cell's), applied to each field: 27 + 81 / NF FMAs per field instead of 108.
 // This sums the same products in another order than four separate
 // convolutions, so results agree with the layer-by-layer form to rounding.
-template <int NF, bool DIS>
+// TX / TY / TZ: the tap sets of the three advection stencils; the fold skips
+// their structurally zero taps (54 FMAs instead of 81 for the first-derivative
+// sets), which leaves every finite result unchanged.
+template <int NF, bool DIS, int TX = 27, int TY = 27, int TZ = 27>
 __global__ __launch_bounds__(kBX* kBY) void k_advect27_v(
     const float* __restrict__ u, const float* __restrict__ v, const float* __restrict__ w,
     const float* p0, const float* p1, const float* p2, const float* b0, const float* b1,
@@ -685,65 +751,68 @@ __global__ __launch_bounds__(kBX* kBY) void k_advect27_v(
   const float* phi[3] = {p0, p1, p2};
   const float* base[3] = {b0, b1, b2};
   float* out[3] = {o0, o1, o2};
-  Win27<1> win[NF];
+  constexpr int PF = kWalkPF;
+  Win27<1, Fetch1, PF> win[NF];
 #pragma unroll
   for (int q = 0; q < NF; ++q) {
     const float* x = phi[q];
-    win[q].start(g, c, [&](int kk, int jj, int ii) { return Fetch1{x[g.idx(kk, jj, ii)], true}; });
+    win[q].start(g, c, [&](int kk, int jj, int ii) { return Fetch1{*g.cell(x, kk, jj, ii), true}; });
   }
-  // the cell's own values (velocity, bases, dissipation, sigma); with three
-  // fields loaded one plane ahead (the last plane's successor re-reads the
-  // last plane), a single field loads them at the plane (faster there)
+  // the cell's own values (velocity, bases, dissipation, sigma), loaded one
+  // plane ahead (the last plane's successor re-reads the last plane) into the
+  // record of the next phase
   struct Cells {
     float u, v, w, b[NF], d[NF], s;
   };
   const bool has_s = post && sigma;
   const float* sg = has_s ? sigma : u;
   auto load_cells = [&](int kk, Cells& cv) {
-    const int64_t m = g.idx(min(kk, g.D - 1), c.j, c.in ? c.i : 0);
-    cv.u = u[m];
-    cv.v = v[m];
-    cv.w = w[m];
+    const int kc = min(kk, g.D - 1), ic = c.in ? c.i : 0;
+    cv.u = *g.cell(u, kc, c.j, ic);
+    cv.v = *g.cell(v, kc, c.j, ic);
+    cv.w = *g.cell(w, kc, c.j, ic);
 #pragma unroll
     for (int q = 0; q < NF; ++q) {
-      cv.b[q] = base[q][m];
-      cv.d[q] = DIS ? dis[q][m] : 0.0f;
+      cv.b[q] = *g.cell(base[q], kc, c.j, ic);
+      cv.d[q] = DIS ? *g.cell(dis[q], kc, c.j, ic) : 0.0f;
     }
-    cv.s = sg[m];  // sigma (or u, unused): every load is issued
+    cv.s = *g.cell(sg, kc, c.j, ic);  // sigma (or u, unused): every load is issued
   };
-  constexpr bool ahead = NF > 1;
-  Cells cur, nxt;
-  if (ahead) load_cells(c.k0, cur);
-  for (int k = c.k0; k < c.k1; ++k) {
-    const int64_t n = g.idx(k, c.j, c.in ? c.i : 0);
-    if (!ahead) load_cells(k, cur);
+  Cells rec[3];
+  for (int d = 0; d < PF; ++d) load_cells(c.k0 + d, rec[d]);
+  walk3(c, [&](auto ph, int k) {
+    constexpr int P = decltype(ph)::value;
+    const Cells& cur = rec[P];
     const float uc = cur.u, vc = cur.v, wc = cur.w;
 #pragma unroll
     for (int q = 0; q < NF; ++q) {
       const float* x = phi[q];
-      win[q].advance(g, c, k, [&](int kk, int jj, int ii) { return Fetch1{x[g.idx(kk, jj, ii)], true}; });
+      win[q].template advance<P>(
+          g, c, k, [&](int kk, int jj, int ii) { return Fetch1{*g.cell(x, kk, jj, ii), true}; });
     }
-    if (ahead) load_cells(k + 1, nxt);
+    load_cells(k + PF, rec[(P + PF) % 3]);
     // the cell's combined weights dif - u ax - v ay - w az, shared by the
     // NF fields: half the FMAs of four stencils per field
     float cw[27];
 #pragma unroll
     for (int t = 0; t < 27; ++t) {
-      const float a0 = DIS ? 0.0f : K.k[0].w[t];
-      cw[t] = __builtin_fmaf(-wc, K.k[3].w[t], __builtin_fmaf(-vc, K.k[2].w[t],
-                                                              __builtin_fmaf(-uc, K.k[1].w[t], a0)));
+      float a = DIS ? 0.0f : K.k[0].w[t];
+      if (tap_on<TX>(t)) a = __builtin_fmaf(-uc, K.k[1].w[t], a);
+      if (tap_on<TY>(t)) a = __builtin_fmaf(-vc, K.k[2].w[t], a);
+      if (tap_on<TZ>(t)) a = __builtin_fmaf(-wc, K.k[3].w[t], a);
+      cw[t] = a;
     }
 #pragma unroll
     for (int q = 0; q < NF; ++q) {
       float t[27];
-      win[q].template taps<27>(t, 0);
+      win[q].template taps<27, P>(t, 0);
       const float bq = cur.b[q];
       // DIS: the dissipation term is given (dis0..2 in place of the phi's
       // diffusion stencil, UM/INHALE_demo.ipynb cell 24)
       float a = DIS ? cur.d[q] : 0.0f;
 #pragma unroll
       for (int s = 0; s < 27; ++s) a = __builtin_fmaf(cw[s], t[s], a);
-      if (c.out) {
+      if (c.out && k < c.k1) {
         float o = f * a + bq;
         if (post) {
           // the scalar's fill (every boundary cell 0) and nbluff dampings
@@ -754,12 +823,10 @@ __global__ __launch_bounds__(kBX* kBY) void k_advect27_v(
           const float s = has_s ? cur.s : 0.0f;
           for (int r = 0; r < nb; ++r) o = bluff_apply(o, s, dt);
         }
-        out[q][n] = o;
+        *g.cell(out[q], k, c.j, c.i) = o;
       }
-      win[q].shift();
     }
-    if (ahead) cur = nxt;
-  }
+  });
 }
 
 // y = scale * sum_q K_q (*) x_q (q < n <= 3); SOURCE_BC: the x = 0 plane takes
@@ -777,32 +844,33 @@ __global__ __launch_bounds__(kBX* kBY) void k_conv27_sum_v(const float* x0, cons
   Col c;
   if (!col_begin<1>(g, c)) return;
   const float* xs[3] = {x0, x1, x2};
-  Win27<1> win[NF];
+  Win27<1, Fetch1, kWalkPF> win[NF];
 #pragma unroll
   for (int q = 0; q < NF; ++q) {
     const float* xq = xs[q];
-    win[q].start(g, c, [&](int kk, int jj, int ii) { return Fetch1{xq[g.idx(kk, jj, ii)], true}; });
+    win[q].start(g, c, [&](int kk, int jj, int ii) { return Fetch1{*g.cell(xq, kk, jj, ii), true}; });
   }
   // boundary_condition_source: the x = 0 cell takes the x = 1 cell's sum
   const bool src = (flags & HFM_SUM_SOURCE_BC) && g.W > 1;
-  for (int k = c.k0; k < c.k1; ++k) {
+  walk3(c, [&](auto ph, int k) {
+    constexpr int P = decltype(ph)::value;
     float acc = 0.0f;
 #pragma unroll
     for (int q = 0; q < NF; ++q) {
       const float* xq = xs[q];
-      win[q].advance(g, c, k, [&](int kk, int jj, int ii) { return Fetch1{xq[g.idx(kk, jj, ii)], true}; });
+      win[q].template advance<P>(
+          g, c, k, [&](int kk, int jj, int ii) { return Fetch1{*g.cell(xq, kk, jj, ii), true}; });
     }
 #pragma unroll
     for (int q = 0; q < NF; ++q) {
       float t[27];
-      win[q].template taps<27>(t, 0);
+      win[q].template taps<27, P>(t, 0);
       acc += q == 0 ? wsum<T0>(K.k[0], t) : q == 1 ? wsum<T1>(K.k[1], t) : wsum<T2>(K.k[2], t);
-      win[q].shift();
     }
     const float from_right = __shfl_down(acc, 1, 64);
     if (src && c.i == 0) acc = from_right;
-    if (c.out) y[g.idx(k, c.j, c.i)] = scale * acc;
-  }
+    if (c.out && k < c.k1) *g.cell(y, k, c.j, c.i) = scale * acc;
+  });
 }
 
 // The urban step's pressure gradient for the three velocity components in one
@@ -839,7 +907,7 @@ __global__ __launch_bounds__(kBX* kBY) void k_grad3(const float* __restrict__ p,
     const int k2 = kt ? g.D - 2 : (kb ? 1 : kk);
     const int j2 = jt ? g.H - 2 : (jb ? 1 : jj);
     const int i2 = ib ? 1 : ii;
-    return Fetch1{p[g.idx(k2, j2, i2)], !zero};
+    return Fetch1{*g.cell(p, k2, j2, i2), !zero};
   };
   const float* fin[3] = {f0, f1, f2};
   float* out[3] = {o0, o1, o2};
@@ -855,26 +923,28 @@ __global__ __launch_bounds__(kBX* kBY) void k_grad3(const float* __restrict__ p,
     const int kc = min(kk, g.D - 1);
 #pragma unroll
     for (int ry = 0; ry < 2; ++ry) {
-      const int64_t n = g.idx(kc, min(c.j + ry, g.H - 1), ic);
+      const int jr = min(c.j + ry, g.H - 1);
 #pragma unroll
-      for (int q = 0; q < 3; ++q) cv.f[ry][q] = fin[q][n];
-      cv.s[ry] = sg[n];
+      for (int q = 0; q < 3; ++q) cv.f[ry][q] = *g.cell(fin[q], kc, jr, ic);
+      cv.s[ry] = *g.cell(sg, kc, jr, ic);
     }
   };
-  Win27<2> win;
+  constexpr int PF = kWalkPF;
+  Win27<2, Fetch1, PF> win;
   win.start(g, c, pat);
-  Cells cur, nxt;
-  load_cells(c.k0, cur);
-  for (int k = c.k0; k < c.k1; ++k) {
-    win.advance(g, c, k, pat);
-    load_cells(k + 1, nxt);
+  Cells rec[3];  // the cells of the plane of phase P in rec[P], loaded PF steps ahead
+  for (int d = 0; d < PF; ++d) load_cells(c.k0 + d, rec[d]);
+  walk3(c, [&](auto ph, int k) {
+    constexpr int P = decltype(ph)::value;
+    const Cells& cur = rec[P];
+    win.template advance<P>(g, c, k, pat);
+    load_cells(k + PF, rec[(P + PF) % 3]);
 #pragma unroll
     for (int ry = 0; ry < 2; ++ry) {
       float v[27];
-      win.taps<27>(v, ry);
+      win.template taps<27, P>(v, ry);
       const int j = c.j + ry;
-      if (c.out && j < g.H) {
-        const int64_t n = g.idx(k, j, c.i);
+      if (c.out && k < c.k1 && j < g.H) {
         const float s = cur.s[ry];
         const bool src_u = j == 1 || j == g.H - 2 || k == g.D - 2, src_v = k == g.D - 2;
 #pragma unroll
@@ -886,14 +956,12 @@ __global__ __launch_bounds__(kBX* kBY) void k_grad3(const float* __restrict__ p,
           float o = fv - ax;
           const bool src = q == 0 ? src_u : q == 1 ? src_v : false;
           if ((flags & HFM_GRAD_POST_BLUFF) && !src) o = bluff_apply(o, s, dt);
-          out[q][n] = o;
+          *g.cell(out[q], k, j, c.i) = o;
         }
-        if (pout) pout[n] = win.centre(ry);
+        if (pout) *g.cell(pout, k, j, c.i) = win.template centre<P>(ry);
       }
     }
-    cur = nxt;
-    win.shift();
-  }
+  });
 }
 
 // Boundary fills: the value of the last reference assignment covering a cell.
@@ -1105,8 +1173,11 @@ using namespace hfm;
 
 namespace {
 
+// a plane of fewer than 2^30 cells: the kernels address a cell as its plane's
+// base plus a 32-bit byte offset
 bool dims_ok(int64_t D, int64_t H, int64_t W) {
-  return D > 0 && H > 0 && W > 0 && D <= 65535 && D * H * W <= INT32_MAX;
+  return D > 0 && H > 0 && W > 0 && D <= 65535 && D * H * W <= INT32_MAX &&
+         H * W < (int64_t(1) << 30);
 }
 
 dim3 grid_for(const Grid3& g) {
@@ -1349,6 +1420,17 @@ extern "C" int hfm_prolong_correct_jacobi_f32(const float* x, const float* wc, c
                         W, flags, stream);
 }
 
+// k_advect27_v with the first-derivative tap sets when the three advection
+// stencils hold them (mask[1..3] of K), else over all 27 taps
+template <int NF, bool DIS, typename... A>
+void launch_advect27(const Kern4& K, const Grid3& g, hipStream_t s, A... args) {
+  if (tapset3(K.mask + 1) == kSetXYZ)
+    hipLaunchKernelGGL((k_advect27_v<NF, DIS, kTapsX, kTapsY, kTapsZ>), grid_col<1>(g),
+                       dim3(kBX, kBY), 0, s, args...);
+  else
+    hipLaunchKernelGGL((k_advect27_v<NF, DIS>), grid_col<1>(g), dim3(kBX, kBY), 0, s, args...);
+}
+
 static int advect27(const char* what, const float* const vel[3], const float* const* phi,
                     const float* const* base, float* const* out, int nf, const float K[4][27],
                     float f, const float* sigma, float dt, int post, int64_t D, int64_t H,
@@ -1384,16 +1466,15 @@ static int advect27(const char* what, const float* const vel[3], const float* co
   }
   const Grid3 g{int(D), int(H), int(W)};
   hipStream_t s = as_stream(stream);
+  const float* const none = nullptr;
   if (nf == 3)
-    hipLaunchKernelGGL((k_advect27_v<3, false>), grid_col<1>(g), dim3(kBX, kBY), 0, s, vel[0],
-                       vel[1], vel[2], pp.a[0], pp.a[1], pp.a[2], bb.a[0], bb.a[1], bb.a[2],
-                       oo.a[0], oo.a[1], oo.a[2], kk, f, g, nullptr, nullptr, nullptr, sigma, dt,
-                       post);
+    launch_advect27<3, false>(kk, g, s, vel[0], vel[1], vel[2], pp.a[0], pp.a[1], pp.a[2],
+                              bb.a[0], bb.a[1], bb.a[2], oo.a[0], oo.a[1], oo.a[2], kk, f, g,
+                              none, none, none, sigma, dt, post);
   else
-    hipLaunchKernelGGL((k_advect27_v<1, false>), grid_col<1>(g), dim3(kBX, kBY), 0, s, vel[0],
-                       vel[1], vel[2], pp.a[0], pp.a[1], pp.a[2], bb.a[0], bb.a[1], bb.a[2],
-                       oo.a[0], oo.a[1], oo.a[2], kk, f, g, nullptr, nullptr, nullptr, sigma, dt,
-                       post);
+    launch_advect27<1, false>(kk, g, s, vel[0], vel[1], vel[2], pp.a[0], pp.a[1], pp.a[2],
+                              bb.a[0], bb.a[1], bb.a[2], oo.a[0], oo.a[1], oo.a[2], kk, f, g,
+                              none, none, none, sigma, dt, post);
   return launch_status(what);
 }
 
@@ -1445,14 +1526,15 @@ extern "C" int hfm_advect27_dis_f32(const float* const vel[3], const float* cons
   }
   const Grid3 g{int(D), int(H), int(W)};
   hipStream_t s = as_stream(stream);
+  const float* const none = nullptr;
   if (nf == 3)
-    hipLaunchKernelGGL((k_advect27_v<3, true>), grid_col<1>(g), dim3(kBX, kBY), 0, s, vel[0], vel[1],
-                       vel[2], pp[0], pp[1], pp[2], bb[0], bb[1], bb[2], oo[0], oo[1], oo[2], kk, f,
-                       g, dd[0], dd[1], dd[2], nullptr, 0.0f, 0);
+    launch_advect27<3, true>(kk, g, s, vel[0], vel[1], vel[2], pp[0], pp[1], pp[2], bb[0], bb[1],
+                             bb[2], oo[0], oo[1], oo[2], kk, f, g, dd[0], dd[1], dd[2], none, 0.0f,
+                             0);
   else
-    hipLaunchKernelGGL((k_advect27_v<1, true>), grid_col<1>(g), dim3(kBX, kBY), 0, s, vel[0], vel[1],
-                       vel[2], pp[0], pp[1], pp[2], bb[0], bb[1], bb[2], oo[0], oo[1], oo[2], kk, f,
-                       g, dd[0], dd[1], dd[2], nullptr, 0.0f, 0);
+    launch_advect27<1, true>(kk, g, s, vel[0], vel[1], vel[2], pp[0], pp[1], pp[2], bb[0], bb[1],
+                             bb[2], oo[0], oo[1], oo[2], kk, f, g, dd[0], dd[1], dd[2], none, 0.0f,
+                             0);
   return launch_status("hfm_advect27_dis_f32");
 }
 

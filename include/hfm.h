@@ -12,6 +12,9 @@
  *   - Every entry point returns 0 on success or a negative HFM_E* code and
  *     never throws.  hfm_last_error() returns a thread-local message.
  *   - Nothing here allocates device memory; all buffers are caller-owned.
+ *   - Reference-mode grids: 1 <= D <= 65535, D*H*W <= 2^31 - 1 cells and
+ *     H*W < 2^30 cells per plane (a cell is addressed as its plane's base
+ *     plus a 32-bit byte offset); other extents return HFM_EINVAL.
  *
  * Reference mode (the reference's Keras layers; FP32):
  *   hfm_conv27_f32          <- Conv3D/Conv2D/Conv1D(k=3, s=1, 'SAME') layers:

@@ -78,6 +78,14 @@ def main(path):
             rc = tile(*shape, mp, C.byref(r), C.byref(z), C.byref(n))
             if rc != 0 or r.value <= 0 or z.value <= 0 or n.value <= 0:
                 bad.append(f"hfm_mg_tile{shape}")
+    # a single plane of 2^30 cells: within the cell limit (2^31 - 1), beyond the
+    # plane limit (a cell is its plane's base plus a 32-bit byte offset)
+    conv = lib.hfm_conv27_f32
+    conv.argtypes = L.SIGNATURES["hfm_conv27_f32"][1]
+    K = (C.c_float * 27)()
+    rc = conv(C.c_void_p(8), C.c_void_p(16), None, K, 1.0, 1, 32768, 32768, 0, None)
+    if rc >= 0:
+        bad.append("hfm_conv27_f32(1 x 32768 x 32768)")
     # a message longer than any buffer the library keeps
     lib.hfm_bc_velocity_inplace_noalloc_f32.argtypes = \
         L.SIGNATURES["hfm_bc_velocity_inplace_noalloc_f32"][1]
