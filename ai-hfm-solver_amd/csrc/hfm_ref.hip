@@ -59,6 +59,24 @@ struct Grid3 {
     return reinterpret_cast<T*>(reinterpret_cast<B*>(plane) +
                                 uint32_t(j * W + i) * uint32_t(sizeof(T)));
   }
+  // cell() for a wave-uniform plane k (the column walkers' planes): the
+  // in-plane offset is tied to k by an empty asm statement, so it is formed
+  // next to the access and the access takes the plane's base in scalar
+  // registers plus the offset (global_load ... saddr), instead of a
+  // loop-invariant 64-bit offset hoisted out of the walk and added to the
+  // base with a VALU instruction per access.  That frees the offsets' vector
+  // registers but holds a base per field in scalar registers, which pays
+  // where it lifts occupancy (k_pg_coef 3 -> 4 waves per SIMD, k_grad3) and
+  // costs elsewhere.  k must be the same in every lane (the asm takes it in
+  // a scalar register).
+  template <typename T>
+  __device__ __forceinline__ T* cellu(T* x, int k, int j, int i) const {
+    using B = std::conditional_t<std::is_const_v<T>, const char, char>;
+    T* plane = x + int64_t(k) * (int64_t(H) * W);
+    uint32_t o = uint32_t(j * W + i) * uint32_t(sizeof(T));
+    asm("" : "+v"(o) : "s"(k));
+    return reinterpret_cast<T*>(reinterpret_cast<B*>(plane) + o);
+  }
   // without short-circuit evaluation (no branches between a cell's loads)
   __device__ __forceinline__ bool in(int k, int j, int i) const {
     return (unsigned(k) < unsigned(D)) & (unsigned(j) < unsigned(H)) & (unsigned(i) < unsigned(W));
@@ -125,6 +143,45 @@ __device__ __forceinline__ float wsum(const Weights& K, const float v[27]) {
   for (int t = 0; t < 27; ++t)
     if (tap_on<TAPS>(t)) acc += K.w[t] * v[t];
   return acc;
+}
+
+// The weights of NK stencils in LDS, for a walker whose weights and field
+// pointers exceed the scalar registers (the compiler would keep part of them
+// in vector-register lanes and read each back with a v_readlane, a VALU
+// instruction, at every use).  Staged once per workgroup (every wave takes
+// part in the barrier, before any returns), read per step as broadcasts (up
+// to four contiguous weights per ds_read).  The index offset zero4() is
+// redefined at every step by an empty asm statement, so the reads stay at
+// their use instead of being hoisted out of the walk into NK * 27 live
+// registers.
+template <int NK>
+struct WeightLds {
+  float w[NK][28];  // rows of 28 floats: 16-byte aligned
+  __device__ __forceinline__ void stage(const Weights* k) {
+    const int t = int(threadIdx.x) + kBX * int(threadIdx.y);
+    for (int q = t; q < NK * 27; q += kBX * kBY) w[q / 27][q % 27] = k[q / 27].w[q % 27];
+    __syncthreads();
+  }
+};
+__device__ __forceinline__ int zero4() {
+  int z = 0;
+  asm volatile("" : "+v"(z));
+  return z << 2;
+}
+// sum_t K[t] v[t] over the tap set in tap order (as wsum), with K from LDS,
+// for two tap vectors (two output rows): each weight read once for both
+template <int TAPS>
+__device__ __forceinline__ void wsum2_lds(const float* w, const float v0[27], const float v1[27],
+                                          float& a0, float& a1) {
+  a0 = 0.0f;
+  a1 = 0.0f;
+#pragma unroll
+  for (int t = 0; t < 27; ++t)
+    if (tap_on<TAPS>(t)) {
+      const float wt = w[t];
+      a0 += wt * v0[t];
+      a1 += wt * v1[t];
+    }
 }
 
 // sum_t K[t] x(k+a-1, j+b-1, i+c-1) over the tap set (zero ghosts)
@@ -353,13 +410,17 @@ __device__ __forceinline__ void walk3(const Col& c, Step&& step) {
 // one).  The walkers load them a plane ahead, as the window's planes: a value
 // loaded in one step and used in the same step makes the walk wait there for
 // every load in flight, the window's next plane included.
-template <int RY>
+// (U: through Grid3::cellu)
+template <int RY, bool U = false>
 __device__ __forceinline__ void own_cells(const Grid3& g, const Col& c, int k, const float* p,
                                           float v[RY]) {
   const int ic = c.in ? c.i : 0;
   const int kc = min(k, g.D - 1);
 #pragma unroll
-  for (int ry = 0; ry < RY; ++ry) v[ry] = *g.cell(p, kc, min(c.j + ry, g.H - 1), ic);
+  for (int ry = 0; ry < RY; ++ry) {
+    const int jr = min(c.j + ry, g.H - 1);
+    v[ry] = U ? *g.cellu(p, kc, jr, ic) : *g.cell(p, kc, jr, ic);
+  }
 }
 
 // (x - A/d) + b/d, the reference's evaluation order
@@ -657,19 +718,19 @@ __global__ __launch_bounds__(kBX* kBY) void k_pg_coef(const float* __restrict__ 
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
     const float* x = f[q];
-    win[q].start(g, c, [&](int kk, int jj, int ii) { return Fetch1{*g.cell(x, kk, jj, ii), true}; });
+    win[q].start(g, c, [&](int kk, int jj, int ii) { return Fetch1{*g.cellu(x, kk, jj, ii), true}; });
   }
   float sv[3][1];  // sigma of the plane of phase P in sv[P], loaded PF steps ahead
-  for (int d = 0; d < PF; ++d) own_cells<1>(g, c, c.k0 + d, sigma, sv[d]);
+  for (int d = 0; d < PF; ++d) own_cells<1, true>(g, c, c.k0 + d, sigma, sv[d]);
   walk3(c, [&](auto ph, int k) {
     constexpr int P = decltype(ph)::value;
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
       const float* x = f[q];
       win[q].template advance<P>(
-          g, c, k, [&](int kk, int jj, int ii) { return Fetch1{*g.cell(x, kk, jj, ii), true}; });
+          g, c, k, [&](int kk, int jj, int ii) { return Fetch1{*g.cellu(x, kk, jj, ii), true}; });
     }
-    own_cells<1>(g, c, k + PF, sigma, sv[(P + PF) % 3]);
+    own_cells<1, true>(g, c, k + PF, sigma, sv[(P + PF) % 3]);
     const float speed = (1.0f / 3.0f) * ((fabsf(win[0].template centre<P>(0)) +
                                           fabsf(win[1].template centre<P>(0))) +
                                          fabsf(win[2].template centre<P>(0)));
@@ -684,7 +745,7 @@ __global__ __launch_bounds__(kBX* kBY) void k_pg_coef(const float* __restrict__ 
       const float az = wsum<TZ>(K.k[3], nb);
       const float num = 2.0f * fabsf(speed * dif);
       const float den = eps + ((fabsf(ax) + fabsf(ay)) + fabsf(az)) / 3.0f;
-      if (c.out && k < c.k1) *g.cell(out[q], k, c.j, c.i) = fminf(num / den, 0.5f) / damp;
+      if (c.out && k < c.k1) *g.cellu(out[q], k, c.j, c.i) = fminf(num / den, 0.5f) / damp;
     }
   });
 }
@@ -836,6 +897,7 @@ struct Kern3 {
   uint32_t mask[3];
 };
 
+
 template <int NF, int T0 = 27, int T1 = 27, int T2 = 27>
 __global__ __launch_bounds__(kBX* kBY) void k_conv27_sum_v(const float* x0, const float* x1,
                                                            const float* x2, float* __restrict__ y,
@@ -890,6 +952,8 @@ __global__ __launch_bounds__(kBX* kBY) void k_grad3(const float* __restrict__ p,
                                                     float* o2, float* __restrict__ pout, Kern3 K,
                                                     const float* __restrict__ sigma, float dt,
                                                     Grid3 g, int flags) {
+  __shared__ __align__(16) WeightLds<3> kw;
+  kw.stage(K.k);
   Col c;
   if (!col_begin<2>(g, c)) return;
   const bool pbc = flags & HFM_GRAD_P_BC;
@@ -907,7 +971,7 @@ __global__ __launch_bounds__(kBX* kBY) void k_grad3(const float* __restrict__ p,
     const int k2 = kt ? g.D - 2 : (kb ? 1 : kk);
     const int j2 = jt ? g.H - 2 : (jb ? 1 : jj);
     const int i2 = ib ? 1 : ii;
-    return Fetch1{*g.cell(p, k2, j2, i2), !zero};
+    return Fetch1{*g.cellu(p, k2, j2, i2), !zero};
   };
   const float* fin[3] = {f0, f1, f2};
   float* out[3] = {o0, o1, o2};
@@ -925,8 +989,8 @@ __global__ __launch_bounds__(kBX* kBY) void k_grad3(const float* __restrict__ p,
     for (int ry = 0; ry < 2; ++ry) {
       const int jr = min(c.j + ry, g.H - 1);
 #pragma unroll
-      for (int q = 0; q < 3; ++q) cv.f[ry][q] = *g.cell(fin[q], kc, jr, ic);
-      cv.s[ry] = *g.cell(sg, kc, jr, ic);
+      for (int q = 0; q < 3; ++q) cv.f[ry][q] = *g.cellu(fin[q], kc, jr, ic);
+      cv.s[ry] = *g.cellu(sg, kc, jr, ic);
     }
   };
   constexpr int PF = kWalkPF;
@@ -939,26 +1003,32 @@ __global__ __launch_bounds__(kBX* kBY) void k_grad3(const float* __restrict__ p,
     const Cells& cur = rec[P];
     win.template advance<P>(g, c, k, pat);
     load_cells(k + PF, rec[(P + PF) % 3]);
+    // the two rows' gradient sums, each weight read once for both
+    float ax[2][3];
+    {
+      float v0[27], v1[27];
+      win.template taps<27, P>(v0, 0);
+      win.template taps<27, P>(v1, 1);
+      wsum2_lds<T0>(&kw.w[0][zero4()], v0, v1, ax[0][0], ax[1][0]);
+      wsum2_lds<T1>(&kw.w[1][zero4()], v0, v1, ax[0][1], ax[1][1]);
+      wsum2_lds<T2>(&kw.w[2][zero4()], v0, v1, ax[0][2], ax[1][2]);
+    }
 #pragma unroll
     for (int ry = 0; ry < 2; ++ry) {
-      float v[27];
-      win.template taps<27, P>(v, ry);
       const int j = c.j + ry;
       if (c.out && k < c.k1 && j < g.H) {
         const float s = cur.s[ry];
         const bool src_u = j == 1 || j == g.H - 2 || k == g.D - 2, src_v = k == g.D - 2;
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
-          const float ax = q == 0 ? wsum<T0>(K.k[0], v)
-                           : q == 1 ? wsum<T1>(K.k[1], v) : wsum<T2>(K.k[2], v);
           float fv = cur.f[ry][q];
           if (flags & HFM_GRAD_PRE_BLUFF) fv = bluff_apply(fv, s, dt);
-          float o = fv - ax;
+          float o = fv - ax[ry][q];
           const bool src = q == 0 ? src_u : q == 1 ? src_v : false;
           if ((flags & HFM_GRAD_POST_BLUFF) && !src) o = bluff_apply(o, s, dt);
-          *g.cell(out[q], k, j, c.i) = o;
+          *g.cellu(out[q], k, j, c.i) = o;
         }
-        if (pout) *g.cell(pout, k, j, c.i) = win.template centre<P>(ry);
+        if (pout) *g.cellu(pout, k, j, c.i) = win.template centre<P>(ry);
       }
     }
   });
