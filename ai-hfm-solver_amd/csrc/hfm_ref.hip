@@ -484,6 +484,56 @@ __global__ __launch_bounds__(kBX* kBY) void k_conv27(const float* __restrict__ x
   });
 }
 
+// The fine residual r = A x - b (k_conv27's RESID form: the same taps and
+// sums; with HFM_FORM_ZERO_OUTLET r = 0 on the x = W-1 plane) restricted in
+// the same walk to the next level, rc = k_restrict_mean(r) with factors
+// (2, 2, 2) and weight 1/8; r itself is not written.  A thread's two rows are
+// a row pair of the restriction, the x pairs are adjacent lanes (an output
+// tile starts on an even x) and the z pairs consecutive planes (chunks start
+// on even planes), and the eight children are summed in k_restrict_mean's
+// order (plane, row, cell), so rc is bitwise the two separate launches'.
+// D, H and W are even.
+template <int TAPS>
+__global__ __launch_bounds__(kBX* kBY) void k_resid_restrict27(const float* __restrict__ x,
+                                                               const float* __restrict__ b,
+                                                               float* __restrict__ rc, Weights K,
+                                                               Grid3 g, int form) {
+  Col c;
+  if (!col_begin<2>(g, c)) return;
+  auto xat = [&](int kk, int jj, int ii) { return Fetch1{*g.cell(x, kk, jj, ii), true}; };
+  constexpr int PF = kWalkPF;
+  Win27<2, Fetch1, PF> win;
+  win.start(g, c, xat);
+  float bv[3][2];  // b of the plane of phase P in bv[P], loaded PF steps ahead
+  for (int d = 0; d < PF; ++d) own_cells<2>(g, c, c.k0 + d, b, bv[d]);
+  const bool zo = (form & HFM_FORM_ZERO_OUTLET) != 0;
+  const Grid3 gc{g.D / 2, g.H / 2, g.W / 2};
+  const bool owner = c.out && (c.i & 1) == 0;  // the lane of a pair's even x
+  constexpr float wgt = 1.0f / 8.0f;
+  float acc = 0.0f;  // the coarse cell's sum over a plane pair
+  walk3(c, [&](auto ph, int k) {
+    constexpr int P = decltype(ph)::value;
+    win.template advance<P>(g, c, k, xat);
+    own_cells<2>(g, c, k + PF, b, bv[(P + PF) % 3]);
+    float r[2];
+#pragma unroll
+    for (int ry = 0; ry < 2; ++ry) {
+      float v[27];
+      win.template taps<TAPS, P>(v, ry);
+      const float ax = wsum<TAPS>(K, v);
+      r[ry] = ax - bv[P][ry];
+      if (zo && c.i == g.W - 1) r[ry] = 0.0f;
+    }
+    const float r0x = __shfl_down(r[0], 1, 64), r1x = __shfl_down(r[1], 1, 64);
+    if (!(k & 1)) acc = 0.0f;
+    acc += wgt * r[0];
+    acc += wgt * r0x;
+    acc += wgt * r[1];
+    acc += wgt * r1x;
+    if ((k & 1) && k < c.k1 && owner) *gc.cell(rc, k >> 1, c.j >> 1, c.i >> 1) = acc;
+  });
+}
+
 __global__ __launch_bounds__(kBX* kBY) void k_var6(const float* __restrict__ x,
                                                    float* __restrict__ y,
                                                    const float* __restrict__ b, Var6 V, Grid3 g,
@@ -1359,6 +1409,29 @@ extern "C" int hfm_restrict_mean_f32(const float* x, float* y, int64_t D, int64_
   hipLaunchKernelGGL(k_restrict_mean, grid_for(gc), dim3(kBX, kBY), 0, as_stream(stream), x, y,
                      gf, gc, fz, fy, fx, 1.0f / float(fz * fy * fx));
   return launch_status("hfm_restrict_mean_f32");
+}
+
+extern "C" int hfm_resid_restrict_f32(const float* x, const float* b, float* rc,
+                                      const float K[27], int64_t D, int64_t H, int64_t W,
+                                      int flags, void* stream) {
+  Weights w;
+  uint32_t mask;
+  if (!x || !b || !rc || rc == x || rc == b || !dims_ok(D, H, W) || !make_weights(K, w, mask) ||
+      D % 2 || H % 2 || W % 2 || (flags & ~HFM_FORM_ZERO_OUTLET)) {
+    set_error("hfm_resid_restrict_f32: bad arguments (D,H,W)=(%lld,%lld,%lld) flags=%d",
+              (long long)D, (long long)H, (long long)W, flags);
+    return HFM_EINVAL;
+  }
+  const Grid3 g{int(D), int(H), int(W)};
+  auto go = [&](auto taps) {
+    hipLaunchKernelGGL(k_resid_restrict27<decltype(taps)::value>, grid_col<2>(g),
+                       dim3(kBX, kBY), 0, as_stream(stream), x, b, rc, w, g, flags);
+  };
+  if (taps_of(mask) == 7)
+    go(std::integral_constant<int, 7>{});
+  else
+    go(std::integral_constant<int, 27>{});
+  return launch_status("hfm_resid_restrict_f32");
 }
 
 extern "C" int hfm_prolong_nearest_f32(const float* xc, float* y, int64_t Dc, int64_t Hc,

@@ -66,6 +66,7 @@ SIGNATURES = {
     "hfm_conv27_f32": (_int, [_vp, _vp, _vp, _vp, _f32, _i64, _i64, _i64, _int, _vp]),
     "hfm_var6_f32": (_int, [_vp, _vp, _vp, _vp, _vp, _i64, _i64, _i64, _int, _vp]),
     "hfm_restrict_mean_f32": (_int, [_vp, _vp, _i64, _i64, _i64, _int, _int, _int, _vp]),
+    "hfm_resid_restrict_f32": (_int, [_vp, _vp, _vp, _vp, _i64, _i64, _i64, _int, _vp]),
     "hfm_prolong_nearest_f32": (_int, [_vp, _vp, _i64, _i64, _i64, _int, _int, _int, _vp]),
     "hfm_prolong_upsum_f32": (_int, [_vp, _vp, _i64, _i64, _i64, _int, _int, _int, _vp]),
     "hfm_sum_f64": (_int, [_vp, _i64, _vp, _vp]),

@@ -102,6 +102,27 @@ class FixedConv:
         """``A(x) - b`` in one pass (the reference's ``r = CNN3D_A(values) - b``)."""
         return self._form(x, b, "resid", zero_outlet)
 
+    def residual_restricted(self, x, b, zero_outlet: bool = False):
+        """``restrict_mean(A(x) - b)`` in one pass, without the fine residual
+        (UM/AI_HFM_v3.ipynb:2573-2575: ``r = CNN3D_A_512(values_p) - b``, the
+        outlet plane zeroed, ``r_256 = CNN3D_res_512(r)``); bitwise the
+        ``residual`` + ``restrict_mean`` pair.  3D fields with even extents."""
+        x, b = _prep(x), _prep(b, "b")
+        D, H, W = self._check(x)
+        (_, _, _), nd = _spatial(x)
+        if b.shape != x.shape:
+            raise ValueError("b must have the shape of x")
+        if nd != 3 or D % 2 or H % 2 or W % 2:
+            raise ValueError("residual_restricted needs a 3D field with even extents")
+        out_shape = list(x.shape)
+        off = 1 if x.dim() > nd else 0
+        for a in range(nd):
+            out_shape[off + a] //= 2
+        y = torch.empty(out_shape, dtype=torch.float32, device=x.device)
+        L.call("hfm_resid_restrict_f32", x.data_ptr(), b.data_ptr(), y.data_ptr(), _kptr(self.K),
+               D, H, W, L.FORM_ZERO_OUTLET if zero_outlet else 0, L.stream_handle(x.device))
+        return y
+
     def subtract_from(self, b, x):
         """``b - A(x)`` in one pass (the reference's ``values_u - CNN3D_pu(values_p)``)."""
         return self._form(x, b, "subtract", False)
@@ -361,11 +382,18 @@ class SawtoothMG:
         sweep the finest correction (the prolonged ``record[1]``) is never
         formed, and ``record[0]`` is left unset."""
         values, b = _prep(values, "values"), _prep(b, "b")
-        r = self.ops[0].residual(values, b, zero_outlet=self.zero_outlet) if isinstance(
-            self.ops[0], FixedConv) else self.ops[0].residual(values, b)
         nlev = len(self.ops)
-        rs = [r]
-        for _ in range(1, nlev):
+        (D, H, W), nd = _spatial(values)
+        if (not self.fine_sweep and nlev > 1 and isinstance(self.ops[0], FixedConv) and nd == 3
+                and D % 2 == 0 and H % 2 == 0 and W % 2 == 0):
+            # the finest level takes no sweep: its residual has no reader but
+            # the restriction, and the two run as one pass
+            rs = [None, self.ops[0].residual_restricted(values, b, zero_outlet=self.zero_outlet)]
+        else:
+            r = self.ops[0].residual(values, b, zero_outlet=self.zero_outlet) if isinstance(
+                self.ops[0], FixedConv) else self.ops[0].residual(values, b)
+            rs = [r]
+        while len(rs) < nlev:
             rs.append(restrict_mean(rs[-1]))
         w = torch.zeros_like(rs[-1])
         for _ in range(self.coarsest_sweeps):

@@ -28,6 +28,8 @@
  *   hfm_var6_f32            <- A_operator_n (ART/BGS_multigrid_v1.ipynb#c40) and its
  *                              Jacobi sweep with divisor A_MAT_TOT (#c42).
  *   hfm_restrict_mean_f32   <- CNN3D_res_n (…7point.ipynb:194-235).
+ *   hfm_resid_restrict_f32  <- "r = CNN3D_A_512(values_p) - b; r_256 = CNN3D_res_512(r)"
+ *                              fused (UM/AI_HFM_v3.ipynb:2573-2575).
  *   hfm_prolong_nearest_f32 <- CNN3D_prol_n / UpSampling3D (…7point.ipynb:251-284).
  *   hfm_prolong_jacobi_f32  <- "w_f = prol(w_c); w_f = w_f - A(w_f)/d + r_f/d"
  *                              fused (…7point.ipynb:426-444).
@@ -108,6 +110,16 @@ int hfm_var6_f32(const float* x, float* y, const float* b, const float* const m[
 /* y (D/fz, H/fy, W/fx) = mean of the fz*fy*fx children (factors 1 or 2). */
 int hfm_restrict_mean_f32(const float* x, float* y, int64_t D, int64_t H, int64_t W,
                           int fz, int fy, int fx, void* stream);
+
+/* rc (D/2, H/2, W/2) = restrict_mean(K (*) x - b) with factors (2, 2, 2), in
+ * one pass and without writing the fine residual: the sawtooth cycle's
+ * "r = CNN3D_A_512(values_p) - b; r[..., nx-1] = 0; r_256 = CNN3D_res_512(r)"
+ * (UM/AI_HFM_v3.ipynb:2573-2575), for the cycles whose finest level takes no
+ * sweep of its own (the fine residual has no other reader).  Bitwise the
+ * hfm_conv27_f32 (HFM_FORM_RESID) + hfm_restrict_mean_f32 pair; flags may be
+ * HFM_FORM_ZERO_OUTLET.  D, H and W even; rc must not alias x or b. */
+int hfm_resid_restrict_f32(const float* x, const float* b, float* rc, const float K[27],
+                           int64_t D, int64_t H, int64_t W, int flags, void* stream);
 
 /* y (Dc*fz, Hc*fy, Wc*fx) = nearest-neighbour repeat of x (Dc, Hc, Wc). */
 int hfm_prolong_nearest_f32(const float* xc, float* y, int64_t Dc, int64_t Hc, int64_t Wc,

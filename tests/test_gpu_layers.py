@@ -53,6 +53,43 @@ def test_conv27_forms(dev, name, shape):
     assert rel(H(lay.jacobi(xt, bt, d))[0, ..., 0], ref) < TOL
 
 
+@pytest.mark.parametrize("name", ["poisson27", "a7", "rand"])
+@pytest.mark.parametrize("shape", [(18, 16, 124), (20, 10, 70), (2, 2, 2), (38, 6, 130)])
+@pytest.mark.parametrize("zero_outlet", [False, True])
+def test_resid_restrict_equals_separate_passes_bitwise(dev, name, shape, zero_outlet):
+    """hfm_resid_restrict_f32 (the urban cycle's residual and its restriction in
+    one pass, UM/AI_HFM_v3.ipynb:2573-2575) is bitwise the RESID launch followed
+    by restrict_mean, on grids whose planes, rows and columns end inside a
+    walker's chunk, tile and wave; and both match the oracle."""
+    import torch
+    from hfm import layers
+    rng = np.random.default_rng([len(name), *shape, int(zero_outlet)])
+    x = rng.standard_normal(shape).astype(np.float32)
+    b = rng.standard_normal(shape).astype(np.float32)
+    lay = layers.FixedConv(KERNELS[name])
+    xt, bt = T(x[None, ..., None], dev), T(b[None, ..., None], dev)
+    sep = layers.restrict_mean(lay.residual(xt, bt, zero_outlet=zero_outlet))
+    fused = lay.residual_restricted(xt, bt, zero_outlet=zero_outlet)
+    torch.cuda.synchronize()
+    assert fused.shape == sep.shape
+    assert torch.equal(fused, sep)
+    r = O.conv_same(x.astype(np.float64), KERNELS[name]) - b
+    if zero_outlet:
+        r = O.zero_outlet(r)
+    # the mean of eight FP32 residuals can cancel: the error is bounded by the
+    # children's size, not the mean's
+    err = np.abs(H(fused)[0, ..., 0] - O.restrict_mean(r))
+    assert err.max() <= TOL * np.abs(r).max()
+
+
+def test_resid_restrict_rejects_odd_extents(dev):
+    from hfm import layers
+    lay = layers.FixedConv(KERNELS["poisson27"])
+    x = T(np.zeros((1, 4, 6, 7, 1)), dev)
+    with pytest.raises(ValueError):
+        lay.residual_restricted(x, x)
+
+
 def test_conv_2d_and_1d_layouts(dev):
     from hfm import layers
     K2 = O.implicit_ad2d_kernel()
