@@ -849,7 +849,11 @@ __global__ __launch_bounds__(kBX* kBY) void k_pg_flux(const float* f0, const flo
 // TX / TY / TZ: the tap sets of the three advection stencils; the fold skips
 // their structurally zero taps (54 FMAs instead of 81 for the first-derivative
 // sets), which leaves every finite result unchanged.
-template <int NF, bool DIS, int TX = 27, int TY = 27, int TZ = 27>
+// SH (three fields): 1 when the velocities are the advected fields (u, v, w =
+// phi, the momentum stages), 2 when the bases are as well (the predictor): the
+// cell's velocities (and bases) are then the windows' centres, not loads of
+// their own, which spares their loads, registers and pointers.
+template <int NF, bool DIS, int SH = 0, int TX = 27, int TY = 27, int TZ = 27>
 __global__ __launch_bounds__(kBX* kBY) void k_advect27_v(
     const float* __restrict__ u, const float* __restrict__ v, const float* __restrict__ w,
     const float* p0, const float* p1, const float* p2, const float* b0, const float* b1,
@@ -877,14 +881,17 @@ __global__ __launch_bounds__(kBX* kBY) void k_advect27_v(
   };
   const bool has_s = post && sigma;
   const float* sg = has_s ? sigma : u;
+  static_assert(SH == 0 || NF == 3, "shared velocities need three fields");
   auto load_cells = [&](int kk, Cells& cv) {
     const int kc = min(kk, g.D - 1), ic = c.in ? c.i : 0;
-    cv.u = *g.cell(u, kc, c.j, ic);
-    cv.v = *g.cell(v, kc, c.j, ic);
-    cv.w = *g.cell(w, kc, c.j, ic);
+    if (SH == 0) {
+      cv.u = *g.cell(u, kc, c.j, ic);
+      cv.v = *g.cell(v, kc, c.j, ic);
+      cv.w = *g.cell(w, kc, c.j, ic);
+    }
 #pragma unroll
     for (int q = 0; q < NF; ++q) {
-      cv.b[q] = *g.cell(base[q], kc, c.j, ic);
+      if (SH < 2) cv.b[q] = *g.cell(base[q], kc, c.j, ic);
       cv.d[q] = DIS ? *g.cell(dis[q], kc, c.j, ic) : 0.0f;
     }
     cv.s = *g.cell(sg, kc, c.j, ic);  // sigma (or u, unused): every load is issued
@@ -894,7 +901,11 @@ __global__ __launch_bounds__(kBX* kBY) void k_advect27_v(
   walk3(c, [&](auto ph, int k) {
     constexpr int P = decltype(ph)::value;
     const Cells& cur = rec[P];
-    const float uc = cur.u, vc = cur.v, wc = cur.w;
+    // the cell's velocities: the windows' centre plane (committed a step
+    // earlier) when they are the advected fields
+    const float uc = SH ? win[0].template centre<P>(0) : cur.u;
+    const float vc = SH ? win[NF > 1 ? 1 : 0].template centre<P>(0) : cur.v;
+    const float wc = SH ? win[NF > 2 ? 2 : 0].template centre<P>(0) : cur.w;
 #pragma unroll
     for (int q = 0; q < NF; ++q) {
       const float* x = phi[q];
@@ -917,7 +928,7 @@ __global__ __launch_bounds__(kBX* kBY) void k_advect27_v(
     for (int q = 0; q < NF; ++q) {
       float t[27];
       win[q].template taps<27, P>(t, 0);
-      const float bq = cur.b[q];
+      const float bq = SH == 2 ? win[q].template centre<P>(0) : cur.b[q];
       // DIS: the dissipation term is given (dis0..2 in place of the phi's
       // diffusion stencil, UM/INHALE_demo.ipynb cell 24)
       float a = DIS ? cur.d[q] : 0.0f;
@@ -1565,13 +1576,35 @@ extern "C" int hfm_prolong_correct_jacobi_f32(const float* x, const float* wc, c
 
 // k_advect27_v with the first-derivative tap sets when the three advection
 // stencils hold them (mask[1..3] of K), else over all 27 taps
-template <int NF, bool DIS, typename... A>
-void launch_advect27(const Kern4& K, const Grid3& g, hipStream_t s, A... args) {
+template <int NF, bool DIS, int SH, typename... A>
+void launch_advect27_sh(const Kern4& K, const Grid3& g, hipStream_t s, A... args) {
   if (tapset3(K.mask + 1) == kSetXYZ)
-    hipLaunchKernelGGL((k_advect27_v<NF, DIS, kTapsX, kTapsY, kTapsZ>), grid_col<1>(g),
+    hipLaunchKernelGGL((k_advect27_v<NF, DIS, SH, kTapsX, kTapsY, kTapsZ>), grid_col<1>(g),
                        dim3(kBX, kBY), 0, s, args...);
   else
-    hipLaunchKernelGGL((k_advect27_v<NF, DIS>), grid_col<1>(g), dim3(kBX, kBY), 0, s, args...);
+    hipLaunchKernelGGL((k_advect27_v<NF, DIS, SH>), grid_col<1>(g), dim3(kBX, kBY), 0, s,
+                       args...);
+}
+// sh: 1 when the three velocities are the three advected fields, 2 when the
+// bases are too (three fields only)
+template <int NF, bool DIS, typename... A>
+void launch_advect27(int sh, const Kern4& K, const Grid3& g, hipStream_t s, A... args) {
+  if (NF == 3 && sh == 2)
+    launch_advect27_sh<NF, DIS, NF == 3 ? 2 : 0>(K, g, s, args...);
+  else if (NF == 3 && sh == 1)
+    launch_advect27_sh<NF, DIS, NF == 3 ? 1 : 0>(K, g, s, args...);
+  else
+    launch_advect27_sh<NF, DIS, 0>(K, g, s, args...);
+}
+// the shared-velocity form of a launch (see k_advect27_v's SH)
+inline int advect_sharing(int nf, const float* const vel[3], const float* const* phi,
+                          const float* const* base) {
+  if (nf != 3) return 0;
+  for (int q = 0; q < 3; ++q)
+    if (vel[q] != phi[q]) return 0;
+  for (int q = 0; q < 3; ++q)
+    if (base[q] != phi[q]) return 1;
+  return 2;
 }
 
 static int advect27(const char* what, const float* const vel[3], const float* const* phi,
@@ -1610,12 +1643,13 @@ static int advect27(const char* what, const float* const vel[3], const float* co
   const Grid3 g{int(D), int(H), int(W)};
   hipStream_t s = as_stream(stream);
   const float* const none = nullptr;
+  const int sh = advect_sharing(nf, vel, phi, base);
   if (nf == 3)
-    launch_advect27<3, false>(kk, g, s, vel[0], vel[1], vel[2], pp.a[0], pp.a[1], pp.a[2],
+    launch_advect27<3, false>(sh, kk, g, s, vel[0], vel[1], vel[2], pp.a[0], pp.a[1], pp.a[2],
                               bb.a[0], bb.a[1], bb.a[2], oo.a[0], oo.a[1], oo.a[2], kk, f, g,
                               none, none, none, sigma, dt, post);
   else
-    launch_advect27<1, false>(kk, g, s, vel[0], vel[1], vel[2], pp.a[0], pp.a[1], pp.a[2],
+    launch_advect27<1, false>(0, kk, g, s, vel[0], vel[1], vel[2], pp.a[0], pp.a[1], pp.a[2],
                               bb.a[0], bb.a[1], bb.a[2], oo.a[0], oo.a[1], oo.a[2], kk, f, g,
                               none, none, none, sigma, dt, post);
   return launch_status(what);
@@ -1670,14 +1704,15 @@ extern "C" int hfm_advect27_dis_f32(const float* const vel[3], const float* cons
   const Grid3 g{int(D), int(H), int(W)};
   hipStream_t s = as_stream(stream);
   const float* const none = nullptr;
+  const int sh = advect_sharing(nf, vel, phi, base);
   if (nf == 3)
-    launch_advect27<3, true>(kk, g, s, vel[0], vel[1], vel[2], pp[0], pp[1], pp[2], bb[0], bb[1],
-                             bb[2], oo[0], oo[1], oo[2], kk, f, g, dd[0], dd[1], dd[2], none, 0.0f,
-                             0);
+    launch_advect27<3, true>(sh, kk, g, s, vel[0], vel[1], vel[2], pp[0], pp[1], pp[2], bb[0],
+                             bb[1], bb[2], oo[0], oo[1], oo[2], kk, f, g, dd[0], dd[1], dd[2],
+                             none, 0.0f, 0);
   else
-    launch_advect27<1, true>(kk, g, s, vel[0], vel[1], vel[2], pp[0], pp[1], pp[2], bb[0], bb[1],
-                             bb[2], oo[0], oo[1], oo[2], kk, f, g, dd[0], dd[1], dd[2], none, 0.0f,
-                             0);
+    launch_advect27<1, true>(0, kk, g, s, vel[0], vel[1], vel[2], pp[0], pp[1], pp[2], bb[0],
+                             bb[1], bb[2], oo[0], oo[1], oo[2], kk, f, g, dd[0], dd[1], dd[2],
+                             none, 0.0f, 0);
   return launch_status("hfm_advect27_dis_f32");
 }
 

@@ -229,6 +229,35 @@ def test_advect_post_equals_separate_passes_bitwise(dev, indicator, nbluff):
     assert torch.equal(got, ref)
 
 
+@pytest.mark.parametrize("dissipative", [False, True])
+@pytest.mark.parametrize("bases_shared", [False, True])
+def test_advect_shared_velocities_bitwise(dev, dissipative, bases_shared):
+    """The momentum stages pass the advected fields as the velocities (and, in
+    the predictor, as the bases): the kernel then takes the cell's velocities and
+    bases from its windows instead of loading them.  Bitwise the same launch with
+    distinct copies of those fields."""
+    import torch
+    from hfm import layers
+    from hfm.timestep import urban_kernels
+    k = urban_kernels(0.4, 1.0, 1 / 20)
+    (u, v, w, a, _), sig = _fields((20, 10, 70), 8, dev)
+    phis = (u, v, w)
+    bases = phis if bases_shared else (a, a.clone(), a.clone())
+    copies = tuple(t.clone() for t in phis)
+    bcopies = tuple(t.clone() for t in bases)
+    if dissipative:
+        dis = (a, sig, a)
+        got = layers.advect_dissipative(phis, phis, bases, dis, 0.5, k["adv"])
+        ref = layers.advect_dissipative(copies, phis, bcopies, dis, 0.5, k["adv"])
+    else:
+        K = (k["dif"], *k["adv"])
+        got = layers.advect(phis, phis, bases, 0.5, K)
+        ref = layers.advect(copies, phis, bcopies, 0.5, K)
+    torch.cuda.synchronize()
+    for g_, r_ in zip(got, ref):
+        assert torch.equal(g_, r_)
+
+
 @pytest.mark.parametrize("driver", ["urban", "urban_noscalar", "inhale", "projection"])
 def test_fused_step_equals_layer_step_bitwise(dev, driver):
     """Two time steps of each driver with the fused passes against the same step
