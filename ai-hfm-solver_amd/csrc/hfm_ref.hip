@@ -1290,11 +1290,23 @@ __global__ __launch_bounds__(256) void k_bc_velocity_src_bluff(float* u, float* 
   }
 }
 
-__global__ __launch_bounds__(256) void k_bluff(const float* __restrict__ in, float* out,
-                                               const float* __restrict__ sigma, float dt,
-                                               int64_t n) {
+// bluff_body on NF (1 to 3) fields in one pass: sigma is read once per cell
+// for all of them (an output may be its input)
+struct BluffFields {
+  const float* in[3];
+  float* out[3];
+};
+template <int NF>
+__global__ __launch_bounds__(256) void k_bluff(BluffFields f, const float* __restrict__ sigma,
+                                               float dt, int64_t n) {
   const int64_t q = int64_t(blockIdx.x) * 256 + threadIdx.x;
-  if (q < n) out[q] = bluff_apply(in[q], sigma[q], dt);
+  if (q >= n) return;
+  const float s = sigma[q];
+  float v[NF];
+#pragma unroll
+  for (int k = 0; k < NF; ++k) v[k] = f.in[k][q];
+#pragma unroll
+  for (int k = 0; k < NF; ++k) f.out[k][q] = bluff_apply(v[k], s, dt);
 }
 
 }  // namespace hfm
@@ -2000,9 +2012,28 @@ extern "C" int hfm_bluff_body_f32(const float* const* in, float* const* out, int
       set_error("hfm_bluff_body_f32: null field %d", f);
       return HFM_EINVAL;
     }
-    if (ncells == 0) continue;
-    hipLaunchKernelGGL(k_bluff, dim3(unsigned((ncells + 255) / 256)), dim3(256), 0,
-                       as_stream(stream), in[f], out[f], sigma, dt, ncells);
+  }
+  // up to three fields per launch, which reads every input of a cell before
+  // writing its outputs; an output that is another field's input keeps the
+  // field-by-field order of one launch per field
+  bool cross = false;
+  for (int a = 0; a < nfields; ++a)
+    for (int b2 = 0; b2 < nfields; ++b2) cross |= a != b2 && out[a] == in[b2];
+  const int per = cross ? 1 : 3;
+  const dim3 grid(unsigned((ncells + 255) / 256));
+  for (int f0 = 0; f0 < nfields && ncells > 0; f0 += per) {
+    const int nf = std::min(per, nfields - f0);
+    BluffFields bf{};
+    for (int k = 0; k < nf; ++k) {
+      bf.in[k] = in[f0 + k];
+      bf.out[k] = out[f0 + k];
+    }
+    if (nf == 3)
+      hipLaunchKernelGGL(k_bluff<3>, grid, dim3(256), 0, as_stream(stream), bf, sigma, dt, ncells);
+    else if (nf == 2)
+      hipLaunchKernelGGL(k_bluff<2>, grid, dim3(256), 0, as_stream(stream), bf, sigma, dt, ncells);
+    else
+      hipLaunchKernelGGL(k_bluff<1>, grid, dim3(256), 0, as_stream(stream), bf, sigma, dt, ncells);
     if (int e = launch_status("hfm_bluff_body_f32")) return e;
   }
   return HFM_OK;

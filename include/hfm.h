@@ -274,7 +274,9 @@ int hfm_bc_velocity_bluff_inplace_f32(float* u, float* v, float* w, float* scrat
 int hfm_bc_velocity_inplace_noalloc_f32(float* u, float* v, float* w, int64_t D, int64_t H,
                                         int64_t W, float ub, const float* inflow, void* stream);
 
-/* out_i = in_i / (1 + dt*sigma) for n fields (in place allowed). */
+/* out_i = in_i / (1 + dt*sigma) for n fields (in place allowed).  Up to three
+ * fields share a launch that reads sigma once per cell; when an output is
+ * another field's input the fields run one launch each, in order. */
 int hfm_bluff_body_f32(const float* const* in, float* const* out, int nfields,
                        const float* sigma, float dt, int64_t ncells, void* stream);
 

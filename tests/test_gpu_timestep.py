@@ -229,6 +229,44 @@ def test_advect_post_equals_separate_passes_bitwise(dev, indicator, nbluff):
     assert torch.equal(got, ref)
 
 
+def test_bluff_fields_grouped_bitwise(dev):
+    """hfm_bluff_body_f32 damps up to three fields per launch (sigma read once):
+    bitwise one field per call, out of place and in place; with an output that
+    is another field's input, the fields keep their one-by-one order."""
+    import torch
+    from hfm import _lib as L
+    (u, v, w, a, _), sig = _fields((6, 10, 70), 9, dev)
+    fields = [u, v, w, a]
+    singles = [torch.empty_like(f) for f in fields]
+    for f, o in zip(fields, singles):
+        L.call("hfm_bluff_body_f32", L.ptr_array([f.data_ptr()]), L.ptr_array([o.data_ptr()]), 1,
+               sig.data_ptr(), 0.4, sig.numel(), L.stream_handle(dev))
+    outs = [torch.empty_like(f) for f in fields]
+    L.call("hfm_bluff_body_f32", L.ptr_array([f.data_ptr() for f in fields]),
+           L.ptr_array([o.data_ptr() for o in outs]), 4, sig.data_ptr(), 0.4, sig.numel(),
+           L.stream_handle(dev))
+    inplace = [f.clone() for f in fields]
+    L.call("hfm_bluff_body_f32", L.ptr_array([f.data_ptr() for f in inplace]),
+           L.ptr_array([f.data_ptr() for f in inplace]), 4, sig.data_ptr(), 0.4, sig.numel(),
+           L.stream_handle(dev))
+    # out[0] is in[1]: field 1 is damped after field 0 wrote it (twice)
+    x0, x1 = u.clone(), v.clone()
+    o1 = torch.empty_like(v)
+    L.call("hfm_bluff_body_f32", L.ptr_array([x0.data_ptr(), x1.data_ptr()]),
+           L.ptr_array([x1.data_ptr(), o1.data_ptr()]), 2, sig.data_ptr(), 0.4, sig.numel(),
+           L.stream_handle(dev))
+    twice = torch.empty_like(u)
+    L.call("hfm_bluff_body_f32", L.ptr_array([singles[0].data_ptr()]),
+           L.ptr_array([twice.data_ptr()]), 1, sig.data_ptr(), 0.4, sig.numel(),
+           L.stream_handle(dev))
+    torch.cuda.synchronize()
+    for got in (outs, inplace):
+        for g_, r_ in zip(got, singles):
+            assert torch.equal(g_, r_)
+    assert torch.equal(x1, singles[0])
+    assert torch.equal(o1, twice)
+
+
 @pytest.mark.parametrize("dissipative", [False, True])
 @pytest.mark.parametrize("bases_shared", [False, True])
 def test_advect_shared_velocities_bitwise(dev, dissipative, bases_shared):
