@@ -3,8 +3,10 @@
 // operator, 2^d mean restriction, nearest upsampling) and their fused
 // smoother/residual forms, plus the boundary-plane fills.  FP32 throughout.
 //
-// One thread per output cell, 64 x 4 thread blocks over (x, y) with one z
-// plane per grid row, so a wavefront reads 64 consecutive floats of a row.
+// 64 x 4 thread blocks, so a wavefront reads 64 consecutive floats of a row.
+// The cell kernels take one output cell per thread and one z plane per grid
+// row; the 3x3x3 stencils on large grids are column walkers (one thread walks
+// a z column of 18 planes with a register window, see below).
 #include <stdarg.h>
 #include <stdio.h>
 
