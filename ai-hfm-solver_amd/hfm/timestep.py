@@ -151,11 +151,16 @@ class UrbanFlow:
     def _solve_pressure(self, p, b):
         return self._pressure_cycles(p, b)
 
-    def step(self, u, v, w, p, alpha):
-        """One time step; returns (u, v, w, p, alpha) as new tensors."""
+    def step(self, u, v, w, p, alpha, donate: bool = False):
+        """One time step; returns (u, v, w, p, alpha) as new tensors.  With
+        ``donate`` the caller gives up u, v and w (as the reference's loop does,
+        reassigning them): the input velocity fill then writes their boundary
+        planes in place instead of copying the three fields, with bitwise the
+        same results."""
         shape = tuple(u.shape)
         u, v, w, p, alpha = (t.reshape(self.shape).float().contiguous() for t in (u, v, w, p, alpha))
-        u, v, w = self._bcv(u, v, w)
+        distinct = len({t.data_ptr() for t in (u, v, w)}) == 3
+        u, v, w = self._bcv_(u, v, w) if donate and distinct else self._bcv(u, v, w)
         out = self._step_fused(u, v, w, p, alpha) if self.fused else \
             self._step_layers(u, v, w, p, alpha)
         return tuple(t.reshape(shape) for t in out)
@@ -210,7 +215,9 @@ class UrbanFlow:
         nrestart, ...)`` (a ``snapshot.SnapshotWriter``; u, v, w, p and, with
         the scalar transport, alpha).  Returns the final fields."""
         for itime in range(1, int(ntime) + 1):
-            u, v, w, p, alpha = self.step(u, v, w, p, alpha)
+            # the caller's fields are left alone; later steps' inputs are the
+            # loop's own
+            u, v, w, p, alpha = self.step(u, v, w, p, alpha, donate=itime > 1)
             if writer is not None:
                 extra = {"alpha": alpha} if self.lscalar else {}
                 writer.maybe_save(itime + nrestart, u=u, v=v, w=w, p=p, **extra)

@@ -426,7 +426,7 @@ def urban(args, dev, torch):
         its = []
         failed = None
         for _ in range(args.urban_steps):
-            state = flow.step(*state)
+            state = flow.step(*state, donate=True)
             its.append(int(flow.last_info["iter"][0]))
             if int(flow.last_info["status"][0]) != 1:
                 failed = L_STATUS[int(flow.last_info["status"][0])]
@@ -598,7 +598,7 @@ def reference_context(args, dev, torch):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(args.ref_urban_steps):
-            state = flow.step(*state)
+            state = flow.step(*state, donate=True)
         torch.cuda.synchronize()
         per = (time.perf_counter() - t0) / args.ref_urban_steps
         out["urban_512"] = {"steps": args.ref_urban_steps, "mg_cycles_per_step": 5,
@@ -624,7 +624,7 @@ def reference_context(args, dev, torch):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(args.ref_inhale_steps):
-            state = flow.step(*state)
+            state = flow.step(*state, donate=True)
         torch.cuda.synchronize()
         per = (time.perf_counter() - t0) / args.ref_inhale_steps
         cells = float(np.prod(sh))
@@ -728,7 +728,7 @@ def profile_urban(args, dev, torch):
         t0 = time.perf_counter()
         its = []
         for _ in range(args.urban_steps):
-            state = flow.step(*state)
+            state = flow.step(*state, donate=True)
             its.append(int(flow.last_info["iter"][0]))
         torch.cuda.synchronize()
         out[name] = {"steps": args.urban_steps,

@@ -296,6 +296,29 @@ def test_advect_shared_velocities_bitwise(dev, dissipative, bases_shared):
         assert torch.equal(g_, r_)
 
 
+@pytest.mark.parametrize("driver", ["urban", "inhale"])
+def test_donated_step_equals_step_bitwise(dev, driver):
+    """step(donate=True) fills the input velocities in place (the caller gives
+    them up): bitwise the copying step, over two steps; the donated inputs hold
+    the filled fields afterwards."""
+    import torch
+    from hfm.timestep import InhaleFlow, UrbanFlow
+    sh = (8, 16, 32)
+    (u, v, w, p, a), sig = _fields(sh, 12, dev)
+    cls = UrbanFlow if driver == "urban" else InhaleFlow
+    f1, f2 = cls(sh, sigma=sig, device=dev), cls(sh, sigma=sig, device=dev)
+    ref = (u, v, w, p, a)
+    got = tuple(t.clone() for t in ref)
+    for _ in range(2):
+        ref = f1.step(*ref)
+        got = f2.step(*got, donate=True)
+    torch.cuda.synchronize()
+    for g_, r_ in zip(got, ref):
+        assert torch.equal(g_, r_)
+    # the caller's fields are untouched without donate
+    assert torch.equal(u, _fields(sh, 12, dev)[0][0])
+
+
 @pytest.mark.parametrize("driver", ["urban", "urban_noscalar", "inhale", "projection"])
 def test_fused_step_equals_layer_step_bitwise(dev, driver):
     """Two time steps of each driver with the fused passes against the same step
