@@ -961,15 +961,16 @@ struct Kern3 {
 };
 
 
-template <int NF, int T0 = 27, int T1 = 27, int T2 = 27>
+// RY: output rows per thread (a window of RY + 2 rows per field).
+template <int NF, int T0 = 27, int T1 = 27, int T2 = 27, int RY = 1>
 __global__ __launch_bounds__(kBX* kBY) void k_conv27_sum_v(const float* x0, const float* x1,
                                                            const float* x2, float* __restrict__ y,
                                                            Kern3 K, float scale, Grid3 g,
                                                            int flags) {
   Col c;
-  if (!col_begin<1>(g, c)) return;
+  if (!col_begin<RY>(g, c)) return;
   const float* xs[3] = {x0, x1, x2};
-  Win27<1, Fetch1, kWalkPF> win[NF];
+  Win27<RY, Fetch1, kWalkPF> win[NF];
 #pragma unroll
   for (int q = 0; q < NF; ++q) {
     const float* xq = xs[q];
@@ -979,7 +980,6 @@ __global__ __launch_bounds__(kBX* kBY) void k_conv27_sum_v(const float* x0, cons
   const bool src = (flags & HFM_SUM_SOURCE_BC) && g.W > 1;
   walk3(c, [&](auto ph, int k) {
     constexpr int P = decltype(ph)::value;
-    float acc = 0.0f;
 #pragma unroll
     for (int q = 0; q < NF; ++q) {
       const float* xq = xs[q];
@@ -987,14 +987,18 @@ __global__ __launch_bounds__(kBX* kBY) void k_conv27_sum_v(const float* x0, cons
           g, c, k, [&](int kk, int jj, int ii) { return Fetch1{*g.cell(xq, kk, jj, ii), true}; });
     }
 #pragma unroll
-    for (int q = 0; q < NF; ++q) {
-      float t[27];
-      win[q].template taps<27, P>(t, 0);
-      acc += q == 0 ? wsum<T0>(K.k[0], t) : q == 1 ? wsum<T1>(K.k[1], t) : wsum<T2>(K.k[2], t);
+    for (int ry = 0; ry < RY; ++ry) {
+      float acc = 0.0f;
+#pragma unroll
+      for (int q = 0; q < NF; ++q) {
+        float t[27];
+        win[q].template taps<27, P>(t, ry);
+        acc += q == 0 ? wsum<T0>(K.k[0], t) : q == 1 ? wsum<T1>(K.k[1], t) : wsum<T2>(K.k[2], t);
+      }
+      const float from_right = __shfl_down(acc, 1, 64);
+      if (src && c.i == 0) acc = from_right;
+      if (c.out && k < c.k1 && c.j + ry < g.H) *g.cell(y, k, c.j + ry, c.i) = scale * acc;
     }
-    const float from_right = __shfl_down(acc, 1, 64);
-    if (src && c.i == 0) acc = from_right;
-    if (c.out && k < c.k1) *g.cell(y, k, c.j, c.i) = scale * acc;
   });
 }
 
@@ -1811,8 +1815,9 @@ extern "C" int hfm_conv27_sum_f32(const float* const* x, float* y, int nf, const
     xs[q] = xs[0];
   }
   const Grid3 g{int(D), int(H), int(W)};
+  constexpr int kRY = 2;  // rows per thread of the three-field divergence
   if (nf == 3 && tapset3(kk.mask) == kSetXYZ)
-    hipLaunchKernelGGL((k_conv27_sum_v<3, kTapsX, kTapsY, kTapsZ>), grid_col<1>(g),
+    hipLaunchKernelGGL((k_conv27_sum_v<3, kTapsX, kTapsY, kTapsZ, kRY>), grid_col<kRY>(g),
                        dim3(kBX, kBY), 0, as_stream(stream), xs[0], xs[1], xs[2], y, kk, scale, g,
                        flags);
   else if (nf == 3)
