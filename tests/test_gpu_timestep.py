@@ -47,6 +47,23 @@ def test_fused_advection_matches_layers(dev):
     assert rel(H(r), O.zero_outlet_plane(C(a, k["poisson"]) - u)) < 1e-6
 
 
+@pytest.mark.parametrize("sh", [(5, 11, 70), (3, 1, 130), (19, 7, 66), (20, 3, 2)])
+def test_divergence_ragged_rows(dev, sh):
+    """The three-field divergence walker computes two rows per thread: grids
+    whose row count is odd (the last pair has one row) or one row high, chunks
+    ending mid-walk, and tiles wider or narrower than a wave."""
+    from hfm import layers
+    from hfm.timestep import urban_kernels
+    k = urban_kernels(0.4, 1.0, 1 / 20)
+    rng = np.random.default_rng([sh[0], sh[1], sh[2]])
+    u, v, w = (rng.standard_normal(sh).astype(np.float32) for _ in range(3))
+    print("divergence", sh, flush=True)
+    b = layers.conv_sum([T(x, dev) for x in (u, v, w)], k["div"], scale=-1.0, source_bc=True)
+    C = lambda x, Kk: O.conv_same(x.astype(np.float64), Kk)  # noqa: E731
+    refb = O.boundary_condition_source(-(C(u, k["div"][0]) + C(v, k["div"][1]) + C(w, k["div"][2])))
+    assert rel(H(b), refb) < 1e-6
+
+
 def _urban_state(sh, seed):
     rng = np.random.default_rng(seed)
     u = (1.0 + 0.05 * rng.standard_normal(sh)).astype(np.float32)
